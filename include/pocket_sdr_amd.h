@@ -1,0 +1,198 @@
+/*
+ * pocket_sdr_amd.h - C ABI of the MI355X (gfx950) PocketSDR search / correlate
+ * hot path (libpocketsdr_amd.so).
+ *
+ * Part 1 re-declares, with identical signatures and semantics, the hot-path
+ * functions of the reference header src/pocket_sdr.h (tomojitakasu/PocketSDR
+ * v0.17).  Each declaration cites the reference prototype it replaces and the
+ * definition whose behaviour it keeps.  All pointers are caller-owned host
+ * memory, exactly as in the reference; the library copies to and from HBM
+ * internally on a per-thread HIP stream.
+ *
+ * Part 2 is additive: plan-based batched entry points whose inputs stay
+ * resident in HBM between runs (no reference counterpart; they batch the
+ * loops of app/pocket_acq/pocket_acq.c:152-192 and src/sdr_ch.c:692-740).
+ *
+ * Types are ABI-identical to src/pocket_sdr.h:105-113 and 242-245; when the
+ * reference header has already been included its typedefs are used.
+ */
+#ifndef POCKET_SDR_AMD_H
+#define POCKET_SDR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef POCKET_SDR_H
+#define SDR_N_CODES    10          /* src/pocket_sdr.h:60 */
+#define SDR_MAX_CORR   107         /* src/pocket_sdr.h:58 */
+#define SDR_FFT_FORWARD 0          /* src/pocket_sdr.h:98 */
+#define SDR_FFT_BACKWARD 1         /* src/pocket_sdr.h:99 */
+typedef uint8_t sdr_cpx8_t;                    /* src/pocket_sdr.h:110 */
+typedef struct { int8_t I, Q; } sdr_cpx16_t;   /* src/pocket_sdr.h:111 */
+typedef float sdr_cpx_t[2];                    /* src/pocket_sdr.h:113 */
+typedef struct {                               /* src/pocket_sdr.h:242-245 */
+    sdr_cpx8_t *data;
+    int IQ, N;
+} sdr_buff_t;
+#endif
+
+/* ======================================================================
+ * Part 1: drop-in replacements (src/pocket_sdr.h:391-465)
+ * ====================================================================== */
+
+/* src/pocket_sdr.h:392, def. src/sdr_func.c:113-134.  Builds the carrier
+ * table and uploads it to the GPU; the FFTW wisdom file is ignored. */
+void sdr_func_init(const char *file);
+
+/* src/pocket_sdr.h:393-394, def. lib/pocketfft/pocketfft_wrap.cpp:62-82 via
+ * src/sdr_func.c:147-171 (32-byte aligned, exits on failure) */
+sdr_cpx_t *sdr_cpx_malloc(int N);
+void sdr_cpx_free(sdr_cpx_t *cpx);
+
+/* src/pocket_sdr.h:395, def. src/sdr_func.c:182-185 */
+float sdr_cpx_abs(sdr_cpx_t cpx);
+
+/* src/pocket_sdr.h:396-397, def. src/sdr_func.c:197-233 (c = a b s) */
+void sdr_cpx_mul(const sdr_cpx_t *a, const sdr_cpx_t *b, int N, float s,
+    sdr_cpx_t *c);
+
+/* src/pocket_sdr.h:398, def. src/sdr_func.c:236-264.  Unnormalised FFT,
+ * dir SDR_FFT_FORWARD / SDR_FFT_BACKWARD; returns 0 for a bad dir or a
+ * length whose prime factors are not all in {2, 3, 5}. */
+int sdr_cpx_fft(sdr_cpx_t *cpx1, int N, int dir, sdr_cpx_t *cpx2);
+
+/* src/pocket_sdr.h:399-400, def. src/sdr_func.c:276-297 */
+sdr_buff_t *sdr_buff_new(int N, int IQ);
+void sdr_buff_free(sdr_buff_t *buff);
+
+/* src/pocket_sdr.h:407-409, def. src/sdr_func.c:664-687.  P (len_fds x N,
+ * bin-major) accumulates |corr|^2 over all N lags.  The reference's 1 ms
+ * sleep every 22 bins is not reproduced. */
+void sdr_search_code(const sdr_cpx_t *code_fft, double T,
+    const sdr_buff_t *buff, int ix, int N, double fs, double fi,
+    const float *fds, int len_fds, float *P);
+
+/* src/pocket_sdr.h:410-412, 415, def. src/sdr_func.c:690-706, 728-744,
+ * 758-769 (host-side helpers over host arrays) */
+float sdr_corr_max(const float *P, int N, int Nmax, int M, double T, int *ix);
+double sdr_fine_dop(const float *P, int N, const float *fds, int len_fds,
+    const int *ix);
+float *sdr_dop_bins(double T, float dop, float max_dop, int *len_fds);
+
+/* src/pocket_sdr.h:416-417, def. src/sdr_func.c:981-1008 */
+void sdr_corr_std(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N,
+    double coff, const double *pos, int n, sdr_cpx_t *corr, sdr_cpx_t *C);
+
+/* src/pocket_sdr.h:418-420, def. src/sdr_func.c:1029-1057 */
+void sdr_corr_std_cpx_code(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code,
+    int N, double coff, const double *pos, int n, sdr_cpx_t *corr,
+    sdr_cpx_t *C);
+
+/* src/pocket_sdr.h:421-423, def. src/sdr_func.c:1060-1072 (Python-facing) */
+void sdr_corr_std_cpx(const sdr_cpx_t *buff, int len_buff, int ix, int N,
+    double fs, double fc, double phi, const float *code, const double *pos,
+    int n, sdr_cpx_t *corr);
+
+/* src/pocket_sdr.h:424-425, def. src/sdr_func.c:1088-1102 */
+void sdr_corr_fft(const sdr_cpx16_t *IQ, const sdr_cpx_t *code_fft, int N,
+    sdr_cpx_t *corr);
+
+/* src/pocket_sdr.h:426-428, def. src/sdr_func.c:1105-1113 (Python-facing) */
+void sdr_corr_fft_cpx(const sdr_cpx_t *buff, int len_buff, int ix, int N,
+    double fs, double fc, double phi, const sdr_cpx_t *code_fft,
+    sdr_cpx_t *corr);
+
+/* src/pocket_sdr.h:429-430, def. src/sdr_func.c:772-836 */
+void sdr_mix_carr(const sdr_buff_t *buff, int ix, int N, double fs, double fc,
+    double phi, sdr_cpx16_t *IQ);
+
+/* ======================================================================
+ * Part 2: batched, HBM-resident plans (additive, no reference counterpart)
+ * ====================================================================== */
+
+/* --- acquisition ----------------------------------------------------------
+ * One signal group = one code length: n_prn code spectra of length M
+ * (conjugated, as made by sdr_gen_code_fft with Nz = N), searched over
+ * len_fds Doppler bins and n_blk non-coherent blocks starting at
+ * ix0 + k * blk_step in IF stream `buff`.  Mirrors search_sig of
+ * app/pocket_acq/pocket_acq.c:152-192 for every PRN of the group.
+ */
+typedef struct {
+    int buff;                  /* index into the plan's IF streams */
+    int M;                     /* FFT length (2 N) */
+    int N;                     /* lags searched (code period in samples) */
+    double T, fs, fi;          /* code period (s), sampling rate, IF (Hz) */
+    int n_prn;
+    const sdr_cpx_t *code_fft; /* n_prn x M */
+    int len_fds;
+    const float *fds;          /* Doppler bins (Hz), as sdr_dop_bins */
+    int n_blk, ix0, blk_step;
+} sdr_search_sig_t;
+
+#define SDR_SEARCH_BATCHED  0  /* shared forward FFT per (bin, block), fused */
+#define SDR_SEARCH_PLAIN    1  /* reference dataflow per (PRN, block), unfused */
+#define SDR_SEARCH_GRAPH    2  /* capture the run in a hipGraph and replay it */
+
+typedef struct sdr_search_plan_tag sdr_search_plan_t;
+
+/* Copies the IF streams and code spectra to HBM and allocates every work
+ * buffer.  Returns NULL (with a message on stderr) on invalid input. */
+sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buffs,
+    int n_sig, const sdr_search_sig_t *sigs, int flags);
+/* Runs the whole search on the device (P surfaces are zeroed first), then
+ * the peak search.  Blocks until done.  Returns 1 on success. */
+int sdr_search_plan_run(sdr_search_plan_t *plan);
+/* Per-PRN results of signal group `sig` as sdr_corr_max / sdr_fine_dop
+ * produce them: cn0[n_prn], ix[2 n_prn] (bin, lag), dop[n_prn]. */
+int sdr_search_plan_result(const sdr_search_plan_t *plan, int sig, float *cn0,
+    int *ix, double *dop);
+/* Copies the power surface of one PRN (len_fds x N, bin-major). */
+int sdr_search_plan_power(const sdr_search_plan_t *plan, int sig, int prn_idx,
+    float *P);
+/* Enables per-kernel HIP-event timing of later runs (0 disables). */
+void sdr_search_plan_timing(sdr_search_plan_t *plan, int on);
+/* Average device time (ms) and launch count of kernel class `name`
+ * ("fwd_col", "fwd_row", "inv_row", "inv_col", "mix", "cvt", "mul",
+ * "peak") over the timed runs.  Returns 0 if the class never ran. */
+int sdr_search_plan_kernel_ms(const sdr_search_plan_t *plan, const char *name,
+    double *avg_ms, long long *count, double *elems);
+void sdr_search_plan_free(sdr_search_plan_t *plan);
+
+/* --- tracking correlators -------------------------------------------------
+ * n_ch channels on one IF stream, n_epoch consecutive epochs of N samples.
+ * Each epoch advances the channel open-loop as src/sdr_ch.c:694-703 (Doppler
+ * fd0 + drift t, no loop filters), mixes with sdr_mix_carr semantics and
+ * correlates with sdr_corr_std semantics at the n_pos tap positions.
+ */
+typedef struct {
+    double fs, fi, fc_sig, T;  /* sampling rate, IF, signal carrier, period */
+    int N;                     /* samples per epoch (same for all channels) */
+    double fd0, drift;         /* Doppler (Hz), Doppler rate (Hz/s) */
+    double phi0, coff0;        /* initial carrier phase (cyc), code offset (s) */
+    const sdr_cpx16_t *code;   /* resampled code bank, N x SDR_N_CODES */
+} sdr_trk_ch_t;
+
+typedef struct sdr_trk_plan_tag sdr_trk_plan_t;
+
+sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
+    const sdr_trk_ch_t *chs, int n_pos, const double *pos, int n_epoch);
+int sdr_trk_plan_run(sdr_trk_plan_t *plan);
+/* corr: n_ch x n_epoch x n_pos complex; C: n_ch x n_epoch x 2 complex
+ * (correlations before / after the code wrap, as sdr_corr_std's C). */
+int sdr_trk_plan_result(const sdr_trk_plan_t *plan, sdr_cpx_t *corr,
+    sdr_cpx_t *C);
+void sdr_trk_plan_timing(sdr_trk_plan_t *plan, int on);
+int sdr_trk_plan_kernel_ms(const sdr_trk_plan_t *plan, const char *name,
+    double *avg_ms, long long *count, double *elems);
+void sdr_trk_plan_free(sdr_trk_plan_t *plan);
+
+/* library version string */
+const char *sdr_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POCKET_SDR_AMD_H */

@@ -1,0 +1,558 @@
+/*
+ * sdr_oracle.c - CPU oracle for the PocketSDR search / correlate hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  A plain-C
+ * restatement of the reference algorithms; each function cites the reference
+ * file:line it follows.  Built with the reference's own SIMD flags
+ * (-O3 -mavx2 -mfma, lib/build/libsdr.mk:62-73) but with -ffp-contract=off:
+ * every place where the reference build fuses a multiply-add is written out
+ * here as an explicit fmaf() so the arithmetic is fixed by the source, not by
+ * the compiler.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <dlfcn.h>
+#include "sdr_oracle.h"
+
+#define PI_D 3.1415926535897932          /* src/pocket_sdr.h:63 */
+#define CSCALE (1 / 11.2f)                /* SDR_CSCALE, src/pocket_sdr.h:61 */
+#define NTBL 256                          /* src/sdr_func.c:53 */
+#define ROUND(x) floor((x) + 0.5)         /* src/sdr_func.c:66 */
+
+/* cpx8 nibble unpack, src/pocket_sdr.h:107-108 */
+static inline int cpx8_I(uint8_t x) { return (int8_t)(uint8_t)(x << 4) >> 4; }
+static inline int cpx8_Q(uint8_t x) { return (int8_t)x >> 4; }
+
+/* ---------------------------------------------------------------------------
+ * carrier-mixed LUT: src/sdr_func.c:121-131
+ * mix_tbl[(j << 8) + i] = cpx8(j) * carrier(i), carrier = round(11.2 e^{-2 pi i/256})
+ * products stored as int8 (two's complement wrap, as the reference's int8_t store).
+ */
+static int8_t mix_tbl[65536 * 2];
+static int mix_init = 0;
+
+void orc_func_init(void)
+{
+    for (int i = 0; i < NTBL; i++) {
+        int8_t cI = (int8_t)ROUND(cos(-2.0 * PI_D * i / NTBL) / CSCALE);
+        int8_t cQ = (int8_t)ROUND(sin(-2.0 * PI_D * i / NTBL) / CSCALE);
+        for (int j = 0; j < 256; j++) {
+            int I = cpx8_I((uint8_t)j), Q = cpx8_Q((uint8_t)j);
+            mix_tbl[2 * ((j << 8) + i)]     = (int8_t)(I * cI - Q * cQ);
+            mix_tbl[2 * ((j << 8) + i) + 1] = (int8_t)(I * cQ + Q * cI);
+        }
+    }
+    mix_init = 1;
+}
+
+const int8_t *orc_mix_tbl(void)
+{
+    if (!mix_init) orc_func_init();
+    return mix_tbl;
+}
+
+/* ---------------------------------------------------------------------------
+ * mix_carr: src/sdr_func.c:772-807; split at the ring-buffer end: 824-836.
+ * p = (u32)((phi - floor(phi)) 2^32), s = (u32)(int)(fc/fs 2^32),
+ * IQ[i] = mix_tbl[(data[i] << 8) + ((p + i s) >> 24)] (mod-2^32 phase).
+ */
+static void mix_seg(const uint8_t *data, int N, double phi, double step,
+                    int8_t *IQ)
+{
+    double scale = (double)(1 << 24) * NTBL;
+    uint32_t p = (uint32_t)((phi - floor(phi)) * scale);
+    uint32_t s = (uint32_t)(int)(step * scale);
+    for (int i = 0; i < N; i++, p += s) {
+        int idx = ((int)data[i] << 8) + (int)(p >> 24);
+        IQ[2 * i]     = mix_tbl[2 * idx];
+        IQ[2 * i + 1] = mix_tbl[2 * idx + 1];
+    }
+}
+
+void orc_mix_carr(const uint8_t *buff, int buffN, int ix, int N, double fs,
+                  double fc, double phi, int8_t *IQ)
+{
+    double step = fc / fs;
+    if (!mix_init) orc_func_init();
+    if (ix + N <= buffN) {
+        mix_seg(buff + ix, N, phi, step, IQ);
+    } else {
+        int n = buffN - ix;
+        mix_seg(buff + ix, n, phi, step, IQ);
+        mix_seg(buff, N - n, phi + step * n, step, IQ + 2 * n);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * dot products: src/sdr_func.c:861-912 (dot_IQ_code) and 915-956
+ * (dot_IQ_cpx_code), AVX2 path.  The first 16*floor(N/16) samples are summed
+ * in int32 with _mm256_sign_epi8 semantics (x * sign(c), with -(-128) kept as
+ * -128), the int32 total is converted to float, then the remaining samples
+ * are added one by one in float with a true int product, then scaled by
+ * (1.0f * SDR_CSCALE).
+ */
+static inline int sgn8(int x, int c)
+{
+    if (c < 0) return (int8_t)(uint8_t)(-x);
+    return c == 0 ? 0 : x;
+}
+
+static void dot_IQ_code(const int8_t *IQ, const int8_t *code, int N, float s,
+                        float *c)
+{
+    int V = N >= 16 ? N / 16 * 16 : 0;
+    int32_t sI = 0, sQ = 0;
+    for (int i = 0; i < V; i++) {
+        sI += sgn8(IQ[2 * i], code[2 * i]);
+        sQ += sgn8(IQ[2 * i + 1], code[2 * i + 1]);
+    }
+    float cI = (float)sI, cQ = (float)sQ;
+    for (int i = V; i < N; i++) {
+        cI += (float)(IQ[2 * i] * code[2 * i]);
+        cQ += (float)(IQ[2 * i + 1] * code[2 * i + 1]);
+    }
+    c[0] = cI * (s * CSCALE);
+    c[1] = cQ * (s * CSCALE);
+}
+
+static void dot_IQ_cpx_code(const int8_t *IQ, const int8_t *code, int N,
+                            float s, float *c)
+{
+    int V = N >= 16 ? N / 16 * 16 : 0;
+    int32_t sR = 0, sI = 0;
+    for (int i = 0; i < V; i++) {
+        int I = IQ[2 * i], Q = IQ[2 * i + 1];
+        int cI = code[2 * i], cQ = code[2 * i + 1];
+        int ncQ = (int8_t)(uint8_t)(-cQ);
+        sR += sgn8(I, cI) + sgn8(Q, cQ);
+        sI += sgn8(Q, cI) + sgn8(I, ncQ);
+    }
+    float cr = (float)sR, ci = (float)sI;
+    for (int i = V; i < N; i++) {
+        int I = IQ[2 * i], Q = IQ[2 * i + 1];
+        int cI = code[2 * i], cQ = code[2 * i + 1];
+        cr += (float)(I * cI + Q * cQ);
+        ci += (float)(Q * cI - I * cQ);
+    }
+    c[0] = cr * (s * CSCALE);
+    c[1] = ci * (s * CSCALE);
+}
+
+/* ---------------------------------------------------------------------------
+ * standard correlator: src/sdr_func.c:981-1008 (real code), 1029-1057
+ * (complex code).  Tap i: p = coff + pos[i], j = floor(p),
+ * k = (int)((p - j) * 10), j wrapped once into [0, N);
+ * corr1 = dot(IQ[0:j], bank_k[N-j:N]), corr2 = dot(IQ[j:N], bank_k[0:N-j]).
+ * Polarity: the reference build contracts the EPL sum to
+ * d += fmaf(c1.re, c2.re, c1.im * c2.im).
+ */
+typedef void (*dot_fn)(const int8_t *, const int8_t *, int, float, float *);
+
+static void corr_std_gen(dot_fn dot, const int8_t *IQ, const int8_t *code,
+                         int N, double coff, const double *pos, int n,
+                         float *corr, float *C)
+{
+    float corr1[ORC_MAX_CORR][2], corr2[ORC_MAX_CORR][2];
+    float dot_EPL = 0.0f, sign;
+
+    for (int i = 0; i < n; i++) {
+        double p = coff + pos[i];
+        int j = (int)floor(p), k = (int)((p - j) * ORC_N_CODES);
+        if (j < 0) j += N; else if (j >= N) j -= N;
+        dot(IQ, code + 2 * ((size_t)k * N + N - j), j, 1.0f, corr1[i]);
+        dot(IQ + 2 * j, code + 2 * (size_t)k * N, N - j, 1.0f, corr2[i]);
+    }
+    for (int i = 0; i < 3; i++) {
+        dot_EPL += fmaf(corr1[i][0], corr2[i][0], corr1[i][1] * corr2[i][1]);
+    }
+    sign = dot_EPL < 0.0f ? -1.0f : 1.0f;
+    for (int i = 0; i < n; i++) {
+        corr[2 * i]     = (corr1[i][0] + sign * corr2[i][0]) / (float)N;
+        corr[2 * i + 1] = (corr1[i][1] + sign * corr2[i][1]) / (float)N;
+    }
+    C[0] = corr1[0][0]; C[1] = corr1[0][1];
+    C[2] = corr2[0][0]; C[3] = corr2[0][1];
+}
+
+void orc_corr_std(const int8_t *IQ, const int8_t *code, int N, double coff,
+                  const double *pos, int n, float *corr, float *C)
+{
+    corr_std_gen(dot_IQ_code, IQ, code, N, coff, pos, n, corr, C);
+}
+
+void orc_corr_std_cpx_code(const int8_t *IQ, const int8_t *code, int N,
+                           double coff, const double *pos, int n, float *corr,
+                           float *C)
+{
+    corr_std_gen(dot_IQ_cpx_code, IQ, code, N, coff, pos, n, corr, C);
+}
+
+/* ---------------------------------------------------------------------------
+ * complex multiply c = a b s: src/sdr_func.c:197-233.  AVX2 lanes:
+ * re = fl(ar br) + fl(-ai bi), im = fl(ar bi) + fl(ai br), then * s.
+ * The scalar tail (N % 4 elements) is contracted by the reference build:
+ * re = fmaf(ar, br, -(ai bi)), im = fmaf(ar, bi, ai br).
+ */
+void orc_cpx_mul(const float *a, const float *b, int N, float s, float *c)
+{
+    int i = 0, V = N >= 4 ? N / 4 * 4 : 0;
+    for (; i < V; i++) {
+        float ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+        float p0 = ar * br, p1 = ai * -bi, p2 = ar * bi, p3 = ai * br;
+        c[2 * i] = (p0 + p1) * s;
+        c[2 * i + 1] = (p2 + p3) * s;
+    }
+    for (; i < N; i++) {
+        float ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+        c[2 * i] = fmaf(ar, br, -(ai * bi)) * s;
+        c[2 * i + 1] = fmaf(ar, bi, ai * br) * s;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * FFTs.  FP64: Stockham autosort, one pass per prime factor, generic radix-r
+ * DFT inside each pass; twiddles from cos/sin of the exact angle.  Used for
+ * the FP64 oracle and, rounded to float, as the FP32 FFT when the reference
+ * PocketFFT build is not loaded.
+ */
+static void fft_f64_impl(const double *in, int N, int sign, double *out)
+{
+    if (N <= 0) return;
+    double *x = (double *)malloc(sizeof(double) * 2 * N);
+    double *y = (double *)malloc(sizeof(double) * 2 * N);
+    memcpy(x, in, sizeof(double) * 2 * N);
+    int rem = N, Ns = 1;
+    while (rem > 1) {
+        int r = 2;
+        while (rem % r) r++;
+        double a[2 * 64], b[2 * 64];
+        double *ar = r <= 64 ? a : (double *)malloc(sizeof(double) * 2 * r);
+        double *br = r <= 64 ? b : (double *)malloc(sizeof(double) * 2 * r);
+        for (int j = 0; j < N / r; j++) {
+            int k = j % Ns;
+            for (int q = 0; q < r; q++) {
+                double xr = x[2 * (j + q * (N / r))], xi = x[2 * (j + q * (N / r)) + 1];
+                double ang = sign * 2.0 * PI_D * (double)q * k / ((double)Ns * r);
+                double wr = cos(ang), wi = sin(ang);
+                ar[2 * q] = xr * wr - xi * wi;
+                ar[2 * q + 1] = xr * wi + xi * wr;
+            }
+            for (int m = 0; m < r; m++) {
+                double sr = 0.0, si = 0.0;
+                for (int q = 0; q < r; q++) {
+                    double ang = sign * 2.0 * PI_D * (double)((q * m) % r) / r;
+                    double wr = cos(ang), wi = sin(ang);
+                    sr += ar[2 * q] * wr - ar[2 * q + 1] * wi;
+                    si += ar[2 * q] * wi + ar[2 * q + 1] * wr;
+                }
+                br[2 * m] = sr; br[2 * m + 1] = si;
+            }
+            int o = (j / Ns) * Ns * r + k;
+            for (int q = 0; q < r; q++) {
+                y[2 * (o + q * Ns)] = br[2 * q];
+                y[2 * (o + q * Ns) + 1] = br[2 * q + 1];
+            }
+        }
+        if (r > 64) { free(ar); free(br); }
+        double *t = x; x = y; y = t;
+        Ns *= r;
+        rem /= r;
+    }
+    memcpy(out, x, sizeof(double) * 2 * N);
+    free(x);
+    free(y);
+}
+
+void orc_fft_f64(const double *in, int N, int dir, double *out)
+{
+    fft_f64_impl(in, N, dir == 0 ? -1 : 1, out);
+}
+
+/* reference PocketFFT (lib/pocketfft/pocketfft_wrap.cpp:35-55), loaded from
+ * oracle/_ref when it has been built from the reference sources */
+typedef void *(*plan_fn)(int, float (*)[2], float (*)[2], int, unsigned);
+typedef void (*exec_fn)(void *, float (*)[2], float (*)[2]);
+typedef void (*destroy_fn)(void *);
+static plan_fn ref_plan = NULL;
+static exec_fn ref_exec = NULL;
+static destroy_fn ref_destroy = NULL;
+
+int orc_load_ref_fft(const char *so_path)
+{
+    void *h = dlopen(so_path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return 0;
+    ref_plan = (plan_fn)dlsym(h, "fftwf_plan_dft_1d");
+    ref_exec = (exec_fn)dlsym(h, "fftwf_execute_dft");
+    ref_destroy = (destroy_fn)dlsym(h, "fftwf_destroy_plan");
+    if (!ref_plan || !ref_exec || !ref_destroy) {
+        ref_plan = NULL; ref_exec = NULL; ref_destroy = NULL;
+        return 0;
+    }
+    return 1;
+}
+
+int orc_ref_fft_loaded(void) { return ref_plan != NULL; }
+
+int orc_fft_f32(const float *in, int N, int dir, float *out)
+{
+    if (dir != 0 && dir != 1) return 0;
+    if (ref_plan) {
+        /* FFTW_FORWARD = -1, FFTW_BACKWARD = +1 (lib/pocketfft/pocketfft.h:14-15) */
+        float (*tmp)[2] = (float (*)[2])malloc(sizeof(float) * 2 * N);
+        memcpy(tmp, in, sizeof(float) * 2 * N);
+        void *p = ref_plan(N, tmp, (float (*)[2])out, dir == 0 ? -1 : 1, 0);
+        ref_exec(p, tmp, (float (*)[2])out);
+        ref_destroy(p);
+        free(tmp);
+        return 1;
+    }
+    double *x = (double *)malloc(sizeof(double) * 2 * N);
+    double *y = (double *)malloc(sizeof(double) * 2 * N);
+    for (int i = 0; i < 2 * N; i++) x[i] = in[i];
+    fft_f64_impl(x, N, dir == 0 ? -1 : 1, y);
+    for (int i = 0; i < 2 * N; i++) out[i] = (float)y[i];
+    free(x);
+    free(y);
+    return 1;
+}
+
+/* ---------------------------------------------------------------------------
+ * FFT correlator: src/sdr_func.c:1088-1102
+ * corr = IFFT(FFT(IQ * CSCALE) * code_fft * (1/N/N)), unnormalised FFTs.
+ */
+void orc_corr_fft(const int8_t *IQ, const float *code_fft, int N, float *corr)
+{
+    float *cpx = (float *)malloc(sizeof(float) * 4 * N);
+    for (int i = 0; i < N; i++) {
+        cpx[2 * i] = IQ[2 * i] * CSCALE;
+        cpx[2 * i + 1] = IQ[2 * i + 1] * CSCALE;
+    }
+    orc_fft_f32(cpx, N, 0, cpx + 2 * N);
+    orc_cpx_mul(cpx + 2 * N, code_fft, N, 1.0f / N / N, cpx);
+    orc_fft_f32(cpx, N, 1, corr);
+    free(cpx);
+}
+
+/* ---------------------------------------------------------------------------
+ * parallel code search: src/sdr_func.c:664-687.  For each Doppler bin:
+ * mix at fi + fds[i] with phi = 0, FFT-correlate, P[i N + j] += |C[j]|^2 for
+ * all N lags.  The reference build contracts the power to
+ * P + fmaf(C0, C0, C1 * C1).  The 1 ms sleep every 22 bins is omitted.
+ */
+void orc_search_code(const float *code_fft, double T, const uint8_t *buff,
+                     int buffN, int ix, int N, double fs, double fi,
+                     const float *fds, int len_fds, float *P)
+{
+    (void)T;
+    float *C = (float *)malloc(sizeof(float) * 2 * N);
+    int8_t *data = (int8_t *)malloc(2 * (size_t)N);
+    for (int i = 0; i < len_fds; i++) {
+        orc_mix_carr(buff, buffN, ix, N, fs, fi + fds[i], 0.0, data);
+        orc_corr_fft(data, code_fft, N, C);
+        for (int j = 0; j < N; j++) {
+            P[(size_t)i * N + j] += fmaf(C[2 * j], C[2 * j], C[2 * j + 1] * C[2 * j + 1]);
+        }
+    }
+    free(C);
+    free(data);
+}
+
+void orc_search_code_f64(const double *code_fft, double T, const uint8_t *buff,
+                         int buffN, int ix, int N, double fs, double fi,
+                         const float *fds, int len_fds, double *P)
+{
+    (void)T;
+    double *x = (double *)malloc(sizeof(double) * 2 * N);
+    double *X = (double *)malloc(sizeof(double) * 2 * N);
+    int8_t *data = (int8_t *)malloc(2 * (size_t)N);
+    double s = 1.0 / ((double)N * N);
+    for (int i = 0; i < len_fds; i++) {
+        orc_mix_carr(buff, buffN, ix, N, fs, fi + fds[i], 0.0, data);
+        for (int j = 0; j < 2 * N; j++) x[j] = data[j] * (double)CSCALE;
+        fft_f64_impl(x, N, -1, X);
+        for (int j = 0; j < N; j++) {
+            double ar = X[2 * j], ai = X[2 * j + 1];
+            double br = code_fft[2 * j], bi = code_fft[2 * j + 1];
+            x[2 * j] = (ar * br - ai * bi) * s;
+            x[2 * j + 1] = (ar * bi + ai * br) * s;
+        }
+        fft_f64_impl(x, N, 1, X);
+        for (int j = 0; j < N; j++) {
+            P[(size_t)i * N + j] += X[2 * j] * X[2 * j] + X[2 * j + 1] * X[2 * j + 1];
+        }
+    }
+    free(x);
+    free(X);
+    free(data);
+}
+
+/* ---------------------------------------------------------------------------
+ * peak search and C/N0: src/sdr_func.c:690-706 (strict '>' so the first
+ * maximum in row-major order wins; running mean in double)
+ */
+float orc_corr_max(const float *P, int N, int Nmax, int M, double T, int *ix)
+{
+    double P_max = 0.0, P_ave = 0.0;
+    int n = 0;
+    for (int i = 0; i < M; i++) {
+        for (int j = 0; j < Nmax; j++) {
+            P_ave += (P[(size_t)i * N + j] - P_ave) / ++n;
+            if (P[(size_t)i * N + j] <= P_max) continue;
+            P_max = P[(size_t)i * N + j];
+            ix[0] = i;
+            ix[1] = j;
+        }
+    }
+    return (P_ave > 0.0) ?
+        (float)(10.0 * log10((P_max - P_ave) / P_ave / T)) : 0.0f;
+}
+
+/* fine Doppler: src/sdr_func.c:709-744.  The reference fits a quadratic
+ * through the three bins around the peak by least squares (RTKLIB lsq); with
+ * three points the fit interpolates, so the vertex is solved here from the
+ * 3x3 Vandermonde system by Cramer's rule. */
+double orc_fine_dop(const float *P, int N, const float *fds, int len_fds,
+                    const int *ix)
+{
+    if (ix[0] == 0 || ix[0] == len_fds - 1) return fds[ix[0]];
+    double x[3], y[3];
+    for (int i = 0; i < 3; i++) {
+        x[i] = fds[ix[0] - 1 + i];
+        y[i] = P[(size_t)(ix[0] - 1 + i) * N + ix[1]];
+    }
+    /* y = p0 + p1 x + p2 x^2 */
+    double d01 = x[0] - x[1], d02 = x[0] - x[2], d12 = x[1] - x[2];
+    double den = d01 * d02 * d12;
+    if (den == 0.0) return fds[ix[0]];
+    double p2 = (x[2] * (y[1] - y[0]) + x[1] * (y[0] - y[2]) + x[0] * (y[2] - y[1])) / den;
+    double p1 = (x[2] * x[2] * (y[0] - y[1]) + x[1] * x[1] * (y[2] - y[0]) +
+                 x[0] * x[0] * (y[1] - y[2])) / den;
+    if (p2 == 0.0) return fds[ix[0]];
+    return -p1 / (2.0 * p2);
+}
+
+/* Doppler bins: src/sdr_func.c:758-769 (float step and float bin values) */
+int orc_dop_bins(double T, float dop, float max_dop, float *fds)
+{
+    float step = (float)(0.5 / T);
+    int len = (int)(2.0 * max_dop / step) + 1;
+    if (fds) {
+        for (int i = 0; i < len; i++) fds[i] = dop - max_dop + i * step;
+    }
+    return len;
+}
+
+/* ---------------------------------------------------------------------------
+ * code resampling: src/sdr_code.c:2556-2569 and 2589-2608 (code_Q = NULL)
+ * j = (int)((coff fs + i) dx) % len_code, dx = len_code / T / fs.
+ */
+void orc_res_code(const int8_t *code, int len_code, double T, double coff,
+                  double fs, int N, int Nz, int8_t *code_res)
+{
+    double dx = len_code / T / fs;
+    memset(code_res, 0, 2 * (size_t)(N + Nz));
+    for (int i = 0; i < N; i++) {
+        int j = (int)((coff * fs + i) * dx) % len_code;
+        code_res[2 * i] = code[j];
+        code_res[2 * i + 1] = code[j];
+    }
+}
+
+void orc_gen_code_fft(const int8_t *code, int len_code, double T, double coff,
+                      double fs, int N, int Nz, float *code_fft)
+{
+    double dx = len_code / T / fs;
+    float *res = (float *)calloc(2 * (size_t)(N + Nz), sizeof(float));
+    for (int i = 0; i < N; i++) {
+        int j = (int)((coff * fs + i) * dx) % len_code;
+        res[2 * i] = code[j];
+    }
+    orc_fft_f32(res, N + Nz, 0, code_fft);
+    for (int i = 0; i < N + Nz; i++) code_fft[2 * i + 1] = -code_fft[2 * i + 1];
+    free(res);
+}
+
+void orc_gen_code_fft_f64(const int8_t *code, int len_code, double T,
+                          double coff, double fs, int N, int Nz,
+                          double *code_fft)
+{
+    double dx = len_code / T / fs;
+    double *res = (double *)calloc(2 * (size_t)(N + Nz), sizeof(double));
+    for (int i = 0; i < N; i++) {
+        int j = (int)((coff * fs + i) * dx) % len_code;
+        res[2 * i] = code[j];
+    }
+    fft_f64_impl(res, N + Nz, -1, code_fft);
+    for (int i = 0; i < N + Nz; i++) code_fft[2 * i + 1] = -code_fft[2 * i + 1];
+    free(res);
+}
+
+/* ---------------------------------------------------------------------------
+ * GPS L1 C/A: src/sdr_code.c:776-812.  G1 = LFSR(R=0x3FF, tap 0x081),
+ * G2 = LFSR(R=0x3FF, tap 0x197), code[i] = G1[i] G2[(i + 1023 - delay) % 1023]
+ * with the IS-GPS-200 G2 delays for PRN 1-32.
+ */
+static const int L1CA_G2_DELAY[32] = {
+    5, 6, 7, 8, 17, 18, 139, 140, 141, 251, 252, 254, 255, 256, 257, 258,
+    469, 470, 471, 472, 473, 474, 509, 512, 513, 514, 515, 516, 859, 860, 861, 862
+};
+
+static void lfsr10(int32_t R, int32_t tap, int8_t *out)
+{
+    for (int i = 0; i < 1023; i++) {
+        out[i] = (R & 1) ? -1 : 1;        /* CHIP[] = {1, -1} */
+        int fb = __builtin_parity((unsigned)(R & tap));
+        R = (fb << 9) | (R >> 1);
+    }
+}
+
+int orc_gen_code_L1CA(int prn, int8_t *code)
+{
+    if (prn < 1 || prn > 32) return 0;
+    int8_t g1[1023], g2[1023];
+    lfsr10(0x3FF, 0x081, g1);
+    lfsr10(0x3FF, 0x197, g2);
+    int d = L1CA_G2_DELAY[prn - 1];
+    for (int i = 0; i < 1023; i++) code[i] = g1[i] * g2[(i + 1023 - d) % 1023];
+    return 1023;
+}
+
+/* ---------------------------------------------------------------------------
+ * open-loop tracking: src/sdr_ch.c:674-703 (NCO / code advance, adj_coff) and
+ * 722-740 (mix + standard correlator).  No loop filters: the Doppler follows
+ * fd(t) = fd0 + drift t with t the end time of the epoch block.
+ * state (optional, 3 doubles per epoch): phi, coff (s), fc used for the mix.
+ */
+void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
+                 const uint8_t *buff, int buffN, int n_epoch,
+                 const double *pos, int n_pos, float *corr, float *C,
+                 double *state)
+{
+    double time = 0.0, phi = ch->phi0, coff = ch->coff0;
+    int N = ch->N;
+    int8_t *IQ = (int8_t *)malloc(2 * (size_t)N);
+    for (int e = 0; e < n_epoch; e++) {
+        double t = (e + 1) * ch->T;
+        double tau = t - time;
+        double fd = ch->fd0 + ch->drift * t;
+        double fc = ch->fi + fd;
+        coff -= fd / ch->fc_sig * tau;
+        phi += fc * tau;
+        phi -= floor(phi);
+        time = t;
+        if (coff >= ch->T) coff -= ch->T;
+        else if (coff < 0.0) coff += ch->T;
+        int ix = (int)(((long long)e * N) % buffN);
+        orc_mix_carr(buff, buffN, ix, N, ch->fs, fc, phi, IQ);
+        orc_corr_std(IQ, code_bank, N, coff * ch->fs, pos, n_pos,
+                     corr + (size_t)e * 2 * n_pos, C + (size_t)e * 4);
+        if (state) {
+            state[3 * e] = phi;
+            state[3 * e + 1] = coff;
+            state[3 * e + 2] = fc;
+        }
+    }
+    free(IQ);
+}

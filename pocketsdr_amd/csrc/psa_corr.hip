@@ -1,0 +1,723 @@
+// psa_corr.hip - standard correlators (src/sdr_func.c:861-1072) and the
+// batched open-loop tracking plan (src/sdr_ch.c:674-740).
+//
+// Bit-exactness with the reference AVX2 build: each half-correlation
+// (corr1 = IQ[0:j] . bank_k[N-j:N], corr2 = IQ[j:N] . bank_k[0:N-j]) is
+//   float(S_vec) + t_0 + t_1 + ... (sequential float adds)
+// where S_vec is the int32 sum over the first 16*floor(len/16) samples with
+// _mm256_sign_epi8 semantics (x * sign(c), -(-128) stays -128) and t_i are the
+// remaining samples' true int products (src/sdr_func.c:872-910).  Integer sums
+// are order independent, so any GPU reduction order reproduces S_vec exactly.
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+#include <string>
+#include "psa_internal.h"
+
+namespace psa {
+
+__device__ __forceinline__ int sgn8(int x, int c)
+{
+    if (c < 0) return (int)(int8_t)(uint8_t)(-x);
+    return c == 0 ? 0 : x;
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// block-wide sum of 4 ints (256 threads); result valid in all threads
+__device__ __forceinline__ int4 block_sum4(int4 v, int4 *red)
+{
+    v.x = wave_sum(v.x); v.y = wave_sum(v.y); v.z = wave_sum(v.z); v.w = wave_sum(v.w);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    int4 r = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+        r.x += red[i].x; r.y += red[i].y; r.z += red[i].z; r.w += red[i].w;
+    }
+    return r;
+}
+
+// Tail of one half: samples [v, len) of data d against code c, added one by
+// one in float to float(S_vec) (scalar loop of src/sdr_func.c:906-909 /
+// 951-954), then scaled by (1.0f * SDR_CSCALE).
+template <bool CPX>
+__device__ void half_epilogue(int svecA, int svecB, const char2 *d, const char2 *c, int v, int len,
+                              float *outA, float *outB)
+{
+    float a = (float)svecA, bq = (float)svecB;
+    for (int i = v; i < len; i++) {
+        int I = d[i].x, Q = d[i].y, cI = c[i].x, cQ = c[i].y;
+        if (CPX) {
+            a += (float)(I * cI + Q * cQ);
+            bq += (float)(Q * cI - I * cQ);
+        } else {
+            a += (float)(I * cI);
+            bq += (float)(Q * cQ);
+        }
+    }
+    const float sc = 1.0f * PSA_CSCALE;
+    *outA = a * sc;
+    *outB = bq * sc;
+}
+
+// One workgroup per tap: vector-part sums of both halves, then the float
+// epilogue of the tap.  corr12[t] = {corr1.re, corr1.im, corr2.re, corr2.im}.
+template <bool CPX>
+__global__ void __launch_bounds__(256) corr_tap_kernel(const char2 *__restrict__ IQ,
+                                                       const char2 *__restrict__ code, int N,
+                                                       const int2 *__restrict__ jk,
+                                                       float4 *__restrict__ corr12)
+{
+    __shared__ int4 red[4];
+    const int t = blockIdx.x;
+    const int j = jk[t].x, k = jk[t].y;
+    const char2 *bank = code + (long long)k * N;
+    const int V1 = j >= 16 ? j / 16 * 16 : 0;
+    const int len2 = N - j;
+    const int V2 = len2 >= 16 ? len2 / 16 * 16 : 0;
+    int4 s = make_int4(0, 0, 0, 0);
+    // half 1: IQ[m] with bank[N - j + m], m < V1
+    for (int m = threadIdx.x; m < V1; m += blockDim.x) {
+        char2 d = IQ[m], c = bank[N - j + m];
+        if (CPX) {
+            s.x += sgn8(d.x, c.x) + sgn8(d.y, c.y);
+            s.y += sgn8(d.y, c.x) + sgn8(d.x, (int)(int8_t)(uint8_t)(-c.y));
+        } else {
+            s.x += sgn8(d.x, c.x);
+            s.y += sgn8(d.y, c.y);
+        }
+    }
+    // half 2: IQ[j + r] with bank[r], r < V2
+    for (int r = threadIdx.x; r < V2; r += blockDim.x) {
+        char2 d = IQ[j + r], c = bank[r];
+        if (CPX) {
+            s.z += sgn8(d.x, c.x) + sgn8(d.y, c.y);
+            s.w += sgn8(d.y, c.x) + sgn8(d.x, (int)(int8_t)(uint8_t)(-c.y));
+        } else {
+            s.z += sgn8(d.x, c.x);
+            s.w += sgn8(d.y, c.y);
+        }
+    }
+    int4 tot = block_sum4(s, red);
+    if (threadIdx.x == 0) {
+        float4 o;
+        half_epilogue<CPX>(tot.x, tot.y, IQ, bank + N - j, V1, j, &o.x, &o.y);
+        half_epilogue<CPX>(tot.z, tot.w, IQ + j, bank, V2, len2, &o.z, &o.w);
+        corr12[t] = o;
+    }
+}
+
+// Polarity and output (src/sdr_func.c:995-1007): the reference build
+// contracts the EPL sum to d += fmaf(c1.re, c2.re, c1.im * c2.im).
+__global__ void corr_final_kernel(const float4 *__restrict__ corr12, int n, int N,
+                                  float2 *__restrict__ corr, float2 *__restrict__ C)
+{
+    float dot = 0.0f;
+    for (int i = 0; i < 3 && i < n; i++) {
+        float4 c = corr12[i];
+        dot = __fadd_rn(dot, __fmaf_rn(c.x, c.z, __fmul_rn(c.y, c.w)));
+    }
+    const float sign = dot < 0.0f ? -1.0f : 1.0f;
+    const float fN = (float)N;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        float4 c = corr12[i];
+        corr[i] = make_float2(__fdiv_rn(__fadd_rn(c.x, __fmul_rn(sign, c.z)), fN),
+                              __fdiv_rn(__fadd_rn(c.y, __fmul_rn(sign, c.w)), fN));
+    }
+    if (threadIdx.x == 0 && n > 0) {
+        float4 c = corr12[0];
+        C[0] = make_float2(c.x, c.y);
+        C[1] = make_float2(c.z, c.w);
+    }
+}
+
+// tap geometry, src/sdr_func.c:988-991 (double arithmetic as the reference)
+static void tap_geometry(int N, double coff, const double *pos, int n, int2 *jk)
+{
+    for (int i = 0; i < n; i++) {
+        double p = coff + pos[i];
+        int j = (int)floor(p), k = (int)((p - j) * PSA_N_CODES);
+        if (j < 0) j += N; else if (j >= N) j -= N;
+        if (j < 0 || j > N || k < 0 || k >= PSA_N_CODES) {
+            PSA_FAIL("corr_std: tap %d position %.3f outside one code period (N=%d)", i, p, N);
+        }
+        jk[i] = make_int2(j, k);
+    }
+}
+
+// Per-thread cache of device code banks: a bank is re-uploaded only when its
+// pointer, size or contents differ from the cached shadow copy.
+struct BankCache {
+    struct Ent {
+        const void *host = nullptr;
+        size_t bytes = 0;
+        std::vector<uint8_t> shadow;
+        void *dev = nullptr;
+        unsigned long long used = 0;
+    } e[4];
+    unsigned long long tick = 0;
+    ~BankCache()
+    {
+        for (auto &x : e) if (x.dev) (void)hipFree(x.dev);
+    }
+    const void *get(const void *host, size_t bytes, hipStream_t st)
+    {
+        tick++;
+        for (auto &x : e) {
+            if (x.host == host && x.bytes == bytes && x.dev &&
+                memcmp(x.shadow.data(), host, bytes) == 0) {
+                x.used = tick;
+                return x.dev;
+            }
+        }
+        Ent *v = &e[0];
+        for (auto &x : e) if (x.used < v->used) v = &x;
+        if (v->bytes < bytes || !v->dev) {
+            if (v->dev) PSA_CHECK(hipFree(v->dev));
+            PSA_CHECK(hipMalloc(&v->dev, bytes));
+        }
+        v->host = host;
+        v->bytes = bytes;
+        v->shadow.assign((const uint8_t *)host, (const uint8_t *)host + bytes);
+        v->used = tick;
+        PSA_CHECK(hipMemcpyAsync(v->dev, host, bytes, hipMemcpyHostToDevice, st));
+        return v->dev;
+    }
+};
+
+static BankCache &bank_cache()
+{
+    thread_local BankCache c;
+    return c;
+}
+
+template <bool CPX>
+static void corr_std_impl(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N, double coff,
+                          const double *pos, int n, sdr_cpx_t *corr, sdr_cpx_t *C)
+{
+    if (n <= 0 || N <= 0) return;
+    if (n > PSA_MAX_CORR) PSA_FAIL("corr_std: n=%d exceeds SDR_MAX_CORR", n);
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    std::vector<int2> jk(n);
+    tap_geometry(N, coff, pos, n, jk.data());
+
+    const void *dcode = bank_cache().get(code, sizeof(sdr_cpx16_t) * (size_t)N * PSA_N_CODES, st);
+    size_t iq_bytes = sizeof(sdr_cpx16_t) * (size_t)N;
+    char *din = (char *)tc.s[SC_IQ].get(iq_bytes + 256 + sizeof(int2) * n);
+    int2 *djk = (int2 *)(din + ((iq_bytes + 255) / 256) * 256);
+    uint8_t *hpin = (uint8_t *)tc.host_pinned(iq_bytes + sizeof(int2) * n);
+    memcpy(hpin, IQ, iq_bytes);
+    memcpy(hpin + iq_bytes, jk.data(), sizeof(int2) * n);
+    PSA_CHECK(hipMemcpyAsync(din, hpin, iq_bytes, hipMemcpyHostToDevice, st));
+    PSA_CHECK(hipMemcpyAsync(djk, hpin + iq_bytes, sizeof(int2) * n, hipMemcpyHostToDevice, st));
+
+    float *dout = (float *)tc.s[SC_MISC].get(sizeof(float4) * n + sizeof(float2) * (n + 2));
+    float4 *c12 = (float4 *)dout;
+    float2 *dcorr = (float2 *)(c12 + n);
+    float2 *dC = dcorr + n;
+    hipLaunchKernelGGL(corr_tap_kernel<CPX>, dim3(n), dim3(256), 0, st,
+                       (const char2 *)din, (const char2 *)dcode, N, djk, c12);
+    PSA_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(corr_final_kernel, dim3(1), dim3(128), 0, st, c12, n, N, dcorr, dC);
+    PSA_CHECK(hipGetLastError());
+    float *hout = (float *)tc.host_pinned(sizeof(float2) * (n + 2));
+    PSA_CHECK(hipMemcpyAsync(hout, dcorr, sizeof(float2) * (n + 2), hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(corr, hout, sizeof(float2) * n);
+    memcpy(C, hout + 2 * n, sizeof(float2) * 2);
+}
+
+// ===========================================================================
+// batched open-loop tracking
+// ===========================================================================
+
+// Per-channel parameters on the device.
+struct TrkChDev {
+    double fs, fi, fc_sig, T, fd0, drift, phi0, coff0;
+};
+
+// Epoch state of one (channel, epoch): mixing parameters and tap geometry.
+struct TrkEpoch {
+    int ix, nseg;
+    uint32_t p0, p1, s;
+    int pad;
+};
+
+// Open-loop NCO / code advance, src/sdr_ch.c:694-703 with adj_coff
+// (674-689), in FP64, one thread per channel, sequential over epochs.
+// Products and sums are kept unfused so the result is fixed by the source.
+__global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int n_epoch, int N,
+                                 int buffN, const double *__restrict__ pos, int n_pos,
+                                 TrkEpoch *__restrict__ ep, int2 *__restrict__ jk)
+{
+#pragma clang fp contract(off)
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_ch) return;
+    const TrkChDev ch = chs[c];
+    const double scale = 4294967296.0;
+    double time = 0.0, phi = ch.phi0, coff = ch.coff0;
+    for (int e = 0; e < n_epoch; e++) {
+        double t = (double)(e + 1) * ch.T;
+        double tau = t - time;
+        double fd = ch.fd0 + ch.drift * t;
+        double fc = ch.fi + fd;
+        coff = coff - fd / ch.fc_sig * tau;
+        phi = phi + fc * tau;
+        phi = phi - floor(phi);
+        time = t;
+        if (coff >= ch.T) coff = coff - ch.T;
+        else if (coff < 0.0) coff = coff + ch.T;
+        // sdr_mix_carr phase setup (src/sdr_func.c:776-778, 824-836)
+        int ix = (int)(((long long)e * N) % buffN);
+        double step = fc / ch.fs;
+        TrkEpoch q;
+        q.ix = ix;
+        q.p0 = (uint32_t)((phi - floor(phi)) * scale);
+        q.s = (uint32_t)(int)(step * scale);
+        if (ix + N <= buffN) {
+            q.nseg = N;
+            q.p1 = 0;
+        } else {
+            int n = buffN - ix;
+            double phi2 = phi + step * n;
+            q.nseg = n;
+            q.p1 = (uint32_t)((phi2 - floor(phi2)) * scale);
+        }
+        q.pad = 0;
+        long long o = (long long)c * n_epoch + e;
+        ep[o] = q;
+        // sdr_corr_std tap geometry (src/sdr_func.c:988-991)
+        double cs = coff * ch.fs;
+        for (int i = 0; i < n_pos; i++) {
+            double p = cs + pos[i];
+            int j = (int)floor(p), k = (int)((p - j) * PSA_N_CODES);
+            if (j < 0) j += N; else if (j >= N) j -= N;
+            jk[o * n_pos + i] = make_int2(j, k);
+        }
+    }
+}
+
+// Fused mix + correlate: one workgroup per (channel, epoch).  The epoch's N
+// samples are carrier-mixed into LDS as separate int8 I and Q planes; each
+// tap then accumulates 4 samples per lane with v_dot4 against the code bank
+// (planar int8, 4 byte-shifted copies so every code word load is aligned).
+// EXACT selects _mm256_sign_epi8 semantics for IF buffers that hold -8
+// nibbles (the only source of -128 mixed values).
+#define TRK_THREADS 256
+
+template <bool EXACT>
+__global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
+    const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
+    const TrkEpoch *__restrict__ ep, const int2 *__restrict__ jk,
+    const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride,
+    int NP,  // padded bank length in bytes (multiple of 4)
+    const uint32_t *__restrict__ carr_g,
+    float2 *__restrict__ corr_out, float2 *__restrict__ C_out)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t carr[256];
+    __shared__ int sums[PSA_MAX_CORR][4];
+    __shared__ float4 c12s[PSA_MAX_CORR];
+    const int Nq = (N + 3) / 4;
+    uint32_t *planeI = lds, *planeQ = lds + Nq;
+    const long long o = blockIdx.x;          // channel * n_epoch + epoch
+    const int ch = (int)(o / n_epoch);
+    const TrkEpoch q = ep[o];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) carr[i] = carr_g[i];
+    for (int i = threadIdx.x; i < n_pos * 4; i += blockDim.x) (&sums[0][0])[i] = 0;
+    __syncthreads();
+
+    // ---- mix N samples into the I / Q planes (zero padded to Nq words) ----
+    for (int w = threadIdx.x; w < Nq; w += blockDim.x) {
+        uint32_t wi = 0, wq = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            int i = w * 4 + b;
+            if (i < N) {
+                int src;
+                uint32_t ph;
+                if (i < q.nseg) { src = q.ix + i; ph = q.p0 + (uint32_t)i * q.s; }
+                else { src = i - q.nseg; ph = q.p1 + (uint32_t)(i - q.nseg) * q.s; }
+                uint32_t d = buf[src];
+                int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+                int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+                uint32_t c = carr[ph >> 24];
+                int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+                wi |= (uint32_t)(uint8_t)(int8_t)(sI * cI - sQ * cQ) << (8 * b);
+                wq |= (uint32_t)(uint8_t)(int8_t)(sI * cQ + sQ * cI) << (8 * b);
+            }
+        }
+        planeI[w] = wi;
+        planeQ[w] = wq;
+    }
+    __syncthreads();
+
+    const uint32_t *cbase = codes + (long long)ch * code_ch_stride;   // [copy][bankI|bankQ][k][NP/4]
+    const int lane = threadIdx.x & 63;
+    // ---- taps ----
+    for (int t = 0; t < n_pos; t++) {
+        const int2 g = jk[o * n_pos + t];
+        const int j = g.x, k = g.y;
+        int a1I = 0, a1Q = 0, a2I = 0, a2Q = 0;
+        for (int w = threadIdx.x; w < Nq; w += blockDim.x) {
+            const int m = w * 4;
+            const uint32_t dI = planeI[w], dQ = planeQ[w];
+            if (m + 3 < j || (m >= j && m + 3 < N)) {
+                // all 4 samples in one half: code index c0 .. c0 + 3
+                const int c0 = m >= j ? m - j : N - j + m;
+                const uint32_t *cp = cbase + (long long)(c0 & 3) * code_copy_stride;
+                const uint32_t cI = cp[((long long)k * NP + (c0 & ~3)) >> 2];
+                const uint32_t cQ = cp[((long long)(PSA_N_CODES + k) * NP + (c0 & ~3)) >> 2];
+                int sI, sQ;
+                if (EXACT) {
+                    sI = 0; sQ = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        sI += sgn8((int8_t)(dI >> (8 * b)), (int8_t)(cI >> (8 * b)));
+                        sQ += sgn8((int8_t)(dQ >> (8 * b)), (int8_t)(cQ >> (8 * b)));
+                    }
+                } else {
+                    sI = __builtin_amdgcn_sdot4((int)dI, (int)cI, 0, false);
+                    sQ = __builtin_amdgcn_sdot4((int)dQ, (int)cQ, 0, false);
+                }
+                if (m >= j) { a2I += sI; a2Q += sQ; } else { a1I += sI; a1Q += sQ; }
+            } else {
+                // straddles the code wrap at m = j or the end of the data
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int mm = m + b;
+                    if (mm >= N) break;
+                    const int ci = mm >= j ? mm - j : N - j + mm;
+                    const uint8_t *cb = (const uint8_t *)cbase;
+                    const int vI = (int8_t)cb[(long long)k * NP + ci];
+                    const int vQ = (int8_t)cb[(long long)(PSA_N_CODES + k) * NP + ci];
+                    const int xI = (int8_t)(dI >> (8 * b)), xQ = (int8_t)(dQ >> (8 * b));
+                    const int pI = EXACT ? sgn8(xI, vI) : xI * vI;
+                    const int pQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
+                    if (mm >= j) { a2I += pI; a2Q += pQ; } else { a1I += pI; a1Q += pQ; }
+                }
+            }
+        }
+        a1I = wave_sum(a1I); a1Q = wave_sum(a1Q);
+        a2I = wave_sum(a2I); a2Q = wave_sum(a2Q);
+        if (lane == 0) {
+            atomicAdd(&sums[t][0], a1I);
+            atomicAdd(&sums[t][1], a1Q);
+            atomicAdd(&sums[t][2], a2I);
+            atomicAdd(&sums[t][3], a2Q);
+        }
+    }
+    __syncthreads();
+
+    // ---- per-tap float epilogue: S_vec = S_total - tail products ----
+    const uint8_t *cb = (const uint8_t *)cbase;
+    for (int t = threadIdx.x; t < n_pos; t += blockDim.x) {
+        const int2 g = jk[o * n_pos + t];
+        const int j = g.x, k = g.y;
+        const int V1 = j >= 16 ? j / 16 * 16 : 0;
+        const int len2 = N - j;
+        const int V2 = len2 >= 16 ? len2 / 16 * 16 : 0;
+        const uint8_t *pI = (const uint8_t *)planeI, *pQ = (const uint8_t *)planeQ;
+        float out[4];
+        for (int h = 0; h < 2; h++) {
+            const int d0 = h == 0 ? 0 : j;          // first data sample of the half
+            const int c0 = h == 0 ? N - j : 0;      // first code sample of the half
+            const int v = h == 0 ? V1 : V2, len = h == 0 ? j : len2;
+            int sI = sums[t][2 * h], sQ = sums[t][2 * h + 1];
+            // remove the tail products (they were summed in the bulk loop)
+            for (int i = v; i < len; i++) {
+                const int xI = (int8_t)pI[d0 + i], xQ = (int8_t)pQ[d0 + i];
+                const int vI = (int8_t)cb[(long long)k * NP + c0 + i];
+                const int vQ = (int8_t)cb[(long long)(PSA_N_CODES + k) * NP + c0 + i];
+                sI -= EXACT ? sgn8(xI, vI) : xI * vI;
+                sQ -= EXACT ? sgn8(xQ, vQ) : xQ * vQ;
+            }
+            float fI = (float)sI, fQ = (float)sQ;
+            for (int i = v; i < len; i++) {
+                const int xI = (int8_t)pI[d0 + i], xQ = (int8_t)pQ[d0 + i];
+                const int vI = (int8_t)cb[(long long)k * NP + c0 + i];
+                const int vQ = (int8_t)cb[(long long)(PSA_N_CODES + k) * NP + c0 + i];
+                fI += (float)(xI * vI);
+                fQ += (float)(xQ * vQ);
+            }
+            out[2 * h] = fI * (1.0f * PSA_CSCALE);
+            out[2 * h + 1] = fQ * (1.0f * PSA_CSCALE);
+        }
+        c12s[t] = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float dot = 0.0f;
+        for (int i = 0; i < 3 && i < n_pos; i++) {
+            float4 c = c12s[i];
+            dot = __fadd_rn(dot, __fmaf_rn(c.x, c.z, __fmul_rn(c.y, c.w)));
+        }
+        sums[0][0] = dot < 0.0f ? -1 : 1;
+        if (n_pos > 0) {
+            C_out[2 * o] = make_float2(c12s[0].x, c12s[0].y);
+            C_out[2 * o + 1] = make_float2(c12s[0].z, c12s[0].w);
+        }
+    }
+    __syncthreads();
+    const float sign = (float)sums[0][0];
+    const float fN = (float)N;
+    for (int t = threadIdx.x; t < n_pos; t += blockDim.x) {
+        float4 c = c12s[t];
+        corr_out[o * n_pos + t] = make_float2(__fdiv_rn(__fadd_rn(c.x, __fmul_rn(sign, c.z)), fN),
+                                              __fdiv_rn(__fadd_rn(c.y, __fmul_rn(sign, c.w)), fN));
+    }
+}
+
+}  // namespace psa
+
+using namespace psa;
+
+struct sdr_trk_plan_tag {
+    int n_ch, n_pos, n_epoch, N, buffN;
+    bool exact;
+    int NP;
+    long long code_ch_stride, code_copy_stride;   // in uint32 words
+    uint8_t *d_buf = nullptr;
+    uint32_t *d_codes = nullptr;
+    TrkChDev *d_ch = nullptr;
+    double *d_pos = nullptr;
+    TrkEpoch *d_ep = nullptr;
+    int2 *d_jk = nullptr;
+    float2 *d_corr = nullptr, *d_C = nullptr;
+    hipStream_t st = nullptr;
+    bool timing = false;
+    hipEvent_t ev[4];
+    double ms_state = 0, ms_corr = 0;
+    long long n_state = 0, n_corr = 0;
+};
+
+extern "C" {
+
+void sdr_corr_std(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N,
+    double coff, const double *pos, int n, sdr_cpx_t *corr, sdr_cpx_t *C)
+{
+    corr_std_impl<false>(IQ, code, N, coff, pos, n, corr, C);
+}
+
+void sdr_corr_std_cpx_code(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code,
+    int N, double coff, const double *pos, int n, sdr_cpx_t *corr,
+    sdr_cpx_t *C)
+{
+    corr_std_impl<true>(IQ, code, N, coff, pos, n, corr, C);
+}
+
+// src/sdr_func.c:1060-1072: float complex buffer -> cpx8 (int8 casts), mix,
+// real code loaded into a cpx16 bank of one code (I = Q = (int8)code[i]),
+// correlate with coff = 0.  The reference's mix_carr_cpx builds the cpx8
+// buffer with SDR_CPX8 (src/pocket_sdr.h:106).
+void sdr_corr_std_cpx(const sdr_cpx_t *buff, int len_buff, int ix, int N,
+    double fs, double fc, double phi, const float *code, const double *pos,
+    int n, sdr_cpx_t *corr)
+{
+    std::vector<uint8_t> b8(N);
+    for (int i = 0, j = ix; i < N; i++, j = (j + 1) % len_buff) {
+        int8_t re = (int8_t)buff[j][0], im = (int8_t)buff[j][1];
+        b8[i] = (uint8_t)(((int8_t)im << 4) | (((int8_t)(re << 4) >> 4) & 0xF));
+    }
+    sdr_buff_t tmp;
+    tmp.data = b8.data();
+    tmp.IQ = 2;
+    tmp.N = N;
+    std::vector<sdr_cpx16_t> IQ((size_t)N * 2);
+    sdr_mix_carr(&tmp, 0, N, fs, fc, phi, IQ.data());
+    for (int i = 0; i < N; i++) IQ[N + i].I = IQ[N + i].Q = (int8_t)code[i];
+    // the reference correlates against IQ + N as a bank; only bank k = 0 is
+    // addressed because coff = 0 and integer pos give k = 0 there
+    std::vector<sdr_cpx16_t> bank((size_t)N * PSA_N_CODES);
+    for (int kk = 0; kk < PSA_N_CODES; kk++) {
+        for (int i = 0; i < N; i++) {
+            size_t src = (size_t)N + (size_t)kk * N + i;
+            bank[(size_t)kk * N + i] = src < IQ.size() ? IQ[src] : sdr_cpx16_t{0, 0};
+        }
+    }
+    sdr_cpx_t C[2];
+    sdr_corr_std(IQ.data(), bank.data(), N, 0.0, pos, n, corr, C);
+}
+
+sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
+    const sdr_trk_ch_t *chs, int n_pos, const double *pos, int n_epoch)
+{
+    if (!buff || !buff->data || buff->N <= 0 || n_ch <= 0 || n_epoch <= 0 ||
+        n_pos <= 0 || n_pos > PSA_MAX_CORR || !chs || !pos) {
+        fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new: invalid arguments\n");
+        return nullptr;
+    }
+    const int N = chs[0].N;
+    for (int c = 0; c < n_ch; c++) {
+        if (chs[c].N != N || !chs[c].code) {
+            fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new: channels need one N and a code bank\n");
+            return nullptr;
+        }
+        for (size_t i = 0; i < (size_t)N * PSA_N_CODES; i++) {
+            int a = chs[c].code[i].I, b = chs[c].code[i].Q;
+            if (a < -1 || a > 1 || b < -1 || b > 1) {
+                fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new: code values must be -1, 0 or 1\n");
+                return nullptr;
+            }
+        }
+    }
+    // planar int8 I / Q for all tap geometries plus one word of slack
+    const int Nq = (N + 3) / 4;
+    if ((size_t)Nq * 8 + 4096 > 150 * 1024) {
+        fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new: N=%d too large for one workgroup\n", N);
+        return nullptr;
+    }
+    ensure_init();
+    sdr_trk_plan_t *p = new sdr_trk_plan_t();
+    p->n_ch = n_ch; p->n_pos = n_pos; p->n_epoch = n_epoch; p->N = N; p->buffN = buff->N;
+    p->NP = (N + 4 + 3) / 4 * 4;
+    // exact sign semantics are needed only if some sample has a -8 nibble
+    p->exact = false;
+    for (int i = 0; i < buff->N && !p->exact; i++) {
+        uint8_t d = buff->data[i];
+        if ((d & 0xF) == 0x8 || (d >> 4) == 0x8) p->exact = true;
+    }
+    PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+    PSA_CHECK(hipMalloc(&p->d_buf, buff->N));
+    PSA_CHECK(hipMemcpy(p->d_buf, buff->data, buff->N, hipMemcpyHostToDevice));
+    // code layout per channel: [copy 0..3][I banks 0..9 | Q banks 0..9][NP bytes]
+    // copy c holds bank bytes shifted by c, so word w of copy c = bytes 4w+c..4w+c+3
+    const size_t bank_bytes = (size_t)2 * PSA_N_CODES * p->NP;
+    p->code_copy_stride = (long long)(bank_bytes / 4);
+    p->code_ch_stride = p->code_copy_stride * 4;
+    std::vector<uint8_t> h((size_t)n_ch * 4 * bank_bytes, 0);
+    for (int c = 0; c < n_ch; c++) {
+        for (int cp = 0; cp < 4; cp++) {
+            uint8_t *dst = h.data() + ((size_t)c * 4 + cp) * bank_bytes;
+            for (int comp = 0; comp < 2; comp++) {
+                for (int k = 0; k < PSA_N_CODES; k++) {
+                    uint8_t *row = dst + ((size_t)comp * PSA_N_CODES + k) * p->NP;
+                    const sdr_cpx16_t *src = chs[c].code + (size_t)k * N;
+                    for (int i = 0; i < N - cp; i++) {
+                        row[i] = (uint8_t)(comp == 0 ? src[i + cp].I : src[i + cp].Q);
+                    }
+                }
+            }
+        }
+    }
+    PSA_CHECK(hipMalloc(&p->d_codes, h.size()));
+    PSA_CHECK(hipMemcpy(p->d_codes, h.data(), h.size(), hipMemcpyHostToDevice));
+    std::vector<TrkChDev> hc(n_ch);
+    for (int c = 0; c < n_ch; c++) {
+        hc[c] = TrkChDev{chs[c].fs, chs[c].fi, chs[c].fc_sig, chs[c].T, chs[c].fd0,
+                         chs[c].drift, chs[c].phi0, chs[c].coff0};
+    }
+    PSA_CHECK(hipMalloc(&p->d_ch, sizeof(TrkChDev) * n_ch));
+    PSA_CHECK(hipMemcpy(p->d_ch, hc.data(), sizeof(TrkChDev) * n_ch, hipMemcpyHostToDevice));
+    PSA_CHECK(hipMalloc(&p->d_pos, sizeof(double) * n_pos));
+    PSA_CHECK(hipMemcpy(p->d_pos, pos, sizeof(double) * n_pos, hipMemcpyHostToDevice));
+    const size_t ne = (size_t)n_ch * n_epoch;
+    PSA_CHECK(hipMalloc(&p->d_ep, sizeof(TrkEpoch) * ne));
+    PSA_CHECK(hipMalloc(&p->d_jk, sizeof(int2) * ne * n_pos));
+    PSA_CHECK(hipMalloc(&p->d_corr, sizeof(float2) * ne * n_pos));
+    PSA_CHECK(hipMalloc(&p->d_C, sizeof(float2) * ne * 2));
+    for (auto &e : p->ev) PSA_CHECK(hipEventCreate(&e));
+    if (p->exact) {
+        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    } else {
+        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    }
+    return p;
+}
+
+int sdr_trk_plan_run(sdr_trk_plan_t *p)
+{
+    if (!p) return 0;
+    hipStream_t st = p->st;
+    const size_t ne = (size_t)p->n_ch * p->n_epoch;
+    if (p->timing) PSA_CHECK(hipEventRecord(p->ev[0], st));
+    hipLaunchKernelGGL(trk_state_kernel, dim3((p->n_ch + 63) / 64), dim3(64), 0, st,
+                       p->d_ch, p->n_ch, p->n_epoch, p->N, p->buffN, p->d_pos, p->n_pos,
+                       p->d_ep, p->d_jk);
+    PSA_CHECK(hipGetLastError());
+    if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
+    const size_t shm = sizeof(uint32_t) * 2 * (size_t)((p->N + 3) / 4);
+    if (p->exact) {
+        hipLaunchKernelGGL(trk_corr_kernel<true>, dim3((unsigned)ne), dim3(TRK_THREADS), shm, st,
+                           p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_jk,
+                           p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
+                           dev_carrier(), p->d_corr, p->d_C);
+    } else {
+        hipLaunchKernelGGL(trk_corr_kernel<false>, dim3((unsigned)ne), dim3(TRK_THREADS), shm, st,
+                           p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_jk,
+                           p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
+                           dev_carrier(), p->d_corr, p->d_C);
+    }
+    PSA_CHECK(hipGetLastError());
+    if (p->timing) PSA_CHECK(hipEventRecord(p->ev[2], st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    if (p->timing) {
+        float a = 0, b = 0;
+        PSA_CHECK(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
+        PSA_CHECK(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
+        p->ms_state += a; p->n_state++;
+        p->ms_corr += b; p->n_corr++;
+    }
+    return 1;
+}
+
+int sdr_trk_plan_result(const sdr_trk_plan_t *p, sdr_cpx_t *corr, sdr_cpx_t *C)
+{
+    if (!p) return 0;
+    const size_t ne = (size_t)p->n_ch * p->n_epoch;
+    if (corr) PSA_CHECK(hipMemcpy(corr, p->d_corr, sizeof(float2) * ne * p->n_pos, hipMemcpyDeviceToHost));
+    if (C) PSA_CHECK(hipMemcpy(C, p->d_C, sizeof(float2) * ne * 2, hipMemcpyDeviceToHost));
+    return 1;
+}
+
+void sdr_trk_plan_timing(sdr_trk_plan_t *p, int on)
+{
+    if (!p) return;
+    p->timing = on != 0;
+    p->ms_state = p->ms_corr = 0;
+    p->n_state = p->n_corr = 0;
+}
+
+int sdr_trk_plan_kernel_ms(const sdr_trk_plan_t *p, const char *name, double *avg_ms,
+                           long long *count, double *elems)
+{
+    if (!p || !name) return 0;
+    std::string n(name);
+    if (n == "trk_state" && p->n_state) {
+        if (avg_ms) *avg_ms = p->ms_state / p->n_state;
+        if (count) *count = p->n_state;
+        if (elems) *elems = (double)p->n_ch * p->n_epoch;
+        return 1;
+    }
+    if (n == "trk_corr" && p->n_corr) {
+        if (avg_ms) *avg_ms = p->ms_corr / p->n_corr;
+        if (count) *count = p->n_corr;
+        if (elems) *elems = (double)p->n_ch * p->n_epoch * p->N;
+        return 1;
+    }
+    return 0;
+}
+
+void sdr_trk_plan_free(sdr_trk_plan_t *p)
+{
+    if (!p) return;
+    (void)hipStreamSynchronize(p->st);
+    (void)hipFree(p->d_buf); (void)hipFree(p->d_codes); (void)hipFree(p->d_ch);
+    (void)hipFree(p->d_pos); (void)hipFree(p->d_ep); (void)hipFree(p->d_jk);
+    (void)hipFree(p->d_corr); (void)hipFree(p->d_C);
+    for (auto &e : p->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(p->st);
+    delete p;
+}
+
+}  // extern "C"

@@ -1,0 +1,490 @@
+// psa_fft.hip - FP32 mixed-radix FFT for gfx950, built for the PCPS
+// correlator of src/sdr_func.c:1088-1102 and sdr_cpx_fft (src/sdr_func.c:236).
+//
+// Length M = M1 * M2 (all prime factors in {2, 3, 5}) is done as a four-step
+// transform: with n = M2 n1 + n2 and k = k1 + M1 k2,
+//   X[k1 + M1 k2] = sum_n2 w_M2^{n2 k2} [ w_M^{n2 k1} sum_n1 x[M2 n1 + n2] w_M1^{n1 k1} ]
+// i.e. a "column" pass (M2 transforms of length M1, stride M2) that also
+// applies the twiddle w_M^{n2 k1}, and a "row" pass (M1 contiguous transforms
+// of length M2).  Each pass is one kernel: a workgroup loads G sub-transforms
+// into LDS with coalesced accesses, runs Stockham radix-2/3/4/5/8 stages in
+// LDS, and stores.  Load and store are templated so the correlator fuses
+// carrier mixing, the code-spectrum product and the power accumulation into
+// the first and last pass.
+#include <math.h>
+#include <map>
+#include "psa_internal.h"
+
+namespace psa {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by -i (forward) or +i (backward): sgn = -1 forward, +1 backward
+__device__ __forceinline__ float2 mul_i(float2 a, float sgn) { return make_float2(-sgn * a.y, sgn * a.x); }
+
+// ---------------------------------------------------------------------------
+// radix-r DFT butterflies, sgn = -1 (forward) / +1 (backward)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bfly2(float2 *a)
+{
+    float2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+}
+
+__device__ __forceinline__ void bfly4(float2 *a, float sgn)
+{
+    float2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]), t3 = mul_i(csub(a[1], a[3]), sgn);
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+}
+
+__device__ __forceinline__ void bfly8(float2 *a, float sgn)
+{
+    const float R = 0.70710678118654752f;
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    bfly4(e, sgn);
+    bfly4(o, sgn);
+    // o[k] *= w8^k, w8 = exp(sgn i pi / 4)
+    o[1] = make_float2(R * (o[1].x - sgn * o[1].y), R * (o[1].y + sgn * o[1].x));
+    o[2] = mul_i(o[2], sgn);
+    o[3] = make_float2(R * (-o[3].x - sgn * o[3].y), R * (-o[3].y + sgn * o[3].x));
+    for (int k = 0; k < 4; k++) {
+        a[k] = cadd(e[k], o[k]);
+        a[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+__device__ __forceinline__ void bfly3(float2 *a, float sgn)
+{
+    const float C = -0.5f, S = 0.86602540378443865f;
+    float2 t = cadd(a[1], a[2]);
+    float2 d = csub(a[1], a[2]);
+    float2 m = make_float2(a[0].x + C * t.x, a[0].y + C * t.y);
+    float2 r = make_float2(-sgn * S * d.y, sgn * S * d.x);   // sgn i S d
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+}
+
+__device__ __forceinline__ void bfly5(float2 *a, float sgn)
+{
+    const float C1 = 0.30901699437494742f, C2 = -0.80901699437494742f;
+    const float S1 = 0.95105651629515357f, S2 = 0.58778525229247313f;
+    float2 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+    float2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+    float2 m1 = make_float2(a[0].x + C1 * t1.x + C2 * t2.x, a[0].y + C1 * t1.y + C2 * t2.y);
+    float2 m2 = make_float2(a[0].x + C2 * t1.x + C1 * t2.x, a[0].y + C2 * t1.y + C1 * t2.y);
+    // s1 = sgn i (S1 d1 + S2 d2), s2 = sgn i (S2 d1 - S1 d2)
+    float2 u1 = make_float2(S1 * d1.x + S2 * d2.x, S1 * d1.y + S2 * d2.y);
+    float2 u2 = make_float2(S2 * d1.x - S1 * d2.x, S2 * d1.y - S1 * d2.y);
+    float2 s1 = mul_i(u1, sgn), s2 = mul_i(u2, sgn);
+    a[0] = cadd(a[0], cadd(t1, t2));
+    a[1] = cadd(m1, s1);
+    a[4] = csub(m1, s1);
+    a[2] = cadd(m2, s2);
+    a[3] = csub(m2, s2);
+}
+
+// One Stockham stage of radix R over G sub-transforms of length L in LDS.
+template <int R>
+__device__ __forceinline__ void stage(const float2 *__restrict__ src, float2 *__restrict__ dst,
+                                      int L, int G, int Ns, const float2 *__restrict__ tw,
+                                      float sgn)
+{
+    const int Lr = L / R;
+    const int tstep = L / (Ns * R);
+    const int nb = G * Lr;
+    for (int bi = threadIdx.x; bi < nb; bi += blockDim.x) {
+        int g = bi / Lr, j = bi - g * Lr;
+        int k = j % Ns;
+        const float2 *s = src + g * L + j;
+        float2 a[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) a[q] = s[q * Lr];
+        if (Ns > 1) {
+#pragma unroll
+            for (int q = 1; q < R; q++) a[q] = cmul(a[q], tw[q * k * tstep]);
+        }
+        if (R == 2) bfly2(a);
+        else if (R == 3) bfly3(a, sgn);
+        else if (R == 4) bfly4(a, sgn);
+        else if (R == 5) bfly5(a, sgn);
+        else if (R == 8) bfly8(a, sgn);
+        float2 *d = dst + g * L + (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int q = 0; q < R; q++) d[q * Ns] = a[q];
+    }
+}
+
+struct RadixSeq {
+    int n;
+    int r[16];
+};
+
+// ---------------------------------------------------------------------------
+// the pass kernel
+// ---------------------------------------------------------------------------
+template <int LM, int SM>
+__global__ void __launch_bounds__(256) pass_kernel(PassArgs a, RadixSeq rs, int ngroups)
+{
+    extern __shared__ float2 smem[];
+    __shared__ uint32_t carr[256];
+    const int L = a.L, G = a.G;
+    float2 *buf0 = smem, *buf1 = smem + G * L;
+    const long long b = blockIdx.x / ngroups;
+    const int grp = blockIdx.x - (int)(b * ngroups);
+    const int g0 = grp * G;
+    const int gcount = min(G, a.nsub - g0);
+    const float sgn = a.dir == 0 ? -1.0f : 1.0f;
+
+    if (LM == LD_MIX) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) carr[i] = a.carr[i];
+        __syncthreads();
+    }
+
+    // ---- load G sub-transforms (element (g, n) -> LDS[g * L + n]) ----
+    const bool in_nfast = (a.in_nstride == 1);
+    const int tot = G * L;
+    int ix = 0, nseg = 0;
+    uint32_t p0 = 0, p1 = 0, s = 0;
+    if (LM == LD_MIX) {
+        ix = a.mix_ix[b]; nseg = a.mix_nseg[b];
+        p0 = a.mix_p0[b]; p1 = a.mix_p1[b]; s = a.mix_s[b];
+    }
+    const float2 *xin = nullptr, *cin = nullptr;
+    if (LM == LD_PROD) {
+        long long xb = (b / a.prod_xdiv) % a.prod_xmod;
+        long long cb = b / a.prod_cdiv;
+        xin = a.in + xb * a.in_bstride;
+        cin = a.cf + cb * a.cf_bstride;
+    }
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        int g, n;
+        if (in_nfast) { g = idx / L; n = idx - g * L; }
+        else { n = idx / G; g = idx - n * G; }
+        float2 v = make_float2(0.f, 0.f);
+        if (g < gcount) {
+            long long e = (long long)(g0 + g) * a.in_gstride + (long long)n * a.in_nstride;
+            if (LM == LD_CPX) {
+                v = a.in[b * a.in_bstride + e];
+            } else if (LM == LD_CPX16) {
+                const char2 q = reinterpret_cast<const char2 *>(a.iq16)[b * a.in_bstride + e];
+                v = make_float2((float)q.x * PSA_CSCALE, (float)q.y * PSA_CSCALE);
+            } else if (LM == LD_MIX) {
+                int en = (int)e;
+                int src;
+                uint32_t ph;
+                if (en < nseg) { src = ix + en; ph = p0 + (uint32_t)en * s; }
+                else { src = a.mix_compact ? en : en - nseg; ph = p1 + (uint32_t)(en - nseg) * s; }
+                uint32_t d = a.buf[src];
+                int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+                int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+                uint32_t c = carr[ph >> 24];
+                int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+                int8_t mI = (int8_t)(sI * cI - sQ * cQ);
+                int8_t mQ = (int8_t)(sI * cQ + sQ * cI);
+                v = make_float2((float)mI * PSA_CSCALE, (float)mQ * PSA_CSCALE);
+            } else if (LM == LD_PROD) {
+                // sdr_cpx_mul AVX2 lanes (src/sdr_func.c:203-213):
+                // re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) + fl(ai br), * s
+                // (tail, src/sdr_func.c:229-232, contracted by the reference
+                // build: fmaf(ar, br, -(ai bi)), fmaf(ar, bi, ai br))
+                float2 x = xin[e], c = cin[e];
+                long long nat = e / a.prod_M2 + (long long)a.prod_M1 * (e % a.prod_M2);
+                float re, im;
+                if (nat < a.prod_V) {
+                    re = __fadd_rn(__fmul_rn(x.x, c.x), __fmul_rn(x.y, -c.y));
+                    im = __fadd_rn(__fmul_rn(x.x, c.y), __fmul_rn(x.y, c.x));
+                } else {
+                    re = __fmaf_rn(x.x, c.x, -__fmul_rn(x.y, c.y));
+                    im = __fmaf_rn(x.x, c.y, __fmul_rn(x.y, c.x));
+                }
+                v = make_float2(__fmul_rn(re, a.prod_scale), __fmul_rn(im, a.prod_scale));
+            }
+        }
+        buf0[g * L + n] = v;
+    }
+    __syncthreads();
+
+    // ---- Stockham stages in LDS ----
+    float2 *src = buf0, *dst = buf1;
+    int Ns = 1;
+    for (int p = 0; p < rs.n; p++) {
+        int r = rs.r[p];
+        switch (r) {
+        case 2: stage<2>(src, dst, L, G, Ns, a.tw, sgn); break;
+        case 3: stage<3>(src, dst, L, G, Ns, a.tw, sgn); break;
+        case 4: stage<4>(src, dst, L, G, Ns, a.tw, sgn); break;
+        case 5: stage<5>(src, dst, L, G, Ns, a.tw, sgn); break;
+        case 8: stage<8>(src, dst, L, G, Ns, a.tw, sgn); break;
+        }
+        __syncthreads();
+        float2 *t = src; src = dst; dst = t;
+        Ns *= r;
+    }
+
+    // ---- store ----
+    const bool out_nfast = (a.out_nstride == 1);
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        int g, k;
+        if (out_nfast) { g = idx / L; k = idx - g * L; }
+        else { k = idx / G; g = idx - k * G; }
+        if (g >= gcount) continue;
+        float2 v = src[g * L + k];
+        const int gg = g0 + g;
+        if (a.thi) {
+            long long t = ((long long)gg * k) % a.M;
+            int ti = (int)t;
+            float2 w = cmul(a.thi[ti >> 10], a.tlo[ti & 1023]);
+            v = cmul(v, w);
+        }
+        long long e = (long long)gg * a.out_gstride + (long long)k * a.out_nstride;
+        if (SM == ST_CPX) {
+            a.out[b * a.out_bstride + e] = v;
+        } else {
+            if (e < a.nlag) {
+                long long row = (b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
+                float *pp = a.P + row * a.p_bstride + e;
+                // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680)
+                *pp = __fadd_rn(*pp, __fmaf_rn(v.x, v.x, __fmul_rn(v.y, v.y)));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side: plans and launches
+// ---------------------------------------------------------------------------
+static bool factor235(int n)
+{
+    if (n < 1) return false;
+    for (int p : {2, 3, 5}) while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+bool fft_supported(int M) { return M >= 1 && factor235(M); }
+
+static void radix_seq(int L, SubPlan &sp)
+{
+    sp.L = L;
+    sp.nrad = 0;
+    int n = L;
+    while (n % 8 == 0) { sp.rad[sp.nrad++] = 8; n /= 8; }
+    if (n % 4 == 0) { sp.rad[sp.nrad++] = 4; n /= 4; }
+    if (n % 2 == 0) { sp.rad[sp.nrad++] = 2; n /= 2; }
+    while (n % 5 == 0) { sp.rad[sp.nrad++] = 5; n /= 5; }
+    while (n % 3 == 0) { sp.rad[sp.nrad++] = 3; n /= 3; }
+}
+
+static float2 *make_twiddle(int L, int sign)
+{
+    const double PI_D = 3.14159265358979323846;
+    std::vector<float2> h(L > 0 ? L : 1);
+    for (int m = 0; m < L; m++) {
+        double ang = sign * 2.0 * PI_D * (double)m / L;
+        h[m] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    float2 *d;
+    PSA_CHECK(hipMalloc(&d, sizeof(float2) * h.size()));
+    PSA_CHECK(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
+    return d;
+}
+
+// w_M^{a} for a = hi * 1024 + lo: T_hi[hi] = exp(s 2 pi i hi 1024 / M),
+// T_lo[lo] = exp(s 2 pi i lo / M), both rounded from double
+static void make_split_twiddle(int M, int sign, float2 **thi, float2 **tlo)
+{
+    const double PI_D = 3.14159265358979323846;
+    int nhi = (M + 1023) / 1024;
+    std::vector<float2> hi(nhi), lo(1024);
+    for (int i = 0; i < nhi; i++) {
+        double ang = sign * 2.0 * PI_D * (double)((long long)i * 1024 % M) / M;
+        hi[i] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    for (int i = 0; i < 1024; i++) {
+        double ang = sign * 2.0 * PI_D * (double)i / M;
+        lo[i] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    PSA_CHECK(hipMalloc(thi, sizeof(float2) * nhi));
+    PSA_CHECK(hipMalloc(tlo, sizeof(float2) * 1024));
+    PSA_CHECK(hipMemcpy(*thi, hi.data(), sizeof(float2) * nhi, hipMemcpyHostToDevice));
+    PSA_CHECK(hipMemcpy(*tlo, lo.data(), sizeof(float2) * 1024, hipMemcpyHostToDevice));
+}
+
+// split M into M1 * M2 with both <= 4096, M1 >= M2, as balanced as possible
+static void split_len(int M, int *M1, int *M2)
+{
+    int best1 = M, best2 = 1;
+    if (M > 4096) {
+        best1 = 0;
+        for (int d = 1; d <= M; d++) {
+            if (M % d) continue;
+            int e = M / d;
+            if (d > 4096 || e > 4096 || d < e) continue;
+            if (!best1 || d < best1) { best1 = d; best2 = e; }
+        }
+    }
+    *M1 = best1;
+    *M2 = best2;
+}
+
+static std::mutex g_plan_mtx;
+static std::map<int, FFTPlan *> g_plans;
+
+const FFTPlan *fft_plan(int M)
+{
+    if (!fft_supported(M)) return nullptr;
+    ensure_init();
+    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    auto it = g_plans.find(M);
+    if (it != g_plans.end()) return it->second;
+    int M1, M2;
+    split_len(M, &M1, &M2);
+    if (M1 == 0) return nullptr;
+    FFTPlan *p = new FFTPlan();
+    p->M = M; p->M1 = M1; p->M2 = M2;
+    radix_seq(M1, p->s1);
+    radix_seq(M2, p->s2);
+    p->s1.tw_fwd = make_twiddle(M1, -1);
+    p->s1.tw_bwd = make_twiddle(M1, +1);
+    p->s2.tw_fwd = make_twiddle(M2, -1);
+    p->s2.tw_bwd = make_twiddle(M2, +1);
+    make_split_twiddle(M, -1, &p->thi_fwd, &p->tlo_fwd);
+    make_split_twiddle(M, +1, &p->thi_bwd, &p->tlo_bwd);
+    g_plans[M] = p;
+    return p;
+}
+
+template <int LM, int SM>
+static void launch_t(const PassArgs &a, const RadixSeq &rs, int nbatch, hipStream_t st)
+{
+    int ngroups = (a.nsub + a.G - 1) / a.G;
+    long long nblk = (long long)ngroups * nbatch;
+    if (nblk <= 0) return;
+    if (nblk > 0x7FFFFFFFLL) PSA_FAIL("FFT launch too large (%lld workgroups)", nblk);
+    size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [] {
+        PSA_CHECK(hipFuncSetAttribute((const void *)pass_kernel<LM, SM>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+    });
+    hipLaunchKernelGGL((pass_kernel<LM, SM>), dim3((unsigned)nblk), dim3(256), shm, st, a, rs, ngroups);
+    PSA_CHECK(hipGetLastError());
+}
+
+void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hipStream_t st)
+{
+    PassArgs a = a_in;
+    a.carr = dev_carrier();
+    RadixSeq rs;
+    // the radix sequence is looked up from the cached plan tables by length
+    SubPlan sp;
+    radix_seq(a.L, sp);
+    rs.n = sp.nrad;
+    for (int i = 0; i < sp.nrad; i++) rs.r[i] = sp.rad[i];
+#define PSA_LAUNCH(L_, S_) if (lm == L_ && sm == S_) { launch_t<L_, S_>(a, rs, nbatch, st); return; }
+    PSA_LAUNCH(LD_CPX, ST_CPX)
+    PSA_LAUNCH(LD_CPX16, ST_CPX)
+    PSA_LAUNCH(LD_MIX, ST_CPX)
+    PSA_LAUNCH(LD_PROD, ST_CPX)
+    PSA_LAUNCH(LD_CPX, ST_POW)
+    PSA_LAUNCH(LD_PROD, ST_POW)
+#undef PSA_LAUNCH
+    PSA_FAIL("unsupported FFT pass mode %d/%d", (int)lm, (int)sm);
+}
+
+// G sub-transforms per workgroup: as many as fit 4096 complex in LDS
+int pass_groups(int L, int nsub)
+{
+    int G = 4096 / (L > 0 ? L : 1);
+    if (G < 1) G = 1;
+    if (G > nsub) G = nsub;
+    return G;
+}
+
+// column pass geometry (M2 transforms of length M1, stride M2), with the
+// inter-pass twiddle applied on store; output [k1][n2] row-major
+void col_pass_args(const FFTPlan *pl, int dir, PassArgs &a)
+{
+    a.L = pl->M1;
+    a.nsub = pl->M2;
+    a.G = pass_groups(a.L, a.nsub);
+    a.in_nstride = pl->M2; a.in_gstride = 1;
+    a.out_nstride = pl->M2; a.out_gstride = 1;
+    a.dir = dir;
+    a.tw = dir == 0 ? pl->s1.tw_fwd : pl->s1.tw_bwd;
+    a.M = pl->M;
+    if (pl->M2 > 1) {
+        a.thi = dir == 0 ? pl->thi_fwd : pl->thi_bwd;
+        a.tlo = dir == 0 ? pl->tlo_fwd : pl->tlo_bwd;
+    } else {
+        a.thi = a.tlo = nullptr;
+    }
+}
+
+// row pass geometry (M1 contiguous transforms of length M2)
+void row_pass_args(const FFTPlan *pl, int dir, PassArgs &a)
+{
+    a.L = pl->M2;
+    a.nsub = pl->M1;
+    a.G = pass_groups(a.L, a.nsub);
+    a.in_nstride = 1; a.in_gstride = pl->M2;
+    a.out_nstride = 1; a.out_gstride = pl->M2;
+    a.dir = dir;
+    a.tw = dir == 0 ? pl->s2.tw_fwd : pl->s2.tw_bwd;
+    a.M = pl->M;
+    a.thi = a.tlo = nullptr;
+}
+
+void fft_c2c(const FFTPlan *pl, const float2 *in, float2 *out, float2 *tmp,
+             int nbatch, int dir, hipStream_t st)
+{
+    PassArgs a = {};
+    col_pass_args(pl, dir, a);
+    a.in = in; a.out = tmp;
+    a.in_bstride = a.out_bstride = pl->M;
+    launch_pass(LD_CPX, ST_CPX, a, nbatch, st);
+    PassArgs r = {};
+    row_pass_args(pl, dir, r);
+    r.in = tmp; r.out = out;
+    r.in_bstride = r.out_bstride = pl->M;
+    // natural order: element k2 of row k1 goes to k1 + M1 k2
+    r.out_nstride = pl->M1;
+    r.out_gstride = 1;
+    launch_pass(LD_CPX, ST_CPX, r, nbatch, st);
+}
+
+__global__ void transpose_spec_kernel(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                      int M, int M1, int M2, long long total)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    long long b = t / M;
+    int i = (int)(t - b * M);          // transposed index k1 * M2 + k2
+    int k1 = i / M2, k2 = i - k1 * M2;
+    out[t] = in[b * M + k1 + (long long)M1 * k2];
+}
+
+void fft_to_transposed(const FFTPlan *pl, const float2 *in, float2 *out,
+                       int nbatch, hipStream_t st)
+{
+    long long total = (long long)pl->M * nbatch;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(transpose_spec_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       in, out, pl->M, pl->M1, pl->M2, total);
+    PSA_CHECK(hipGetLastError());
+}
+
+}  // namespace psa

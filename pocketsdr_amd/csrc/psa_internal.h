@@ -1,0 +1,184 @@
+// psa_internal.h - shared definitions for the MI355X (gfx950) PocketSDR hot path.
+//
+// Layouts (identical to the reference ABI, src/pocket_sdr.h:105-113):
+//   cpx8   1 byte / complex sample, I = low nibble, Q = high nibble (int4 each)
+//   cpx16  {int8 I, int8 Q}
+//   cpx    {float re, float im}
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <mutex>
+#include <vector>
+
+#include "../../include/pocket_sdr_amd.h"
+
+#define PSA_CSCALE (1 / 11.2f)     // SDR_CSCALE, src/pocket_sdr.h:61
+#define PSA_N_CODES 10             // SDR_N_CODES, src/pocket_sdr.h:60
+#define PSA_MAX_CORR 107           // SDR_MAX_CORR, src/pocket_sdr.h:58
+
+// Any HIP failure ends the process with a message, the reference's convention
+// for allocation failures (src/sdr_cmn.c:43-46): no partial results.
+#define PSA_CHECK(call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            fprintf(stderr, "pocketsdr_amd: HIP error %s at %s:%d: %s\n",      \
+                    hipGetErrorName(e_), __FILE__, __LINE__, #call);           \
+            exit(-1);                                                          \
+        }                                                                      \
+    } while (0)
+
+#define PSA_FAIL(...)                                                          \
+    do {                                                                       \
+        fprintf(stderr, "pocketsdr_amd: ");                                    \
+        fprintf(stderr, __VA_ARGS__);                                          \
+        fprintf(stderr, "\n");                                                 \
+        exit(-1);                                                              \
+    } while (0)
+
+namespace psa {
+
+// ---------------------------------------------------------------------------
+// runtime (psa_runtime.hip)
+// ---------------------------------------------------------------------------
+
+// carrier table: round(11.2 cos/sin(-2 pi i / 256)) packed as
+// (uint8 cI) | (uint8 cQ) << 8, src/sdr_func.c:122-124
+void ensure_init();
+const uint32_t *host_carrier();
+const uint32_t *dev_carrier();     // the same 256 words in device memory
+
+// Per-calling-thread HIP stream plus grow-only device scratch.  The reference
+// hot path is called concurrently from one pthread per receiver channel
+// (src/sdr_rcv.c:495-521), so every host thread owns its own stream and
+// buffers; nothing global is locked while kernels run.
+struct Scratch {
+    void *ptr = nullptr;
+    size_t size = 0;
+    void *get(size_t bytes);
+};
+
+struct ThreadCtx {
+    hipStream_t stream = nullptr;
+    Scratch s[8];
+    void *pinned = nullptr;
+    size_t pinned_size = 0;
+    void *host_pinned(size_t bytes);
+};
+ThreadCtx &tctx();
+
+// scratch slot ids
+enum { SC_IN = 0, SC_IQ, SC_CODE, SC_A, SC_B, SC_C, SC_P, SC_MISC };
+
+// ---------------------------------------------------------------------------
+// FFT (psa_fft.hip): four-step mixed-radix FP32 FFT, M = M1 * M2
+// ---------------------------------------------------------------------------
+struct SubPlan {
+    int L = 0;              // sub-transform length
+    int nrad = 0;
+    int rad[16];            // radix sequence (product = L)
+    float2 *tw_fwd = nullptr;   // exp(-2 pi i m / L), m < L (device)
+    float2 *tw_bwd = nullptr;   // exp(+2 pi i m / L)
+};
+
+struct FFTPlan {
+    int M = 0, M1 = 0, M2 = 0;
+    SubPlan s1, s2;             // lengths M1 (column pass) and M2 (row pass)
+    // inter-pass twiddle w_M^{a} = T_hi[a >> 10] * T_lo[a & 1023], a < M
+    float2 *thi_fwd = nullptr, *tlo_fwd = nullptr;
+    float2 *thi_bwd = nullptr, *tlo_bwd = nullptr;
+};
+bool fft_supported(int M);
+const FFTPlan *fft_plan(int M);   // cached, thread-safe; nullptr if unsupported
+
+// load / store modes of the sub-transform passes
+enum LoadMode {
+    LD_CPX = 0,      // float2 from global
+    LD_CPX16 = 1,    // cpx16 int8 pairs -> float * CSCALE
+    LD_MIX = 2,      // cpx8 IF samples, carrier-mixed through the LUT, * CSCALE
+    LD_PROD = 3,     // (X * Cf) * s, the reference sdr_cpx_mul arithmetic
+};
+enum StoreMode {
+    ST_CPX = 0,      // float2 to global (optionally times inter-pass twiddle)
+    ST_POW = 1,      // P += |x|^2 (reference contraction P + fma(re, re, im*im))
+};
+
+// Arguments shared by every pass kernel.  A launch processes `nbatch`
+// transforms; batch index b selects the operands through the *_bstride /
+// *_bdiv fields as documented in psa_fft.hip.
+struct PassArgs {
+    // geometry
+    int L;                  // sub-transform length
+    int G;                  // sub-transforms per workgroup
+    int nsub;               // sub-transforms per batch item (columns or rows)
+    long long in_nstride, in_gstride;   // element strides in the input
+    long long out_nstride, out_gstride;
+    int dir;                // 0 forward (exp -), 1 backward (exp +)
+    const float2 *tw;       // sub-transform twiddles for dir
+    // inter-pass twiddle (applied on store when thi != nullptr):
+    // factor w_M^{(n_idx * g_idx) mod M}
+    const float2 *thi, *tlo;
+    int M;
+    // plain complex input / output
+    const float2 *in;
+    float2 *out;
+    long long in_bstride, out_bstride;
+    // LD_MIX: IF ring buffer and per-batch mixing parameters
+    const uint32_t *carr;       // carrier table (set by launch_pass)
+    const uint8_t *buf;
+    int buffN;
+    const int *mix_ix;          // [nbatch] start index in the ring
+    const uint32_t *mix_p0;     // [nbatch] phase of sample 0
+    const uint32_t *mix_p1;     // [nbatch] phase at the wrap split
+    const uint32_t *mix_s;      // [nbatch] phase step
+    const int *mix_nseg;        // [nbatch] samples before the wrap split
+    int mix_compact;            // 1: buf holds the block contiguously (no ring)
+    // LD_CPX16: cpx16 input
+    const int8_t *iq16;
+    // LD_PROD: X (in, batch b -> b / prod_xdiv % prod_xmod) times Cf
+    const float2 *cf;           // code spectrum, batch b -> (b / prod_cdiv)
+    int prod_xdiv, prod_xmod, prod_cdiv;
+    long long cf_bstride;
+    float prod_scale;
+    int prod_V;                 // natural indices >= prod_V use the scalar-tail form
+    int prod_M1, prod_M2;       // transposed index e -> natural e / M2 + M1 (e % M2)
+    // ST_POW: P accumulate; batch b -> row (b / p_div + p_off1) * p_mul
+    // + p_off2 + b % p_div, row start P + row * p_bstride
+    float *P;
+    long long p_bstride;
+    int p_div, p_off1, p_mul, p_off2;
+    int nlag;               // only lags < nlag are accumulated
+};
+
+void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
+                 hipStream_t st);
+int pass_groups(int L, int nsub);
+// column pass: M2 transforms of length M1 at stride M2, output [k1][n2]
+// row-major, inter-pass twiddle w_M^{n2 k1} on store (when M2 > 1)
+void col_pass_args(const FFTPlan *pl, int dir, PassArgs &a);
+// row pass: M1 contiguous transforms of length M2, output [k1][k2]
+void row_pass_args(const FFTPlan *pl, int dir, PassArgs &a);
+
+// Full forward / backward transforms of `nbatch` contiguous length-M arrays
+// (natural order in and out), used by sdr_cpx_fft and the code spectra.
+void fft_c2c(const FFTPlan *pl, const float2 *in, float2 *out, float2 *tmp,
+             int nbatch, int dir, hipStream_t st);
+// natural-order spectrum -> the digit-transposed order used between the
+// forward and inverse passes of the correlator (index k1*M2 + k2 holds
+// bin k1 + M1*k2)
+void fft_to_transposed(const FFTPlan *pl, const float2 *in, float2 *out,
+                       int nbatch, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// misc kernels (psa_mix.hip)
+// ---------------------------------------------------------------------------
+void launch_mix(const uint8_t *buf, int buffN, int ix, int N, uint32_t p0,
+                uint32_t p1, int nseg, uint32_t s, int8_t *IQ, hipStream_t st);
+void mix_params(int buffN, int ix, int N, double fs, double fc, double phi,
+                uint32_t *p0, uint32_t *p1, int *nseg, uint32_t *s);
+void launch_cpx_mul(const float2 *a, const float2 *b, int N, float s,
+                    float2 *c, hipStream_t st);
+
+}  // namespace psa

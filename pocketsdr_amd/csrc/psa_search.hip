@@ -1,0 +1,831 @@
+// psa_search.hip - FFT correlator and parallel code search on gfx950.
+//
+// Drop-in entry points: sdr_cpx_fft, sdr_cpx_mul, sdr_mix_carr, sdr_corr_fft,
+// sdr_corr_fft_cpx, sdr_search_code (src/sdr_func.c:197-264, 664-687,
+// 772-848, 1088-1113).  Batched entry points: the sdr_search_plan_* family.
+//
+// Correlator dataflow (per Doppler bin b and block):
+//   fwd column pass  : load IF bytes, carrier-mix (LUT semantics), * CSCALE,
+//                      length-M1 FFTs, twiddle         -> tmp
+//   fwd row pass     : length-M2 FFTs                  -> X (digit-transposed)
+//   inv row pass     : (X * Cf) * (1/M/M) with the sdr_cpx_mul arithmetic,
+//                      length-M2 inverse FFTs, twiddle -> Z
+//   inv column pass  : length-M1 inverse FFTs, P += re^2 + im^2 (natural lag)
+// The code spectrum Cf is held in the same digit-transposed order as X, so no
+// transpose of data is ever made.  In the batched plan the two forward passes
+// run once per (bin, block) and are shared by every PRN of the signal.
+#include <math.h>
+#include <string.h>
+#include <string>
+#include <algorithm>
+#include "psa_internal.h"
+
+namespace psa {
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void fill_kernel(float *p, long long n, float v)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+static void fill(float *p, long long n, float v, hipStream_t st)
+{
+    if (n <= 0) return;
+    long long blocks = std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    PSA_CHECK(hipGetLastError());
+}
+
+// P[row][j] += re^2 + im^2 (reference contraction P + fmaf(re, re, im * im))
+__global__ void power_kernel(const float2 *__restrict__ C, int M, int nlag, long long nrow,
+                             float *__restrict__ P, long long pstride)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = nrow * nlag;
+    if (t >= total) return;
+    long long r = t / nlag;
+    int j = (int)(t - r * nlag);
+    float2 v = C[r * M + j];
+    float *pp = P + r * pstride + j;
+    *pp = __fadd_rn(*pp, __fmaf_rn(v.x, v.x, __fmul_rn(v.y, v.y)));
+}
+
+// Peak search and C/N0 (src/sdr_func.c:690-706) plus fine Doppler
+// (src/sdr_func.c:728-744) over one PRN's surface P[len_fds][N].  The first
+// maximum in row-major order wins (strict '>'); values <= 0 never become the
+// peak.  The mean is an FP64 sum divided by the count.
+#define PEAK_THREADS 1024
+__global__ void __launch_bounds__(PEAK_THREADS) peak_kernel(const float *__restrict__ Pall, int N,
+                                                            int len_fds, double T,
+                                                            const float *__restrict__ fds,
+                                                            float *__restrict__ cn0,
+                                                            int *__restrict__ ixo,
+                                                            double *__restrict__ dop)
+{
+    __shared__ float smax[PEAK_THREADS / 64];
+    __shared__ long long sidx[PEAK_THREADS / 64];
+    __shared__ double ssum[PEAK_THREADS / 64];
+    const int prn = blockIdx.x;
+    const float *P = Pall + (long long)prn * len_fds * N;
+    const long long tot = (long long)len_fds * N;
+    float bm = 0.0f;
+    long long bi = -1;
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
+        float v = P[i];
+        s += (double)v;
+        if (v > bm) { bm = v; bi = i; }
+    }
+    // wave reduce: max value, smallest index on ties
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        float om = __shfl_xor(bm, o, 64);
+        long long oi = __shfl_xor(bi, o, 64);
+        double os = __shfl_xor(s, o, 64);
+        s += os;
+        if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { smax[w] = bm; sidx[w] = bi; ssum[w] = s; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bm = smax[0]; bi = sidx[0]; s = ssum[0];
+        for (int i = 1; i < PEAK_THREADS / 64; i++) {
+            s += ssum[i];
+            long long oi = sidx[i];
+            float om = smax[i];
+            if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+        }
+        double Pave = s / (double)tot;
+        double Pmax = bi >= 0 ? (double)bm : 0.0;
+        int i0 = bi >= 0 ? (int)(bi / N) : 0, i1 = bi >= 0 ? (int)(bi % N) : 0;
+        cn0[prn] = Pave > 0.0 ? (float)(10.0 * log10((Pmax - Pave) / Pave / T)) : 0.0f;
+        ixo[2 * prn] = i0;
+        ixo[2 * prn + 1] = i1;
+        double d = fds[i0];
+        if (i0 > 0 && i0 < len_fds - 1) {
+            double x0 = fds[i0 - 1], x1 = fds[i0], x2 = fds[i0 + 1];
+            double y0 = P[(long long)(i0 - 1) * N + i1], y1 = P[(long long)i0 * N + i1];
+            double y2 = P[(long long)(i0 + 1) * N + i1];
+            double d01 = x0 - x1, d02 = x0 - x2, d12 = x1 - x2;
+            double den = d01 * d02 * d12;
+            if (den != 0.0) {
+                double p2 = (x2 * (y1 - y0) + x1 * (y0 - y2) + x0 * (y2 - y1)) / den;
+                double p1 = (x2 * x2 * (y0 - y1) + x1 * x1 * (y2 - y0) + x0 * x0 * (y1 - y2)) / den;
+                if (p2 != 0.0) d = -p1 / (2.0 * p2);
+            }
+        }
+        dop[prn] = d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pass argument helpers for the correlator
+// ---------------------------------------------------------------------------
+static void set_prod(const FFTPlan *pl, PassArgs &a, float scale)
+{
+    a.prod_scale = scale;
+    a.prod_V = pl->M >= 4 ? pl->M / 4 * 4 : 0;
+    a.prod_M1 = pl->M1;
+    a.prod_M2 = pl->M2;
+}
+
+static void args_fwd_col(const FFTPlan *pl, PassArgs &a)
+{
+    col_pass_args(pl, 0, a);
+}
+
+static void args_fwd_row(const FFTPlan *pl, PassArgs &a)
+{
+    row_pass_args(pl, 0, a);   // digit-transposed output
+}
+
+static void args_inv_row(const FFTPlan *pl, PassArgs &a)
+{
+    row_pass_args(pl, 1, a);
+    if (pl->M2 > 1) {          // twiddle w_M^{-n_a k1} on store
+        a.thi = pl->thi_bwd;
+        a.tlo = pl->tlo_bwd;
+    }
+}
+
+static void args_inv_col(const FFTPlan *pl, PassArgs &a)
+{
+    col_pass_args(pl, 1, a);   // natural-order output n_a + M2 n_b
+    a.thi = a.tlo = nullptr;
+}
+
+static const FFTPlan *need_plan(int M, const char *who)
+{
+    const FFTPlan *pl = fft_plan(M);
+    if (!pl) PSA_FAIL("%s: FFT length %d has prime factors other than 2, 3, 5", who, M);
+    return pl;
+}
+
+// Per-thread cache of device code spectra in digit-transposed order.
+struct SpecCache {
+    const void *host = nullptr;
+    int M = 0;
+    std::vector<uint8_t> shadow;
+    float2 *dev = nullptr;
+    ~SpecCache() { if (dev) (void)hipFree(dev); }
+    const float2 *get(const sdr_cpx_t *code_fft, const FFTPlan *pl, hipStream_t st)
+    {
+        size_t bytes = sizeof(float2) * (size_t)pl->M;
+        if (host == code_fft && M == pl->M && dev && memcmp(shadow.data(), code_fft, bytes) == 0) {
+            return dev;
+        }
+        if (dev && M != pl->M) { PSA_CHECK(hipFree(dev)); dev = nullptr; }
+        if (!dev) PSA_CHECK(hipMalloc(&dev, bytes * 2));
+        shadow.assign((const uint8_t *)code_fft, (const uint8_t *)code_fft + bytes);
+        host = code_fft;
+        M = pl->M;
+        PSA_CHECK(hipMemcpyAsync(dev + pl->M, code_fft, bytes, hipMemcpyHostToDevice, st));
+        fft_to_transposed(pl, dev + pl->M, dev, 1, st);
+        return dev;
+    }
+};
+
+static SpecCache &spec_cache()
+{
+    thread_local SpecCache c;
+    return c;
+}
+
+// copy N samples of the ring starting at ix into contiguous pinned memory
+static const uint8_t *gather_ring(const sdr_buff_t *buff, int ix, int N, uint8_t *dst)
+{
+    if (ix + N <= buff->N) {
+        memcpy(dst, buff->data + ix, N);
+    } else {
+        int n = buff->N - ix;
+        memcpy(dst, buff->data + ix, n);
+        memcpy(dst + n, buff->data, N - n);
+    }
+    return dst;
+}
+
+}  // namespace psa
+
+using namespace psa;
+
+// ===========================================================================
+// drop-in entry points
+// ===========================================================================
+extern "C" {
+
+int sdr_cpx_fft(sdr_cpx_t *cpx1, int N, int dir, sdr_cpx_t *cpx2)
+{
+    if (dir != SDR_FFT_FORWARD && dir != SDR_FFT_BACKWARD) return 0;
+    const FFTPlan *pl = fft_plan(N);
+    if (!pl) {
+        fprintf(stderr, "pocketsdr_amd: sdr_cpx_fft: unsupported FFT length N=%d\n", N);
+        return 0;
+    }
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    size_t bytes = sizeof(float2) * (size_t)N;
+    float2 *a = (float2 *)tc.s[SC_A].get(bytes);
+    float2 *b = (float2 *)tc.s[SC_B].get(bytes);
+    float2 *c = (float2 *)tc.s[SC_C].get(bytes);
+    void *hp = tc.host_pinned(bytes);
+    memcpy(hp, cpx1, bytes);
+    PSA_CHECK(hipMemcpyAsync(a, hp, bytes, hipMemcpyHostToDevice, st));
+    fft_c2c(pl, a, c, b, 1, dir, st);
+    PSA_CHECK(hipMemcpyAsync(hp, c, bytes, hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(cpx2, hp, bytes);
+    return 1;
+}
+
+void sdr_cpx_mul(const sdr_cpx_t *a, const sdr_cpx_t *b, int N, float s, sdr_cpx_t *c)
+{
+    if (N <= 0) return;
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    size_t bytes = sizeof(float2) * (size_t)N;
+    float2 *da = (float2 *)tc.s[SC_A].get(bytes);
+    float2 *db = (float2 *)tc.s[SC_B].get(bytes);
+    float2 *dc = (float2 *)tc.s[SC_C].get(bytes);
+    uint8_t *hp = (uint8_t *)tc.host_pinned(2 * bytes);
+    memcpy(hp, a, bytes);
+    memcpy(hp + bytes, b, bytes);
+    PSA_CHECK(hipMemcpyAsync(da, hp, bytes, hipMemcpyHostToDevice, st));
+    PSA_CHECK(hipMemcpyAsync(db, hp + bytes, bytes, hipMemcpyHostToDevice, st));
+    launch_cpx_mul(da, db, N, s, dc, st);
+    PSA_CHECK(hipMemcpyAsync(hp, dc, bytes, hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(c, hp, bytes);
+}
+
+void sdr_mix_carr(const sdr_buff_t *buff, int ix, int N, double fs, double fc,
+    double phi, sdr_cpx16_t *IQ)
+{
+    if (N <= 0) return;
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    uint32_t p0, p1, s;
+    int nseg;
+    mix_params(buff->N, ix, N, fs, fc, phi, &p0, &p1, &nseg, &s);
+    uint8_t *hp = (uint8_t *)tc.host_pinned((size_t)N * 3);
+    gather_ring(buff, ix, N, hp);
+    uint8_t *din = (uint8_t *)tc.s[SC_IN].get(N);
+    int8_t *dout = (int8_t *)tc.s[SC_IQ].get((size_t)N * 2);
+    PSA_CHECK(hipMemcpyAsync(din, hp, N, hipMemcpyHostToDevice, st));
+    // contiguous copy: the second segment follows the first directly
+    launch_mix(din, N, 0, N, p0, p1, nseg, s, dout, st);
+    PSA_CHECK(hipMemcpyAsync(hp + N, dout, (size_t)N * 2, hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(IQ, hp + N, (size_t)N * 2);
+}
+
+void sdr_corr_fft(const sdr_cpx16_t *IQ, const sdr_cpx_t *code_fft, int N, sdr_cpx_t *corr)
+{
+    if (N <= 0) return;
+    const FFTPlan *pl = need_plan(N, "sdr_corr_fft");
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    const float2 *cft = spec_cache().get(code_fft, pl, st);
+    size_t bytes = sizeof(float2) * (size_t)N;
+    int8_t *diq = (int8_t *)tc.s[SC_IQ].get((size_t)N * 2);
+    float2 *ta = (float2 *)tc.s[SC_A].get(bytes);
+    float2 *tb = (float2 *)tc.s[SC_B].get(bytes);
+    uint8_t *hp = (uint8_t *)tc.host_pinned(bytes);
+    memcpy(hp, IQ, (size_t)N * 2);
+    PSA_CHECK(hipMemcpyAsync(diq, hp, (size_t)N * 2, hipMemcpyHostToDevice, st));
+
+    PassArgs a = {};
+    args_fwd_col(pl, a);
+    a.iq16 = diq; a.in_bstride = N; a.out = ta; a.out_bstride = N;
+    launch_pass(LD_CPX16, ST_CPX, a, 1, st);
+    PassArgs r = {};
+    args_fwd_row(pl, r);
+    r.in = ta; r.out = tb; r.in_bstride = r.out_bstride = N;
+    launch_pass(LD_CPX, ST_CPX, r, 1, st);
+    PassArgs ir = {};
+    args_inv_row(pl, ir);
+    ir.in = tb; ir.in_bstride = N; ir.cf = cft; ir.cf_bstride = N;
+    ir.prod_xdiv = 1; ir.prod_xmod = 1; ir.prod_cdiv = 1;
+    set_prod(pl, ir, 1.0f / N / N);
+    ir.out = ta; ir.out_bstride = N;
+    launch_pass(LD_PROD, ST_CPX, ir, 1, st);
+    PassArgs ic = {};
+    args_inv_col(pl, ic);
+    ic.in = ta; ic.in_bstride = N; ic.out = tb; ic.out_bstride = N;
+    launch_pass(LD_CPX, ST_CPX, ic, 1, st);
+    PSA_CHECK(hipMemcpyAsync(hp, tb, bytes, hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(corr, hp, bytes);
+}
+
+// src/sdr_func.c:1105-1113: float complex buffer -> cpx8, mix, FFT correlate
+void sdr_corr_fft_cpx(const sdr_cpx_t *buff, int len_buff, int ix, int N,
+    double fs, double fc, double phi, const sdr_cpx_t *code_fft, sdr_cpx_t *corr)
+{
+    std::vector<uint8_t> b8(N);
+    for (int i = 0, j = ix; i < N; i++, j = (j + 1) % len_buff) {
+        int8_t re = (int8_t)buff[j][0], im = (int8_t)buff[j][1];
+        b8[i] = (uint8_t)(((int8_t)im << 4) | (((int8_t)(re << 4) >> 4) & 0xF));
+    }
+    sdr_buff_t tmp;
+    tmp.data = b8.data();
+    tmp.IQ = 2;
+    tmp.N = N;
+    std::vector<sdr_cpx16_t> IQ(N);
+    sdr_mix_carr(&tmp, 0, N, fs, fc, phi, IQ.data());
+    sdr_corr_fft(IQ.data(), code_fft, N, corr);
+}
+
+void sdr_search_code(const sdr_cpx_t *code_fft, double T, const sdr_buff_t *buff, int ix,
+    int N, double fs, double fi, const float *fds, int len_fds, float *P)
+{
+    (void)T;
+    if (N <= 0 || len_fds <= 0) return;
+    const FFTPlan *pl = need_plan(N, "sdr_search_code");
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    const float2 *cft = spec_cache().get(code_fft, pl, st);
+    const size_t nb = (size_t)len_fds;
+    const size_t bytes = sizeof(float2) * (size_t)N * nb;
+    const size_t pbytes = sizeof(float) * (size_t)N * nb;
+    // mixing parameters per bin: phi = 0 (src/sdr_func.c:674)
+    std::vector<int> hix(nb), hnseg(nb);
+    std::vector<uint32_t> hp0(nb), hp1(nb), hs(nb);
+    for (size_t i = 0; i < nb; i++) {
+        mix_params(buff->N, ix, N, fs, fi + fds[i], 0.0, &hp0[i], &hp1[i], &hnseg[i], &hs[i]);
+        hix[i] = 0;
+    }
+    const size_t prm = nb * sizeof(uint32_t);
+    uint8_t *hp = (uint8_t *)tc.host_pinned(std::max<size_t>((size_t)N + 5 * prm + 64, pbytes));
+    gather_ring(buff, ix, N, hp);
+    uint8_t *q = hp + ((N + 63) / 64) * 64;
+    memcpy(q, hix.data(), prm); memcpy(q + prm, hnseg.data(), prm);
+    memcpy(q + 2 * prm, hp0.data(), prm); memcpy(q + 3 * prm, hp1.data(), prm);
+    memcpy(q + 4 * prm, hs.data(), prm);
+    uint8_t *din = (uint8_t *)tc.s[SC_IN].get((size_t)N + 5 * prm + 64);
+    uint8_t *dq = din + ((N + 63) / 64) * 64;
+    PSA_CHECK(hipMemcpyAsync(din, hp, (q - hp) + 5 * prm, hipMemcpyHostToDevice, st));
+    float2 *ta = (float2 *)tc.s[SC_A].get(bytes);
+    float2 *tb = (float2 *)tc.s[SC_B].get(bytes);
+    float *dP = (float *)tc.s[SC_P].get(pbytes);
+    // P is accumulated in place: upload it after the parameter copy has been
+    // staged (the pinned buffer is reused for P)
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(hp, P, pbytes);
+    PSA_CHECK(hipMemcpyAsync(dP, hp, pbytes, hipMemcpyHostToDevice, st));
+
+    PassArgs a = {};
+    args_fwd_col(pl, a);
+    a.buf = din; a.buffN = N; a.mix_compact = 1;
+    a.mix_ix = (const int *)dq; a.mix_nseg = (const int *)(dq + prm);
+    a.mix_p0 = (const uint32_t *)(dq + 2 * prm); a.mix_p1 = (const uint32_t *)(dq + 3 * prm);
+    a.mix_s = (const uint32_t *)(dq + 4 * prm);
+    a.out = ta; a.out_bstride = N;
+    launch_pass(LD_MIX, ST_CPX, a, (int)nb, st);
+    PassArgs r = {};
+    args_fwd_row(pl, r);
+    r.in = ta; r.out = tb; r.in_bstride = r.out_bstride = N;
+    launch_pass(LD_CPX, ST_CPX, r, (int)nb, st);
+    PassArgs ir = {};
+    args_inv_row(pl, ir);
+    ir.in = tb; ir.in_bstride = N; ir.cf = cft; ir.cf_bstride = 0;
+    ir.prod_xdiv = 1; ir.prod_xmod = (int)nb; ir.prod_cdiv = (int)nb;
+    set_prod(pl, ir, 1.0f / N / N);
+    ir.out = ta; ir.out_bstride = N;
+    launch_pass(LD_PROD, ST_CPX, ir, (int)nb, st);
+    PassArgs ic = {};
+    args_inv_col(pl, ic);
+    ic.in = ta; ic.in_bstride = N;
+    ic.P = dP; ic.p_bstride = N; ic.p_div = 1; ic.p_off1 = 0; ic.p_mul = 1; ic.p_off2 = 0;
+    ic.nlag = N;
+    launch_pass(LD_CPX, ST_POW, ic, (int)nb, st);
+    PSA_CHECK(hipMemcpyAsync(hp, dP, pbytes, hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(P, hp, pbytes);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// batched search plan
+// ===========================================================================
+namespace psa {
+
+// host copy of the per-(block, bin) mixing parameters
+struct PlainMix {
+    std::vector<int> ix, nseg;
+    std::vector<uint32_t> p0, p1, s;
+};
+
+struct SigDev {
+    sdr_search_sig_t h;
+    PlainMix hm;
+    const FFTPlan *pl = nullptr;
+    float2 *cft = nullptr;      // n_prn x M, digit-transposed (batched)
+    float2 *cfn = nullptr;      // n_prn x M, natural order (plain)
+    float *P = nullptr;         // n_prn x len_fds x N
+    float *fds = nullptr;       // device copy of the bins
+    int *mix_ix = nullptr, *mix_nseg = nullptr;   // [n_blk x len_fds]
+    uint32_t *mix_p0 = nullptr, *mix_p1 = nullptr, *mix_s = nullptr;
+    float *cn0 = nullptr;
+    int *ix = nullptr;
+    double *dop = nullptr;
+    int bin_chunk = 1, prn_chunk = 1;
+};
+
+enum { K_FWD_COL = 0, K_FWD_ROW, K_INV_ROW, K_INV_COL, K_MIX, K_CVT, K_MUL, K_POW, K_PEAK, K_FILL, K_NCLS };
+static const char *k_names[K_NCLS] = {"fwd_col", "fwd_row", "inv_row", "inv_col", "mix",
+                                      "cvt", "mul", "pow", "peak", "fill"};
+
+}  // namespace psa
+
+struct sdr_search_plan_tag {
+    int flags = 0;
+    std::vector<uint8_t *> d_buf;
+    std::vector<int> buffN;
+    std::vector<psa::SigDev> sig;
+    float2 *tmp = nullptr, *X = nullptr, *Z = nullptr;
+    int8_t *iq16 = nullptr;     // plain path mixed samples
+    size_t tmp_elems = 0, x_elems = 0, z_elems = 0;
+    hipStream_t st = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool timing = false;
+    struct Ev { int cls; hipEvent_t a, b; double elems; };
+    std::vector<Ev> evs;
+    size_t ev_used = 0;
+    double ms[psa::K_NCLS] = {0};
+    long long cnt[psa::K_NCLS] = {0};
+    double elems[psa::K_NCLS] = {0};
+};
+
+namespace psa {
+
+// records a timed launch when timing is on
+struct Tick {
+    sdr_search_plan_t *p;
+    int cls;
+    double elems;
+    size_t slot;
+    Tick(sdr_search_plan_t *p_, int c, double e) : p(p_), cls(c), elems(e), slot(0)
+    {
+        if (!p->timing) return;
+        if (p->ev_used == p->evs.size()) {
+            sdr_search_plan_t::Ev ev;
+            PSA_CHECK(hipEventCreate(&ev.a));
+            PSA_CHECK(hipEventCreate(&ev.b));
+            p->evs.push_back(ev);
+        }
+        slot = p->ev_used++;
+        p->evs[slot].cls = cls;
+        p->evs[slot].elems = elems;
+        PSA_CHECK(hipEventRecord(p->evs[slot].a, p->st));
+    }
+    ~Tick()
+    {
+        if (p->timing) PSA_CHECK(hipEventRecord(p->evs[slot].b, p->st));
+    }
+};
+
+// the batched, shared-forward-FFT schedule of one signal group
+static void run_batched(sdr_search_plan_t *p, SigDev &s)
+{
+    hipStream_t st = p->st;
+    const FFTPlan *pl = s.pl;
+    const int M = pl->M, N = s.h.N, nbin = s.h.len_fds, nprn = s.h.n_prn;
+    const uint8_t *buf = p->d_buf[s.h.buff];
+    const int buffN = p->buffN[s.h.buff];
+    {
+        Tick t(p, K_FILL, (double)nprn * nbin * N);
+        fill(s.P, (long long)nprn * nbin * N, 0.0f, st);
+    }
+    for (int k = 0; k < s.h.n_blk; k++) {
+        for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
+            const int nbc = std::min(s.bin_chunk, nbin - b0);
+            const long long mo = (long long)k * nbin + b0;
+            PassArgs a = {};
+            args_fwd_col(pl, a);
+            a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
+            a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
+            a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
+            a.out = p->tmp; a.out_bstride = M;
+            {
+                Tick t(p, K_FWD_COL, (double)nbc * M);
+                launch_pass(LD_MIX, ST_CPX, a, nbc, st);
+            }
+            PassArgs r = {};
+            args_fwd_row(pl, r);
+            r.in = p->tmp; r.out = p->X; r.in_bstride = r.out_bstride = M;
+            {
+                Tick t(p, K_FWD_ROW, (double)nbc * M);
+                launch_pass(LD_CPX, ST_CPX, r, nbc, st);
+            }
+            for (int q0 = 0; q0 < nprn; q0 += s.prn_chunk) {
+                const int nqc = std::min(s.prn_chunk, nprn - q0);
+                const int nbatch = nqc * nbc;      // b = prn_local * nbc + bin_local
+                PassArgs ir = {};
+                args_inv_row(pl, ir);
+                ir.in = p->X; ir.in_bstride = M;
+                ir.cf = s.cft + (long long)q0 * M; ir.cf_bstride = M;
+                ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
+                set_prod(pl, ir, 1.0f / M / M);
+                ir.out = p->Z; ir.out_bstride = M;
+                {
+                    Tick t(p, K_INV_ROW, (double)nbatch * M);
+                    launch_pass(LD_PROD, ST_CPX, ir, nbatch, st);
+                }
+                PassArgs ic = {};
+                args_inv_col(pl, ic);
+                ic.in = p->Z; ic.in_bstride = M;
+                ic.P = s.P; ic.p_bstride = N;
+                ic.p_div = nbc; ic.p_off1 = q0; ic.p_mul = nbin; ic.p_off2 = b0;
+                ic.nlag = N;
+                {
+                    Tick t(p, K_INV_COL, (double)nbatch * M);
+                    launch_pass(LD_CPX, ST_POW, ic, nbatch, st);
+                }
+            }
+        }
+    }
+}
+
+// the reference dataflow, one sdr_search_code-equivalent per (PRN, block):
+// mix -> cpx16, forward FFT, sdr_cpx_mul, inverse FFT, power accumulate,
+// each its own kernel, nothing shared across PRNs
+static void run_plain(sdr_search_plan_t *p, SigDev &s)
+{
+    hipStream_t st = p->st;
+    const FFTPlan *pl = s.pl;
+    const int M = pl->M, N = s.h.N, nbin = s.h.len_fds, nprn = s.h.n_prn;
+    const uint8_t *buf = p->d_buf[s.h.buff];
+    const int buffN = p->buffN[s.h.buff];
+    {
+        Tick t(p, K_FILL, (double)nprn * nbin * N);
+        fill(s.P, (long long)nprn * nbin * N, 0.0f, st);
+    }
+    for (int q = 0; q < nprn; q++) {
+        for (int k = 0; k < s.h.n_blk; k++) {
+            for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
+                const int nbc = std::min(s.bin_chunk, nbin - b0);
+                {
+                    Tick t(p, K_MIX, (double)nbc * M);
+                    for (int b = 0; b < nbc; b++) {
+                        size_t i = (size_t)k * nbin + b0 + b;
+                        launch_mix(buf, buffN, s.hm.ix[i], M, s.hm.p0[i], s.hm.p1[i], s.hm.nseg[i],
+                                   s.hm.s[i], p->iq16 + (size_t)b * M * 2, st);
+                    }
+                }
+                PassArgs a = {};
+                args_fwd_col(pl, a);
+                a.iq16 = p->iq16; a.in_bstride = M; a.out = p->tmp; a.out_bstride = M;
+                {
+                    Tick t(p, K_FWD_COL, (double)nbc * M);
+                    launch_pass(LD_CPX16, ST_CPX, a, nbc, st);
+                }
+                PassArgs r = {};
+                args_fwd_row(pl, r);
+                r.in = p->tmp; r.out = p->X; r.in_bstride = r.out_bstride = M;
+                r.out_nstride = pl->M1; r.out_gstride = 1;    // natural order
+                {
+                    Tick t(p, K_FWD_ROW, (double)nbc * M);
+                    launch_pass(LD_CPX, ST_CPX, r, nbc, st);
+                }
+                {
+                    Tick t(p, K_MUL, (double)nbc * M);
+                    for (int b = 0; b < nbc; b++) {
+                        launch_cpx_mul(p->X + (size_t)b * M, s.cfn + (size_t)q * M, M,
+                                       1.0f / M / M, p->Z + (size_t)b * M, st);
+                    }
+                }
+                {
+                    Tick t(p, K_INV_ROW, (double)nbc * M);
+                    fft_c2c(pl, p->Z, p->X, p->tmp, nbc, 1, st);
+                }
+                {
+                    Tick t(p, K_POW, (double)nbc * N);
+                    long long total = (long long)nbc * N;
+                    hipLaunchKernelGGL(power_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                                       p->X, M, N, (long long)nbc,
+                                       s.P + ((long long)q * nbin + b0) * N, (long long)N);
+                    PSA_CHECK(hipGetLastError());
+                }
+            }
+        }
+    }
+}
+
+static void run_all(sdr_search_plan_t *p)
+{
+    for (auto &s : p->sig) {
+        if (p->flags & SDR_SEARCH_PLAIN) run_plain(p, s);
+        else run_batched(p, s);
+        Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N);
+        hipLaunchKernelGGL(peak_kernel, dim3(s.h.n_prn), dim3(PEAK_THREADS), 0, p->st, s.P, s.h.N,
+                           s.h.len_fds, s.h.T, s.fds, s.cn0, s.ix, s.dop);
+        PSA_CHECK(hipGetLastError());
+    }
+}
+
+}  // namespace psa
+
+extern "C" {
+
+sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buffs,
+    int n_sig, const sdr_search_sig_t *sigs, int flags)
+{
+    if (n_buff <= 0 || !buffs || n_sig <= 0 || !sigs) {
+        fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: invalid arguments\n");
+        return nullptr;
+    }
+    for (int i = 0; i < n_sig; i++) {
+        const sdr_search_sig_t &g = sigs[i];
+        if (g.buff < 0 || g.buff >= n_buff || g.M <= 0 || g.N <= 0 || g.N > g.M || g.n_prn <= 0 ||
+            !g.code_fft || g.len_fds <= 0 || !g.fds || g.n_blk <= 0 || !fft_supported(g.M) ||
+            g.M > buffs[g.buff]->N) {
+            fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: invalid signal group %d\n", i);
+            return nullptr;
+        }
+        for (int k = 0; k < g.n_blk; k++) {
+            long long ix = (long long)g.ix0 + (long long)k * g.blk_step;
+            if (ix < 0 || ix >= buffs[g.buff]->N) {
+                fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: block %d of group %d outside the buffer\n", k, i);
+                return nullptr;
+            }
+        }
+    }
+    ensure_init();
+    sdr_search_plan_t *p = new sdr_search_plan_t();
+    p->flags = flags;
+    PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+    for (int i = 0; i < n_buff; i++) {
+        uint8_t *d;
+        PSA_CHECK(hipMalloc(&d, buffs[i]->N));
+        PSA_CHECK(hipMemcpy(d, buffs[i]->data, buffs[i]->N, hipMemcpyHostToDevice));
+        p->d_buf.push_back(d);
+        p->buffN.push_back(buffs[i]->N);
+    }
+    const size_t budget = (size_t)1 << 30;   // bytes per intermediate array
+    p->sig.resize(n_sig);
+    for (int i = 0; i < n_sig; i++) {
+        SigDev &s = p->sig[i];
+        s.h = sigs[i];
+        s.h.code_fft = nullptr;
+        s.h.fds = nullptr;
+        s.pl = fft_plan(sigs[i].M);
+        const int M = s.h.M, nprn = s.h.n_prn, nbin = s.h.len_fds, nblk = s.h.n_blk;
+        const size_t spec = sizeof(float2) * (size_t)M;
+        // code spectra: natural (plain) and digit-transposed (batched)
+        float2 *nat;
+        PSA_CHECK(hipMalloc(&nat, spec * nprn));
+        PSA_CHECK(hipMemcpy(nat, sigs[i].code_fft, spec * nprn, hipMemcpyHostToDevice));
+        if (flags & SDR_SEARCH_PLAIN) {
+            s.cfn = nat;
+        } else {
+            PSA_CHECK(hipMalloc(&s.cft, spec * nprn));
+            fft_to_transposed(s.pl, nat, s.cft, nprn, p->st);
+            PSA_CHECK(hipStreamSynchronize(p->st));
+            PSA_CHECK(hipFree(nat));
+        }
+        PSA_CHECK(hipMalloc(&s.P, sizeof(float) * (size_t)nprn * nbin * s.h.N));
+        PSA_CHECK(hipMalloc(&s.fds, sizeof(float) * nbin));
+        PSA_CHECK(hipMemcpy(s.fds, sigs[i].fds, sizeof(float) * nbin, hipMemcpyHostToDevice));
+        // mixing parameters per (block, bin), phi = 0 (src/sdr_func.c:674)
+        PlainMix &pm = s.hm;
+        const size_t nm = (size_t)nblk * nbin;
+        pm.ix.resize(nm); pm.nseg.resize(nm); pm.p0.resize(nm); pm.p1.resize(nm); pm.s.resize(nm);
+        for (int k = 0; k < nblk; k++) {
+            int ix = s.h.ix0 + k * s.h.blk_step;
+            for (int b = 0; b < nbin; b++) {
+                size_t j = (size_t)k * nbin + b;
+                pm.ix[j] = ix;
+                mix_params(p->buffN[s.h.buff], ix, M, s.h.fs, s.h.fi + sigs[i].fds[b], 0.0,
+                           &pm.p0[j], &pm.p1[j], &pm.nseg[j], &pm.s[j]);
+            }
+        }
+        PSA_CHECK(hipMalloc(&s.mix_ix, sizeof(int) * nm));
+        PSA_CHECK(hipMalloc(&s.mix_nseg, sizeof(int) * nm));
+        PSA_CHECK(hipMalloc(&s.mix_p0, sizeof(uint32_t) * nm));
+        PSA_CHECK(hipMalloc(&s.mix_p1, sizeof(uint32_t) * nm));
+        PSA_CHECK(hipMalloc(&s.mix_s, sizeof(uint32_t) * nm));
+        PSA_CHECK(hipMemcpy(s.mix_ix, pm.ix.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+        PSA_CHECK(hipMemcpy(s.mix_nseg, pm.nseg.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+        PSA_CHECK(hipMemcpy(s.mix_p0, pm.p0.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
+        PSA_CHECK(hipMemcpy(s.mix_p1, pm.p1.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
+        PSA_CHECK(hipMemcpy(s.mix_s, pm.s.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
+        PSA_CHECK(hipMalloc(&s.cn0, sizeof(float) * nprn));
+        PSA_CHECK(hipMalloc(&s.ix, sizeof(int) * 2 * nprn));
+        PSA_CHECK(hipMalloc(&s.dop, sizeof(double) * nprn));
+        // chunking of the intermediates
+        s.bin_chunk = (int)std::max<size_t>(1, std::min<size_t>(nbin, budget / spec));
+        size_t pairs = std::max<size_t>(1, budget / spec);
+        s.prn_chunk = (int)std::max<size_t>(1, std::min<size_t>(nprn, pairs / s.bin_chunk));
+        if (flags & SDR_SEARCH_PLAIN) s.prn_chunk = 1;
+        p->tmp_elems = std::max(p->tmp_elems, (size_t)s.bin_chunk * M);
+        p->x_elems = std::max(p->x_elems, (size_t)s.bin_chunk * M);
+        p->z_elems = std::max(p->z_elems, (size_t)s.bin_chunk * s.prn_chunk * M);
+    }
+    PSA_CHECK(hipMalloc(&p->tmp, sizeof(float2) * p->tmp_elems));
+    PSA_CHECK(hipMalloc(&p->X, sizeof(float2) * p->x_elems));
+    PSA_CHECK(hipMalloc(&p->Z, sizeof(float2) * p->z_elems));
+    if (flags & SDR_SEARCH_PLAIN) PSA_CHECK(hipMalloc(&p->iq16, 2 * p->tmp_elems));
+    return p;
+}
+
+int sdr_search_plan_run(sdr_search_plan_t *p)
+{
+    if (!p) return 0;
+    const bool graph = (p->flags & SDR_SEARCH_GRAPH) && !p->timing;
+    if (graph) {
+        if (!p->gexec) {
+            hipGraph_t g;
+            PSA_CHECK(hipStreamBeginCapture(p->st, hipStreamCaptureModeThreadLocal));
+            run_all(p);
+            PSA_CHECK(hipStreamEndCapture(p->st, &g));
+            PSA_CHECK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+            PSA_CHECK(hipGraphDestroy(g));
+        }
+        PSA_CHECK(hipGraphLaunch(p->gexec, p->st));
+        PSA_CHECK(hipStreamSynchronize(p->st));
+        return 1;
+    }
+    p->ev_used = 0;
+    run_all(p);
+    PSA_CHECK(hipStreamSynchronize(p->st));
+    if (p->timing) {
+        for (size_t i = 0; i < p->ev_used; i++) {
+            float ms = 0;
+            PSA_CHECK(hipEventElapsedTime(&ms, p->evs[i].a, p->evs[i].b));
+            p->ms[p->evs[i].cls] += ms;
+            p->cnt[p->evs[i].cls]++;
+            p->elems[p->evs[i].cls] += p->evs[i].elems;
+        }
+    }
+    return 1;
+}
+
+int sdr_search_plan_result(const sdr_search_plan_t *p, int sig, float *cn0, int *ix, double *dop)
+{
+    if (!p || sig < 0 || sig >= (int)p->sig.size()) return 0;
+    const SigDev &s = p->sig[sig];
+    const int n = s.h.n_prn;
+    if (cn0) PSA_CHECK(hipMemcpy(cn0, s.cn0, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (ix) PSA_CHECK(hipMemcpy(ix, s.ix, sizeof(int) * 2 * n, hipMemcpyDeviceToHost));
+    if (dop) PSA_CHECK(hipMemcpy(dop, s.dop, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return 1;
+}
+
+int sdr_search_plan_power(const sdr_search_plan_t *p, int sig, int prn_idx, float *P)
+{
+    if (!p || sig < 0 || sig >= (int)p->sig.size()) return 0;
+    const SigDev &s = p->sig[sig];
+    if (prn_idx < 0 || prn_idx >= s.h.n_prn) return 0;
+    size_t n = (size_t)s.h.len_fds * s.h.N;
+    PSA_CHECK(hipMemcpy(P, s.P + (size_t)prn_idx * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return 1;
+}
+
+void sdr_search_plan_timing(sdr_search_plan_t *p, int on)
+{
+    if (!p) return;
+    p->timing = on != 0;
+    for (int i = 0; i < K_NCLS; i++) { p->ms[i] = 0; p->cnt[i] = 0; p->elems[i] = 0; }
+}
+
+int sdr_search_plan_kernel_ms(const sdr_search_plan_t *p, const char *name, double *avg_ms,
+                              long long *count, double *elems)
+{
+    if (!p || !name) return 0;
+    for (int i = 0; i < K_NCLS; i++) {
+        if (strcmp(name, k_names[i]) == 0) {
+            if (!p->cnt[i]) return 0;
+            if (avg_ms) *avg_ms = p->ms[i] / p->cnt[i];
+            if (count) *count = p->cnt[i];
+            if (elems) *elems = p->elems[i] / p->cnt[i];
+            return 1;
+        }
+    }
+    return 0;
+}
+
+void sdr_search_plan_free(sdr_search_plan_t *p)
+{
+    if (!p) return;
+    (void)hipStreamSynchronize(p->st);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    for (auto d : p->d_buf) (void)hipFree(d);
+    for (auto &s : p->sig) {
+        (void)hipFree(s.cft); (void)hipFree(s.cfn); (void)hipFree(s.P); (void)hipFree(s.fds);
+        (void)hipFree(s.mix_ix); (void)hipFree(s.mix_nseg); (void)hipFree(s.mix_p0);
+        (void)hipFree(s.mix_p1); (void)hipFree(s.mix_s);
+        (void)hipFree(s.cn0); (void)hipFree(s.ix); (void)hipFree(s.dop);
+    }
+    (void)hipFree(p->tmp); (void)hipFree(p->X); (void)hipFree(p->Z); (void)hipFree(p->iq16);
+    for (auto &e : p->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    (void)hipStreamDestroy(p->st);
+    delete p;
+}
+
+}  // extern "C"
