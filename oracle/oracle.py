@@ -1,0 +1,212 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Loaded by tests/,
+by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py, never by
+the product package.  When oracle/_ref/libpocketfft_ref.so (the reference's
+PocketFFT wrapper, lib/pocketfft/pocketfft_wrap.cpp, built from the reference
+sources by ``make -C oracle ref``) is present, the FP32 FFTs go through it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_FFT = os.path.join(HERE, "_ref", "libpocketfft_ref.so")
+
+if not os.path.exists(LIB):
+    raise ImportError("oracle/liboracle.so not built (make -C oracle)")
+lib = C.CDLL(LIB)
+
+vp, fp, dp, ip = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+
+def _d(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+class orc_trk_ch_t(C.Structure):
+    _fields_ = [("fs", C.c_double), ("fi", C.c_double), ("fc_sig", C.c_double),
+                ("T", C.c_double), ("N", C.c_int), ("fd0", C.c_double), ("drift", C.c_double),
+                ("phi0", C.c_double), ("coff0", C.c_double)]
+
+
+_d("orc_func_init", None, [])
+_d("orc_mix_tbl", C.POINTER(C.c_int8), [])
+_d("orc_load_ref_fft", C.c_int, [C.c_char_p])
+_d("orc_ref_fft_loaded", C.c_int, [])
+_d("orc_mix_carr", None, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, vp])
+_d("orc_corr_std", None, [vp, vp, C.c_int, C.c_double, dp, C.c_int, vp, vp])
+_d("orc_corr_std_cpx_code", None, [vp, vp, C.c_int, C.c_double, dp, C.c_int, vp, vp])
+_d("orc_cpx_mul", None, [vp, vp, C.c_int, C.c_float, vp])
+_d("orc_fft_f32", C.c_int, [vp, C.c_int, C.c_int, vp])
+_d("orc_fft_f64", None, [vp, C.c_int, C.c_int, vp])
+_d("orc_corr_fft", None, [vp, vp, C.c_int, vp])
+_d("orc_search_code", None, [vp, C.c_double, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                             C.c_double, fp, C.c_int, fp])
+_d("orc_search_code_f64", None, [vp, C.c_double, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                 C.c_double, fp, C.c_int, dp])
+_d("orc_corr_max", C.c_float, [fp, C.c_int, C.c_int, C.c_int, C.c_double, ip])
+_d("orc_fine_dop", C.c_double, [fp, C.c_int, fp, C.c_int, ip])
+_d("orc_dop_bins", C.c_int, [C.c_double, C.c_float, C.c_float, fp])
+_d("orc_res_code", None, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, vp])
+_d("orc_gen_code_fft", None, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, vp])
+_d("orc_gen_code_fft_f64", None, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_int, vp])
+_d("orc_gen_code_L1CA", C.c_int, [C.c_int, vp])
+_d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, C.c_int, vp, vp, dp])
+
+lib.orc_func_init()
+REF_LOADED = bool(os.path.exists(REF_FFT) and lib.orc_load_ref_fft(REF_FFT.encode()))
+
+
+def _p(a):
+    return a.ctypes.data_as(vp)
+
+
+def mix_tbl():
+    return np.ctypeslib.as_array(lib.orc_mix_tbl(), shape=(65536 * 2,)).reshape(65536, 2).copy()
+
+
+def mix_carr(buff, ix, N, fs, fc, phi):
+    buff = np.ascontiguousarray(buff, np.uint8)
+    IQ = np.zeros((N, 2), np.int8)
+    lib.orc_mix_carr(_p(buff), len(buff), ix, N, fs, fc, phi, _p(IQ))
+    return IQ
+
+
+def corr_std(IQ, code, N, coff, pos, cpx_code=False):
+    IQ = np.ascontiguousarray(IQ, np.int8)
+    code = np.ascontiguousarray(code, np.int8)
+    pos = np.ascontiguousarray(pos, np.float64)
+    corr = np.zeros(len(pos), np.complex64)
+    Cc = np.zeros(2, np.complex64)
+    f = lib.orc_corr_std_cpx_code if cpx_code else lib.orc_corr_std
+    f(_p(IQ), _p(code), N, coff, pos.ctypes.data_as(dp), len(pos), _p(corr), _p(Cc))
+    return corr, Cc
+
+
+def cpx_mul(a, b, s):
+    a = np.ascontiguousarray(a, np.complex64)
+    b = np.ascontiguousarray(b, np.complex64)
+    c = np.zeros_like(a)
+    lib.orc_cpx_mul(_p(a), _p(b), len(a), s, _p(c))
+    return c
+
+
+def fft_f32(x, dir=0):
+    x = np.ascontiguousarray(x, np.complex64)
+    y = np.zeros_like(x)
+    lib.orc_fft_f32(_p(x), len(x), dir, _p(y))
+    return y
+
+
+def fft_f64(x, dir=0):
+    x = np.ascontiguousarray(x, np.complex128)
+    y = np.zeros_like(x)
+    lib.orc_fft_f64(_p(x), len(x), dir, _p(y))
+    return y
+
+
+def corr_fft(IQ, code_fft):
+    IQ = np.ascontiguousarray(IQ, np.int8)
+    code_fft = np.ascontiguousarray(code_fft, np.complex64)
+    out = np.zeros(len(code_fft), np.complex64)
+    lib.orc_corr_fft(_p(IQ), _p(code_fft), len(code_fft), _p(out))
+    return out
+
+
+def search_code(code_fft, T, buff, ix, fs, fi, fds, P=None):
+    code_fft = np.ascontiguousarray(code_fft, np.complex64)
+    buff = np.ascontiguousarray(buff, np.uint8)
+    fds = np.ascontiguousarray(fds, np.float32)
+    N = len(code_fft)
+    if P is None:
+        P = np.zeros((len(fds), N), np.float32)
+    lib.orc_search_code(_p(code_fft), T, _p(buff), len(buff), ix, N, fs, fi,
+                        fds.ctypes.data_as(fp), len(fds), P.ctypes.data_as(fp))
+    return P
+
+
+def search_code_f64(code_fft, T, buff, ix, fs, fi, fds, P=None):
+    code_fft = np.ascontiguousarray(code_fft, np.complex128)
+    buff = np.ascontiguousarray(buff, np.uint8)
+    fds = np.ascontiguousarray(fds, np.float32)
+    N = len(code_fft)
+    if P is None:
+        P = np.zeros((len(fds), N), np.float64)
+    lib.orc_search_code_f64(_p(code_fft), T, _p(buff), len(buff), ix, N, fs, fi,
+                            fds.ctypes.data_as(fp), len(fds), P.ctypes.data_as(dp))
+    return P
+
+
+def corr_max(P, N, Nmax, M, T, ix=None):
+    P = np.ascontiguousarray(P, np.float32)
+    ix = np.zeros(2, np.int32) if ix is None else np.ascontiguousarray(ix, np.int32)
+    cn0 = lib.orc_corr_max(P.ctypes.data_as(fp), N, Nmax, M, T, ix.ctypes.data_as(ip))
+    return cn0, ix
+
+
+def fine_dop(P, N, fds, ix):
+    P = np.ascontiguousarray(P, np.float32)
+    fds = np.ascontiguousarray(fds, np.float32)
+    ix = np.ascontiguousarray(ix, np.int32)
+    return lib.orc_fine_dop(P.ctypes.data_as(fp), N, fds.ctypes.data_as(fp), len(fds),
+                            ix.ctypes.data_as(ip))
+
+
+def dop_bins(T, dop, max_dop):
+    n = lib.orc_dop_bins(T, dop, max_dop, None)
+    out = np.zeros(n, np.float32)
+    lib.orc_dop_bins(T, dop, max_dop, out.ctypes.data_as(fp))
+    return out
+
+
+def res_code(code, T, coff, fs, N, Nz=0):
+    code = np.ascontiguousarray(code, np.int8)
+    out = np.zeros((N + Nz, 2), np.int8)
+    lib.orc_res_code(_p(code), len(code), T, coff, fs, N, Nz, _p(out))
+    return out
+
+
+def gen_code_fft(code, T, coff, fs, N, Nz):
+    code = np.ascontiguousarray(code, np.int8)
+    out = np.zeros(N + Nz, np.complex64)
+    lib.orc_gen_code_fft(_p(code), len(code), T, coff, fs, N, Nz, _p(out))
+    return out
+
+
+def gen_code_fft_f64(code, T, coff, fs, N, Nz):
+    code = np.ascontiguousarray(code, np.int8)
+    out = np.zeros(N + Nz, np.complex128)
+    lib.orc_gen_code_fft_f64(_p(code), len(code), T, coff, fs, N, Nz, _p(out))
+    return out
+
+
+def gen_code_L1CA(prn):
+    out = np.zeros(1023, np.int8)
+    if not lib.orc_gen_code_L1CA(prn, _p(out)):
+        raise ValueError("bad PRN")
+    return out
+
+
+def code_bank(code, T, fs, N):
+    return np.concatenate([res_code(code, T, -k / fs / 10, fs, N) for k in range(10)])
+
+
+def trk_run(ch, code_bank_arr, buff, n_epoch, pos):
+    """ch: dict fs, fi, fc_sig, T, N, fd0, drift, phi0, coff0"""
+    c = orc_trk_ch_t(ch["fs"], ch["fi"], ch["fc_sig"], ch["T"], ch["N"], ch["fd0"],
+                     ch["drift"], ch["phi0"], ch["coff0"])
+    code_bank_arr = np.ascontiguousarray(code_bank_arr, np.int8)
+    buff = np.ascontiguousarray(buff, np.uint8)
+    pos = np.ascontiguousarray(pos, np.float64)
+    corr = np.zeros((n_epoch, len(pos)), np.complex64)
+    Cc = np.zeros((n_epoch, 2), np.complex64)
+    state = np.zeros((n_epoch, 3), np.float64)
+    lib.orc_trk_run(C.byref(c), _p(code_bank_arr), _p(buff), len(buff), n_epoch,
+                    pos.ctypes.data_as(dp), len(pos), _p(corr), _p(Cc), state.ctypes.data_as(dp))
+    return corr, Cc, state

@@ -213,7 +213,9 @@ static void corr_std_impl(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N,
     size_t iq_bytes = sizeof(sdr_cpx16_t) * (size_t)N;
     char *din = (char *)tc.s[SC_IQ].get(iq_bytes + 256 + sizeof(int2) * n);
     int2 *djk = (int2 *)(din + ((iq_bytes + 255) / 256) * 256);
-    uint8_t *hpin = (uint8_t *)tc.host_pinned(iq_bytes + sizeof(int2) * n);
+    // one pinned staging buffer for the whole call (inputs, then outputs)
+    uint8_t *hpin = (uint8_t *)tc.host_pinned(std::max(iq_bytes + sizeof(int2) * n,
+                                                       sizeof(float2) * (n + 2)));
     memcpy(hpin, IQ, iq_bytes);
     memcpy(hpin + iq_bytes, jk.data(), sizeof(int2) * n);
     PSA_CHECK(hipMemcpyAsync(din, hpin, iq_bytes, hipMemcpyHostToDevice, st));
@@ -228,7 +230,8 @@ static void corr_std_impl(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N,
     PSA_CHECK(hipGetLastError());
     hipLaunchKernelGGL(corr_final_kernel, dim3(1), dim3(128), 0, st, c12, n, N, dcorr, dC);
     PSA_CHECK(hipGetLastError());
-    float *hout = (float *)tc.host_pinned(sizeof(float2) * (n + 2));
+    PSA_CHECK(hipStreamSynchronize(st));
+    float *hout = (float *)hpin;
     PSA_CHECK(hipMemcpyAsync(hout, dcorr, sizeof(float2) * (n + 2), hipMemcpyDeviceToHost, st));
     PSA_CHECK(hipStreamSynchronize(st));
     memcpy(corr, hout, sizeof(float2) * n);
@@ -556,6 +559,10 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
         return nullptr;
     }
     const int N = chs[0].N;
+    if (N <= 0 || N > buff->N) {
+        fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new: need 0 < N <= buffer length\n");
+        return nullptr;
+    }
     for (int c = 0; c < n_ch; c++) {
         if (chs[c].N != N || !chs[c].code) {
             fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new: channels need one N and a code bank\n");

@@ -347,6 +347,7 @@ const FFTPlan *fft_plan(int M)
 {
     if (!fft_supported(M)) return nullptr;
     ensure_init();
+    fft_init_attrs();
     std::lock_guard<std::mutex> lk(g_plan_mtx);
     auto it = g_plans.find(M);
     if (it != g_plans.end()) return it->second;
@@ -375,13 +376,28 @@ static void launch_t(const PassArgs &a, const RadixSeq &rs, int nbatch, hipStrea
     if (nblk <= 0) return;
     if (nblk > 0x7FFFFFFFLL) PSA_FAIL("FFT launch too large (%lld workgroups)", nblk);
     size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
-    static std::once_flag attr_once;
-    std::call_once(attr_once, [] {
-        PSA_CHECK(hipFuncSetAttribute((const void *)pass_kernel<LM, SM>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
-    });
     hipLaunchKernelGGL((pass_kernel<LM, SM>), dim3((unsigned)nblk), dim3(256), shm, st, a, rs, ngroups);
     PSA_CHECK(hipGetLastError());
+}
+
+template <int LM, int SM>
+static void set_attr()
+{
+    PSA_CHECK(hipFuncSetAttribute((const void *)pass_kernel<LM, SM>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+}
+
+void fft_init_attrs()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        set_attr<LD_CPX, ST_CPX>();
+        set_attr<LD_CPX16, ST_CPX>();
+        set_attr<LD_MIX, ST_CPX>();
+        set_attr<LD_PROD, ST_CPX>();
+        set_attr<LD_CPX, ST_POW>();
+        set_attr<LD_PROD, ST_POW>();
+    });
 }
 
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hipStream_t st)

@@ -92,6 +92,7 @@ struct FFTPlan {
 };
 bool fft_supported(int M);
 const FFTPlan *fft_plan(int M);   // cached, thread-safe; nullptr if unsupported
+void fft_init_attrs();            // LDS limits of the pass kernels (once)
 
 // load / store modes of the sub-transform passes
 enum LoadMode {
@@ -174,8 +175,11 @@ void fft_to_transposed(const FFTPlan *pl, const float2 *in, float2 *out,
 // ---------------------------------------------------------------------------
 // misc kernels (psa_mix.hip)
 // ---------------------------------------------------------------------------
+// compact = 1: buf holds the N samples contiguously (second ring segment
+// directly after the first); compact = 0: buf is the whole ring
 void launch_mix(const uint8_t *buf, int buffN, int ix, int N, uint32_t p0,
-                uint32_t p1, int nseg, uint32_t s, int8_t *IQ, hipStream_t st);
+                uint32_t p1, int nseg, uint32_t s, int8_t *IQ, hipStream_t st,
+                int compact = 0);
 void mix_params(int buffN, int ix, int N, double fs, double fc, double phi,
                 uint32_t *p0, uint32_t *p1, int *nseg, uint32_t *s);
 void launch_cpx_mul(const float2 *a, const float2 *b, int N, float s,

@@ -41,7 +41,7 @@ __device__ __forceinline__ uint32_t mix_one(uint32_t d, uint32_t c)
 // Each thread mixes 4 consecutive samples and stores them as one 8-byte word.
 __global__ void __launch_bounds__(256) mix_kernel(const uint8_t *__restrict__ buf, int buffN, int ix,
                                                   int N, uint32_t p0, uint32_t p1, int nseg, uint32_t s,
-                                                  const uint32_t *__restrict__ carr_g,
+                                                  int compact, const uint32_t *__restrict__ carr_g,
                                                   int8_t *__restrict__ IQ)
 {
     __shared__ uint32_t carr[256];
@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) mix_kernel(const uint8_t *__restrict__ bu
             int src;
             uint32_t ph;
             if (i < nseg) { src = ix + i; ph = p0 + (uint32_t)i * s; }
-            else { src = i - nseg; ph = p1 + (uint32_t)(i - nseg) * s; }
+            else { src = compact ? i : i - nseg; ph = p1 + (uint32_t)(i - nseg) * s; }
             (void)buffN;
             out[q] = mix_one(buf[src], carr[ph >> 24]);
         }
@@ -75,12 +75,12 @@ __global__ void __launch_bounds__(256) mix_kernel(const uint8_t *__restrict__ bu
 }
 
 void launch_mix(const uint8_t *buf, int buffN, int ix, int N, uint32_t p0,
-                uint32_t p1, int nseg, uint32_t s, int8_t *IQ, hipStream_t st)
+                uint32_t p1, int nseg, uint32_t s, int8_t *IQ, hipStream_t st, int compact)
 {
     if (N <= 0) return;
     int nthr = (N + 3) / 4;
     hipLaunchKernelGGL(mix_kernel, dim3((nthr + 255) / 256), dim3(256), 0, st,
-                       buf, buffN, ix, N, p0, p1, nseg, s, dev_carrier(), IQ);
+                       buf, buffN, ix, N, p0, p1, nseg, s, compact, dev_carrier(), IQ);
     PSA_CHECK(hipGetLastError());
 }
 

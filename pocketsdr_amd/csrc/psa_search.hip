@@ -277,7 +277,7 @@ void sdr_mix_carr(const sdr_buff_t *buff, int ix, int N, double fs, double fc,
     int8_t *dout = (int8_t *)tc.s[SC_IQ].get((size_t)N * 2);
     PSA_CHECK(hipMemcpyAsync(din, hp, N, hipMemcpyHostToDevice, st));
     // contiguous copy: the second segment follows the first directly
-    launch_mix(din, N, 0, N, p0, p1, nseg, s, dout, st);
+    launch_mix(din, N, 0, N, p0, p1, nseg, s, dout, st, 1);
     PSA_CHECK(hipMemcpyAsync(hp + N, dout, (size_t)N * 2, hipMemcpyDeviceToHost, st));
     PSA_CHECK(hipStreamSynchronize(st));
     memcpy(IQ, hp + N, (size_t)N * 2);

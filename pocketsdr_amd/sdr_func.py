@@ -1,0 +1,328 @@
+"""numpy front end of libpocketsdr_amd (the search / correlate hot path).
+
+Function names and argument meaning follow the reference C API
+(src/pocket_sdr.h:391-465); arrays follow its layouts:
+
+* IF buffer ``buff``: uint8 array of cpx8 samples (I low nibble, Q high nibble)
+* mixed samples (cpx16): int8 array of shape (N, 2)
+* complex arrays (sdr_cpx_t): complex64 arrays
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import lib
+
+c_float_p = C.POINTER(C.c_float)
+c_double_p = C.POINTER(C.c_double)
+c_int_p = C.POINTER(C.c_int)
+c_void_p = C.c_void_p
+
+
+class sdr_buff_t(C.Structure):
+    """src/pocket_sdr.h:242-245"""
+    _fields_ = [("data", C.POINTER(C.c_uint8)), ("IQ", C.c_int), ("N", C.c_int)]
+
+
+class sdr_search_sig_t(C.Structure):
+    _fields_ = [("buff", C.c_int), ("M", C.c_int), ("N", C.c_int),
+                ("T", C.c_double), ("fs", C.c_double), ("fi", C.c_double),
+                ("n_prn", C.c_int), ("code_fft", c_void_p),
+                ("len_fds", C.c_int), ("fds", c_float_p),
+                ("n_blk", C.c_int), ("ix0", C.c_int), ("blk_step", C.c_int)]
+
+
+class sdr_trk_ch_t(C.Structure):
+    _fields_ = [("fs", C.c_double), ("fi", C.c_double), ("fc_sig", C.c_double),
+                ("T", C.c_double), ("N", C.c_int), ("fd0", C.c_double),
+                ("drift", C.c_double), ("phi0", C.c_double), ("coff0", C.c_double),
+                ("code", c_void_p)]
+
+
+def _decl(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_decl("sdr_func_init", None, [C.c_char_p])
+_decl("sdr_cpx_mul", None, [c_void_p, c_void_p, C.c_int, C.c_float, c_void_p])
+_decl("sdr_cpx_fft", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p])
+_decl("sdr_search_code", None, [c_void_p, C.c_double, C.POINTER(sdr_buff_t), C.c_int, C.c_int,
+                                C.c_double, C.c_double, c_float_p, C.c_int, c_float_p])
+_decl("sdr_corr_max", C.c_float, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_double, c_int_p])
+_decl("sdr_fine_dop", C.c_double, [c_float_p, C.c_int, c_float_p, C.c_int, c_int_p])
+_decl("sdr_dop_bins", c_float_p, [C.c_double, C.c_float, C.c_float, c_int_p])
+_decl("sdr_corr_std", None, [c_void_p, c_void_p, C.c_int, C.c_double, c_double_p, C.c_int,
+                             c_void_p, c_void_p])
+_decl("sdr_corr_std_cpx_code", None, [c_void_p, c_void_p, C.c_int, C.c_double, c_double_p,
+                                      C.c_int, c_void_p, c_void_p])
+_decl("sdr_corr_std_cpx", None, [c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, c_float_p, c_double_p, C.c_int, c_void_p])
+_decl("sdr_corr_fft", None, [c_void_p, c_void_p, C.c_int, c_void_p])
+_decl("sdr_corr_fft_cpx", None, [c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, c_void_p, c_void_p])
+_decl("sdr_mix_carr", None, [C.POINTER(sdr_buff_t), C.c_int, C.c_int, C.c_double, C.c_double,
+                             C.c_double, c_void_p])
+_decl("sdr_search_plan_new", c_void_p, [C.c_int, C.POINTER(C.POINTER(sdr_buff_t)), C.c_int,
+                                        C.POINTER(sdr_search_sig_t), C.c_int])
+_decl("sdr_search_plan_run", C.c_int, [c_void_p])
+_decl("sdr_search_plan_result", C.c_int, [c_void_p, C.c_int, c_float_p, c_int_p, c_double_p])
+_decl("sdr_search_plan_power", C.c_int, [c_void_p, C.c_int, C.c_int, c_float_p])
+_decl("sdr_search_plan_timing", None, [c_void_p, C.c_int])
+_decl("sdr_search_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
+                                             C.POINTER(C.c_longlong), c_double_p])
+_decl("sdr_search_plan_free", None, [c_void_p])
+_decl("sdr_trk_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sdr_trk_ch_t),
+                                     C.c_int, c_double_p, C.c_int])
+_decl("sdr_trk_plan_run", C.c_int, [c_void_p])
+_decl("sdr_trk_plan_result", C.c_int, [c_void_p, c_void_p, c_void_p])
+_decl("sdr_trk_plan_timing", None, [c_void_p, C.c_int])
+_decl("sdr_trk_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
+                                          C.POINTER(C.c_longlong), c_double_p])
+_decl("sdr_trk_plan_free", None, [c_void_p])
+_decl("sdr_amd_version", C.c_char_p, [])
+
+
+def _ptr(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def _fp(a):
+    return a.ctypes.data_as(c_float_p)
+
+
+def _dp(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+def _buff(data):
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    b = sdr_buff_t(data.ctypes.data_as(C.POINTER(C.c_uint8)), 2, len(data))
+    b._keep = data
+    return b
+
+
+def func_init():
+    lib.sdr_func_init(b"")
+
+
+def mix_carr(buff, ix, N, fs, fc, phi):
+    """sdr_mix_carr (src/sdr_func.c:824-836) -> int8 (N, 2)"""
+    b = _buff(buff)
+    IQ = np.zeros((N, 2), dtype=np.int8)
+    lib.sdr_mix_carr(C.byref(b), ix, N, fs, fc, phi, _ptr(IQ))
+    return IQ
+
+
+def corr_std(IQ, code, N, coff, pos, cpx_code=False):
+    """sdr_corr_std / sdr_corr_std_cpx_code -> (corr[n] complex64, C[2])"""
+    IQ = np.ascontiguousarray(IQ, dtype=np.int8)
+    code = np.ascontiguousarray(code, dtype=np.int8)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = len(pos)
+    corr = np.zeros(n, dtype=np.complex64)
+    Cc = np.zeros(2, dtype=np.complex64)
+    f = lib.sdr_corr_std_cpx_code if cpx_code else lib.sdr_corr_std
+    f(_ptr(IQ), _ptr(code), N, coff, _dp(pos), n, _ptr(corr), _ptr(Cc))
+    return corr, Cc
+
+
+def corr_std_cpx(buff, ix, N, fs, fc, phi, code, pos):
+    """sdr_corr_std_cpx (src/sdr_func.c:1060-1072), buff complex64"""
+    buff = np.ascontiguousarray(buff, dtype=np.complex64)
+    code = np.ascontiguousarray(code, dtype=np.float32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    corr = np.zeros(len(pos), dtype=np.complex64)
+    lib.sdr_corr_std_cpx(_ptr(buff), len(buff), ix, N, fs, fc, phi, _fp(code), _dp(pos),
+                         len(pos), _ptr(corr))
+    return corr
+
+
+def corr_fft(IQ, code_fft):
+    """sdr_corr_fft (src/sdr_func.c:1088-1102)"""
+    IQ = np.ascontiguousarray(IQ, dtype=np.int8)
+    code_fft = np.ascontiguousarray(code_fft, dtype=np.complex64)
+    N = len(code_fft)
+    corr = np.zeros(N, dtype=np.complex64)
+    lib.sdr_corr_fft(_ptr(IQ), _ptr(code_fft), N, _ptr(corr))
+    return corr
+
+
+def corr_fft_cpx(buff, ix, N, fs, fc, phi, code_fft):
+    buff = np.ascontiguousarray(buff, dtype=np.complex64)
+    code_fft = np.ascontiguousarray(code_fft, dtype=np.complex64)
+    corr = np.zeros(N, dtype=np.complex64)
+    lib.sdr_corr_fft_cpx(_ptr(buff), len(buff), ix, N, fs, fc, phi, _ptr(code_fft), _ptr(corr))
+    return corr
+
+
+def cpx_fft(x, dir=0):
+    """sdr_cpx_fft (src/sdr_func.c:236-264); raises on unsupported length"""
+    x = np.ascontiguousarray(x, dtype=np.complex64)
+    y = np.zeros_like(x)
+    if not lib.sdr_cpx_fft(_ptr(x), len(x), dir, _ptr(y)):
+        raise ValueError("sdr_cpx_fft failed (N=%d dir=%d)" % (len(x), dir))
+    return y
+
+
+def cpx_mul(a, b, s):
+    a = np.ascontiguousarray(a, dtype=np.complex64)
+    b = np.ascontiguousarray(b, dtype=np.complex64)
+    c = np.zeros_like(a)
+    lib.sdr_cpx_mul(_ptr(a), _ptr(b), len(a), s, _ptr(c))
+    return c
+
+
+def search_code(code_fft, T, buff, ix, fs, fi, fds, P=None):
+    """sdr_search_code (src/sdr_func.c:664-687): returns P (len_fds, N)"""
+    code_fft = np.ascontiguousarray(code_fft, dtype=np.complex64)
+    fds = np.ascontiguousarray(fds, dtype=np.float32)
+    N = len(code_fft)
+    if P is None:
+        P = np.zeros((len(fds), N), dtype=np.float32)
+    b = _buff(buff)
+    lib.sdr_search_code(_ptr(code_fft), T, C.byref(b), ix, N, fs, fi, _fp(fds), len(fds), _fp(P))
+    return P
+
+
+def corr_max(P, N, Nmax, M, T):
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    ix = np.zeros(2, dtype=np.int32)
+    cn0 = lib.sdr_corr_max(_fp(P), N, Nmax, M, T, ix.ctypes.data_as(c_int_p))
+    return cn0, ix
+
+
+def fine_dop(P, N, fds, ix):
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    fds = np.ascontiguousarray(fds, dtype=np.float32)
+    ix = np.ascontiguousarray(ix, dtype=np.int32)
+    return lib.sdr_fine_dop(_fp(P), N, _fp(fds), len(fds), ix.ctypes.data_as(c_int_p))
+
+
+def dop_bins(T, dop, max_dop):
+    n = C.c_int(0)
+    p = lib.sdr_dop_bins(T, dop, max_dop, C.byref(n))
+    out = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+    C.CDLL(None).free(C.cast(p, c_void_p))
+    return out
+
+
+SEARCH_BATCHED = 0
+SEARCH_PLAIN = 1
+SEARCH_GRAPH = 2
+
+
+class SearchPlan:
+    """Batched HBM-resident acquisition over signal groups (sdr_search_plan_*).
+
+    buffs: list of uint8 cpx8 IF streams.  sigs: list of dicts with keys
+    buff, M, N, T, fs, fi, code_fft (n_prn, M) complex64, fds float32, n_blk,
+    ix0, blk_step.
+    """
+
+    def __init__(self, buffs, sigs, flags=SEARCH_BATCHED):
+        self._bufs = [_buff(b) for b in buffs]
+        arr = (C.POINTER(sdr_buff_t) * len(buffs))(*[C.pointer(b) for b in self._bufs])
+        self._keep = []
+        sa = (sdr_search_sig_t * len(sigs))()
+        for i, s in enumerate(sigs):
+            cf = np.ascontiguousarray(s["code_fft"], dtype=np.complex64)
+            fds = np.ascontiguousarray(s["fds"], dtype=np.float32)
+            self._keep += [cf, fds]
+            sa[i] = sdr_search_sig_t(s["buff"], s["M"], s["N"], s["T"], s["fs"], s["fi"],
+                                     cf.shape[0], _ptr(cf), len(fds), _fp(fds), s["n_blk"],
+                                     s["ix0"], s["blk_step"])
+        self.sigs = sigs
+        self.p = lib.sdr_search_plan_new(len(buffs), arr, len(sigs), sa, flags)
+        if not self.p:
+            raise ValueError("sdr_search_plan_new failed")
+
+    def run(self):
+        if not lib.sdr_search_plan_run(self.p):
+            raise RuntimeError("sdr_search_plan_run failed")
+
+    def result(self, sig):
+        n = self.sigs[sig]["code_fft"].shape[0]
+        cn0 = np.zeros(n, np.float32)
+        ix = np.zeros((n, 2), np.int32)
+        dop = np.zeros(n, np.float64)
+        lib.sdr_search_plan_result(self.p, sig, _fp(cn0), ix.ctypes.data_as(c_int_p), _dp(dop))
+        return cn0, ix, dop
+
+    def power(self, sig, prn_idx):
+        s = self.sigs[sig]
+        P = np.zeros((len(s["fds"]), s["N"]), np.float32)
+        lib.sdr_search_plan_power(self.p, sig, prn_idx, _fp(P))
+        return P
+
+    def timing(self, on=True):
+        lib.sdr_search_plan_timing(self.p, 1 if on else 0)
+
+    def kernel_ms(self, name):
+        ms = C.c_double(0)
+        cnt = C.c_longlong(0)
+        el = C.c_double(0)
+        if not lib.sdr_search_plan_kernel_ms(self.p, name.encode(), C.byref(ms), C.byref(cnt),
+                                             C.byref(el)):
+            return None
+        return ms.value, cnt.value, el.value
+
+    def close(self):
+        if self.p:
+            lib.sdr_search_plan_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
+
+class TrkPlan:
+    """Batched open-loop tracking correlators (sdr_trk_plan_*)."""
+
+    def __init__(self, buff, chans, pos, n_epoch):
+        self._buf = _buff(buff)
+        self._keep = []
+        ca = (sdr_trk_ch_t * len(chans))()
+        for i, c in enumerate(chans):
+            code = np.ascontiguousarray(c["code"], dtype=np.int8)
+            self._keep.append(code)
+            ca[i] = sdr_trk_ch_t(c["fs"], c["fi"], c["fc_sig"], c["T"], c["N"], c["fd0"],
+                                 c["drift"], c["phi0"], c["coff0"], _ptr(code))
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.n_ch, self.n_epoch, self.n_pos = len(chans), n_epoch, len(self.pos)
+        self.p = lib.sdr_trk_plan_new(C.byref(self._buf), len(chans), ca, self.n_pos,
+                                      _dp(self.pos), n_epoch)
+        if not self.p:
+            raise ValueError("sdr_trk_plan_new failed")
+
+    def run(self):
+        if not lib.sdr_trk_plan_run(self.p):
+            raise RuntimeError("sdr_trk_plan_run failed")
+
+    def result(self):
+        corr = np.zeros((self.n_ch, self.n_epoch, self.n_pos), np.complex64)
+        Cc = np.zeros((self.n_ch, self.n_epoch, 2), np.complex64)
+        lib.sdr_trk_plan_result(self.p, _ptr(corr), _ptr(Cc))
+        return corr, Cc
+
+    def timing(self, on=True):
+        lib.sdr_trk_plan_timing(self.p, 1 if on else 0)
+
+    def kernel_ms(self, name):
+        ms = C.c_double(0)
+        cnt = C.c_longlong(0)
+        el = C.c_double(0)
+        if not lib.sdr_trk_plan_kernel_ms(self.p, name.encode(), C.byref(ms), C.byref(cnt),
+                                          C.byref(el)):
+            return None
+        return ms.value, cnt.value, el.value
+
+    def close(self):
+        if self.p:
+            lib.sdr_trk_plan_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()

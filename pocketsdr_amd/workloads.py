@@ -1,0 +1,104 @@
+"""Seeded synthetic workloads of the search / correlate hot path.
+
+acq_l1ca      GPS L1 C/A acquisition as pocket_acq runs it
+              (app/pocket_acq/pocket_acq.c:152-192): complex IQ cpx8 at fs,
+              1 ms coherent with code zero-padded to M = 2N, T_AQC = 10 ms of
+              non-coherent blocks, Doppler +/-5 kHz in 1/(2T) steps.
+trk_l1ca      open-loop tracking correlators on one IF stream: per channel
+              E/P/L at +/-0.5 chip, the N tap at -120 samples and 40 extra taps
+              over +/-2 chips (src/sdr_ch.c:228-231, 804-814).
+"""
+import numpy as np
+
+from . import codes, synth
+
+L1_FREQ = 1575.42e6
+
+# G2 delays of PRN 33-64 (IS-GPS-200 extended table, same values as the
+# reference's src/sdr_code.c:159-161)
+_L1CA_G2_DELAY_33_64 = (863, 950, 947, 948, 950, 67, 103, 91, 19, 679, 225, 625, 946, 638, 161,
+                        1001, 554, 280, 710, 709, 775, 864, 558, 220, 397, 55, 898, 759, 367, 299,
+                        1018, 729)
+
+
+def l1ca_code(prn):
+    if prn <= 32:
+        return codes.gen_code_L1CA(prn)
+    if prn > 64:
+        raise ValueError("PRN 1-64 only")
+    c1 = codes.gen_code_L1CA(1)  # ensures the shift registers exist
+    del c1
+    d = _L1CA_G2_DELAY_33_64[prn - 33]
+    idx = (np.arange(1023) + 1023 - d) % 1023
+    return (codes._G1 * codes._G2[idx]).astype(np.int8)
+
+
+def dop_bins(T, dop, max_dop):
+    """sdr_dop_bins arithmetic (src/sdr_func.c:758-769) in float32"""
+    step = np.float32(0.5 / T)
+    n = int(2.0 * float(np.float32(max_dop)) / float(step)) + 1
+    return (np.float32(dop) - np.float32(max_dop) + np.arange(n, dtype=np.float32) * step).astype(np.float32)
+
+
+def acq_l1ca(seed=1, fs=12e6, n_prn=32, n_vis=8, t_int=10e-3, max_dop=5000.0,
+             cn0=(38.0, 50.0), bits=2, code_fft_fn=None):
+    """Returns dict(buff, sig, truth).  code_fft_fn(code, T, fs, N) gives the
+    conjugated, zero-padded code spectrum (default: the library's GPU FFT)."""
+    rng = np.random.default_rng(seed)
+    T = 1e-3
+    N = int(fs * T)
+    M = 2 * N
+    n_samp = int(fs * (t_int + T))
+    prns = list(range(1, n_prn + 1))
+    vis = sorted(rng.choice(prns, n_vis, replace=False).tolist())
+    cd = {p: l1ca_code(p) for p in prns}
+    sats = synth.random_sats(rng, vis, cd, T, L1_FREQ, cn0, (-max_dop, max_dop))
+    buff = synth.synth_if(fs, n_samp, sats, seed, bits=bits)
+    fn = code_fft_fn or (lambda c, T_, fs_, N_: codes.gen_code_fft(c, T_, 0.0, fs_, N_, N_))
+    code_fft = np.stack([fn(cd[p], T, fs, N) for p in prns]).astype(np.complex64)
+    fds = dop_bins(T, 0.0, max_dop)
+    n_blk = len(range(0, n_samp - 2 * N + 1, N))
+    sig = dict(buff=0, M=M, N=N, T=T, fs=fs, fi=0.0, code_fft=code_fft, fds=fds,
+               n_blk=n_blk, ix0=0, blk_step=N, prns=prns)
+    return dict(buff=buff, sig=sig, truth=sats)
+
+
+def trk_positions(fs, T=1e-3, len_code=1023, sp_corr=1.0, nposx=40, width_chip=4.0):
+    """P, E, L, N (src/sdr_ch.c:224-231) + nposx extra taps (src/sdr_ch.c:804-814)"""
+    sc = T / len_code * fs
+    sp = sp_corr * sc
+    pos = [0.0, -0.5 * sp, 0.5 * sp, -120.0]
+    width = width_chip / 1.023e6
+    for i in range(nposx):
+        pos.append(float(i - (nposx - 1) // 2) / nposx * width * fs)
+    return np.array(pos, np.float64)
+
+
+def trk_l1ca(seed=2, fs=24e6, n_ch=64, n_epoch=1000, n_vis=8, cn0=(38.0, 50.0),
+             drift=(-2.0, 2.0), bits=2, buff_epochs=None, nposx=40):
+    """Returns dict(buff, chans, pos, n_epoch).  The IF holds n_vis of the
+    channels' PRNs; their channels start on the true code phase / Doppler."""
+    rng = np.random.default_rng(seed)
+    T = 1e-3
+    N = int(fs * T)
+    n_buf = N * (buff_epochs or n_epoch)
+    prns = list(range(1, n_ch + 1))
+    vis = sorted(rng.choice(prns, min(n_vis, n_ch), replace=False).tolist())
+    cd = {p: l1ca_code(p) for p in prns}
+    sats = synth.random_sats(rng, vis, cd, T, L1_FREQ, cn0, (-5000.0, 5000.0), drift)
+    truth = {s["prn"]: s for s in sats}
+    buff = synth.synth_if(fs, n_buf, sats, seed, bits=bits)
+    chans = []
+    for p in prns:
+        if p in truth:
+            s = truth[p]
+            fd0, dr, coff0, phi0 = s["fd"], s["drift"], s["tau"], 0.0
+        else:
+            fd0 = float(rng.uniform(-5000, 5000))
+            dr = float(rng.uniform(*drift))
+            coff0 = float(rng.uniform(0, T))
+            phi0 = float(rng.uniform(0, 1))
+        chans.append(dict(fs=fs, fi=0.0, fc_sig=L1_FREQ, T=T, N=N, fd0=fd0, drift=dr,
+                          phi0=phi0, coff0=coff0, code=codes.code_bank(cd[p], T, fs, N), prn=p))
+    return dict(buff=buff, chans=chans, pos=trk_positions(fs, T, nposx=nposx), n_epoch=n_epoch,
+                truth=sats)
