@@ -81,7 +81,8 @@ void orc_mix_carr(const uint8_t *buff, int buffN, int ix, int N, double fs,
     } else {
         int n = buffN - ix;
         mix_seg(buff + ix, n, phi, step, IQ);
-        mix_seg(buff, N - n, phi + step * n, step, IQ + 2 * n);
+        /* the reference build fuses phi + step * n (gcc -mfma) */
+        mix_seg(buff, N - n, fma((double)n, step, phi), step, IQ + 2 * n);
     }
 }
 
@@ -538,8 +539,9 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
         double tau = t - time;
         double fd = ch->fd0 + ch->drift * t;
         double fc = ch->fi + fd;
-        coff -= fd / ch->fc_sig * tau;
-        phi += fc * tau;
+        /* fused as in the reference build of src/sdr_ch.c:698-700 */
+        coff = fma(-(fd / ch->fc_sig), tau, coff);
+        phi = fma(fc, tau, phi);
         phi -= floor(phi);
         time = t;
         if (coff >= ch->T) coff -= ch->T;

@@ -72,3 +72,107 @@ def gen_code_fft(code, T, coff, fs, N, Nz):
     x = np.zeros(N + Nz, np.complex64)
     x[:N] = np.asarray(code, np.float32)[res_index(len(code), T, coff, fs, N)]
     return np.conj(sdr_func.cpx_fft(x, 0)).astype(np.complex64)
+
+
+# ---------------------------------------------------------------------------
+# other signals of the batched search (ICD parameter tables in
+# data/code_params.json, generation steps as src/sdr_code.c restates them)
+# ---------------------------------------------------------------------------
+import json as _json
+import os as _os
+
+_PARAMS = None
+
+
+def _params():
+    global _PARAMS
+    if _PARAMS is None:
+        with open(_os.path.join(_os.path.dirname(__file__), "data", "code_params.json")) as f:
+            _PARAMS = _json.load(f)
+    return _PARAMS
+
+
+def lfsr(N, R, tap, n):
+    """Fibonacci LFSR of src/sdr_code.c:772-781: chip = {+1, -1}[R & 1],
+    R = (parity(R & tap) << (n - 1)) | (R >> 1)."""
+    out = np.empty(N, np.int8)
+    for i in range(N):
+        out[i] = -1 if R & 1 else 1
+        R = ((bin(R & tap).count("1") & 1) << (n - 1)) | (R >> 1)
+    return out
+
+
+def rev_reg(R, N):
+    RR = 0
+    for i in range(N):
+        RR = (RR << 1) | ((R >> i) & 1)
+    return RR
+
+
+_L5 = {}
+
+
+def gen_code_L5Q(prn):
+    """GPS L5 Q5 (IS-GPS-705): XA (8190-chip short cycle, repeated) times XB
+    advanced per PRN; 10230 chips (src/sdr_code.c:1007-1061)."""
+    if not 1 <= prn <= 32:
+        raise ValueError("L5Q PRN must be 1-32")
+    if not _L5:
+        xa = lfsr(8190, 0x1FFF, 0x001B, 13)
+        _L5["XA"] = np.concatenate([xa, xa[:10230 - 8190]])
+        _L5["XB"] = lfsr(10230, 0x1FFF, 0x16E3, 13)
+    adv = _params()["L5Q_XB_adv"][prn - 1]
+    return (_L5["XA"] * _L5["XB"][(np.arange(10230) + adv) % 10230]).astype(np.int8)
+
+
+def gen_code_L6D(prn):
+    """QZSS L6D (IS-QZSS-L6): 10-stage x 20-stage Kasami-like product, TDM
+    with zero chips, 20460 entries (src/sdr_code.c:1166-1193)."""
+    if not 193 <= prn <= 202:
+        raise ValueError("L6D PRN must be 193-202")
+    c1 = lfsr(10230, 0x3FF, 0x0F3, 10)
+    c2 = lfsr(10230, rev_reg(_params()["L6D_R_init"][prn - 193], 20), 0x00053, 20)
+    out = np.zeros(20460, np.int8)
+    out[0::2] = c1 * c2
+    return out
+
+
+def gen_code_E1B(prn):
+    """Galileo E1-B memory code (hex table) with a BOC(1,1) sub-carrier,
+    8184 entries (src/sdr_code.c:1386-1400, 742-754)."""
+    if not 1 <= prn <= 36:
+        raise ValueError("E1B PRN must be 1-36")
+    h = _params()["E1B_hex"][prn - 1]
+    bits = np.array([(int(c, 16) >> (3 - j)) & 1 for c in h for j in range(4)], np.int8)[:4092]
+    chips = np.where(bits == 1, -1, 1).astype(np.int8)
+    return np.stack([chips, -chips], 1).reshape(-1).astype(np.int8)
+
+
+_LEG = None
+
+
+def gen_code_L1CP(prn):
+    """GPS L1C pilot (IS-GPS-800): Weil code from the 10223 Legendre sequence
+    with a 7-chip insertion, TMBOC(6,1,4/33), 122760 entries
+    (src/sdr_code.c:838-913)."""
+    global _LEG
+    if not 1 <= prn <= 32:
+        raise ValueError("L1CP PRN must be 1-32")
+    if _LEG is None:
+        L = np.ones(10223, np.int8)
+        i = np.arange(1, 10223, dtype=np.int64)
+        L[(i * i) % 10223] = -1
+        _LEG = L
+    w = _params()["L1CP_weil_idx"][prn - 1]
+    p = _params()["L1CP_ins_idx"][prn - 1]
+    t = np.arange(10230)
+    src = np.where(t < p - 1, t, t - 7)
+    code = (_LEG[src % 10223] * _LEG[(src + w) % 10223]).astype(np.int8)
+    code[p - 1:p + 6] = [1, -1, -1, 1, -1, 1, 1]
+    i = np.arange(10230) % 33
+    b6 = (i == 0) | (i == 4) | (i == 6) | (i == 29)
+    j = np.arange(12)
+    sub_boc61 = np.where(j % 2 == 1, -1, 1)
+    sub_boc11 = np.where(j >= 6, -1, 1)
+    sub = np.where(b6[:, None], sub_boc61[None, :], sub_boc11[None, :])
+    return (code[:, None] * sub).reshape(-1).astype(np.int8)

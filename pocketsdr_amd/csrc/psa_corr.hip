@@ -255,8 +255,9 @@ struct TrkEpoch {
 };
 
 // Open-loop NCO / code advance, src/sdr_ch.c:694-703 with adj_coff
-// (674-689), in FP64, one thread per channel, sequential over epochs.
-// Products and sums are kept unfused so the result is fixed by the source.
+// (674-689), in FP64, one thread per channel, sequential over epochs.  The
+// reference build fuses coff -= fd / fc * tau and phi += fc * tau into
+// fnmadd / fmadd; the same fused forms are written here explicitly.
 __global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int n_epoch, int N,
                                  int buffN, const double *__restrict__ pos, int n_pos,
                                  TrkEpoch *__restrict__ ep, int2 *__restrict__ jk)
@@ -272,8 +273,8 @@ __global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int
         double tau = t - time;
         double fd = ch.fd0 + ch.drift * t;
         double fc = ch.fi + fd;
-        coff = coff - fd / ch.fc_sig * tau;
-        phi = phi + fc * tau;
+        coff = fma(-(fd / ch.fc_sig), tau, coff);
+        phi = fma(fc, tau, phi);
         phi = phi - floor(phi);
         time = t;
         if (coff >= ch.T) coff = coff - ch.T;
@@ -290,7 +291,7 @@ __global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int
             q.p1 = 0;
         } else {
             int n = buffN - ix;
-            double phi2 = phi + step * n;
+            double phi2 = fma((double)n, step, phi);
             q.nseg = n;
             q.p1 = (uint32_t)((phi2 - floor(phi2)) * scale);
         }

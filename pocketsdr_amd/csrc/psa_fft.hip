@@ -24,7 +24,7 @@ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(__builtin_fmaf(a.x, b.x, -a.y * b.y), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 // multiply by -i (forward) or +i (backward): sgn = -1 forward, +1 backward
 __device__ __forceinline__ float2 mul_i(float2 a, float sgn) { return make_float2(-sgn * a.y, sgn * a.x); }

@@ -7,7 +7,8 @@ namespace psa {
 
 // Phase setup of mix_carr (src/sdr_func.c:776-778), including the split at
 // the ring-buffer end (src/sdr_func.c:831-835) where the second segment
-// restarts from phi + step * n, re-derived in double.
+// restarts from phi + step * n, re-derived in double (the reference build
+// fuses it: fma(n, step, phi)).
 void mix_params(int buffN, int ix, int N, double fs, double fc, double phi,
                 uint32_t *p0, uint32_t *p1, int *nseg, uint32_t *s)
 {
@@ -20,7 +21,7 @@ void mix_params(int buffN, int ix, int N, double fs, double fc, double phi,
         *p1 = 0;
     } else {
         int n = buffN - ix;
-        double phi2 = phi + step * n;
+        double phi2 = fma((double)n, step, phi);
         *nseg = n;
         *p1 = (uint32_t)((phi2 - floor(phi2)) * scale);
     }

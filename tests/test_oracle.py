@@ -91,7 +91,8 @@ def test_mix_phase_and_wrap(orc):
     # ring split: the second segment restarts from phi + step * n
     IQ2 = orc.mix_carr(buff, 900, 300, fs, fc, phi)
     assert np.array_equal(IQ2[:100], orc.mix_carr(buff[900:], 0, 100, fs, fc, phi))
-    phi2 = phi + fc / fs * 100
+    from fractions import Fraction
+    phi2 = float(Fraction(fc / fs) * 100 + Fraction(phi))    # fma(n, step, phi)
     assert np.array_equal(IQ2[100:], orc.mix_carr(buff, 0, 200, fs, fc, phi2))
 
 
@@ -376,7 +377,7 @@ def test_trk_run_composition(orc):
     phi = 0.25
     coff = 0.3e-3
     for e in range(7):
-        coff -= fd[e] / 1575.42e6 * 1e-3
+        coff -= fd[e] / 1575.42e6 * 1e-3                   # fused in the oracle
         phi += (1000.0 + fd[e]) * 1e-3
         phi -= np.floor(phi)
         assert abs(state[e, 0] - phi) < 1e-12 and abs(state[e, 1] - coff) < 1e-15
