@@ -159,6 +159,11 @@ void sdr_search_plan_timing(sdr_search_plan_t *plan, int on);
  * "peak") over the timed runs.  Returns 0 if the class never ran. */
 int sdr_search_plan_kernel_ms(const sdr_search_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);
+/* Average algorithmic HBM bytes per launch of kernel class `name` over the
+ * timed runs: the compulsory reads and writes of its operands (IF bytes once
+ * per block, shared spectra once per launch, power rows read and written). */
+int sdr_search_plan_kernel_bytes(const sdr_search_plan_t *plan, const char *name,
+    double *avg_bytes);
 void sdr_search_plan_free(sdr_search_plan_t *plan);
 
 /* --- tracking correlators -------------------------------------------------

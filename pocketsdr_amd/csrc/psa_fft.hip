@@ -98,6 +98,10 @@ __device__ __forceinline__ void bfly5(float2 *a, float sgn)
 }
 
 // One Stockham stage of radix R over G sub-transforms of length L in LDS.
+// Called with compile-time L, G, Ns on the specialised path (divisions become
+// multiplies) and with runtime values on the generic path.
+#define PASS_THREADS 256
+
 template <int R>
 __device__ __forceinline__ void stage(const float2 *__restrict__ src, float2 *__restrict__ dst,
                                       int L, int G, int Ns, const float2 *__restrict__ tw,
@@ -106,9 +110,9 @@ __device__ __forceinline__ void stage(const float2 *__restrict__ src, float2 *__
     const int Lr = L / R;
     const int tstep = L / (Ns * R);
     const int nb = G * Lr;
-    for (int bi = threadIdx.x; bi < nb; bi += blockDim.x) {
-        int g = bi / Lr, j = bi - g * Lr;
-        int k = j % Ns;
+    for (int bi = threadIdx.x; bi < nb; bi += PASS_THREADS) {
+        const int g = bi / Lr, j = bi - g * Lr;
+        const int k = j % Ns;
         const float2 *s = src + g * L + j;
         float2 a[R];
 #pragma unroll
@@ -122,7 +126,7 @@ __device__ __forceinline__ void stage(const float2 *__restrict__ src, float2 *__
         else if (R == 4) bfly4(a, sgn);
         else if (R == 5) bfly5(a, sgn);
         else if (R == 8) bfly8(a, sgn);
-        float2 *d = dst + g * L + (j / Ns) * Ns * R + k;
+        float2 *d = dst + g * L + (j - k) * R + k;
 #pragma unroll
         for (int q = 0; q < R; q++) d[q * Ns] = a[q];
     }
@@ -133,30 +137,86 @@ struct RadixSeq {
     int r[16];
 };
 
+// radix plan of a sub-transform length: 8s first, then 4, 2, 5s, 3s
+struct RadPlan {
+    int n;
+    int r[16];
+};
+__host__ __device__ constexpr RadPlan make_rad_plan(int L)
+{
+    RadPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    int n = L;
+    while (n % 8 == 0 && n > 1) { p.r[p.n++] = 8; n /= 8; }
+    if (n % 4 == 0) { p.r[p.n++] = 4; n /= 4; }
+    if (n % 2 == 0) { p.r[p.n++] = 2; n /= 2; }
+    while (n % 5 == 0 && n > 1) { p.r[p.n++] = 5; n /= 5; }
+    while (n % 3 == 0 && n > 1) { p.r[p.n++] = 3; n /= 3; }
+    return p;
+}
+
+// compile-time stage sequence (recursion over the plan)
+template <int CL, int CG, int IDX, int NS>
+__device__ __forceinline__ float2 *run_stages_ct(float2 *src, float2 *dst, const float2 *tw, float sgn)
+{
+    constexpr RadPlan P = make_rad_plan(CL);
+    if constexpr (IDX < P.n) {
+        constexpr int R = P.r[IDX];
+        stage<R>(src, dst, CL, CG, NS, tw, sgn);
+        __syncthreads();
+        return run_stages_ct<CL, CG, IDX + 1, NS * R>(dst, src, tw, sgn);
+    } else {
+        return src;
+    }
+}
+
+__device__ float2 *run_stages_rt(float2 *src, float2 *dst, int L, int G, const RadixSeq &rs,
+                                 const float2 *tw, float sgn)
+{
+    int Ns = 1;
+    for (int p = 0; p < rs.n; p++) {
+        const int r = rs.r[p];
+        switch (r) {
+        case 2: stage<2>(src, dst, L, G, Ns, tw, sgn); break;
+        case 3: stage<3>(src, dst, L, G, Ns, tw, sgn); break;
+        case 4: stage<4>(src, dst, L, G, Ns, tw, sgn); break;
+        case 5: stage<5>(src, dst, L, G, Ns, tw, sgn); break;
+        case 8: stage<8>(src, dst, L, G, Ns, tw, sgn); break;
+        }
+        __syncthreads();
+        float2 *t = src; src = dst; dst = t;
+        Ns *= r;
+    }
+    return src;
+}
+
 // ---------------------------------------------------------------------------
-// the pass kernel
+// the pass kernel: CL / CG > 0 select a compile-time sub-transform length and
+// group count, 0 the generic runtime path
 // ---------------------------------------------------------------------------
-template <int LM, int SM>
-__global__ void __launch_bounds__(256) pass_kernel(PassArgs a, RadixSeq rs, int ngroups)
+template <int LM, int SM, int CL, int CG>
+__global__ void __launch_bounds__(PASS_THREADS) pass_kernel(PassArgs a, RadixSeq rs, int ngroups)
 {
     extern __shared__ float2 smem[];
     __shared__ uint32_t carr[256];
-    const int L = a.L, G = a.G;
+    const int L = CL > 0 ? CL : a.L;
+    const int G = CL > 0 ? CG : a.G;
     float2 *buf0 = smem, *buf1 = smem + G * L;
-    const long long b = blockIdx.x / ngroups;
-    const int grp = blockIdx.x - (int)(b * ngroups);
+    const int b = (int)(blockIdx.x / (unsigned)ngroups);
+    const int grp = (int)blockIdx.x - b * ngroups;
     const int g0 = grp * G;
     const int gcount = min(G, a.nsub - g0);
     const float sgn = a.dir == 0 ? -1.0f : 1.0f;
 
     if (LM == LD_MIX) {
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) carr[i] = a.carr[i];
+        for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
         __syncthreads();
     }
 
     // ---- load G sub-transforms (element (g, n) -> LDS[g * L + n]) ----
+    // element index within the batch item: e = (g0 + g) * in_gstride + n * in_nstride
     const bool in_nfast = (a.in_nstride == 1);
     const int tot = G * L;
+    const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
     int ix = 0, nseg = 0;
     uint32_t p0 = 0, p1 = 0, s = 0;
     if (LM == LD_MIX) {
@@ -164,54 +224,58 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs a, RadixSeq rs, int 
         p0 = a.mix_p0[b]; p1 = a.mix_p1[b]; s = a.mix_s[b];
     }
     const float2 *xin = nullptr, *cin = nullptr;
+    bool prod_tail = false;
     if (LM == LD_PROD) {
-        long long xb = (b / a.prod_xdiv) % a.prod_xmod;
-        long long cb = b / a.prod_cdiv;
+        const long long xb = (b / a.prod_xdiv) % a.prod_xmod;
+        const long long cb = b / a.prod_cdiv;
         xin = a.in + xb * a.in_bstride;
         cin = a.cf + cb * a.cf_bstride;
+        prod_tail = a.prod_V < a.M;
     }
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const float2 *pin = (LM == LD_CPX) ? a.in + (long long)b * a.in_bstride : nullptr;
+    const char2 *qin = (LM == LD_CPX16) ? reinterpret_cast<const char2 *>(a.iq16) + (long long)b * a.in_bstride : nullptr;
+    for (int idx = threadIdx.x; idx < tot; idx += PASS_THREADS) {
         int g, n;
         if (in_nfast) { g = idx / L; n = idx - g * L; }
         else { n = idx / G; g = idx - n * G; }
         float2 v = make_float2(0.f, 0.f);
         if (g < gcount) {
-            long long e = (long long)(g0 + g) * a.in_gstride + (long long)n * a.in_nstride;
+            const int e = (g0 + g) * igs + n * ins;
             if (LM == LD_CPX) {
-                v = a.in[b * a.in_bstride + e];
+                v = pin[e];
             } else if (LM == LD_CPX16) {
-                const char2 q = reinterpret_cast<const char2 *>(a.iq16)[b * a.in_bstride + e];
+                const char2 q = qin[e];
                 v = make_float2((float)q.x * PSA_CSCALE, (float)q.y * PSA_CSCALE);
             } else if (LM == LD_MIX) {
-                int en = (int)e;
                 int src;
                 uint32_t ph;
-                if (en < nseg) { src = ix + en; ph = p0 + (uint32_t)en * s; }
-                else { src = a.mix_compact ? en : en - nseg; ph = p1 + (uint32_t)(en - nseg) * s; }
-                uint32_t d = a.buf[src];
-                int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
-                int sQ = (int)((int8_t)(uint8_t)d) >> 4;
-                uint32_t c = carr[ph >> 24];
-                int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
-                int8_t mI = (int8_t)(sI * cI - sQ * cQ);
-                int8_t mQ = (int8_t)(sI * cQ + sQ * cI);
+                if (e < nseg) { src = ix + e; ph = p0 + (uint32_t)e * s; }
+                else { src = a.mix_compact ? e : e - nseg; ph = p1 + (uint32_t)(e - nseg) * s; }
+                const uint32_t d = a.buf[src];
+                const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+                const int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+                const uint32_t c = carr[ph >> 24];
+                const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+                const int8_t mI = (int8_t)(sI * cI - sQ * cQ);
+                const int8_t mQ = (int8_t)(sI * cQ + sQ * cI);
                 v = make_float2((float)mI * PSA_CSCALE, (float)mQ * PSA_CSCALE);
             } else if (LM == LD_PROD) {
                 // sdr_cpx_mul AVX2 lanes (src/sdr_func.c:203-213):
-                // re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) + fl(ai br), * s
-                // (tail, src/sdr_func.c:229-232, contracted by the reference
-                // build: fmaf(ar, br, -(ai bi)), fmaf(ar, bi, ai br))
-                float2 x = xin[e], c = cin[e];
-                long long nat = e / a.prod_M2 + (long long)a.prod_M1 * (e % a.prod_M2);
+                // re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) + fl(ai br), * s;
+                // the last M % 4 natural indices take the scalar tail form
+                // (src/sdr_func.c:229-232, contracted by the reference build)
+                const float2 x = xin[e], c = cin[e];
+                bool tail = false;
+                if (prod_tail) tail = (e / a.prod_M2 + a.prod_M1 * (e % a.prod_M2)) >= a.prod_V;
                 float re, im;
-                if (nat < a.prod_V) {
-                    re = __fadd_rn(__fmul_rn(x.x, c.x), __fmul_rn(x.y, -c.y));
-                    im = __fadd_rn(__fmul_rn(x.x, c.y), __fmul_rn(x.y, c.x));
+                if (!tail) {
+                    re = x.x * c.x + x.y * -c.y;
+                    im = x.x * c.y + x.y * c.x;
                 } else {
-                    re = __fmaf_rn(x.x, c.x, -__fmul_rn(x.y, c.y));
-                    im = __fmaf_rn(x.x, c.y, __fmul_rn(x.y, c.x));
+                    re = __builtin_fmaf(x.x, c.x, -(x.y * c.y));
+                    im = __builtin_fmaf(x.x, c.y, x.y * c.x);
                 }
-                v = make_float2(__fmul_rn(re, a.prod_scale), __fmul_rn(im, a.prod_scale));
+                v = make_float2(re * a.prod_scale, im * a.prod_scale);
             }
         }
         buf0[g * L + n] = v;
@@ -219,46 +283,38 @@ __global__ void __launch_bounds__(256) pass_kernel(PassArgs a, RadixSeq rs, int 
     __syncthreads();
 
     // ---- Stockham stages in LDS ----
-    float2 *src = buf0, *dst = buf1;
-    int Ns = 1;
-    for (int p = 0; p < rs.n; p++) {
-        int r = rs.r[p];
-        switch (r) {
-        case 2: stage<2>(src, dst, L, G, Ns, a.tw, sgn); break;
-        case 3: stage<3>(src, dst, L, G, Ns, a.tw, sgn); break;
-        case 4: stage<4>(src, dst, L, G, Ns, a.tw, sgn); break;
-        case 5: stage<5>(src, dst, L, G, Ns, a.tw, sgn); break;
-        case 8: stage<8>(src, dst, L, G, Ns, a.tw, sgn); break;
-        }
-        __syncthreads();
-        float2 *t = src; src = dst; dst = t;
-        Ns *= r;
-    }
+    float2 *res;
+    if constexpr (CL > 0) res = run_stages_ct<CL, CG, 0, 1>(buf0, buf1, a.tw, sgn);
+    else res = run_stages_rt(buf0, buf1, L, G, rs, a.tw, sgn);
 
     // ---- store ----
     const bool out_nfast = (a.out_nstride == 1);
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int ogs = (int)a.out_gstride, ons = (int)a.out_nstride;
+    float2 *pout = (SM == ST_CPX) ? a.out + (long long)b * a.out_bstride : nullptr;
+    float *prow = nullptr;
+    if (SM == ST_POW) {
+        const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
+        prow = a.P + row * a.p_bstride;
+    }
+    for (int idx = threadIdx.x; idx < tot; idx += PASS_THREADS) {
         int g, k;
         if (out_nfast) { g = idx / L; k = idx - g * L; }
         else { k = idx / G; g = idx - k * G; }
         if (g >= gcount) continue;
-        float2 v = src[g * L + k];
+        float2 v = res[g * L + k];
         const int gg = g0 + g;
         if (a.thi) {
-            long long t = ((long long)gg * k) % a.M;
-            int ti = (int)t;
-            float2 w = cmul(a.thi[ti >> 10], a.tlo[ti & 1023]);
-            v = cmul(v, w);
+            // w_M^{gg k}: gg < nsub and k < L with nsub * L = M, so gg k < M
+            const int ti = gg * k;
+            v = cmul(v, cmul(a.thi[ti >> 10], a.tlo[ti & 1023]));
         }
-        long long e = (long long)gg * a.out_gstride + (long long)k * a.out_nstride;
+        const int e = gg * ogs + k * ons;
         if (SM == ST_CPX) {
-            a.out[b * a.out_bstride + e] = v;
+            pout[e] = v;
         } else {
             if (e < a.nlag) {
-                long long row = (b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
-                float *pp = a.P + row * a.p_bstride + e;
                 // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680)
-                *pp = __fadd_rn(*pp, __fmaf_rn(v.x, v.x, __fmul_rn(v.y, v.y)));
+                prow[e] = prow[e] + __builtin_fmaf(v.x, v.x, v.y * v.y);
             }
         }
     }
@@ -368,49 +424,71 @@ const FFTPlan *fft_plan(int M)
     return p;
 }
 
-template <int LM, int SM>
+// (L, G) pairs with compile-time kernels: the sub-lengths of the plans for
+// M = 24000, 48000, 96000, 192000, 960000 (column pass G >= 8 keeps the
+// strided loads in >= 64-byte segments; G * L <= 4096 complex per buffer)
+#define PSA_SPECS(X) \
+    X(160, 16) X(150, 16) X(240, 16) X(200, 20) X(320, 12) X(300, 13) \
+    X(480, 8) X(400, 10) X(1000, 4) X(960, 4)
+
+static int spec_groups(int L)
+{
+#define PSA_G(L_, G_) if (L == L_) return G_;
+    PSA_SPECS(PSA_G)
+#undef PSA_G
+    return 0;
+}
+
+template <int LM, int SM, int CL, int CG>
 static void launch_t(const PassArgs &a, const RadixSeq &rs, int nbatch, hipStream_t st)
 {
-    int ngroups = (a.nsub + a.G - 1) / a.G;
-    long long nblk = (long long)ngroups * nbatch;
+    const int ngroups = (a.nsub + a.G - 1) / a.G;
+    const long long nblk = (long long)ngroups * nbatch;
     if (nblk <= 0) return;
     if (nblk > 0x7FFFFFFFLL) PSA_FAIL("FFT launch too large (%lld workgroups)", nblk);
-    size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
-    hipLaunchKernelGGL((pass_kernel<LM, SM>), dim3((unsigned)nblk), dim3(256), shm, st, a, rs, ngroups);
+    if ((long long)nbatch * a.M > 0x7FFFFFFFLL && (a.in_bstride == 0 || a.out_bstride == 0)) {
+        PSA_FAIL("FFT launch too large");
+    }
+    const size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
+    hipLaunchKernelGGL((pass_kernel<LM, SM, CL, CG>), dim3((unsigned)nblk), dim3(PASS_THREADS), shm, st,
+                       a, rs, ngroups);
     PSA_CHECK(hipGetLastError());
 }
 
-template <int LM, int SM>
+template <int LM, int SM, int CL, int CG>
 static void set_attr()
 {
-    PSA_CHECK(hipFuncSetAttribute((const void *)pass_kernel<LM, SM>,
+    PSA_CHECK(hipFuncSetAttribute((const void *)pass_kernel<LM, SM, CL, CG>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+}
+
+template <int CL, int CG>
+static void set_attrs_l()
+{
+    set_attr<LD_CPX, ST_CPX, CL, CG>();
+    set_attr<LD_CPX16, ST_CPX, CL, CG>();
+    set_attr<LD_MIX, ST_CPX, CL, CG>();
+    set_attr<LD_PROD, ST_CPX, CL, CG>();
+    set_attr<LD_CPX, ST_POW, CL, CG>();
+    set_attr<LD_PROD, ST_POW, CL, CG>();
 }
 
 void fft_init_attrs()
 {
     static std::once_flag once;
     std::call_once(once, [] {
-        set_attr<LD_CPX, ST_CPX>();
-        set_attr<LD_CPX16, ST_CPX>();
-        set_attr<LD_MIX, ST_CPX>();
-        set_attr<LD_PROD, ST_CPX>();
-        set_attr<LD_CPX, ST_POW>();
-        set_attr<LD_PROD, ST_POW>();
+        set_attrs_l<0, 0>();
+#define PSA_A(L_, G_) set_attrs_l<L_, G_>();
+        PSA_SPECS(PSA_A)
+#undef PSA_A
     });
 }
 
-void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hipStream_t st)
+template <int CL, int CG>
+static void launch_modes(LoadMode lm, StoreMode sm, const PassArgs &a, const RadixSeq &rs,
+                         int nbatch, hipStream_t st)
 {
-    PassArgs a = a_in;
-    a.carr = dev_carrier();
-    RadixSeq rs;
-    // the radix sequence is looked up from the cached plan tables by length
-    SubPlan sp;
-    radix_seq(a.L, sp);
-    rs.n = sp.nrad;
-    for (int i = 0; i < sp.nrad; i++) rs.r[i] = sp.rad[i];
-#define PSA_LAUNCH(L_, S_) if (lm == L_ && sm == S_) { launch_t<L_, S_>(a, rs, nbatch, st); return; }
+#define PSA_LAUNCH(L_, S_) if (lm == L_ && sm == S_) { launch_t<L_, S_, CL, CG>(a, rs, nbatch, st); return; }
     PSA_LAUNCH(LD_CPX, ST_CPX)
     PSA_LAUNCH(LD_CPX16, ST_CPX)
     PSA_LAUNCH(LD_MIX, ST_CPX)
@@ -421,10 +499,28 @@ void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hi
     PSA_FAIL("unsupported FFT pass mode %d/%d", (int)lm, (int)sm);
 }
 
-// G sub-transforms per workgroup: as many as fit 4096 complex in LDS
+void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hipStream_t st)
+{
+    PassArgs a = a_in;
+    a.carr = dev_carrier();
+    RadixSeq rs;
+    const RadPlan rp = make_rad_plan(a.L);
+    rs.n = rp.n;
+    for (int i = 0; i < 16; i++) rs.r[i] = rp.r[i];
+    if ((long long)a.M * 2 > 0x7FFFFFFFLL) PSA_FAIL("FFT length too large");
+#define PSA_L(L_, G_) if (a.L == L_ && a.G == G_) { launch_modes<L_, G_>(lm, sm, a, rs, nbatch, st); return; }
+    PSA_SPECS(PSA_L)
+#undef PSA_L
+    launch_modes<0, 0>(lm, sm, a, rs, nbatch, st);
+}
+
+// G sub-transforms per workgroup: the specialised count where one exists,
+// else as many as fit 4096 complex per LDS buffer
 int pass_groups(int L, int nsub)
 {
-    int G = 4096 / (L > 0 ? L : 1);
+    int G = spec_groups(L);
+    if (G > 0) return G;
+    G = 4096 / (L > 0 ? L : 1);
     if (G < 1) G = 1;
     if (G > nsub) G = nsub;
     return G;

@@ -454,23 +454,25 @@ struct sdr_search_plan_tag {
     hipStream_t st = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool timing = false;
-    struct Ev { int cls; hipEvent_t a, b; double elems; };
+    struct Ev { int cls; hipEvent_t a, b; double elems, bytes; };
     std::vector<Ev> evs;
     size_t ev_used = 0;
     double ms[psa::K_NCLS] = {0};
     long long cnt[psa::K_NCLS] = {0};
     double elems[psa::K_NCLS] = {0};
+    double bytes[psa::K_NCLS] = {0};
 };
 
 namespace psa {
 
-// records a timed launch when timing is on
+// records a timed launch when timing is on; `bytes` is the launch's
+// algorithmic HBM traffic (compulsory reads + writes of its operands)
 struct Tick {
     sdr_search_plan_t *p;
     int cls;
     double elems;
     size_t slot;
-    Tick(sdr_search_plan_t *p_, int c, double e) : p(p_), cls(c), elems(e), slot(0)
+    Tick(sdr_search_plan_t *p_, int c, double e, double bytes = 0.0) : p(p_), cls(c), elems(e), slot(0)
     {
         if (!p->timing) return;
         if (p->ev_used == p->evs.size()) {
@@ -482,6 +484,7 @@ struct Tick {
         slot = p->ev_used++;
         p->evs[slot].cls = cls;
         p->evs[slot].elems = elems;
+        p->evs[slot].bytes = bytes;
         PSA_CHECK(hipEventRecord(p->evs[slot].a, p->st));
     }
     ~Tick()
@@ -513,14 +516,14 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
             a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
             a.out = p->tmp; a.out_bstride = M;
             {
-                Tick t(p, K_FWD_COL, (double)nbc * M);
+                Tick t(p, K_FWD_COL, (double)nbc * M, (double)M + 8.0 * nbc * M);
                 launch_pass(LD_MIX, ST_CPX, a, nbc, st);
             }
             PassArgs r = {};
             args_fwd_row(pl, r);
             r.in = p->tmp; r.out = p->X; r.in_bstride = r.out_bstride = M;
             {
-                Tick t(p, K_FWD_ROW, (double)nbc * M);
+                Tick t(p, K_FWD_ROW, (double)nbc * M, 16.0 * nbc * M);
                 launch_pass(LD_CPX, ST_CPX, r, nbc, st);
             }
             for (int q0 = 0; q0 < nprn; q0 += s.prn_chunk) {
@@ -534,7 +537,7 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 set_prod(pl, ir, 1.0f / M / M);
                 ir.out = p->Z; ir.out_bstride = M;
                 {
-                    Tick t(p, K_INV_ROW, (double)nbatch * M);
+                    Tick t(p, K_INV_ROW, (double)nbatch * M, 8.0 * M * (nbc + nqc + (double)nbatch));
                     launch_pass(LD_PROD, ST_CPX, ir, nbatch, st);
                 }
                 PassArgs ic = {};
@@ -544,7 +547,7 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 ic.p_div = nbc; ic.p_off1 = q0; ic.p_mul = nbin; ic.p_off2 = b0;
                 ic.nlag = N;
                 {
-                    Tick t(p, K_INV_COL, (double)nbatch * M);
+                    Tick t(p, K_INV_COL, (double)nbatch * M, (double)nbatch * (8.0 * M + 8.0 * N));
                     launch_pass(LD_CPX, ST_POW, ic, nbatch, st);
                 }
             }
@@ -622,7 +625,8 @@ static void run_all(sdr_search_plan_t *p)
     for (auto &s : p->sig) {
         if (p->flags & SDR_SEARCH_PLAIN) run_plain(p, s);
         else run_batched(p, s);
-        Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N);
+        Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N,
+               4.0 * s.h.n_prn * s.h.len_fds * s.h.N);
         hipLaunchKernelGGL(peak_kernel, dim3(s.h.n_prn), dim3(PEAK_THREADS), 0, p->st, s.P, s.h.N,
                            s.h.len_fds, s.h.T, s.fds, s.cn0, s.ix, s.dop);
         PSA_CHECK(hipGetLastError());
@@ -761,6 +765,7 @@ int sdr_search_plan_run(sdr_search_plan_t *p)
             p->ms[p->evs[i].cls] += ms;
             p->cnt[p->evs[i].cls]++;
             p->elems[p->evs[i].cls] += p->evs[i].elems;
+            p->bytes[p->evs[i].cls] += p->evs[i].bytes;
         }
     }
     return 1;
@@ -791,7 +796,19 @@ void sdr_search_plan_timing(sdr_search_plan_t *p, int on)
 {
     if (!p) return;
     p->timing = on != 0;
-    for (int i = 0; i < K_NCLS; i++) { p->ms[i] = 0; p->cnt[i] = 0; p->elems[i] = 0; }
+    for (int i = 0; i < K_NCLS; i++) { p->ms[i] = 0; p->cnt[i] = 0; p->elems[i] = 0; p->bytes[i] = 0; }
+}
+
+int sdr_search_plan_kernel_bytes(const sdr_search_plan_t *p, const char *name, double *avg_bytes)
+{
+    if (!p || !name) return 0;
+    for (int i = 0; i < K_NCLS; i++) {
+        if (strcmp(name, k_names[i]) == 0 && p->cnt[i]) {
+            if (avg_bytes) *avg_bytes = p->bytes[i] / p->cnt[i];
+            return 1;
+        }
+    }
+    return 0;
 }
 
 int sdr_search_plan_kernel_ms(const sdr_search_plan_t *p, const char *name, double *avg_ms,

@@ -102,3 +102,56 @@ def trk_l1ca(seed=2, fs=24e6, n_ch=64, n_epoch=1000, n_vis=8, cn0=(38.0, 50.0),
                           phi0=phi0, coff0=coff0, code=codes.code_bank(cd[p], T, fs, N), prn=p))
     return dict(buff=buff, chans=chans, pos=trk_positions(fs, T, nposx=nposx), n_epoch=n_epoch,
                 truth=sats)
+
+
+# ---------------------------------------------------------------------------
+# batched uneven search: four signals on three IF streams
+# ---------------------------------------------------------------------------
+MULTI_SIGNALS = (
+    # name, stream, fs, code period, prns, generator, carrier, n_visible
+    ("L1CA", 0, 12e6, 1e-3, list(range(1, 33)), "L1CA", L1_FREQ, 8),
+    ("E1B", 0, 12e6, 4e-3, list(range(1, 37)), "E1B", L1_FREQ, 9),
+    ("L5Q", 1, 24e6, 1e-3, list(range(1, 33)), "L5Q", 1176.45e6, 8),
+    ("L6D", 2, 24e6, 4e-3, list(range(193, 203)), "L6D", 1278.75e6, 3),
+)
+
+
+def sig_code(gen, prn):
+    if gen == "L1CA":
+        return l1ca_code(prn)
+    return getattr(codes, "gen_code_" + gen)(prn)
+
+
+def acq_multi(seed=5, t_int=10e-3, max_dop=5000.0, cn0=(38.0, 50.0), bits=2, code_fft_fn=None,
+              signals=MULTI_SIGNALS, prn_limit=None):
+    """GPS L1 C/A and Galileo E1-B on one 12 Msps L1 stream, GPS L5-Q and
+    QZSS L6D on 24 Msps streams; T_AQC = 10 ms plus one code period per
+    stream (pocket_acq's buffer length), Doppler bins 1/(2T) over +/-5 kHz."""
+    rng = np.random.default_rng(seed)
+    fn = code_fft_fn or (lambda c, T, fs, N: codes.gen_code_fft(c, T, 0.0, fs, N, N))
+    streams = {}
+    for name, st, fs, T, prns, gen, fc, nv in signals:
+        streams.setdefault(st, dict(fs=fs, Tmax=0.0, sats=[]))
+        streams[st]["Tmax"] = max(streams[st]["Tmax"], T)
+    sigs, truth = [], {}
+    for name, st, fs, T, prns, gen, fc, nv in signals:
+        if prn_limit:
+            prns = prns[:prn_limit]
+            nv = min(nv, len(prns))
+        cd = {p: sig_code(gen, p) for p in prns}
+        vis = sorted(rng.choice(prns, nv, replace=False).tolist())
+        sats = synth.random_sats(rng, vis, cd, T, fc, cn0, (-max_dop, max_dop))
+        streams[st]["sats"] += sats
+        truth[name] = sats
+        N = int(fs * T)
+        n_samp = int(fs * (t_int + streams[st]["Tmax"]))
+        n_blk = len(range(0, int(fs * (t_int + T)) - 2 * N + 1, N))
+        code_fft = np.stack([fn(cd[p], T, fs, N) for p in prns]).astype(np.complex64)
+        sigs.append(dict(name=name, buff=st, M=2 * N, N=N, T=T, fs=fs, fi=0.0, code_fft=code_fft,
+                         fds=dop_bins(T, 0.0, max_dop), n_blk=n_blk, ix0=0, blk_step=N, prns=prns))
+    buffs = []
+    for st in sorted(streams):
+        d = streams[st]
+        n_samp = int(d["fs"] * (t_int + d["Tmax"]))
+        buffs.append(synth.synth_if(d["fs"], n_samp, d["sats"], seed * 10 + st, bits=bits))
+    return dict(buffs=buffs, sigs=sigs, truth=truth)

@@ -223,3 +223,35 @@ def test_trk_plan_bitexact(sdr, orc, variant):
         assert np.array_equal(corr[c].view(np.uint32), rc.view(np.uint32)), c
         assert np.array_equal(Cc[c].view(np.uint32), rC.view(np.uint32)), c
     plan.close()
+
+
+@pytest.fixture(scope="module")
+def multi_acq():
+    from oracle import oracle as orc
+    from pocketsdr_amd import workloads
+    return workloads.acq_multi(seed=7, prn_limit=3, cn0=(44, 50),
+                               code_fft_fn=lambda c, T, fs, N: orc.gen_code_fft(c, T, 0.0, fs, N, N))
+
+
+def test_search_plan_multi(sdr, orc, multi_acq):
+    """Uneven batched search: L1CA (M 24000), E1B (96000), L5Q (48000) and
+    L6D (192000) in one plan, against the oracle's sdr_search_code."""
+    w = multi_acq
+    plan = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_GRAPH)
+    plan.run()
+    for i, s in enumerate(w["sigs"]):
+        cn0, ix, dop = plan.result(i)
+        for q in range(s["code_fft"].shape[0]):
+            R = np.zeros((len(s["fds"]), s["M"]), np.float32)
+            for k in range(s["n_blk"]):
+                orc.search_code(s["code_fft"][q], s["T"], w["buffs"][s["buff"]],
+                                s["ix0"] + k * s["blk_step"], s["fs"], s["fi"], s["fds"], R)
+            P = plan.power(i, q)
+            assert rel(P, R[:, :s["N"]]) < 1e-5, (s["name"], q)
+            rc, rix = orc.corr_max(R, s["M"], s["N"], len(s["fds"]), s["T"])
+            assert ix[q].tolist() == rix.tolist(), (s["name"], q)
+            assert abs(cn0[q] - rc) <= 1e-3 * abs(rc) + 1e-3
+        for sat in w["truth"][s["name"]]:
+            q = s["prns"].index(sat["prn"])
+            assert cn0[q] > 38.0, (s["name"], sat["prn"], cn0[q])
+    plan.close()
