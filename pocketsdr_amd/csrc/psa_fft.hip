@@ -154,21 +154,6 @@ __host__ __device__ constexpr RadPlan make_rad_plan(int L)
     return p;
 }
 
-// compile-time stage sequence (recursion over the plan)
-template <int CL, int CG, int IDX, int NS>
-__device__ __forceinline__ float2 *run_stages_ct(float2 *src, float2 *dst, const float2 *tw, float sgn)
-{
-    constexpr RadPlan P = make_rad_plan(CL);
-    if constexpr (IDX < P.n) {
-        constexpr int R = P.r[IDX];
-        stage<R>(src, dst, CL, CG, NS, tw, sgn);
-        __syncthreads();
-        return run_stages_ct<CL, CG, IDX + 1, NS * R>(dst, src, tw, sgn);
-    } else {
-        return src;
-    }
-}
-
 __device__ float2 *run_stages_rt(float2 *src, float2 *dst, int L, int G, const RadixSeq &rs,
                                  const float2 *tw, float sgn)
 {
@@ -190,131 +175,304 @@ __device__ float2 *run_stages_rt(float2 *src, float2 *dst, int L, int G, const R
 }
 
 // ---------------------------------------------------------------------------
-// the pass kernel: CL / CG > 0 select a compile-time sub-transform length and
-// group count, 0 the generic runtime path
+// element loaders / storers of the pass kernels
 // ---------------------------------------------------------------------------
-template <int LM, int SM, int CL, int CG>
-__global__ void __launch_bounds__(PASS_THREADS) pass_kernel(PassArgs a, RadixSeq rs, int ngroups)
-{
-    extern __shared__ float2 smem[];
-    __shared__ uint32_t carr[256];
-    const int L = CL > 0 ? CL : a.L;
-    const int G = CL > 0 ? CG : a.G;
-    float2 *buf0 = smem, *buf1 = smem + G * L;
-    const int b = (int)(blockIdx.x / (unsigned)ngroups);
-    const int grp = (int)blockIdx.x - b * ngroups;
-    const int g0 = grp * G;
-    const int gcount = min(G, a.nsub - g0);
-    const float sgn = a.dir == 0 ? -1.0f : 1.0f;
-
-    if (LM == LD_MIX) {
-        for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
-        __syncthreads();
-    }
-
-    // ---- load G sub-transforms (element (g, n) -> LDS[g * L + n]) ----
-    // element index within the batch item: e = (g0 + g) * in_gstride + n * in_nstride
-    const bool in_nfast = (a.in_nstride == 1);
-    const int tot = G * L;
-    const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
-    int ix = 0, nseg = 0;
-    uint32_t p0 = 0, p1 = 0, s = 0;
-    if (LM == LD_MIX) {
-        ix = a.mix_ix[b]; nseg = a.mix_nseg[b];
-        p0 = a.mix_p0[b]; p1 = a.mix_p1[b]; s = a.mix_s[b];
-    }
+template <int LM>
+struct Loader {
+    const float2 *pin = nullptr;
+    const char2 *qin = nullptr;
     const float2 *xin = nullptr, *cin = nullptr;
+    const uint8_t *buf = nullptr;
+    const uint32_t *carr = nullptr;     // LDS copy of the carrier table
+    int ix = 0, nseg = 0, compact = 0;
+    uint32_t p0 = 0, p1 = 0, s = 0;
     bool prod_tail = false;
-    if (LM == LD_PROD) {
-        const long long xb = (b / a.prod_xdiv) % a.prod_xmod;
-        const long long cb = b / a.prod_cdiv;
-        xin = a.in + xb * a.in_bstride;
-        cin = a.cf + cb * a.cf_bstride;
-        prod_tail = a.prod_V < a.M;
-    }
-    const float2 *pin = (LM == LD_CPX) ? a.in + (long long)b * a.in_bstride : nullptr;
-    const char2 *qin = (LM == LD_CPX16) ? reinterpret_cast<const char2 *>(a.iq16) + (long long)b * a.in_bstride : nullptr;
-    for (int idx = threadIdx.x; idx < tot; idx += PASS_THREADS) {
-        int g, n;
-        if (in_nfast) { g = idx / L; n = idx - g * L; }
-        else { n = idx / G; g = idx - n * G; }
-        float2 v = make_float2(0.f, 0.f);
-        if (g < gcount) {
-            const int e = (g0 + g) * igs + n * ins;
-            if (LM == LD_CPX) {
-                v = pin[e];
-            } else if (LM == LD_CPX16) {
-                const char2 q = qin[e];
-                v = make_float2((float)q.x * PSA_CSCALE, (float)q.y * PSA_CSCALE);
-            } else if (LM == LD_MIX) {
-                int src;
-                uint32_t ph;
-                if (e < nseg) { src = ix + e; ph = p0 + (uint32_t)e * s; }
-                else { src = a.mix_compact ? e : e - nseg; ph = p1 + (uint32_t)(e - nseg) * s; }
-                const uint32_t d = a.buf[src];
-                const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
-                const int sQ = (int)((int8_t)(uint8_t)d) >> 4;
-                const uint32_t c = carr[ph >> 24];
-                const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
-                const int8_t mI = (int8_t)(sI * cI - sQ * cQ);
-                const int8_t mQ = (int8_t)(sI * cQ + sQ * cI);
-                v = make_float2((float)mI * PSA_CSCALE, (float)mQ * PSA_CSCALE);
-            } else if (LM == LD_PROD) {
-                // sdr_cpx_mul AVX2 lanes (src/sdr_func.c:203-213):
-                // re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) + fl(ai br), * s;
-                // the last M % 4 natural indices take the scalar tail form
-                // (src/sdr_func.c:229-232, contracted by the reference build)
-                const float2 x = xin[e], c = cin[e];
-                bool tail = false;
-                if (prod_tail) tail = (e / a.prod_M2 + a.prod_M1 * (e % a.prod_M2)) >= a.prod_V;
-                float re, im;
-                if (!tail) {
-                    re = x.x * c.x + x.y * -c.y;
-                    im = x.x * c.y + x.y * c.x;
-                } else {
-                    re = __builtin_fmaf(x.x, c.x, -(x.y * c.y));
-                    im = __builtin_fmaf(x.x, c.y, x.y * c.x);
-                }
-                v = make_float2(re * a.prod_scale, im * a.prod_scale);
-            }
+    int prod_V = 0, prod_M1 = 1, prod_M2 = 1;
+    float prod_scale = 1.0f;
+
+    __device__ __forceinline__ void init(const PassArgs &a, int b, const uint32_t *carr_lds)
+    {
+        if (LM == LD_CPX) pin = a.in + (long long)b * a.in_bstride;
+        if (LM == LD_CPX16) qin = reinterpret_cast<const char2 *>(a.iq16) + (long long)b * a.in_bstride;
+        if (LM == LD_MIX) {
+            carr = carr_lds; buf = a.buf; compact = a.mix_compact;
+            ix = a.mix_ix[b]; nseg = a.mix_nseg[b];
+            p0 = a.mix_p0[b]; p1 = a.mix_p1[b]; s = a.mix_s[b];
         }
-        buf0[g * L + n] = v;
+        if (LM == LD_PROD) {
+            const long long xb = (b / a.prod_xdiv) % a.prod_xmod;
+            const long long cb = b / a.prod_cdiv;
+            xin = a.in + xb * a.in_bstride;
+            cin = a.cf + cb * a.cf_bstride;
+            prod_tail = a.prod_V < a.M;
+            prod_V = a.prod_V; prod_M1 = a.prod_M1; prod_M2 = a.prod_M2;
+            prod_scale = a.prod_scale;
+        }
     }
-    __syncthreads();
 
-    // ---- Stockham stages in LDS ----
-    float2 *res;
-    if constexpr (CL > 0) res = run_stages_ct<CL, CG, 0, 1>(buf0, buf1, a.tw, sgn);
-    else res = run_stages_rt(buf0, buf1, L, G, rs, a.tw, sgn);
+    // element e of the batch item (natural index for LD_MIX / LD_CPX16)
+    __device__ __forceinline__ float2 load(int e) const
+    {
+        if (LM == LD_CPX) return pin[e];
+        if (LM == LD_CPX16) {
+            const char2 q = qin[e];
+            return make_float2((float)q.x * PSA_CSCALE, (float)q.y * PSA_CSCALE);
+        }
+        if (LM == LD_MIX) {
+            int src;
+            uint32_t ph;
+            if (e < nseg) { src = ix + e; ph = p0 + (uint32_t)e * s; }
+            else { src = compact ? e : e - nseg; ph = p1 + (uint32_t)(e - nseg) * s; }
+            const uint32_t d = buf[src];
+            const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+            const int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+            const uint32_t c = carr[ph >> 24];
+            const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+            const int8_t mI = (int8_t)(sI * cI - sQ * cQ);
+            const int8_t mQ = (int8_t)(sI * cQ + sQ * cI);
+            return make_float2((float)mI * PSA_CSCALE, (float)mQ * PSA_CSCALE);
+        }
+        // LD_PROD: sdr_cpx_mul AVX2 lanes (src/sdr_func.c:203-213):
+        // re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) + fl(ai br), * s; the
+        // last M % 4 natural indices take the scalar tail form
+        // (src/sdr_func.c:229-232, contracted by the reference build)
+        const float2 x = xin[e], c = cin[e];
+        bool tail = false;
+        if (prod_tail) tail = (e / prod_M2 + prod_M1 * (e % prod_M2)) >= prod_V;
+        float re, im;
+        if (!tail) {
+            re = x.x * c.x + x.y * -c.y;
+            im = x.x * c.y + x.y * c.x;
+        } else {
+            re = __builtin_fmaf(x.x, c.x, -(x.y * c.y));
+            im = __builtin_fmaf(x.x, c.y, x.y * c.x);
+        }
+        return make_float2(re * prod_scale, im * prod_scale);
+    }
+};
 
-    // ---- store ----
-    const bool out_nfast = (a.out_nstride == 1);
-    const int ogs = (int)a.out_gstride, ons = (int)a.out_nstride;
-    float2 *pout = (SM == ST_CPX) ? a.out + (long long)b * a.out_bstride : nullptr;
+template <int SM>
+struct Storer {
+    float2 *pout = nullptr;
     float *prow = nullptr;
-    if (SM == ST_POW) {
-        const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
-        prow = a.P + row * a.p_bstride;
+    const float2 *thi = nullptr, *tlo = nullptr;
+    int nlag = 0, ogs = 0, ons = 0;
+
+    __device__ __forceinline__ void init(const PassArgs &a, int b)
+    {
+        thi = a.thi; tlo = a.tlo;
+        ogs = (int)a.out_gstride; ons = (int)a.out_nstride;
+        if (SM == ST_CPX) pout = a.out + (long long)b * a.out_bstride;
+        if (SM == ST_POW) {
+            const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
+            prow = a.P + row * a.p_bstride;
+            nlag = a.nlag;
+        }
     }
-    for (int idx = threadIdx.x; idx < tot; idx += PASS_THREADS) {
-        int g, k;
-        if (out_nfast) { g = idx / L; k = idx - g * L; }
-        else { k = idx / G; g = idx - k * G; }
-        if (g >= gcount) continue;
-        float2 v = res[g * L + k];
-        const int gg = g0 + g;
-        if (a.thi) {
+
+    // output k of global sub-transform gg
+    __device__ __forceinline__ void store(int gg, int k, float2 v) const
+    {
+        if (thi) {
             // w_M^{gg k}: gg < nsub and k < L with nsub * L = M, so gg k < M
             const int ti = gg * k;
-            v = cmul(v, cmul(a.thi[ti >> 10], a.tlo[ti & 1023]));
+            v = cmul(v, cmul(thi[ti >> 10], tlo[ti & 1023]));
         }
         const int e = gg * ogs + k * ons;
         if (SM == ST_CPX) {
             pout[e] = v;
-        } else {
-            if (e < a.nlag) {
-                // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680)
-                prow[e] = prow[e] + __builtin_fmaf(v.x, v.x, v.y * v.y);
+        } else if (e < nlag) {
+            // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680)
+            prow[e] = prow[e] + __builtin_fmaf(v.x, v.x, v.y * v.y);
+        }
+    }
+};
+
+template <int R>
+__device__ __forceinline__ void bfly(float2 *a, float sgn)
+{
+    if (R == 2) bfly2(a);
+    else if (R == 3) bfly3(a, sgn);
+    else if (R == 4) bfly4(a, sgn);
+    else if (R == 5) bfly5(a, sgn);
+    else if (R == 8) bfly8(a, sgn);
+}
+
+// ---------------------------------------------------------------------------
+// generic pass kernel (runtime length): LDS ping-pong for load, every stage
+// and store
+// ---------------------------------------------------------------------------
+template <int LM, int SM>
+__global__ void __launch_bounds__(PASS_THREADS) pass_kernel_rt(PassArgs a, RadixSeq rs, int ngroups)
+{
+    extern __shared__ float2 smem[];
+    __shared__ uint32_t carr[256];
+    const int L = a.L, G = a.G;
+    float2 *buf0 = smem, *buf1 = smem + G * L;
+    const int b = (int)(blockIdx.x / (unsigned)ngroups);
+    const int g0 = ((int)blockIdx.x - b * ngroups) * G;
+    const int gcount = min(G, a.nsub - g0);
+    const float sgn = a.dir == 0 ? -1.0f : 1.0f;
+    if (LM == LD_MIX) {
+        for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
+        __syncthreads();
+    }
+    Loader<LM> ld;
+    ld.init(a, b, carr);
+    const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
+    const bool in_nfast = (ins == 1);
+    const int tot = G * L;
+    for (int idx = threadIdx.x; idx < tot; idx += PASS_THREADS) {
+        int g, n;
+        if (in_nfast) { g = idx / L; n = idx - g * L; }
+        else { n = idx / G; g = idx - n * G; }
+        buf0[g * L + n] = g < gcount ? ld.load((g0 + g) * igs + n * ins) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 *res = run_stages_rt(buf0, buf1, L, G, rs, a.tw, sgn);
+    Storer<SM> st;
+    st.init(a, b);
+    const bool out_nfast = (a.out_nstride == 1);
+    for (int idx = threadIdx.x; idx < tot; idx += PASS_THREADS) {
+        int g, k;
+        if (out_nfast) { g = idx / L; k = idx - g * L; }
+        else { k = idx / G; g = idx - k * G; }
+        if (g < gcount) st.store(g0 + g, k, res[g * L + k]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// specialised pass kernel (compile-time length CL, CG sub-transforms per
+// workgroup).  The first radix stage runs in registers on elements loaded
+// straight from global memory; the middle stages run in place in one LDS
+// buffer (rows padded by one element to spread the banks); the last stage
+// reads LDS and stores straight to global memory.
+// ---------------------------------------------------------------------------
+template <int R, int CL, int CG, int NS>
+__device__ __forceinline__ void stage_inplace(float2 *buf, const float2 *__restrict__ tw, float sgn)
+{
+    constexpr int LP = CL + 1;
+    constexpr int Lr = CL / R;
+    constexpr int tstep = CL / (NS * R);
+    constexpr int NB = CG * Lr;
+    constexpr int PER = (NB + PASS_THREADS - 1) / PASS_THREADS;
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+        const int bi = threadIdx.x + p * PASS_THREADS;
+        if (bi < NB) {
+            const int g = bi / Lr, j = bi - g * Lr;
+            const int k = j % NS;
+#pragma unroll
+            for (int q = 0; q < R; q++) v[p][q] = buf[g * LP + j + q * Lr];
+#pragma unroll
+            for (int q = 1; q < R; q++) v[p][q] = cmul(v[p][q], tw[q * k * tstep]);
+            bfly<R>(v[p], sgn);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+        const int bi = threadIdx.x + p * PASS_THREADS;
+        if (bi < NB) {
+            const int g = bi / Lr, j = bi - g * Lr;
+            const int k = j % NS;
+            float2 *d = buf + g * LP + (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; q++) d[q * NS] = v[p][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int CL, int CG, int IDX, int NS>
+__device__ __forceinline__ void middle_stages(float2 *buf, const float2 *tw, float sgn)
+{
+    constexpr RadPlan P = make_rad_plan(CL);
+    if constexpr (IDX < P.n - 1) {
+        constexpr int R = P.r[IDX];
+        stage_inplace<R, CL, CG, NS>(buf, tw, sgn);
+        middle_stages<CL, CG, IDX + 1, NS * R>(buf, tw, sgn);
+    }
+}
+
+template <int LM, int SM, int CL, int CG>
+__global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int ngroups)
+{
+    constexpr RadPlan P = make_rad_plan(CL);
+    static_assert(P.n >= 2, "specialised lengths need at least two radix stages");
+    constexpr int LP = CL + 1;
+    constexpr int R0 = P.r[0];
+    constexpr int RL = P.r[P.n - 1];
+    constexpr int Lr0 = CL / R0, LrL = CL / RL;
+    __shared__ float2 buf[CG * LP];
+    __shared__ uint32_t carr[256];
+    const int b = (int)(blockIdx.x / (unsigned)ngroups);
+    const int g0 = ((int)blockIdx.x - b * ngroups) * CG;
+    const int gcount = min(CG, a.nsub - g0);
+    const float sgn = a.dir == 0 ? -1.0f : 1.0f;
+    if (LM == LD_MIX) {
+        for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
+        __syncthreads();
+    }
+    Loader<LM> ld;
+    ld.init(a, b, carr);
+    const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
+    const bool in_nfast = (ins == 1);
+
+    // ---- first stage (Ns = 1, no twiddles): global -> registers -> LDS ----
+    {
+        constexpr int NB = CG * Lr0;
+        constexpr int PER = (NB + PASS_THREADS - 1) / PASS_THREADS;
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int bi = threadIdx.x + p * PASS_THREADS;
+            if (bi < NB) {
+                int g, j;
+                if (in_nfast) { g = bi / Lr0; j = bi - g * Lr0; }
+                else { j = bi / CG; g = bi - j * CG; }
+                float2 v[R0];
+                if (g < gcount) {
+                    const int e0 = (g0 + g) * igs + j * ins;
+#pragma unroll
+                    for (int q = 0; q < R0; q++) v[q] = ld.load(e0 + q * Lr0 * ins);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < R0; q++) v[q] = make_float2(0.f, 0.f);
+                }
+                bfly<R0>(v, sgn);
+#pragma unroll
+                for (int q = 0; q < R0; q++) buf[g * LP + j * R0 + q] = v[q];
+            }
+        }
+        __syncthreads();
+    }
+    // ---- middle stages in place ----
+    middle_stages<CL, CG, 1, R0>(buf, a.tw, sgn);
+    // ---- last stage (Ns = L / RL, k = j): LDS -> registers -> global ----
+    {
+        Storer<SM> st;
+        st.init(a, b);
+        const bool out_nfast = (a.out_nstride == 1);
+        constexpr int NB = CG * LrL;
+        constexpr int PER = (NB + PASS_THREADS - 1) / PASS_THREADS;
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int bi = threadIdx.x + p * PASS_THREADS;
+            if (bi < NB) {
+                int g, j;
+                if (out_nfast) { g = bi / LrL; j = bi - g * LrL; }
+                else { j = bi / CG; g = bi - j * CG; }
+                if (g < gcount) {
+                    float2 v[RL];
+#pragma unroll
+                    for (int q = 0; q < RL; q++) v[q] = buf[g * LP + j + q * LrL];
+#pragma unroll
+                    for (int q = 1; q < RL; q++) v[q] = cmul(v[q], a.tw[q * j]);
+                    bfly<RL>(v, sgn);
+#pragma unroll
+                    for (int q = 0; q < RL; q++) st.store(g0 + g, j + q * LrL, v[q]);
+                }
             }
         }
     }
@@ -449,38 +607,35 @@ static void launch_t(const PassArgs &a, const RadixSeq &rs, int nbatch, hipStrea
     if ((long long)nbatch * a.M > 0x7FFFFFFFLL && (a.in_bstride == 0 || a.out_bstride == 0)) {
         PSA_FAIL("FFT launch too large");
     }
-    const size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
-    hipLaunchKernelGGL((pass_kernel<LM, SM, CL, CG>), dim3((unsigned)nblk), dim3(PASS_THREADS), shm, st,
-                       a, rs, ngroups);
+    if constexpr (CL > 0) {
+        hipLaunchKernelGGL((pass_kernel_ct<LM, SM, CL, CG>), dim3((unsigned)nblk), dim3(PASS_THREADS), 0, st,
+                           a, ngroups);
+    } else {
+        const size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
+        hipLaunchKernelGGL((pass_kernel_rt<LM, SM>), dim3((unsigned)nblk), dim3(PASS_THREADS), shm, st,
+                           a, rs, ngroups);
+    }
     PSA_CHECK(hipGetLastError());
 }
 
-template <int LM, int SM, int CL, int CG>
+template <int LM, int SM>
 static void set_attr()
 {
-    PSA_CHECK(hipFuncSetAttribute((const void *)pass_kernel<LM, SM, CL, CG>,
+    PSA_CHECK(hipFuncSetAttribute((const void *)pass_kernel_rt<LM, SM>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
 }
 
-template <int CL, int CG>
-static void set_attrs_l()
-{
-    set_attr<LD_CPX, ST_CPX, CL, CG>();
-    set_attr<LD_CPX16, ST_CPX, CL, CG>();
-    set_attr<LD_MIX, ST_CPX, CL, CG>();
-    set_attr<LD_PROD, ST_CPX, CL, CG>();
-    set_attr<LD_CPX, ST_POW, CL, CG>();
-    set_attr<LD_PROD, ST_POW, CL, CG>();
-}
-
+// the generic kernels take up to 64 KiB of dynamic LDS (two 4096-point buffers)
 void fft_init_attrs()
 {
     static std::once_flag once;
     std::call_once(once, [] {
-        set_attrs_l<0, 0>();
-#define PSA_A(L_, G_) set_attrs_l<L_, G_>();
-        PSA_SPECS(PSA_A)
-#undef PSA_A
+        set_attr<LD_CPX, ST_CPX>();
+        set_attr<LD_CPX16, ST_CPX>();
+        set_attr<LD_MIX, ST_CPX>();
+        set_attr<LD_PROD, ST_CPX>();
+        set_attr<LD_CPX, ST_POW>();
+        set_attr<LD_PROD, ST_POW>();
     });
 }
 

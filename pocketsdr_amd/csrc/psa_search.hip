@@ -55,67 +55,109 @@ __global__ void power_kernel(const float2 *__restrict__ C, int M, int nlag, long
 }
 
 // Peak search and C/N0 (src/sdr_func.c:690-706) plus fine Doppler
-// (src/sdr_func.c:728-744) over one PRN's surface P[len_fds][N].  The first
-// maximum in row-major order wins (strict '>'); values <= 0 never become the
-// peak.  The mean is an FP64 sum divided by the count.
-#define PEAK_THREADS 1024
-__global__ void __launch_bounds__(PEAK_THREADS) peak_kernel(const float *__restrict__ Pall, int N,
-                                                            int len_fds, double T,
-                                                            const float *__restrict__ fds,
-                                                            float *__restrict__ cn0,
-                                                            int *__restrict__ ixo,
-                                                            double *__restrict__ dop)
+// (src/sdr_func.c:728-744) over each PRN's surface P[len_fds][N], in two
+// stages: partial (max, first index, FP64 sum) per chunk of PEAK_CHUNK values,
+// then one workgroup per PRN combines its chunks.  The first maximum in
+// row-major order wins (strict '>'); values <= 0 never become the peak.  The
+// mean is an FP64 sum divided by the count.
+#define PEAK_THREADS 256
+#define PEAK_CHUNK 16384
+
+struct PeakPart {
+    float m;
+    int pad;
+    long long i;
+    double s;
+};
+
+__device__ __forceinline__ void peak_merge(float &bm, long long &bi, float om, long long oi)
 {
-    __shared__ float smax[PEAK_THREADS / 64];
-    __shared__ long long sidx[PEAK_THREADS / 64];
-    __shared__ double ssum[PEAK_THREADS / 64];
-    const int prn = blockIdx.x;
-    const float *P = Pall + (long long)prn * len_fds * N;
-    const long long tot = (long long)len_fds * N;
+    if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+}
+
+__device__ __forceinline__ void peak_block_reduce(float &bm, long long &bi, double &s, PeakPart *red)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(bm, o, 64);
+        const long long oi = __shfl_xor(bi, o, 64);
+        s += __shfl_xor(s, o, 64);
+        peak_merge(bm, bi, om, oi);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w].m = bm; red[w].i = bi; red[w].s = s; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < PEAK_THREADS / 64; i++) {
+            s += red[i].s;
+            peak_merge(bm, bi, red[i].m, red[i].i);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PEAK_THREADS) peak_part_kernel(const float *__restrict__ Pall,
+                                                                 long long tot, int nchunk,
+                                                                 PeakPart *__restrict__ part)
+{
+    __shared__ PeakPart red[PEAK_THREADS / 64];
+    const int prn = blockIdx.x / nchunk, c = blockIdx.x - prn * nchunk;
+    const float *P = Pall + (long long)prn * tot;
+    const long long i0 = (long long)c * PEAK_CHUNK;
+    const long long i1 = min(tot, i0 + PEAK_CHUNK);
     float bm = 0.0f;
     long long bi = -1;
     double s = 0.0;
-    for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
-        float v = P[i];
+    for (long long i = i0 + threadIdx.x; i < i1; i += PEAK_THREADS) {
+        const float v = P[i];
         s += (double)v;
         if (v > bm) { bm = v; bi = i; }
     }
-    // wave reduce: max value, smallest index on ties
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        float om = __shfl_xor(bm, o, 64);
-        long long oi = __shfl_xor(bi, o, 64);
-        double os = __shfl_xor(s, o, 64);
-        s += os;
-        if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { smax[w] = bm; sidx[w] = bi; ssum[w] = s; }
-    __syncthreads();
+    peak_block_reduce(bm, bi, s, red);
     if (threadIdx.x == 0) {
-        bm = smax[0]; bi = sidx[0]; s = ssum[0];
-        for (int i = 1; i < PEAK_THREADS / 64; i++) {
-            s += ssum[i];
-            long long oi = sidx[i];
-            float om = smax[i];
-            if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
-        }
-        double Pave = s / (double)tot;
-        double Pmax = bi >= 0 ? (double)bm : 0.0;
-        int i0 = bi >= 0 ? (int)(bi / N) : 0, i1 = bi >= 0 ? (int)(bi % N) : 0;
+        PeakPart q;
+        q.m = bm; q.pad = 0; q.i = bi; q.s = s;
+        part[blockIdx.x] = q;
+    }
+}
+
+__global__ void __launch_bounds__(PEAK_THREADS) peak_final_kernel(const float *__restrict__ Pall, int N,
+                                                                  int len_fds, double T, int nchunk,
+                                                                  const PeakPart *__restrict__ part,
+                                                                  const float *__restrict__ fds,
+                                                                  float *__restrict__ cn0,
+                                                                  int *__restrict__ ixo,
+                                                                  double *__restrict__ dop)
+{
+    __shared__ PeakPart red[PEAK_THREADS / 64];
+    const int prn = blockIdx.x;
+    const long long tot = (long long)len_fds * N;
+    const float *P = Pall + (long long)prn * tot;
+    float bm = 0.0f;
+    long long bi = -1;
+    double s = 0.0;
+    for (int c = threadIdx.x; c < nchunk; c += PEAK_THREADS) {
+        const PeakPart q = part[(long long)prn * nchunk + c];
+        s += q.s;
+        peak_merge(bm, bi, q.m, q.i);
+    }
+    peak_block_reduce(bm, bi, s, red);
+    if (threadIdx.x == 0) {
+        const double Pave = s / (double)tot;
+        const double Pmax = bi >= 0 ? (double)bm : 0.0;
+        const int i0 = bi >= 0 ? (int)(bi / N) : 0, i1 = bi >= 0 ? (int)(bi % N) : 0;
         cn0[prn] = Pave > 0.0 ? (float)(10.0 * log10((Pmax - Pave) / Pave / T)) : 0.0f;
         ixo[2 * prn] = i0;
         ixo[2 * prn + 1] = i1;
         double d = fds[i0];
         if (i0 > 0 && i0 < len_fds - 1) {
-            double x0 = fds[i0 - 1], x1 = fds[i0], x2 = fds[i0 + 1];
-            double y0 = P[(long long)(i0 - 1) * N + i1], y1 = P[(long long)i0 * N + i1];
-            double y2 = P[(long long)(i0 + 1) * N + i1];
-            double d01 = x0 - x1, d02 = x0 - x2, d12 = x1 - x2;
-            double den = d01 * d02 * d12;
+            const double x0 = fds[i0 - 1], x1 = fds[i0], x2 = fds[i0 + 1];
+            const double y0 = P[(long long)(i0 - 1) * N + i1], y1 = P[(long long)i0 * N + i1];
+            const double y2 = P[(long long)(i0 + 1) * N + i1];
+            const double d01 = x0 - x1, d02 = x0 - x2, d12 = x1 - x2;
+            const double den = d01 * d02 * d12;
             if (den != 0.0) {
-                double p2 = (x2 * (y1 - y0) + x1 * (y0 - y2) + x0 * (y2 - y1)) / den;
-                double p1 = (x2 * x2 * (y0 - y1) + x1 * x1 * (y2 - y0) + x0 * x0 * (y1 - y2)) / den;
+                const double p2 = (x2 * (y1 - y0) + x1 * (y0 - y2) + x0 * (y2 - y1)) / den;
+                const double p1 = (x2 * x2 * (y0 - y1) + x1 * x1 * (y2 - y0) + x0 * x0 * (y1 - y2)) / den;
                 if (p2 != 0.0) d = -p1 / (2.0 * p2);
             }
         }
@@ -434,6 +476,8 @@ struct SigDev {
     float *cn0 = nullptr;
     int *ix = nullptr;
     double *dop = nullptr;
+    PeakPart *part = nullptr;   // [n_prn x nchunk] partial peak reductions
+    int nchunk = 0;
     int bin_chunk = 1, prn_chunk = 1;
 };
 
@@ -627,8 +671,12 @@ static void run_all(sdr_search_plan_t *p)
         else run_batched(p, s);
         Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N,
                4.0 * s.h.n_prn * s.h.len_fds * s.h.N);
-        hipLaunchKernelGGL(peak_kernel, dim3(s.h.n_prn), dim3(PEAK_THREADS), 0, p->st, s.P, s.h.N,
-                           s.h.len_fds, s.h.T, s.fds, s.cn0, s.ix, s.dop);
+        const long long tot = (long long)s.h.len_fds * s.h.N;
+        hipLaunchKernelGGL(peak_part_kernel, dim3(s.h.n_prn * s.nchunk), dim3(PEAK_THREADS), 0, p->st,
+                           s.P, tot, s.nchunk, s.part);
+        PSA_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(peak_final_kernel, dim3(s.h.n_prn), dim3(PEAK_THREADS), 0, p->st, s.P, s.h.N,
+                           s.h.len_fds, s.h.T, s.nchunk, s.part, s.fds, s.cn0, s.ix, s.dop);
         PSA_CHECK(hipGetLastError());
     }
 }
@@ -671,7 +719,10 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         p->d_buf.push_back(d);
         p->buffN.push_back(buffs[i]->N);
     }
-    const size_t budget = (size_t)1 << 30;   // bytes per intermediate array
+    // bytes per intermediate array (the inverse passes' Z is the largest);
+    // PSA_CHUNK_MB overrides for tuning
+    size_t budget = (size_t)1 << 30;
+    if (const char *e = getenv("PSA_CHUNK_MB")) budget = (size_t)atol(e) << 20;
     p->sig.resize(n_sig);
     for (int i = 0; i < n_sig; i++) {
         SigDev &s = p->sig[i];
@@ -722,6 +773,8 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         PSA_CHECK(hipMalloc(&s.cn0, sizeof(float) * nprn));
         PSA_CHECK(hipMalloc(&s.ix, sizeof(int) * 2 * nprn));
         PSA_CHECK(hipMalloc(&s.dop, sizeof(double) * nprn));
+        s.nchunk = (int)(((long long)nbin * s.h.N + PEAK_CHUNK - 1) / PEAK_CHUNK);
+        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * s.nchunk));
         // chunking of the intermediates
         s.bin_chunk = (int)std::max<size_t>(1, std::min<size_t>(nbin, budget / spec));
         size_t pairs = std::max<size_t>(1, budget / spec);
@@ -837,7 +890,7 @@ void sdr_search_plan_free(sdr_search_plan_t *p)
         (void)hipFree(s.cft); (void)hipFree(s.cfn); (void)hipFree(s.P); (void)hipFree(s.fds);
         (void)hipFree(s.mix_ix); (void)hipFree(s.mix_nseg); (void)hipFree(s.mix_p0);
         (void)hipFree(s.mix_p1); (void)hipFree(s.mix_s);
-        (void)hipFree(s.cn0); (void)hipFree(s.ix); (void)hipFree(s.dop);
+        (void)hipFree(s.cn0); (void)hipFree(s.ix); (void)hipFree(s.dop); (void)hipFree(s.part);
     }
     (void)hipFree(p->tmp); (void)hipFree(p->X); (void)hipFree(p->Z); (void)hipFree(p->iq16);
     for (auto &e : p->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
