@@ -50,6 +50,27 @@ def kernel_stats(plan, names):
     return out
 
 
+SPEC_G = {160: 16, 150: 16, 240: 16, 200: 20, 320: 12, 300: 13, 480: 8, 400: 10, 1000: 4, 960: 4}
+MODES = {"fwd_col": (2, 0), "fwd_row": (0, 0), "inv_row": (3, 0), "inv_col": (0, 1)}
+
+
+def kernel_instance(cls, M):
+    """Name of the pass kernel a (class, FFT length) launches, as rocprof
+    lists it: the four-step split M = M1 * M2 (M1 the smallest divisor >= M2
+    with both <= 4096), column passes of length M1, row passes of length M2."""
+    if cls not in MODES:
+        return cls
+    M1, M2 = M, 1
+    if M > 4096:
+        M1 = min(d for d in range(1, M + 1) if M % d == 0 and d <= 4096 and M // d <= 4096 and d >= M // d)
+        M2 = M // M1
+    L = M1 if cls in ("fwd_col", "inv_col") else M2
+    lm, sm = MODES[cls]
+    if L in SPEC_G:
+        return "pass_kernel_ct<%d, %d, %d, %d>" % (lm, sm, L, SPEC_G[L])
+    return "pass_kernel_rt<%d, %d>" % (lm, sm)
+
+
 def build_workload(name, seed):
     from pocketsdr_amd import workloads
     if name == "l1ca":
@@ -87,16 +108,21 @@ def bench_acq(args, sdr, orc):
         plan.run()
     ks = {}
     for n in ["fwd_col", "fwd_row", "inv_row", "inv_col", "peak", "fill"]:
-        r = plan.kernel_ms(n)
-        if r:
-            ks[n] = dict(avg_ms=r[0], launches=r[1], elems=r[2], bytes=r[3])
+        for i, s in enumerate(sigs):
+            r = plan.kernel_ms("%s@%d" % (n, i))
+            if r:
+                ks["%s@%s" % (n, s["name"])] = dict(avg_ms=r[0], launches=r[1], elems=r[2], bytes=r[3],
+                                                    M=s["M"], cls=n)
     plan.close()
+    # dominant kernel: one instantiation (kernel class x signal group)
     dom = max(ks, key=lambda k: ks[k]["avg_ms"] * ks[k]["launches"])
-    gbs = ks[dom]["bytes"] / (ks[dom]["avg_ms"] * 1e-3) / 1e9
+    d = ks[dom]
+    gbs = d["bytes"] / (d["avg_ms"] * 1e-3) / 1e9
     res["roofline"] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                            frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
-                           alg_bytes_per_launch=round(ks[dom]["bytes"]),
-                           avg_launch_ms=round(ks[dom]["avg_ms"], 5))
+                           instantiation=kernel_instance(d["cls"], d["M"]),
+                           alg_bytes_per_launch=round(d["bytes"]),
+                           avg_launch_ms=round(d["avg_ms"], 5))
     res["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 5), launches=v["launches"],
                               gbs=round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1))
                       for k, v in ks.items()}
@@ -154,6 +180,21 @@ def bench_trk(args, sdr, workloads):
                 correlator_kernel_ms=round(ks[0], 4) if ks else None)
 
 
+def copy_probe(mb):
+    """A plain device-to-device copy of known size (torch's copy kernel), the
+    calibration of the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mb << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty(n, dtype=torch.uint8, device="cuda")
+    a.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(copy_probe_bytes=n, launches=3)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,7 +208,12 @@ def main():
     ap.add_argument("--cpu-prns", type=int, default=4)
     ap.add_argument("--workload", default="multi", choices=["multi", "l1ca"])
     ap.add_argument("--trk-epochs", type=int, default=1000)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a torch device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(--gpus %d is not supported)" % args.gpus, file=sys.stderr)

@@ -155,8 +155,9 @@ int sdr_search_plan_power(const sdr_search_plan_t *plan, int sig, int prn_idx,
 /* Enables per-kernel HIP-event timing of later runs (0 disables). */
 void sdr_search_plan_timing(sdr_search_plan_t *plan, int on);
 /* Average device time (ms) and launch count of kernel class `name`
- * ("fwd_col", "fwd_row", "inv_row", "inv_col", "mix", "cvt", "mul",
- * "peak") over the timed runs.  Returns 0 if the class never ran. */
+ * ("fwd_col", "fwd_row", "inv_row", "inv_col", "mix", "mul", "pow", "peak",
+ * "fill") over the timed runs; "name@g" restricts it to signal group g.
+ * Returns 0 if the class never ran. */
 int sdr_search_plan_kernel_ms(const sdr_search_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);
 /* Average algorithmic HBM bytes per launch of kernel class `name` over the

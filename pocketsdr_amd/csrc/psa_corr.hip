@@ -310,14 +310,18 @@ __global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int
 }
 
 // Fused mix + correlate: one workgroup per (channel, epoch).  The epoch's N
-// samples are carrier-mixed into LDS as separate int8 I and Q planes; each
-// tap then accumulates 4 samples per lane with v_dot4 against the code bank
-// (planar int8, 4 byte-shifted copies so every code word load is aligned).
-// EXACT selects _mm256_sign_epi8 semantics for IF buffers that hold -8
-// nibbles (the only source of -128 mixed values).
+// samples are carrier-mixed into LDS as separate int8 I and Q planes, then
+// each thread keeps its QPT words (4 samples each) of both planes in
+// registers for all taps.  Taps run in chunks of TRK_TC: per word one aligned
+// code-word load (4 byte-shifted copies of each bank) and v_dot4 products,
+// accumulated per half (before / after the code wrap at sample j), then wave
+// reductions into LDS.  EXACT selects _mm256_sign_epi8 semantics for IF
+// buffers that hold -8 nibbles (the only source of -128 mixed values); SAMEQ
+// (I = Q in every bank, true for real codes) loads one code plane.
 #define TRK_THREADS 256
+#define TRK_TC 4
 
-template <bool EXACT>
+template <bool EXACT, bool SAMEQ, int QPT>
 __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
     const TrkEpoch *__restrict__ ep, const int2 *__restrict__ jk,
@@ -335,12 +339,12 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const long long o = blockIdx.x;          // channel * n_epoch + epoch
     const int ch = (int)(o / n_epoch);
     const TrkEpoch q = ep[o];
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) carr[i] = carr_g[i];
-    for (int i = threadIdx.x; i < n_pos * 4; i += blockDim.x) (&sums[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
+    for (int i = threadIdx.x; i < n_pos * 4; i += TRK_THREADS) (&sums[0][0])[i] = 0;
     __syncthreads();
 
     // ---- mix N samples into the I / Q planes (zero padded to Nq words) ----
-    for (int w = threadIdx.x; w < Nq; w += blockDim.x) {
+    for (int w = threadIdx.x; w < Nq; w += TRK_THREADS) {
         uint32_t wi = 0, wq = 0;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
@@ -364,65 +368,91 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     }
     __syncthreads();
 
+    uint32_t dI[QPT], dQ[QPT];
+#pragma unroll
+    for (int i = 0; i < QPT; i++) {
+        const int w = threadIdx.x + i * TRK_THREADS;
+        dI[i] = w < Nq ? planeI[w] : 0u;
+        dQ[i] = w < Nq ? planeQ[w] : 0u;
+    }
     const uint32_t *cbase = codes + (long long)ch * code_ch_stride;   // [copy][bankI|bankQ][k][NP/4]
+    const uint8_t *cb = (const uint8_t *)cbase;
+    const int cstride = (int)code_copy_stride;
     const int lane = threadIdx.x & 63;
+    const int QOFF = PSA_N_CODES * NP;       // byte offset of the Q banks
     // ---- taps ----
-    for (int t = 0; t < n_pos; t++) {
-        const int2 g = jk[o * n_pos + t];
-        const int j = g.x, k = g.y;
-        int a1I = 0, a1Q = 0, a2I = 0, a2Q = 0;
-        for (int w = threadIdx.x; w < Nq; w += blockDim.x) {
-            const int m = w * 4;
-            const uint32_t dI = planeI[w], dQ = planeQ[w];
-            if (m + 3 < j || (m >= j && m + 3 < N)) {
-                // all 4 samples in one half: code index c0 .. c0 + 3
-                const int c0 = m >= j ? m - j : N - j + m;
-                const uint32_t *cp = cbase + (long long)(c0 & 3) * code_copy_stride;
-                const uint32_t cI = cp[((long long)k * NP + (c0 & ~3)) >> 2];
-                const uint32_t cQ = cp[((long long)(PSA_N_CODES + k) * NP + (c0 & ~3)) >> 2];
-                int sI, sQ;
-                if (EXACT) {
-                    sI = 0; sQ = 0;
+    for (int t0 = 0; t0 < n_pos; t0 += TRK_TC) {
+        int acc[TRK_TC][4];
+#pragma unroll
+        for (int tt = 0; tt < TRK_TC; tt++) {
+            acc[tt][0] = acc[tt][1] = acc[tt][2] = acc[tt][3] = 0;
+            const int t = t0 + tt;
+            if (t >= n_pos) continue;
+            const int2 g = jk[o * n_pos + t];
+            const int j = g.x, kb = g.y * NP;
+#pragma unroll
+            for (int i = 0; i < QPT; i++) {
+                const int w = threadIdx.x + i * TRK_THREADS;
+                if (w >= Nq) continue;
+                const int m = w * 4;
+                const bool h2 = m >= j;
+                const int c0 = h2 ? m - j : N - j + m;
+                const int lim = h2 ? N : j;
+                if (m + 3 < lim) {
+                    // all 4 samples in one half: code bytes c0 .. c0 + 3
+                    const uint32_t *cp = cbase + (c0 & 3) * cstride;
+                    const int wo = (kb + (c0 & ~3)) >> 2;
+                    const uint32_t cI = cp[wo];
+                    const uint32_t cQ = SAMEQ ? cI : cp[wo + (QOFF >> 2)];
+                    int sI, sQ;
+                    if (EXACT) {
+                        sI = 0; sQ = 0;
+#pragma unroll
+                        for (int b = 0; b < 4; b++) {
+                            sI += sgn8((int8_t)(dI[i] >> (8 * b)), (int8_t)(cI >> (8 * b)));
+                            sQ += sgn8((int8_t)(dQ[i] >> (8 * b)), (int8_t)(cQ >> (8 * b)));
+                        }
+                    } else {
+                        sI = __builtin_amdgcn_sdot4((int)dI[i], (int)cI, 0, false);
+                        sQ = __builtin_amdgcn_sdot4((int)dQ[i], (int)cQ, 0, false);
+                    }
+                    acc[tt][0] += h2 ? 0 : sI;
+                    acc[tt][1] += h2 ? 0 : sQ;
+                    acc[tt][2] += h2 ? sI : 0;
+                    acc[tt][3] += h2 ? sQ : 0;
+                } else {
+                    // straddles the code wrap at m = j or the end of the data
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
-                        sI += sgn8((int8_t)(dI >> (8 * b)), (int8_t)(cI >> (8 * b)));
-                        sQ += sgn8((int8_t)(dQ >> (8 * b)), (int8_t)(cQ >> (8 * b)));
+                        const int mm = m + b;
+                        if (mm >= N) break;
+                        const bool hh = mm >= j;
+                        const int ci = hh ? mm - j : N - j + mm;
+                        const int vI = (int8_t)cb[kb + ci];
+                        const int vQ = SAMEQ ? vI : (int8_t)cb[QOFF + kb + ci];
+                        const int xI = (int8_t)(dI[i] >> (8 * b)), xQ = (int8_t)(dQ[i] >> (8 * b));
+                        const int pI = EXACT ? sgn8(xI, vI) : xI * vI;
+                        const int pQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
+                        acc[tt][0] += hh ? 0 : pI;
+                        acc[tt][1] += hh ? 0 : pQ;
+                        acc[tt][2] += hh ? pI : 0;
+                        acc[tt][3] += hh ? pQ : 0;
                     }
-                } else {
-                    sI = __builtin_amdgcn_sdot4((int)dI, (int)cI, 0, false);
-                    sQ = __builtin_amdgcn_sdot4((int)dQ, (int)cQ, 0, false);
-                }
-                if (m >= j) { a2I += sI; a2Q += sQ; } else { a1I += sI; a1Q += sQ; }
-            } else {
-                // straddles the code wrap at m = j or the end of the data
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const int mm = m + b;
-                    if (mm >= N) break;
-                    const int ci = mm >= j ? mm - j : N - j + mm;
-                    const uint8_t *cb = (const uint8_t *)cbase;
-                    const int vI = (int8_t)cb[(long long)k * NP + ci];
-                    const int vQ = (int8_t)cb[(long long)(PSA_N_CODES + k) * NP + ci];
-                    const int xI = (int8_t)(dI >> (8 * b)), xQ = (int8_t)(dQ >> (8 * b));
-                    const int pI = EXACT ? sgn8(xI, vI) : xI * vI;
-                    const int pQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
-                    if (mm >= j) { a2I += pI; a2Q += pQ; } else { a1I += pI; a1Q += pQ; }
                 }
             }
         }
-        a1I = wave_sum(a1I); a1Q = wave_sum(a1Q);
-        a2I = wave_sum(a2I); a2Q = wave_sum(a2Q);
-        if (lane == 0) {
-            atomicAdd(&sums[t][0], a1I);
-            atomicAdd(&sums[t][1], a1Q);
-            atomicAdd(&sums[t][2], a2I);
-            atomicAdd(&sums[t][3], a2Q);
+#pragma unroll
+        for (int tt = 0; tt < TRK_TC; tt++) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int v = wave_sum(acc[tt][c]);
+                if (lane == 0 && t0 + tt < n_pos) atomicAdd(&sums[t0 + tt][c], v);
+            }
         }
     }
     __syncthreads();
 
     // ---- per-tap float epilogue: S_vec = S_total - tail products ----
-    const uint8_t *cb = (const uint8_t *)cbase;
     for (int t = threadIdx.x; t < n_pos; t += blockDim.x) {
         const int2 g = jk[o * n_pos + t];
         const int j = g.x, k = g.y;
@@ -484,9 +514,12 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
 
 using namespace psa;
 
+static int trk_qpt(int N);
+
 struct sdr_trk_plan_tag {
     int n_ch, n_pos, n_epoch, N, buffN;
-    bool exact;
+    bool exact, sameq;
+    int qpt;
     int NP;
     long long code_ch_stride, code_copy_stride;   // in uint32 words
     uint8_t *d_buf = nullptr;
@@ -587,6 +620,13 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
     sdr_trk_plan_t *p = new sdr_trk_plan_t();
     p->n_ch = n_ch; p->n_pos = n_pos; p->n_epoch = n_epoch; p->N = N; p->buffN = buff->N;
     p->NP = (N + 4 + 3) / 4 * 4;
+    p->qpt = trk_qpt(N);
+    p->sameq = true;
+    for (int c = 0; c < n_ch && p->sameq; c++) {
+        for (size_t i = 0; i < (size_t)N * PSA_N_CODES; i++) {
+            if (chs[c].code[i].I != chs[c].code[i].Q) { p->sameq = false; break; }
+        }
+    }
     // exact sign semantics are needed only if some sample has a -8 nibble
     p->exact = false;
     for (int i = 0; i < buff->N && !p->exact; i++) {
@@ -633,15 +673,54 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
     PSA_CHECK(hipMalloc(&p->d_corr, sizeof(float2) * ne * n_pos));
     PSA_CHECK(hipMalloc(&p->d_C, sizeof(float2) * ne * 2));
     for (auto &e : p->ev) PSA_CHECK(hipEventCreate(&e));
-    if (p->exact) {
-        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    } else {
-        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<false>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    }
     return p;
 }
+
+}  // extern "C"
+
+// QPT: words of each plane per thread, the smallest instantiated count that
+// covers ceil(N / 4) words with TRK_THREADS threads
+#define TRK_QPTS(X) X(4) X(8) X(16) X(24) X(32) X(48) X(64) X(80)
+
+template <bool EXACT, bool SAMEQ, int QPT>
+static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<EXACT, SAMEQ, QPT>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    });
+    hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ, QPT>), dim3(ne), dim3(TRK_THREADS), shm, p->st,
+                       p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_jk,
+                       p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
+                       dev_carrier(), p->d_corr, p->d_C);
+}
+
+static int trk_qpt(int N)
+{
+    const int need = ((N + 3) / 4 + TRK_THREADS - 1) / TRK_THREADS;
+#define PSA_Q(Q_) if (need <= Q_) return Q_;
+    TRK_QPTS(PSA_Q)
+#undef PSA_Q
+    return 0;
+}
+
+static void trk_launch(sdr_trk_plan_t *p, size_t shm, unsigned ne)
+{
+#define PSA_Q(Q_)                                                                        \
+    if (p->qpt == Q_) {                                                                  \
+        if (p->exact && p->sameq) trk_launch_t<true, true, Q_>(p, shm, ne);              \
+        else if (p->exact) trk_launch_t<true, false, Q_>(p, shm, ne);                    \
+        else if (p->sameq) trk_launch_t<false, true, Q_>(p, shm, ne);                    \
+        else trk_launch_t<false, false, Q_>(p, shm, ne);                                 \
+        return;                                                                          \
+    }
+    TRK_QPTS(PSA_Q)
+#undef PSA_Q
+    PSA_FAIL("tracking plan: no kernel for N=%d", p->N);
+}
+
+extern "C" {
 
 int sdr_trk_plan_run(sdr_trk_plan_t *p)
 {
@@ -655,17 +734,7 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
     PSA_CHECK(hipGetLastError());
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
     const size_t shm = sizeof(uint32_t) * 2 * (size_t)((p->N + 3) / 4);
-    if (p->exact) {
-        hipLaunchKernelGGL(trk_corr_kernel<true>, dim3((unsigned)ne), dim3(TRK_THREADS), shm, st,
-                           p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_jk,
-                           p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
-                           dev_carrier(), p->d_corr, p->d_C);
-    } else {
-        hipLaunchKernelGGL(trk_corr_kernel<false>, dim3((unsigned)ne), dim3(TRK_THREADS), shm, st,
-                           p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_jk,
-                           p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
-                           dev_carrier(), p->d_corr, p->d_C);
-    }
+    trk_launch(p, shm, (unsigned)ne);
     PSA_CHECK(hipGetLastError());
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[2], st));
     PSA_CHECK(hipStreamSynchronize(st));

@@ -145,6 +145,11 @@ struct PassArgs {
     float prod_scale;
     int prod_V;                 // natural indices >= prod_V use the scalar-tail form
     int prod_M1, prod_M2;       // transposed index e -> natural e / M2 + M1 (e % M2)
+    // PRN loop of the specialised LD_PROD kernel (prod_loop > 1): launch item
+    // b = pg * prod_nbin + bin runs PRNs pg * prod_loop + pq (pq < prod_loop,
+    // < prod_nprn) with the X rows kept in registers; output / spectrum item
+    // prn * prod_nbin + bin
+    int prod_loop, prod_nbin, prod_nprn;
     // ST_POW: P accumulate; batch b -> row (b / p_div + p_off1) * p_mul
     // + p_off2 + b % p_div, row start P + row * p_bstride
     float *P;
@@ -156,6 +161,7 @@ struct PassArgs {
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
                  hipStream_t st);
 int pass_groups(int L, int nsub);
+bool spec_length(int L);          // L has a compile-time specialised kernel
 // column pass: M2 transforms of length M1 at stride M2, output [k1][n2]
 // row-major, inter-pass twiddle w_M^{n2 k1} on store (when M2 > 1)
 void col_pass_args(const FFTPlan *pl, int dir, PassArgs &a);

@@ -477,6 +477,7 @@ struct SigDev {
     int *ix = nullptr;
     double *dop = nullptr;
     PeakPart *part = nullptr;   // [n_prn x nchunk] partial peak reductions
+    int prn_loop = 4;           // PRNs per inverse-row workgroup
     int nchunk = 0;
     int bin_chunk = 1, prn_chunk = 1;
 };
@@ -498,13 +499,13 @@ struct sdr_search_plan_tag {
     hipStream_t st = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool timing = false;
-    struct Ev { int cls; hipEvent_t a, b; double elems, bytes; };
+    struct Ev { int cls, sig; hipEvent_t a, b; double elems, bytes; };
+    int cur_sig = 0;
     std::vector<Ev> evs;
     size_t ev_used = 0;
-    double ms[psa::K_NCLS] = {0};
-    long long cnt[psa::K_NCLS] = {0};
-    double elems[psa::K_NCLS] = {0};
-    double bytes[psa::K_NCLS] = {0};
+    // per (kernel class, signal group) sums over the timed runs
+    struct Stat { double ms = 0, elems = 0, bytes = 0; long long cnt = 0; };
+    std::vector<Stat> stats;     // [K_NCLS * n_sig]
 };
 
 namespace psa {
@@ -527,6 +528,7 @@ struct Tick {
         }
         slot = p->ev_used++;
         p->evs[slot].cls = cls;
+        p->evs[slot].sig = p->cur_sig;
         p->evs[slot].elems = elems;
         p->evs[slot].bytes = bytes;
         PSA_CHECK(hipEventRecord(p->evs[slot].a, p->st));
@@ -580,9 +582,17 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
                 set_prod(pl, ir, 1.0f / M / M);
                 ir.out = p->Z; ir.out_bstride = M;
+                // PRN groups of up to PSA_PRN_LOOP share the X rows of a bin in
+                // registers (specialised lengths only)
+                int loop = spec_length(pl->M2) ? std::min(nqc, s.prn_loop) : 1;
+                int nlaunch = nbatch;
+                if (loop > 1) {
+                    ir.prod_loop = loop; ir.prod_nbin = nbc; ir.prod_nprn = nqc;
+                    nlaunch = nbc * ((nqc + loop - 1) / loop);
+                }
                 {
                     Tick t(p, K_INV_ROW, (double)nbatch * M, 8.0 * M * (nbc + nqc + (double)nbatch));
-                    launch_pass(LD_PROD, ST_CPX, ir, nbatch, st);
+                    launch_pass(LD_PROD, ST_CPX, ir, nlaunch, st);
                 }
                 PassArgs ic = {};
                 args_inv_col(pl, ic);
@@ -667,6 +677,7 @@ static void run_plain(sdr_search_plan_t *p, SigDev &s)
 static void run_all(sdr_search_plan_t *p)
 {
     for (auto &s : p->sig) {
+        p->cur_sig = (int)(&s - p->sig.data());
         if (p->flags & SDR_SEARCH_PLAIN) run_plain(p, s);
         else run_batched(p, s);
         Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N,
@@ -723,6 +734,8 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     // PSA_CHUNK_MB overrides for tuning
     size_t budget = (size_t)1 << 30;
     if (const char *e = getenv("PSA_CHUNK_MB")) budget = (size_t)atol(e) << 20;
+    int prn_loop = 4;
+    if (const char *e = getenv("PSA_PRN_LOOP")) prn_loop = std::max(1, atoi(e));
     p->sig.resize(n_sig);
     for (int i = 0; i < n_sig; i++) {
         SigDev &s = p->sig[i];
@@ -773,6 +786,7 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         PSA_CHECK(hipMalloc(&s.cn0, sizeof(float) * nprn));
         PSA_CHECK(hipMalloc(&s.ix, sizeof(int) * 2 * nprn));
         PSA_CHECK(hipMalloc(&s.dop, sizeof(double) * nprn));
+        s.prn_loop = prn_loop;
         s.nchunk = (int)(((long long)nbin * s.h.N + PEAK_CHUNK - 1) / PEAK_CHUNK);
         PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * s.nchunk));
         // chunking of the intermediates
@@ -812,13 +826,15 @@ int sdr_search_plan_run(sdr_search_plan_t *p)
     run_all(p);
     PSA_CHECK(hipStreamSynchronize(p->st));
     if (p->timing) {
+        if (p->stats.empty()) p->stats.resize((size_t)K_NCLS * p->sig.size());
         for (size_t i = 0; i < p->ev_used; i++) {
             float ms = 0;
             PSA_CHECK(hipEventElapsedTime(&ms, p->evs[i].a, p->evs[i].b));
-            p->ms[p->evs[i].cls] += ms;
-            p->cnt[p->evs[i].cls]++;
-            p->elems[p->evs[i].cls] += p->evs[i].elems;
-            p->bytes[p->evs[i].cls] += p->evs[i].bytes;
+            auto &q = p->stats[(size_t)p->evs[i].cls * p->sig.size() + p->evs[i].sig];
+            q.ms += ms;
+            q.cnt++;
+            q.elems += p->evs[i].elems;
+            q.bytes += p->evs[i].bytes;
         }
     }
     return 1;
@@ -849,35 +865,54 @@ void sdr_search_plan_timing(sdr_search_plan_t *p, int on)
 {
     if (!p) return;
     p->timing = on != 0;
-    for (int i = 0; i < K_NCLS; i++) { p->ms[i] = 0; p->cnt[i] = 0; p->elems[i] = 0; p->bytes[i] = 0; }
+    p->stats.assign((size_t)K_NCLS * p->sig.size(), sdr_search_plan_t::Stat());
+}
+
+// kernel class name, optionally "@<signal group>"; sums over the matching
+// (class, group) statistics
+static bool plan_stat(const sdr_search_plan_t *p, const char *name, sdr_search_plan_t::Stat *out)
+{
+    if (!p || !name || p->stats.empty()) return false;
+    std::string n(name);
+    int sig = -1;
+    size_t at = n.find('@');
+    if (at != std::string::npos) {
+        sig = atoi(n.c_str() + at + 1);
+        n = n.substr(0, at);
+    }
+    const size_t ns = p->sig.size();
+    for (int i = 0; i < K_NCLS; i++) {
+        if (n != k_names[i]) continue;
+        sdr_search_plan_t::Stat t;
+        for (size_t g = 0; g < ns; g++) {
+            if (sig >= 0 && (int)g != sig) continue;
+            const auto &q = p->stats[(size_t)i * ns + g];
+            t.ms += q.ms; t.cnt += q.cnt; t.elems += q.elems; t.bytes += q.bytes;
+        }
+        if (!t.cnt) return false;
+        *out = t;
+        return true;
+    }
+    return false;
 }
 
 int sdr_search_plan_kernel_bytes(const sdr_search_plan_t *p, const char *name, double *avg_bytes)
 {
-    if (!p || !name) return 0;
-    for (int i = 0; i < K_NCLS; i++) {
-        if (strcmp(name, k_names[i]) == 0 && p->cnt[i]) {
-            if (avg_bytes) *avg_bytes = p->bytes[i] / p->cnt[i];
-            return 1;
-        }
-    }
-    return 0;
+    sdr_search_plan_t::Stat t;
+    if (!plan_stat(p, name, &t)) return 0;
+    if (avg_bytes) *avg_bytes = t.bytes / t.cnt;
+    return 1;
 }
 
 int sdr_search_plan_kernel_ms(const sdr_search_plan_t *p, const char *name, double *avg_ms,
                               long long *count, double *elems)
 {
-    if (!p || !name) return 0;
-    for (int i = 0; i < K_NCLS; i++) {
-        if (strcmp(name, k_names[i]) == 0) {
-            if (!p->cnt[i]) return 0;
-            if (avg_ms) *avg_ms = p->ms[i] / p->cnt[i];
-            if (count) *count = p->cnt[i];
-            if (elems) *elems = p->elems[i] / p->cnt[i];
-            return 1;
-        }
-    }
-    return 0;
+    sdr_search_plan_t::Stat t;
+    if (!plan_stat(p, name, &t)) return 0;
+    if (avg_ms) *avg_ms = t.ms / t.cnt;
+    if (count) *count = t.cnt;
+    if (elems) *elems = t.elems / t.cnt;
+    return 1;
 }
 
 void sdr_search_plan_free(sdr_search_plan_t *p)

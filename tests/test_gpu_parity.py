@@ -205,11 +205,18 @@ def test_search_plan(sdr, orc, small_acq, flags):
     plan.close()
 
 
-@pytest.mark.parametrize("variant", ["plain", "exact", "wrap"])
+@pytest.mark.parametrize("variant", ["plain", "exact", "wrap", "iq", "long"])
 def test_trk_plan_bitexact(sdr, orc, variant):
     from pocketsdr_amd import workloads
-    w = workloads.trk_l1ca(seed=4, fs=4e6, n_ch=5, n_epoch=12, n_vis=2, nposx=8,
+    fs = 24e6 if variant == "long" else 4e6
+    w = workloads.trk_l1ca(seed=4, fs=fs, n_ch=5, n_epoch=12, n_vis=2, nposx=8,
                            buff_epochs=(7 if variant == "wrap" else None))
+    if variant == "iq":
+        # distinct I and Q code planes (the general bank layout)
+        rng = np.random.default_rng(9)
+        for ch in w["chans"]:
+            ch["code"] = ch["code"].copy()
+            ch["code"][:, 1] = rng.integers(-1, 2, len(ch["code"])).astype(np.int8)
     buff = w["buff"].copy()
     if variant == "wrap":
         buff = np.concatenate([buff, buff[:777]])        # ring length not a multiple of N
