@@ -318,7 +318,7 @@ __global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int
 // reductions into LDS.  EXACT selects _mm256_sign_epi8 semantics for IF
 // buffers that hold -8 nibbles (the only source of -128 mixed values); SAMEQ
 // (I = Q in every bank, true for real codes) loads one code plane.
-#define TRK_THREADS 256
+#define TRK_THREADS 1024
 #define TRK_TC 4
 
 template <bool EXACT, bool SAMEQ, int QPT>
@@ -680,7 +680,7 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
 
 // QPT: words of each plane per thread, the smallest instantiated count that
 // covers ceil(N / 4) words with TRK_THREADS threads
-#define TRK_QPTS(X) X(4) X(8) X(16) X(24) X(32) X(48) X(64) X(80)
+#define TRK_QPTS(X) X(2) X(4) X(6) X(8) X(12) X(16) X(20)
 
 template <bool EXACT, bool SAMEQ, int QPT>
 static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)

@@ -431,80 +431,6 @@ __device__ __forceinline__ void last_stage(const PassArgs &a, int b, int g0, int
     }
 }
 
-// LD_PROD with a PRN loop: the first-stage X operands stay in registers and
-// each PRN of the group re-runs the transform with its own code spectrum.
-template <int SM, int CL, int CG>
-__device__ void prod_prn_loop(const PassArgs &a, int b, int g0, int gcount, float2 *buf, float sgn)
-{
-    constexpr RadPlan P = make_rad_plan(CL);
-    constexpr int LP = CL + 1;
-    constexpr int R0 = P.r[0];
-    constexpr int Lr0 = CL / R0;
-    constexpr int NB = CG * Lr0;
-    constexpr int PER = (NB + PASS_THREADS - 1) / PASS_THREADS;
-    const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
-    const bool in_nfast = (ins == 1);
-    const int bin = b % a.prod_nbin, pg = b / a.prod_nbin;
-    const float2 *xin = a.in + (long long)bin * a.in_bstride;
-    const bool prod_tail = a.prod_V < a.M;
-    float2 xr[PER][R0];
-    int eb[PER];
-#pragma unroll
-    for (int p = 0; p < PER; p++) {
-        const int bi = threadIdx.x + p * PASS_THREADS;
-        int g, j;
-        if (in_nfast) { g = bi / Lr0; j = bi - g * Lr0; }
-        else { j = bi / CG; g = bi - j * CG; }
-        const bool ok = bi < NB && g < gcount;
-        eb[p] = ok ? (g0 + g) * igs + j * ins : -1;
-#pragma unroll
-        for (int q = 0; q < R0; q++) xr[p][q] = ok ? xin[eb[p] + q * Lr0 * ins] : make_float2(0.f, 0.f);
-    }
-    for (int pq = 0; pq < a.prod_loop; pq++) {
-        const int prn = pg * a.prod_loop + pq;
-        if (prn >= a.prod_nprn) break;
-        const int beff = prn * a.prod_nbin + bin;
-        const float2 *cin = a.cf + (long long)prn * a.cf_bstride;
-#pragma unroll
-        for (int p = 0; p < PER; p++) {
-            const int bi = threadIdx.x + p * PASS_THREADS;
-            if (bi < NB) {
-                int g, j;
-                if (in_nfast) { g = bi / Lr0; j = bi - g * Lr0; }
-                else { j = bi / CG; g = bi - j * CG; }
-                float2 v[R0];
-#pragma unroll
-                for (int q = 0; q < R0; q++) {
-                    if (eb[p] >= 0) {
-                        const int e = eb[p] + q * Lr0 * ins;
-                        const float2 x = xr[p][q], c = cin[e];
-                        bool tail = false;
-                        if (prod_tail) tail = (e / a.prod_M2 + a.prod_M1 * (e % a.prod_M2)) >= a.prod_V;
-                        float re, im;
-                        if (!tail) {
-                            re = x.x * c.x + x.y * -c.y;
-                            im = x.x * c.y + x.y * c.x;
-                        } else {
-                            re = __builtin_fmaf(x.x, c.x, -(x.y * c.y));
-                            im = __builtin_fmaf(x.x, c.y, x.y * c.x);
-                        }
-                        v[q] = make_float2(re * a.prod_scale, im * a.prod_scale);
-                    } else {
-                        v[q] = make_float2(0.f, 0.f);
-                    }
-                }
-                bfly<R0>(v, sgn);
-#pragma unroll
-                for (int q = 0; q < R0; q++) buf[g * LP + j * R0 + q] = v[q];
-            }
-        }
-        __syncthreads();
-        middle_stages<CL, CG, 1, R0>(buf, a.tw, sgn);
-        last_stage<SM, CL, CG>(a, beff, g0, gcount, buf, sgn);
-        __syncthreads();
-    }
-}
-
 template <int LM, int SM, int CL, int CG>
 __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int ngroups)
 {
@@ -529,12 +455,6 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
     const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
     const bool in_nfast = (ins == 1);
 
-    if constexpr (LM == LD_PROD) {
-        if (a.prod_loop > 1) {
-            prod_prn_loop<SM, CL, CG>(a, b, g0, gcount, buf, sgn);
-            return;
-        }
-    }
     // ---- first stage (Ns = 1, no twiddles): global -> registers -> LDS ----
     {
         constexpr int NB = CG * Lr0;

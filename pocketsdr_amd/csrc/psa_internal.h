@@ -145,11 +145,6 @@ struct PassArgs {
     float prod_scale;
     int prod_V;                 // natural indices >= prod_V use the scalar-tail form
     int prod_M1, prod_M2;       // transposed index e -> natural e / M2 + M1 (e % M2)
-    // PRN loop of the specialised LD_PROD kernel (prod_loop > 1): launch item
-    // b = pg * prod_nbin + bin runs PRNs pg * prod_loop + pq (pq < prod_loop,
-    // < prod_nprn) with the X rows kept in registers; output / spectrum item
-    // prn * prod_nbin + bin
-    int prod_loop, prod_nbin, prod_nprn;
     // ST_POW: P accumulate; batch b -> row (b / p_div + p_off1) * p_mul
     // + p_off2 + b % p_div, row start P + row * p_bstride
     float *P;

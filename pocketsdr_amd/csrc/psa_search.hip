@@ -477,7 +477,6 @@ struct SigDev {
     int *ix = nullptr;
     double *dop = nullptr;
     PeakPart *part = nullptr;   // [n_prn x nchunk] partial peak reductions
-    int prn_loop = 4;           // PRNs per inverse-row workgroup
     int nchunk = 0;
     int bin_chunk = 1, prn_chunk = 1;
 };
@@ -582,17 +581,9 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
                 set_prod(pl, ir, 1.0f / M / M);
                 ir.out = p->Z; ir.out_bstride = M;
-                // PRN groups of up to PSA_PRN_LOOP share the X rows of a bin in
-                // registers (specialised lengths only)
-                int loop = spec_length(pl->M2) ? std::min(nqc, s.prn_loop) : 1;
-                int nlaunch = nbatch;
-                if (loop > 1) {
-                    ir.prod_loop = loop; ir.prod_nbin = nbc; ir.prod_nprn = nqc;
-                    nlaunch = nbc * ((nqc + loop - 1) / loop);
-                }
                 {
                     Tick t(p, K_INV_ROW, (double)nbatch * M, 8.0 * M * (nbc + nqc + (double)nbatch));
-                    launch_pass(LD_PROD, ST_CPX, ir, nlaunch, st);
+                    launch_pass(LD_PROD, ST_CPX, ir, nbatch, st);
                 }
                 PassArgs ic = {};
                 args_inv_col(pl, ic);
@@ -734,8 +725,7 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     // PSA_CHUNK_MB overrides for tuning
     size_t budget = (size_t)1 << 30;
     if (const char *e = getenv("PSA_CHUNK_MB")) budget = (size_t)atol(e) << 20;
-    int prn_loop = 4;
-    if (const char *e = getenv("PSA_PRN_LOOP")) prn_loop = std::max(1, atoi(e));
+
     p->sig.resize(n_sig);
     for (int i = 0; i < n_sig; i++) {
         SigDev &s = p->sig[i];
@@ -786,7 +776,6 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         PSA_CHECK(hipMalloc(&s.cn0, sizeof(float) * nprn));
         PSA_CHECK(hipMalloc(&s.ix, sizeof(int) * 2 * nprn));
         PSA_CHECK(hipMalloc(&s.dop, sizeof(double) * nprn));
-        s.prn_loop = prn_loop;
         s.nchunk = (int)(((long long)nbin * s.h.N + PEAK_CHUNK - 1) / PEAK_CHUNK);
         PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * s.nchunk));
         // chunking of the intermediates
