@@ -50,19 +50,22 @@ def kernel_stats(plan, names):
     return out
 
 
-SPEC_G = {160: 16, 150: 16, 240: 16, 200: 20, 320: 12, 300: 13, 480: 8, 400: 10, 1000: 4, 960: 4}
+SPEC_G = {150: 16, 160: 16, 200: 16, 240: 16, 300: 16, 320: 12, 400: 16, 480: 8, 1000: 8, 960: 4}
 MODES = {"fwd_col": (2, 0), "fwd_row": (0, 0), "inv_row": (3, 0), "inv_col": (0, 1)}
 
 
 def kernel_instance(cls, M):
     """Name of the pass kernel a (class, FFT length) launches, as rocprof
-    lists it: the four-step split M = M1 * M2 (M1 the smallest divisor >= M2
-    with both <= 4096), column passes of length M1, row passes of length M2."""
+    lists it: the four-step split M = M1 * M2 (rows M2 a multiple of 16 when
+    possible, then the most balanced), column passes of length M1, row passes
+    of length M2."""
     if cls not in MODES:
         return cls
     M1, M2 = M, 1
     if M > 4096:
-        M1 = min(d for d in range(1, M + 1) if M % d == 0 and d <= 4096 and M // d <= 4096 and d >= M // d)
+        cands = [(0 if (M // d) % 16 == 0 else 1, abs(d - M // d), d) for d in range(1, 4097)
+                 if M % d == 0 and M // d <= 4096]
+        M1 = min(cands)[2]
         M2 = M // M1
     L = M1 if cls in ("fwd_col", "inv_col") else M2
     lm, sm = MODES[cls]

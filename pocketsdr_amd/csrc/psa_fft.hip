@@ -13,6 +13,7 @@
 // the first and last pass.
 #include <math.h>
 #include <map>
+#include <cstdlib>
 #include "psa_internal.h"
 
 namespace psa {
@@ -547,17 +548,22 @@ static void make_split_twiddle(int M, int sign, float2 **thi, float2 **tlo)
     PSA_CHECK(hipMemcpy(*tlo, lo.data(), sizeof(float2) * 1024, hipMemcpyHostToDevice));
 }
 
-// split M into M1 * M2 with both <= 4096, M1 >= M2, as balanced as possible
+// Split M into M1 * M2 (column length M1, row length M2), both <= 4096.
+// Rows of a multiple of 16 complex (128 bytes) keep the 16-wide column
+// segments of the column passes on cache-line boundaries; among those
+// splits the most balanced one wins (any split when none has such rows).
 static void split_len(int M, int *M1, int *M2)
 {
     int best1 = M, best2 = 1;
     if (M > 4096) {
         best1 = 0;
-        for (int d = 1; d <= M; d++) {
+        long long bestscore = 0;
+        for (int d = 1; d <= 4096 && d <= M; d++) {
             if (M % d) continue;
-            int e = M / d;
-            if (d > 4096 || e > 4096 || d < e) continue;
-            if (!best1 || d < best1) { best1 = d; best2 = e; }
+            const int e = M / d;          // d = M1, e = M2
+            if (e > 4096) continue;
+            const long long score = (e % 16 == 0 ? 0 : (1LL << 40)) + (long long)std::abs(d - e);
+            if (!best1 || score < bestscore) { best1 = d; best2 = e; bestscore = score; }
         }
     }
     *M1 = best1;
@@ -593,11 +599,13 @@ const FFTPlan *fft_plan(int M)
 }
 
 // (L, G) pairs with compile-time kernels: the sub-lengths of the plans for
-// M = 24000, 48000, 96000, 192000, 960000 (column pass G >= 8 keeps the
-// strided loads in >= 64-byte segments; G * L <= 4096 complex per buffer)
+// M = 24000 (150 x 160), 48000 (200 x 240), 96000 (300 x 320),
+// 192000 (400 x 480) and 960000 (1000 x 960).  Column lengths take G = 16
+// (one 128-byte segment per row; 8 for 1000 to fit LDS), row lengths keep
+// about 4096 complex per workgroup.
 #define PSA_SPECS(X) \
-    X(160, 16) X(150, 16) X(240, 16) X(200, 20) X(320, 12) X(300, 13) \
-    X(480, 8) X(400, 10) X(1000, 4) X(960, 4)
+    X(150, 16) X(160, 16) X(200, 16) X(240, 16) X(300, 16) X(320, 12) \
+    X(400, 16) X(480, 8) X(1000, 8) X(960, 4)
 
 bool spec_length(int L);
 
