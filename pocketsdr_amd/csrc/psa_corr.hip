@@ -381,6 +381,11 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const int lane = threadIdx.x & 63;
     const int QOFF = PSA_N_CODES * NP;       // byte offset of the Q banks
     // ---- taps ----
+    // For tap (j, k): half 1 pairs sample m < j with code byte N - j + m,
+    // half 2 pairs m >= j with m - j.  A full word (4 samples in one half)
+    // starting at m = 4w reads code bytes c0 .. c0 + 3: in half 2
+    // c0 = 4w - j, i.e. copy (-j) & 3, word w + (-j - copy) / 4; in half 1
+    // c0 = N - j + 4w, copy (N - j) & 3, word w + (N - j - copy) / 4.
     for (int t0 = 0; t0 < n_pos; t0 += TRK_TC) {
         int acc[TRK_TC][4];
 #pragma unroll
@@ -390,39 +395,38 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
             if (t >= n_pos) continue;
             const int2 g = jk[o * n_pos + t];
             const int j = g.x, kb = g.y * NP;
+            const int cp2 = (-j) & 3, cp1 = (N - j) & 3;
+            const uint32_t *p2 = cbase + cp2 * cstride + (kb >> 2) + (-j - cp2) / 4;
+            const uint32_t *p1 = cbase + cp1 * cstride + (kb >> 2) + (N - j - cp1) / 4;
+            const int wq = SAMEQ ? 0 : (QOFF >> 2);
 #pragma unroll
             for (int i = 0; i < QPT; i++) {
                 const int w = threadIdx.x + i * TRK_THREADS;
-                if (w >= Nq) continue;
                 const int m = w * 4;
                 const bool h2 = m >= j;
-                const int c0 = h2 ? m - j : N - j + m;
-                const int lim = h2 ? N : j;
-                if (m + 3 < lim) {
-                    // all 4 samples in one half: code bytes c0 .. c0 + 3
-                    const uint32_t *cp = cbase + (c0 & 3) * cstride;
-                    const int wo = (kb + (c0 & ~3)) >> 2;
-                    const uint32_t cI = cp[wo];
-                    const uint32_t cQ = SAMEQ ? cI : cp[wo + (QOFF >> 2)];
-                    int sI, sQ;
+                const bool full = w < Nq && m + 3 < (h2 ? N : j);
+                if (full) {
+                    const uint32_t *cp = h2 ? p2 : p1;
+                    const uint32_t cI = cp[w];
+                    const uint32_t cQ = SAMEQ ? cI : cp[w + wq];
                     if (EXACT) {
-                        sI = 0; sQ = 0;
+                        int sI = 0, sQ = 0;
 #pragma unroll
                         for (int b = 0; b < 4; b++) {
                             sI += sgn8((int8_t)(dI[i] >> (8 * b)), (int8_t)(cI >> (8 * b)));
                             sQ += sgn8((int8_t)(dQ[i] >> (8 * b)), (int8_t)(cQ >> (8 * b)));
                         }
+                        if (h2) { acc[tt][2] += sI; acc[tt][3] += sQ; }
+                        else { acc[tt][0] += sI; acc[tt][1] += sQ; }
+                    } else if (h2) {
+                        acc[tt][2] = __builtin_amdgcn_sdot4((int)dI[i], (int)cI, acc[tt][2], false);
+                        acc[tt][3] = __builtin_amdgcn_sdot4((int)dQ[i], (int)cQ, acc[tt][3], false);
                     } else {
-                        sI = __builtin_amdgcn_sdot4((int)dI[i], (int)cI, 0, false);
-                        sQ = __builtin_amdgcn_sdot4((int)dQ[i], (int)cQ, 0, false);
+                        acc[tt][0] = __builtin_amdgcn_sdot4((int)dI[i], (int)cI, acc[tt][0], false);
+                        acc[tt][1] = __builtin_amdgcn_sdot4((int)dQ[i], (int)cQ, acc[tt][1], false);
                     }
-                    acc[tt][0] += h2 ? 0 : sI;
-                    acc[tt][1] += h2 ? 0 : sQ;
-                    acc[tt][2] += h2 ? sI : 0;
-                    acc[tt][3] += h2 ? sQ : 0;
-                } else {
+                } else if (w < Nq) {
                     // straddles the code wrap at m = j or the end of the data
-#pragma unroll
                     for (int b = 0; b < 4; b++) {
                         const int mm = m + b;
                         if (mm >= N) break;
@@ -433,10 +437,8 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
                         const int xI = (int8_t)(dI[i] >> (8 * b)), xQ = (int8_t)(dQ[i] >> (8 * b));
                         const int pI = EXACT ? sgn8(xI, vI) : xI * vI;
                         const int pQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
-                        acc[tt][0] += hh ? 0 : pI;
-                        acc[tt][1] += hh ? 0 : pQ;
-                        acc[tt][2] += hh ? pI : 0;
-                        acc[tt][3] += hh ? pQ : 0;
+                        if (hh) { acc[tt][2] += pI; acc[tt][3] += pQ; }
+                        else { acc[tt][0] += pI; acc[tt][1] += pQ; }
                     }
                 }
             }

@@ -258,6 +258,11 @@ struct Storer {
     float *prow = nullptr;
     const float2 *thi = nullptr, *tlo = nullptr;
     int nlag = 0, ogs = 0, ons = 0;
+    bool pinit = false, track = false;
+    long long ibase = 0;            // row-major index of lag 0 of this row
+    float bm = 0.0f;                // peak tracking of the values stored
+    long long bi = -1;
+    double bs = 0.0;
 
     __device__ __forceinline__ void init(const PassArgs &a, int b)
     {
@@ -268,11 +273,14 @@ struct Storer {
             const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
             prow = a.P + row * a.p_bstride;
             nlag = a.nlag;
+            pinit = a.p_init != 0;
+            track = a.p_part != nullptr;
+            if (track) ibase = (row % a.p_nbin) * (long long)a.nlag;
         }
     }
 
     // output k of global sub-transform gg
-    __device__ __forceinline__ void store(int gg, int k, float2 v) const
+    __device__ __forceinline__ void store(int gg, int k, float2 v)
     {
         if (thi) {
             // w_M^{gg k}: gg < nsub and k < L with nsub * L = M, so gg k < M
@@ -283,8 +291,44 @@ struct Storer {
         if (SM == ST_CPX) {
             pout[e] = v;
         } else if (e < nlag) {
-            // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680)
-            prow[e] = prow[e] + __builtin_fmaf(v.x, v.x, v.y * v.y);
+            // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680);
+            // on the first block P is zero and 0 + x = x exactly
+            const float pw = __builtin_fmaf(v.x, v.x, v.y * v.y);
+            const float nv = pinit ? pw : prow[e] + pw;
+            prow[e] = nv;
+            if (track) {
+                bs += (double)nv;
+                const long long gi = ibase + e;
+                if (nv > bm || (nv == bm && bi >= 0 && gi < bi)) {
+                    if (nv > 0.0f) { bm = nv; bi = gi; }
+                }
+            }
+        }
+    }
+
+    // workgroup reduction of the tracked peak into part[slot]
+    __device__ __forceinline__ void flush_peak(PeakPart *part, long long slot, PeakPart *red)
+    {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const float om = __shfl_xor(bm, o, 64);
+            const long long oi = __shfl_xor(bi, o, 64);
+            bs += __shfl_xor(bs, o, 64);
+            if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+        }
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { red[w].m = bm; red[w].i = bi; red[w].s = bs; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < PASS_THREADS / 64; i++) {
+                bs += red[i].s;
+                const long long oi = red[i].i;
+                const float om = red[i].m;
+                if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+            }
+            PeakPart q;
+            q.m = bm; q.pad = 0; q.i = bi; q.s = bs;
+            part[slot] = q;
         }
     }
 };
@@ -399,15 +443,13 @@ __device__ __forceinline__ void middle_stages(float2 *buf, const float2 *tw, flo
 
 // last stage (Ns = L / RL, k = j): LDS -> registers -> global
 template <int SM, int CL, int CG>
-__device__ __forceinline__ void last_stage(const PassArgs &a, int b, int g0, int gcount,
-                                           const float2 *buf, float sgn)
+__device__ __forceinline__ void last_stage(const PassArgs &a, Storer<SM> &st, int g0, int gcount,
+                                           const float2 *buf, float sgn, const float2 *tw)
 {
     constexpr RadPlan P = make_rad_plan(CL);
     constexpr int LP = CL + 1;
     constexpr int RL = P.r[P.n - 1];
     constexpr int LrL = CL / RL;
-    Storer<SM> st;
-    st.init(a, b);
     const bool out_nfast = (a.out_nstride == 1);
     constexpr int NB = CG * LrL;
     constexpr int PER = (NB + PASS_THREADS - 1) / PASS_THREADS;
@@ -423,7 +465,7 @@ __device__ __forceinline__ void last_stage(const PassArgs &a, int b, int g0, int
 #pragma unroll
                 for (int q = 0; q < RL; q++) v[q] = buf[g * LP + j + q * LrL];
 #pragma unroll
-                for (int q = 1; q < RL; q++) v[q] = cmul(v[q], a.tw[q * j]);
+                for (int q = 1; q < RL; q++) v[q] = cmul(v[q], tw[q * j]);
                 bfly<RL>(v, sgn);
 #pragma unroll
                 for (int q = 0; q < RL; q++) st.store(g0 + g, j + q * LrL, v[q]);
@@ -432,30 +474,21 @@ __device__ __forceinline__ void last_stage(const PassArgs &a, int b, int g0, int
     }
 }
 
+// one item of the specialised pass: first stage from global, middle stages
+// in LDS, last stage to global
 template <int LM, int SM, int CL, int CG>
-__global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int ngroups)
+__device__ __forceinline__ void pass_item(const PassArgs &a, int b, int g0, int gcount, float2 *buf,
+                                          const uint32_t *carr, float sgn, PeakPart *red, long long pslot)
 {
+    const float2 *tw = a.tw;
     constexpr RadPlan P = make_rad_plan(CL);
-    static_assert(P.n >= 2, "specialised lengths need at least two radix stages");
     constexpr int LP = CL + 1;
     constexpr int R0 = P.r[0];
-    constexpr int RL = P.r[P.n - 1];
-    constexpr int Lr0 = CL / R0, LrL = CL / RL;
-    __shared__ float2 buf[CG * LP];
-    __shared__ uint32_t carr[256];
-    const int b = (int)(blockIdx.x / (unsigned)ngroups);
-    const int g0 = ((int)blockIdx.x - b * ngroups) * CG;
-    const int gcount = min(CG, a.nsub - g0);
-    const float sgn = a.dir == 0 ? -1.0f : 1.0f;
-    if (LM == LD_MIX) {
-        for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
-        __syncthreads();
-    }
+    constexpr int Lr0 = CL / R0;
     Loader<LM> ld;
     ld.init(a, b, carr);
     const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
     const bool in_nfast = (ins == 1);
-
     // ---- first stage (Ns = 1, no twiddles): global -> registers -> LDS ----
     {
         constexpr int NB = CG * Lr0;
@@ -484,9 +517,51 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
         __syncthreads();
     }
     // ---- middle stages in place ----
-    middle_stages<CL, CG, 1, R0>(buf, a.tw, sgn);
+    middle_stages<CL, CG, 1, R0>(buf, tw, sgn);
     // ---- last stage ----
-    last_stage<SM, CL, CG>(a, b, g0, gcount, buf, sgn);
+    Storer<SM> st;
+    st.init(a, b);
+    last_stage<SM, CL, CG>(a, st, g0, gcount, buf, sgn, tw);
+    if (SM == ST_POW && a.p_part) st.flush_peak(a.p_part, pslot, red);
+}
+
+template <int LM, int SM, int CL, int CG>
+__global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int ngroups)
+{
+    static_assert(make_rad_plan(CL).n >= 2, "specialised lengths need at least two radix stages");
+    __shared__ float2 buf[CG * (CL + 1)];
+    __shared__ uint32_t carr[256];
+    __shared__ PeakPart red[PASS_THREADS / 64];
+    int b, grp;
+    if (LM == LD_PROD && a.xcd_nbin > 0) {
+        // XCD-aware order: workgroups go round-robin to the 8 XCDs, so tile
+        // T = grp * nbin + bin keeps XCD T % 8 and its PRNs run back to back
+        // there, re-reading the shared X rows from that XCD's L2
+        const int w = (int)blockIdx.x;
+        const int r = w >> 3;
+        const int prn = r % a.xcd_nprn;
+        const int T = (r / a.xcd_nprn) * 8 + (w & 7);
+        if (T >= ngroups * a.xcd_nbin) return;
+        grp = T / a.xcd_nbin;
+        const int bin = T - grp * a.xcd_nbin;
+        b = prn * a.xcd_nbin + bin;
+    } else {
+        b = (int)(blockIdx.x / (unsigned)ngroups);
+        grp = (int)blockIdx.x - b * ngroups;
+    }
+    const int g0 = grp * CG;
+    const int gcount = min(CG, a.nsub - g0);
+    const float sgn = a.dir == 0 ? -1.0f : 1.0f;
+    if (LM == LD_MIX) {
+        for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
+        __syncthreads();
+    }
+    long long pslot = 0;
+    if (SM == ST_POW && a.p_part) {
+        const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
+        pslot = row * ngroups + grp;
+    }
+    pass_item<LM, SM, CL, CG>(a, b, g0, gcount, buf, carr, sgn, red, pslot);
 }
 
 // ---------------------------------------------------------------------------
@@ -628,7 +703,13 @@ static void launch_t(const PassArgs &a, const RadixSeq &rs, int nbatch, hipStrea
         PSA_FAIL("FFT launch too large");
     }
     if constexpr (CL > 0) {
-        hipLaunchKernelGGL((pass_kernel_ct<LM, SM, CL, CG>), dim3((unsigned)nblk), dim3(PASS_THREADS), 0, st,
+        long long nb = nblk;
+        if (LM == LD_PROD && a.xcd_nbin > 0) {
+            // tiles padded to a multiple of 8 (one per XCD)
+            const long long ntile = (long long)ngroups * a.xcd_nbin;
+            nb = (ntile + 7) / 8 * 8 * a.xcd_nprn;
+        }
+        hipLaunchKernelGGL((pass_kernel_ct<LM, SM, CL, CG>), dim3((unsigned)nb), dim3(PASS_THREADS), 0, st,
                            a, ngroups);
     } else {
         const size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
@@ -696,6 +777,11 @@ bool spec_length(int L)
 
 // G sub-transforms per workgroup: the specialised count where one exists,
 // else as many as fit 4096 complex per LDS buffer
+int pass_ngroups(const PassArgs &a)
+{
+    return (a.nsub + a.G - 1) / a.G;
+}
+
 int pass_groups(int L, int nsub)
 {
     int G = spec_groups(L);

@@ -94,6 +94,15 @@ bool fft_supported(int M);
 const FFTPlan *fft_plan(int M);   // cached, thread-safe; nullptr if unsupported
 void fft_init_attrs();            // LDS limits of the pass kernels (once)
 
+// partial peak reduction record: max value, its first row-major index within
+// the PRN's surface (-1: none > 0), FP64 sum of the values
+struct PeakPart {
+    float m;
+    int pad;
+    long long i;
+    double s;
+};
+
 // load / store modes of the sub-transform passes
 enum LoadMode {
     LD_CPX = 0,      // float2 from global
@@ -151,10 +160,19 @@ struct PassArgs {
     long long p_bstride;
     int p_div, p_off1, p_mul, p_off2;
     int nlag;               // only lags < nlag are accumulated
+    int p_init;             // 1: first block, P = |x|^2 (P starts at zero)
+    PeakPart *p_part;       // last block: one partial per workgroup at
+                            // part[row * ngroups + group] (nullptr: none)
+    int p_nbin;             // rows per PRN (row-major index (row % p_nbin) N + e)
+    // LD_PROD XCD-aware order (xcd_nbin > 0, specialised lengths): items
+    // b = prn * xcd_nbin + bin, prn < xcd_nprn; the PRNs of one (row group,
+    // bin) tile run back to back on one XCD
+    int xcd_nbin, xcd_nprn;
 };
 
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
                  hipStream_t st);
+int pass_ngroups(const PassArgs &a);   // workgroups per batch item
 int pass_groups(int L, int nsub);
 bool spec_length(int L);          // L has a compile-time specialised kernel
 // column pass: M2 transforms of length M1 at stride M2, output [k1][n2]

@@ -63,13 +63,6 @@ __global__ void power_kernel(const float2 *__restrict__ C, int M, int nlag, long
 #define PEAK_THREADS 256
 #define PEAK_CHUNK 16384
 
-struct PeakPart {
-    float m;
-    int pad;
-    long long i;
-    double s;
-};
-
 __device__ __forceinline__ void peak_merge(float &bm, long long &bi, float om, long long oi)
 {
     if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
@@ -477,7 +470,10 @@ struct SigDev {
     int *ix = nullptr;
     double *dop = nullptr;
     PeakPart *part = nullptr;   // [n_prn x nchunk] partial peak reductions
-    int nchunk = 0;
+    int nchunk = 0;             // partials per PRN of the separate peak pass
+    int nchunk_fused = 0;       // partials per PRN emitted by the last column pass
+    bool fused_peak = false;    // the last column pass emits the peak partials
+    bool xcd_order = true;      // XCD-aware workgroup order of the inverse row pass
     int bin_chunk = 1, prn_chunk = 1;
 };
 
@@ -546,10 +542,7 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
     const int M = pl->M, N = s.h.N, nbin = s.h.len_fds, nprn = s.h.n_prn;
     const uint8_t *buf = p->d_buf[s.h.buff];
     const int buffN = p->buffN[s.h.buff];
-    {
-        Tick t(p, K_FILL, (double)nprn * nbin * N);
-        fill(s.P, (long long)nprn * nbin * N, 0.0f, st);
-    }
+    // P is not cleared: the first block stores it, later blocks accumulate
     for (int k = 0; k < s.h.n_blk; k++) {
         for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
             const int nbc = std::min(s.bin_chunk, nbin - b0);
@@ -581,6 +574,10 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
                 set_prod(pl, ir, 1.0f / M / M);
                 ir.out = p->Z; ir.out_bstride = M;
+                if (s.xcd_order && spec_length(pl->M2)) {
+                    ir.xcd_nbin = nbc;
+                    ir.xcd_nprn = nqc;
+                }
                 {
                     Tick t(p, K_INV_ROW, (double)nbatch * M, 8.0 * M * (nbc + nqc + (double)nbatch));
                     launch_pass(LD_PROD, ST_CPX, ir, nbatch, st);
@@ -591,8 +588,13 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 ic.P = s.P; ic.p_bstride = N;
                 ic.p_div = nbc; ic.p_off1 = q0; ic.p_mul = nbin; ic.p_off2 = b0;
                 ic.nlag = N;
+                ic.p_nbin = nbin;
+                ic.p_init = k == 0;
+                if (k == s.h.n_blk - 1 && s.fused_peak) ic.p_part = s.part;
                 {
-                    Tick t(p, K_INV_COL, (double)nbatch * M, (double)nbatch * (8.0 * M + 8.0 * N));
+                    // P: written on the first block, read and written after
+                    Tick t(p, K_INV_COL, (double)nbatch * M,
+                           (double)nbatch * (8.0 * M + (k == 0 ? 4.0 : 8.0) * N));
                     launch_pass(LD_CPX, ST_POW, ic, nbatch, st);
                 }
             }
@@ -674,11 +676,15 @@ static void run_all(sdr_search_plan_t *p)
         Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N,
                4.0 * s.h.n_prn * s.h.len_fds * s.h.N);
         const long long tot = (long long)s.h.len_fds * s.h.N;
-        hipLaunchKernelGGL(peak_part_kernel, dim3(s.h.n_prn * s.nchunk), dim3(PEAK_THREADS), 0, p->st,
-                           s.P, tot, s.nchunk, s.part);
-        PSA_CHECK(hipGetLastError());
+        const bool fused = s.fused_peak && !(p->flags & SDR_SEARCH_PLAIN);
+        if (!fused) {
+            hipLaunchKernelGGL(peak_part_kernel, dim3(s.h.n_prn * s.nchunk), dim3(PEAK_THREADS), 0, p->st,
+                               s.P, tot, s.nchunk, s.part);
+            PSA_CHECK(hipGetLastError());
+        }
+        const int nch = fused ? s.nchunk_fused : s.nchunk;
         hipLaunchKernelGGL(peak_final_kernel, dim3(s.h.n_prn), dim3(PEAK_THREADS), 0, p->st, s.P, s.h.N,
-                           s.h.len_fds, s.h.T, s.nchunk, s.part, s.fds, s.cn0, s.ix, s.dop);
+                           s.h.len_fds, s.h.T, nch, s.part, s.fds, s.cn0, s.ix, s.dop);
         PSA_CHECK(hipGetLastError());
     }
 }
@@ -725,6 +731,7 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     // PSA_CHUNK_MB overrides for tuning
     size_t budget = (size_t)1 << 30;
     if (const char *e = getenv("PSA_CHUNK_MB")) budget = (size_t)atol(e) << 20;
+    const bool xcd_order = !getenv("PSA_NO_XCD_ORDER");
 
     p->sig.resize(n_sig);
     for (int i = 0; i < n_sig; i++) {
@@ -777,7 +784,16 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         PSA_CHECK(hipMalloc(&s.ix, sizeof(int) * 2 * nprn));
         PSA_CHECK(hipMalloc(&s.dop, sizeof(double) * nprn));
         s.nchunk = (int)(((long long)nbin * s.h.N + PEAK_CHUNK - 1) / PEAK_CHUNK);
-        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * s.nchunk));
+        // the last column pass emits one partial per workgroup when its
+        // length has a specialised kernel
+        s.fused_peak = spec_length(s.pl->M1) && !(flags & SDR_SEARCH_PLAIN) && !getenv("PSA_NO_FUSED_PEAK");
+        {
+            PassArgs tmp = {};
+            col_pass_args(s.pl, 1, tmp);
+            s.nchunk_fused = nbin * pass_ngroups(tmp);
+        }
+        s.xcd_order = xcd_order;
+        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused)));
         // chunking of the intermediates
         s.bin_chunk = (int)std::max<size_t>(1, std::min<size_t>(nbin, budget / spec));
         size_t pairs = std::max<size_t>(1, budget / spec);
