@@ -1,17 +1,22 @@
 """Benchmark of the MI355X search / correlate hot path (one GPU).
 
-Default workload: GPS L1 C/A acquisition as pocket_acq runs it
-(app/pocket_acq/pocket_acq.c:152-192): cpx8 IQ at 12 Msps, 1 ms coherent
-(N = 12000, FFT length M = 24000), 10 non-coherent blocks, 21 Doppler bins
-(+/-5 kHz), 32 PRNs, seeded synthetic IF with 8 visible satellites.  A "step"
-is the whole search (all 672 cells) with the IF and code spectra resident in
-HBM.  value = cells / s.
+Default workload: the batched uneven search of four signal groups in one
+plan (GPS L1 C/A 1 ms @ 12 Msps, Galileo E1-B 4 ms @ 12 Msps, GPS L5-Q 1 ms
+@ 24 Msps, QZSS L6D 4 ms @ 24 Msps; all PRNs of each system; Doppler bins of
+1/(2T) over +/-5 kHz; T_AQC = 10 ms of data per stream as pocket_acq reads it,
+app/pocket_acq/pocket_acq.c:152-192; 5070 search cells), seeded synthetic
+2-bit IF.  A "step" is the whole search with the IF and code spectra resident
+in HBM, replayed as one hipGraph.  value = search cells / s of the precision
+form (FP16 words, MFMA DFT-matrix stages, FP32 accumulation).
 
-Also reported on the same line: the plain FP32 per-PRN GPU path (reference
-dataflow, one search_code per PRN and block, unfused), the tracking
+Also on the same line: the FP32 batched path and the plain FP32 per-PRN GPU
+path (the reference dataflow: one search_code per PRN and block, unfused) on
+the same workload, the acquisition decisions of each form, the tracking
 correlators (64 channels at 24 Msps, 44 taps, Msps*ch), the roofline of the
-dominant kernel from HIP events, and the CPU baseline (the oracle restatement
-of sdr_search_code with the reference PocketFFT, single thread).
+dominant kernel from HIP events, and the CPU baseline (the oracle
+restatement of sdr_search_code with the reference PocketFFT, one thread),
+whose sample also gives the peak-power error of the timed form against the
+FP64 oracle.
 """
 import argparse
 import json
@@ -41,26 +46,33 @@ def time_plan(plan, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def kernel_stats(plan, names):
-    out = {}
-    for n in names:
-        r = plan.kernel_ms(n)
-        if r:
-            out[n] = dict(avg_ms=r[0], launches=r[1], elems=r[2])
-    return out
-
-
 SPEC_G = {150: 16, 160: 16, 200: 16, 240: 16, 300: 16, 320: 12, 400: 16, 480: 8, 1000: 8, 960: 4}
 MODES = {"fwd_col": (2, 0), "fwd_row": (0, 0), "inv_row": (3, 0), "inv_col": (0, 1)}
 
 
-def kernel_instance(cls, M):
-    """Name of the pass kernel a (class, FFT length) launches, as rocprof
-    lists it: the four-step split M = M1 * M2 (rows M2 a multiple of 16 when
-    possible, then the most balanced), column passes of length M1, row passes
-    of length M2."""
+# precision-form passes: (L, R0, R1, R2, G) of the column and row lengths
+HCOL = {150: (10, 15, 1, 32), 200: (10, 20, 1, 32), 300: (15, 20, 1, 32), 400: (16, 25, 1, 32),
+        960: (32, 30, 1, 16)}
+HROW = {160: (16, 10, 1, 32), 240: (16, 15, 1, 32), 320: (16, 20, 1, 16), 480: (16, 30, 1, 16),
+        1000: (10, 10, 10, 4)}
+
+
+def kernel_instance(cls, M, form="fp32"):
+    """Name of the pass kernel a (class, FFT length, form) launches, as
+    rocprof lists it.  FP32: the four-step split M = M1 * M2 (rows M2 a
+    multiple of 16 when possible, then the most balanced), column passes of
+    length M1, row passes of length M2.  FP16 / BF16: the MFMA split."""
     if cls not in MODES:
         return cls
+    if form in ("fp16", "bf16"):
+        for L1, (a, b, c, g) in HCOL.items():
+            if M % L1 == 0 and M // L1 in HROW:
+                col = cls in ("fwd_col", "inv_col")
+                L = L1 if col else M // L1
+                r0, r1, r2, G = HCOL[L] if col else HROW[L]
+                lm, sm = MODES[cls]
+                return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s>" % (
+                    lm, sm, 0 if form == "fp16" else 1, L, r0, r1, r2, G, "false" if col else "true")
     M1, M2 = M, 1
     if M > 4096:
         cands = [(0 if (M // d) % 16 == 0 else 1, abs(d - M // d), d) for d in range(1, 4097)
@@ -88,23 +100,50 @@ def build_workload(name, seed):
     raise SystemExit("unknown workload " + name)
 
 
+FORM_FLAGS = {"fp32": 0, "fp16": 4, "bf16": 8}
+FORM_DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
+
+
+def run_form(sdr, buffs, sigs, form, graph, steps, warmup):
+    """Builds the batched plan of one arithmetic form and times it; returns
+    (plan, s/step, per-signal (cn0, ix, dop))."""
+    from pocketsdr_amd import sdr_func
+    flags = FORM_FLAGS[form] | (sdr_func.SEARCH_GRAPH if graph else 0)
+    plan = sdr.SearchPlan(buffs, sigs, flags)
+    t = time_plan(plan, steps, warmup)
+    res = [plan.result(i) for i in range(len(sigs))]
+    return plan, t, res
+
+
+def decisions(sigs, res, truth):
+    det = {}
+    for s, (cn0, ix, dop) in zip(sigs, res):
+        found = sorted(s["prns"][q] for q in range(len(s["prns"])) if cn0[q] >= 38.0)
+        det[s["name"]] = dict(found=found, truth=sorted(t["prn"] for t in truth[s["name"]]))
+    return det
+
+
+def same_decisions(sigs, ra, rb):
+    """Same detections, and the same (bin, lag) for every detection."""
+    for s, (ca, ia, _), (cb, ib, _) in zip(sigs, ra, rb):
+        da, db = ca >= 38.0, cb >= 38.0
+        if not np.array_equal(da, db) or not np.array_equal(ia[da], ib[da]):
+            return False
+    return True
+
+
 def bench_acq(args, sdr, orc):
     from pocketsdr_amd import sdr_func
     buffs, sigs, truth, wname = build_workload(args.workload, args.seed)
     cells = sum(s["code_fft"].shape[0] * len(s["fds"]) for s in sigs)
     res = {}
-    flags = sdr_func.SEARCH_BATCHED | (sdr_func.SEARCH_GRAPH if args.graph else 0)
-    plan = sdr.SearchPlan(buffs, sigs, flags)
-    log("batched plan built (%d cells); warmup %d steps %d" % (cells, args.warmup, args.steps))
-    t_b = time_plan(plan, args.steps, args.warmup)
-    det = {}
-    idx_b = []
-    for i, s in enumerate(sigs):
-        cn0, ix, dop = plan.result(i)
-        idx_b.append(ix.copy())
-        found = sorted(s["prns"][q] for q in range(len(s["prns"])) if cn0[q] >= 38.0)
-        det[s["name"]] = dict(found=found, truth=sorted(t["prn"] for t in truth[s["name"]]))
-    log("batched: %.3f ms/step, detections %s" % (t_b * 1e3, det))
+    form = args.form
+    plan, t_b, r_main = run_form(sdr, buffs, sigs, form, args.graph, args.steps, args.warmup)
+    log("%s batched: %.3f ms/step (%d cells)" % (form, t_b * 1e3, cells))
+    forms = [sdr_func.FORM_FP32, sdr_func.FORM_PLAIN, sdr_func.FORM_FP16, sdr_func.FORM_BF16]
+    names = ["fp32", "plain", "fp16", "bf16"]
+    res["form_per_signal"] = {s["name"]: names[forms.index(plan.form(i))] for i, s in enumerate(sigs)}
+    res["detections"] = decisions(sigs, r_main, truth)
     # per-kernel HIP events on the plan's stream over timed steps
     plan.timing(True)
     for _ in range(args.steps):
@@ -115,57 +154,97 @@ def bench_acq(args, sdr, orc):
             r = plan.kernel_ms("%s@%d" % (n, i))
             if r:
                 ks["%s@%s" % (n, s["name"])] = dict(avg_ms=r[0], launches=r[1], elems=r[2], bytes=r[3],
-                                                    M=s["M"], cls=n)
-    plan.close()
+                                                    M=s["M"], cls=n, form=res["form_per_signal"][s["name"]])
     # dominant kernel: one instantiation (kernel class x signal group)
     dom = max(ks, key=lambda k: ks[k]["avg_ms"] * ks[k]["launches"])
     d = ks[dom]
     gbs = d["bytes"] / (d["avg_ms"] * 1e-3) / 1e9
+    inst = kernel_instance(d["cls"], d["M"], d["form"])
     res["roofline"] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
-                           instantiation=kernel_instance(d["cls"], d["M"]),
-                           alg_bytes_per_launch=round(d["bytes"]),
+                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic_of(inst), kernel=dom,
+                           instantiation=inst, alg_bytes_per_launch=round(d["bytes"]),
                            avg_launch_ms=round(d["avg_ms"], 5))
     res["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 5), launches=v["launches"],
                               gbs=round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1))
                       for k, v in ks.items()}
+    if form != "fp32" and not args.skip_fp32:
+        p32, t32, r32 = run_form(sdr, buffs, sigs, "fp32", args.graph, args.steps, args.warmup)
+        p32.close()
+        res["fp32_batched_cells_per_s"] = round(cells / t32, 1)
+        res["fp32_batched_ms_per_step"] = round(t32 * 1e3, 4)
+        res["same_decisions_as_fp32"] = same_decisions(sigs, r_main, r32)
+        log("fp32 batched: %.3f ms/step; same decisions: %s" % (t32 * 1e3, res["same_decisions_as_fp32"]))
     if not args.skip_plain:
-        plan = sdr.SearchPlan(buffs, sigs, sdr_func.SEARCH_PLAIN)
-        t_p = time_plan(plan, max(1, args.steps // 4), 1)
-        same = all(np.array_equal(plan.result(i)[1], idx_b[i]) for i in range(len(sigs)))
-        plan.close()
+        pp = sdr.SearchPlan(buffs, sigs, sdr_func.SEARCH_PLAIN)
+        t_p = time_plan(pp, max(1, args.steps // 4), 1)
+        rp = [pp.result(i) for i in range(len(sigs))]
+        pp.close()
         res["plain_fp32_cells_per_s"] = round(cells / t_p, 1)
         res["batched_over_plain"] = round(t_p / t_b, 3)
-        res["plain_same_indices"] = bool(same)
-        log("plain: %.3f ms/step (batched is %.2fx)" % (t_p * 1e3, t_p / t_b))
-    res["detections"] = det
+        res["same_decisions_as_plain"] = same_decisions(sigs, r_main, rp)
+        log("plain: %.3f ms/step (batched %s is %.2fx)" % (t_p * 1e3, form, t_p / t_b))
     if orc is not None and not args.skip_cpu:
-        # bounded sample: the first cpu_prns PRNs of every signal, all bins and blocks
-        t0 = time.perf_counter()
-        n_cells = 0
-        parts = []
-        for s in sigs:
-            nq = min(args.cpu_prns, s["code_fft"].shape[0])
-            for q in range(nq):
-                P = np.zeros((len(s["fds"]), s["M"]), np.float32)
-                for k in range(s["n_blk"]):
-                    orc.search_code(s["code_fft"][q], s["T"], buffs[s["buff"]],
-                                    s["ix0"] + k * s["blk_step"], s["fs"], s["fi"], s["fds"], P)
-            n_cells += nq * len(s["fds"])
-            parts.append("%s %d PRNs x %d bins x %d blocks" % (s["name"], nq, len(s["fds"]), s["n_blk"]))
-        t_cpu = time.perf_counter() - t0
-        # cells/s of the full workload's mix: time per cell of each signal weighted by its cells
-        res["cpu_baseline"] = dict(
-            value=round(n_cells / t_cpu, 2), unit="cells/s", cores=1, kind="port",
-            sample="; ".join(parts) + " (%.1f s): oracle C restatement of sdr_search_code with "
-                   "the reference PocketFFT (%s), one thread, no 1 ms sleeps"
-                   % (t_cpu, "loaded" if orc.REF_LOADED else "absent"))
-        log("cpu: %.1f cells/s over %.1f s" % (n_cells / t_cpu, t_cpu))
+        res["cpu_baseline"], res["peak_power_err_vs_fp64"] = cpu_leg(args, orc, plan, buffs, sigs)
+    plan.close()
     cfg = dict(workload=wname, cells=cells,
                signals=[dict(name=s["name"], fs=s["fs"], T=s["T"], M=s["M"], n_prn=s["code_fft"].shape[0],
                              n_bins=len(s["fds"]), n_blk=s["n_blk"]) for s in sigs],
-               path="batched" + ("+hipgraph" if args.graph else ""))
+               path="batched" + ("+hipgraph" if args.graph else ""), form=form)
     return cells / t_b, t_b, res, cfg
+
+
+def cpu_leg(args, orc, plan, buffs, sigs):
+    """CPU baseline: the oracle's sdr_search_code (reference PocketFFT, one
+    thread) timed on a bounded sample, the first cpu_prns PRNs of every signal
+    over all bins and blocks.  Untimed, on the same sample: the FP64 oracle's
+    power at each PRN's peak cell against the timed GPU form's surface."""
+    t0 = time.perf_counter()
+    n_cells = 0
+    parts, surf = [], []
+    for i, s in enumerate(sigs):
+        nq = min(args.cpu_prns, s["code_fft"].shape[0])
+        for q in range(nq):
+            P = np.zeros((len(s["fds"]), s["M"]), np.float32)
+            for k in range(s["n_blk"]):
+                orc.search_code(s["code_fft"][q], s["T"], buffs[s["buff"]],
+                                s["ix0"] + k * s["blk_step"], s["fs"], s["fi"], s["fds"], P)
+            surf.append((i, q, P))
+        n_cells += nq * len(s["fds"])
+        parts.append("%s %d PRNs x %d bins x %d blocks" % (s["name"], nq, len(s["fds"]), s["n_blk"]))
+    t_cpu = time.perf_counter() - t0
+    log("cpu: %.1f cells/s over %.1f s" % (n_cells / t_cpu, t_cpu))
+    base = dict(value=round(n_cells / t_cpu, 2), unit="cells/s", cores=1, kind="port",
+                sample="; ".join(parts) + " (%.1f s): oracle C restatement of sdr_search_code with "
+                       "the reference PocketFFT (%s), one thread, no 1 ms sleeps"
+                       % (t_cpu, "loaded" if orc.REF_LOADED else "absent"))
+    worst = 0.0
+    for i, q, P in surf:
+        s = sigs[i]
+        b, lag = np.unravel_index(int(np.argmax(P[:, :s["N"]])), (len(s["fds"]), s["N"]))
+        ref = np.zeros((1, s["M"]), np.float64)
+        for k in range(s["n_blk"]):
+            orc.search_code_f64(s["code_fft"][q], s["T"], buffs[s["buff"]], s["ix0"] + k * s["blk_step"],
+                                s["fs"], s["fi"], s["fds"][b:b + 1], ref)
+        g = plan.power(i, q)
+        worst = max(worst, abs(float(g[b, lag]) - ref[0, lag]) / ref[0, lag])
+    return base, dict(max_rel=float("%.3e" % worst), cells=len(surf),
+                      note="peak cell of each sampled PRN, timed form vs FP64 oracle")
+
+
+def traffic_of(inst):
+    """HBM bytes per launch of a pass instantiation from the committed
+    FETCH_SIZE / WRITE_SIZE summary of the same workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "search_multi_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 5 and cells[1].strip("`") == inst:
+            try:
+                return round((float(cells[3]) + float(cells[4])) * 1e6)
+            except ValueError:
+                return None
+    return None
 
 
 def bench_trk(args, sdr, workloads):
@@ -210,6 +289,9 @@ def main():
     ap.add_argument("--skip-trk", action="store_true")
     ap.add_argument("--cpu-prns", type=int, default=4)
     ap.add_argument("--workload", default="multi", choices=["multi", "l1ca"])
+    ap.add_argument("--form", default="fp16", choices=["fp16", "bf16", "fp32"],
+                    help="arithmetic form of the timed batched search")
+    ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--trk-epochs", type=int, default=1000)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a torch device copy of this many MiB (counter calibration)")
@@ -232,7 +314,7 @@ def main():
         res.update(bench_trk(args, sdr, workloads))
     line = dict(metric=METRIC, value=round(value, 1), unit="cells/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=round(t_step * 1e3, 4), higher_is_better=True,
-                scaling="weak", vs_baseline=None, dtype="f32",
+                scaling="weak", vs_baseline=None, dtype=FORM_DTYPE[args.form],
                 data="synthetic (seeded cpx8 IF, 2-bit, ~25% visible PRNs, C/N0 38-50 dB-Hz)",
                 config=cfg)
     line.update(res)

@@ -135,6 +135,13 @@ typedef struct {
 #define SDR_SEARCH_BATCHED  0  /* shared forward FFT per (bin, block), fused */
 #define SDR_SEARCH_PLAIN    1  /* reference dataflow per (PRN, block), unfused */
 #define SDR_SEARCH_GRAPH    2  /* capture the run in a hipGraph and replay it */
+/* Precision form of the batched search: FP16 (or BF16) complex words between
+ * the FFT passes, every radix stage a DFT-matrix product on the matrix cores
+ * with FP32 accumulation, FP32 twiddles and power surfaces.  Applies to FFT
+ * lengths with an MFMA split (24000, 48000, 96000, 192000, 960000); other
+ * groups of the plan run the FP32 batched path.  Not combined with PLAIN. */
+#define SDR_SEARCH_FP16     4
+#define SDR_SEARCH_BF16     8
 
 typedef struct sdr_search_plan_tag sdr_search_plan_t;
 
@@ -152,6 +159,12 @@ int sdr_search_plan_result(const sdr_search_plan_t *plan, int sig, float *cn0,
 /* Copies the power surface of one PRN (len_fds x N, bin-major). */
 int sdr_search_plan_power(const sdr_search_plan_t *plan, int sig, int prn_idx,
     float *P);
+/* Arithmetic form signal group `sig` runs in (-1 on a bad argument). */
+#define SDR_FORM_FP32   0      /* batched FP32 passes */
+#define SDR_FORM_PLAIN  1      /* per-PRN reference dataflow, FP32 */
+#define SDR_FORM_FP16   2      /* batched, FP16 words, MFMA stages */
+#define SDR_FORM_BF16   3      /* batched, BF16 words, MFMA stages */
+int sdr_search_plan_form(const sdr_search_plan_t *plan, int sig);
 /* Enables per-kernel HIP-event timing of later runs (0 disables). */
 void sdr_search_plan_timing(sdr_search_plan_t *plan, int on);
 /* Average device time (ms) and launch count of kernel class `name`

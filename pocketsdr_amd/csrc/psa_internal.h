@@ -91,6 +91,8 @@ struct FFTPlan {
     float2 *thi_bwd = nullptr, *tlo_bwd = nullptr;
 };
 bool fft_supported(int M);
+// device table of scale * exp(sign 2 pi i m / L), m < L, rounded from double
+float2 *make_twiddle(int L, int sign, double scale);
 const FFTPlan *fft_plan(int M);   // cached, thread-safe; nullptr if unsupported
 void fft_init_attrs();            // LDS limits of the pass kernels (once)
 
@@ -168,6 +170,15 @@ struct PassArgs {
     // b = prn * xcd_nbin + bin, prn < xcd_nprn; the PRNs of one (row group,
     // bin) tile run back to back on one XCD
     int xcd_nbin, xcd_nprn;
+    // precision form (psa_fft16.hip): a complex value is one 32-bit word of
+    // two FP16 or BF16 halves, real part in the low half
+    const uint32_t *hin;        // LD_CPX / LD_PROD (X) input words
+    uint32_t *hout;             // ST_CPX output words
+    const uint32_t *hcf;        // LD_PROD code-spectrum words
+    const uint4 *amat[3];       // MFMA A operands of the stage DFT matrices
+    const float2 *tfull;        // ST_CPX: twiddle per output index, pre-scaled; nullptr: none
+    float osc;                  // ST_CPX scale applied when tfull is nullptr
+    float kappa;                // ST_POW: values scaled by kappa before |.|^2
 };
 
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
@@ -180,6 +191,31 @@ bool spec_length(int L);          // L has a compile-time specialised kernel
 void col_pass_args(const FFTPlan *pl, int dir, PassArgs &a);
 // row pass: M1 contiguous transforms of length M2, output [k1][k2]
 void row_pass_args(const FFTPlan *pl, int dir, PassArgs &a);
+
+// ---------------------------------------------------------------------------
+// precision form (psa_fft16.hip): FP16 / BF16 words, MFMA DFT-matrix stages
+// ---------------------------------------------------------------------------
+enum { HDT_F16 = 0, HDT_BF16 = 1 };
+
+struct HPlan {
+    int M = 0, M1 = 0, M2 = 0;  // column length M1, row length M2
+    int dt = HDT_F16;
+    float2 *tw1[2] = {}, *tw2[2] = {};   // w_L^m of the two lengths, [dir]
+    float2 *tfull[2] = {};               // inter-pass twiddle by output index, [dir]
+    const uint4 *am1[2][3] = {}, *am2[2][3] = {};   // [dir][stage]
+    // power-of-two scales: forward column / row pass outputs, inverse row
+    // pass output (folded into tfull where a pass has the twiddle)
+    float osc_fc = 1, osc_fr = 1, osc_ir = 1;
+};
+// cached per (M, dt); nullptr when M has no MFMA split
+const HPlan *hfft_plan(int M, int dt);
+void h_col_args(const HPlan *hp, int dir, PassArgs &a);
+void h_row_args(const HPlan *hp, int dir, PassArgs &a);
+void launch_hpass(int dt, LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
+                  hipStream_t st);
+// natural-order FP32 code spectra (n x M) -> digit-transposed words * scale
+void h_code_spec(const HPlan *hp, const float2 *nat, uint32_t *out, int n, float scale,
+                 hipStream_t st);
 
 // Full forward / backward transforms of `nbatch` contiguous length-M arrays
 // (natural order in and out), used by sdr_cpx_fft and the code spectra.

@@ -1,0 +1,210 @@
+// psa_pass.h - device-side element loaders and storers shared by the FP32
+// pass kernels (psa_fft.hip) and the FP16 / BF16 MFMA pass kernels
+// (psa_fft16.hip).
+#pragma once
+#include "psa_internal.h"
+
+namespace psa {
+
+#define PASS_THREADS 256
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(__builtin_fmaf(a.x, b.x, -a.y * b.y), __builtin_fmaf(a.x, b.y, a.y * b.x));
+}
+// multiply by -i (forward) or +i (backward): sgn = -1 forward, +1 backward
+__device__ __forceinline__ float2 mul_i(float2 a, float sgn) { return make_float2(-sgn * a.y, sgn * a.x); }
+
+// ---------------------------------------------------------------------------
+// element loaders / storers of the pass kernels
+// ---------------------------------------------------------------------------
+template <int LM>
+struct Loader {
+    const float2 *pin = nullptr;
+    const char2 *qin = nullptr;
+    const float2 *xin = nullptr, *cin = nullptr;
+    const uint8_t *buf = nullptr;
+    const uint32_t *carr = nullptr;     // LDS copy of the carrier table
+    int ix = 0, nseg = 0, compact = 0;
+    uint32_t p0 = 0, p1 = 0, s = 0;
+    bool prod_tail = false;
+    int prod_V = 0, prod_M1 = 1, prod_M2 = 1;
+    float prod_scale = 1.0f;
+
+    __device__ __forceinline__ void init(const PassArgs &a, int b, const uint32_t *carr_lds)
+    {
+        if (LM == LD_CPX) pin = a.in + (long long)b * a.in_bstride;
+        if (LM == LD_CPX16) qin = reinterpret_cast<const char2 *>(a.iq16) + (long long)b * a.in_bstride;
+        if (LM == LD_MIX) {
+            carr = carr_lds; buf = a.buf; compact = a.mix_compact;
+            ix = a.mix_ix[b]; nseg = a.mix_nseg[b];
+            p0 = a.mix_p0[b]; p1 = a.mix_p1[b]; s = a.mix_s[b];
+        }
+        if (LM == LD_PROD) {
+            const long long xb = (b / a.prod_xdiv) % a.prod_xmod;
+            const long long cb = b / a.prod_cdiv;
+            xin = a.in + xb * a.in_bstride;
+            cin = a.cf + cb * a.cf_bstride;
+            prod_tail = a.prod_V < a.M;
+            prod_V = a.prod_V; prod_M1 = a.prod_M1; prod_M2 = a.prod_M2;
+            prod_scale = a.prod_scale;
+        }
+    }
+
+    // LD_MIX: the carrier-mixed int8 pair of sample e, as mix_tbl holds it
+    // (src/sdr_func.c:113-134, 786-806)
+    __device__ __forceinline__ int2 mix_int(int e) const
+    {
+        int src;
+        uint32_t ph;
+        if (e < nseg) { src = ix + e; ph = p0 + (uint32_t)e * s; }
+        else { src = compact ? e : e - nseg; ph = p1 + (uint32_t)(e - nseg) * s; }
+        const uint32_t d = buf[src];
+        const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+        const int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+        const uint32_t c = carr[ph >> 24];
+        const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+        return make_int2((int8_t)(sI * cI - sQ * cQ), (int8_t)(sI * cQ + sQ * cI));
+    }
+
+    // element e of the batch item (natural index for LD_MIX / LD_CPX16)
+    __device__ __forceinline__ float2 load(int e) const
+    {
+        if (LM == LD_CPX) return pin[e];
+        if (LM == LD_CPX16) {
+            const char2 q = qin[e];
+            return make_float2((float)q.x * PSA_CSCALE, (float)q.y * PSA_CSCALE);
+        }
+        if (LM == LD_MIX) {
+            const int2 m = mix_int(e);
+            return make_float2((float)m.x * PSA_CSCALE, (float)m.y * PSA_CSCALE);
+        }
+        // LD_PROD: sdr_cpx_mul AVX2 lanes (src/sdr_func.c:203-213):
+        // re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) + fl(ai br), * s; the
+        // last M % 4 natural indices take the scalar tail form
+        // (src/sdr_func.c:229-232, contracted by the reference build)
+        const float2 x = xin[e], c = cin[e];
+        bool tail = false;
+        if (prod_tail) tail = (e / prod_M2 + prod_M1 * (e % prod_M2)) >= prod_V;
+        float re, im;
+        if (!tail) {
+            re = x.x * c.x + x.y * -c.y;
+            im = x.x * c.y + x.y * c.x;
+        } else {
+            re = __builtin_fmaf(x.x, c.x, -(x.y * c.y));
+            im = __builtin_fmaf(x.x, c.y, x.y * c.x);
+        }
+        return make_float2(re * prod_scale, im * prod_scale);
+    }
+};
+
+template <int SM>
+struct Storer {
+    float2 *pout = nullptr;
+    float *prow = nullptr;
+    const float2 *thi = nullptr, *tlo = nullptr;
+    int nlag = 0, ogs = 0, ons = 0;
+    bool pinit = false, track = false;
+    long long ibase = 0;            // row-major index of lag 0 of this row
+    float bm = 0.0f;                // peak tracking of the values stored
+    long long bi = -1;
+    double bs = 0.0;
+
+    __device__ __forceinline__ void init(const PassArgs &a, int b)
+    {
+        thi = a.thi; tlo = a.tlo;
+        ogs = (int)a.out_gstride; ons = (int)a.out_nstride;
+        if (SM == ST_CPX) pout = a.out + (long long)b * a.out_bstride;
+        if (SM == ST_POW) {
+            const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
+            prow = a.P + row * a.p_bstride;
+            nlag = a.nlag;
+            pinit = a.p_init != 0;
+            track = a.p_part != nullptr;
+            if (track) ibase = (row % a.p_nbin) * (long long)a.nlag;
+        }
+    }
+
+    // output k of global sub-transform gg
+    __device__ __forceinline__ void store(int gg, int k, float2 v)
+    {
+        if (thi) {
+            // w_M^{gg k}: gg < nsub and k < L with nsub * L = M, so gg k < M
+            const int ti = gg * k;
+            v = cmul(v, cmul(thi[ti >> 10], tlo[ti & 1023]));
+        }
+        const int e = gg * ogs + k * ons;
+        if (SM == ST_CPX) {
+            pout[e] = v;
+        } else if (e < nlag) {
+            // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680);
+            // on the first block P is zero and 0 + x = x exactly
+            const float pw = __builtin_fmaf(v.x, v.x, v.y * v.y);
+            const float nv = pinit ? pw : prow[e] + pw;
+            prow[e] = nv;
+            if (track) {
+                bs += (double)nv;
+                const long long gi = ibase + e;
+                if (nv > bm || (nv == bm && bi >= 0 && gi < bi)) {
+                    if (nv > 0.0f) { bm = nv; bi = gi; }
+                }
+            }
+        }
+    }
+
+    // workgroup reduction of the tracked peak into part[slot]
+    __device__ __forceinline__ void flush_peak(PeakPart *part, long long slot, PeakPart *red)
+    {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const float om = __shfl_xor(bm, o, 64);
+            const long long oi = __shfl_xor(bi, o, 64);
+            bs += __shfl_xor(bs, o, 64);
+            if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+        }
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { red[w].m = bm; red[w].i = bi; red[w].s = bs; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < PASS_THREADS / 64; i++) {
+                bs += red[i].s;
+                const long long oi = red[i].i;
+                const float om = red[i].m;
+                if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+            }
+            PeakPart q;
+            q.m = bm; q.pad = 0; q.i = bi; q.s = bs;
+            part[slot] = q;
+        }
+    }
+};
+
+// Batch item b and workgroup-group grp of this workgroup.  With xcd_nbin > 0
+// (inverse row passes) the order is XCD-aware: workgroups go round-robin to
+// the 8 XCDs, so tile T = grp * nbin + bin keeps XCD T % 8 and its PRNs run
+// back to back there, re-reading the shared X rows from that XCD's L2.  The
+// grid is padded to whole rounds of 8; returns false for the padding.
+__device__ __forceinline__ bool block_item(const PassArgs &a, bool prod, int ngroups, int *b, int *grp)
+{
+    if (prod && a.xcd_nbin > 0) {
+        const int w = (int)blockIdx.x;
+        const int r = w >> 3;
+        const int prn = r % a.xcd_nprn;
+        const int T = (r / a.xcd_nprn) * 8 + (w & 7);
+        if (T >= ngroups * a.xcd_nbin) return false;
+        *grp = T / a.xcd_nbin;
+        const int bin = T - *grp * a.xcd_nbin;
+        *b = prn * a.xcd_nbin + bin;
+    } else {
+        *b = (int)(blockIdx.x / (unsigned)ngroups);
+        *grp = (int)blockIdx.x - *b * ngroups;
+    }
+    return true;
+}
+
+}  // namespace psa

@@ -462,6 +462,9 @@ struct SigDev {
     const FFTPlan *pl = nullptr;
     float2 *cft = nullptr;      // n_prn x M, digit-transposed (batched)
     float2 *cfn = nullptr;      // n_prn x M, natural order (plain)
+    const HPlan *hp = nullptr;  // precision form (FP16 / BF16 MFMA passes)
+    uint32_t *cfh = nullptr;    // n_prn x M words, digit-transposed, scaled
+    float kappa = 1.0f;         // power scale of the precision form
     float *P = nullptr;         // n_prn x len_fds x N
     float *fds = nullptr;       // device copy of the bins
     int *mix_ix = nullptr, *mix_nseg = nullptr;   // [n_blk x len_fds]
@@ -602,6 +605,84 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
     }
 }
 
+// The precision form of run_batched: FP16 / BF16 words between the passes,
+// MFMA DFT-matrix stages.  Scales (powers of two): forward column pass output
+// osc_fc (folded into its twiddle table), forward row pass osc_fr, code
+// spectra 2^-ceil(log2 N) (|Cf| <= 1), inverse row pass osc_ir; the inverse
+// column pass multiplies by kappa, which undoes them together with the
+// reference's 1/11.2 sample scale and 1/M^2 product scale.
+static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
+{
+    hipStream_t st = p->st;
+    const HPlan *hp = s.hp;
+    const int dt = hp->dt;
+    const int M = hp->M, N = s.h.N, nbin = s.h.len_fds, nprn = s.h.n_prn;
+    const uint8_t *buf = p->d_buf[s.h.buff];
+    const int buffN = p->buffN[s.h.buff];
+    uint32_t *tmp = reinterpret_cast<uint32_t *>(p->tmp);
+    uint32_t *X = reinterpret_cast<uint32_t *>(p->X);
+    uint32_t *Z = reinterpret_cast<uint32_t *>(p->Z);
+    for (int k = 0; k < s.h.n_blk; k++) {
+        for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
+            const int nbc = std::min(s.bin_chunk, nbin - b0);
+            const long long mo = (long long)k * nbin + b0;
+            PassArgs a = {};
+            h_col_args(hp, 0, a);
+            a.tfull = hp->tfull[0];
+            a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
+            a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
+            a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
+            a.hout = tmp; a.out_bstride = M;
+            {
+                Tick t(p, K_FWD_COL, (double)nbc * M, (double)M + 4.0 * nbc * M);
+                launch_hpass(dt, LD_MIX, ST_CPX, a, nbc, st);
+            }
+            PassArgs r = {};
+            h_row_args(hp, 0, r);
+            r.osc = hp->osc_fr;
+            r.hin = tmp; r.hout = X; r.in_bstride = r.out_bstride = M;
+            {
+                Tick t(p, K_FWD_ROW, (double)nbc * M, 8.0 * nbc * M);
+                launch_hpass(dt, LD_CPX, ST_CPX, r, nbc, st);
+            }
+            for (int q0 = 0; q0 < nprn; q0 += s.prn_chunk) {
+                const int nqc = std::min(s.prn_chunk, nprn - q0);
+                const int nbatch = nqc * nbc;      // b = prn_local * nbc + bin_local
+                PassArgs ir = {};
+                h_row_args(hp, 1, ir);
+                ir.tfull = hp->tfull[1];
+                ir.hin = X; ir.in_bstride = M;
+                ir.hcf = s.cfh + (long long)q0 * M; ir.cf_bstride = M;
+                ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
+                ir.hout = Z; ir.out_bstride = M;
+                if (s.xcd_order) {
+                    ir.xcd_nbin = nbc;
+                    ir.xcd_nprn = nqc;
+                }
+                {
+                    Tick t(p, K_INV_ROW, (double)nbatch * M, 4.0 * M * (nbc + nqc + (double)nbatch));
+                    launch_hpass(dt, LD_PROD, ST_CPX, ir, nbatch, st);
+                }
+                PassArgs ic = {};
+                h_col_args(hp, 1, ic);
+                ic.kappa = s.kappa;
+                ic.hin = Z; ic.in_bstride = M;
+                ic.P = s.P; ic.p_bstride = N;
+                ic.p_div = nbc; ic.p_off1 = q0; ic.p_mul = nbin; ic.p_off2 = b0;
+                ic.nlag = N;
+                ic.p_nbin = nbin;
+                ic.p_init = k == 0;
+                if (k == s.h.n_blk - 1 && s.fused_peak) ic.p_part = s.part;
+                {
+                    Tick t(p, K_INV_COL, (double)nbatch * M,
+                           (double)nbatch * (4.0 * M + (k == 0 ? 4.0 : 8.0) * N));
+                    launch_hpass(dt, LD_CPX, ST_POW, ic, nbatch, st);
+                }
+            }
+        }
+    }
+}
+
 // the reference dataflow, one sdr_search_code-equivalent per (PRN, block):
 // mix -> cpx16, forward FFT, sdr_cpx_mul, inverse FFT, power accumulate,
 // each its own kernel, nothing shared across PRNs
@@ -672,6 +753,7 @@ static void run_all(sdr_search_plan_t *p)
     for (auto &s : p->sig) {
         p->cur_sig = (int)(&s - p->sig.data());
         if (p->flags & SDR_SEARCH_PLAIN) run_plain(p, s);
+        else if (s.hp) run_batched_h(p, s);
         else run_batched(p, s);
         Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N,
                4.0 * s.h.n_prn * s.h.len_fds * s.h.N);
@@ -746,8 +828,21 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         float2 *nat;
         PSA_CHECK(hipMalloc(&nat, spec * nprn));
         PSA_CHECK(hipMemcpy(nat, sigs[i].code_fft, spec * nprn, hipMemcpyHostToDevice));
+        if (!(flags & SDR_SEARCH_PLAIN) && (flags & (SDR_SEARCH_FP16 | SDR_SEARCH_BF16))) {
+            s.hp = hfft_plan(M, (flags & SDR_SEARCH_BF16) ? HDT_BF16 : HDT_F16);
+        }
         if (flags & SDR_SEARCH_PLAIN) {
             s.cfn = nat;
+        } else if (s.hp) {
+            // |code_fft| <= N (a zero-padded code of N samples in [-1, 1])
+            const int e = (int)ceil(log2((double)s.h.N));
+            const double sc = ldexp(1.0, -e);
+            PSA_CHECK(hipMalloc(&s.cfh, sizeof(uint32_t) * (size_t)M * nprn));
+            h_code_spec(s.hp, nat, s.cfh, nprn, (float)sc, p->st);
+            PSA_CHECK(hipStreamSynchronize(p->st));
+            PSA_CHECK(hipFree(nat));
+            s.kappa = (float)((double)PSA_CSCALE / ((double)M * M * s.hp->osc_fc * s.hp->osc_fr * sc *
+                                                     s.hp->osc_ir));
         } else {
             PSA_CHECK(hipMalloc(&s.cft, spec * nprn));
             fft_to_transposed(s.pl, nat, s.cft, nprn, p->st);
@@ -786,17 +881,21 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         s.nchunk = (int)(((long long)nbin * s.h.N + PEAK_CHUNK - 1) / PEAK_CHUNK);
         // the last column pass emits one partial per workgroup when its
         // length has a specialised kernel
-        s.fused_peak = spec_length(s.pl->M1) && !(flags & SDR_SEARCH_PLAIN) && !getenv("PSA_NO_FUSED_PEAK");
+        s.fused_peak = (s.hp || spec_length(s.pl->M1)) && !(flags & SDR_SEARCH_PLAIN) &&
+                       !getenv("PSA_NO_FUSED_PEAK");
         {
             PassArgs tmp = {};
-            col_pass_args(s.pl, 1, tmp);
+            if (s.hp) h_col_args(s.hp, 1, tmp);
+            else col_pass_args(s.pl, 1, tmp);
             s.nchunk_fused = nbin * pass_ngroups(tmp);
         }
         s.xcd_order = xcd_order;
         PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused)));
-        // chunking of the intermediates
-        s.bin_chunk = (int)std::max<size_t>(1, std::min<size_t>(nbin, budget / spec));
-        size_t pairs = std::max<size_t>(1, budget / spec);
+        // chunking of the intermediates (one word per complex in the
+        // precision form)
+        const size_t zspec = s.hp ? spec / 2 : spec;
+        s.bin_chunk = (int)std::max<size_t>(1, std::min<size_t>(nbin, budget / zspec));
+        size_t pairs = std::max<size_t>(1, budget / zspec);
         s.prn_chunk = (int)std::max<size_t>(1, std::min<size_t>(nprn, pairs / s.bin_chunk));
         if (flags & SDR_SEARCH_PLAIN) s.prn_chunk = 1;
         p->tmp_elems = std::max(p->tmp_elems, (size_t)s.bin_chunk * M);
@@ -864,6 +963,15 @@ int sdr_search_plan_power(const sdr_search_plan_t *p, int sig, int prn_idx, floa
     size_t n = (size_t)s.h.len_fds * s.h.N;
     PSA_CHECK(hipMemcpy(P, s.P + (size_t)prn_idx * n, sizeof(float) * n, hipMemcpyDeviceToHost));
     return 1;
+}
+
+int sdr_search_plan_form(const sdr_search_plan_t *p, int sig)
+{
+    if (!p || sig < 0 || sig >= (int)p->sig.size()) return -1;
+    if (p->flags & SDR_SEARCH_PLAIN) return SDR_FORM_PLAIN;
+    const SigDev &s = p->sig[sig];
+    if (!s.hp) return SDR_FORM_FP32;
+    return s.hp->dt == HDT_BF16 ? SDR_FORM_BF16 : SDR_FORM_FP16;
 }
 
 void sdr_search_plan_timing(sdr_search_plan_t *p, int on)

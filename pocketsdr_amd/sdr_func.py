@@ -70,6 +70,7 @@ _decl("sdr_search_plan_new", c_void_p, [C.c_int, C.POINTER(C.POINTER(sdr_buff_t)
 _decl("sdr_search_plan_run", C.c_int, [c_void_p])
 _decl("sdr_search_plan_result", C.c_int, [c_void_p, C.c_int, c_float_p, c_int_p, c_double_p])
 _decl("sdr_search_plan_power", C.c_int, [c_void_p, C.c_int, C.c_int, c_float_p])
+_decl("sdr_search_plan_form", C.c_int, [c_void_p, C.c_int])
 _decl("sdr_search_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_search_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                              C.POINTER(C.c_longlong), c_double_p])
@@ -213,6 +214,9 @@ def dop_bins(T, dop, max_dop):
 SEARCH_BATCHED = 0
 SEARCH_PLAIN = 1
 SEARCH_GRAPH = 2
+SEARCH_FP16 = 4
+SEARCH_BF16 = 8
+FORM_FP32, FORM_PLAIN, FORM_FP16, FORM_BF16 = 0, 1, 2, 3
 
 
 class SearchPlan:
@@ -257,6 +261,10 @@ class SearchPlan:
         P = np.zeros((len(s["fds"]), s["N"]), np.float32)
         lib.sdr_search_plan_power(self.p, sig, prn_idx, _fp(P))
         return P
+
+    def form(self, sig):
+        """FORM_FP32 / FORM_PLAIN / FORM_FP16 / FORM_BF16 of signal group sig."""
+        return lib.sdr_search_plan_form(self.p, sig)
 
     def timing(self, on=True):
         lib.sdr_search_plan_timing(self.p, 1 if on else 0)
