@@ -97,6 +97,11 @@ def build_workload(name, seed):
         w = workloads.acq_multi(seed=seed)
         return w["buffs"], w["sigs"], w["truth"], \
             "acq_batched_uneven_L1CA_E1B_L5Q_L6D"
+    if name == "l1cp":
+        # one GPU's share of the 8-channel array: one RF channel, full size
+        w = workloads.acq_l1cp(seed=seed, n_rf=1)
+        return w["buffs"], w["sigs"], w["truth"], \
+            "acq_weak_L1CP_48Msps_10ms_x20_241bins_32prn_1rf_of_8"
     raise SystemExit("unknown workload " + name)
 
 
@@ -116,10 +121,21 @@ def run_form(sdr, buffs, sigs, form, graph, steps, warmup):
 
 
 def decisions(sigs, res, truth):
+    """Per signal: PRNs over the 38 dB-Hz threshold, the planted PRNs, and
+    the planted PRNs whose peak cell sits on the true code phase (2 samples
+    plus the code Doppler over the blocks) and Doppler (one bin)."""
     det = {}
     for s, (cn0, ix, dop) in zip(sigs, res):
         found = sorted(s["prns"][q] for q in range(len(s["prns"])) if cn0[q] >= 38.0)
-        det[s["name"]] = dict(found=found, truth=sorted(t["prn"] for t in truth[s["name"]]))
+        located = []
+        for t in truth[s["name"]]:
+            q = s["prns"].index(t["prn"])
+            dl = (ix[q][1] / s["fs"] - t["tau"] + s["T"] / 2) % s["T"] - s["T"] / 2
+            tol = 2.0 / s["fs"] + abs(t["fd"]) / t["fc"] * (s["n_blk"] + 1) * s["T"]   # + code Doppler
+            if abs(dl) < tol and abs(float(s["fds"][ix[q][0]]) - t["fd"]) <= 0.5 / s["T"]:
+                located.append(t["prn"])
+        det[s["name"]] = dict(found=found, truth=sorted(t["prn"] for t in truth[s["name"]]),
+                              located=sorted(located))
     return det
 
 
@@ -203,14 +219,15 @@ def cpu_leg(args, orc, plan, buffs, sigs):
     parts, surf = [], []
     for i, s in enumerate(sigs):
         nq = min(args.cpu_prns, s["code_fft"].shape[0])
+        nb = min(args.cpu_bins, len(s["fds"])) if args.cpu_bins else len(s["fds"])
         for q in range(nq):
-            P = np.zeros((len(s["fds"]), s["M"]), np.float32)
+            P = np.zeros((nb, s["M"]), np.float32)
             for k in range(s["n_blk"]):
                 orc.search_code(s["code_fft"][q], s["T"], buffs[s["buff"]],
-                                s["ix0"] + k * s["blk_step"], s["fs"], s["fi"], s["fds"], P)
+                                s["ix0"] + k * s["blk_step"], s["fs"], s["fi"], s["fds"][:nb], P)
             surf.append((i, q, P))
-        n_cells += nq * len(s["fds"])
-        parts.append("%s %d PRNs x %d bins x %d blocks" % (s["name"], nq, len(s["fds"]), s["n_blk"]))
+        n_cells += nq * nb
+        parts.append("%s %d PRNs x %d bins x %d blocks" % (s["name"], nq, nb, s["n_blk"]))
     t_cpu = time.perf_counter() - t0
     log("cpu: %.1f cells/s over %.1f s" % (n_cells / t_cpu, t_cpu))
     base = dict(value=round(n_cells / t_cpu, 2), unit="cells/s", cores=1, kind="port",
@@ -220,7 +237,7 @@ def cpu_leg(args, orc, plan, buffs, sigs):
     worst = 0.0
     for i, q, P in surf:
         s = sigs[i]
-        b, lag = np.unravel_index(int(np.argmax(P[:, :s["N"]])), (len(s["fds"]), s["N"]))
+        b, lag = np.unravel_index(int(np.argmax(P[:, :s["N"]])), (P.shape[0], s["N"]))
         ref = np.zeros((1, s["M"]), np.float64)
         for k in range(s["n_blk"]):
             orc.search_code_f64(s["code_fft"][q], s["T"], buffs[s["buff"]], s["ix0"] + k * s["blk_step"],
@@ -288,7 +305,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-trk", action="store_true")
     ap.add_argument("--cpu-prns", type=int, default=4)
-    ap.add_argument("--workload", default="multi", choices=["multi", "l1ca"])
+    ap.add_argument("--workload", default="multi", choices=["multi", "l1ca", "l1cp"])
+    ap.add_argument("--cpu-bins", type=int, default=0,
+                    help="CPU sample: only the first this many Doppler bins (0: all)")
     ap.add_argument("--form", default="fp16", choices=["fp16", "bf16", "fp32"],
                     help="arithmetic form of the timed batched search")
     ap.add_argument("--skip-fp32", action="store_true")

@@ -155,3 +155,44 @@ def acq_multi(seed=5, t_int=10e-3, max_dop=5000.0, cn0=(38.0, 50.0), bits=2, cod
         n_samp = int(d["fs"] * (t_int + d["Tmax"]))
         buffs.append(synth.synth_if(d["fs"], n_samp, d["sats"], seed * 10 + st, bits=bits))
     return dict(buffs=buffs, sigs=sigs, truth=truth)
+
+
+# ---------------------------------------------------------------------------
+# weak-signal wide search: GPS L1C-P on an antenna array
+# ---------------------------------------------------------------------------
+def acq_l1cp(seed=9, n_rf=8, fs=48e6, n_prn=32, n_vis=8, n_blk=20, max_dop=6000.0,
+             cn0=(25.0, 35.0), bits=3, code_fft_fn=None, prn_limit=None, bin_limit=None):
+    """GPS L1C pilot (10 ms, 10230 chips TMBOC, 122760 replica entries,
+    src/sdr_code.c:884-913) at 48 Msps complex 3-bit on n_rf RF channels of
+    one antenna array: every channel sees the same satellites (code phase,
+    Doppler, C/N0) with its own carrier phase and noise.  10 ms coherent
+    (N = 480000, M = 960000) x n_blk non-coherent blocks, Doppler bins of
+    1/(2T) = 50 Hz over +/-6 kHz (241 bins).  One signal group per RF channel;
+    the groups share one code-spectrum array.  prn_limit / bin_limit cut the
+    PRN list / take the bin_limit bins around each visible satellite's
+    Doppler (reduced sizes for tests)."""
+    rng = np.random.default_rng(seed)
+    T = 10e-3
+    N = int(round(fs * T))
+    M = 2 * N
+    n_samp = (n_blk + 1) * N
+    prns = list(range(1, n_prn + 1))[:prn_limit] if prn_limit else list(range(1, n_prn + 1))
+    vis = sorted(rng.choice(prns, min(n_vis, len(prns)), replace=False).tolist())
+    cd = {p: codes.gen_code_L1CP(p) for p in prns}
+    sats = synth.random_sats(rng, vis, cd, T, L1_FREQ, cn0, (-max_dop, max_dop))
+    fn = code_fft_fn or (lambda c, T_, fs_, N_: codes.gen_code_fft(c, T_, 0.0, fs_, N_, N_))
+    code_fft = np.stack([fn(cd[p], T, fs, N) for p in prns]).astype(np.complex64)
+    fds = dop_bins(T, 0.0, max_dop)
+    if bin_limit:
+        keep = set()
+        for s in sats:
+            c = int(np.argmin(np.abs(fds - s["fd"])))
+            keep.update(range(max(0, c - bin_limit // 2), min(len(fds), c - bin_limit // 2 + bin_limit)))
+        fds = fds[sorted(keep)]
+    buffs, sigs = [], []
+    for r in range(n_rf):
+        rs = [dict(s, phi=float(rng.uniform(0, 1))) for s in sats]
+        buffs.append(synth.synth_if(fs, n_samp, rs, seed * 100 + r, bits=bits))
+        sigs.append(dict(name="L1CP@rf%d" % r, buff=r, M=M, N=N, T=T, fs=fs, fi=0.0, code_fft=code_fft,
+                         fds=fds, n_blk=n_blk, ix0=0, blk_step=N, prns=prns))
+    return dict(buffs=buffs, sigs=sigs, truth={s["name"]: sats for s in sigs})
