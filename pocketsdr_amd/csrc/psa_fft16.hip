@@ -28,28 +28,64 @@ typedef float fx16 __attribute__((ext_vector_type(16)));
 #define MP_THREADS 256
 #define MP_WAVES (MP_THREADS / 64)
 
+typedef _Float16 hx2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bx2 __attribute__((ext_vector_type(2)));
+typedef float fx2 __attribute__((ext_vector_type(2)));
+
 // complex value -> one word (re in the low half), round to nearest even
+// (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32)
 template <int DT>
 __device__ __forceinline__ uint32_t pack_c(float re, float im)
 {
-    if constexpr (DT == HDT_F16) {
-        const _Float16 x = (_Float16)re, y = (_Float16)im;
-        return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
-    } else {
-        const __bf16 x = (__bf16)re, y = (__bf16)im;
-        return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
-    }
+    const fx2 v = {re, im};
+    if constexpr (DT == HDT_F16) return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, hx2));
+    else return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bx2));
 }
 
 template <int DT>
 __device__ __forceinline__ float2 unpack_c(uint32_t w)
 {
     if constexpr (DT == HDT_F16) {
-        return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu)),
-                           (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
+        const fx2 v = __builtin_convertvector(__builtin_bit_cast(hx2, w), fx2);
+        return make_float2(v.x, v.y);
     } else {
         return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xFFFF0000u));
     }
+}
+
+// x * c of two packed words: FP16 in packed half arithmetic (the product of
+// two words in range stays in range, |c| <= 1), BF16 through FP32
+template <int DT>
+__device__ __forceinline__ uint32_t cmul_w(uint32_t xw, uint32_t cw)
+{
+    if constexpr (DT == HDT_F16) {
+        const hx2 x = __builtin_bit_cast(hx2, xw), c = __builtin_bit_cast(hx2, cw);
+        const hx2 t = x * c.xx;                                    // (xr cr, xi cr)
+        const hx2 u = {-c.y, c.y};
+        return __builtin_bit_cast(uint32_t, __builtin_elementwise_fma(x.yx, u, t));  // (xr cr - xi ci, xi cr + xr ci)
+    } else {
+        const float2 p = cmul(unpack_c<DT>(xw), unpack_c<DT>(cw));
+        return pack_c<DT>(p.x, p.y);
+    }
+}
+
+// raw buffer access with 32-bit element offsets from a per-item base (no
+// 64-bit address arithmetic per element)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ uint32_t bload32(__amdgpu_buffer_rsrc_t r, int i)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0);
+}
+__device__ __forceinline__ float2 bload_f2(__amdgpu_buffer_rsrc_t r, int i)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0));
+}
+__device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, int i, uint32_t v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, i * 4, 0, 0);
 }
 
 template <int DT>
@@ -65,30 +101,34 @@ __device__ __forceinline__ fx16 mfma(uint4 a, uint4 b, fx16 c)
 }
 
 // LDS layout of a workgroup's G sub-transforms: row g at g * RP, element n
-// at phi(n).  When R0 is even one pad word follows every R0 words, so the
-// first stage's stride-R0 stores spread over the banks; the row pitch RP is
-// odd so that lanes walking across rows do too.
+// at n + n / R0 when R0 is even (one pad word after every R0 words, so the
+// first stage's stride-R0 stores spread over the banks), else at n.  The row
+// pitch RP is odd so that lanes walking across rows do too.
 template <int L, int R0>
 struct MGeom {
-    static constexpr int PAD = (R0 % 2 == 0) ? R0 : 0;
-    static constexpr int RP = (L + (PAD ? L / PAD : 0)) | 1;
-    __device__ static __forceinline__ int phi(int n) { return PAD ? n + n / PAD : n; }
+    static constexpr int PAD = (R0 % 2 == 0) ? 1 : 0;
+    static constexpr int RP = (L + (PAD ? L / R0 : 0)) | 1;
 };
 
 // One Stockham radix-R stage (NS = product of the earlier radices) over the
-// G sub-transforms of the workgroup.  Column c = (g, j), j < L / R, holds the
-// inputs j + t L / R (t < R) and produces outputs q = (j / NS) NS R + j % NS
-// + t' NS.  NF: columns run along j (rows contiguous in memory) rather than
-// along g.  loadw(g, n) returns input word n of sub-transform g;
-// out(g, q, v) takes output q as FP32.
-template <int DT, int L, int R, int NS, int G, bool NF, class LoadW, class Out>
-__device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const LoadW &loadw, const Out &out)
+// G sub-transforms of the workgroup.  Column (g, j), j < L / R, reads inputs
+// t < R at j + t L / R through loadw(g, j, t) (a packed word); output t' of
+// the column (FP32) is multiplied by prep(g, j, t') - complex when CX, else
+// by its real part - and handed to put(g, j, t', v).  Its Stockham position
+// is (j / NS) NS R + j % NS + t' NS.  NF: columns run along j (rows
+// contiguous in memory) rather than along g.  Outputs t' >= TH are not
+// produced.  Each wave walks its tiles software-pipelined: the next tile's
+// inputs and this tile's multipliers are requested before the MFMAs.
+template <int DT, int L, int R, int G, bool NF, int TH, bool CX, class LoadW, class Prep, class Put>
+__device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const LoadW &loadw, const Prep &prep,
+                                       const Put &put)
 {
     constexpr int LR = L / R;
     constexpr int C = G * LR;
     constexpr int NT = (C + 31) / 32;
     constexpr int KS = (2 * R + 15) / 16;       // K steps of 16 halves
-    constexpr int RT = (2 * R + 31) / 32;       // 32-row output tiles
+    constexpr int RT = (2 * TH + 31) / 32;      // 32-row output tiles needed
+    constexpr int RTA = (2 * R + 31) / 32;      // row tiles of the table
     const int lane = threadIdx.x & 63, h = lane >> 5, wave = threadIdx.x >> 6;
     // BF16: A = A_hi + A_lo (the second table follows the first), so the DFT
     // coefficients carry 16 significant bits
@@ -99,26 +139,54 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
 #pragma unroll
         for (int r = 0; r < RT; r++) {
 #pragma unroll
-            for (int k = 0; k < KS; k++) A[x][r][k] = amat[((x * RT + r) * KS + k) * 64 + lane];
+            for (int k = 0; k < KS; k++) A[x][r][k] = amat[((x * RTA + r) * KS + k) * 64 + lane];
         }
     }
-    for (int tile = wave; tile < NT; tile += MP_WAVES) {
+    auto column = [&](int tile, int &g, int &j) -> bool {
         const int c = tile * 32 + (lane & 31);
-        const bool cv = (C % 32 == 0) || c < C;
-        int g, j;
         if (NF) { g = c / LR; j = c - g * LR; }
         else { j = c / G; g = c - j * G; }
-        // B: lane (column, h) supplies inputs t = 8 k + 4 h + u, u < 4
-        uint4 B[KS];
+        return (C % 32 == 0) || c < C;
+    };
+    // B: lane (column, h) supplies inputs t = 8 k + 4 h + u, u < 4
+    auto load_b = [&](int g, int j, bool cv, uint4 *B) {
 #pragma unroll
         for (int k = 0; k < KS; k++) {
             uint32_t w[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int t = 8 * k + 4 * h + u;
-                w[u] = (cv && t < R) ? loadw(g, j + t * LR) : 0u;
+                w[u] = (cv && t < R) ? loadw(g, j, t) : 0u;
             }
             B[k] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    };
+    int tile = wave, g = 0, j = 0;
+    bool cv = column(tile, g, j);
+    uint4 B[KS];
+    if (tile < NT) load_b(g, j, cv, B);
+    for (; tile < NT; tile += MP_WAVES) {
+        const int tn = tile + MP_WAVES;
+        int gn = 0, jn = 0;
+        bool cvn = false;
+        uint4 Bn[KS];
+        if (tn < NT) {
+            cvn = column(tn, gn, jn);
+            load_b(gn, jn, cvn, Bn);
+        }
+        // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r + 4 m +
+        // 2 h + w, real (imaginary) part
+        float2 mul[RT][4][2];
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+#pragma unroll
+                for (int w = 0; w < 2; w++) {
+                    const int t = 16 * r + 4 * m + 2 * h + w;
+                    mul[r][m][w] = (cv && t < TH) ? prep(g, j, t) : make_float2(0.0f, 0.0f);
+                }
+            }
         }
         fx16 acc[RT];
 #pragma unroll
@@ -132,9 +200,6 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
             }
         }
         if (cv) {
-            const int qb = (j / NS) * NS * R + (j % NS);
-            // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r +
-            // 4 m + 2 h + w, real (imaginary) part
 #pragma unroll
             for (int r = 0; r < RT; r++) {
 #pragma unroll
@@ -142,35 +207,37 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
 #pragma unroll
                     for (int w = 0; w < 2; w++) {
                         const int t = 16 * r + 4 * m + 2 * h + w;
-                        if (t < R) out(g, qb + t * NS, make_float2(acc[r][4 * m + 2 * w], acc[r][4 * m + 2 * w + 1]));
+                        if (t < TH) {
+                            float2 v = make_float2(acc[r][4 * m + 2 * w], acc[r][4 * m + 2 * w + 1]);
+                            const float2 f = mul[r][m][w];
+                            v = CX ? cmul(v, f) : make_float2(v.x * f.x, v.y * f.x);
+                            put(g, j, t, v);
+                        }
                     }
                 }
             }
         }
+        if (tn < NT) {
+#pragma unroll
+            for (int k = 0; k < KS; k++) B[k] = Bn[k];
+            g = gn; j = jn; cv = cvn;
+        }
     }
-}
-
-// output q of a stage -> input of the next stage (radix RN, NSN = product of
-// the radices before it): times w_L^{t k L / (NSN RN)} with t = q / (L / RN),
-// k = (q mod L / RN) mod NSN, packed into LDS
-template <int DT, int L, int R0, int RN, int NSN>
-__device__ __forceinline__ void to_lds(uint32_t *buf, const float2 *tw, int g, int q, float2 v)
-{
-    constexpr int LRN = L / RN;
-    constexpr int TST = L / (NSN * RN);
-    const int tn = q / LRN, jn = q - tn * LRN;
-    const int k = jn % NSN;
-    v = cmul(v, tw[tn * k * TST]);
-    buf[g * MGeom<L, R0>::RP + MGeom<L, R0>::phi(q)] = pack_c<DT>(v.x, v.y);
 }
 
 template <int LM, int SM, int DT, int L, int R0, int R1, int R2, int G, bool NF>
 __global__ void __launch_bounds__(MP_THREADS) mpass_kernel(PassArgs a, int ngroups)
 {
     using Geo = MGeom<L, R0>;
+    constexpr int RP = Geo::RP, PD = Geo::PAD;
     constexpr bool S3 = R2 > 1;
-    __shared__ uint32_t buf0[G * Geo::RP];
-    __shared__ uint32_t buf1[S3 ? G * Geo::RP : 1];
+    constexpr int RL = S3 ? R2 : R1;          // last radix
+    constexpr int NSL = L / RL;               // its NS (= columns per sub-transform)
+    // power passes keep lags < M / 2 only: outputs t' >= ceil(RL / 2) of the
+    // last stage land at k >= L / 2 (lag >= M / 2) and are not produced
+    constexpr int THL = SM == ST_POW ? (RL + 1) / 2 : RL;
+    __shared__ uint32_t buf0[G * RP];
+    __shared__ uint32_t buf1[S3 ? G * RP : 1];
     __shared__ float2 tws[L];
     __shared__ uint32_t carr[LM == LD_MIX ? 256 : 1];
     __shared__ PeakPart red[MP_THREADS / 64];
@@ -187,62 +254,90 @@ __global__ void __launch_bounds__(MP_THREADS) mpass_kernel(PassArgs a, int ngrou
     // ---- stage 0: global -> MFMA -> LDS ----
     {
         const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
-        const uint32_t *xin = nullptr, *cin = nullptr;
+        const int tstride = (L / R0) * ins;   // input t of column j at j ins + t tstride
         Loader<LD_MIX> mx;
         if constexpr (LM == LD_MIX) mx.init(a, b, carr);
-        if constexpr (LM == LD_CPX) xin = a.hin + (long long)b * a.in_bstride;
+        long long xo = 0, co = 0;
+        if constexpr (LM == LD_CPX) xo = (long long)b * a.in_bstride;
         if constexpr (LM == LD_PROD) {
-            xin = a.hin + (long long)((b / a.prod_xdiv) % a.prod_xmod) * a.in_bstride;
-            cin = a.hcf + (long long)(b / a.prod_cdiv) * a.cf_bstride;
+            xo = (long long)((b / a.prod_xdiv) % a.prod_xmod) * a.in_bstride;
+            co = (long long)(b / a.prod_cdiv) * a.cf_bstride;
         }
-        auto ld = [&](int g, int n) -> uint32_t {
+        const __amdgpu_buffer_rsrc_t xin = rsrc(a.hin + xo), cin = rsrc(a.hcf + co);
+        auto ld = [&](int g, int j, int t) -> uint32_t {
             if (g >= gcount) return 0u;
-            const int e = (g0 + g) * igs + n * ins;
+            const int e = (g0 + g) * igs + j * ins + t * tstride;
             if constexpr (LM == LD_MIX) {
                 const int2 m = mx.mix_int(e);
                 return pack_c<DT>((float)m.x, (float)m.y);
             } else if constexpr (LM == LD_CPX) {
-                return xin[e];
+                return bload32(xin, e);
             } else {
-                const float2 p = cmul(unpack_c<DT>(xin[e]), unpack_c<DT>(cin[e]));
-                return pack_c<DT>(p.x, p.y);
+                return cmul_w<DT>(bload32(xin, e), bload32(cin, e));
             }
         };
-        auto st = [&](int g, int q, float2 v) { to_lds<DT, L, R0, R1, R0>(buf0, tws, g, q, v); };
-        mstage<DT, L, R0, 1, G, NF>(a.amat[0], ld, st);
+        // output t of column j sits at q = j R0 + t; the next stage's
+        // twiddle is w_L^{(j / R2) t R2} (w_L^{j t} with two stages)
+        auto tw0 = [&](int g, int j, int t) -> float2 { return tws[(j / R2) * t * R2]; };
+        auto st = [&](int g, int j, int t, float2 v) {
+            buf0[g * RP + j * (R0 + PD) + t] = pack_c<DT>(v.x, v.y);
+        };
+        mstage<DT, L, R0, G, NF, R0, true>(a.amat[0], ld, tw0, st);
     }
     __syncthreads();
 
     // ---- last stage: LDS -> MFMA -> global ----
     Storer<ST_POW> sp;
     if constexpr (SM == ST_POW) sp.init(a, b);
-    uint32_t *hout = nullptr;
-    if constexpr (SM == ST_CPX) hout = a.hout + (long long)b * a.out_bstride;
-    const float2 *tf = a.tfull;
+    const __amdgpu_buffer_rsrc_t hout = rsrc(a.hout + (SM == ST_CPX ? (long long)b * a.out_bstride : 0LL));
+    const __amdgpu_buffer_rsrc_t tfr = rsrc(a.tfull);
+    const bool tf = a.tfull != nullptr;
     const float osc = a.osc, kap = a.kappa;
     const int ogs = (int)a.out_gstride, ons = (int)a.out_nstride;
-    auto outg = [&](int g, int k, float2 v) {
-        if (g >= gcount) return;
-        const int gg = g0 + g;
+    // output t of column j (< NSL) is element k = j + t NSL of its
+    // sub-transform; ST_CPX multiplies by the inter-pass twiddle (its table
+    // is laid out like the output, index e) or by osc, ST_POW by kappa
+    auto mulg = [&](int g, int j, int t) -> float2 {
         if constexpr (SM == ST_POW) {
-            sp.store(gg, k, make_float2(v.x * kap, v.y * kap));
+            return make_float2(kap, 0.0f);
         } else {
-            // the twiddle table is laid out like the output (index e)
-            const int e = gg * ogs + k * ons;
-            if (tf) v = cmul(v, tf[e]);
-            else v = make_float2(v.x * osc, v.y * osc);
-            hout[e] = pack_c<DT>(v.x, v.y);
+            if (!tf || g >= gcount) return make_float2(osc, 0.0f);
+            return bload_f2(tfr, (g0 + g) * ogs + (j + t * NSL) * ons);
         }
     };
-    auto ld0 = [&](int g, int n) -> uint32_t { return buf0[g * Geo::RP + Geo::phi(n)]; };
+    auto outg = [&](int g, int j, int t, float2 v) {
+        if (g >= gcount) return;
+        const int gg = g0 + g;
+        const int k = j + t * NSL;
+        if constexpr (SM == ST_POW) sp.store(gg, k, v);
+        else bstore32(hout, gg * ogs + k * ons, pack_c<DT>(v.x, v.y));
+    };
+    constexpr bool CXL = SM == ST_CPX;
+    // stage 1 input t of column j: n = j + t R0 R2 (LDS position n + n / R0)
+    auto ld0 = [&](int g, int j, int t) -> uint32_t {
+        return buf0[g * RP + j + t * R0 * R2 + (PD ? j / R0 + t * R2 : 0)];
+    };
     if constexpr (!S3) {
-        mstage<DT, L, R1, R0, G, NF>(a.amat[1], ld0, outg);
+        mstage<DT, L, R1, G, NF, THL, CXL>(a.amat[1], ld0, mulg, outg);
     } else {
-        auto st1 = [&](int g, int q, float2 v) { to_lds<DT, L, R0, R2, R0 * R1>(buf1, tws, g, q, v); };
-        mstage<DT, L, R1, R0, G, NF>(a.amat[1], ld0, st1);
+        // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
+        // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
+        auto tw1 = [&](int g, int j, int t) -> float2 {
+            const int jh = j / R0, jl = j - jh * R0;
+            return tws[jh * (jl + t * R0)];
+        };
+        auto st1 = [&](int g, int j, int t, float2 v) {
+            const int jh = j / R0, jl = j - jh * R0;
+            const int q = jh * R0 * R1 + jl + t * R0;
+            buf1[g * RP + q + (PD ? jh * R1 + t : 0)] = pack_c<DT>(v.x, v.y);
+        };
+        mstage<DT, L, R1, G, NF, R1, true>(a.amat[1], ld0, tw1, st1);
         __syncthreads();
-        auto ld1 = [&](int g, int n) -> uint32_t { return buf1[g * Geo::RP + Geo::phi(n)]; };
-        mstage<DT, L, R2, R0 * R1, G, NF>(a.amat[2], ld1, outg);
+        // stage 2 input t of column j: n = j + t R0 R1
+        auto ld1 = [&](int g, int j, int t) -> uint32_t {
+            return buf1[g * RP + j + t * R0 * R1 + (PD ? j / R0 + t * R1 : 0)];
+        };
+        mstage<DT, L, R2, G, NF, THL, CXL>(a.amat[2], ld1, mulg, outg);
     }
     if constexpr (SM == ST_POW) {
         if (a.p_part) {

@@ -828,7 +828,8 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         float2 *nat;
         PSA_CHECK(hipMalloc(&nat, spec * nprn));
         PSA_CHECK(hipMemcpy(nat, sigs[i].code_fft, spec * nprn, hipMemcpyHostToDevice));
-        if (!(flags & SDR_SEARCH_PLAIN) && (flags & (SDR_SEARCH_FP16 | SDR_SEARCH_BF16))) {
+        // the precision form's last pass skips lags >= M / 2
+        if (!(flags & SDR_SEARCH_PLAIN) && (flags & (SDR_SEARCH_FP16 | SDR_SEARCH_BF16)) && 2 * s.h.N <= M) {
             s.hp = hfft_plan(M, (flags & SDR_SEARCH_BF16) ? HDT_BF16 : HDT_F16);
         }
         if (flags & SDR_SEARCH_PLAIN) {
