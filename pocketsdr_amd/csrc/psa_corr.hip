@@ -247,65 +247,84 @@ struct TrkChDev {
     double fs, fi, fc_sig, T, fd0, drift, phi0, coff0;
 };
 
-// Epoch state of one (channel, epoch): mixing parameters and tap geometry.
+// Epoch state of one (channel, epoch): mixing parameters and the code offset
+// in samples (cs = coff fs, the origin of the tap geometry).
 struct TrkEpoch {
     int ix, nseg;
     uint32_t p0, p1, s;
     int pad;
+    double cs;
 };
 
 // Open-loop NCO / code advance, src/sdr_ch.c:694-703 with adj_coff
-// (674-689), in FP64, one thread per channel, sequential over epochs.  The
-// reference build fuses coff -= fd / fc * tau and phi += fc * tau into
-// fnmadd / fmadd; the same fused forms are written here explicitly.
-__global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int n_epoch, int N,
-                                 int buffN, const double *__restrict__ pos, int n_pos,
-                                 TrkEpoch *__restrict__ ep, int2 *__restrict__ jk)
+// (674-689), in FP64, one 64-thread workgroup per channel.  Per chunk of 64
+// epochs the lanes compute the epoch-local terms (tau, fc = fi + fd and
+// -fd / fc_sig) in parallel, lane 0 runs the order-dependent recurrence
+// (phase wrap, code-offset wrap) over the chunk, and the lanes then derive
+// the mixing parameters.  The reference build fuses coff -= fd / fc * tau
+// and phi += fc * tau into fnmadd / fmadd; the same fused forms are written
+// here explicitly.
+#define TRK_SCHUNK 64
+__global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *__restrict__ chs, int n_epoch,
+                                                              int N, int buffN, TrkEpoch *__restrict__ ep)
 {
 #pragma clang fp contract(off)
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_ch) return;
+    __shared__ double s_a[TRK_SCHUNK], s_tau[TRK_SCHUNK], s_fc[TRK_SCHUNK], s_coff[TRK_SCHUNK], s_phi[TRK_SCHUNK];
+    const int c = blockIdx.x, lane = threadIdx.x;
     const TrkChDev ch = chs[c];
     const double scale = 4294967296.0;
-    double time = 0.0, phi = ch.phi0, coff = ch.coff0;
-    for (int e = 0; e < n_epoch; e++) {
-        double t = (double)(e + 1) * ch.T;
-        double tau = t - time;
-        double fd = ch.fd0 + ch.drift * t;
-        double fc = ch.fi + fd;
-        coff = fma(-(fd / ch.fc_sig), tau, coff);
-        phi = fma(fc, tau, phi);
-        phi = phi - floor(phi);
-        time = t;
-        if (coff >= ch.T) coff = coff - ch.T;
-        else if (coff < 0.0) coff = coff + ch.T;
-        // sdr_mix_carr phase setup (src/sdr_func.c:776-778, 824-836)
-        int ix = (int)(((long long)e * N) % buffN);
-        double step = fc / ch.fs;
-        TrkEpoch q;
-        q.ix = ix;
-        q.p0 = (uint32_t)((phi - floor(phi)) * scale);
-        q.s = (uint32_t)(int)(step * scale);
-        if (ix + N <= buffN) {
-            q.nseg = N;
-            q.p1 = 0;
-        } else {
-            int n = buffN - ix;
-            double phi2 = fma((double)n, step, phi);
-            q.nseg = n;
-            q.p1 = (uint32_t)((phi2 - floor(phi2)) * scale);
+    double phi = ch.phi0, coff = ch.coff0;    // lane 0's running state
+    for (int e0 = 0; e0 < n_epoch; e0 += TRK_SCHUNK) {
+        const int e = e0 + lane;
+        double fc = 0.0;
+        if (e < n_epoch) {
+            // t of this epoch and of the previous one ((double)e T, 0 at e = 0)
+            const double t = (double)(e + 1) * ch.T;
+            const double time = (double)e * ch.T;
+            const double fd = ch.fd0 + ch.drift * t;
+            fc = ch.fi + fd;
+            s_a[lane] = -(fd / ch.fc_sig);
+            s_tau[lane] = t - time;
+            s_fc[lane] = fc;
         }
-        q.pad = 0;
-        long long o = (long long)c * n_epoch + e;
-        ep[o] = q;
-        // sdr_corr_std tap geometry (src/sdr_func.c:988-991)
-        double cs = coff * ch.fs;
-        for (int i = 0; i < n_pos; i++) {
-            double p = cs + pos[i];
-            int j = (int)floor(p), k = (int)((p - j) * PSA_N_CODES);
-            if (j < 0) j += N; else if (j >= N) j -= N;
-            jk[o * n_pos + i] = make_int2(j, k);
+        __syncthreads();
+        if (lane == 0) {
+            const int ne = min(TRK_SCHUNK, n_epoch - e0);
+            for (int i = 0; i < ne; i++) {
+                const double tau = s_tau[i];
+                coff = fma(s_a[i], tau, coff);
+                phi = fma(s_fc[i], tau, phi);
+                phi = phi - floor(phi);
+                if (coff >= ch.T) coff = coff - ch.T;
+                else if (coff < 0.0) coff = coff + ch.T;
+                s_coff[i] = coff;
+                s_phi[i] = phi;
+            }
         }
+        __syncthreads();
+        if (e < n_epoch) {
+            const double ph = s_phi[lane];
+            // sdr_mix_carr phase setup (src/sdr_func.c:776-778, 824-836)
+            const int ix = (int)(((long long)e * N) % buffN);
+            const double step = fc / ch.fs;
+            TrkEpoch q;
+            q.ix = ix;
+            q.p0 = (uint32_t)((ph - floor(ph)) * scale);
+            q.s = (uint32_t)(int)(step * scale);
+            if (ix + N <= buffN) {
+                q.nseg = N;
+                q.p1 = 0;
+            } else {
+                const int n = buffN - ix;
+                const double phi2 = fma((double)n, step, ph);
+                q.nseg = n;
+                q.p1 = (uint32_t)((phi2 - floor(phi2)) * scale);
+            }
+            q.pad = 0;
+            q.cs = s_coff[lane] * ch.fs;
+            ep[(long long)c * n_epoch + e] = q;
+        }
+        __syncthreads();
     }
 }
 
@@ -324,7 +343,7 @@ __global__ void trk_state_kernel(const TrkChDev *__restrict__ chs, int n_ch, int
 template <bool EXACT, bool SAMEQ, int QPT>
 __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
-    const TrkEpoch *__restrict__ ep, const int2 *__restrict__ jk,
+    const TrkEpoch *__restrict__ ep, const double *__restrict__ pos,
     const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride,
     int NP,  // padded bank length in bytes (multiple of 4)
     const uint32_t *__restrict__ carr_g,
@@ -334,6 +353,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     __shared__ uint32_t carr[256];
     __shared__ int sums[PSA_MAX_CORR][4];
     __shared__ float4 c12s[PSA_MAX_CORR];
+    __shared__ int2 jk[PSA_MAX_CORR];
     const int Nq = (N + 3) / 4;
     uint32_t *planeI = lds, *planeQ = lds + Nq;
     const long long o = blockIdx.x;          // channel * n_epoch + epoch
@@ -341,6 +361,15 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const TrkEpoch q = ep[o];
     for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
     for (int i = threadIdx.x; i < n_pos * 4; i += TRK_THREADS) (&sums[0][0])[i] = 0;
+    // sdr_corr_std tap geometry (src/sdr_func.c:988-991)
+    for (int i = threadIdx.x; i < n_pos; i += TRK_THREADS) {
+        const double p = q.cs + pos[i];
+        int j = (int)floor(p);
+        const int k = (int)((p - j) * PSA_N_CODES);
+        if (j < 0) j += N;
+        else if (j >= N) j -= N;
+        jk[i] = make_int2(j, k);
+    }
     __syncthreads();
 
     // ---- mix N samples into the I / Q planes (zero padded to Nq words) ----
@@ -393,7 +422,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
             acc[tt][0] = acc[tt][1] = acc[tt][2] = acc[tt][3] = 0;
             const int t = t0 + tt;
             if (t >= n_pos) continue;
-            const int2 g = jk[o * n_pos + t];
+            const int2 g = jk[t];
             const int j = g.x, kb = g.y * NP;
             const int cp2 = (-j) & 3, cp1 = (N - j) & 3;
             const uint32_t *p2 = cbase + cp2 * cstride + (kb >> 2) + (-j - cp2) / 4;
@@ -456,7 +485,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
 
     // ---- per-tap float epilogue: S_vec = S_total - tail products ----
     for (int t = threadIdx.x; t < n_pos; t += blockDim.x) {
-        const int2 g = jk[o * n_pos + t];
+        const int2 g = jk[t];
         const int j = g.x, k = g.y;
         const int V1 = j >= 16 ? j / 16 * 16 : 0;
         const int len2 = N - j;
@@ -529,7 +558,6 @@ struct sdr_trk_plan_tag {
     TrkChDev *d_ch = nullptr;
     double *d_pos = nullptr;
     TrkEpoch *d_ep = nullptr;
-    int2 *d_jk = nullptr;
     float2 *d_corr = nullptr, *d_C = nullptr;
     hipStream_t st = nullptr;
     bool timing = false;
@@ -671,7 +699,6 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
     PSA_CHECK(hipMemcpy(p->d_pos, pos, sizeof(double) * n_pos, hipMemcpyHostToDevice));
     const size_t ne = (size_t)n_ch * n_epoch;
     PSA_CHECK(hipMalloc(&p->d_ep, sizeof(TrkEpoch) * ne));
-    PSA_CHECK(hipMalloc(&p->d_jk, sizeof(int2) * ne * n_pos));
     PSA_CHECK(hipMalloc(&p->d_corr, sizeof(float2) * ne * n_pos));
     PSA_CHECK(hipMalloc(&p->d_C, sizeof(float2) * ne * 2));
     for (auto &e : p->ev) PSA_CHECK(hipEventCreate(&e));
@@ -693,7 +720,7 @@ static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     });
     hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ, QPT>), dim3(ne), dim3(TRK_THREADS), shm, p->st,
-                       p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_jk,
+                       p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
                        p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
                        dev_carrier(), p->d_corr, p->d_C);
 }
@@ -730,9 +757,8 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
     hipStream_t st = p->st;
     const size_t ne = (size_t)p->n_ch * p->n_epoch;
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[0], st));
-    hipLaunchKernelGGL(trk_state_kernel, dim3((p->n_ch + 63) / 64), dim3(64), 0, st,
-                       p->d_ch, p->n_ch, p->n_epoch, p->N, p->buffN, p->d_pos, p->n_pos,
-                       p->d_ep, p->d_jk);
+    hipLaunchKernelGGL(trk_state_kernel, dim3(p->n_ch), dim3(TRK_SCHUNK), 0, st, p->d_ch, p->n_epoch, p->N,
+                       p->buffN, p->d_ep);
     PSA_CHECK(hipGetLastError());
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
     const size_t shm = sizeof(uint32_t) * 2 * (size_t)((p->N + 3) / 4);
@@ -792,7 +818,7 @@ void sdr_trk_plan_free(sdr_trk_plan_t *p)
     if (!p) return;
     (void)hipStreamSynchronize(p->st);
     (void)hipFree(p->d_buf); (void)hipFree(p->d_codes); (void)hipFree(p->d_ch);
-    (void)hipFree(p->d_pos); (void)hipFree(p->d_ep); (void)hipFree(p->d_jk);
+    (void)hipFree(p->d_pos); (void)hipFree(p->d_ep);
     (void)hipFree(p->d_corr); (void)hipFree(p->d_C);
     for (auto &e : p->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(p->st);
