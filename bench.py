@@ -50,11 +50,11 @@ SPEC_G = {150: 16, 160: 16, 200: 16, 240: 16, 300: 16, 320: 12, 400: 16, 480: 8,
 MODES = {"fwd_col": (2, 0), "fwd_row": (0, 0), "inv_row": (3, 0), "inv_col": (0, 1)}
 
 
-# precision-form passes: (L, R0, R1, R2, G) of the column and row lengths
-HCOL = {150: (10, 15, 1, 32), 200: (10, 20, 1, 32), 300: (15, 20, 1, 32), 400: (16, 25, 1, 32),
-        960: (32, 30, 1, 16)}
-HROW = {160: (16, 10, 1, 32), 240: (16, 15, 1, 32), 320: (16, 20, 1, 16), 480: (16, 30, 1, 16),
-        1000: (10, 10, 10, 4)}
+# precision-form passes: (R0, R1, R2, G, threads) of the column and row lengths
+HCOL = {150: (10, 15, 1, 32, 256), 200: (10, 20, 1, 32, 512), 300: (15, 20, 1, 32, 512),
+        400: (16, 25, 1, 32, 512), 960: (32, 30, 1, 16, 512)}
+HROW = {160: (16, 10, 1, 32, 256), 240: (16, 15, 1, 32, 256), 320: (16, 20, 1, 16, 256),
+        480: (16, 30, 1, 16, 512), 1000: (10, 10, 10, 4, 256)}
 
 
 def kernel_instance(cls, M, form="fp32"):
@@ -65,14 +65,17 @@ def kernel_instance(cls, M, form="fp32"):
     if cls not in MODES:
         return cls
     if form in ("fp16", "bf16"):
-        for L1, (a, b, c, g) in HCOL.items():
-            if M % L1 == 0 and M // L1 in HROW:
-                col = cls in ("fwd_col", "inv_col")
-                L = L1 if col else M // L1
-                r0, r1, r2, G = HCOL[L] if col else HROW[L]
-                lm, sm = MODES[cls]
-                return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s>" % (
-                    lm, sm, 0 if form == "fp16" else 1, L, r0, r1, r2, G, "false" if col else "true")
+        # the library's choice: smallest largest radix, then the shorter column
+        cands = [(max(HCOL[L1][:3] + HROW[M // L1][:3]) * 4096 + L1, L1)
+                 for L1 in HCOL if M % L1 == 0 and M // L1 in HROW]
+        if cands:
+            L1 = min(cands)[1]
+            col = cls in ("fwd_col", "inv_col")
+            L = L1 if col else M // L1
+            r0, r1, r2, G, nth = HCOL[L] if col else HROW[L]
+            lm, sm = MODES[cls]
+            return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s, %d>" % (
+                lm, sm, 0 if form == "fp16" else 1, L, r0, r1, r2, G, "false" if col else "true", nth)
     M1, M2 = M, 1
     if M > 4096:
         cands = [(0 if (M // d) % 16 == 0 else 1, abs(d - M // d), d) for d in range(1, 4097)

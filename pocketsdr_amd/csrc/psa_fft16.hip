@@ -14,6 +14,8 @@
 // R outputs of each column.  The next stage's twiddle is applied to the FP32
 // accumulators before they are packed into LDS.
 #include <math.h>
+#include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -25,8 +27,6 @@ typedef _Float16 hx8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bx8 __attribute__((ext_vector_type(8)));
 typedef float fx16 __attribute__((ext_vector_type(16)));
 
-#define MP_THREADS 256
-#define MP_WAVES (MP_THREADS / 64)
 
 typedef _Float16 hx2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bx2 __attribute__((ext_vector_type(2)));
@@ -113,13 +113,13 @@ struct MGeom {
 // One Stockham radix-R stage (NS = product of the earlier radices) over the
 // G sub-transforms of the workgroup.  Column (g, j), j < L / R, reads inputs
 // t < R at j + t L / R through loadw(g, j, t) (a packed word); output t' of
-// the column (FP32) is multiplied by prep(g, j, t') - complex when CX, else
-// by its real part - and handed to put(g, j, t', v).  Its Stockham position
+// the column (FP32) is multiplied by prep(g, j, t') - complex for MULK 2, by
+// its real part for MULK 1, not at all for 0 - and handed to put(g, j, t', v).  Its Stockham position
 // is (j / NS) NS R + j % NS + t' NS.  NF: columns run along j (rows
 // contiguous in memory) rather than along g.  Outputs t' >= TH are not
 // produced.  Each wave walks its tiles software-pipelined: the next tile's
 // inputs and this tile's multipliers are requested before the MFMAs.
-template <int DT, int L, int R, int G, bool NF, int TH, bool CX, class LoadW, class Prep, class Put>
+template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, class LoadW, class Prep, class Put>
 __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const LoadW &loadw, const Prep &prep,
                                        const Put &put)
 {
@@ -165,8 +165,8 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
     bool cv = column(tile, g, j);
     uint4 B[KS];
     if (tile < NT) load_b(g, j, cv, B);
-    for (; tile < NT; tile += MP_WAVES) {
-        const int tn = tile + MP_WAVES;
+    for (; tile < NT; tile += NTH / 64) {
+        const int tn = tile + NTH / 64;
         int gn = 0, jn = 0;
         bool cvn = false;
         uint4 Bn[KS];
@@ -184,7 +184,7 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
 #pragma unroll
                 for (int w = 0; w < 2; w++) {
                     const int t = 16 * r + 4 * m + 2 * h + w;
-                    mul[r][m][w] = (cv && t < TH) ? prep(g, j, t) : make_float2(0.0f, 0.0f);
+                    if (MULK) mul[r][m][w] = (cv && t < TH) ? prep(g, j, t) : make_float2(0.0f, 0.0f);
                 }
             }
         }
@@ -210,7 +210,8 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
                         if (t < TH) {
                             float2 v = make_float2(acc[r][4 * m + 2 * w], acc[r][4 * m + 2 * w + 1]);
                             const float2 f = mul[r][m][w];
-                            v = CX ? cmul(v, f) : make_float2(v.x * f.x, v.y * f.x);
+                            if (MULK == 2) v = cmul(v, f);
+                            else if (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
                             put(g, j, t, v);
                         }
                     }
@@ -225,8 +226,8 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
     }
 }
 
-template <int LM, int SM, int DT, int L, int R0, int R1, int R2, int G, bool NF>
-__global__ void __launch_bounds__(MP_THREADS) mpass_kernel(PassArgs a, int ngroups)
+template <int LM, int SM, int DT, int L, int R0, int R1, int R2, int G, bool NF, int NTH>
+__global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
 {
     using Geo = MGeom<L, R0>;
     constexpr int RP = Geo::RP, PD = Geo::PAD;
@@ -240,14 +241,14 @@ __global__ void __launch_bounds__(MP_THREADS) mpass_kernel(PassArgs a, int ngrou
     __shared__ uint32_t buf1[S3 ? G * RP : 1];
     __shared__ float2 tws[L];
     __shared__ uint32_t carr[LM == LD_MIX ? 256 : 1];
-    __shared__ PeakPart red[MP_THREADS / 64];
+    __shared__ PeakPart red[NTH / 64];
     int b, grp;
     if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp)) return;
     const int g0 = grp * G;
     const int gcount = min(G, a.nsub - g0);
-    for (int i = threadIdx.x; i < L; i += MP_THREADS) tws[i] = a.tw[i];
+    for (int i = threadIdx.x; i < L; i += NTH) tws[i] = a.tw[i];
     if constexpr (LM == LD_MIX) {
-        for (int i = threadIdx.x; i < 256; i += MP_THREADS) carr[i] = a.carr[i];
+        for (int i = threadIdx.x; i < 256; i += NTH) carr[i] = a.carr[i];
     }
     __syncthreads();
 
@@ -282,7 +283,7 @@ __global__ void __launch_bounds__(MP_THREADS) mpass_kernel(PassArgs a, int ngrou
         auto st = [&](int g, int j, int t, float2 v) {
             buf0[g * RP + j * (R0 + PD) + t] = pack_c<DT>(v.x, v.y);
         };
-        mstage<DT, L, R0, G, NF, R0, true>(a.amat[0], ld, tw0, st);
+        mstage<DT, NTH, L, R0, G, NF, R0, 2>(a.amat[0], ld, tw0, st);
     }
     __syncthreads();
 
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(MP_THREADS) mpass_kernel(PassArgs a, int ngrou
         return buf0[g * RP + j + t * R0 * R2 + (PD ? j / R0 + t * R2 : 0)];
     };
     if constexpr (!S3) {
-        mstage<DT, L, R1, G, NF, THL, CXL>(a.amat[1], ld0, mulg, outg);
+        mstage<DT, NTH, L, R1, G, NF, THL, CXL ? 2 : 1>(a.amat[1], ld0, mulg, outg);
     } else {
         // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
         // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
@@ -331,13 +332,13 @@ __global__ void __launch_bounds__(MP_THREADS) mpass_kernel(PassArgs a, int ngrou
             const int q = jh * R0 * R1 + jl + t * R0;
             buf1[g * RP + q + (PD ? jh * R1 + t : 0)] = pack_c<DT>(v.x, v.y);
         };
-        mstage<DT, L, R1, G, NF, R1, true>(a.amat[1], ld0, tw1, st1);
+        mstage<DT, NTH, L, R1, G, NF, R1, 2>(a.amat[1], ld0, tw1, st1);
         __syncthreads();
         // stage 2 input t of column j: n = j + t R0 R1
         auto ld1 = [&](int g, int j, int t) -> uint32_t {
             return buf1[g * RP + j + t * R0 * R1 + (PD ? j / R0 + t * R1 : 0)];
         };
-        mstage<DT, L, R2, G, NF, THL, CXL>(a.amat[2], ld1, mulg, outg);
+        mstage<DT, NTH, L, R2, G, NF, THL, CXL ? 2 : 1>(a.amat[2], ld1, mulg, outg);
     }
     if constexpr (SM == ST_POW) {
         if (a.p_part) {
@@ -400,19 +401,19 @@ __global__ void h_spec_kernel(const float2 *__restrict__ in, uint32_t *__restric
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// (L, R0, R1, R2, G): column-pass lengths (G columns = 128 bytes of words per
+// (L, R0, R1, R2, G, threads): column-pass lengths (G columns = 128 bytes of words per
 // row segment; 16 for 960 to fit two workgroups per CU) and row-pass lengths
 // (G rows, about 4-8 K points per workgroup, G L / R a multiple of 32 where
 // possible).  Splits: 24000 = 150 x 160, 48000 = 200 x 240, 96000 = 300 x 320,
 // 192000 = 400 x 480, 960000 = 960 x 1000.
-#define PSA_HCOL(X) X(150, 10, 15, 1, 32) X(200, 10, 20, 1, 32) X(300, 15, 20, 1, 32) \
-    X(400, 16, 25, 1, 32) X(960, 32, 30, 1, 16)
-#define PSA_HROW(X) X(160, 16, 10, 1, 32) X(240, 16, 15, 1, 32) X(320, 16, 20, 1, 16) \
-    X(480, 16, 30, 1, 16) X(1000, 10, 10, 10, 4)
+#define PSA_HCOL(X) X(150, 10, 15, 1, 32, 256) X(200, 10, 20, 1, 32, 512) X(300, 15, 20, 1, 32, 512) \
+    X(400, 16, 25, 1, 32, 512) X(960, 32, 30, 1, 16, 512)
+#define PSA_HROW(X) X(160, 16, 10, 1, 32, 256) X(240, 16, 15, 1, 32, 256) X(320, 16, 20, 1, 16, 256) \
+    X(480, 16, 30, 1, 16, 512) X(1000, 10, 10, 10, 4, 256)
 
 struct HSpec { int L, r[3], G; };
 static const HSpec h_col_specs[] = {
-#define PSA_S(L_, A_, B_, C_, G_) {L_, {A_, B_, C_}, G_},
+#define PSA_S(L_, A_, B_, C_, G_, T_) {L_, {A_, B_, C_}, G_},
     PSA_HCOL(PSA_S)
 };
 static const HSpec h_row_specs[] = {
@@ -428,6 +429,7 @@ static const HSpec *find_spec(const HSpec *t, int n, int L)
 static const HSpec *col_spec(int L) { return find_spec(h_col_specs, (int)(sizeof(h_col_specs) / sizeof(HSpec)), L); }
 static const HSpec *row_spec(int L) { return find_spec(h_row_specs, (int)(sizeof(h_row_specs) / sizeof(HSpec)), L); }
 
+// A-operand table of the R-point DFT (BF16: hi part, then lo part)
 static const uint4 *make_amat(int R, int dir, int dt)
 {
     static std::mutex mtx;
@@ -437,7 +439,7 @@ static const uint4 *make_amat(int R, int dir, int dt)
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     const int KS = (2 * R + 15) / 16, RT = (2 * R + 31) / 32;
-    const int n = 64 * KS * RT;         // uint4 per table
+    const int n = 64 * KS * RT;         // uint4 per part
     uint4 *d;
     PSA_CHECK(hipMalloc(&d, sizeof(uint4) * n * (dt == HDT_BF16 ? 2 : 1)));
     if (dt == HDT_F16) {
@@ -482,9 +484,16 @@ const HPlan *hfft_plan(int M, int dt)
     static std::mutex mtx;
     static std::map<int, HPlan *> plans;
     if (dt != HDT_F16 && dt != HDT_BF16) return nullptr;
-    int M1 = 0, M2 = 0;
+    // among the splits with specialised passes: the smallest largest radix
+    // (fewest padded MFMA rows and registers), then the shorter column pass
+    int M1 = 0, M2 = 0, best = 1 << 30;
     for (const HSpec &c : h_col_specs) {
-        if (M % c.L == 0 && row_spec(M / c.L)) { M1 = c.L; M2 = M / c.L; break; }
+        const HSpec *r = M % c.L == 0 ? row_spec(M / c.L) : nullptr;
+        if (!r) continue;
+        int mr = 0;
+        for (int i = 0; i < 3; i++) mr = std::max(mr, std::max(c.r[i], r->r[i]));
+        const int score = mr * 4096 + c.L;
+        if (score < best) { best = score; M1 = c.L; M2 = r->L; }
     }
     if (!M1) return nullptr;
     ensure_init();
@@ -550,7 +559,7 @@ void h_row_args(const HPlan *hp, int dir, PassArgs &a)
     for (int s = 0; s < 3; s++) a.amat[s] = hp->am2[dir][s];
 }
 
-template <int LM, int SM, int DT, int L, int R0, int R1, int R2, int G, bool NF>
+template <int LM, int SM, int DT, int L, int R0, int R1, int R2, int G, bool NF, int NTH>
 static void hlaunch_t(const PassArgs &a, int nbatch, hipStream_t st)
 {
     const int ngroups = (a.nsub + G - 1) / G;
@@ -561,7 +570,7 @@ static void hlaunch_t(const PassArgs &a, int nbatch, hipStream_t st)
     }
     if (nb <= 0) return;
     if (nb > 0x7FFFFFFFLL) PSA_FAIL("FFT launch too large (%lld workgroups)", nb);
-    hipLaunchKernelGGL((mpass_kernel<LM, SM, DT, L, R0, R1, R2, G, NF>), dim3((unsigned)nb), dim3(MP_THREADS), 0,
+    hipLaunchKernelGGL((mpass_kernel<LM, SM, DT, L, R0, R1, R2, G, NF, NTH>), dim3((unsigned)nb), dim3(NTH), 0,
                        st, a, ngroups);
     PSA_CHECK(hipGetLastError());
 }
@@ -570,18 +579,18 @@ template <int DT>
 static void hlaunch_dt(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch, hipStream_t st)
 {
     // column lengths: forward (mixing) and inverse (power) column passes
-#define PSA_C(L_, A_, B_, C_, G_)                                                          \
+#define PSA_C(L_, A_, B_, C_, G_, T_)                                                          \
     if (a.L == L_ && a.in_nstride != 1) {                                                  \
-        if (lm == LD_MIX && sm == ST_CPX) { hlaunch_t<LD_MIX, ST_CPX, DT, L_, A_, B_, C_, G_, false>(a, nbatch, st); return; } \
-        if (lm == LD_CPX && sm == ST_POW) { hlaunch_t<LD_CPX, ST_POW, DT, L_, A_, B_, C_, G_, false>(a, nbatch, st); return; } \
+        if (lm == LD_MIX && sm == ST_CPX) { hlaunch_t<LD_MIX, ST_CPX, DT, L_, A_, B_, C_, G_, false, T_>(a, nbatch, st); return; } \
+        if (lm == LD_CPX && sm == ST_POW) { hlaunch_t<LD_CPX, ST_POW, DT, L_, A_, B_, C_, G_, false, T_>(a, nbatch, st); return; } \
     }
     PSA_HCOL(PSA_C)
 #undef PSA_C
     // row lengths: forward row pass and inverse (code product) row pass
-#define PSA_R(L_, A_, B_, C_, G_)                                                          \
+#define PSA_R(L_, A_, B_, C_, G_, T_)                                                          \
     if (a.L == L_ && a.in_nstride == 1) {                                                  \
-        if (lm == LD_CPX && sm == ST_CPX) { hlaunch_t<LD_CPX, ST_CPX, DT, L_, A_, B_, C_, G_, true>(a, nbatch, st); return; } \
-        if (lm == LD_PROD && sm == ST_CPX) { hlaunch_t<LD_PROD, ST_CPX, DT, L_, A_, B_, C_, G_, true>(a, nbatch, st); return; } \
+        if (lm == LD_CPX && sm == ST_CPX) { hlaunch_t<LD_CPX, ST_CPX, DT, L_, A_, B_, C_, G_, true, T_>(a, nbatch, st); return; } \
+        if (lm == LD_PROD && sm == ST_CPX) { hlaunch_t<LD_PROD, ST_CPX, DT, L_, A_, B_, C_, G_, true, T_>(a, nbatch, st); return; } \
     }
     PSA_HROW(PSA_R)
 #undef PSA_R

@@ -111,8 +111,8 @@ struct Storer {
     int nlag = 0, ogs = 0, ons = 0;
     bool pinit = false, track = false;
     long long ibase = 0;            // row-major index of lag 0 of this row
-    float bm = 0.0f;                // peak tracking of the values stored
-    long long bi = -1;
+    float bm = 0.0f;                // peak tracking of the values stored:
+    int be = -1;                    // lag of the maximum in this row (-1: none)
     double bs = 0.0;
 
     __device__ __forceinline__ void init(const PassArgs &a, int b)
@@ -149,17 +149,18 @@ struct Storer {
             prow[e] = nv;
             if (track) {
                 bs += (double)nv;
-                const long long gi = ibase + e;
-                if (nv > bm || (nv == bm && bi >= 0 && gi < bi)) {
-                    if (nv > 0.0f) { bm = nv; bi = gi; }
+                if (nv > bm || (nv == bm && be >= 0 && e < be)) {
+                    if (nv > 0.0f) { bm = nv; be = e; }
                 }
             }
         }
     }
 
-    // workgroup reduction of the tracked peak into part[slot]
+    // workgroup reduction of the tracked peak into part[slot]; red holds one
+    // record per wave
     __device__ __forceinline__ void flush_peak(PeakPart *part, long long slot, PeakPart *red)
     {
+        long long bi = be >= 0 ? ibase + be : -1;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
             const float om = __shfl_xor(bm, o, 64);
@@ -171,7 +172,7 @@ struct Storer {
         if ((threadIdx.x & 63) == 0) { red[w].m = bm; red[w].i = bi; red[w].s = bs; }
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int i = 1; i < PASS_THREADS / 64; i++) {
+            for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
                 bs += red[i].s;
                 const long long oi = red[i].i;
                 const float om = red[i].m;
