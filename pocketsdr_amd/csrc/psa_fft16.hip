@@ -142,13 +142,19 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
             for (int k = 0; k < KS; k++) A[x][r][k] = amat[((x * RTA + r) * KS + k) * 64 + lane];
         }
     }
+    // column of this lane in a tile; past the last column the indices are
+    // clamped to a valid one (the lane's results are then discarded)
     auto column = [&](int tile, int &g, int &j) -> bool {
         const int c = tile * 32 + (lane & 31);
-        if (NF) { g = c / LR; j = c - g * LR; }
-        else { j = c / G; g = c - j * G; }
-        return (C % 32 == 0) || c < C;
+        const bool ok = (C % 32 == 0) || c < C;
+        const int cc = ok ? c : C - 1;
+        if (NF) { g = cc / LR; j = cc - g * LR; }
+        else { j = cc / G; g = cc - j * G; }
+        return ok;
     };
-    // B: lane (column, h) supplies inputs t = 8 k + 4 h + u, u < 4
+    // B: lane (column, h) supplies inputs t = 8 k + 4 h + u, u < 4.  Every
+    // load is issued (inputs t >= R read t = R - 1 and are zeroed after), so
+    // no load waits inside a branch.
     auto load_b = [&](int g, int j, bool cv, uint4 *B) {
 #pragma unroll
         for (int k = 0; k < KS; k++) {
@@ -156,7 +162,8 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int t = 8 * k + 4 * h + u;
-                w[u] = (cv && t < R) ? loadw(g, j, t) : 0u;
+                const uint32_t v = loadw(g, j, t < R ? t : R - 1);
+                w[u] = (cv && t < R) ? v : 0u;
             }
             B[k] = make_uint4(w[0], w[1], w[2], w[3]);
         }
@@ -184,7 +191,7 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
 #pragma unroll
                 for (int w = 0; w < 2; w++) {
                     const int t = 16 * r + 4 * m + 2 * h + w;
-                    if (MULK) mul[r][m][w] = (cv && t < TH) ? prep(g, j, t) : make_float2(0.0f, 0.0f);
+                    if (MULK) mul[r][m][w] = prep(g, j, t < TH ? t : TH - 1);
                 }
             }
         }
@@ -265,17 +272,21 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             co = (long long)(b / a.prod_cdiv) * a.cf_bstride;
         }
         const __amdgpu_buffer_rsrc_t xin = rsrc(a.hin + xo), cin = rsrc(a.hcf + co);
+        // rows past the end of the batch item (last group) read row 0 and
+        // are zeroed, without a branch around the loads
         auto ld = [&](int g, int j, int t) -> uint32_t {
-            if (g >= gcount) return 0u;
-            const int e = (g0 + g) * igs + j * ins + t * tstride;
+            const bool ok = g < gcount;
+            const int e = (g0 + (ok ? g : 0)) * igs + j * ins + t * tstride;
+            uint32_t v;
             if constexpr (LM == LD_MIX) {
                 const int2 m = mx.mix_int(e);
-                return pack_c<DT>((float)m.x, (float)m.y);
+                v = pack_c<DT>((float)m.x, (float)m.y);
             } else if constexpr (LM == LD_CPX) {
-                return bload32(xin, e);
+                v = bload32(xin, e);
             } else {
-                return cmul_w<DT>(bload32(xin, e), bload32(cin, e));
+                v = cmul_w<DT>(bload32(xin, e), bload32(cin, e));
             }
+            return ok ? v : 0u;
         };
         // output t of column j sits at q = j R0 + t; the next stage's
         // twiddle is w_L^{(j / R2) t R2} (w_L^{j t} with two stages)
@@ -302,8 +313,8 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         if constexpr (SM == ST_POW) {
             return make_float2(kap, 0.0f);
         } else {
-            if (!tf || g >= gcount) return make_float2(osc, 0.0f);
-            return bload_f2(tfr, (g0 + g) * ogs + (j + t * NSL) * ons);
+            if (!tf) return make_float2(osc, 0.0f);          // uniform
+            return bload_f2(tfr, (g0 + (g < gcount ? g : 0)) * ogs + (j + t * NSL) * ons);
         }
     };
     auto outg = [&](int g, int j, int t, float2 v) {
