@@ -114,7 +114,8 @@ struct MGeom {
 // G sub-transforms of the workgroup.  Column (g, j), j < L / R, reads inputs
 // t < R at j + t L / R through loadw(g, j, t) (a packed word); output t' of
 // the column (FP32) is multiplied by prep(g, j, t') - complex for MULK 2, by
-// its real part for MULK 1, not at all for 0 - and handed to put(g, j, t', v).  Its Stockham position
+// its real part for MULK 1 (the imaginary part then carries a value for put),
+// not at all for 0 - and handed to put(g, j, t', v, prep value).  Its Stockham position
 // is (j / NS) NS R + j % NS + t' NS.  NF: columns run along j (rows
 // contiguous in memory) rather than along g.  Outputs t' >= TH are not
 // produced.  Each wave walks its tiles software-pipelined: the next tile's
@@ -219,7 +220,7 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
                             const float2 f = mul[r][m][w];
                             if (MULK == 2) v = cmul(v, f);
                             else if (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
-                            put(g, j, t, v);
+                            put(g, j, t, v, f);
                         }
                     }
                 }
@@ -291,7 +292,7 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         // output t of column j sits at q = j R0 + t; the next stage's
         // twiddle is w_L^{(j / R2) t R2} (w_L^{j t} with two stages)
         auto tw0 = [&](int g, int j, int t) -> float2 { return tws[(j / R2) * t * R2]; };
-        auto st = [&](int g, int j, int t, float2 v) {
+        auto st = [&](int g, int j, int t, float2 v, float2) {
             buf0[g * RP + j * (R0 + PD) + t] = pack_c<DT>(v.x, v.y);
         };
         mstage<DT, NTH, L, R0, G, NF, R0, 2>(a.amat[0], ld, tw0, st);
@@ -307,22 +308,27 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     const float osc = a.osc, kap = a.kappa;
     const int ogs = (int)a.out_gstride, ons = (int)a.out_nstride;
     // output t of column j (< NSL) is element k = j + t NSL of its
-    // sub-transform; ST_CPX multiplies by the inter-pass twiddle (its table
-    // is laid out like the output, index e) or by osc, ST_POW by kappa
+    // sub-transform, at e = gg ogs + k ons.  ST_CPX multiplies by the
+    // inter-pass twiddle (its table is laid out like the output) or by osc;
+    // ST_POW by kappa, and the previous P value rides along in .y (fetched
+    // with the tile's other loads; 0 on the first block)
+    const bool pacc = SM == ST_POW && !a.p_init;
+    const float *prow = sp.prow;
+    const int nlag = a.nlag;
     auto mulg = [&](int g, int j, int t) -> float2 {
+        const int e = (g0 + (g < gcount ? g : 0)) * ogs + (j + t * NSL) * ons;
         if constexpr (SM == ST_POW) {
-            return make_float2(kap, 0.0f);
+            return make_float2(kap, pacc ? prow[e < nlag ? e : 0] : 0.0f);
         } else {
             if (!tf) return make_float2(osc, 0.0f);          // uniform
-            return bload_f2(tfr, (g0 + (g < gcount ? g : 0)) * ogs + (j + t * NSL) * ons);
+            return bload_f2(tfr, e);
         }
     };
-    auto outg = [&](int g, int j, int t, float2 v) {
+    auto outg = [&](int g, int j, int t, float2 v, float2 f) {
         if (g >= gcount) return;
-        const int gg = g0 + g;
-        const int k = j + t * NSL;
-        if constexpr (SM == ST_POW) sp.store(gg, k, v);
-        else bstore32(hout, gg * ogs + k * ons, pack_c<DT>(v.x, v.y));
+        const int e = (g0 + g) * ogs + (j + t * NSL) * ons;
+        if constexpr (SM == ST_POW) sp.store_pow(e, v, f.y);
+        else bstore32(hout, e, pack_c<DT>(v.x, v.y));
     };
     constexpr bool CXL = SM == ST_CPX;
     // stage 1 input t of column j: n = j + t R0 R2 (LDS position n + n / R0)
@@ -338,7 +344,7 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             const int jh = j / R0, jl = j - jh * R0;
             return tws[jh * (jl + t * R0)];
         };
-        auto st1 = [&](int g, int j, int t, float2 v) {
+        auto st1 = [&](int g, int j, int t, float2 v, float2) {
             const int jh = j / R0, jl = j - jh * R0;
             const int q = jh * R0 * R1 + jl + t * R0;
             buf1[g * RP + q + (PD ? jh * R1 + t : 0)] = pack_c<DT>(v.x, v.y);

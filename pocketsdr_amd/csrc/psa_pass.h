@@ -156,6 +156,22 @@ struct Storer {
         }
     }
 
+    // ST_POW at lag e given the previous P value (ignored on the first
+    // block); values at e >= nlag are dropped
+    __device__ __forceinline__ void store_pow(int e, float2 v, float pold)
+    {
+        if (e >= nlag) return;
+        const float pw = __builtin_fmaf(v.x, v.x, v.y * v.y);
+        const float nv = pinit ? pw : pold + pw;
+        prow[e] = nv;
+        if (track) {
+            bs += (double)nv;
+            if (nv > bm || (nv == bm && be >= 0 && e < be)) {
+                if (nv > 0.0f) { bm = nv; be = e; }
+            }
+        }
+    }
+
     // workgroup reduction of the tracked peak into part[slot]; red holds one
     // record per wave
     __device__ __forceinline__ void flush_peak(PeakPart *part, long long slot, PeakPart *red)
