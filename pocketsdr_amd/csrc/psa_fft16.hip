@@ -112,17 +112,23 @@ struct MGeom {
 
 // One Stockham radix-R stage (NS = product of the earlier radices) over the
 // G sub-transforms of the workgroup.  Column (g, j), j < L / R, reads inputs
-// t < R at j + t L / R through loadw(g, j, t) (a packed word); output t' of
-// the column (FP32) is multiplied by prep(g, j, t') - complex for MULK 2, by
-// its real part for MULK 1 (the imaginary part then carries a value for put),
-// not at all for 0 - and handed to put(g, j, t', v, prep value).  Its Stockham position
-// is (j / NS) NS R + j % NS + t' NS.  NF: columns run along j (rows
-// contiguous in memory) rather than along g.  Outputs t' >= TH are not
-// produced.  Each wave walks its tiles software-pipelined: the next tile's
-// inputs and this tile's multipliers are requested before the MFMAs.
-template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, class LoadW, class Prep, class Put>
-__device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const LoadW &loadw, const Prep &prep,
-                                       const Put &put)
+// t < R at j + t L / R; its output t' sits at the Stockham position
+// (j / NS) NS R + j % NS + t' NS.  The callers' functors split addressing
+// into a per-column part, computed once per tile, and a per-element part:
+//   ib = ibase(g, j)   (.x: base, .y: row valid), word = iat(ib.x, t);
+//   ob = obase(g, j)   (.x, .y, .z: bases and stride, .w: row valid),
+//   f = omul(ob, t')   multiplier - complex for MULK 2, its real part for
+//                      MULK 1 (.y then carries a value for oput), none for 0,
+//   oput(ob, t', v, f) stores output t' (FP32, multiplied).
+// NF: columns run along j (rows contiguous in memory) rather than along g.
+// Outputs t' >= TH are not produced.  Each wave walks its tiles
+// software-pipelined: the next tile's inputs and this tile's multipliers are
+// requested before the MFMAs; every load is issued unconditionally on
+// clamped indices and masked afterwards, so no load waits inside a branch.
+template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, class IBase, class IAt, class OBase,
+          class OMul, class OPut>
+__device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBase &ibase, const IAt &iat,
+                                       const OBase &obase, const OMul &omul, const OPut &oput)
 {
     constexpr int LR = L / R;
     constexpr int C = G * LR;
@@ -153,18 +159,19 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
         else { j = cc / G; g = cc - j * G; }
         return ok;
     };
-    // B: lane (column, h) supplies inputs t = 8 k + 4 h + u, u < 4.  Every
-    // load is issued (inputs t >= R read t = R - 1 and are zeroed after), so
-    // no load waits inside a branch.
+    // B: lane (column, h) supplies inputs t = 8 k + 4 h + u, u < 4 (inputs
+    // t >= R read t = R - 1 and are zeroed)
     auto load_b = [&](int g, int j, bool cv, uint4 *B) {
+        const int2 ib = ibase(g, j);
+        const bool ok = cv && ib.y;
 #pragma unroll
         for (int k = 0; k < KS; k++) {
             uint32_t w[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int t = 8 * k + 4 * h + u;
-                const uint32_t v = loadw(g, j, t < R ? t : R - 1);
-                w[u] = (cv && t < R) ? v : 0u;
+                const uint32_t v = iat(ib.x, t < R ? t : R - 1);
+                w[u] = (ok && t < R) ? v : 0u;
             }
             B[k] = make_uint4(w[0], w[1], w[2], w[3]);
         }
@@ -182,6 +189,7 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
             cvn = column(tn, gn, jn);
             load_b(gn, jn, cvn, Bn);
         }
+        const int4 ob = obase(g, j);
         // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r + 4 m +
         // 2 h + w, real (imaginary) part
         float2 mul[RT][4][2];
@@ -192,7 +200,7 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
 #pragma unroll
                 for (int w = 0; w < 2; w++) {
                     const int t = 16 * r + 4 * m + 2 * h + w;
-                    if (MULK) mul[r][m][w] = prep(g, j, t < TH ? t : TH - 1);
+                    if (MULK) mul[r][m][w] = omul(ob, t < TH ? t : TH - 1);
                 }
             }
         }
@@ -207,7 +215,7 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
                 for (int k = 0; k < KS; k++) acc[r] = mfma<DT>(A[x][r][k], B[k], acc[r]);
             }
         }
-        if (cv) {
+        if (cv && ob.w) {
 #pragma unroll
             for (int r = 0; r < RT; r++) {
 #pragma unroll
@@ -220,7 +228,7 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const Loa
                             const float2 f = mul[r][m][w];
                             if (MULK == 2) v = cmul(v, f);
                             else if (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
-                            put(g, j, t, v, f);
+                            oput(ob, t, v, f);
                         }
                     }
                 }
@@ -245,6 +253,8 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     // power passes keep lags < M / 2 only: outputs t' >= ceil(RL / 2) of the
     // last stage land at k >= L / 2 (lag >= M / 2) and are not produced
     constexpr int THL = SM == ST_POW ? (RL + 1) / 2 : RL;
+    // the forward column and inverse row passes apply the inter-pass twiddle
+    constexpr bool TW = SM == ST_CPX && LM != LD_CPX;
     __shared__ uint32_t buf0[G * RP];
     __shared__ uint32_t buf1[S3 ? G * RP : 1];
     __shared__ float2 tws[L];
@@ -260,6 +270,10 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     }
     __syncthreads();
 
+    // LDS stages: input t of column j of a later stage is element
+    // n = j + t LRs (LRs = L / R of that stage), at n + n / R0 (PD)
+    auto lds_base = [&](int g, int j) -> int2 { return make_int2(g * RP + j + (PD ? j / R0 : 0), 1); };
+
     // ---- stage 0: global -> MFMA -> LDS ----
     {
         const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
@@ -273,29 +287,28 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             co = (long long)(b / a.prod_cdiv) * a.cf_bstride;
         }
         const __amdgpu_buffer_rsrc_t xin = rsrc(a.hin + xo), cin = rsrc(a.hcf + co);
-        // rows past the end of the batch item (last group) read row 0 and
-        // are zeroed, without a branch around the loads
-        auto ld = [&](int g, int j, int t) -> uint32_t {
+        // rows past the end of the batch item (last group) read row 0
+        auto ibase = [&](int g, int j) -> int2 {
             const bool ok = g < gcount;
-            const int e = (g0 + (ok ? g : 0)) * igs + j * ins + t * tstride;
-            uint32_t v;
+            return make_int2((g0 + (ok ? g : 0)) * igs + j * ins, ok);
+        };
+        auto iat = [&](int base, int t) -> uint32_t {
+            const int e = base + t * tstride;
             if constexpr (LM == LD_MIX) {
                 const int2 m = mx.mix_int(e);
-                v = pack_c<DT>((float)m.x, (float)m.y);
+                return pack_c<DT>((float)m.x, (float)m.y);
             } else if constexpr (LM == LD_CPX) {
-                v = bload32(xin, e);
+                return bload32(xin, e);
             } else {
-                v = cmul_w<DT>(bload32(xin, e), bload32(cin, e));
+                return cmul_w<DT>(bload32(xin, e), bload32(cin, e));
             }
-            return ok ? v : 0u;
         };
-        // output t of column j sits at q = j R0 + t; the next stage's
-        // twiddle is w_L^{(j / R2) t R2} (w_L^{j t} with two stages)
-        auto tw0 = [&](int g, int j, int t) -> float2 { return tws[(j / R2) * t * R2]; };
-        auto st = [&](int g, int j, int t, float2 v, float2) {
-            buf0[g * RP + j * (R0 + PD) + t] = pack_c<DT>(v.x, v.y);
-        };
-        mstage<DT, NTH, L, R0, G, NF, R0, 2>(a.amat[0], ld, tw0, st);
+        // output t of column j sits at q = j R0 + t (LDS j (R0 + PD) + t); the
+        // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages)
+        auto obase = [&](int g, int j) -> int4 { return make_int4(g * RP + j * (R0 + PD), 0, (j / R2) * R2, 1); };
+        auto omul = [&](const int4 &ob, int t) -> float2 { return tws[t * ob.z]; };
+        auto oput = [&](const int4 &ob, int t, float2 v, float2) { buf0[ob.x + t] = pack_c<DT>(v.x, v.y); };
+        mstage<DT, NTH, L, R0, G, NF, R0, 2>(a.amat[0], ibase, iat, obase, omul, oput);
     }
     __syncthreads();
 
@@ -304,9 +317,9 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     if constexpr (SM == ST_POW) sp.init(a, b);
     const __amdgpu_buffer_rsrc_t hout = rsrc(a.hout + (SM == ST_CPX ? (long long)b * a.out_bstride : 0LL));
     const __amdgpu_buffer_rsrc_t tfr = rsrc(a.tfull);
-    const bool tf = a.tfull != nullptr;
     const float osc = a.osc, kap = a.kappa;
     const int ogs = (int)a.out_gstride, ons = (int)a.out_nstride;
+    const int estep = NSL * ons;
     // output t of column j (< NSL) is element k = j + t NSL of its
     // sub-transform, at e = gg ogs + k ons.  ST_CPX multiplies by the
     // inter-pass twiddle (its table is laid out like the output) or by osc;
@@ -315,47 +328,42 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     const bool pacc = SM == ST_POW && !a.p_init;
     const float *prow = sp.prow;
     const int nlag = a.nlag;
-    auto mulg = [&](int g, int j, int t) -> float2 {
-        const int e = (g0 + (g < gcount ? g : 0)) * ogs + (j + t * NSL) * ons;
-        if constexpr (SM == ST_POW) {
-            return make_float2(kap, pacc ? prow[e < nlag ? e : 0] : 0.0f);
-        } else {
-            if (!tf) return make_float2(osc, 0.0f);          // uniform
-            return bload_f2(tfr, e);
-        }
+    auto gbase = [&](int g, int j) -> int4 {
+        const bool ok = g < gcount;
+        return make_int4((g0 + (ok ? g : 0)) * ogs + j * ons, 0, 0, ok);
     };
-    auto outg = [&](int g, int j, int t, float2 v, float2 f) {
-        if (g >= gcount) return;
-        const int e = (g0 + g) * ogs + (j + t * NSL) * ons;
+    auto gmul = [&](const int4 &ob, int t) -> float2 {
+        const int e = ob.x + t * estep;
+        if constexpr (SM == ST_POW) return make_float2(kap, pacc ? prow[e < nlag ? e : 0] : 0.0f);
+        else if constexpr (TW) return bload_f2(tfr, e);
+        else return make_float2(osc, 0.0f);
+    };
+    auto gput = [&](const int4 &ob, int t, float2 v, float2 f) {
+        const int e = ob.x + t * estep;
         if constexpr (SM == ST_POW) sp.store_pow(e, v, f.y);
         else bstore32(hout, e, pack_c<DT>(v.x, v.y));
     };
-    constexpr bool CXL = SM == ST_CPX;
-    // stage 1 input t of column j: n = j + t R0 R2 (LDS position n + n / R0)
-    auto ld0 = [&](int g, int j, int t) -> uint32_t {
-        return buf0[g * RP + j + t * R0 * R2 + (PD ? j / R0 + t * R2 : 0)];
-    };
+    constexpr int MULKL = TW ? 2 : 1;
+    // stage 1 input t of column j: n = j + t R0 R2
+    auto iat0 = [&](int base, int t) -> uint32_t { return buf0[base + t * (R0 * R2 + PD * R2)]; };
     if constexpr (!S3) {
-        mstage<DT, NTH, L, R1, G, NF, THL, CXL ? 2 : 1>(a.amat[1], ld0, mulg, outg);
+        mstage<DT, NTH, L, R1, G, NF, THL, MULKL>(a.amat[1], lds_base, iat0, gbase, gmul, gput);
     } else {
         // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
         // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
-        auto tw1 = [&](int g, int j, int t) -> float2 {
+        auto obase1 = [&](int g, int j) -> int4 {
             const int jh = j / R0, jl = j - jh * R0;
-            return tws[jh * (jl + t * R0)];
+            return make_int4(g * RP + jh * R0 * R1 + jl + (PD ? jh * R1 : 0), jh * jl, jh * R0, 1);
         };
-        auto st1 = [&](int g, int j, int t, float2 v, float2) {
-            const int jh = j / R0, jl = j - jh * R0;
-            const int q = jh * R0 * R1 + jl + t * R0;
-            buf1[g * RP + q + (PD ? jh * R1 + t : 0)] = pack_c<DT>(v.x, v.y);
+        auto omul1 = [&](const int4 &ob, int t) -> float2 { return tws[ob.y + t * ob.z]; };
+        auto oput1 = [&](const int4 &ob, int t, float2 v, float2) {
+            buf1[ob.x + t * (R0 + PD)] = pack_c<DT>(v.x, v.y);
         };
-        mstage<DT, NTH, L, R1, G, NF, R1, 2>(a.amat[1], ld0, tw1, st1);
+        mstage<DT, NTH, L, R1, G, NF, R1, 2>(a.amat[1], lds_base, iat0, obase1, omul1, oput1);
         __syncthreads();
         // stage 2 input t of column j: n = j + t R0 R1
-        auto ld1 = [&](int g, int j, int t) -> uint32_t {
-            return buf1[g * RP + j + t * R0 * R1 + (PD ? j / R0 + t * R1 : 0)];
-        };
-        mstage<DT, NTH, L, R2, G, NF, THL, CXL ? 2 : 1>(a.amat[2], ld1, mulg, outg);
+        auto iat1 = [&](int base, int t) -> uint32_t { return buf1[base + t * (R0 * R1 + PD * R1)]; };
+        mstage<DT, NTH, L, R2, G, NF, THL, MULKL>(a.amat[2], lds_base, iat1, gbase, gmul, gput);
     }
     if constexpr (SM == ST_POW) {
         if (a.p_part) {
@@ -616,6 +624,9 @@ static void hlaunch_dt(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
 
 void launch_hpass(int dt, LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hipStream_t st)
 {
+    // the forward column and inverse row passes take the inter-pass twiddle,
+    // the forward row pass does not
+    if (sm == ST_CPX && (lm != LD_CPX) != (a_in.tfull != nullptr)) PSA_FAIL("MFMA pass: twiddle table mismatch");
     PassArgs a = a_in;
     a.carr = dev_carrier();
     if (dt == HDT_F16) hlaunch_dt<HDT_F16>(lm, sm, a, nbatch, st);
