@@ -810,8 +810,8 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         p->buffN.push_back(buffs[i]->N);
     }
     // bytes per intermediate array (the inverse passes' Z is the largest);
-    // PSA_CHUNK_MB overrides for tuning
-    size_t budget = (size_t)1 << 30;
+    // PSA_CHUNK_MB overrides
+    size_t budget = (size_t)4 << 30;
     if (const char *e = getenv("PSA_CHUNK_MB")) budget = (size_t)atol(e) << 20;
     const bool xcd_order = !getenv("PSA_NO_XCD_ORDER");
 
@@ -895,9 +895,11 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         // chunking of the intermediates (one word per complex in the
         // precision form)
         const size_t zspec = s.hp ? spec / 2 : spec;
-        s.bin_chunk = (int)std::max<size_t>(1, std::min<size_t>(nbin, budget / zspec));
+        // (equal chunks: no small remainder launch)
+        auto balance = [](int n, int cmax) { const int k = (n + cmax - 1) / cmax; return (n + k - 1) / k; };
+        s.bin_chunk = balance(nbin, (int)std::max<size_t>(1, std::min<size_t>(nbin, budget / zspec)));
         size_t pairs = std::max<size_t>(1, budget / zspec);
-        s.prn_chunk = (int)std::max<size_t>(1, std::min<size_t>(nprn, pairs / s.bin_chunk));
+        s.prn_chunk = balance(nprn, (int)std::max<size_t>(1, std::min<size_t>(nprn, pairs / s.bin_chunk)));
         if (flags & SDR_SEARCH_PLAIN) s.prn_chunk = 1;
         p->tmp_elems = std::max(p->tmp_elems, (size_t)s.bin_chunk * M);
         p->x_elems = std::max(p->x_elems, (size_t)s.bin_chunk * M);
