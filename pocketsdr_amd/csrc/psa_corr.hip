@@ -329,18 +329,22 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
 }
 
 // Fused mix + correlate: one workgroup per (channel, epoch).  The epoch's N
-// samples are carrier-mixed into LDS as separate int8 I and Q planes, then
-// each thread keeps its QPT words (4 samples each) of both planes in
-// registers for all taps.  Taps run in chunks of TRK_TC: per word one aligned
-// code-word load (4 byte-shifted copies of each bank) and v_dot4 products,
-// accumulated per half (before / after the code wrap at sample j), then wave
-// reductions into LDS.  EXACT selects _mm256_sign_epi8 semantics for IF
-// buffers that hold -8 nibbles (the only source of -128 mixed values); SAMEQ
-// (I = Q in every bank, true for real codes) loads one code plane.
+// samples are carrier-mixed into LDS as separate int8 I and Q planes.  Each
+// wave then owns up to TRK_TPW taps and walks all data words (4 samples, one
+// per lane per step): per word one aligned code-word load per tap and v_dot4
+// products.  The bank of each tap is read circularly shifted by j from a
+// doubled copy (bank twice in a row, 4 byte-shifted copies so every word
+// load is aligned), so both halves of sdr_corr_std's split (before / after the
+// code wrap at sample j) read one contiguous sequence; a step's 64 words lie
+// wholly in one half except at the split, so the half choice is a
+// wave-uniform branch.  The one word that straddles j is left to the
+// epilogue.  EXACT selects _mm256_sign_epi8 semantics for IF buffers that
+// hold -8 nibbles (the only source of -128 mixed values); SAMEQ (I = Q in
+// every bank, true for real codes) loads one code plane.
 #define TRK_THREADS 1024
-#define TRK_TC 4
+#define TRK_TPW 3
 
-template <bool EXACT, bool SAMEQ, int QPT>
+template <bool EXACT, bool SAMEQ>
 __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
     const TrkEpoch *__restrict__ ep, const double *__restrict__ pos,
@@ -397,87 +401,78 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     }
     __syncthreads();
 
-    uint32_t dI[QPT], dQ[QPT];
-#pragma unroll
-    for (int i = 0; i < QPT; i++) {
-        const int w = threadIdx.x + i * TRK_THREADS;
-        dI[i] = w < Nq ? planeI[w] : 0u;
-        dQ[i] = w < Nq ? planeQ[w] : 0u;
-    }
     const uint32_t *cbase = codes + (long long)ch * code_ch_stride;   // [copy][bankI|bankQ][k][NP/4]
     const uint8_t *cb = (const uint8_t *)cbase;
     const int cstride = (int)code_copy_stride;
-    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
     const int QOFF = PSA_N_CODES * NP;       // byte offset of the Q banks
-    // ---- taps ----
-    // For tap (j, k): half 1 pairs sample m < j with code byte N - j + m,
-    // half 2 pairs m >= j with m - j.  A full word (4 samples in one half)
-    // starting at m = 4w reads code bytes c0 .. c0 + 3: in half 2
-    // c0 = 4w - j, i.e. copy (-j) & 3, word w + (-j - copy) / 4; in half 1
-    // c0 = N - j + 4w, copy (N - j) & 3, word w + (N - j - copy) / 4.
-    for (int t0 = 0; t0 < n_pos; t0 += TRK_TC) {
-        int acc[TRK_TC][4];
+    // ---- taps: wave `wave` owns t = t0 + s nwave, s < TRK_TPW ----
+    for (int t0 = wave; t0 < n_pos; t0 += nwave * TRK_TPW) {
+        int jt[TRK_TPW];
+        const uint32_t *cp[TRK_TPW];
+        int acc[TRK_TPW][4];
 #pragma unroll
-        for (int tt = 0; tt < TRK_TC; tt++) {
-            acc[tt][0] = acc[tt][1] = acc[tt][2] = acc[tt][3] = 0;
-            const int t = t0 + tt;
-            if (t >= n_pos) continue;
-            const int2 g = jk[t];
-            const int j = g.x, kb = g.y * NP;
-            const int cp2 = (-j) & 3, cp1 = (N - j) & 3;
-            const uint32_t *p2 = cbase + cp2 * cstride + (kb >> 2) + (-j - cp2) / 4;
-            const uint32_t *p1 = cbase + cp1 * cstride + (kb >> 2) + (N - j - cp1) / 4;
-            const int wq = SAMEQ ? 0 : (QOFF >> 2);
+        for (int s = 0; s < TRK_TPW; s++) {
+            const int t = t0 + s * nwave;
+            const int2 g = jk[t < n_pos ? t : 0];
+            // code sample of data sample m: doubled bank byte N - j + m
+            const int c0 = N - g.x, cpy = c0 & 3;
+            jt[s] = t < n_pos ? g.x : N;
+            cp[s] = cbase + cpy * cstride + ((g.y * NP) >> 2) + (c0 - cpy) / 4;
+            acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0;
+        }
+        for (int wb = 0; wb < Nq; wb += 64) {
+            const int w = wb + lane;
+            // lanes past the last word read word Nq - 1 of the code (inside
+            // the doubled row) against zero data
+            const int wc = w < Nq ? w : Nq - 1;
+            const uint32_t xI = w < Nq ? planeI[w] : 0u;
+            const uint32_t xQ = w < Nq ? planeQ[w] : 0u;
 #pragma unroll
-            for (int i = 0; i < QPT; i++) {
-                const int w = threadIdx.x + i * TRK_THREADS;
-                const int m = w * 4;
-                const bool h2 = m >= j;
-                const bool full = w < Nq && m + 3 < (h2 ? N : j);
-                if (full) {
-                    const uint32_t *cp = h2 ? p2 : p1;
-                    const uint32_t cI = cp[w];
-                    const uint32_t cQ = SAMEQ ? cI : cp[w + wq];
-                    if (EXACT) {
-                        int sI = 0, sQ = 0;
+            for (int s = 0; s < TRK_TPW; s++) {
+                const int j = jt[s];
+                const uint32_t cI = cp[s][wc];
+                const uint32_t cQ = SAMEQ ? cI : cp[s][wc + (QOFF >> 2)];
+                int dI, dQ;
+                if (EXACT) {
+                    dI = 0; dQ = 0;
 #pragma unroll
-                        for (int b = 0; b < 4; b++) {
-                            sI += sgn8((int8_t)(dI[i] >> (8 * b)), (int8_t)(cI >> (8 * b)));
-                            sQ += sgn8((int8_t)(dQ[i] >> (8 * b)), (int8_t)(cQ >> (8 * b)));
-                        }
-                        if (h2) { acc[tt][2] += sI; acc[tt][3] += sQ; }
-                        else { acc[tt][0] += sI; acc[tt][1] += sQ; }
-                    } else if (h2) {
-                        acc[tt][2] = __builtin_amdgcn_sdot4((int)dI[i], (int)cI, acc[tt][2], false);
-                        acc[tt][3] = __builtin_amdgcn_sdot4((int)dQ[i], (int)cQ, acc[tt][3], false);
-                    } else {
-                        acc[tt][0] = __builtin_amdgcn_sdot4((int)dI[i], (int)cI, acc[tt][0], false);
-                        acc[tt][1] = __builtin_amdgcn_sdot4((int)dQ[i], (int)cQ, acc[tt][1], false);
-                    }
-                } else if (w < Nq) {
-                    // straddles the code wrap at m = j or the end of the data
                     for (int b = 0; b < 4; b++) {
-                        const int mm = m + b;
-                        if (mm >= N) break;
-                        const bool hh = mm >= j;
-                        const int ci = hh ? mm - j : N - j + mm;
-                        const int vI = (int8_t)cb[kb + ci];
-                        const int vQ = SAMEQ ? vI : (int8_t)cb[QOFF + kb + ci];
-                        const int xI = (int8_t)(dI[i] >> (8 * b)), xQ = (int8_t)(dQ[i] >> (8 * b));
-                        const int pI = EXACT ? sgn8(xI, vI) : xI * vI;
-                        const int pQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
-                        if (hh) { acc[tt][2] += pI; acc[tt][3] += pQ; }
-                        else { acc[tt][0] += pI; acc[tt][1] += pQ; }
+                        dI += sgn8((int8_t)(xI >> (8 * b)), (int8_t)(cI >> (8 * b)));
+                        dQ += sgn8((int8_t)(xQ >> (8 * b)), (int8_t)(cQ >> (8 * b)));
                     }
+                }
+                // this step covers samples 4 wb .. 4 wb + 255 (wave-uniform test)
+                if (4 * wb + 256 <= j) {
+                    if (EXACT) { acc[s][0] += dI; acc[s][1] += dQ; }
+                    else {
+                        acc[s][0] = __builtin_amdgcn_sdot4((int)xI, (int)cI, acc[s][0], false);
+                        acc[s][1] = __builtin_amdgcn_sdot4((int)xQ, (int)cQ, acc[s][1], false);
+                    }
+                } else if (4 * wb >= j) {
+                    if (EXACT) { acc[s][2] += dI; acc[s][3] += dQ; }
+                    else {
+                        acc[s][2] = __builtin_amdgcn_sdot4((int)xI, (int)cI, acc[s][2], false);
+                        acc[s][3] = __builtin_amdgcn_sdot4((int)xQ, (int)cQ, acc[s][3], false);
+                    }
+                } else {
+                    if (!EXACT) {
+                        dI = __builtin_amdgcn_sdot4((int)xI, (int)cI, 0, false);
+                        dQ = __builtin_amdgcn_sdot4((int)xQ, (int)cQ, 0, false);
+                    }
+                    // words wholly before / after sample j; the straddling
+                    // word is added by the epilogue
+                    if (4 * w + 4 <= j) { acc[s][0] += dI; acc[s][1] += dQ; }
+                    else if (4 * w >= j) { acc[s][2] += dI; acc[s][3] += dQ; }
                 }
             }
         }
 #pragma unroll
-        for (int tt = 0; tt < TRK_TC; tt++) {
+        for (int s = 0; s < TRK_TPW; s++) {
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const int v = wave_sum(acc[tt][c]);
-                if (lane == 0 && t0 + tt < n_pos) atomicAdd(&sums[t0 + tt][c], v);
+                const int v = wave_sum(acc[s][c]);
+                if (lane == 0 && t0 + s * nwave < n_pos) sums[t0 + s * nwave][c] = v;
             }
         }
     }
@@ -491,12 +486,26 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
         const int len2 = N - j;
         const int V2 = len2 >= 16 ? len2 / 16 * 16 : 0;
         const uint8_t *pI = (const uint8_t *)planeI, *pQ = (const uint8_t *)planeQ;
+        // the word holding samples j - j % 4 .. j - j % 4 + 3 straddles the
+        // split when j % 4 != 0: its samples go to their halves here
+        int st[4] = {0, 0, 0, 0};
+        if (j & 3) {
+            for (int m = j & ~3; m < (j & ~3) + 4 && m < N; m++) {
+                const int ci = m < j ? N - j + m : m - j;
+                const int xI = (int8_t)pI[m], xQ = (int8_t)pQ[m];
+                const int vI = (int8_t)cb[(long long)k * NP + ci];
+                const int vQ = (int8_t)cb[QOFF + (long long)k * NP + ci];
+                const int h = m < j ? 0 : 2;
+                st[h] += EXACT ? sgn8(xI, vI) : xI * vI;
+                st[h + 1] += EXACT ? sgn8(xQ, vQ) : xQ * vQ;
+            }
+        }
         float out[4];
         for (int h = 0; h < 2; h++) {
             const int d0 = h == 0 ? 0 : j;          // first data sample of the half
             const int c0 = h == 0 ? N - j : 0;      // first code sample of the half
             const int v = h == 0 ? V1 : V2, len = h == 0 ? j : len2;
-            int sI = sums[t][2 * h], sQ = sums[t][2 * h + 1];
+            int sI = sums[t][2 * h] + st[2 * h], sQ = sums[t][2 * h + 1] + st[2 * h + 1];
             // remove the tail products (they were summed in the bulk loop)
             for (int i = v; i < len; i++) {
                 const int xI = (int8_t)pI[d0 + i], xQ = (int8_t)pQ[d0 + i];
@@ -545,12 +554,9 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
 
 using namespace psa;
 
-static int trk_qpt(int N);
-
 struct sdr_trk_plan_tag {
     int n_ch, n_pos, n_epoch, N, buffN;
     bool exact, sameq;
-    int qpt;
     int NP;
     long long code_ch_stride, code_copy_stride;   // in uint32 words
     uint8_t *d_buf = nullptr;
@@ -649,8 +655,7 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
     ensure_init();
     sdr_trk_plan_t *p = new sdr_trk_plan_t();
     p->n_ch = n_ch; p->n_pos = n_pos; p->n_epoch = n_epoch; p->N = N; p->buffN = buff->N;
-    p->NP = (N + 4 + 3) / 4 * 4;
-    p->qpt = trk_qpt(N);
+    p->NP = (2 * N + 4 + 3) / 4 * 4;
     p->sameq = true;
     for (int c = 0; c < n_ch && p->sameq; c++) {
         for (size_t i = 0; i < (size_t)N * PSA_N_CODES; i++) {
@@ -666,8 +671,9 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
     PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
     PSA_CHECK(hipMalloc(&p->d_buf, buff->N));
     PSA_CHECK(hipMemcpy(p->d_buf, buff->data, buff->N, hipMemcpyHostToDevice));
-    // code layout per channel: [copy 0..3][I banks 0..9 | Q banks 0..9][NP bytes]
-    // copy c holds bank bytes shifted by c, so word w of copy c = bytes 4w+c..4w+c+3
+    // code layout per channel: [copy 0..3][I banks 0..9 | Q banks 0..9][NP bytes],
+    // each bank doubled (byte i holds bank sample i mod N, i < 2N); copy c holds
+    // those bytes shifted by c, so word w of copy c = bytes 4w+c..4w+c+3
     const size_t bank_bytes = (size_t)2 * PSA_N_CODES * p->NP;
     p->code_copy_stride = (long long)(bank_bytes / 4);
     p->code_ch_stride = p->code_copy_stride * 4;
@@ -679,8 +685,9 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
                 for (int k = 0; k < PSA_N_CODES; k++) {
                     uint8_t *row = dst + ((size_t)comp * PSA_N_CODES + k) * p->NP;
                     const sdr_cpx16_t *src = chs[c].code + (size_t)k * N;
-                    for (int i = 0; i < N - cp; i++) {
-                        row[i] = (uint8_t)(comp == 0 ? src[i + cp].I : src[i + cp].Q);
+                    for (int i = 0; i < 2 * N - cp; i++) {
+                        const sdr_cpx16_t &v = src[(i + cp) % N];
+                        row[i] = (uint8_t)(comp == 0 ? v.I : v.Q);
                     }
                 }
             }
@@ -707,46 +714,26 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
 
 }  // extern "C"
 
-// QPT: words of each plane per thread, the smallest instantiated count that
-// covers ceil(N / 4) words with TRK_THREADS threads
-#define TRK_QPTS(X) X(2) X(4) X(6) X(8) X(12) X(16) X(20)
-
-template <bool EXACT, bool SAMEQ, int QPT>
+template <bool EXACT, bool SAMEQ>
 static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)
 {
     static std::once_flag once;
     std::call_once(once, [] {
-        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<EXACT, SAMEQ, QPT>,
+        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<EXACT, SAMEQ>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     });
-    hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ, QPT>), dim3(ne), dim3(TRK_THREADS), shm, p->st,
+    hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ>), dim3(ne), dim3(TRK_THREADS), shm, p->st,
                        p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
                        p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
                        dev_carrier(), p->d_corr, p->d_C);
 }
 
-static int trk_qpt(int N)
-{
-    const int need = ((N + 3) / 4 + TRK_THREADS - 1) / TRK_THREADS;
-#define PSA_Q(Q_) if (need <= Q_) return Q_;
-    TRK_QPTS(PSA_Q)
-#undef PSA_Q
-    return 0;
-}
-
 static void trk_launch(sdr_trk_plan_t *p, size_t shm, unsigned ne)
 {
-#define PSA_Q(Q_)                                                                        \
-    if (p->qpt == Q_) {                                                                  \
-        if (p->exact && p->sameq) trk_launch_t<true, true, Q_>(p, shm, ne);              \
-        else if (p->exact) trk_launch_t<true, false, Q_>(p, shm, ne);                    \
-        else if (p->sameq) trk_launch_t<false, true, Q_>(p, shm, ne);                    \
-        else trk_launch_t<false, false, Q_>(p, shm, ne);                                 \
-        return;                                                                          \
-    }
-    TRK_QPTS(PSA_Q)
-#undef PSA_Q
-    PSA_FAIL("tracking plan: no kernel for N=%d", p->N);
+    if (p->exact && p->sameq) trk_launch_t<true, true>(p, shm, ne);
+    else if (p->exact) trk_launch_t<true, false>(p, shm, ne);
+    else if (p->sameq) trk_launch_t<false, true>(p, shm, ne);
+    else trk_launch_t<false, false>(p, shm, ne);
 }
 
 extern "C" {
