@@ -288,7 +288,34 @@ __device__ __forceinline__ void last_stage(const PassArgs &a, Storer<SM> &st, in
             int g, j;
             if (out_nfast) { g = bi / LrL; j = bi - g * LrL; }
             else { j = bi / CG; g = bi - j * CG; }
-            if (g < gcount) {
+            if (SM == ST_POW) {
+                // previous P values requested before the transform (clamped
+                // indices, 0 on the first block), stored after it
+                const bool okg = g < gcount;
+                const int eb = (g0 + (okg ? g : 0)) * st.ogs + j * st.ons;
+                float pold[RL];
+#pragma unroll
+                for (int q = 0; q < RL; q++) {
+                    const int e = eb + q * LrL * st.ons;
+                    pold[q] = st.pinit ? 0.0f : st.prow[e < st.nlag ? e : 0];
+                }
+                float2 v[RL];
+#pragma unroll
+                for (int q = 0; q < RL; q++) v[q] = buf[g * LP + j + q * LrL];
+#pragma unroll
+                for (int q = 1; q < RL; q++) v[q] = cmul(v[q], tw[q * j]);
+                bfly<RL>(v, sgn);
+                if (okg) {
+#pragma unroll
+                    for (int q = 0; q < RL; q++) {
+                        if (st.thi) {     // inter-pass twiddle (not used by the power passes)
+                            const int ti = (g0 + g) * (j + q * LrL);
+                            v[q] = cmul(v[q], cmul(st.thi[ti >> 10], st.tlo[ti & 1023]));
+                        }
+                        st.store_pow(eb + q * LrL * st.ons, v[q], pold[q]);
+                    }
+                }
+            } else if (g < gcount) {
                 float2 v[RL];
 #pragma unroll
                 for (int q = 0; q < RL; q++) v[q] = buf[g * LP + j + q * LrL];
@@ -329,13 +356,14 @@ __device__ __forceinline__ void pass_item(const PassArgs &a, int b, int g0, int 
                 if (in_nfast) { g = bi / Lr0; j = bi - g * Lr0; }
                 else { j = bi / CG; g = bi - j * CG; }
                 float2 v[R0];
-                if (g < gcount) {
-                    const int e0 = (g0 + g) * igs + j * ins;
+                // rows past the end of the item read row 0 and are zeroed, so
+                // the loads are not serialised behind a branch
+                const bool okg = g < gcount;
+                const int e0 = (g0 + (okg ? g : 0)) * igs + j * ins;
 #pragma unroll
-                    for (int q = 0; q < R0; q++) v[q] = ld.load(e0 + q * Lr0 * ins);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < R0; q++) v[q] = make_float2(0.f, 0.f);
+                for (int q = 0; q < R0; q++) {
+                    const float2 x = ld.load(e0 + q * Lr0 * ins);
+                    v[q] = okg ? x : make_float2(0.f, 0.f);
                 }
                 bfly<R0>(v, sgn);
 #pragma unroll
