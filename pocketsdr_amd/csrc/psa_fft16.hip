@@ -123,10 +123,12 @@ struct MGeom {
 // NF: columns run along j (rows contiguous in memory) rather than along g.
 // Outputs t' >= TH are not produced.  Each wave walks its tiles
 // software-pipelined: the next tile's inputs and this tile's multipliers are
-// requested before the MFMAs; every load is issued unconditionally on
-// clamped indices and masked afterwards, so no load waits inside a branch.
-template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, class IBase, class IAt, class OBase,
-          class OMul, class OPut>
+// requested before the MFMAs (PF: stages that read global memory; stages fed
+// from LDS load their inputs at the top of the tile); every load is issued
+// unconditionally on clamped indices and masked afterwards, so no load waits
+// inside a branch.
+template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, class IBase, class IAt,
+          class OBase, class OMul, class OPut>
 __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBase &ibase, const IAt &iat,
                                        const OBase &obase, const OMul &omul, const OPut &oput)
 {
@@ -179,16 +181,17 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBa
     int tile = wave, g = 0, j = 0;
     bool cv = column(tile, g, j);
     uint4 B[KS];
-    if (tile < NT) load_b(g, j, cv, B);
+    if (PF && tile < NT) load_b(g, j, cv, B);
     for (; tile < NT; tile += NTH / 64) {
         const int tn = tile + NTH / 64;
         int gn = 0, jn = 0;
         bool cvn = false;
-        uint4 Bn[KS];
-        if (tn < NT) {
+        uint4 Bn[PF ? KS : 1];
+        if (PF && tn < NT) {
             cvn = column(tn, gn, jn);
             load_b(gn, jn, cvn, Bn);
         }
+        if (!PF) load_b(g, j, cv, B);
         const int4 ob = obase(g, j);
         // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r + 4 m +
         // 2 h + w, real (imaginary) part
@@ -235,9 +238,13 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBa
             }
         }
         if (tn < NT) {
+            if (PF) {
 #pragma unroll
-            for (int k = 0; k < KS; k++) B[k] = Bn[k];
-            g = gn; j = jn; cv = cvn;
+                for (int k = 0; k < KS; k++) B[k] = Bn[k];
+                g = gn; j = jn; cv = cvn;
+            } else {
+                cv = column(tn, g, j);
+            }
         }
     }
 }
@@ -308,7 +315,7 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         auto obase = [&](int g, int j) -> int4 { return make_int4(g * RP + j * (R0 + PD), 0, (j / R2) * R2, 1); };
         auto omul = [&](const int4 &ob, int t) -> float2 { return tws[t * ob.z]; };
         auto oput = [&](const int4 &ob, int t, float2 v, float2) { buf0[ob.x + t] = pack_c<DT>(v.x, v.y); };
-        mstage<DT, NTH, L, R0, G, NF, R0, 2>(a.amat[0], ibase, iat, obase, omul, oput);
+        mstage<DT, NTH, L, R0, G, NF, R0, 2, true>(a.amat[0], ibase, iat, obase, omul, oput);
     }
     __syncthreads();
 
@@ -347,7 +354,7 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     // stage 1 input t of column j: n = j + t R0 R2
     auto iat0 = [&](int base, int t) -> uint32_t { return buf0[base + t * (R0 * R2 + PD * R2)]; };
     if constexpr (!S3) {
-        mstage<DT, NTH, L, R1, G, NF, THL, MULKL>(a.amat[1], lds_base, iat0, gbase, gmul, gput);
+        mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false>(a.amat[1], lds_base, iat0, gbase, gmul, gput);
     } else {
         // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
         // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
@@ -359,11 +366,11 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         auto oput1 = [&](const int4 &ob, int t, float2 v, float2) {
             buf1[ob.x + t * (R0 + PD)] = pack_c<DT>(v.x, v.y);
         };
-        mstage<DT, NTH, L, R1, G, NF, R1, 2>(a.amat[1], lds_base, iat0, obase1, omul1, oput1);
+        mstage<DT, NTH, L, R1, G, NF, R1, 2, false>(a.amat[1], lds_base, iat0, obase1, omul1, oput1);
         __syncthreads();
         // stage 2 input t of column j: n = j + t R0 R1
         auto iat1 = [&](int base, int t) -> uint32_t { return buf1[base + t * (R0 * R1 + PD * R1)]; };
-        mstage<DT, NTH, L, R2, G, NF, THL, MULKL>(a.amat[2], lds_base, iat1, gbase, gmul, gput);
+        mstage<DT, NTH, L, R2, G, NF, THL, MULKL, false>(a.amat[2], lds_base, iat1, gbase, gmul, gput);
     }
     if constexpr (SM == ST_POW) {
         if (a.p_part) {
