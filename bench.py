@@ -50,32 +50,36 @@ SPEC_G = {150: 16, 160: 16, 200: 16, 240: 16, 300: 16, 320: 12, 400: 16, 480: 8,
 MODES = {"fwd_col": (2, 0), "fwd_row": (0, 0), "inv_row": (3, 0), "inv_col": (0, 1)}
 
 
-# precision-form passes: (R0, R1, R2, G, threads) of the column and row lengths
-HCOL = {150: (10, 15, 1, 32, 256), 200: (10, 20, 1, 32, 512), 300: (15, 20, 1, 32, 512),
-        400: (16, 25, 1, 32, 512), 960: (32, 30, 1, 16, 512)}
-HROW = {160: (16, 10, 1, 32, 256), 240: (16, 15, 1, 32, 256), 320: (16, 20, 1, 16, 256),
-        480: (16, 30, 1, 16, 512), 1000: (10, 10, 10, 4, 256)}
+# precision-form passes: (R0, R1, R2, G, threads) per (length, workgroup-shape
+# variant) of the column and row passes
+HCOL = {(150, 0): (10, 15, 1, 32, 256), (150, 1): (10, 15, 1, 32, 512),
+        (200, 0): (10, 20, 1, 32, 512), (200, 1): (10, 20, 1, 32, 256),
+        (300, 0): (15, 20, 1, 32, 512), (300, 1): (15, 20, 1, 32, 256),
+        (400, 0): (16, 25, 1, 32, 512), (400, 1): (16, 25, 1, 32, 256),
+        (960, 0): (32, 30, 1, 16, 512), (960, 1): (32, 30, 1, 16, 256)}
+HROW = {(160, 0): (16, 10, 1, 32, 256), (160, 1): (16, 10, 1, 32, 512),
+        (240, 0): (16, 15, 1, 32, 256), (240, 1): (16, 15, 1, 32, 512),
+        (320, 0): (16, 20, 1, 16, 256), (320, 1): (16, 20, 1, 32, 512),
+        (480, 0): (16, 30, 1, 16, 512), (480, 1): (16, 30, 1, 16, 256),
+        (1000, 0): (10, 10, 10, 4, 256), (1000, 1): (10, 10, 10, 8, 512)}
 
 
-def kernel_instance(cls, M, form="fp32"):
+def kernel_instance(cls, M, form="fp32", tiles=None):
     """Name of the pass kernel a (class, FFT length, form) launches, as
     rocprof lists it.  FP32: the four-step split M = M1 * M2 (rows M2 a
     multiple of 16 when possible, then the most balanced), column passes of
-    length M1, row passes of length M2.  FP16 / BF16: the MFMA split."""
+    length M1, row passes of length M2.  FP16 / BF16: the plan's tiles
+    (M1, M2, column shape, row shape)."""
     if cls not in MODES:
         return cls
-    if form in ("fp16", "bf16"):
-        # the library's choice: smallest largest radix, then the shorter column
-        cands = [(max(HCOL[L1][:3] + HROW[M // L1][:3]) * 4096 + L1, L1)
-                 for L1 in HCOL if M % L1 == 0 and M // L1 in HROW]
-        if cands:
-            L1 = min(cands)[1]
-            col = cls in ("fwd_col", "inv_col")
-            L = L1 if col else M // L1
-            r0, r1, r2, G, nth = HCOL[L] if col else HROW[L]
-            lm, sm = MODES[cls]
-            return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s, %d>" % (
-                lm, sm, 0 if form == "fp16" else 1, L, r0, r1, r2, G, "false" if col else "true", nth)
+    if form in ("fp16", "bf16") and tiles:
+        M1, M2, vc, vr = tiles
+        col = cls in ("fwd_col", "inv_col")
+        L, v = (M1, vc) if col else (M2, vr)
+        r0, r1, r2, G, nth = HCOL[(L, v)] if col else HROW[(L, v)]
+        lm, sm = MODES[cls]
+        return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s, %d>" % (
+            lm, sm, 0 if form == "fp16" else 1, L, r0, r1, r2, G, "false" if col else "true", nth)
     M1, M2 = M, 1
     if M > 4096:
         cands = [(0 if (M // d) % 16 == 0 else 1, abs(d - M // d), d) for d in range(1, 4097)
@@ -108,7 +112,7 @@ def build_workload(name, seed):
     raise SystemExit("unknown workload " + name)
 
 
-FORM_FLAGS = {"fp32": 0, "fp16": 4, "bf16": 8}
+FORM_FLAGS = {"fp32": 0, "fp16": 4 | 16, "bf16": 8 | 16}     # precision forms tuned
 FORM_DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 
 
@@ -162,6 +166,10 @@ def bench_acq(args, sdr, orc):
     forms = [sdr_func.FORM_FP32, sdr_func.FORM_PLAIN, sdr_func.FORM_FP16, sdr_func.FORM_BF16]
     names = ["fp32", "plain", "fp16", "bf16"]
     res["form_per_signal"] = {s["name"]: names[forms.index(plan.form(i))] for i, s in enumerate(sigs)}
+    tiles = {s["name"]: plan.tiles(i) for i, s in enumerate(sigs)}
+    if any(tiles.values()):
+        res["tuned_tiles"] = {k: dict(M1=v[0], M2=v[1], col_shape=v[2], row_shape=v[3])
+                              for k, v in tiles.items() if v}
     res["detections"] = decisions(sigs, r_main, truth)
     # per-kernel HIP events on the plan's stream over timed steps
     plan.timing(True)
@@ -173,13 +181,14 @@ def bench_acq(args, sdr, orc):
             r = plan.kernel_ms("%s@%d" % (n, i))
             if r:
                 ks["%s@%s" % (n, s["name"])] = dict(avg_ms=r[0], launches=r[1], elems=r[2], bytes=r[3],
-                                                    M=s["M"], cls=n, form=res["form_per_signal"][s["name"]])
+                                                    M=s["M"], cls=n, form=res["form_per_signal"][s["name"]],
+                                                    tiles=tiles[s["name"]])
     # dominant kernel: the instantiation (as rocprof names it) with the most
     # device time; signal groups whose FFT lengths share a pass length share
     # its launches, so time and algorithmic bytes are summed over them
     inst_tot = {}
     for k, v in ks.items():
-        inst = kernel_instance(v["cls"], v["M"], v["form"])
+        inst = kernel_instance(v["cls"], v["M"], v["form"], v["tiles"])
         t = inst_tot.setdefault(inst, dict(ms=0.0, launches=0, bytes=0.0, parts=[]))
         t["ms"] += v["avg_ms"] * v["launches"]
         t["launches"] += v["launches"]

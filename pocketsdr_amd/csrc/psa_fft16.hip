@@ -433,19 +433,29 @@ __global__ void h_spec_kernel(const float2 *__restrict__ in, uint32_t *__restric
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// (L, R0, R1, R2, G, threads): column-pass lengths (G columns = 128 bytes of words per
-// row segment; 16 for 960 to fit two workgroups per CU) and row-pass lengths
-// (G rows, about 4-8 K points per workgroup, G L / R a multiple of 32 where
-// possible).  Splits: 24000 = 150 x 160, 48000 = 200 x 240, 96000 = 300 x 320,
-// 192000 = 400 x 480, 960000 = 960 x 1000.
-#define PSA_HCOL(X) X(150, 10, 15, 1, 32, 256) X(200, 10, 20, 1, 32, 512) X(300, 15, 20, 1, 32, 512) \
-    X(400, 16, 25, 1, 32, 512) X(960, 32, 30, 1, 16, 512)
-#define PSA_HROW(X) X(160, 16, 10, 1, 32, 256) X(240, 16, 15, 1, 32, 256) X(320, 16, 20, 1, 16, 256) \
-    X(480, 16, 30, 1, 16, 512) X(1000, 10, 10, 10, 4, 256)
+// (L, R0, R1, R2, G, threads, variant): column-pass lengths (G columns =
+// 128 bytes of words per row segment; 16 for 960 to fit two workgroups per
+// CU) and row-pass lengths (G rows, 4-8 K points per workgroup, G L / R a
+// multiple of 32 where possible).  Each length has two workgroup shapes; the
+// plan's tuner (SDR_SEARCH_TUNE) times both on the real shapes, variant 0 is
+// the default.  Splits: 24000 = 150 x 160, 48000 = 150 x 320 or 200 x 240,
+// 96000 = 300 x 320 or 200 x 480, 192000 = 400 x 480, 960000 = 960 x 1000.
+#define PSA_HCOL(X) \
+    X(150, 10, 15, 1, 32, 256, 0) X(150, 10, 15, 1, 32, 512, 1) \
+    X(200, 10, 20, 1, 32, 512, 0) X(200, 10, 20, 1, 32, 256, 1) \
+    X(300, 15, 20, 1, 32, 512, 0) X(300, 15, 20, 1, 32, 256, 1) \
+    X(400, 16, 25, 1, 32, 512, 0) X(400, 16, 25, 1, 32, 256, 1) \
+    X(960, 32, 30, 1, 16, 512, 0) X(960, 32, 30, 1, 16, 256, 1)
+#define PSA_HROW(X) \
+    X(160, 16, 10, 1, 32, 256, 0) X(160, 16, 10, 1, 32, 512, 1) \
+    X(240, 16, 15, 1, 32, 256, 0) X(240, 16, 15, 1, 32, 512, 1) \
+    X(320, 16, 20, 1, 16, 256, 0) X(320, 16, 20, 1, 32, 512, 1) \
+    X(480, 16, 30, 1, 16, 512, 0) X(480, 16, 30, 1, 16, 256, 1) \
+    X(1000, 10, 10, 10, 4, 256, 0) X(1000, 10, 10, 10, 8, 512, 1)
 
-struct HSpec { int L, r[3], G; };
+struct HSpec { int L, r[3], G, nth, var; };
 static const HSpec h_col_specs[] = {
-#define PSA_S(L_, A_, B_, C_, G_, T_) {L_, {A_, B_, C_}, G_},
+#define PSA_S(L_, A_, B_, C_, G_, T_, V_) {L_, {A_, B_, C_}, G_, T_, V_},
     PSA_HCOL(PSA_S)
 };
 static const HSpec h_row_specs[] = {
@@ -453,13 +463,38 @@ static const HSpec h_row_specs[] = {
 #undef PSA_S
 };
 
-static const HSpec *find_spec(const HSpec *t, int n, int L)
+static const HSpec *find_spec(const HSpec *t, int n, int L, int var)
 {
-    for (int i = 0; i < n; i++) if (t[i].L == L) return &t[i];
+    for (int i = 0; i < n; i++) if (t[i].L == L && t[i].var == var) return &t[i];
     return nullptr;
 }
-static const HSpec *col_spec(int L) { return find_spec(h_col_specs, (int)(sizeof(h_col_specs) / sizeof(HSpec)), L); }
-static const HSpec *row_spec(int L) { return find_spec(h_row_specs, (int)(sizeof(h_row_specs) / sizeof(HSpec)), L); }
+static const HSpec *col_spec(int L, int var = 0)
+{
+    return find_spec(h_col_specs, (int)(sizeof(h_col_specs) / sizeof(HSpec)), L, var);
+}
+static const HSpec *row_spec(int L, int var = 0)
+{
+    return find_spec(h_row_specs, (int)(sizeof(h_row_specs) / sizeof(HSpec)), L, var);
+}
+
+int h_splits(int M, int *m1, int nmax)
+{
+    // the default first: the smallest largest radix (fewest padded MFMA rows
+    // and registers), then the shorter column pass
+    std::vector<std::pair<int, int>> c;
+    for (const HSpec &cs : h_col_specs) {
+        if (cs.var != 0) continue;
+        const HSpec *r = M % cs.L == 0 ? row_spec(M / cs.L) : nullptr;
+        if (!r) continue;
+        int mr = 0;
+        for (int i = 0; i < 3; i++) mr = std::max(mr, std::max(cs.r[i], r->r[i]));
+        c.push_back({mr * 4096 + cs.L, cs.L});
+    }
+    std::sort(c.begin(), c.end());
+    int n = 0;
+    for (auto &x : c) if (n < nmax) m1[n++] = x.second;
+    return n;
+}
 
 // A-operand table of the R-point DFT (BF16: hi part, then lo part)
 static const uint4 *make_amat(int R, int dir, int dt)
@@ -511,26 +546,21 @@ static float pow2_near(double x)    // 2^round(log2 x)
     return (float)ldexp(1.0, (int)lround(log2(x)));
 }
 
-const HPlan *hfft_plan(int M, int dt)
+const HPlan *hfft_plan(int M, int dt, int M1)
 {
     static std::mutex mtx;
-    static std::map<int, HPlan *> plans;
+    static std::map<std::vector<int>, HPlan *> plans;
     if (dt != HDT_F16 && dt != HDT_BF16) return nullptr;
-    // among the splits with specialised passes: the smallest largest radix
-    // (fewest padded MFMA rows and registers), then the shorter column pass
-    int M1 = 0, M2 = 0, best = 1 << 30;
-    for (const HSpec &c : h_col_specs) {
-        const HSpec *r = M % c.L == 0 ? row_spec(M / c.L) : nullptr;
-        if (!r) continue;
-        int mr = 0;
-        for (int i = 0; i < 3; i++) mr = std::max(mr, std::max(c.r[i], r->r[i]));
-        const int score = mr * 4096 + c.L;
-        if (score < best) { best = score; M1 = c.L; M2 = r->L; }
-    }
-    if (!M1) return nullptr;
+    int cand[8];
+    const int nc = h_splits(M, cand, 8);
+    if (nc == 0) return nullptr;
+    if (M1 == 0) M1 = cand[0];
+    if (std::find(cand, cand + nc, M1) == cand + nc) return nullptr;
+    const int M2 = M / M1;
     ensure_init();
     std::lock_guard<std::mutex> lk(mtx);
-    auto it = plans.find(M * 2 + dt);
+    const std::vector<int> key = {M, dt, M1};
+    auto it = plans.find(key);
     if (it != plans.end()) return it->second;
     HPlan *p = new HPlan();
     p->M = M; p->M1 = M1; p->M2 = M2; p->dt = dt;
@@ -551,16 +581,17 @@ const HPlan *hfft_plan(int M, int dt)
             p->am2[dir][s] = rs->r[s] > 1 ? make_amat(rs->r[s], dir, dt) : nullptr;
         }
     }
-    plans[M * 2 + dt] = p;
+    plans[key] = p;
     return p;
 }
 
-void h_col_args(const HPlan *hp, int dir, PassArgs &a)
+void h_col_args(const HPlan *hp, int dir, PassArgs &a, int var)
 {
-    const HSpec *cs = col_spec(hp->M1);
+    const HSpec *cs = col_spec(hp->M1, var);
     a.L = hp->M1;
     a.nsub = hp->M2;
     a.G = cs->G;
+    a.hvar = var;
     a.in_nstride = hp->M2; a.in_gstride = 1;
     a.out_nstride = hp->M2; a.out_gstride = 1;
     a.dir = dir;
@@ -573,12 +604,13 @@ void h_col_args(const HPlan *hp, int dir, PassArgs &a)
     for (int s = 0; s < 3; s++) a.amat[s] = hp->am1[dir][s];
 }
 
-void h_row_args(const HPlan *hp, int dir, PassArgs &a)
+void h_row_args(const HPlan *hp, int dir, PassArgs &a, int var)
 {
-    const HSpec *rs = row_spec(hp->M2);
+    const HSpec *rs = row_spec(hp->M2, var);
     a.L = hp->M2;
     a.nsub = hp->M1;
     a.G = rs->G;
+    a.hvar = var;
     a.in_nstride = 1; a.in_gstride = hp->M2;
     a.out_nstride = 1; a.out_gstride = hp->M2;
     a.dir = dir;
@@ -611,16 +643,16 @@ template <int DT>
 static void hlaunch_dt(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch, hipStream_t st)
 {
     // column lengths: forward (mixing) and inverse (power) column passes
-#define PSA_C(L_, A_, B_, C_, G_, T_)                                                          \
-    if (a.L == L_ && a.in_nstride != 1) {                                                  \
+#define PSA_C(L_, A_, B_, C_, G_, T_, V_)                                                      \
+    if (a.L == L_ && a.hvar == V_ && a.in_nstride != 1) {                                  \
         if (lm == LD_MIX && sm == ST_CPX) { hlaunch_t<LD_MIX, ST_CPX, DT, L_, A_, B_, C_, G_, false, T_>(a, nbatch, st); return; } \
         if (lm == LD_CPX && sm == ST_POW) { hlaunch_t<LD_CPX, ST_POW, DT, L_, A_, B_, C_, G_, false, T_>(a, nbatch, st); return; } \
     }
     PSA_HCOL(PSA_C)
 #undef PSA_C
     // row lengths: forward row pass and inverse (code product) row pass
-#define PSA_R(L_, A_, B_, C_, G_, T_)                                                          \
-    if (a.L == L_ && a.in_nstride == 1) {                                                  \
+#define PSA_R(L_, A_, B_, C_, G_, T_, V_)                                                      \
+    if (a.L == L_ && a.hvar == V_ && a.in_nstride == 1) {                                  \
         if (lm == LD_CPX && sm == ST_CPX) { hlaunch_t<LD_CPX, ST_CPX, DT, L_, A_, B_, C_, G_, true, T_>(a, nbatch, st); return; } \
         if (lm == LD_PROD && sm == ST_CPX) { hlaunch_t<LD_PROD, ST_CPX, DT, L_, A_, B_, C_, G_, true, T_>(a, nbatch, st); return; } \
     }

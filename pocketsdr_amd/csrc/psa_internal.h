@@ -179,6 +179,7 @@ struct PassArgs {
     const float2 *tfull;        // ST_CPX: twiddle per output index, pre-scaled; nullptr: none
     float osc;                  // ST_CPX scale applied when tfull is nullptr
     float kappa;                // ST_POW: values scaled by kappa before |.|^2
+    int hvar;                   // workgroup shape variant of the MFMA pass
 };
 
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
@@ -207,10 +208,14 @@ struct HPlan {
     // pass output (folded into tfull where a pass has the twiddle)
     float osc_fc = 1, osc_fr = 1, osc_ir = 1;
 };
-// cached per (M, dt); nullptr when M has no MFMA split
-const HPlan *hfft_plan(int M, int dt);
-void h_col_args(const HPlan *hp, int dir, PassArgs &a);
-void h_row_args(const HPlan *hp, int dir, PassArgs &a);
+// cached per (M, dt, M1); M1 = 0 takes the default split; nullptr when M
+// has no MFMA split (or M1 is not one of its splits)
+const HPlan *hfft_plan(int M, int dt, int M1 = 0);
+// column lengths M1 of the MFMA splits of M, default first; returns the count
+int h_splits(int M, int *m1, int nmax);
+// pass geometry; var selects one of the two workgroup shapes of the length
+void h_col_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);
+void h_row_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);
 void launch_hpass(int dt, LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
                   hipStream_t st);
 // natural-order FP32 code spectra (n x M) -> digit-transposed words * scale

@@ -465,6 +465,8 @@ struct SigDev {
     const HPlan *hp = nullptr;  // precision form (FP16 / BF16 MFMA passes)
     uint32_t *cfh = nullptr;    // n_prn x M words, digit-transposed, scaled
     float kappa = 1.0f;         // power scale of the precision form
+    int hvar_col = 0, hvar_row = 0;   // workgroup shapes of its passes
+    float2 *cf_nat = nullptr;   // natural-order FP32 spectra (until set up)
     float *P = nullptr;         // n_prn x len_fds x N
     float *fds = nullptr;       // device copy of the bins
     int *mix_ix = nullptr, *mix_nseg = nullptr;   // [n_blk x len_fds]
@@ -478,6 +480,7 @@ struct SigDev {
     bool fused_peak = false;    // the last column pass emits the peak partials
     bool xcd_order = true;      // XCD-aware workgroup order of the inverse row pass
     int bin_chunk = 1, prn_chunk = 1;
+    float tuned_ms = 0.0f;      // tuner: best first-block time (0: not tuned)
 };
 
 enum { K_FWD_COL = 0, K_FWD_ROW, K_INV_ROW, K_INV_COL, K_MIX, K_CVT, K_MUL, K_POW, K_PEAK, K_FILL, K_NCLS };
@@ -627,7 +630,7 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
             const int nbc = std::min(s.bin_chunk, nbin - b0);
             const long long mo = (long long)k * nbin + b0;
             PassArgs a = {};
-            h_col_args(hp, 0, a);
+            h_col_args(hp, 0, a, s.hvar_col);
             a.tfull = hp->tfull[0];
             a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
             a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
@@ -638,7 +641,7 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
                 launch_hpass(dt, LD_MIX, ST_CPX, a, nbc, st);
             }
             PassArgs r = {};
-            h_row_args(hp, 0, r);
+            h_row_args(hp, 0, r, s.hvar_row);
             r.osc = hp->osc_fr;
             r.hin = tmp; r.hout = X; r.in_bstride = r.out_bstride = M;
             {
@@ -649,7 +652,7 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
                 const int nqc = std::min(s.prn_chunk, nprn - q0);
                 const int nbatch = nqc * nbc;      // b = prn_local * nbc + bin_local
                 PassArgs ir = {};
-                h_row_args(hp, 1, ir);
+                h_row_args(hp, 1, ir, s.hvar_row);
                 ir.tfull = hp->tfull[1];
                 ir.hin = X; ir.in_bstride = M;
                 ir.hcf = s.cfh + (long long)q0 * M; ir.cf_bstride = M;
@@ -664,7 +667,7 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
                     launch_hpass(dt, LD_PROD, ST_CPX, ir, nbatch, st);
                 }
                 PassArgs ic = {};
-                h_col_args(hp, 1, ic);
+                h_col_args(hp, 1, ic, s.hvar_col);
                 ic.kappa = s.kappa;
                 ic.hin = Z; ic.in_bstride = M;
                 ic.P = s.P; ic.p_bstride = N;
@@ -746,6 +749,72 @@ static void run_plain(sdr_search_plan_t *p, SigDev &s)
             }
         }
     }
+}
+
+// Precision-form group set up for split M1 (column length) and workgroup
+// shapes vc, vr: code spectra in that split's digit-transposed order, scaled
+// by 2^-ceil(log2 N) (|code_fft| <= N for a zero-padded code of N samples in
+// [-1, 1]), the power scale kappa, the peak partial buffer.
+static void h_setup(sdr_search_plan_t *p, SigDev &s, int M1, int vc, int vr)
+{
+    const int M = s.h.M, nprn = s.h.n_prn;
+    s.hp = hfft_plan(M, s.hp->dt, M1);
+    s.hvar_col = vc;
+    s.hvar_row = vr;
+    const int e = (int)ceil(log2((double)s.h.N));
+    const double sc = ldexp(1.0, -e);
+    h_code_spec(s.hp, s.cf_nat, s.cfh, nprn, (float)sc, p->st);
+    PSA_CHECK(hipStreamSynchronize(p->st));
+    s.kappa = (float)((double)PSA_CSCALE / ((double)M * M * s.hp->osc_fc * s.hp->osc_fr * sc * s.hp->osc_ir));
+    PassArgs tmp = {};
+    h_col_args(s.hp, 1, tmp, vc);
+    const int nf = s.h.len_fds * pass_ngroups(tmp);
+    if (nf != s.nchunk_fused || !s.part) {
+        if (s.part) PSA_CHECK(hipFree(s.part));
+        s.nchunk_fused = nf;
+        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused)));
+    }
+}
+
+// SDR_SEARCH_TUNE: time every (split, column shape, row shape) of the group's
+// MFMA passes on its own shapes - the first block of the search, best of two
+// runs after a warm-up - and keep the fastest.
+static void tune_group(sdr_search_plan_t *p, SigDev &s)
+{
+    int m1[8];
+    const int ns = h_splits(s.h.M, m1, 8);
+    const int nblk = s.h.n_blk;
+    hipEvent_t e0, e1;
+    PSA_CHECK(hipEventCreate(&e0));
+    PSA_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bm = s.hp->M1, bc = 0, br = 0;
+    s.h.n_blk = 1;
+    for (int i = 0; i < ns; i++) {
+        for (int vc = 0; vc < 2; vc++) {
+            for (int vr = 0; vr < 2; vr++) {
+                h_setup(p, s, m1[i], vc, vr);
+                float t = 1e30f;
+                for (int rep = 0; rep < 3; rep++) {
+                    PSA_CHECK(hipEventRecord(e0, p->st));
+                    run_batched_h(p, s);
+                    PSA_CHECK(hipEventRecord(e1, p->st));
+                    PSA_CHECK(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    PSA_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    if (rep > 0) t = std::min(t, ms);
+                }
+                if (t < best) { best = t; bm = m1[i]; bc = vc; br = vr; }
+            }
+        }
+    }
+    s.h.n_blk = nblk;
+    s.hp = hfft_plan(s.h.M, s.hp->dt, bm);
+    s.hvar_col = bc;
+    s.hvar_row = br;
+    s.tuned_ms = best;
+    PSA_CHECK(hipEventDestroy(e0));
+    PSA_CHECK(hipEventDestroy(e1));
 }
 
 static void run_all(sdr_search_plan_t *p)
@@ -835,15 +904,9 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         if (flags & SDR_SEARCH_PLAIN) {
             s.cfn = nat;
         } else if (s.hp) {
-            // |code_fft| <= N (a zero-padded code of N samples in [-1, 1])
-            const int e = (int)ceil(log2((double)s.h.N));
-            const double sc = ldexp(1.0, -e);
+            // converted for the chosen split below
+            s.cf_nat = nat;
             PSA_CHECK(hipMalloc(&s.cfh, sizeof(uint32_t) * (size_t)M * nprn));
-            h_code_spec(s.hp, nat, s.cfh, nprn, (float)sc, p->st);
-            PSA_CHECK(hipStreamSynchronize(p->st));
-            PSA_CHECK(hipFree(nat));
-            s.kappa = (float)((double)PSA_CSCALE / ((double)M * M * s.hp->osc_fc * s.hp->osc_fr * sc *
-                                                     s.hp->osc_ir));
         } else {
             PSA_CHECK(hipMalloc(&s.cft, spec * nprn));
             fft_to_transposed(s.pl, nat, s.cft, nprn, p->st);
@@ -884,14 +947,13 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         // length has a specialised kernel
         s.fused_peak = (s.hp || spec_length(s.pl->M1)) && !(flags & SDR_SEARCH_PLAIN) &&
                        !getenv("PSA_NO_FUSED_PEAK");
-        {
-            PassArgs tmp = {};
-            if (s.hp) h_col_args(s.hp, 1, tmp);
-            else col_pass_args(s.pl, 1, tmp);
-            s.nchunk_fused = nbin * pass_ngroups(tmp);
-        }
         s.xcd_order = xcd_order;
-        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused)));
+        if (!s.hp) {
+            PassArgs tmp = {};
+            col_pass_args(s.pl, 1, tmp);
+            s.nchunk_fused = nbin * pass_ngroups(tmp);
+            PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused)));
+        }
         // chunking of the intermediates (one word per complex in the
         // precision form)
         const size_t zspec = s.hp ? spec / 2 : spec;
@@ -909,6 +971,16 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     PSA_CHECK(hipMalloc(&p->X, sizeof(float2) * p->x_elems));
     PSA_CHECK(hipMalloc(&p->Z, sizeof(float2) * p->z_elems));
     if (flags & SDR_SEARCH_PLAIN) PSA_CHECK(hipMalloc(&p->iq16, 2 * p->tmp_elems));
+    // precision form: tile selection (SDR_SEARCH_TUNE), then the code spectra
+    // in the chosen split's order
+    for (auto &s : p->sig) {
+        if (!s.hp) continue;
+        p->cur_sig = (int)(&s - p->sig.data());
+        if (flags & SDR_SEARCH_TUNE) tune_group(p, s);
+        h_setup(p, s, s.hp->M1, s.hvar_col, s.hvar_row);
+        PSA_CHECK(hipFree(s.cf_nat));
+        s.cf_nat = nullptr;
+    }
     return p;
 }
 
@@ -975,6 +1047,18 @@ int sdr_search_plan_form(const sdr_search_plan_t *p, int sig)
     const SigDev &s = p->sig[sig];
     if (!s.hp) return SDR_FORM_FP32;
     return s.hp->dt == HDT_BF16 ? SDR_FORM_BF16 : SDR_FORM_FP16;
+}
+
+int sdr_search_plan_tiles(const sdr_search_plan_t *p, int sig, int *tiles)
+{
+    if (!p || sig < 0 || sig >= (int)p->sig.size() || !tiles) return 0;
+    const SigDev &s = p->sig[sig];
+    if (!s.hp) return 0;
+    tiles[0] = s.hp->M1;
+    tiles[1] = s.hp->M2;
+    tiles[2] = s.hvar_col;
+    tiles[3] = s.hvar_row;
+    return 1;
 }
 
 void sdr_search_plan_timing(sdr_search_plan_t *p, int on)

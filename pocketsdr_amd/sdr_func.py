@@ -71,6 +71,7 @@ _decl("sdr_search_plan_run", C.c_int, [c_void_p])
 _decl("sdr_search_plan_result", C.c_int, [c_void_p, C.c_int, c_float_p, c_int_p, c_double_p])
 _decl("sdr_search_plan_power", C.c_int, [c_void_p, C.c_int, C.c_int, c_float_p])
 _decl("sdr_search_plan_form", C.c_int, [c_void_p, C.c_int])
+_decl("sdr_search_plan_tiles", C.c_int, [c_void_p, C.c_int, c_int_p])
 _decl("sdr_search_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_search_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                              C.POINTER(C.c_longlong), c_double_p])
@@ -216,6 +217,7 @@ SEARCH_PLAIN = 1
 SEARCH_GRAPH = 2
 SEARCH_FP16 = 4
 SEARCH_BF16 = 8
+SEARCH_TUNE = 16
 FORM_FP32, FORM_PLAIN, FORM_FP16, FORM_BF16 = 0, 1, 2, 3
 
 
@@ -265,6 +267,13 @@ class SearchPlan:
     def form(self, sig):
         """FORM_FP32 / FORM_PLAIN / FORM_FP16 / FORM_BF16 of signal group sig."""
         return lib.sdr_search_plan_form(self.p, sig)
+
+    def tiles(self, sig):
+        """(M1, M2, column shape, row shape) of a precision-form group, else None."""
+        t = (C.c_int * 4)()
+        if not lib.sdr_search_plan_tiles(self.p, sig, C.cast(t, c_int_p)):
+            return None
+        return tuple(t)
 
     def timing(self, on=True):
         lib.sdr_search_plan_timing(self.p, 1 if on else 0)

@@ -37,12 +37,14 @@ def peak_f64(orc, w, s, q, b):
     return P[0]
 
 
-@pytest.mark.parametrize("form", ["fp16", "bf16"])
+@pytest.mark.parametrize("form", ["fp16", "bf16", "fp16-tuned"])
 def test_precision_form(sdr, orc, multi_weak, form):
     w = multi_weak
-    flag = sdr.SEARCH_FP16 if form == "fp16" else sdr.SEARCH_BF16
-    want = sdr.FORM_FP16 if form == "fp16" else sdr.FORM_BF16
-    surf_tol = 2e-2 if form == "fp16" else 5e-2
+    flag = sdr.SEARCH_BF16 if form == "bf16" else sdr.SEARCH_FP16
+    if form.endswith("tuned"):
+        flag |= sdr.SEARCH_TUNE        # tiles chosen by timing the candidates
+    want = sdr.FORM_BF16 if form == "bf16" else sdr.FORM_FP16
+    surf_tol = 5e-2 if form == "bf16" else 2e-2
     p32 = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_BATCHED)
     ph = sdr.SearchPlan(w["buffs"], w["sigs"], flag | sdr.SEARCH_GRAPH)
     p32.run()
@@ -51,6 +53,8 @@ def test_precision_form(sdr, orc, multi_weak, form):
     worst = 0.0
     for i, s in enumerate(w["sigs"]):
         assert ph.form(i) == want, s["name"]
+        M1, M2, vc, vr = ph.tiles(i)
+        assert M1 * M2 == s["M"] and vc in (0, 1) and vr in (0, 1)
         c32, ix32, d32 = p32.result(i)
         ch, ixh, dh = ph.result(i)
         det32 = c32 >= CN0_THRES
