@@ -68,8 +68,8 @@ def kernel_instance(cls, M, form="fp32", tiles=None):
     """Name of the pass kernel a (class, FFT length, form) launches, as
     rocprof lists it.  FP32: the four-step split M = M1 * M2 (rows M2 a
     multiple of 16 when possible, then the most balanced), column passes of
-    length M1, row passes of length M2.  FP16 / BF16: the plan's tiles
-    (M1, M2, column shape, row shape)."""
+    length M1, row passes of length M2 (the plan's tiles when given).
+    FP16 / BF16: the plan's tiles (M1, M2, column shape, row shape)."""
     if cls not in MODES:
         return cls
     if form in ("fp16", "bf16") and tiles:
@@ -81,7 +81,9 @@ def kernel_instance(cls, M, form="fp32", tiles=None):
         return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s, %d>" % (
             lm, sm, 0 if form == "fp16" else 1, L, r0, r1, r2, G, "false" if col else "true", nth)
     M1, M2 = M, 1
-    if M > 4096:
+    if tiles:
+        M1, M2 = tiles[0], tiles[1]
+    elif M > 4096:
         cands = [(0 if (M // d) % 16 == 0 else 1, abs(d - M // d), d) for d in range(1, 4097)
                  if M % d == 0 and M // d <= 4096]
         M1 = min(cands)[2]
@@ -112,7 +114,7 @@ def build_workload(name, seed):
     raise SystemExit("unknown workload " + name)
 
 
-FORM_FLAGS = {"fp32": 0, "fp16": 4 | 16, "bf16": 8 | 16}     # precision forms tuned
+FORM_FLAGS = {"fp32": 16, "fp16": 4 | 16, "bf16": 8 | 16}    # batched forms, tuned tiles
 FORM_DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 
 
@@ -167,9 +169,8 @@ def bench_acq(args, sdr, orc):
     names = ["fp32", "plain", "fp16", "bf16"]
     res["form_per_signal"] = {s["name"]: names[forms.index(plan.form(i))] for i, s in enumerate(sigs)}
     tiles = {s["name"]: plan.tiles(i) for i, s in enumerate(sigs)}
-    if any(tiles.values()):
-        res["tuned_tiles"] = {k: dict(M1=v[0], M2=v[1], col_shape=v[2], row_shape=v[3])
-                              for k, v in tiles.items() if v}
+    res["tuned_tiles"] = {k: dict(M1=v[0], M2=v[1], col_shape=v[2], row_shape=v[3])
+                          for k, v in tiles.items() if v}
     res["detections"] = decisions(sigs, r_main, truth)
     # per-kernel HIP events on the plan's stream over timed steps
     plan.timing(True)
@@ -208,7 +209,9 @@ def bench_acq(args, sdr, orc):
                       for k, v in ks.items()}
     if form != "fp32" and not args.skip_fp32:
         p32, t32, r32 = run_form(sdr, buffs, sigs, "fp32", args.graph, args.steps, args.warmup)
+        p32_tiles = [p32.tiles(i) for i in range(len(sigs))]
         p32.close()
+        res["fp32_batched_tiles"] = {s["name"]: list(p32_tiles[i][:2]) for i, s in enumerate(sigs)}
         res["fp32_batched_cells_per_s"] = round(cells / t32, 1)
         res["fp32_batched_ms_per_step"] = round(t32 * 1e3, 4)
         res["same_decisions_as_fp32"] = same_decisions(sigs, r_main, r32)

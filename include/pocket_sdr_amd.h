@@ -142,9 +142,10 @@ typedef struct {
  * groups of the plan run the FP32 batched path.  Not combined with PLAIN. */
 #define SDR_SEARCH_FP16     4
 #define SDR_SEARCH_BF16     8
-/* With FP16 / BF16: sdr_search_plan_new times each signal group's candidate
- * FFT splits and workgroup shapes on the group's first block and keeps the
- * fastest (adds a few runs of one block per candidate to plan creation). */
+/* Batched forms: sdr_search_plan_new times each signal group's candidate
+ * FFT splits (and, in the precision form, workgroup shapes) on the group's
+ * first block and keeps the fastest (adds a few runs of one block per
+ * candidate to plan creation). */
 #define SDR_SEARCH_TUNE     16
 
 typedef struct sdr_search_plan_tag sdr_search_plan_t;
@@ -169,9 +170,9 @@ int sdr_search_plan_power(const sdr_search_plan_t *plan, int sig, int prn_idx,
 #define SDR_FORM_FP16   2      /* batched, FP16 words, MFMA stages */
 #define SDR_FORM_BF16   3      /* batched, BF16 words, MFMA stages */
 int sdr_search_plan_form(const sdr_search_plan_t *plan, int sig);
-/* Precision-form tiles of group `sig`: tiles[0..3] = column length M1, row
- * length M2, column and row workgroup-shape variant (0 or 1).  Returns 0 for
- * groups not in the precision form. */
+/* Tiles of group `sig`: tiles[0..3] = column length M1, row length M2 of the
+ * FFT split, column and row workgroup-shape variant (0 or 1; precision form
+ * only).  Returns 0 on a bad argument. */
 int sdr_search_plan_tiles(const sdr_search_plan_t *plan, int sig, int *tiles);
 /* Enables per-kernel HIP-event timing of later runs (0 disables). */
 void sdr_search_plan_timing(sdr_search_plan_t *plan, int on);

@@ -487,19 +487,20 @@ static void split_len(int M, int *M1, int *M2)
 }
 
 static std::mutex g_plan_mtx;
-static std::map<int, FFTPlan *> g_plans;
+static std::map<std::pair<int, int>, FFTPlan *> g_plans;
 
-const FFTPlan *fft_plan(int M)
+const FFTPlan *fft_plan(int M, int M1)
 {
     if (!fft_supported(M)) return nullptr;
     ensure_init();
     fft_init_attrs();
+    int M2;
+    if (M1 == 0) split_len(M, &M1, &M2);
+    if (M1 <= 0 || M % M1 || M1 > 4096 || M / M1 > 4096) return nullptr;
+    M2 = M / M1;
     std::lock_guard<std::mutex> lk(g_plan_mtx);
-    auto it = g_plans.find(M);
+    auto it = g_plans.find({M, M1});
     if (it != g_plans.end()) return it->second;
-    int M1, M2;
-    split_len(M, &M1, &M2);
-    if (M1 == 0) return nullptr;
     FFTPlan *p = new FFTPlan();
     p->M = M; p->M1 = M1; p->M2 = M2;
     radix_seq(M1, p->s1);
@@ -510,7 +511,7 @@ const FFTPlan *fft_plan(int M)
     p->s2.tw_bwd = make_twiddle(M2, +1, 1.0);
     make_split_twiddle(M, -1, &p->thi_fwd, &p->tlo_fwd);
     make_split_twiddle(M, +1, &p->thi_bwd, &p->tlo_bwd);
-    g_plans[M] = p;
+    g_plans[{M, M1}] = p;
     return p;
 }
 
@@ -614,6 +615,19 @@ void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hi
 bool spec_length(int L)
 {
     return spec_groups(L) > 0;
+}
+
+int fft_splits(int M, int *m1, int nmax)
+{
+    int d1, d2, n = 0;
+    split_len(M, &d1, &d2);
+    if (d1 > 0 && n < nmax) m1[n++] = d1;
+    if (M <= 4096) return n;
+    for (int L = 1; L <= 4096 && n < nmax; L++) {
+        if (M % L || L == d1 || !spec_length(L) || !spec_length(M / L)) continue;
+        m1[n++] = L;
+    }
+    return n;
 }
 
 // G sub-transforms per workgroup: the specialised count where one exists,

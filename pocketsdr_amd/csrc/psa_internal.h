@@ -93,7 +93,12 @@ struct FFTPlan {
 bool fft_supported(int M);
 // device table of scale * exp(sign 2 pi i m / L), m < L, rounded from double
 float2 *make_twiddle(int L, int sign, double scale);
-const FFTPlan *fft_plan(int M);   // cached, thread-safe; nullptr if unsupported
+// cached per (M, M1), thread-safe; M1 = 0 takes the default split (rows a
+// multiple of 16, then the most balanced); nullptr if unsupported
+const FFTPlan *fft_plan(int M, int M1 = 0);
+// column lengths of the splits of M: the default, then the other splits
+// whose both lengths have specialised kernels; returns the count
+int fft_splits(int M, int *m1, int nmax);
 void fft_init_attrs();            // LDS limits of the pass kernels (once)
 
 // partial peak reduction record: max value, its first row-major index within

@@ -776,28 +776,51 @@ static void h_setup(sdr_search_plan_t *p, SigDev &s, int M1, int vc, int vr)
     }
 }
 
-// SDR_SEARCH_TUNE: time every (split, column shape, row shape) of the group's
-// MFMA passes on its own shapes - the first block of the search, best of two
-// runs after a warm-up - and keep the fastest.
+// FP32 batched group set up for split M1: code spectra in its
+// digit-transposed order, the peak partial buffer.
+static void f_setup(sdr_search_plan_t *p, SigDev &s, int M1)
+{
+    s.pl = fft_plan(s.h.M, M1);
+    fft_to_transposed(s.pl, s.cf_nat, s.cft, s.h.n_prn, p->st);
+    PSA_CHECK(hipStreamSynchronize(p->st));
+    s.fused_peak = spec_length(s.pl->M1) && !getenv("PSA_NO_FUSED_PEAK");
+    PassArgs tmp = {};
+    col_pass_args(s.pl, 1, tmp);
+    const int nf = s.h.len_fds * pass_ngroups(tmp);
+    if (nf != s.nchunk_fused || !s.part) {
+        if (s.part) PSA_CHECK(hipFree(s.part));
+        s.nchunk_fused = nf;
+        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)s.h.n_prn * std::max(s.nchunk, s.nchunk_fused)));
+    }
+}
+
+// SDR_SEARCH_TUNE: time every candidate tiling of the group's passes on its
+// own shapes - the first block of the search, best of two runs after a
+// warm-up - and keep the fastest.  Precision form: (split, column shape, row
+// shape); FP32: the split.
 static void tune_group(sdr_search_plan_t *p, SigDev &s)
 {
     int m1[8];
-    const int ns = h_splits(s.h.M, m1, 8);
+    const bool h = s.hp != nullptr;
+    const int ns = h ? h_splits(s.h.M, m1, 8) : fft_splits(s.h.M, m1, 8);
+    const int nv = h ? 2 : 1;
     const int nblk = s.h.n_blk;
     hipEvent_t e0, e1;
     PSA_CHECK(hipEventCreate(&e0));
     PSA_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
-    int bm = s.hp->M1, bc = 0, br = 0;
+    int bm = m1[0], bc = 0, br = 0;
     s.h.n_blk = 1;
     for (int i = 0; i < ns; i++) {
-        for (int vc = 0; vc < 2; vc++) {
-            for (int vr = 0; vr < 2; vr++) {
-                h_setup(p, s, m1[i], vc, vr);
+        for (int vc = 0; vc < nv; vc++) {
+            for (int vr = 0; vr < nv; vr++) {
+                if (h) h_setup(p, s, m1[i], vc, vr);
+                else f_setup(p, s, m1[i]);
                 float t = 1e30f;
                 for (int rep = 0; rep < 3; rep++) {
                     PSA_CHECK(hipEventRecord(e0, p->st));
-                    run_batched_h(p, s);
+                    if (h) run_batched_h(p, s);
+                    else run_batched(p, s);
                     PSA_CHECK(hipEventRecord(e1, p->st));
                     PSA_CHECK(hipEventSynchronize(e1));
                     float ms = 0.0f;
@@ -809,7 +832,8 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
         }
     }
     s.h.n_blk = nblk;
-    s.hp = hfft_plan(s.h.M, s.hp->dt, bm);
+    if (h) s.hp = hfft_plan(s.h.M, s.hp->dt, bm);
+    else s.pl = fft_plan(s.h.M, bm);
     s.hvar_col = bc;
     s.hvar_row = br;
     s.tuned_ms = best;
@@ -908,10 +932,8 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
             s.cf_nat = nat;
             PSA_CHECK(hipMalloc(&s.cfh, sizeof(uint32_t) * (size_t)M * nprn));
         } else {
+            s.cf_nat = nat;
             PSA_CHECK(hipMalloc(&s.cft, spec * nprn));
-            fft_to_transposed(s.pl, nat, s.cft, nprn, p->st);
-            PSA_CHECK(hipStreamSynchronize(p->st));
-            PSA_CHECK(hipFree(nat));
         }
         PSA_CHECK(hipMalloc(&s.P, sizeof(float) * (size_t)nprn * nbin * s.h.N));
         PSA_CHECK(hipMalloc(&s.fds, sizeof(float) * nbin));
@@ -948,11 +970,8 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         s.fused_peak = (s.hp || spec_length(s.pl->M1)) && !(flags & SDR_SEARCH_PLAIN) &&
                        !getenv("PSA_NO_FUSED_PEAK");
         s.xcd_order = xcd_order;
-        if (!s.hp) {
-            PassArgs tmp = {};
-            col_pass_args(s.pl, 1, tmp);
-            s.nchunk_fused = nbin * pass_ngroups(tmp);
-            PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused)));
+        if (flags & SDR_SEARCH_PLAIN) {
+            PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * s.nchunk));
         }
         // chunking of the intermediates (one word per complex in the
         // precision form)
@@ -971,13 +990,14 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     PSA_CHECK(hipMalloc(&p->X, sizeof(float2) * p->x_elems));
     PSA_CHECK(hipMalloc(&p->Z, sizeof(float2) * p->z_elems));
     if (flags & SDR_SEARCH_PLAIN) PSA_CHECK(hipMalloc(&p->iq16, 2 * p->tmp_elems));
-    // precision form: tile selection (SDR_SEARCH_TUNE), then the code spectra
+    // batched forms: tile selection (SDR_SEARCH_TUNE), then the code spectra
     // in the chosen split's order
     for (auto &s : p->sig) {
-        if (!s.hp) continue;
+        if (!s.cf_nat) continue;
         p->cur_sig = (int)(&s - p->sig.data());
         if (flags & SDR_SEARCH_TUNE) tune_group(p, s);
-        h_setup(p, s, s.hp->M1, s.hvar_col, s.hvar_row);
+        if (s.hp) h_setup(p, s, s.hp->M1, s.hvar_col, s.hvar_row);
+        else f_setup(p, s, s.pl->M1);
         PSA_CHECK(hipFree(s.cf_nat));
         s.cf_nat = nullptr;
     }
@@ -1053,9 +1073,8 @@ int sdr_search_plan_tiles(const sdr_search_plan_t *p, int sig, int *tiles)
 {
     if (!p || sig < 0 || sig >= (int)p->sig.size() || !tiles) return 0;
     const SigDev &s = p->sig[sig];
-    if (!s.hp) return 0;
-    tiles[0] = s.hp->M1;
-    tiles[1] = s.hp->M2;
+    tiles[0] = s.hp ? s.hp->M1 : s.pl->M1;
+    tiles[1] = s.hp ? s.hp->M2 : s.pl->M2;
     tiles[2] = s.hvar_col;
     tiles[3] = s.hvar_row;
     return 1;

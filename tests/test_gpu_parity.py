@@ -240,11 +240,16 @@ def multi_acq():
                                code_fft_fn=lambda c, T, fs, N: orc.gen_code_fft(c, T, 0.0, fs, N, N))
 
 
-def test_search_plan_multi(sdr, orc, multi_acq):
+@pytest.mark.parametrize("tune", [False, True])
+def test_search_plan_multi(sdr, orc, multi_acq, tune):
     """Uneven batched search: L1CA (M 24000), E1B (96000), L5Q (48000) and
-    L6D (192000) in one plan, against the oracle's sdr_search_code."""
+    L6D (192000) in one plan, against the oracle's sdr_search_code; with the
+    tuner choosing each group's FFT split."""
     w = multi_acq
-    plan = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_GRAPH)
+    plan = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_GRAPH | (sdr.SEARCH_TUNE if tune else 0))
+    for i, s in enumerate(w["sigs"]):
+        M1, M2, _, _ = plan.tiles(i)
+        assert M1 * M2 == s["M"]
     plan.run()
     for i, s in enumerate(w["sigs"]):
         cn0, ix, dop = plan.result(i)
