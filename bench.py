@@ -56,12 +56,14 @@ HCOL = {(150, 0): (10, 15, 1, 32, 256), (150, 1): (10, 15, 1, 32, 512),
         (200, 0): (10, 20, 1, 32, 512), (200, 1): (10, 20, 1, 32, 256),
         (300, 0): (15, 20, 1, 32, 512), (300, 1): (15, 20, 1, 32, 256),
         (400, 0): (16, 25, 1, 32, 512), (400, 1): (16, 25, 1, 32, 256),
-        (960, 0): (32, 30, 1, 16, 512), (960, 1): (32, 30, 1, 16, 256)}
+        (960, 0): (32, 30, 1, 16, 512), (960, 1): (32, 30, 1, 16, 256),
+        (1000, 0): (10, 10, 10, 8, 512), (1000, 1): (10, 10, 10, 16, 512)}
 HROW = {(160, 0): (16, 10, 1, 32, 256), (160, 1): (16, 10, 1, 32, 512),
         (240, 0): (16, 15, 1, 32, 256), (240, 1): (16, 15, 1, 32, 512),
         (320, 0): (16, 20, 1, 16, 256), (320, 1): (16, 20, 1, 32, 512),
         (480, 0): (16, 30, 1, 16, 512), (480, 1): (16, 30, 1, 16, 256),
-        (1000, 0): (10, 10, 10, 4, 256), (1000, 1): (10, 10, 10, 8, 512)}
+        (1000, 0): (10, 10, 10, 4, 256), (1000, 1): (10, 10, 10, 8, 512),
+        (960, 0): (32, 30, 1, 8, 256), (960, 1): (32, 30, 1, 8, 512)}
 
 
 def kernel_instance(cls, M, form="fp32", tiles=None):

@@ -439,19 +439,22 @@ __global__ void h_spec_kernel(const float2 *__restrict__ in, uint32_t *__restric
 // multiple of 32 where possible).  Each length has two workgroup shapes; the
 // plan's tuner (SDR_SEARCH_TUNE) times both on the real shapes, variant 0 is
 // the default.  Splits: 24000 = 150 x 160, 48000 = 150 x 320 or 200 x 240,
-// 96000 = 300 x 320 or 200 x 480, 192000 = 400 x 480, 960000 = 960 x 1000.
+// 96000 = 300 x 320 or 200 x 480, 192000 = 400 x 480, 960000 = 960 x 1000
+// or 1000 x 960.
 #define PSA_HCOL(X) \
     X(150, 10, 15, 1, 32, 256, 0) X(150, 10, 15, 1, 32, 512, 1) \
     X(200, 10, 20, 1, 32, 512, 0) X(200, 10, 20, 1, 32, 256, 1) \
     X(300, 15, 20, 1, 32, 512, 0) X(300, 15, 20, 1, 32, 256, 1) \
     X(400, 16, 25, 1, 32, 512, 0) X(400, 16, 25, 1, 32, 256, 1) \
-    X(960, 32, 30, 1, 16, 512, 0) X(960, 32, 30, 1, 16, 256, 1)
+    X(960, 32, 30, 1, 16, 512, 0) X(960, 32, 30, 1, 16, 256, 1) \
+    X(1000, 10, 10, 10, 8, 512, 0) X(1000, 10, 10, 10, 16, 512, 1)
 #define PSA_HROW(X) \
     X(160, 16, 10, 1, 32, 256, 0) X(160, 16, 10, 1, 32, 512, 1) \
     X(240, 16, 15, 1, 32, 256, 0) X(240, 16, 15, 1, 32, 512, 1) \
     X(320, 16, 20, 1, 16, 256, 0) X(320, 16, 20, 1, 32, 512, 1) \
     X(480, 16, 30, 1, 16, 512, 0) X(480, 16, 30, 1, 16, 256, 1) \
-    X(1000, 10, 10, 10, 4, 256, 0) X(1000, 10, 10, 10, 8, 512, 1)
+    X(1000, 10, 10, 10, 4, 256, 0) X(1000, 10, 10, 10, 8, 512, 1) \
+    X(960, 32, 30, 1, 8, 256, 0) X(960, 32, 30, 1, 8, 512, 1)
 
 struct HSpec { int L, r[3], G, nth, var; };
 static const HSpec h_col_specs[] = {
