@@ -22,6 +22,35 @@ __device__ __forceinline__ int sgn8(int x, int c)
     return c == 0 ? 0 : x;
 }
 
+// aI += sum over NW words of x_I . c_I (4 int8 lanes each), aQ likewise;
+// EXACT: _mm256_sign_epi8 products
+template <bool EXACT, int NW = 4>
+__device__ __forceinline__ void dot4x4(const uint32_t *xi, const uint32_t *xq, const uint32_t *ci,
+                                       const uint32_t *cq, int &aI, int &aQ)
+{
+#pragma unroll
+    for (int v = 0; v < NW; v++) {
+        if (EXACT) {
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                aI += sgn8((int8_t)(xi[v] >> (8 * b)), (int8_t)(ci[v] >> (8 * b)));
+                aQ += sgn8((int8_t)(xq[v] >> (8 * b)), (int8_t)(cq[v] >> (8 * b)));
+            }
+        } else {
+            aI = __builtin_amdgcn_sdot4((int)xi[v], (int)ci[v], aI, false);
+            aQ = __builtin_amdgcn_sdot4((int)xq[v], (int)cq[v], aQ, false);
+        }
+    }
+}
+
+// 16-byte load from a 4-byte aligned address
+__device__ __forceinline__ uint4 load_u4(const uint32_t *p)
+{
+    uint4 v;
+    __builtin_memcpy(&v, p, sizeof(v));
+    return v;
+}
+
 __device__ __forceinline__ int wave_sum(int v)
 {
 #pragma unroll
@@ -341,8 +370,8 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
 // epilogue.  EXACT selects _mm256_sign_epi8 semantics for IF buffers that
 // hold -8 nibbles (the only source of -128 mixed values); SAMEQ (I = Q in
 // every bank, true for real codes) loads one code plane.
-#define TRK_THREADS 1024
-#define TRK_TPW 3
+#define TRK_THREADS 512
+#define TRK_TPW 6
 
 template <bool EXACT, bool SAMEQ>
 __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
@@ -359,7 +388,8 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     __shared__ float4 c12s[PSA_MAX_CORR];
     __shared__ int2 jk[PSA_MAX_CORR];
     const int Nq = (N + 3) / 4;
-    uint32_t *planeI = lds, *planeQ = lds + Nq;
+    const int Nq4 = (Nq + 255) / 256 * 256;    // planes padded to whole steps
+    uint32_t *planeI = lds, *planeQ = lds + Nq4;
     const long long o = blockIdx.x;          // channel * n_epoch + epoch
     const int ch = (int)(o / n_epoch);
     const TrkEpoch q = ep[o];
@@ -376,8 +406,8 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     }
     __syncthreads();
 
-    // ---- mix N samples into the I / Q planes (zero padded to Nq words) ----
-    for (int w = threadIdx.x; w < Nq; w += TRK_THREADS) {
+    // ---- mix N samples into the I / Q planes (zero padded to Nq4 words) ----
+    for (int w = threadIdx.x; w < Nq4; w += TRK_THREADS) {
         uint32_t wi = 0, wq = 0;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
@@ -421,113 +451,102 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
             cp[s] = cbase + cpy * cstride + ((g.y * NP) >> 2) + (c0 - cpy) / 4;
             acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0;
         }
-        for (int wb = 0; wb < Nq; wb += 64) {
-            const int w = wb + lane;
-            // lanes past the last word read word Nq - 1 of the code (inside
-            // the doubled row) against zero data
-            const int wc = w < Nq ? w : Nq - 1;
-            const uint32_t xI = w < Nq ? planeI[w] : 0u;
-            const uint32_t xQ = w < Nq ? planeQ[w] : 0u;
+        // a step covers 256 words: lane l takes words 4 l .. 4 l + 3 of it with
+        // one 16-byte load per tap (planes padded with zero words to Nq4)
+        for (int wb = 0; wb < Nq; wb += 256) {
+            const int w = wb + 4 * lane;
+            const uint4 xI = *reinterpret_cast<const uint4 *>(planeI + w);
+            const uint4 xQ = *reinterpret_cast<const uint4 *>(planeQ + w);
+            uint4 cI[TRK_TPW], cQ[TRK_TPW];
+#pragma unroll
+            for (int s = 0; s < TRK_TPW; s++) {
+                cI[s] = load_u4(cp[s] + w);
+                cQ[s] = SAMEQ ? cI[s] : load_u4(cp[s] + w + (QOFF >> 2));
+            }
 #pragma unroll
             for (int s = 0; s < TRK_TPW; s++) {
                 const int j = jt[s];
-                const uint32_t cI = cp[s][wc];
-                const uint32_t cQ = SAMEQ ? cI : cp[s][wc + (QOFF >> 2)];
-                int dI, dQ;
-                if (EXACT) {
-                    dI = 0; dQ = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; b++) {
-                        dI += sgn8((int8_t)(xI >> (8 * b)), (int8_t)(cI >> (8 * b)));
-                        dQ += sgn8((int8_t)(xQ >> (8 * b)), (int8_t)(cQ >> (8 * b)));
-                    }
-                }
-                // this step covers samples 4 wb .. 4 wb + 255 (wave-uniform test)
-                if (4 * wb + 256 <= j) {
-                    if (EXACT) { acc[s][0] += dI; acc[s][1] += dQ; }
-                    else {
-                        acc[s][0] = __builtin_amdgcn_sdot4((int)xI, (int)cI, acc[s][0], false);
-                        acc[s][1] = __builtin_amdgcn_sdot4((int)xQ, (int)cQ, acc[s][1], false);
-                    }
+                const uint32_t xi[4] = {xI.x, xI.y, xI.z, xI.w}, xq[4] = {xQ.x, xQ.y, xQ.z, xQ.w};
+                const uint32_t ci[4] = {cI[s].x, cI[s].y, cI[s].z, cI[s].w};
+                const uint32_t cq[4] = {cQ[s].x, cQ[s].y, cQ[s].z, cQ[s].w};
+                // this step covers samples 4 wb .. 4 wb + 1023 (wave-uniform test)
+                if (4 * wb + 1024 <= j) {
+                    dot4x4<EXACT>(xi, xq, ci, cq, acc[s][0], acc[s][1]);
                 } else if (4 * wb >= j) {
-                    if (EXACT) { acc[s][2] += dI; acc[s][3] += dQ; }
-                    else {
-                        acc[s][2] = __builtin_amdgcn_sdot4((int)xI, (int)cI, acc[s][2], false);
-                        acc[s][3] = __builtin_amdgcn_sdot4((int)xQ, (int)cQ, acc[s][3], false);
-                    }
+                    dot4x4<EXACT>(xi, xq, ci, cq, acc[s][2], acc[s][3]);
                 } else {
-                    if (!EXACT) {
-                        dI = __builtin_amdgcn_sdot4((int)xI, (int)cI, 0, false);
-                        dQ = __builtin_amdgcn_sdot4((int)xQ, (int)cQ, 0, false);
-                    }
                     // words wholly before / after sample j; the straddling
                     // word is added by the epilogue
-                    if (4 * w + 4 <= j) { acc[s][0] += dI; acc[s][1] += dQ; }
-                    else if (4 * w >= j) { acc[s][2] += dI; acc[s][3] += dQ; }
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        int dI = 0, dQ = 0;
+                        dot4x4<EXACT, 1>(xi + v, xq + v, ci + v, cq + v, dI, dQ);
+                        const int m = 4 * (w + v);
+                        if (m + 4 <= j) { acc[s][0] += dI; acc[s][1] += dQ; }
+                        else if (m >= j) { acc[s][2] += dI; acc[s][3] += dQ; }
+                    }
                 }
             }
         }
-#pragma unroll
+        // ---- per-tap float epilogue (the owning wave): the integer sums hold
+        // the first 16 floor(len / 16) samples of each half, the rest are
+        // added in float one by one (dot_IQ_code's AVX2 body and scalar
+        // tail, src/sdr_func.c:861-912).  Lanes 0-15 / 16-31 fetch the tail
+        // samples of half 1 / 2, lanes 32-35 the word straddling j, all at
+        // once; lane 0 then adds the tail in order.
+        const uint8_t *pI = (const uint8_t *)planeI, *pQ = (const uint8_t *)planeQ;
         for (int s = 0; s < TRK_TPW; s++) {
+            const int t = t0 + s * nwave;
+            if (t >= n_pos) break;
+            int S[4];
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int v = wave_sum(acc[s][c]);
-                if (lane == 0 && t0 + s * nwave < n_pos) sums[t0 + s * nwave][c] = v;
+            for (int c = 0; c < 4; c++) S[c] = wave_sum(acc[s][c]);
+            const int2 g = jk[t];
+            const int j = g.x, k = g.y, len2 = N - j;
+            const int V1 = j >= 16 ? j / 16 * 16 : 0;
+            const int V2 = len2 >= 16 ? len2 / 16 * 16 : 0;
+            int m = -1, ci = 0, slot = 0;   // slot 0/1: tail of half 1/2; 2/3: straddle in half 1/2
+            if (lane < 16) {
+                const int i = V1 + lane;
+                if (i < j) { m = i; ci = N - j + i; slot = 0; }
+            } else if (lane < 32) {
+                const int i = V2 + lane - 16;
+                if (i < len2) { m = j + i; ci = i; slot = 1; }
+            } else if (lane < 36) {
+                const int mm = (j & ~3) + lane - 32;
+                if ((j & 3) && mm < N) { m = mm; ci = mm < j ? N - j + mm : mm - j; slot = mm < j ? 2 : 3; }
+            }
+            int xI = 0, xQ = 0, vI = 0, vQ = 0;
+            if (m >= 0) {
+                xI = (int8_t)pI[m]; xQ = (int8_t)pQ[m];
+                vI = (int8_t)cb[(long long)k * NP + ci];
+                vQ = (int8_t)cb[QOFF + (long long)k * NP + ci];
+            }
+            const int eI = EXACT ? sgn8(xI, vI) : xI * vI, eQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
+            const int fI = xI * vI, fQ = xQ * vQ;
+            // integer sums: + straddle - tail, per half
+            const int sgn = slot >= 2 ? 1 : -1;
+            const bool h1 = slot == 0 || slot == 2;
+            const int aI1 = wave_sum(m >= 0 && h1 ? sgn * eI : 0), aQ1 = wave_sum(m >= 0 && h1 ? sgn * eQ : 0);
+            const int aI2 = wave_sum(m >= 0 && !h1 ? sgn * eI : 0), aQ2 = wave_sum(m >= 0 && !h1 ? sgn * eQ : 0);
+            float f1I = (float)(S[0] + aI1), f1Q = (float)(S[1] + aQ1);
+            float f2I = (float)(S[2] + aI2), f2Q = (float)(S[3] + aQ2);
+            for (int i = 0; i < j - V1; i++) {
+                f1I += (float)__builtin_amdgcn_readlane(fI, i);
+                f1Q += (float)__builtin_amdgcn_readlane(fQ, i);
+            }
+            for (int i = 0; i < len2 - V2; i++) {
+                f2I += (float)__builtin_amdgcn_readlane(fI, 16 + i);
+                f2Q += (float)__builtin_amdgcn_readlane(fQ, 16 + i);
+            }
+            if (lane == 0) {
+                c12s[t] = make_float4(f1I * (1.0f * PSA_CSCALE), f1Q * (1.0f * PSA_CSCALE),
+                                      f2I * (1.0f * PSA_CSCALE), f2Q * (1.0f * PSA_CSCALE));
             }
         }
     }
     __syncthreads();
 
-    // ---- per-tap float epilogue: S_vec = S_total - tail products ----
-    for (int t = threadIdx.x; t < n_pos; t += blockDim.x) {
-        const int2 g = jk[t];
-        const int j = g.x, k = g.y;
-        const int V1 = j >= 16 ? j / 16 * 16 : 0;
-        const int len2 = N - j;
-        const int V2 = len2 >= 16 ? len2 / 16 * 16 : 0;
-        const uint8_t *pI = (const uint8_t *)planeI, *pQ = (const uint8_t *)planeQ;
-        // the word holding samples j - j % 4 .. j - j % 4 + 3 straddles the
-        // split when j % 4 != 0: its samples go to their halves here
-        int st[4] = {0, 0, 0, 0};
-        if (j & 3) {
-            for (int m = j & ~3; m < (j & ~3) + 4 && m < N; m++) {
-                const int ci = m < j ? N - j + m : m - j;
-                const int xI = (int8_t)pI[m], xQ = (int8_t)pQ[m];
-                const int vI = (int8_t)cb[(long long)k * NP + ci];
-                const int vQ = (int8_t)cb[QOFF + (long long)k * NP + ci];
-                const int h = m < j ? 0 : 2;
-                st[h] += EXACT ? sgn8(xI, vI) : xI * vI;
-                st[h + 1] += EXACT ? sgn8(xQ, vQ) : xQ * vQ;
-            }
-        }
-        float out[4];
-        for (int h = 0; h < 2; h++) {
-            const int d0 = h == 0 ? 0 : j;          // first data sample of the half
-            const int c0 = h == 0 ? N - j : 0;      // first code sample of the half
-            const int v = h == 0 ? V1 : V2, len = h == 0 ? j : len2;
-            int sI = sums[t][2 * h] + st[2 * h], sQ = sums[t][2 * h + 1] + st[2 * h + 1];
-            // remove the tail products (they were summed in the bulk loop)
-            for (int i = v; i < len; i++) {
-                const int xI = (int8_t)pI[d0 + i], xQ = (int8_t)pQ[d0 + i];
-                const int vI = (int8_t)cb[(long long)k * NP + c0 + i];
-                const int vQ = (int8_t)cb[(long long)(PSA_N_CODES + k) * NP + c0 + i];
-                sI -= EXACT ? sgn8(xI, vI) : xI * vI;
-                sQ -= EXACT ? sgn8(xQ, vQ) : xQ * vQ;
-            }
-            float fI = (float)sI, fQ = (float)sQ;
-            for (int i = v; i < len; i++) {
-                const int xI = (int8_t)pI[d0 + i], xQ = (int8_t)pQ[d0 + i];
-                const int vI = (int8_t)cb[(long long)k * NP + c0 + i];
-                const int vQ = (int8_t)cb[(long long)(PSA_N_CODES + k) * NP + c0 + i];
-                fI += (float)(xI * vI);
-                fQ += (float)(xQ * vQ);
-            }
-            out[2 * h] = fI * (1.0f * PSA_CSCALE);
-            out[2 * h + 1] = fQ * (1.0f * PSA_CSCALE);
-        }
-        c12s[t] = make_float4(out[0], out[1], out[2], out[3]);
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
         float dot = 0.0f;
         for (int i = 0; i < 3 && i < n_pos; i++) {
@@ -647,15 +666,16 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
         }
     }
     // planar int8 I / Q for all tap geometries plus one word of slack
-    const int Nq = (N + 3) / 4;
-    if ((size_t)Nq * 8 + 4096 > 150 * 1024) {
+    const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
+    if ((size_t)Nq4 * 8 + 4096 > 150 * 1024) {
         fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new: N=%d too large for one workgroup\n", N);
         return nullptr;
     }
     ensure_init();
     sdr_trk_plan_t *p = new sdr_trk_plan_t();
     p->n_ch = n_ch; p->n_pos = n_pos; p->n_epoch = n_epoch; p->N = N; p->buffN = buff->N;
-    p->NP = (2 * N + 4 + 3) / 4 * 4;
+    // doubled bank plus the words the last (zero-data) step reads past it
+    p->NP = (2 * N + 1024 + 16 + 3) / 4 * 4;
     p->sameq = true;
     for (int c = 0; c < n_ch && p->sameq; c++) {
         for (size_t i = 0; i < (size_t)N * PSA_N_CODES; i++) {
@@ -748,7 +768,7 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
                        p->buffN, p->d_ep);
     PSA_CHECK(hipGetLastError());
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
-    const size_t shm = sizeof(uint32_t) * 2 * (size_t)((p->N + 3) / 4);
+    const size_t shm = sizeof(uint32_t) * 2 * (size_t)(((p->N + 3) / 4 + 255) / 256 * 256);
     trk_launch(p, shm, (unsigned)ne);
     PSA_CHECK(hipGetLastError());
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[2], st));
