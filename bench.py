@@ -163,6 +163,8 @@ def bench_acq(args, sdr, orc):
     from pocketsdr_amd import sdr_func
     buffs, sigs, truth, wname = build_workload(args.workload, args.seed)
     cells = sum(s["code_fft"].shape[0] * len(s["fds"]) for s in sigs)
+    # counter summary of the same workload (profiles/search_<workload>_traffic.md)
+    traffic_profile = "search_%s_traffic.md" % args.workload
     res = {}
     form = args.form
     plan, t_b, r_main = run_form(sdr, buffs, sigs, form, args.graph, args.steps, args.warmup)
@@ -184,6 +186,7 @@ def bench_acq(args, sdr, orc):
             r = plan.kernel_ms("%s@%d" % (n, i))
             if r:
                 ks["%s@%s" % (n, s["name"])] = dict(avg_ms=r[0], launches=r[1], elems=r[2], bytes=r[3],
+                                                    grid=plan.kernel_grid("%s@%d" % (n, i)),
                                                     M=s["M"], cls=n, form=res["form_per_signal"][s["name"]],
                                                     tiles=tiles[s["name"]])
     # dominant kernel: the instantiation (as rocprof names it) with the most
@@ -192,18 +195,20 @@ def bench_acq(args, sdr, orc):
     inst_tot = {}
     for k, v in ks.items():
         inst = kernel_instance(v["cls"], v["M"], v["form"], v["tiles"])
-        t = inst_tot.setdefault(inst, dict(ms=0.0, launches=0, bytes=0.0, parts=[]))
+        t = inst_tot.setdefault(inst, dict(ms=0.0, launches=0, bytes=0.0, parts=[], grids={}))
         t["ms"] += v["avg_ms"] * v["launches"]
         t["launches"] += v["launches"]
         t["bytes"] += v["bytes"] * v["launches"]
         t["parts"].append(k)
+        g = v["grid"] or 0
+        t["grids"][g] = t["grids"].get(g, 0) + v["launches"]
     inst = max(inst_tot, key=lambda i: inst_tot[i]["ms"])
     d = inst_tot[inst]
     avg_ms = d["ms"] / d["launches"]
     alg = d["bytes"] / d["launches"]
     gbs = alg / (avg_ms * 1e-3) / 1e9
     res["roofline"] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic_of(inst), kernel=inst,
+                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic_of(inst, d["grids"], traffic_profile), kernel=inst,
                            launches_of=d["parts"], alg_bytes_per_launch=round(alg),
                            avg_launch_ms=round(avg_ms, 5))
     res["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 5), launches=v["launches"],
@@ -276,20 +281,29 @@ def cpu_leg(args, orc, plan, buffs, sigs):
                       note="peak cell of each sampled PRN, timed form vs FP64 oracle")
 
 
-def traffic_of(inst):
+def traffic_of(inst, grids, profile="search_multi_traffic.md"):
     """HBM bytes per launch of a pass instantiation from the committed
-    FETCH_SIZE / WRITE_SIZE summary of the same workload (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "search_multi_traffic.md")
+    FETCH_SIZE / WRITE_SIZE summary of the same workload: the measured
+    traffic of each launch size (grid), averaged with the weights of the timed
+    launches, `grids` = {grid: launches}.  None if a size is missing."""
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
+    per = {}
     for line in open(path):
         cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 5 and cells[1].strip("`") == inst:
+        if len(cells) > 6 and cells[1].strip("`") == inst:
             try:
-                return round((float(cells[3]) + float(cells[4])) * 1e6)
+                per[int(cells[2])] = (float(cells[4]) + float(cells[5])) * 1e6
             except ValueError:
-                return None
-    return None
+                pass
+    tot = n = 0
+    for g, c in grids.items():
+        if int(round(g)) not in per:
+            return None
+        tot += per[int(round(g))] * c
+        n += c
+    return round(tot / n) if n else None
 
 
 def bench_trk(args, sdr, workloads):

@@ -187,6 +187,10 @@ int sdr_search_plan_kernel_ms(const sdr_search_plan_t *plan, const char *name,
  * per block, shared spectra once per launch, power rows read and written). */
 int sdr_search_plan_kernel_bytes(const sdr_search_plan_t *plan, const char *name,
     double *avg_bytes);
+/* Average launch size (workgroups x workgroup threads, the grid size a
+ * profiler reports) of kernel class `name` over the timed runs. */
+int sdr_search_plan_kernel_grid(const sdr_search_plan_t *plan, const char *name,
+    double *avg_threads);
 void sdr_search_plan_free(sdr_search_plan_t *plan);
 
 /* --- tracking correlators -------------------------------------------------

@@ -18,6 +18,8 @@
 
 namespace psa {
 
+thread_local long long pass_grid = 0;
+
 // ---------------------------------------------------------------------------
 // radix-r DFT butterflies, sgn = -1 (forward) / +1 (backward)
 // ---------------------------------------------------------------------------
@@ -551,10 +553,12 @@ static void launch_t(const PassArgs &a, const RadixSeq &rs, int nbatch, hipStrea
             const long long ntile = (long long)ngroups * a.xcd_nbin;
             nb = (ntile + 7) / 8 * 8 * a.xcd_nprn;
         }
+        pass_grid = nb * PASS_THREADS;
         hipLaunchKernelGGL((pass_kernel_ct<LM, SM, CL, CG>), dim3((unsigned)nb), dim3(PASS_THREADS), 0, st,
                            a, ngroups);
     } else {
         const size_t shm = sizeof(float2) * 2 * (size_t)a.G * a.L;
+        pass_grid = nblk * PASS_THREADS;
         hipLaunchKernelGGL((pass_kernel_rt<LM, SM>), dim3((unsigned)nblk), dim3(PASS_THREADS), shm, st,
                            a, rs, ngroups);
     }

@@ -70,22 +70,45 @@ __device__ __forceinline__ uint32_t cmul_w(uint32_t xw, uint32_t cw)
 }
 
 // raw buffer access with 32-bit element offsets from a per-item base (no
-// 64-bit address arithmetic per element)
+// 64-bit address arithmetic per element).  The resource covers 0x7FFFFFFF
+// bytes; a lane with ok = false uses byte offset 0x80000000, past that
+// range, where the hardware's range check returns 0 for loads and drops
+// stores (masking without branches).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, 0x7FFFFFFF, 0x00020000);
 }
-__device__ __forceinline__ uint32_t bload32(__amdgpu_buffer_rsrc_t r, int i)
+__device__ __forceinline__ int boff(int i, uint32_t size, bool ok)
 {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0);
+    return (int)(ok ? (uint32_t)i * size : 0x80000000u);
+}
+__device__ __forceinline__ uint32_t bload32(__amdgpu_buffer_rsrc_t r, int i, bool ok = true)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, boff(i, 4, ok), 0, 0);
 }
 __device__ __forceinline__ float2 bload_f2(__amdgpu_buffer_rsrc_t r, int i)
 {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0));
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, boff(i, 8, true), 0, 0));
 }
-__device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, int i, uint32_t v)
+// element base + c of a word / float2 array, masked on the base: with c a
+// compile-time constant the select is shared by the accesses of one base and
+// c becomes the instruction's immediate offset
+__device__ __forceinline__ uint32_t bload32_at(__amdgpu_buffer_rsrc_t r, int base, int c, bool ok)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(v, r, i * 4, 0, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)((uint32_t)boff(base, 4, ok) + (uint32_t)c * 4u), 0, 0);
+}
+__device__ __forceinline__ float2 bload_f2_at(__amdgpu_buffer_rsrc_t r, int base, int c, bool ok)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                          r, (int)((uint32_t)boff(base, 8, ok) + (uint32_t)c * 8u), 0, 0));
+}
+__device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, int i, uint32_t v, bool ok = true)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, boff(i, 4, ok), 0, 0);
+}
+__device__ __forceinline__ void bstore32_at(__amdgpu_buffer_rsrc_t r, int base, int c, uint32_t v, bool ok)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)((uint32_t)boff(base, 4, ok) + (uint32_t)c * 4u), 0, 0);
 }
 
 template <int DT>
@@ -115,42 +138,93 @@ struct MGeom {
 // t < R at j + t L / R; its output t' sits at the Stockham position
 // (j / NS) NS R + j % NS + t' NS.  The callers' functors split addressing
 // into a per-column part, computed once per tile, and a per-element part:
-//   ib = ibase(g, j)   (.x: base, .y: row valid), word = iat(ib.x, t);
-//   ob = obase(g, j)   (.x, .y, .z: bases and stride, .w: row valid),
-//   f = omul(ob, t')   multiplier - complex for MULK 2, its real part for
-//                      MULK 1 (.y then carries a value for oput), none for 0,
-//   oput(ob, t', v, f) stores output t' (FP32, multiplied).
+//   ib = ibase(g, j)      base of the column's inputs (rows past the end of
+//                         the item clamped to a valid one),
+//   raw = iload(ib, tc, th, ok)      issues the loads of input t,
+//   word = icook(ib, tc, th, ok, raw)  forms the packed input from them,
+//   ob = obase(g, j)      (.x, .y, .z: bases and stride, .w: row valid),
+//   f = omul(ob, tc, th, ok)  multiplier of output t' - complex for MULK 2,
+//                         its real part for MULK 1 (.y then carries a value
+//                         for oput), none for 0,
+//   oput(ob, tc, th, v, f, ok)  stores output t' (FP32, multiplied) if ok.
+// Element indices come split as t = tc + th: tc a compile-time constant of
+// the unrolled loops, th = 4 h (inputs) or 2 h (outputs) from the lane half
+// h, so that per-element offsets fold into immediates; ok = false marks an
+// input t >= R (its value is not used) or an output that is not stored.
 // NF: columns run along j (rows contiguous in memory) rather than along g.
-// Outputs t' >= TH are not produced.  Each wave walks its tiles
-// software-pipelined: the next tile's inputs and this tile's multipliers are
-// requested before the MFMAs (PF: stages that read global memory; stages fed
-// from LDS load their inputs at the top of the tile); every load is issued
-// unconditionally on clamped indices and masked afterwards, so no load waits
-// inside a branch.
-template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, class IBase, class IAt,
-          class OBase, class OMul, class OPut>
-__device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBase &ibase, const IAt &iat,
-                                       const OBase &obase, const OMul &omul, const OPut &oput)
+// Outputs t' >= TH are not produced.
+//
+// Inputs t >= R of the B operand meet zero columns of the DFT table, and the
+// results of columns past the end are discarded, so inputs are loaded on
+// clamped indices and never masked.  Each wave walks its tiles
+// software-pipelined (PF, stages that read global memory): the loads of the
+// next tile are issued before this tile's MFMAs and only consumed (icook)
+// after this tile's outputs are stored, so they stay in flight across the
+// whole tile; the loop body has no branches (masked outputs select a dummy
+// LDS word or an out-of-range buffer offset), so nothing pulls a wait forward.
+// The A operands of one stage (R-point DFT, outputs t' < TH) in registers.
+// load() issues the loads; complete() waits for them.  Callers issue the
+// loads early (before a barrier or other loads) and complete them before the
+// stage's tile loop: left pending into the loop, the wait the first MFMA needs
+// for them would also wait, in every iteration, for the previous tile's
+// output stores.
+template <int DT, int R, int TH>
+struct ATab {
+    static constexpr int KS = (2 * R + 15) / 16;       // K steps of 16 halves
+    static constexpr int RT = (2 * TH + 31) / 32;      // 32-row output tiles needed
+    static constexpr int RTA = (2 * R + 31) / 32;      // row tiles of the table
+    // BF16: A = A_hi + A_lo (the second table follows the first), so the DFT
+    // coefficients carry 16 significant bits
+    static constexpr int NA = DT == HDT_BF16 ? 2 : 1;
+    uint4 v[NA][RT][KS];
+    __device__ __forceinline__ void load(const uint4 *__restrict__ amat)
+    {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int x = 0; x < NA; x++) {
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+#pragma unroll
+                for (int k = 0; k < KS; k++) v[x][r][k] = amat[((x * RTA + r) * KS + k) * 64 + lane];
+            }
+        }
+    }
+    __device__ __forceinline__ void complete() const
+    {
+#pragma unroll
+        for (int x = 0; x < NA; x++) {
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+#pragma unroll
+                for (int k = 0; k < KS; k++) {
+                    const uint4 &q = v[x][r][k];
+                    asm volatile("" : : "v"(q.x), "v"(q.y), "v"(q.z), "v"(q.w));
+                }
+            }
+        }
+    }
+};
+
+// pre() runs after the first tile's loads are issued and before anything
+// uses them or the stage's table (e.g. filling LDS tables and the barrier);
+// the stage then completes the table loads.
+template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, class IBase, class ILoad,
+          class ICook, class OBase, class OMul, class OPut, class Pre>
+__device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &ibase, const ILoad &iload,
+                                       const ICook &icook, const OBase &obase, const OMul &omul,
+                                       const OPut &oput, const Pre &pre)
 {
     constexpr int LR = L / R;
     constexpr int C = G * LR;
     constexpr int NT = (C + 31) / 32;
-    constexpr int KS = (2 * R + 15) / 16;       // K steps of 16 halves
-    constexpr int RT = (2 * TH + 31) / 32;      // 32-row output tiles needed
-    constexpr int RTA = (2 * R + 31) / 32;      // row tiles of the table
-    const int lane = threadIdx.x & 63, h = lane >> 5, wave = threadIdx.x >> 6;
-    // BF16: A = A_hi + A_lo (the second table follows the first), so the DFT
-    // coefficients carry 16 significant bits
-    constexpr int NA = DT == HDT_BF16 ? 2 : 1;
-    uint4 A[NA][RT][KS];
-#pragma unroll
-    for (int x = 0; x < NA; x++) {
-#pragma unroll
-        for (int r = 0; r < RT; r++) {
-#pragma unroll
-            for (int k = 0; k < KS; k++) A[x][r][k] = amat[((x * RTA + r) * KS + k) * 64 + lane];
-        }
-    }
+    constexpr int KS = ATab<DT, R, TH>::KS;
+    constexpr int RT = ATab<DT, R, TH>::RT;
+    constexpr int NA = ATab<DT, R, TH>::NA;
+    constexpr int NW = NTH / 64;
+    // the wave index in a scalar register: tile loop control is wave-uniform
+    const int lane = threadIdx.x & 63, h = lane >> 5;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const auto &A = at.v;
     // column of this lane in a tile; past the last column the indices are
     // clamped to a valid one (the lane's results are then discarded)
     auto column = [&](int tile, int &g, int &j) -> bool {
@@ -162,36 +236,56 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBa
         return ok;
     };
     // B: lane (column, h) supplies inputs t = 8 k + 4 h + u, u < 4 (inputs
-    // t >= R read t = R - 1 and are zeroed)
-    auto load_b = [&](int g, int j, bool cv, uint4 *B) {
-        const int2 ib = ibase(g, j);
-        const bool ok = cv && ib.y;
+    // t >= R read t = R - 1)
+    using Raw = decltype(iload(0, 0, 0, true));
+    // input t = tc + 4 h is used if t < R
+    auto inok = [&](int tc) -> bool { return tc + 4 < R || (tc < R && h == 0); };
+    auto fetch = [&](int ib, Raw (&w)[KS][4]) {
 #pragma unroll
         for (int k = 0; k < KS; k++) {
-            uint32_t w[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int t = 8 * k + 4 * h + u;
-                const uint32_t v = iat(ib.x, t < R ? t : R - 1);
-                w[u] = (ok && t < R) ? v : 0u;
-            }
-            B[k] = make_uint4(w[0], w[1], w[2], w[3]);
+            for (int u = 0; u < 4; u++) w[k][u] = iload(ib, 8 * k + u, 4 * h, inok(8 * k + u));
         }
     };
-    int tile = wave, g = 0, j = 0;
-    bool cv = column(tile, g, j);
-    uint4 B[KS];
-    if (PF && tile < NT) load_b(g, j, cv, B);
-    for (; tile < NT; tile += NTH / 64) {
-        const int tn = tile + NTH / 64;
-        int gn = 0, jn = 0;
-        bool cvn = false;
-        uint4 Bn[PF ? KS : 1];
-        if (PF && tn < NT) {
-            cvn = column(tn, gn, jn);
-            load_b(gn, jn, cvn, Bn);
+    auto cook = [&](int ib, const Raw (&w)[KS][4], uint4 *B) {
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            uint32_t c[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) c[u] = icook(ib, 8 * k + u, 4 * h, inok(8 * k + u), w[k][u]);
+            B[k] = make_uint4(c[0], c[1], c[2], c[3]);
         }
-        if (!PF) load_b(g, j, cv, B);
+    };
+    // output t' = tc + 2 h is produced if t' < TH
+    auto outok = [&](int tc) -> bool { return tc + 2 < TH || (tc < TH && h == 0); };
+    // (a wave without tiles fetches the last one, unused, so that no load
+    // sits in a branch)
+    int tile = wave, g = 0, j = 0;
+    bool cv = column(tile < NT ? tile : NT - 1, g, j);
+    uint4 B[KS];
+    Raw raw[KS][4];
+    const int ib0 = ibase(g, j);
+    if (PF) fetch(ib0, raw);
+    pre();
+    at.complete();
+    if (PF) cook(ib0, raw, B);
+    for (; tile < NT; tile += NW) {
+        const int tn = tile + NW;
+        int gn = 0, jn = 0, ibn = 0;
+        bool cvn = false;
+        if (PF) {
+            // next tile (a wave-uniform branch: the loads are not waited on
+            // inside it)
+            if (tn < NT) {
+                cvn = column(tn, gn, jn);
+                ibn = ibase(gn, jn);
+                fetch(ibn, raw);
+            }
+        } else {
+            const int ib = ibase(g, j);
+            fetch(ib, raw);
+            cook(ib, raw, B);
+        }
         const int4 ob = obase(g, j);
         // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r + 4 m +
         // 2 h + w, real (imaginary) part
@@ -202,8 +296,8 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBa
             for (int m = 0; m < 4; m++) {
 #pragma unroll
                 for (int w = 0; w < 2; w++) {
-                    const int t = 16 * r + 4 * m + 2 * h + w;
-                    if (MULK) mul[r][m][w] = omul(ob, t < TH ? t : TH - 1);
+                    const int tc = 16 * r + 4 * m + w;
+                    if (MULK && tc < TH) mul[r][m][w] = omul(ob, tc, 2 * h, outok(tc));
                 }
             }
         }
@@ -218,33 +312,28 @@ __device__ __forceinline__ void mstage(const uint4 *__restrict__ amat, const IBa
                 for (int k = 0; k < KS; k++) acc[r] = mfma<DT>(A[x][r][k], B[k], acc[r]);
             }
         }
-        if (cv && ob.w) {
+        const bool ok = cv && ob.w;
 #pragma unroll
-            for (int r = 0; r < RT; r++) {
+        for (int r = 0; r < RT; r++) {
 #pragma unroll
-                for (int m = 0; m < 4; m++) {
+            for (int m = 0; m < 4; m++) {
 #pragma unroll
-                    for (int w = 0; w < 2; w++) {
-                        const int t = 16 * r + 4 * m + 2 * h + w;
-                        if (t < TH) {
-                            float2 v = make_float2(acc[r][4 * m + 2 * w], acc[r][4 * m + 2 * w + 1]);
-                            const float2 f = mul[r][m][w];
-                            if (MULK == 2) v = cmul(v, f);
-                            else if (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
-                            oput(ob, t, v, f);
-                        }
-                    }
+                for (int w = 0; w < 2; w++) {
+                    const int tc = 16 * r + 4 * m + w;
+                    if (tc >= TH) continue;      // past TH in both halves
+                    float2 v = make_float2(acc[r][4 * m + 2 * w], acc[r][4 * m + 2 * w + 1]);
+                    const float2 f = MULK ? mul[r][m][w] : make_float2(0.0f, 0.0f);
+                    if (MULK == 2) v = cmul(v, f);
+                    else if (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
+                    oput(ob, tc, 2 * h, v, f, ok && outok(tc));
                 }
             }
         }
-        if (tn < NT) {
-            if (PF) {
-#pragma unroll
-                for (int k = 0; k < KS; k++) B[k] = Bn[k];
-                g = gn; j = jn; cv = cvn;
-            } else {
-                cv = column(tn, g, j);
-            }
+        if (PF) {
+            if (tn < NT) cook(ibn, raw, B);
+            g = gn; j = jn; cv = cvn;
+        } else if (tn < NT) {
+            cv = column(tn, g, j);
         }
     }
 }
@@ -262,28 +351,46 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     constexpr int THL = SM == ST_POW ? (RL + 1) / 2 : RL;
     // the forward column and inverse row passes apply the inter-pass twiddle
     constexpr bool TW = SM == ST_CPX && LM != LD_CPX;
-    __shared__ uint32_t buf0[G * RP];
-    __shared__ uint32_t buf1[S3 ? G * RP : 1];
-    __shared__ float2 tws[L];
-    __shared__ uint32_t carr[LM == LD_MIX ? 256 : 1];
+    // one dummy word per lane after the rows takes the masked LDS stores
+    __shared__ uint32_t buf0[G * RP + 64];
+    __shared__ uint32_t buf1[S3 ? G * RP + 64 : 1];
+    // the LDS tables are filled without branches (entries past the end are
+    // copies, unused)
+    constexpr int NTW = (L + NTH - 1) / NTH;
+    __shared__ float2 tws[NTW * NTH];
+    __shared__ uint32_t carr[LM == LD_MIX ? NTH : 1];
     __shared__ PeakPart red[NTH / 64];
     int b, grp;
     if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp)) return;
     const int g0 = grp * G;
     const int gcount = min(G, a.nsub - g0);
-    for (int i = threadIdx.x; i < L; i += NTH) tws[i] = a.tw[i];
-    if constexpr (LM == LD_MIX) {
-        for (int i = threadIdx.x; i < 256; i += NTH) carr[i] = a.carr[i];
-    }
-    __syncthreads();
+    // issue order: stage-0 table, the LDS tables' words, the first tile's
+    // inputs (in mstage); the LDS fill then waits only for the first two
+    ATab<DT, R0, R0> at0;
+    at0.load(a.amat[0]);
+    float2 twv[NTW];
+#pragma unroll
+    for (int q = 0; q < NTW; q++) twv[q] = a.tw[min((int)threadIdx.x + q * NTH, L - 1)];
+    uint32_t cav = 0;
+    if constexpr (LM == LD_MIX) cav = a.carr[threadIdx.x & 255];
+    auto fill = [&]() {
+#pragma unroll
+        for (int q = 0; q < NTW; q++) tws[threadIdx.x + q * NTH] = twv[q];
+        if constexpr (LM == LD_MIX) carr[threadIdx.x] = cav;
+        __syncthreads();
+    };
+    auto barrier = []() { __syncthreads(); };
 
     // LDS stages: input t of column j of a later stage is element
     // n = j + t LRs (LRs = L / R of that stage), at n + n / R0 (PD)
-    auto lds_base = [&](int g, int j) -> int2 { return make_int2(g * RP + j + (PD ? j / R0 : 0), 1); };
+    auto lds_base = [&](int g, int j) -> int { return g * RP + j + (PD ? j / R0 : 0); };
+    const int dummy = G * RP + (int)(threadIdx.x & 63);
 
     // ---- stage 0: global -> MFMA -> LDS ----
     {
-        const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
+        // row passes (NF) read and write rows with unit element stride (a
+        // compile-time constant: per-element offsets become immediates)
+        const int igs = (int)a.in_gstride, ins = NF ? 1 : (int)a.in_nstride;
         const int tstride = (L / R0) * ins;   // input t of column j at j ins + t tstride
         Loader<LD_MIX> mx;
         if constexpr (LM == LD_MIX) mx.init(a, b, carr);
@@ -295,66 +402,89 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         }
         const __amdgpu_buffer_rsrc_t xin = rsrc(a.hin + xo), cin = rsrc(a.hcf + co);
         // rows past the end of the batch item (last group) read row 0
-        auto ibase = [&](int g, int j) -> int2 {
-            const bool ok = g < gcount;
-            return make_int2((g0 + (ok ? g : 0)) * igs + j * ins, ok);
-        };
-        auto iat = [&](int base, int t) -> uint32_t {
-            const int e = base + t * tstride;
+        auto ibase = [&](int g, int j) -> int { return (g0 + (g < gcount ? g : 0)) * igs + j * ins; };
+        // unused inputs (ok = false): the IF byte of t = R0 - 1, or a masked
+        // load (returns 0)
+        auto iload = [&](int base, int tc, int th, bool ok) {
             if constexpr (LM == LD_MIX) {
-                const int2 m = mx.mix_int(e);
+                return mx.mix_byte(base + (ok ? tc + th : R0 - 1) * tstride);
+            } else {
+                const int bh = base + th * tstride, c = tc * tstride;
+                if constexpr (LM == LD_CPX) return bload32_at(xin, bh, c, ok);
+                else return make_uint2(bload32_at(xin, bh, c, ok), bload32_at(cin, bh, c, ok));
+            }
+        };
+        auto icook = [&](int base, int tc, int th, bool ok, auto raw) -> uint32_t {
+            if constexpr (LM == LD_MIX) {
+                const int2 m = mx.mix_prod(base + (ok ? tc + th : R0 - 1) * tstride, raw);
                 return pack_c<DT>((float)m.x, (float)m.y);
             } else if constexpr (LM == LD_CPX) {
-                return bload32(xin, e);
+                return raw;
             } else {
-                return cmul_w<DT>(bload32(xin, e), bload32(cin, e));
+                return cmul_w<DT>(raw.x, raw.y);
             }
         };
         // output t of column j sits at q = j R0 + t (LDS j (R0 + PD) + t); the
         // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages)
         auto obase = [&](int g, int j) -> int4 { return make_int4(g * RP + j * (R0 + PD), 0, (j / R2) * R2, 1); };
-        auto omul = [&](const int4 &ob, int t) -> float2 { return tws[t * ob.z]; };
-        auto oput = [&](const int4 &ob, int t, float2 v, float2) { buf0[ob.x + t] = pack_c<DT>(v.x, v.y); };
-        mstage<DT, NTH, L, R0, G, NF, R0, 2, true>(a.amat[0], ibase, iat, obase, omul, oput);
+        auto omul = [&](const int4 &ob, int tc, int th, bool ok) -> float2 { return tws[(ok ? tc + th : 0) * ob.z]; };
+        auto oput = [&](const int4 &ob, int tc, int th, float2 v, float2, bool ok) {
+            buf0[ok ? ob.x + tc + th : dummy] = pack_c<DT>(v.x, v.y);
+        };
+        mstage<DT, NTH, L, R0, G, NF, R0, 2, true>(at0, ibase, iload, icook, obase, omul, oput, fill);
     }
-    __syncthreads();
 
     // ---- last stage: LDS -> MFMA -> global ----
     Storer<ST_POW> sp;
     if constexpr (SM == ST_POW) sp.init(a, b);
     const __amdgpu_buffer_rsrc_t hout = rsrc(a.hout + (SM == ST_CPX ? (long long)b * a.out_bstride : 0LL));
     const __amdgpu_buffer_rsrc_t tfr = rsrc(a.tfull);
-    const float osc = a.osc, kap = a.kappa;
-    const int ogs = (int)a.out_gstride, ons = (int)a.out_nstride;
+    const float osc = a.osc, kap2 = a.kappa * a.kappa;
+    const int ogs = (int)a.out_gstride, ons = NF ? 1 : (int)a.out_nstride;
     const int estep = NSL * ons;
     // output t of column j (< NSL) is element k = j + t NSL of its
     // sub-transform, at e = gg ogs + k ons.  ST_CPX multiplies by the
     // inter-pass twiddle (its table is laid out like the output) or by osc;
-    // ST_POW by kappa, and the previous P value rides along in .y (fetched
-    // with the tile's other loads; 0 on the first block)
+    // ST_POW scales the power by kappa^2 (the multiplier is 1), and the
+    // previous P value rides along in .y (fetched with the tile's other
+    // loads; 0 on the first block)
     const bool pacc = SM == ST_POW && !a.p_init;
-    const float *prow = sp.prow;
+    const __amdgpu_buffer_rsrc_t prr = rsrc(sp.prow);
     const int nlag = a.nlag;
     auto gbase = [&](int g, int j) -> int4 {
         const bool ok = g < gcount;
         return make_int4((g0 + (ok ? g : 0)) * ogs + j * ons, 0, 0, ok);
     };
-    auto gmul = [&](const int4 &ob, int t) -> float2 {
-        const int e = ob.x + t * estep;
-        if constexpr (SM == ST_POW) return make_float2(kap, pacc ? prow[e < nlag ? e : 0] : 0.0f);
-        else if constexpr (TW) return bload_f2(tfr, e);
-        else return make_float2(osc, 0.0f);
+    auto gmul = [&](const int4 &ob, int tc, int th, bool ok) -> float2 {
+        if constexpr (SM == ST_POW) {
+            const int e = ob.x + (tc + th) * estep;
+            return make_float2(1.0f, __uint_as_float(bload32(prr, e, pacc && ok && e < nlag)));
+        } else if constexpr (TW) {
+            return bload_f2_at(tfr, ob.x + th * estep, tc * estep, ok);
+        } else {
+            return make_float2(osc, 0.0f);
+        }
     };
-    auto gput = [&](const int4 &ob, int t, float2 v, float2 f) {
-        const int e = ob.x + t * estep;
-        if constexpr (SM == ST_POW) sp.store_pow(e, v, f.y);
-        else bstore32(hout, e, pack_c<DT>(v.x, v.y));
+    auto gput = [&](const int4 &ob, int tc, int th, float2 v, float2 f, bool ok) {
+        if constexpr (SM == ST_POW) {
+            sp.store_pow_sel(ob.x + (tc + th) * estep, v, kap2, f.y, ok, [&](bool ok2, int e2, float nv) {
+                bstore32(prr, e2, __float_as_uint(nv), ok2);
+            });
+        } else {
+            bstore32_at(hout, ob.x + th * estep, tc * estep, pack_c<DT>(v.x, v.y), ok);
+        }
     };
     constexpr int MULKL = TW ? 2 : 1;
     // stage 1 input t of column j: n = j + t R0 R2
-    auto iat0 = [&](int base, int t) -> uint32_t { return buf0[base + t * (R0 * R2 + PD * R2)]; };
+    // (unused inputs read t = R - 1 of the stage: written words)
+    auto iat0 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+        return buf0[base + (ok ? tc + th : R1 - 1) * (R0 * R2 + PD * R2)];
+    };
+    auto ident = [](int, int, int, bool, uint32_t w) -> uint32_t { return w; };
     if constexpr (!S3) {
-        mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false>(a.amat[1], lds_base, iat0, gbase, gmul, gput);
+        ATab<DT, R1, THL> at1;
+        at1.load(a.amat[1]);
+        mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false>(at1, lds_base, iat0, ident, gbase, gmul, gput, barrier);
     } else {
         // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
         // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
@@ -362,15 +492,22 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             const int jh = j / R0, jl = j - jh * R0;
             return make_int4(g * RP + jh * R0 * R1 + jl + (PD ? jh * R1 : 0), jh * jl, jh * R0, 1);
         };
-        auto omul1 = [&](const int4 &ob, int t) -> float2 { return tws[ob.y + t * ob.z]; };
-        auto oput1 = [&](const int4 &ob, int t, float2 v, float2) {
-            buf1[ob.x + t * (R0 + PD)] = pack_c<DT>(v.x, v.y);
+        auto omul1 = [&](const int4 &ob, int tc, int th, bool ok) -> float2 {
+            return tws[ob.y + (ok ? tc + th : 0) * ob.z];
         };
-        mstage<DT, NTH, L, R1, G, NF, R1, 2, false>(a.amat[1], lds_base, iat0, obase1, omul1, oput1);
-        __syncthreads();
+        auto oput1 = [&](const int4 &ob, int tc, int th, float2 v, float2, bool ok) {
+            buf1[ok ? ob.x + (tc + th) * (R0 + PD) : dummy] = pack_c<DT>(v.x, v.y);
+        };
+        ATab<DT, R1, R1> at1;
+        at1.load(a.amat[1]);
+        mstage<DT, NTH, L, R1, G, NF, R1, 2, false>(at1, lds_base, iat0, ident, obase1, omul1, oput1, barrier);
         // stage 2 input t of column j: n = j + t R0 R1
-        auto iat1 = [&](int base, int t) -> uint32_t { return buf1[base + t * (R0 * R1 + PD * R1)]; };
-        mstage<DT, NTH, L, R2, G, NF, THL, MULKL, false>(a.amat[2], lds_base, iat1, gbase, gmul, gput);
+        auto iat1 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+            return buf1[base + (ok ? tc + th : R2 - 1) * (R0 * R1 + PD * R1)];
+        };
+        ATab<DT, R2, THL> at2;
+        at2.load(a.amat[2]);
+        mstage<DT, NTH, L, R2, G, NF, THL, MULKL, false>(at2, lds_base, iat1, ident, gbase, gmul, gput, barrier);
     }
     if constexpr (SM == ST_POW) {
         if (a.p_part) {
@@ -637,6 +774,8 @@ static void hlaunch_t(const PassArgs &a, int nbatch, hipStream_t st)
     }
     if (nb <= 0) return;
     if (nb > 0x7FFFFFFFLL) PSA_FAIL("FFT launch too large (%lld workgroups)", nb);
+    if (NF && (a.in_nstride != 1 || a.out_nstride != 1)) PSA_FAIL("MFMA row pass needs unit element strides");
+    pass_grid = nb * NTH;
     hipLaunchKernelGGL((mpass_kernel<LM, SM, DT, L, R0, R1, R2, G, NF, NTH>), dim3((unsigned)nb), dim3(NTH), 0,
                        st, a, ngroups);
     PSA_CHECK(hipGetLastError());

@@ -187,6 +187,10 @@ struct PassArgs {
     int hvar;                   // workgroup shape variant of the MFMA pass
 };
 
+// threads (workgroups x workgroup size) of the last pass launched by this
+// host thread, for the plans' per-launch statistics
+extern thread_local long long pass_grid;
+
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
                  hipStream_t st);
 int pass_ngroups(const PassArgs &a);   // workgroups per batch item

@@ -57,14 +57,17 @@ struct Loader {
     }
 
     // LD_MIX: the carrier-mixed int8 pair of sample e, as mix_tbl holds it
-    // (src/sdr_func.c:113-134, 786-806)
-    __device__ __forceinline__ int2 mix_int(int e) const
+    // (src/sdr_func.c:113-134, 786-806): mix_byte fetches the IF byte,
+    // mix_prod forms the product (split so that callers can keep loads in
+    // flight)
+    __device__ __forceinline__ uint32_t mix_byte(int e) const
     {
-        int src;
-        uint32_t ph;
-        if (e < nseg) { src = ix + e; ph = p0 + (uint32_t)e * s; }
-        else { src = compact ? e : e - nseg; ph = p1 + (uint32_t)(e - nseg) * s; }
-        const uint32_t d = buf[src];
+        return buf[e < nseg ? ix + e : (compact ? e : e - nseg)];
+    }
+    __device__ __forceinline__ int2 mix_int(int e) const { return mix_prod(e, mix_byte(e)); }
+    __device__ __forceinline__ int2 mix_prod(int e, uint32_t d) const
+    {
+        const uint32_t ph = e < nseg ? p0 + (uint32_t)e * s : p1 + (uint32_t)(e - nseg) * s;
         const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
         const int sQ = (int)((int8_t)(uint8_t)d) >> 4;
         const uint32_t c = carr[ph >> 24];
@@ -172,10 +175,33 @@ struct Storer {
         }
     }
 
+    // the same without branches (bitwise conditions, selects): the store
+    // goes through put(ok, e, value), which drops it when !ok, and only ok
+    // values enter the peak tracking.  The sum for the mean is kept per lane
+    // in FP32 (fsum, a few dozen values) and joins bs at flush_peak.
+    float fsum = 0.0f;
+    // The power is k2 (re^2 + im^2).
+    template <class PutF>
+    __device__ __forceinline__ void store_pow_sel(int e, float2 v, float k2, float pold, bool ok, const PutF &put)
+    {
+        ok = ok & (e < nlag);
+        const float pw = k2 * __builtin_fmaf(v.x, v.x, v.y * v.y);
+        const float nv = pinit ? pw : pold + pw;
+        put(ok, e, nv);
+        if (track) {
+            fsum += ok ? nv : 0.0f;
+            const bool take = ok & (nv > 0.0f) & ((nv > bm) | ((nv == bm) & (be >= 0) & (e < be)));
+            bm = take ? nv : bm;
+            be = take ? e : be;
+        }
+    }
+
     // workgroup reduction of the tracked peak into part[slot]; red holds one
     // record per wave
     __device__ __forceinline__ void flush_peak(PeakPart *part, long long slot, PeakPart *red)
     {
+        bs += (double)fsum;
+        fsum = 0.0f;
         long long bi = be >= 0 ? ibase + be : -1;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {

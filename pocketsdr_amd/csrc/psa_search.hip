@@ -500,12 +500,12 @@ struct sdr_search_plan_tag {
     hipStream_t st = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool timing = false;
-    struct Ev { int cls, sig; hipEvent_t a, b; double elems, bytes; };
+    struct Ev { int cls, sig; hipEvent_t a, b; double elems, bytes, grid; };
     int cur_sig = 0;
     std::vector<Ev> evs;
     size_t ev_used = 0;
     // per (kernel class, signal group) sums over the timed runs
-    struct Stat { double ms = 0, elems = 0, bytes = 0; long long cnt = 0; };
+    struct Stat { double ms = 0, elems = 0, bytes = 0, grid = 0; long long cnt = 0; };
     std::vector<Stat> stats;     // [K_NCLS * n_sig]
 };
 
@@ -532,11 +532,14 @@ struct Tick {
         p->evs[slot].sig = p->cur_sig;
         p->evs[slot].elems = elems;
         p->evs[slot].bytes = bytes;
+        pass_grid = 0;
         PSA_CHECK(hipEventRecord(p->evs[slot].a, p->st));
     }
     ~Tick()
     {
-        if (p->timing) PSA_CHECK(hipEventRecord(p->evs[slot].b, p->st));
+        if (!p->timing) return;
+        p->evs[slot].grid = (double)pass_grid;
+        PSA_CHECK(hipEventRecord(p->evs[slot].b, p->st));
     }
 };
 
@@ -1034,6 +1037,7 @@ int sdr_search_plan_run(sdr_search_plan_t *p)
             q.cnt++;
             q.elems += p->evs[i].elems;
             q.bytes += p->evs[i].bytes;
+            q.grid += p->evs[i].grid;
         }
     }
     return 1;
@@ -1106,7 +1110,7 @@ static bool plan_stat(const sdr_search_plan_t *p, const char *name, sdr_search_p
         for (size_t g = 0; g < ns; g++) {
             if (sig >= 0 && (int)g != sig) continue;
             const auto &q = p->stats[(size_t)i * ns + g];
-            t.ms += q.ms; t.cnt += q.cnt; t.elems += q.elems; t.bytes += q.bytes;
+            t.ms += q.ms; t.cnt += q.cnt; t.elems += q.elems; t.bytes += q.bytes; t.grid += q.grid;
         }
         if (!t.cnt) return false;
         *out = t;
@@ -1120,6 +1124,14 @@ int sdr_search_plan_kernel_bytes(const sdr_search_plan_t *p, const char *name, d
     sdr_search_plan_t::Stat t;
     if (!plan_stat(p, name, &t)) return 0;
     if (avg_bytes) *avg_bytes = t.bytes / t.cnt;
+    return 1;
+}
+
+int sdr_search_plan_kernel_grid(const sdr_search_plan_t *p, const char *name, double *avg_threads)
+{
+    sdr_search_plan_t::Stat t;
+    if (!plan_stat(p, name, &t)) return 0;
+    if (avg_threads) *avg_threads = t.grid / t.cnt;
     return 1;
 }
 

@@ -76,6 +76,7 @@ _decl("sdr_search_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_search_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                              C.POINTER(C.c_longlong), c_double_p])
 _decl("sdr_search_plan_kernel_bytes", C.c_int, [c_void_p, C.c_char_p, c_double_p])
+_decl("sdr_search_plan_kernel_grid", C.c_int, [c_void_p, C.c_char_p, c_double_p])
 _decl("sdr_search_plan_free", None, [c_void_p])
 _decl("sdr_trk_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sdr_trk_ch_t),
                                      C.c_int, c_double_p, C.c_int])
@@ -288,6 +289,13 @@ class SearchPlan:
         by = C.c_double(0)
         lib.sdr_search_plan_kernel_bytes(self.p, name.encode(), C.byref(by))
         return ms.value, cnt.value, el.value, by.value
+
+    def kernel_grid(self, name):
+        """Average launch size in threads (the profiler's grid size)."""
+        g = C.c_double(0)
+        if not lib.sdr_search_plan_kernel_grid(self.p, name.encode(), C.byref(g)):
+            return None
+        return g.value
 
     def close(self):
         if self.p:
