@@ -117,6 +117,14 @@ def build_workload(name, seed):
 
 
 FORM_FLAGS = {"fp32": 16, "fp16": 4 | 16, "bf16": 8 | 16}    # batched forms, tuned tiles
+# the fastest tilings of each workload's FFT lengths on MI355X (every split
+# and workgroup shape timed over 3 x 5 steps), fixed so that repeated runs
+# and their counter profiles launch the same kernels; groups without an
+# entry are timed by the plan tuner (SDR_SEARCH_TUNE) at plan creation
+TILES = {"multi": "h24000=150/1/0,h96000=300/0/0,h48000=150/1/0,h192000=400/0/1,"
+                  "f24000=150,f96000=200,f48000=200,f192000=200",
+         "l1ca": "h24000=150/1/0,f24000=150",
+         "l1cp": "h960000=960/1/1"}
 FORM_DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 
 
@@ -173,8 +181,10 @@ def bench_acq(args, sdr, orc):
     names = ["fp32", "plain", "fp16", "bf16"]
     res["form_per_signal"] = {s["name"]: names[forms.index(plan.form(i))] for i, s in enumerate(sigs)}
     tiles = {s["name"]: plan.tiles(i) for i, s in enumerate(sigs)}
-    res["tuned_tiles"] = {k: dict(M1=v[0], M2=v[1], col_shape=v[2], row_shape=v[3])
-                          for k, v in tiles.items() if v}
+    res["tiles"] = {k: dict(M1=v[0], M2=v[1], col_shape=v[2], row_shape=v[3])
+                    for k, v in tiles.items() if v}
+    res["tiles_source"] = ("fixed: " + os.environ["PSA_TILES"]) if os.environ.get("PSA_TILES") \
+        else "plan tuner (SDR_SEARCH_TUNE)"
     res["detections"] = decisions(sigs, r_main, truth)
     # per-kernel HIP events on the plan's stream over timed steps
     plan.timing(True)
@@ -211,6 +221,7 @@ def bench_acq(args, sdr, orc):
                            frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic_of(inst, d["grids"], traffic_profile), kernel=inst,
                            launches_of=d["parts"], alg_bytes_per_launch=round(alg),
                            avg_launch_ms=round(avg_ms, 5))
+    res["cell_model"] = cell_model(sigs, form, t_b)
     res["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 5), launches=v["launches"],
                               gbs=round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1))
                       for k, v in ks.items()}
@@ -294,7 +305,7 @@ def traffic_of(inst, grids, profile="search_multi_traffic.md"):
         cells = [c.strip() for c in line.split("|")]
         if len(cells) > 6 and cells[1].strip("`") == inst:
             try:
-                per[int(cells[2])] = (float(cells[4]) + float(cells[5])) * 1e6
+                per[int(cells[2])] = (float(cells[4]) + float(cells[5])) * 1048576.0
             except ValueError:
                 pass
     tot = n = 0
@@ -319,6 +330,46 @@ def bench_trk(args, sdr, workloads):
     return dict(correlator_msps_ch=round(msps_ch, 1), correlator_ms_per_step=round(t * 1e3, 3),
                 correlator_config=dict(n_ch=n_ch, fs=24e6, n_epoch=w["n_epoch"], n_pos=len(w["pos"])),
                 correlator_kernel_ms=round(ks[0], 4) if ks else None)
+
+
+ONCHIP_BYTES = 672 * 1024      # one CU: 160 KiB LDS + 512 KiB VGPR/AGPR
+
+
+def cell_model(sigs, form, t_step):
+    """Whole-step HBM bytes of the per-cell model: for one (signal, PRN, bin,
+    block) cell-step B = M / (n_prn n_bins) (IF bytes) + b_cf M / (n_bins
+    n_blk) (code spectrum) + 4 N / n_blk (final power surface) + (p - 1) 4 M
+    b_w (one HBM round trip of each FFT when it does not fit one CU), with
+    b_w = b_cf = 4 (FP16/BF16) or 8 (FP32).  'capacity': p = 1 when M b_w fits
+    one CU's LDS + registers (a lower bound no multi-pass FFT reaches);
+    'four_step': p = 2 for every length (the passes built here)."""
+    bw = 8 if form == "fp32" else 4
+    out = {}
+    for name, rule in (("capacity", None), ("four_step", 2)):
+        tot = 0.0
+        for s in sigs:
+            M, N = s["M"], s["M"] // 2
+            nprn, nbin, nblk = s["code_fft"].shape[0], len(s["fds"]), s["n_blk"]
+            p = rule or (1 if M * bw <= ONCHIP_BYTES else 2)
+            B = M / (nprn * nbin) + bw * M / (nbin * nblk) + 4.0 * N / nblk + (p - 1) * 4.0 * M * bw
+            tot += B * nprn * nbin * nblk
+        gbs = tot / t_step / 1e9
+        out[name] = dict(bytes_per_step=round(tot), achieved_GBs=round(gbs, 1),
+                         frac=round(gbs / HBM_PEAK_GBS, 4))
+    return out
+
+
+def tiles_arg(spec):
+    """PSA_TILES string from --tiles: given as is, or '@FILE' = the tilings a
+    previous bench line used (tiles, fp32_batched_tiles)."""
+    if not spec.startswith("@"):
+        return spec
+    line = json.loads(open(spec[1:]).read().strip().splitlines()[-1])
+    sigs = {s["name"]: s["M"] for s in line["config"]["signals"]}
+    ent = ["h%d=%d/%d/%d" % (sigs[n], t["M1"], t["col_shape"], t["row_shape"])
+           for n, t in line.get("tiles", line.get("tuned_tiles", {})).items()]
+    ent += ["f%d=%d" % (sigs[n], t[0]) for n, t in (line.get("fp32_batched_tiles") or {}).items() if t]
+    return ",".join(ent)
 
 
 def copy_probe(mb):
@@ -356,7 +407,13 @@ def main():
     ap.add_argument("--trk-epochs", type=int, default=1000)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a torch device copy of this many MiB (counter calibration)")
+    ap.add_argument("--tiles", default="auto",
+                    help="plan tilings: 'auto' (the workload's measured best, else tuned), 'tune' "
+                         "(the plan tuner times every tiling), a PSA_TILES string "
+                         "(e.g. h96000=300/0/0,f96000=200) or '@FILE' (those of a bench JSON line)")
     args = ap.parse_args()
+    if args.tiles != "tune":
+        os.environ["PSA_TILES"] = TILES.get(args.workload, "") if args.tiles == "auto" else tiles_arg(args.tiles)
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 #include "psa_pass.h"
 
@@ -145,7 +146,8 @@ struct MGeom {
 //   ob = obase(g, j)      (.x, .y, .z: bases and stride, .w: row valid),
 //   f = omul(ob, tc, th, ok)  multiplier of output t' - complex for MULK 2,
 //                         its real part for MULK 1 (.y then carries a value
-//                         for oput), none for 0,
+//                         for oput), none for 0; MULK 3: f is passed to oput
+//                         unapplied (a packed FP16 twiddle),
 //   oput(ob, tc, th, v, f, ok)  stores output t' (FP32, multiplied) if ok.
 // Element indices come split as t = tc + th: tc a compile-time constant of
 // the unrolled loops, th = 4 h (inputs) or 2 h (outputs) from the lane half
@@ -289,7 +291,8 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
         const int4 ob = obase(g, j);
         // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r + 4 m +
         // 2 h + w, real (imaginary) part
-        float2 mul[RT][4][2];
+        using MulT = decltype(omul(ob, 0, 0, true));
+        MulT mul[RT][4][2];
 #pragma unroll
         for (int r = 0; r < RT; r++) {
 #pragma unroll
@@ -322,9 +325,9 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
                     const int tc = 16 * r + 4 * m + w;
                     if (tc >= TH) continue;      // past TH in both halves
                     float2 v = make_float2(acc[r][4 * m + 2 * w], acc[r][4 * m + 2 * w + 1]);
-                    const float2 f = MULK ? mul[r][m][w] : make_float2(0.0f, 0.0f);
-                    if (MULK == 2) v = cmul(v, f);
-                    else if (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
+                    const MulT f = MULK ? mul[r][m][w] : MulT{};
+                    if constexpr (MULK == 2) v = cmul(v, f);
+                    else if constexpr (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
                     oput(ob, tc, 2 * h, v, f, ok && outok(tc));
                 }
             }
@@ -351,13 +354,22 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     constexpr int THL = SM == ST_POW ? (RL + 1) / 2 : RL;
     // the forward column and inverse row passes apply the inter-pass twiddle
     constexpr bool TW = SM == ST_CPX && LM != LD_CPX;
+    // FP16: unit-magnitude twiddles (the stage twiddles, the inverse row
+    // pass's inter-pass twiddle) are applied as packed FP16 words after the
+    // result is packed (v_cvt_pk + two packed-half ops instead of four FP32
+    // ops and v_cvt_pk); |v w| = |v| keeps the packing range the same.  The
+    // forward column pass's inter-pass twiddle carries its scale and stays
+    // FP32, as does all of BF16.
+    constexpr bool PKT = DT == HDT_F16;
+    constexpr bool TWP = PKT && TW && LM == LD_PROD;
+    using TwT = std::conditional_t<PKT, uint32_t, float2>;
     // one dummy word per lane after the rows takes the masked LDS stores
     __shared__ uint32_t buf0[G * RP + 64];
     __shared__ uint32_t buf1[S3 ? G * RP + 64 : 1];
     // the LDS tables are filled without branches (entries past the end are
     // copies, unused)
     constexpr int NTW = (L + NTH - 1) / NTH;
-    __shared__ float2 tws[NTW * NTH];
+    __shared__ TwT tws[NTW * NTH];
     __shared__ uint32_t carr[LM == LD_MIX ? NTH : 1];
     __shared__ PeakPart red[NTH / 64];
     int b, grp;
@@ -375,7 +387,10 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     if constexpr (LM == LD_MIX) cav = a.carr[threadIdx.x & 255];
     auto fill = [&]() {
 #pragma unroll
-        for (int q = 0; q < NTW; q++) tws[threadIdx.x + q * NTH] = twv[q];
+        for (int q = 0; q < NTW; q++) {
+            if constexpr (PKT) tws[threadIdx.x + q * NTH] = pack_c<DT>(twv[q].x, twv[q].y);
+            else tws[threadIdx.x + q * NTH] = twv[q];
+        }
         if constexpr (LM == LD_MIX) carr[threadIdx.x] = cav;
         __syncthreads();
     };
@@ -427,18 +442,20 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         // output t of column j sits at q = j R0 + t (LDS j (R0 + PD) + t); the
         // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages)
         auto obase = [&](int g, int j) -> int4 { return make_int4(g * RP + j * (R0 + PD), 0, (j / R2) * R2, 1); };
-        auto omul = [&](const int4 &ob, int tc, int th, bool ok) -> float2 { return tws[(ok ? tc + th : 0) * ob.z]; };
-        auto oput = [&](const int4 &ob, int tc, int th, float2 v, float2, bool ok) {
-            buf0[ok ? ob.x + tc + th : dummy] = pack_c<DT>(v.x, v.y);
+        auto omul = [&](const int4 &ob, int tc, int th, bool ok) -> TwT { return tws[(ok ? tc + th : 0) * ob.z]; };
+        auto oput = [&](const int4 &ob, int tc, int th, float2 v, TwT f, bool ok) {
+            uint32_t w = pack_c<DT>(v.x, v.y);
+            if constexpr (PKT) w = cmul_w<DT>(w, f);
+            buf0[ok ? ob.x + tc + th : dummy] = w;
         };
-        mstage<DT, NTH, L, R0, G, NF, R0, 2, true>(at0, ibase, iload, icook, obase, omul, oput, fill);
+        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true>(at0, ibase, iload, icook, obase, omul, oput, fill);
     }
 
     // ---- last stage: LDS -> MFMA -> global ----
     Storer<ST_POW> sp;
     if constexpr (SM == ST_POW) sp.init(a, b);
     const __amdgpu_buffer_rsrc_t hout = rsrc(a.hout + (SM == ST_CPX ? (long long)b * a.out_bstride : 0LL));
-    const __amdgpu_buffer_rsrc_t tfr = rsrc(a.tfull);
+    const __amdgpu_buffer_rsrc_t tfr = rsrc(TWP ? (const void *)a.tfullh : (const void *)a.tfull);
     const float osc = a.osc, kap2 = a.kappa * a.kappa;
     const int ogs = (int)a.out_gstride, ons = NF ? 1 : (int)a.out_nstride;
     const int estep = NSL * ons;
@@ -455,26 +472,31 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         const bool ok = g < gcount;
         return make_int4((g0 + (ok ? g : 0)) * ogs + j * ons, 0, 0, ok);
     };
-    auto gmul = [&](const int4 &ob, int tc, int th, bool ok) -> float2 {
+    auto gmul = [&](const int4 &ob, int tc, int th, bool ok) {
         if constexpr (SM == ST_POW) {
             const int e = ob.x + (tc + th) * estep;
             return make_float2(1.0f, __uint_as_float(bload32(prr, e, pacc && ok && e < nlag)));
+        } else if constexpr (TWP) {
+            return bload32_at(tfr, ob.x + th * estep, tc * estep, ok);
         } else if constexpr (TW) {
             return bload_f2_at(tfr, ob.x + th * estep, tc * estep, ok);
         } else {
             return make_float2(osc, 0.0f);
         }
     };
-    auto gput = [&](const int4 &ob, int tc, int th, float2 v, float2 f, bool ok) {
+    using GMulT = decltype(gmul(int4{}, 0, 0, true));
+    auto gput = [&](const int4 &ob, int tc, int th, float2 v, GMulT f, bool ok) {
         if constexpr (SM == ST_POW) {
             sp.store_pow_sel(ob.x + (tc + th) * estep, v, kap2, f.y, ok, [&](bool ok2, int e2, float nv) {
                 bstore32(prr, e2, __float_as_uint(nv), ok2);
             });
         } else {
-            bstore32_at(hout, ob.x + th * estep, tc * estep, pack_c<DT>(v.x, v.y), ok);
+            uint32_t w = pack_c<DT>(v.x, v.y);
+            if constexpr (TWP) w = cmul_w<DT>(w, f);
+            bstore32_at(hout, ob.x + th * estep, tc * estep, w, ok);
         }
     };
-    constexpr int MULKL = TW ? 2 : 1;
+    constexpr int MULKL = TWP ? 3 : TW ? 2 : 1;
     // stage 1 input t of column j: n = j + t R0 R2
     // (unused inputs read t = R - 1 of the stage: written words)
     auto iat0 = [&](int base, int tc, int th, bool ok) -> uint32_t {
@@ -492,15 +514,18 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             const int jh = j / R0, jl = j - jh * R0;
             return make_int4(g * RP + jh * R0 * R1 + jl + (PD ? jh * R1 : 0), jh * jl, jh * R0, 1);
         };
-        auto omul1 = [&](const int4 &ob, int tc, int th, bool ok) -> float2 {
+        auto omul1 = [&](const int4 &ob, int tc, int th, bool ok) -> TwT {
             return tws[ob.y + (ok ? tc + th : 0) * ob.z];
         };
-        auto oput1 = [&](const int4 &ob, int tc, int th, float2 v, float2, bool ok) {
-            buf1[ok ? ob.x + (tc + th) * (R0 + PD) : dummy] = pack_c<DT>(v.x, v.y);
+        auto oput1 = [&](const int4 &ob, int tc, int th, float2 v, TwT f, bool ok) {
+            uint32_t w = pack_c<DT>(v.x, v.y);
+            if constexpr (PKT) w = cmul_w<DT>(w, f);
+            buf1[ok ? ob.x + (tc + th) * (R0 + PD) : dummy] = w;
         };
         ATab<DT, R1, R1> at1;
         at1.load(a.amat[1]);
-        mstage<DT, NTH, L, R1, G, NF, R1, 2, false>(at1, lds_base, iat0, ident, obase1, omul1, oput1, barrier);
+        mstage<DT, NTH, L, R1, G, NF, R1, PKT ? 3 : 2, false>(at1, lds_base, iat0, ident, obase1, omul1, oput1,
+                                                              barrier);
         // stage 2 input t of column j: n = j + t R0 R1
         auto iat1 = [&](int base, int tc, int th, bool ok) -> uint32_t {
             return buf1[base + (ok ? tc + th : R2 - 1) * (R0 * R1 + PD * R1)];
@@ -681,6 +706,22 @@ static float2 *make_digit_twiddle(int M, int M2, int sign, double scale)
     return d;
 }
 
+// FP16 (re, im) words of a float2 table, round to nearest even
+__global__ void pack_f16_kernel(const float2 *__restrict__ in, uint32_t *__restrict__ out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pack_c<HDT_F16>(in[i].x, in[i].y);
+}
+static uint32_t *pack_table_f16(const float2 *d, int n)
+{
+    uint32_t *o;
+    PSA_CHECK(hipMalloc(&o, sizeof(uint32_t) * n));
+    hipLaunchKernelGGL(pack_f16_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, d, o, n);
+    PSA_CHECK(hipGetLastError());
+    PSA_CHECK(hipDeviceSynchronize());
+    return o;
+}
+
 static float pow2_near(double x)    // 2^round(log2 x)
 {
     return (float)ldexp(1.0, (int)lround(log2(x)));
@@ -715,6 +756,7 @@ const HPlan *hfft_plan(int M, int dt, int M1)
         p->tw1[dir] = make_twiddle(M1, sg, 1.0);
         p->tw2[dir] = make_twiddle(M2, sg, 1.0);
         p->tfull[dir] = make_digit_twiddle(M, M2, sg, dir == 0 ? p->osc_fc : p->osc_ir);
+        if (dir == 1 && dt == HDT_F16) p->tfullh = pack_table_f16(p->tfull[dir], M);
         const HSpec *cs = col_spec(M1), *rs = row_spec(M2);
         for (int s = 0; s < 3; s++) {
             p->am1[dir][s] = cs->r[s] > 1 ? make_amat(cs->r[s], dir, dt) : nullptr;
@@ -808,6 +850,7 @@ void launch_hpass(int dt, LoadMode lm, StoreMode sm, const PassArgs &a_in, int n
     // the forward column and inverse row passes take the inter-pass twiddle,
     // the forward row pass does not
     if (sm == ST_CPX && (lm != LD_CPX) != (a_in.tfull != nullptr)) PSA_FAIL("MFMA pass: twiddle table mismatch");
+    if (dt == HDT_F16 && lm == LD_PROD && !a_in.tfullh) PSA_FAIL("MFMA pass: FP16 twiddle table missing");
     PassArgs a = a_in;
     a.carr = dev_carrier();
     if (dt == HDT_F16) hlaunch_dt<HDT_F16>(lm, sm, a, nbatch, st);

@@ -182,6 +182,7 @@ struct PassArgs {
     const uint32_t *hcf;        // LD_PROD code-spectrum words
     const uint4 *amat[3];       // MFMA A operands of the stage DFT matrices
     const float2 *tfull;        // ST_CPX: twiddle per output index, pre-scaled; nullptr: none
+    const uint32_t *tfullh;     // the same as FP16 words (inverse row pass, FP16 form)
     float osc;                  // ST_CPX scale applied when tfull is nullptr
     float kappa;                // ST_POW: values scaled by kappa before |.|^2
     int hvar;                   // workgroup shape variant of the MFMA pass
@@ -212,6 +213,7 @@ struct HPlan {
     int dt = HDT_F16;
     float2 *tw1[2] = {}, *tw2[2] = {};   // w_L^m of the two lengths, [dir]
     float2 *tfull[2] = {};               // inter-pass twiddle by output index, [dir]
+    uint32_t *tfullh = nullptr;          // FP16 words of tfull[1] (FP16 plans)
     const uint4 *am1[2][3] = {}, *am2[2][3] = {};   // [dir][stage]
     // power-of-two scales: forward column / row pass outputs, inverse row
     // pass output (folded into tfull where a pass has the twiddle)

@@ -657,6 +657,7 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
                 PassArgs ir = {};
                 h_row_args(hp, 1, ir, s.hvar_row);
                 ir.tfull = hp->tfull[1];
+                ir.tfullh = hp->tfullh;
                 ir.hin = X; ir.in_bstride = M;
                 ir.hcf = s.cfh + (long long)q0 * M; ir.cf_bstride = M;
                 ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
@@ -797,6 +798,34 @@ static void f_setup(sdr_search_plan_t *p, SigDev &s, int M1)
     }
 }
 
+// PSA_TILES fixes the tiling of FFT length M instead of timing it:
+// comma-separated entries [h|f]M=M1/col_shape/row_shape, the prefix h or f
+// restricting an entry to the precision form or to FP32 (the shapes apply to
+// the precision form), e.g. "h96000=300/0/0,f96000=200".  Returns false when
+// the variable has no valid entry for the group.
+static bool fixed_tiles(const SigDev &s, int *m1, int *vc, int *vr)
+{
+    const char *env = getenv("PSA_TILES");
+    if (!env) return false;
+    int cand[8];
+    const int ns = s.hp ? h_splits(s.h.M, cand, 8) : fft_splits(s.h.M, cand, 8);
+    for (const char *q = env; *q;) {
+        int M = 0, a = 0, c = 0, r = 0;
+        bool form_ok = true;
+        if (*q == 'h' || *q == 'f') form_ok = (*q++ == 'h') == (s.hp != nullptr);
+        const int n = sscanf(q, "%d=%d/%d/%d", &M, &a, &c, &r);
+        if (form_ok && n >= 2 && M == s.h.M && std::find(cand, cand + ns, a) != cand + ns && c >= 0 && c < 2 && r >= 0 &&
+            r < 2) {
+            *m1 = a; *vc = s.hp ? c : 0; *vr = s.hp ? r : 0;
+            return true;
+        }
+        const char *comma = strchr(q, ',');
+        if (!comma) break;
+        q = comma + 1;
+    }
+    return false;
+}
+
 // SDR_SEARCH_TUNE: time every candidate tiling of the group's passes on its
 // own shapes - the first block of the search, best of two runs after a
 // warm-up - and keep the fastest.  Precision form: (split, column shape, row
@@ -813,8 +842,14 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
     PSA_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
     int bm = m1[0], bc = 0, br = 0;
+    const bool fixed = fixed_tiles(s, &bm, &bc, &br);
+    if (!fixed && !(p->flags & SDR_SEARCH_TUNE)) {
+        PSA_CHECK(hipEventDestroy(e0));
+        PSA_CHECK(hipEventDestroy(e1));
+        return;
+    }
     s.h.n_blk = 1;
-    for (int i = 0; i < ns; i++) {
+    for (int i = 0; i < (fixed ? 0 : ns); i++) {
         for (int vc = 0; vc < nv; vc++) {
             for (int vr = 0; vr < nv; vr++) {
                 if (h) h_setup(p, s, m1[i], vc, vr);
@@ -839,7 +874,7 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
     else s.pl = fft_plan(s.h.M, bm);
     s.hvar_col = bc;
     s.hvar_row = br;
-    s.tuned_ms = best;
+    s.tuned_ms = fixed ? 0.0f : best;
     PSA_CHECK(hipEventDestroy(e0));
     PSA_CHECK(hipEventDestroy(e1));
 }
@@ -998,7 +1033,7 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     for (auto &s : p->sig) {
         if (!s.cf_nat) continue;
         p->cur_sig = (int)(&s - p->sig.data());
-        if (flags & SDR_SEARCH_TUNE) tune_group(p, s);
+        if ((flags & SDR_SEARCH_TUNE) || getenv("PSA_TILES")) tune_group(p, s);
         if (s.hp) h_setup(p, s, s.hp->M1, s.hvar_col, s.hvar_row);
         else f_setup(p, s, s.pl->M1);
         PSA_CHECK(hipFree(s.cf_nat));

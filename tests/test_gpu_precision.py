@@ -37,12 +37,20 @@ def peak_f64(orc, w, s, q, b):
     return P[0]
 
 
-@pytest.mark.parametrize("form", ["fp16", "bf16", "fp16-tuned"])
-def test_precision_form(sdr, orc, multi_weak, form):
+# non-default splits and workgroup shapes, fixed through PSA_TILES
+FIXED = {24000: (150, 1, 1), 96000: (200, 1, 1), 48000: (200, 0, 1), 192000: (200, 1, 0)}
+
+
+@pytest.mark.parametrize("form", ["fp16", "bf16", "fp16-tuned", "fp16-fixed"])
+def test_precision_form(sdr, orc, multi_weak, form, monkeypatch):
     w = multi_weak
     flag = sdr.SEARCH_BF16 if form == "bf16" else sdr.SEARCH_FP16
     if form.endswith("tuned"):
         flag |= sdr.SEARCH_TUNE        # tiles chosen by timing the candidates
+    if form.endswith("fixed"):
+        monkeypatch.setenv("PSA_TILES", ",".join("h%d=%d/%d/%d" % ((M,) + t) for M, t in FIXED.items()))
+    else:
+        monkeypatch.delenv("PSA_TILES", raising=False)
     want = sdr.FORM_BF16 if form == "bf16" else sdr.FORM_FP16
     surf_tol = 5e-2 if form == "bf16" else 2e-2
     p32 = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_BATCHED)
@@ -55,6 +63,8 @@ def test_precision_form(sdr, orc, multi_weak, form):
         assert ph.form(i) == want, s["name"]
         M1, M2, vc, vr = ph.tiles(i)
         assert M1 * M2 == s["M"] and vc in (0, 1) and vr in (0, 1)
+        if form.endswith("fixed"):
+            assert (M1, vc, vr) == FIXED[s["M"]], (s["name"], ph.tiles(i))
         c32, ix32, d32 = p32.result(i)
         ch, ixh, dh = ph.result(i)
         det32 = c32 >= CN0_THRES
