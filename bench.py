@@ -317,6 +317,44 @@ def traffic_of(inst, grids, profile="search_multi_traffic.md"):
     return round(tot / n) if n else None
 
 
+def bench_raw(args, sdr, orc):
+    """Front end (write_buff): one second of the 8-channel L1 array's raw
+    stream (Pocket SDR FE 8CH, RAW32, 2-bit I/Q, 48 Msps) unpacked to the 8
+    cpx8 IF buffers, inputs and outputs resident in HBM.  Algorithmic bytes:
+    4 raw + 8 cpx8 bytes per sample.  CPU: the oracle's write_buff on the
+    first 1/10 s, one thread."""
+    import torch
+    fmt, N, fs = sdr.FMT_RAW32, 48_000_000, 48e6
+    luts = [sdr.raw_gen_lut(fmt, k, 2, 2) for k in range(8)]
+    plan = sdr.RawPlan(fmt, luts, [0] * 8)
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    draw = torch.randint(0, 256, (N * 4,), dtype=torch.uint8, device="cuda", generator=g)
+    dout = torch.empty((8, N), dtype=torch.uint8, device="cuda")
+    ptrs = [dout[k].data_ptr() for k in range(8)]
+    for _ in range(2):
+        plan.unpack_dev(draw.data_ptr(), N, 0, ptrs)
+    ks = []
+    for i in range(max(3, args.steps)):
+        plan.unpack_dev(draw.data_ptr(), N, i, ptrs)
+        ks.append(plan.kernel_ms())
+    kms = sorted(ks)[len(ks) // 2]
+    gbs = 12.0 * N / (kms * 1e-3) / 1e9
+    res = dict(format="RAW32 (8 ch, 2-bit I/Q)", samples=N, kernel_ms=round(kms, 4),
+               msps_per_ch=round(N / (kms * 1e-3) / 1e6, 1), alg_bytes=12 * N,
+               roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=round(gbs / HBM_PEAK_GBS, 4)))
+    if orc is not None:
+        n = N // 10
+        raw = draw[:n * 4].cpu().numpy()
+        t0 = time.perf_counter()
+        orc.write_buff(fmt, raw, n, 0, [0] * 8, luts)
+        dt = time.perf_counter() - t0
+        res["cpu_msps_per_ch"] = round(n / dt / 1e6, 1)
+        res["cpu_sample"] = "%d samples x 8 ch, oracle write_buff, one thread (%.2f s)" % (n, dt)
+    plan.close()
+    return res
+
+
 def bench_trk(args, sdr, workloads):
     w = workloads.trk_l1ca(seed=args.seed + 1, n_epoch=args.trk_epochs)
     plan = sdr.TrkPlan(w["buff"], w["chans"], w["pos"], w["n_epoch"])
@@ -397,6 +435,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-trk", action="store_true")
+    ap.add_argument("--skip-raw", action="store_true")
     ap.add_argument("--cpu-prns", type=int, default=4)
     ap.add_argument("--workload", default="multi", choices=["multi", "l1ca", "l1cp"])
     ap.add_argument("--cpu-bins", type=int, default=0,
@@ -430,6 +469,8 @@ def main():
     value, t_step, res, cfg = bench_acq(args, sdr, orc)
     if not args.skip_trk:
         res.update(bench_trk(args, sdr, workloads))
+    if not args.skip_raw:
+        res["raw_unpack"] = bench_raw(args, sdr, orc)
     line = dict(metric=METRIC, value=round(value, 1), unit="cells/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=round(t_step * 1e3, 4), higher_is_better=True,
                 scaling="weak", vs_baseline=None, dtype=FORM_DTYPE[args.form],

@@ -221,6 +221,40 @@ int sdr_trk_plan_kernel_ms(const sdr_trk_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);
 void sdr_trk_plan_free(sdr_trk_plan_t *plan);
 
+/* --- front end: raw data to cpx8 IF samples ---------------------------------
+ * The receiver's write_buff (src/sdr_rcv.c:834-914, a static function there)
+ * on the GPU, for the formats SDR_FMT_INT8 ... SDR_FMT_CS16
+ * (src/pocket_sdr.h:85-92); the per-channel LPF and the array combine that
+ * follow it are not included.  Tables as src/sdr_rcv.c:776-831 builds them:
+ * raw formats take lut[k] = the cpx8 table of RF channel k (256 entries, or
+ * 4 x 256 when rtoc[k]: the fs/4 real-to-complex phases); INT8, INT8X2,
+ * CS8 take lut[0] = the int8 AGC table (256 entries), CS16 lut[0] / lut[1]
+ * = its I / Q tables (65536 entries; lut[1] NULL: the I table).  Sample j of
+ * block ix uses phase (ix N + j) & 3, as write_buff.
+ */
+typedef struct sdr_raw_plan sdr_raw_plan_t;
+
+/* gen_LUTC (rtoc = 0) / gen_LUTC_rtoc (rtoc = 1: 1024 entries),
+ * src/sdr_rcv.c:776-799, 817-831 */
+void sdr_raw_gen_lut(int fmt, int rfch, int IQ, int bits, int rtoc,
+    sdr_cpx8_t *lut);
+/* gen_LUT8, src/sdr_rcv.c:802-814 (65536 entries for SDR_FMT_CS16) */
+void sdr_raw_gen_lut8(int fmt, double off, double scale, int8_t *lut);
+sdr_raw_plan_t *sdr_raw_plan_new(int fmt, const int *rtoc,
+    const void *const *lut);
+/* RF channels the format carries (1, 2, 4 or 8) */
+int sdr_raw_plan_nch(const sdr_raw_plan_t *plan);
+/* host pointers: N samples (N x bytes-per-sample raw bytes) of block ix to
+ * out[k][0..N-1] for every channel k; returns 1 on success */
+int sdr_raw_unpack(sdr_raw_plan_t *plan, const uint8_t *raw, int N,
+    int64_t ix, sdr_cpx8_t *const *out);
+/* the same on device memory (16-byte aligned pointers), HBM-resident */
+int sdr_raw_unpack_dev(sdr_raw_plan_t *plan, const uint8_t *raw, long long N,
+    int64_t ix, sdr_cpx8_t *const *out);
+/* kernel time of the last unpack call (ms, HIP events) */
+double sdr_raw_plan_kernel_ms(const sdr_raw_plan_t *plan);
+void sdr_raw_plan_free(sdr_raw_plan_t *plan);
+
 /* library version string */
 const char *sdr_amd_version(void);
 

@@ -58,6 +58,10 @@ _d("orc_gen_code_fft_f64", None, [vp, C.c_int, C.c_double, C.c_double, C.c_doubl
                                   C.c_int, vp])
 _d("orc_gen_code_L1CA", C.c_int, [C.c_int, vp])
 _d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, C.c_int, vp, vp, dp])
+_d("orc_gen_lutc", None, [C.c_int, C.c_int, C.c_int, C.c_int, vp])
+_d("orc_gen_lutc_rtoc", None, [C.c_int, vp])
+_d("orc_gen_lut8", None, [C.c_int, C.c_double, C.c_double, vp])
+_d("orc_write_buff", C.c_int, [C.c_int, vp, C.c_int, C.c_longlong, ip, C.POINTER(vp), C.POINTER(vp)])
 
 lib.orc_func_init()
 REF_LOADED = bool(os.path.exists(REF_FFT) and lib.orc_load_ref_fft(REF_FFT.encode()))
@@ -210,3 +214,45 @@ def trk_run(ch, code_bank_arr, buff, n_epoch, pos):
     lib.orc_trk_run(C.byref(c), _p(code_bank_arr), _p(buff), len(buff), n_epoch,
                     pos.ctypes.data_as(dp), len(pos), _p(corr), _p(Cc), state.ctypes.data_as(dp))
     return corr, Cc, state
+
+
+# front-end raw data to cpx8 (src/sdr_rcv.c:776-936)
+FMT_INT8, FMT_INT8X2, FMT_RAW8, FMT_RAW16, FMT_RAW16I, FMT_RAW32, FMT_CS8, FMT_CS16 = range(1, 9)
+RAW_NCH = {FMT_INT8: 1, FMT_INT8X2: 1, FMT_CS8: 1, FMT_CS16: 1, FMT_RAW8: 2, FMT_RAW16: 4,
+           FMT_RAW16I: 8, FMT_RAW32: 8}
+RAW_BPS = {FMT_INT8: 1, FMT_INT8X2: 2, FMT_CS8: 2, FMT_CS16: 4, FMT_RAW8: 1, FMT_RAW16: 2,
+           FMT_RAW16I: 2, FMT_RAW32: 4}
+
+
+def gen_lutc(fmt, rfch, IQ, bits):
+    out = np.zeros(256, np.uint8)
+    lib.orc_gen_lutc(fmt, rfch, IQ, bits, _p(out))
+    return out
+
+
+def gen_lutc_rtoc(rfch):
+    out = np.zeros(1024, np.uint8)
+    lib.orc_gen_lutc_rtoc(rfch, _p(out))
+    return out
+
+
+def gen_lut8(fmt, off, scale):
+    out = np.zeros(65536 if fmt == FMT_CS16 else 256, np.int8)
+    lib.orc_gen_lut8(fmt, off, scale, _p(out))
+    return out
+
+
+def write_buff(fmt, raw, N, ix, rtoc, luts):
+    """Per-channel cpx8 arrays (nch x N) of N samples of block ix.  luts: per
+    channel cpx8 LUT (raw formats) or [LUT8, LUTQ or None] (int8 formats)."""
+    raw = np.ascontiguousarray(raw, np.uint8)
+    nch = RAW_NCH[fmt]
+    out = np.zeros((8, N), np.uint8)
+    luts = [None if l is None else np.ascontiguousarray(l) for l in luts] + [None] * 8
+    lp = (vp * 8)(*[None if l is None else l.ctypes.data for l in luts[:8]])
+    op = (vp * 8)(*[out[k].ctypes.data for k in range(8)])
+    rt = (C.c_int * 8)(*(list(rtoc) + [0] * 8)[:8])
+    n = lib.orc_write_buff(fmt, _p(raw), N, ix, rt, lp, op)
+    assert n == nch
+    return out[:nch]
+

@@ -558,3 +558,140 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
     }
     free(IQ);
 }
+
+
+/* ---------------------------------------------------------------------------
+ * front-end raw data to cpx8, src/sdr_rcv.c:776-936 (write_buff and its LUT
+ * generators); the LPF and array-combine steps after it are not included
+ * ------------------------------------------------------------------------- */
+#define FMT_INT8 1      /* src/pocket_sdr.h:85-92 */
+#define FMT_INT8X2 2
+#define FMT_RAW8 3
+#define FMT_RAW16 4
+#define FMT_RAW16I 5
+#define FMT_RAW32 6
+#define FMT_CS8 7
+#define FMT_CS16 8
+
+/* SDR_CPX8, src/pocket_sdr.h:105 */
+static inline uint8_t mk_cpx8(int re, int im)
+{
+    return (uint8_t)(((int8_t)im << 4) | (((int8_t)(re << 4) >> 4) & 0xF));
+}
+
+/* src/sdr_rcv.c:776-799 */
+void orc_gen_lutc(int fmt, int rfch, int IQ, int bits, uint8_t *lut)
+{
+    static const int8_t vi2[] = {1, 3, -1, -3}, vq2[] = {-1, -3, 1, 3};
+    static const int8_t vi3[] = {1, 3, 5, 7, -1, -3, -5, -7};
+    for (int i = 0; i < 256; i++) {
+        int I, Q = 0;
+        if (fmt == FMT_RAW16I) {
+            I = vi2[(i >> ((rfch % 4) * 2)) & 3];
+        } else if (IQ == 1 && bits == 2) {
+            I = vi2[(i >> (rfch % 2 ? 4 : 0)) & 3];
+        } else if (IQ == 1 && bits == 3) {
+            int v = (i >> (rfch % 2 ? 4 : 0)) & 0xF;
+            I = vi3[((v << 1) & 6) + ((v >> 3) & 1)];
+        } else {
+            I = vi2[(i >> (rfch % 2 ? 4 : 0)) & 3];
+            Q = vq2[(i >> (rfch % 2 ? 6 : 2)) & 3];
+        }
+        lut[i] = mk_cpx8(I, Q);
+    }
+}
+
+/* src/sdr_rcv.c:817-831 */
+void orc_gen_lutc_rtoc(int rfch, uint8_t *lut)
+{
+    static const int8_t vi3[] = {1, 3, 5, 7, -1, -3, -5, -7};
+    for (int i = 0; i < 256; i++) {
+        int v = (i >> (rfch % 2 ? 4 : 0)) & 0xF;
+        int I = vi3[((v << 1) & 6) + ((v >> 3) & 1)];
+        lut[0 * 256 + i] = mk_cpx8(I, 0);
+        lut[1 * 256 + i] = mk_cpx8(0, -I);
+        lut[2 * 256 + i] = mk_cpx8(-I, 0);
+        lut[3 * 256 + i] = mk_cpx8(0, I);
+    }
+}
+
+/* src/sdr_rcv.c:802-814 */
+void orc_gen_lut8(int fmt, double off, double scale, int8_t *lut)
+{
+    int N = fmt == FMT_CS16 ? 65536 : 256;
+    if (scale <= 0.0) scale = 1.0;
+    for (int i = -N / 2; i < N / 2; i++) {
+        int val = (int)floor((i - off) / scale + 0.5);
+        int j = fmt == FMT_CS16 ? (uint16_t)i : (uint8_t)i;
+        lut[j] = (int8_t)(val < -7 ? -7 : (val > 7 ? 7 : val));
+    }
+}
+
+/* src/sdr_rcv.c:834-914 */
+int orc_write_buff(int fmt, const uint8_t *raw, int N, long long ix, const int *rtoc,
+                   const void *const *lut, uint8_t *const *out)
+{
+    const int8_t *L8 = (const int8_t *)lut[0];
+    const int ph0 = (int)((ix * N) & 3);
+#define WR(k, byte, ph) (out[k][j] = rtoc[k] ? ((const uint8_t *)lut[k])[((ph) << 8) | (byte)] \
+                                             : ((const uint8_t *)lut[k])[(byte)])
+    if (fmt == FMT_INT8) {
+        for (int j = 0; j < N; j++) {
+            int I = L8[raw[j]];
+            if (rtoc[0]) {
+                int ph = (ph0 + j) & 3;
+                out[0][j] = ph == 0 ? mk_cpx8(I, 0) : ph == 1 ? mk_cpx8(0, -I) :
+                            ph == 2 ? mk_cpx8(-I, 0) : mk_cpx8(0, I);
+            } else {
+                out[0][j] = mk_cpx8(I, 0);
+            }
+        }
+        return 1;
+    }
+    if (fmt == FMT_INT8X2 || fmt == FMT_CS8) {
+        for (int j = 0; j < N; j++) {
+            int I = L8[raw[2 * j]], Q = L8[raw[2 * j + 1]];
+            out[0][j] = mk_cpx8(I, fmt == FMT_INT8X2 ? -Q : Q);
+        }
+        return 1;
+    }
+    if (fmt == FMT_CS16) {
+        const int8_t *LQ = lut[1] ? (const int8_t *)lut[1] : L8;
+        for (int j = 0; j < N; j++) {
+            uint16_t wi = (uint16_t)(raw[4 * j] | (raw[4 * j + 1] << 8));
+            uint16_t wq = (uint16_t)(raw[4 * j + 2] | (raw[4 * j + 3] << 8));
+            out[0][j] = mk_cpx8(L8[wi], LQ[wq]);
+        }
+        return 1;
+    }
+    if (fmt == FMT_RAW8) {
+        for (int j = 0; j < N; j++) {
+            int ph = (ph0 + j) & 3;
+            WR(0, raw[j], ph); WR(1, raw[j], ph);
+        }
+        return 2;
+    }
+    if (fmt == FMT_RAW16) {
+        for (int j = 0; j < N; j++) {
+            int ph = (ph0 + j) & 3;
+            WR(0, raw[2 * j], ph); WR(1, raw[2 * j], ph);
+            WR(2, raw[2 * j + 1], ph); WR(3, raw[2 * j + 1], ph);
+        }
+        return 4;
+    }
+    if (fmt == FMT_RAW16I) {
+        for (int j = 0; j < N; j++) {
+            for (int k = 0; k < 8; k++) out[k][j] = ((const uint8_t *)lut[k])[raw[2 * j + (k >> 2)]];
+        }
+        return 8;
+    }
+    if (fmt == FMT_RAW32) {
+        for (int j = 0; j < N; j++) {
+            int ph = (ph0 + j) & 3;
+            for (int k = 0; k < 8; k++) WR(k, raw[4 * j + (k >> 1)], ph);
+        }
+        return 8;
+    }
+#undef WR
+    return 0;
+}

@@ -101,6 +101,20 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
                  const double *pos, int n_pos, float *corr, float *C,
                  double *state);
 
+/* Front-end raw data to cpx8 IF buffers, src/sdr_rcv.c:776-936.  fmt: the
+ * SDR_FMT_* codes of src/pocket_sdr.h:85-92.  LUT generators: gen_LUTC
+ * (256 cpx8), gen_LUTC_rtoc (4 x 256 cpx8, the fs/4 real-to-complex phases),
+ * gen_LUT8 (256, or 65536 for CS16, int8 clipped to +-7). */
+void orc_gen_lutc(int fmt, int rfch, int IQ, int bits, uint8_t *lut);
+void orc_gen_lutc_rtoc(int rfch, uint8_t *lut);
+void orc_gen_lut8(int fmt, double off, double scale, int8_t *lut);
+/* write_buff for N samples of block index ix: lut[k] (cpx8 LUT of RF channel
+ * k, or LUT8 / LUTQ for the int8 formats), rtoc[k] the real-to-complex flag
+ * of channel k; out[k] receives N cpx8 samples of channel k.  Returns the
+ * number of channels written. */
+int orc_write_buff(int fmt, const uint8_t *raw, int N, long long ix, const int *rtoc,
+                   const void *const *lut, uint8_t *const *out);
+
 #ifdef __cplusplus
 }
 #endif

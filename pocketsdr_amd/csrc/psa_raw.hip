@@ -1,0 +1,333 @@
+// psa_raw.hip - front-end raw data to cpx8 IF samples: the receiver's
+// write_buff (src/sdr_rcv.c:834-914) and its LUT generators
+// (src/sdr_rcv.c:776-831) for the Pocket SDR / Spider SDR packed raw formats
+// (RAW8 2 channels, RAW16 4, RAW16I 8, RAW32 8; 2-bit I/Q, 2- or 3-bit I and
+// the fs/4 real-to-complex phases) and the int8 / int16 formats (INT8,
+// INT8X2, CS8, CS16 through the AGC table).  The optional per-channel LPF and
+// the array combine that follow write_buff in the receiver are not part of it.
+//
+// Each thread converts 16 consecutive samples of every channel: one vector
+// load of 16 x (bytes per sample), table lookups in LDS (the int16 tables of
+// CS16, 64 KiB each, are read through the cache), one 16-byte store per
+// channel.  The work is a byte stream: bound by HBM.
+#include <string.h>
+#include <algorithm>
+#include <vector>
+#include "psa_internal.h"
+
+namespace psa {
+
+enum { F_INT8 = 1, F_INT8X2 = 2, F_RAW8 = 3, F_RAW16 = 4, F_RAW16I = 5, F_RAW32 = 6, F_CS8 = 7, F_CS16 = 8 };
+
+static int raw_nch(int fmt)
+{
+    switch (fmt) {
+    case F_RAW8: return 2;
+    case F_RAW16: return 4;
+    case F_RAW16I: case F_RAW32: return 8;
+    case F_INT8: case F_INT8X2: case F_CS8: case F_CS16: return 1;
+    default: return 0;
+    }
+}
+static int raw_bps(int fmt)      // raw bytes per sample
+{
+    switch (fmt) {
+    case F_INT8: case F_RAW8: return 1;
+    case F_INT8X2: case F_CS8: case F_RAW16: case F_RAW16I: return 2;
+    case F_RAW32: case F_CS16: return 4;
+    default: return 0;
+    }
+}
+
+// SDR_CPX8 (src/pocket_sdr.h:105) for components in [-8, 7]
+__host__ __device__ __forceinline__ uint8_t mk_cpx8(int re, int im)
+{
+    return (uint8_t)((((unsigned)im & 0xF) << 4) | ((unsigned)re & 0xF));
+}
+
+struct RawArgs {
+    const uint8_t *raw;
+    long long N;
+    int ph0;                    // phase of sample 0: (ix N) & 3
+    int rtoc;                   // bit k: channel k takes the fs/4 phase tables
+    const uint8_t *lutc;        // raw formats: [8][1024] cpx8 (phase p at p * 256)
+    const int8_t *l8, *lq;      // int8 formats: I (and CS16 Q) tables
+    uint8_t *out[8];
+};
+
+// cpx8 of channel k for raw byte `byte` at phase ph
+template <int FMT>
+__device__ __forceinline__ uint8_t raw_ch(const uint8_t *lut, int rtoc, int k, uint32_t byte, int ph)
+{
+    const int off = (FMT != F_RAW16I && ((rtoc >> k) & 1)) ? ph << 8 : 0;
+    return lut[k * 1024 + off + byte];
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(256) raw_unpack_kernel(RawArgs a)
+{
+    constexpr int NCH = FMT == F_RAW8 ? 2 : FMT == F_RAW16 ? 4 : (FMT == F_RAW16I || FMT == F_RAW32) ? 8 : 1;
+    constexpr int BPS = (FMT == F_INT8 || FMT == F_RAW8) ? 1
+                        : (FMT == F_RAW32 || FMT == F_CS16) ? 4 : 2;
+    constexpr bool RAWF = FMT >= F_RAW8 && FMT <= F_RAW32;
+    constexpr int LUTB = RAWF ? 8 * 1024 : (FMT == F_CS16 ? 4 : 256);
+    __shared__ __align__(16) uint8_t lut[LUTB];
+    // table to LDS (CS16 reads its 64 KiB tables from global memory)
+    if constexpr (RAWF) {
+        for (int i = threadIdx.x; i < LUTB / 16; i += 256)
+            reinterpret_cast<uint4 *>(lut)[i] = reinterpret_cast<const uint4 *>(a.lutc)[i];
+    } else if constexpr (FMT != F_CS16) {
+        lut[threadIdx.x] = (uint8_t)a.l8[threadIdx.x];
+    }
+    __syncthreads();
+    const int8_t *l8s = reinterpret_cast<const int8_t *>(lut);
+    const long long stride = (long long)gridDim.x * 256 * 16;
+    for (long long s0 = ((long long)blockIdx.x * 256 + threadIdx.x) * 16; s0 < a.N; s0 += stride) {
+        const int nq = (int)min(16LL, a.N - s0);
+        // raw bytes of the 16 samples (a partial group reads byte by byte)
+        uint32_t w[BPS * 4];
+        if (nq == 16) {
+#pragma unroll
+            for (int v = 0; v < BPS; v++) {
+                const uint4 q = reinterpret_cast<const uint4 *>(a.raw + s0 * BPS)[v];
+                w[4 * v] = q.x; w[4 * v + 1] = q.y; w[4 * v + 2] = q.z; w[4 * v + 3] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < BPS * 4; i++) w[i] = 0;
+            for (int i = 0; i < nq * BPS; i++) w[i >> 2] |= (uint32_t)a.raw[s0 * BPS + i] << (8 * (i & 3));
+        }
+        auto rb = [&](int i) -> uint32_t { return (w[i >> 2] >> (8 * (i & 3))) & 0xFF; };   // raw byte i
+        uint32_t o[NCH][4];
+#pragma unroll
+        for (int k = 0; k < NCH; k++) o[k][0] = o[k][1] = o[k][2] = o[k][3] = 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int ph = (a.ph0 + (int)(s0 & 3) + q) & 3;
+            uint32_t c[NCH];
+            if constexpr (FMT == F_INT8) {
+                const int I = l8s[rb(q)];
+                const bool r = a.rtoc & 1;
+                c[0] = !r || ph == 0 ? mk_cpx8(I, 0) : ph == 1 ? mk_cpx8(0, -I) : ph == 2 ? mk_cpx8(-I, 0)
+                                                                                          : mk_cpx8(0, I);
+            } else if constexpr (FMT == F_INT8X2) {
+                c[0] = mk_cpx8(l8s[rb(2 * q)], -l8s[rb(2 * q + 1)]);
+            } else if constexpr (FMT == F_CS8) {
+                c[0] = mk_cpx8(l8s[rb(2 * q)], l8s[rb(2 * q + 1)]);
+            } else if constexpr (FMT == F_CS16) {
+                const uint32_t wi = w[q] & 0xFFFF, wq = w[q] >> 16;
+                c[0] = mk_cpx8(a.l8[wi], a.lq[wq]);
+            } else if constexpr (FMT == F_RAW8) {
+                c[0] = raw_ch<FMT>(lut, a.rtoc, 0, rb(q), ph);
+                c[1] = raw_ch<FMT>(lut, a.rtoc, 1, rb(q), ph);
+            } else if constexpr (FMT == F_RAW16) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) c[k] = raw_ch<FMT>(lut, a.rtoc, k, rb(2 * q + (k >> 1)), ph);
+            } else if constexpr (FMT == F_RAW16I) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) c[k] = raw_ch<FMT>(lut, a.rtoc, k, rb(2 * q + (k >> 2)), ph);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; k++) c[k] = raw_ch<FMT>(lut, a.rtoc, k, rb(4 * q + (k >> 1)), ph);
+            }
+#pragma unroll
+            for (int k = 0; k < NCH; k++) o[k][q >> 2] |= c[k] << (8 * (q & 3));
+        }
+        if (nq == 16) {
+#pragma unroll
+            for (int k = 0; k < NCH; k++)
+                *reinterpret_cast<uint4 *>(a.out[k] + s0) = make_uint4(o[k][0], o[k][1], o[k][2], o[k][3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NCH; k++) {
+                for (int q = 0; q < nq; q++) a.out[k][s0 + q] = (uint8_t)(o[k][q >> 2] >> (8 * (q & 3)));
+            }
+        }
+    }
+}
+
+}  // namespace psa
+
+using namespace psa;
+
+struct sdr_raw_plan {
+    int fmt = 0, nch = 0, rtoc = 0;
+    uint8_t *lutc = nullptr;            // device [8][1024]
+    int8_t *l8 = nullptr, *lq = nullptr;
+    uint8_t *draw = nullptr, *dout = nullptr;   // staging of the host-pointer entry
+    size_t raw_cap = 0, out_cap = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float last_ms = 0.0f;
+};
+
+namespace psa {
+
+static void raw_launch(sdr_raw_plan_t *p, const uint8_t *raw, long long N, long long ix, uint8_t *const *out)
+{
+    RawArgs a = {};
+    a.raw = raw; a.N = N;
+    a.ph0 = (int)((ix * N) & 3);
+    a.rtoc = p->rtoc;
+    a.lutc = p->lutc; a.l8 = p->l8; a.lq = p->lq ? p->lq : p->l8;
+    for (int k = 0; k < 8; k++) a.out[k] = k < p->nch ? out[k] : nullptr;
+    const long long groups = (N + 15) / 16;
+    const int blocks = (int)std::max(1LL, std::min((groups + 255) / 256, 8192LL));
+    PSA_CHECK(hipEventRecord(p->e0, p->st));
+    switch (p->fmt) {
+#define PSA_RAW(F) case F: hipLaunchKernelGGL(raw_unpack_kernel<F>, dim3(blocks), dim3(256), 0, p->st, a); break;
+    PSA_RAW(F_INT8) PSA_RAW(F_INT8X2) PSA_RAW(F_RAW8) PSA_RAW(F_RAW16) PSA_RAW(F_RAW16I) PSA_RAW(F_RAW32)
+    PSA_RAW(F_CS8) PSA_RAW(F_CS16)
+#undef PSA_RAW
+    }
+    PSA_CHECK(hipGetLastError());
+    PSA_CHECK(hipEventRecord(p->e1, p->st));
+}
+
+}  // namespace psa
+
+extern "C" {
+
+// gen_LUTC / gen_LUTC_rtoc, src/sdr_rcv.c:776-799, 817-831
+void sdr_raw_gen_lut(int fmt, int rfch, int IQ, int bits, int rtoc, sdr_cpx8_t *lut)
+{
+    static const int8_t vi2[] = {1, 3, -1, -3}, vq2[] = {-1, -3, 1, 3};
+    static const int8_t vi3[] = {1, 3, 5, 7, -1, -3, -5, -7};
+    for (int i = 0; i < 256; i++) {
+        const int nib = (i >> (rfch % 2 ? 4 : 0)) & 0xF;
+        if (rtoc) {
+            const int I = vi3[((nib << 1) & 6) + ((nib >> 3) & 1)];
+            lut[i] = mk_cpx8(I, 0);
+            lut[256 + i] = mk_cpx8(0, -I);
+            lut[512 + i] = mk_cpx8(-I, 0);
+            lut[768 + i] = mk_cpx8(0, I);
+            continue;
+        }
+        int I, Q = 0;
+        if (fmt == F_RAW16I) I = vi2[(i >> ((rfch % 4) * 2)) & 3];
+        else if (IQ == 1 && bits == 2) I = vi2[nib & 3];
+        else if (IQ == 1 && bits == 3) I = vi3[((nib << 1) & 6) + ((nib >> 3) & 1)];
+        else {
+            I = vi2[nib & 3];
+            Q = vq2[(i >> (rfch % 2 ? 6 : 2)) & 3];
+        }
+        lut[i] = mk_cpx8(I, Q);
+    }
+}
+
+// gen_LUT8, src/sdr_rcv.c:802-814 (256 entries; 65536 for CS16)
+void sdr_raw_gen_lut8(int fmt, double off, double scale, int8_t *lut)
+{
+    const int N = fmt == F_CS16 ? 65536 : 256;
+    if (scale <= 0.0) scale = 1.0;
+    for (int i = -N / 2; i < N / 2; i++) {
+        const int val = (int)floor((i - off) / scale + 0.5);
+        const int j = fmt == F_CS16 ? (uint16_t)i : (uint8_t)i;
+        lut[j] = (int8_t)std::min(7, std::max(-7, val));
+    }
+}
+
+sdr_raw_plan_t *sdr_raw_plan_new(int fmt, const int *rtoc, const void *const *lut)
+{
+    const int nch = raw_nch(fmt);
+    if (!nch || !lut || !lut[0]) {
+        fprintf(stderr, "pocketsdr_amd: sdr_raw_plan_new: bad format %d or tables\n", fmt);
+        return nullptr;
+    }
+    ensure_init();
+    sdr_raw_plan_t *p = new sdr_raw_plan_t();
+    p->fmt = fmt;
+    p->nch = nch;
+    if (fmt >= F_RAW8 && fmt <= F_RAW32) {
+        std::vector<uint8_t> h(8 * 1024, 0);
+        for (int k = 0; k < nch; k++) {
+            if (!lut[k]) {
+                fprintf(stderr, "pocketsdr_amd: sdr_raw_plan_new: channel %d has no table\n", k);
+                delete p;
+                return nullptr;
+            }
+            const int r = fmt != F_RAW16I && rtoc && rtoc[k];
+            p->rtoc |= r << k;
+            memcpy(&h[k * 1024], lut[k], r ? 1024 : 256);
+        }
+        PSA_CHECK(hipMalloc(&p->lutc, h.size()));
+        PSA_CHECK(hipMemcpy(p->lutc, h.data(), h.size(), hipMemcpyHostToDevice));
+    } else {
+        const size_t n = fmt == F_CS16 ? 65536 : 256;
+        p->rtoc = fmt == F_INT8 && rtoc && rtoc[0] ? 1 : 0;
+        PSA_CHECK(hipMalloc(&p->l8, n));
+        PSA_CHECK(hipMemcpy(p->l8, lut[0], n, hipMemcpyHostToDevice));
+        if (fmt == F_CS16 && lut[1]) {
+            PSA_CHECK(hipMalloc(&p->lq, n));
+            PSA_CHECK(hipMemcpy(p->lq, lut[1], n, hipMemcpyHostToDevice));
+        }
+    }
+    PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+    PSA_CHECK(hipEventCreate(&p->e0));
+    PSA_CHECK(hipEventCreate(&p->e1));
+    return p;
+}
+
+int sdr_raw_plan_nch(const sdr_raw_plan_t *p) { return p ? p->nch : 0; }
+
+int sdr_raw_unpack(sdr_raw_plan_t *p, const uint8_t *raw, int N, int64_t ix, sdr_cpx8_t *const *out)
+{
+    if (!p || !raw || !out || N < 0) return 0;
+    if (N == 0) return 1;
+    const size_t rb = (size_t)N * raw_bps(p->fmt), ob = (size_t)N * p->nch;
+    if (rb > p->raw_cap) {
+        (void)hipFree(p->draw);
+        PSA_CHECK(hipMalloc(&p->draw, rb));
+        p->raw_cap = rb;
+    }
+    if (ob > p->out_cap) {
+        (void)hipFree(p->dout);
+        PSA_CHECK(hipMalloc(&p->dout, ob));
+        p->out_cap = ob;
+    }
+    PSA_CHECK(hipMemcpyAsync(p->draw, raw, rb, hipMemcpyHostToDevice, p->st));
+    uint8_t *o[8];
+    for (int k = 0; k < p->nch; k++) o[k] = p->dout + (size_t)k * N;
+    raw_launch(p, p->draw, N, ix, o);
+    for (int k = 0; k < p->nch; k++)
+        PSA_CHECK(hipMemcpyAsync(out[k], o[k], N, hipMemcpyDeviceToHost, p->st));
+    PSA_CHECK(hipStreamSynchronize(p->st));
+    PSA_CHECK(hipEventElapsedTime(&p->last_ms, p->e0, p->e1));
+    return 1;
+}
+
+int sdr_raw_unpack_dev(sdr_raw_plan_t *p, const uint8_t *raw, long long N, int64_t ix, sdr_cpx8_t *const *out)
+{
+    if (!p || !raw || !out || N < 0) return 0;
+    if (((uintptr_t)raw & 15) != 0) {
+        fprintf(stderr, "pocketsdr_amd: sdr_raw_unpack_dev: raw data must be 16-byte aligned\n");
+        return 0;
+    }
+    for (int k = 0; k < p->nch; k++) {
+        if (!out[k] || ((uintptr_t)out[k] & 15) != 0) {
+            fprintf(stderr, "pocketsdr_amd: sdr_raw_unpack_dev: output %d must be 16-byte aligned\n", k);
+            return 0;
+        }
+    }
+    if (N == 0) return 1;
+    raw_launch(p, raw, N, ix, out);
+    PSA_CHECK(hipStreamSynchronize(p->st));
+    PSA_CHECK(hipEventElapsedTime(&p->last_ms, p->e0, p->e1));
+    return 1;
+}
+
+double sdr_raw_plan_kernel_ms(const sdr_raw_plan_t *p) { return p ? p->last_ms : 0.0; }
+
+void sdr_raw_plan_free(sdr_raw_plan_t *p)
+{
+    if (!p) return;
+    if (p->st) (void)hipStreamSynchronize(p->st);
+    (void)hipFree(p->lutc); (void)hipFree(p->l8); (void)hipFree(p->lq);
+    (void)hipFree(p->draw); (void)hipFree(p->dout);
+    if (p->e0) (void)hipEventDestroy(p->e0);
+    if (p->e1) (void)hipEventDestroy(p->e1);
+    if (p->st) (void)hipStreamDestroy(p->st);
+    delete p;
+}
+
+}  // extern "C"

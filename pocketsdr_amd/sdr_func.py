@@ -86,6 +86,14 @@ _decl("sdr_trk_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_trk_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                           C.POINTER(C.c_longlong), c_double_p])
 _decl("sdr_trk_plan_free", None, [c_void_p])
+_decl("sdr_raw_gen_lut", None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_void_p])
+_decl("sdr_raw_gen_lut8", None, [C.c_int, C.c_double, C.c_double, c_void_p])
+_decl("sdr_raw_plan_new", c_void_p, [C.c_int, c_int_p, C.POINTER(c_void_p)])
+_decl("sdr_raw_plan_nch", C.c_int, [c_void_p])
+_decl("sdr_raw_unpack", C.c_int, [c_void_p, c_void_p, C.c_int, C.c_int64, C.POINTER(c_void_p)])
+_decl("sdr_raw_unpack_dev", C.c_int, [c_void_p, c_void_p, C.c_longlong, C.c_int64, C.POINTER(c_void_p)])
+_decl("sdr_raw_plan_kernel_ms", C.c_double, [c_void_p])
+_decl("sdr_raw_plan_free", None, [c_void_p])
 _decl("sdr_amd_version", C.c_char_p, [])
 
 
@@ -354,3 +362,65 @@ class TrkPlan:
 
     def __del__(self):
         self.close()
+
+
+# --- front end: raw data to cpx8 (write_buff, src/sdr_rcv.c:776-936) ----------
+FMT_INT8, FMT_INT8X2, FMT_RAW8, FMT_RAW16, FMT_RAW16I, FMT_RAW32, FMT_CS8, FMT_CS16 = range(1, 9)
+RAW_BPS = {FMT_INT8: 1, FMT_INT8X2: 2, FMT_CS8: 2, FMT_CS16: 4, FMT_RAW8: 1, FMT_RAW16: 2,
+           FMT_RAW16I: 2, FMT_RAW32: 4}
+
+
+def raw_gen_lut(fmt, rfch, IQ, bits, rtoc=False):
+    """gen_LUTC / gen_LUTC_rtoc: uint8 cpx8 table (1024 entries with rtoc)."""
+    out = np.zeros(1024 if rtoc else 256, np.uint8)
+    lib.sdr_raw_gen_lut(fmt, rfch, IQ, bits, 1 if rtoc else 0, _ptr(out))
+    return out
+
+
+def raw_gen_lut8(fmt, off, scale):
+    """gen_LUT8: int8 table (65536 entries for CS16)."""
+    out = np.zeros(65536 if fmt == FMT_CS16 else 256, np.int8)
+    lib.sdr_raw_gen_lut8(fmt, off, scale, _ptr(out))
+    return out
+
+
+class RawPlan:
+    """sdr_raw_plan_*: write_buff for one format and its channel tables."""
+
+    def __init__(self, fmt, luts, rtoc=None):
+        self._luts = [None if l is None else np.ascontiguousarray(l) for l in luts]
+        lp = (c_void_p * 8)(*[None if l is None else l.ctypes.data for l in self._luts[:8]])
+        rt = (C.c_int * 8)(*((list(rtoc) if rtoc else []) + [0] * 8)[:8])
+        self.p = lib.sdr_raw_plan_new(fmt, rt, lp)
+        if not self.p:
+            raise ValueError("sdr_raw_plan_new failed")
+        self.fmt = fmt
+        self.nch = lib.sdr_raw_plan_nch(self.p)
+
+    def unpack(self, raw, N, ix=0):
+        """Host arrays: (nch, N) uint8 cpx8 samples of block ix."""
+        raw = np.ascontiguousarray(raw, np.uint8)
+        assert raw.size >= N * RAW_BPS[self.fmt]
+        out = np.zeros((self.nch, N), np.uint8)
+        op = (c_void_p * 8)(*[out[k].ctypes.data for k in range(self.nch)])
+        if not lib.sdr_raw_unpack(self.p, _ptr(raw), N, ix, op):
+            raise RuntimeError("sdr_raw_unpack failed")
+        return out
+
+    def unpack_dev(self, raw_ptr, N, ix, out_ptrs):
+        """Device pointers (16-byte aligned): raw bytes and one output per channel."""
+        op = (c_void_p * 8)(*list(out_ptrs)[:self.nch])
+        if not lib.sdr_raw_unpack_dev(self.p, c_void_p(raw_ptr), N, ix, op):
+            raise RuntimeError("sdr_raw_unpack_dev failed")
+
+    def kernel_ms(self):
+        return lib.sdr_raw_plan_kernel_ms(self.p)
+
+    def close(self):
+        if self.p:
+            lib.sdr_raw_plan_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
+
