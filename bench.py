@@ -320,8 +320,9 @@ def traffic_of(inst, grids, profile="search_multi_traffic.md"):
 def bench_raw(args, sdr, orc):
     """Front end (write_buff): one second of the 8-channel L1 array's raw
     stream (Pocket SDR FE 8CH, RAW32, 2-bit I/Q, 48 Msps) unpacked to the 8
-    cpx8 IF buffers, inputs and outputs resident in HBM.  Algorithmic bytes:
-    4 raw + 8 cpx8 bytes per sample.  CPU: the oracle's write_buff on the
+    cpx8 IF buffers, then combined into 4 array beams (sdr_arch_combine),
+    inputs and outputs resident in HBM.  Algorithmic bytes per sample: unpack
+    4 raw + 8 cpx8, combine 8 + 4 cpx8.  CPU: the oracle's write_buff on the
     first 1/10 s, one thread."""
     import torch
     fmt, N, fs = sdr.FMT_RAW32, 48_000_000, 48e6
@@ -343,6 +344,17 @@ def bench_raw(args, sdr, orc):
                msps_per_ch=round(N / (kms * 1e-3) / 1e6, 1), alg_bytes=12 * N,
                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                              frac=round(gbs / HBM_PEAK_GBS, 4)))
+    # array channels: 4 beams combined from the 8 RF channels
+    w = np.random.default_rng(args.seed).integers(-300, 300, (4, 8, 2)).astype(np.int16)
+    beams = torch.empty((4, N), dtype=torch.uint8, device="cuda")
+    bp = [beams[m].data_ptr() for m in range(4)]
+    cms = sorted(sdr.arch_combine_dev(ptrs, N, w, bp) for _ in range(max(3, args.steps)))
+    cms = cms[len(cms) // 2]
+    cgbs = 12.0 * N / (cms * 1e-3) / 1e9
+    res["arch_combine"] = dict(rf_channels=8, beams=4, samples=N, kernel_ms=round(cms, 4),
+                               alg_bytes=12 * N,
+                               roofline=dict(bound="hbm", achieved=round(cgbs, 1), peak=HBM_PEAK_GBS,
+                                             unit="GB/s", frac=round(cgbs / HBM_PEAK_GBS, 4)))
     if orc is not None:
         n = N // 10
         raw = draw[:n * 4].cpu().numpy()
@@ -470,7 +482,9 @@ def main():
     if not args.skip_trk:
         res.update(bench_trk(args, sdr, workloads))
     if not args.skip_raw:
-        res["raw_unpack"] = bench_raw(args, sdr, orc)
+        r = bench_raw(args, sdr, orc)
+        res["arch_combine"] = r.pop("arch_combine")
+        res["raw_unpack"] = r
     line = dict(metric=METRIC, value=round(value, 1), unit="cells/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=round(t_step * 1e3, 4), higher_is_better=True,
                 scaling="weak", vs_baseline=None, dtype=FORM_DTYPE[args.form],

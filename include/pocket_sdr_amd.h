@@ -255,6 +255,17 @@ int sdr_raw_unpack_dev(sdr_raw_plan_t *plan, const uint8_t *raw, long long N,
 double sdr_raw_plan_kernel_ms(const sdr_raw_plan_t *plan);
 void sdr_raw_plan_free(sdr_raw_plan_t *plan);
 
+/* Array channels: sdr_arch_combine (src/sdr_array.c:553-579) for narch beams
+ * at once.  in[j]: cpx8 samples of RF channel j (nrfch 2-8); w: Q8 weights
+ * [beam][channel][re, im] as sdr_arch_t.w (src/pocket_sdr.h:297) holds them;
+ * out[m]: cpx8 samples of beam m (narch 1-8).  Host pointers (returns 1) or
+ * device pointers, 16-byte aligned (returns the kernel time in ms, < 0 on
+ * error). */
+int sdr_arch_combine(int nrfch, const uint8_t *const *in, int N, int narch,
+    const int16_t *w, uint8_t *const *out);
+double sdr_arch_combine_dev(int nrfch, const uint8_t *const *in, long long N,
+    int narch, const int16_t *w, uint8_t *const *out);
+
 /* library version string */
 const char *sdr_amd_version(void);
 

@@ -62,6 +62,7 @@ _d("orc_gen_lutc", None, [C.c_int, C.c_int, C.c_int, C.c_int, vp])
 _d("orc_gen_lutc_rtoc", None, [C.c_int, vp])
 _d("orc_gen_lut8", None, [C.c_int, C.c_double, C.c_double, vp])
 _d("orc_write_buff", C.c_int, [C.c_int, vp, C.c_int, C.c_longlong, ip, C.POINTER(vp), C.POINTER(vp)])
+_d("orc_arch_combine", None, [C.c_int, C.POINTER(vp), C.c_int, vp, vp])
 
 lib.orc_func_init()
 REF_LOADED = bool(os.path.exists(REF_FFT) and lib.orc_load_ref_fft(REF_FFT.encode()))
@@ -255,4 +256,15 @@ def write_buff(fmt, raw, N, ix, rtoc, luts):
     n = lib.orc_write_buff(fmt, _p(raw), N, ix, rt, lp, op)
     assert n == nch
     return out[:nch]
+
+
+def arch_combine(ins, w):
+    """ins: (nrfch, N) uint8 cpx8; w: (nrfch, 2) int16 Q8 weights -> (N,) cpx8."""
+    ins = np.ascontiguousarray(ins, np.uint8)
+    w = np.ascontiguousarray(w, np.int16)
+    nrfch, N = ins.shape
+    out = np.zeros(N, np.uint8)
+    ip_ = (vp * nrfch)(*[ins[j].ctypes.data for j in range(nrfch)])
+    lib.orc_arch_combine(nrfch, ip_, N, _p(w), _p(out))
+    return out
 

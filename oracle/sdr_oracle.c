@@ -695,3 +695,30 @@ int orc_write_buff(int fmt, const uint8_t *raw, int N, long long ix, const int *
 #undef WR
     return 0;
 }
+
+/* src/sdr_array.c:486-498 (build_lut) and 553-579 (sdr_arch_combine) */
+void orc_arch_combine(int nrfch, const uint8_t *const *in, int N, const int16_t *w, uint8_t *out)
+{
+    int32_t lut[8][256][2];
+    for (int j = 0; j < nrfch; j++) {
+        int32_t wr = w[j * 2], wi = w[j * 2 + 1];
+        for (int b = 0; b < 256; b++) {
+            int32_t I = cpx8_I((uint8_t)b), Q = cpx8_Q((uint8_t)b);
+            lut[j][b][0] = I * wr - Q * wi;
+            lut[j][b][1] = I * wi + Q * wr;
+        }
+    }
+    for (int i = 0; i < N; i++) {
+        int32_t I = 0, Q = 0;
+        for (int j = 0; j < nrfch; j++) {
+            I += lut[j][in[j][i]][0];
+            Q += lut[j][in[j][i]][1];
+        }
+        I += (I >= 0) ? 128 : -128;          /* ARRAY_W_SCALE / 2, src/sdr_array.c:27 */
+        Q += (Q >= 0) ? 128 : -128;
+        I /= 256;
+        Q /= 256;
+        out[i] = mk_cpx8(I < -8 ? -8 : (I > 7 ? 7 : I), Q < -8 ? -8 : (Q > 7 ? 7 : Q));
+    }
+}
+

@@ -114,6 +114,9 @@ void orc_gen_lut8(int fmt, double off, double scale, int8_t *lut);
  * number of channels written. */
 int orc_write_buff(int fmt, const uint8_t *raw, int N, long long ix, const int *rtoc,
                    const void *const *lut, uint8_t *const *out);
+/* sdr_arch_combine, src/sdr_array.c:553-579, with build_lut
+ * (src/sdr_array.c:486-498): one beam, w = Q8 weights [channel][re, im] */
+void orc_arch_combine(int nrfch, const uint8_t *const *in, int N, const int16_t *w, uint8_t *out);
 
 #ifdef __cplusplus
 }

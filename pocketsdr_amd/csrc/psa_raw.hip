@@ -3,8 +3,9 @@
 // (src/sdr_rcv.c:776-831) for the Pocket SDR / Spider SDR packed raw formats
 // (RAW8 2 channels, RAW16 4, RAW16I 8, RAW32 8; 2-bit I/Q, 2- or 3-bit I and
 // the fs/4 real-to-complex phases) and the int8 / int16 formats (INT8,
-// INT8X2, CS8, CS16 through the AGC table).  The optional per-channel LPF and
-// the array combine that follow write_buff in the receiver are not part of it.
+// INT8X2, CS8, CS16 through the AGC table), and the array combine that
+// follows it in the receiver (below).  The optional per-channel LPF is not
+// included.
 //
 // Each thread converts 16 consecutive samples of every channel: one vector
 // load of 16 x (bytes per sample), table lookups in LDS (the int16 tables of
@@ -328,6 +329,171 @@ void sdr_raw_plan_free(sdr_raw_plan_t *p)
     if (p->e1) (void)hipEventDestroy(p->e1);
     if (p->st) (void)hipStreamDestroy(p->st);
     delete p;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// array combine: sdr_arch_combine (src/sdr_array.c:553-579) with its weight
+// table (build_lut, src/sdr_array.c:486-498) folded into the arithmetic: the
+// table entry of channel j for cpx8 b is (I wr - Q wi, I wi + Q wr) with (I, Q)
+// the sign-extended nibbles of b and (wr, wi) the channel's Q8 weights, so the
+// products are formed directly (exact integers).  All beams of one call read
+// each input sample once.
+// ---------------------------------------------------------------------------
+namespace psa {
+
+struct ArchArgs {
+    const uint8_t *in[8];
+    long long N;
+    int nrfch, narch;
+    int16_t w[8][8][2];         // [beam][channel][re, im]
+    uint8_t *out[8];
+};
+
+// one beam per grid row (blockIdx.y; its weights are wave-uniform), NRF RF
+// channels unrolled; each thread combines 16 consecutive samples
+template <int NRF>
+__global__ void __launch_bounds__(256) arch_combine_kernel(ArchArgs a)
+{
+    const int m = blockIdx.y;
+    int wr[NRF], wi[NRF];
+#pragma unroll
+    for (int j = 0; j < NRF; j++) {
+        wr[j] = __builtin_amdgcn_readfirstlane((int)a.w[m][j][0]);
+        wi[j] = __builtin_amdgcn_readfirstlane((int)a.w[m][j][1]);
+    }
+    uint8_t *out = a.out[m];
+    const long long stride = (long long)gridDim.x * 256 * 16;
+    for (long long s0 = ((long long)blockIdx.x * 256 + threadIdx.x) * 16; s0 < a.N; s0 += stride) {
+        const int nq = (int)min(16LL, a.N - s0);
+        uint32_t x[NRF][4];
+#pragma unroll
+        for (int j = 0; j < NRF; j++) {
+            if (nq == 16) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(a.in[j] + s0);
+                x[j][0] = v.x; x[j][1] = v.y; x[j][2] = v.z; x[j][3] = v.w;
+            } else {
+                x[j][0] = x[j][1] = x[j][2] = x[j][3] = 0;
+                for (int q = 0; q < nq; q++) x[j][q >> 2] |= (uint32_t)a.in[j][s0 + q] << (8 * (q & 3));
+            }
+        }
+        uint32_t o[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            int32_t I = 0, Q = 0;
+#pragma unroll
+            for (int j = 0; j < NRF; j++) {
+                // sign-extended nibbles of the cpx8 byte (bit-field extracts)
+                const int bi = __builtin_amdgcn_sbfe((int)x[j][q >> 2], 8 * (q & 3), 4);
+                const int bq = __builtin_amdgcn_sbfe((int)x[j][q >> 2], 8 * (q & 3) + 4, 4);
+                I += bi * wr[j] - bq * wi[j];
+                Q += bi * wi[j] + bq * wr[j];
+            }
+            // round half away from zero, divide by the Q8 scale (C division
+            // truncates), clip to the cpx8 range
+            I = (I + (I >= 0 ? 128 : -128)) / 256;
+            Q = (Q + (Q >= 0 ? 128 : -128)) / 256;
+            I = I < -8 ? -8 : (I > 7 ? 7 : I);
+            Q = Q < -8 ? -8 : (Q > 7 ? 7 : Q);
+            o[q >> 2] |= (uint32_t)mk_cpx8(I, Q) << (8 * (q & 3));
+        }
+        if (nq == 16) {
+            *reinterpret_cast<uint4 *>(out + s0) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+            for (int q = 0; q < nq; q++) out[s0 + q] = (uint8_t)(o[q >> 2] >> (8 * (q & 3)));
+        }
+    }
+}
+
+static float arch_launch(int nrfch, const uint8_t *const *in, long long N, int narch, const int16_t *w,
+                         uint8_t *const *out, hipStream_t st)
+{
+    ArchArgs a = {};
+    for (int j = 0; j < nrfch; j++) a.in[j] = in[j];
+    for (int m = 0; m < narch; m++) {
+        a.out[m] = out[m];
+        for (int j = 0; j < nrfch; j++) {
+            a.w[m][j][0] = w[(m * nrfch + j) * 2];
+            a.w[m][j][1] = w[(m * nrfch + j) * 2 + 1];
+        }
+    }
+    a.N = N; a.nrfch = nrfch; a.narch = narch;
+    const long long groups = (N + 15) / 16;
+    const int blocks = (int)std::max(1LL, std::min((groups + 255) / 256, 4096LL));
+    hipEvent_t e0, e1;
+    PSA_CHECK(hipEventCreate(&e0));
+    PSA_CHECK(hipEventCreate(&e1));
+    PSA_CHECK(hipEventRecord(e0, st));
+    const dim3 grid(blocks, narch);
+    switch (nrfch) {
+#define PSA_ARCH(R) case R: hipLaunchKernelGGL(arch_combine_kernel<R>, grid, dim3(256), 0, st, a); break;
+    PSA_ARCH(2) PSA_ARCH(3) PSA_ARCH(4) PSA_ARCH(5) PSA_ARCH(6) PSA_ARCH(7) PSA_ARCH(8)
+#undef PSA_ARCH
+    }
+    PSA_CHECK(hipGetLastError());
+    PSA_CHECK(hipEventRecord(e1, st));
+    PSA_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    PSA_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    PSA_CHECK(hipEventDestroy(e0));
+    PSA_CHECK(hipEventDestroy(e1));
+    return ms;
+}
+
+static bool arch_args_ok(int nrfch, int narch, const void *in, const int16_t *w, const void *out)
+{
+    if (nrfch < 2 || nrfch > 8 || narch < 1 || narch > 8 || !in || !w || !out) {
+        fprintf(stderr, "pocketsdr_amd: sdr_arch_combine: 2-8 RF channels and 1-8 beams\n");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace psa
+
+extern "C" {
+
+double sdr_arch_combine_dev(int nrfch, const uint8_t *const *in, long long N, int narch, const int16_t *w,
+                            uint8_t *const *out)
+{
+    if (!arch_args_ok(nrfch, narch, in, w, out) || N < 0) return -1.0;
+    for (int j = 0; j < nrfch; j++) {
+        if (!in[j] || ((uintptr_t)in[j] & 15)) {
+            fprintf(stderr, "pocketsdr_amd: sdr_arch_combine_dev: input %d must be 16-byte aligned\n", j);
+            return -1.0;
+        }
+    }
+    for (int m = 0; m < narch; m++) {
+        if (!out[m] || ((uintptr_t)out[m] & 15)) {
+            fprintf(stderr, "pocketsdr_amd: sdr_arch_combine_dev: output %d must be 16-byte aligned\n", m);
+            return -1.0;
+        }
+    }
+    if (N == 0) return 0.0;
+    ensure_init();
+    return arch_launch(nrfch, in, N, narch, w, out, tctx().stream);
+}
+
+int sdr_arch_combine(int nrfch, const uint8_t *const *in, int N, int narch, const int16_t *w, uint8_t *const *out)
+{
+    if (!arch_args_ok(nrfch, narch, in, w, out) || N < 0) return 0;
+    if (N == 0) return 1;
+    ensure_init();
+    ThreadCtx &c = tctx();
+    uint8_t *d = static_cast<uint8_t *>(c.s[SC_MISC].get((size_t)N * (nrfch + narch) + 16 * 16));
+    const uint8_t *di[8];
+    uint8_t *dout[8];
+    size_t off = 0;
+    for (int j = 0; j < nrfch; j++, off += ((size_t)N + 15) / 16 * 16) {
+        di[j] = d + off;
+        PSA_CHECK(hipMemcpyAsync(d + off, in[j], N, hipMemcpyHostToDevice, c.stream));
+    }
+    for (int m = 0; m < narch; m++, off += ((size_t)N + 15) / 16 * 16) dout[m] = d + off;
+    arch_launch(nrfch, di, N, narch, w, dout, c.stream);
+    for (int m = 0; m < narch; m++) PSA_CHECK(hipMemcpyAsync(out[m], dout[m], N, hipMemcpyDeviceToHost, c.stream));
+    PSA_CHECK(hipStreamSynchronize(c.stream));
+    return 1;
 }
 
 }  // extern "C"

@@ -94,6 +94,10 @@ _decl("sdr_raw_unpack", C.c_int, [c_void_p, c_void_p, C.c_int, C.c_int64, C.POIN
 _decl("sdr_raw_unpack_dev", C.c_int, [c_void_p, c_void_p, C.c_longlong, C.c_int64, C.POINTER(c_void_p)])
 _decl("sdr_raw_plan_kernel_ms", C.c_double, [c_void_p])
 _decl("sdr_raw_plan_free", None, [c_void_p])
+_decl("sdr_arch_combine", C.c_int, [C.c_int, C.POINTER(c_void_p), C.c_int, C.c_int, c_void_p,
+                                     C.POINTER(c_void_p)])
+_decl("sdr_arch_combine_dev", C.c_double, [C.c_int, C.POINTER(c_void_p), C.c_longlong, C.c_int,
+                                            c_void_p, C.POINTER(c_void_p)])
 _decl("sdr_amd_version", C.c_char_p, [])
 
 
@@ -423,4 +427,30 @@ class RawPlan:
 
     def __del__(self):
         self.close()
+
+
+def arch_combine(ins, w):
+    """sdr_arch_combine for several beams: ins (nrfch, N) uint8 cpx8, w
+    (narch, nrfch, 2) int16 Q8 weights -> (narch, N) uint8 cpx8."""
+    ins = np.ascontiguousarray(ins, np.uint8)
+    w = np.ascontiguousarray(w, np.int16)
+    nrfch, N = ins.shape
+    narch = w.shape[0]
+    out = np.zeros((narch, N), np.uint8)
+    ip_ = (c_void_p * 8)(*[ins[j].ctypes.data for j in range(nrfch)])
+    op = (c_void_p * 8)(*[out[m].ctypes.data for m in range(narch)])
+    if not lib.sdr_arch_combine(nrfch, ip_, N, narch, _ptr(w), op):
+        raise RuntimeError("sdr_arch_combine failed")
+    return out
+
+
+def arch_combine_dev(in_ptrs, N, w, out_ptrs):
+    """Device pointers (16-byte aligned); returns the kernel time (ms)."""
+    w = np.ascontiguousarray(w, np.int16)
+    ip_ = (c_void_p * 8)(*list(in_ptrs))
+    op = (c_void_p * 8)(*list(out_ptrs))
+    ms = lib.sdr_arch_combine_dev(len(in_ptrs), ip_, N, len(out_ptrs), _ptr(w), op)
+    if ms < 0:
+        raise RuntimeError("sdr_arch_combine_dev failed")
+    return ms
 

@@ -10,7 +10,10 @@ the conjugate of CS8.  The library's table generators are host code and are
 compared with the oracle's on the CPU; the GPU unpack (marked gpu) is
 compared bit for bit with the oracle for every format, odd lengths, block
 indices and real-to-complex channels, through the host entry point and the
-device one.
+device one.  The array combine (sdr_arch_combine, src/sdr_array.c:553-579):
+the oracle keeps the reference's weight table and is pinned by identity and
+quadrature weights, equal-weight sums and the half-away-from-zero rounding;
+the GPU kernel forms the products directly and matches it bit for bit.
 """
 import numpy as np
 import pytest
@@ -208,3 +211,45 @@ def test_raw_unpack_dev_matches_host(sdr, orc):
     assert np.array_equal(dout.cpu().numpy(), orc.write_buff(fmt, raw, N, ix, rtoc, luts))
     assert plan.kernel_ms() > 0
     plan.close()
+
+
+# ---------------------------------------------------------------------------
+# array combine (sdr_arch_combine, src/sdr_array.c:553-579)
+# ---------------------------------------------------------------------------
+def cpx8(I, Q):
+    return np.uint8(((Q & 0xF) << 4) | (I & 0xF))
+
+
+def test_oracle_arch_combine(orc):
+    vals = np.array([cpx8(i, q) for i in range(-8, 8) for q in range(-8, 8)], np.uint8)
+    I, Q = dec(vals)
+    zero = np.zeros_like(vals)
+    # identity on channel 0 (Q8 weight 256), channel 1 weighted 0
+    assert np.array_equal(orc.arch_combine(np.stack([vals, zero]), [[256, 0], [0, 0]]), vals)
+    # weight j: (I, Q) -> (-Q, I), -(-8) clipped to 7
+    oI, oQ = dec(orc.arch_combine(np.stack([vals, zero]), [[0, 256], [0, 0]]))
+    assert np.array_equal(oI, np.minimum(-Q, 7)) and np.array_equal(oQ, I)
+    # two equal channels at half weight sum back to the input
+    assert np.array_equal(orc.arch_combine(np.stack([vals, vals]), [[128, 0], [128, 0]]), vals)
+    # half weight on one channel: x / 2 rounded half away from zero (C
+    # division after adding +-128): 1 -> 1, -1 -> -1, 3 -> 2, -3 -> -2, 2 -> 1
+    x = np.array([cpx8(v, 0) for v in (1, -1, 3, -3, 2, 0)], np.uint8)
+    oI, _ = dec(orc.arch_combine(np.stack([x, np.zeros_like(x)]), [[128, 0], [0, 0]]))
+    assert oI.tolist() == [1, -1, 2, -2, 1, 0]
+
+
+@pytest.mark.gpu
+def test_arch_combine_bitexact(sdr, orc):
+    import torch
+    rng = np.random.default_rng(3)
+    for nrfch, narch, N in ((2, 1, 1), (4, 3, 999), (8, 8, 4096 + 7), (8, 4, 1 << 18)):
+        ins = rng.integers(0, 256, (nrfch, N), dtype=np.uint8)
+        w = rng.integers(-400, 400, (narch, nrfch, 2)).astype(np.int16)
+        want = np.stack([orc.arch_combine(ins, w[m]) for m in range(narch)])
+        assert np.array_equal(sdr.arch_combine(ins, w), want), (nrfch, narch, N)
+        if N % 16 == 0:
+            din = torch.from_numpy(ins).cuda()
+            dout = torch.zeros((narch, N), dtype=torch.uint8, device="cuda")
+            ms = sdr.arch_combine_dev([din[j].data_ptr() for j in range(nrfch)], N, w,
+                                      [dout[m].data_ptr() for m in range(narch)])
+            assert ms > 0 and np.array_equal(dout.cpu().numpy(), want)
