@@ -156,6 +156,12 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     int n_sig, const sdr_search_sig_t *sigs, int flags);
 /* Runs the whole search on the device (P surfaces are zeroed first), then
  * the peak search.  Blocks until done.  Returns 1 on success. */
+/* The same on IF buffers in device memory (buffN bytes each, e.g. written
+ * by sdr_raw_unpack_dev / sdr_arch_combine_dev): read in place at every
+ * run, so new samples written there between runs are searched by the next
+ * run; the caller keeps them allocated while the plan lives. */
+sdr_search_plan_t *sdr_search_plan_new_dev(int n_buff, const uint8_t *const *dbuf,
+    const int *buffN, int n_sig, const sdr_search_sig_t *sigs, int flags);
 int sdr_search_plan_run(sdr_search_plan_t *plan);
 /* Per-PRN results of signal group `sig` as sdr_corr_max / sdr_fine_dop
  * produce them: cn0[n_prn], ix[2 n_prn] (bin, lag), dop[n_prn]. */

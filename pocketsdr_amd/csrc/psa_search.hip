@@ -492,6 +492,7 @@ static const char *k_names[K_NCLS] = {"fwd_col", "fwd_row", "inv_row", "inv_col"
 struct sdr_search_plan_tag {
     int flags = 0;
     std::vector<uint8_t *> d_buf;
+    bool own_buf = true;        // false: IF buffers in the caller's device memory
     std::vector<int> buffN;
     std::vector<psa::SigDev> sig;
     float2 *tmp = nullptr, *X = nullptr, *Z = nullptr;
@@ -904,26 +905,32 @@ static void run_all(sdr_search_plan_t *p)
 
 }  // namespace psa
 
-extern "C" {
-
-sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buffs,
-    int n_sig, const sdr_search_sig_t *sigs, int flags)
+// IF buffers: host data (copied to the plan's own device memory) or device
+// data (read in place at every run; not owned)
+static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, const int *bN, bool dev,
+                                      int n_sig, const sdr_search_sig_t *sigs, int flags)
 {
-    if (n_buff <= 0 || !buffs || n_sig <= 0 || !sigs) {
+    if (n_buff <= 0 || !data || !bN || n_sig <= 0 || !sigs) {
         fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: invalid arguments\n");
         return nullptr;
+    }
+    for (int i = 0; i < n_buff; i++) {
+        if (!data[i] || bN[i] <= 0) {
+            fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: invalid buffer %d\n", i);
+            return nullptr;
+        }
     }
     for (int i = 0; i < n_sig; i++) {
         const sdr_search_sig_t &g = sigs[i];
         if (g.buff < 0 || g.buff >= n_buff || g.M <= 0 || g.N <= 0 || g.N > g.M || g.n_prn <= 0 ||
             !g.code_fft || g.len_fds <= 0 || !g.fds || g.n_blk <= 0 || !fft_supported(g.M) ||
-            g.M > buffs[g.buff]->N) {
+            g.M > bN[g.buff]) {
             fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: invalid signal group %d\n", i);
             return nullptr;
         }
         for (int k = 0; k < g.n_blk; k++) {
             long long ix = (long long)g.ix0 + (long long)k * g.blk_step;
-            if (ix < 0 || ix >= buffs[g.buff]->N) {
+            if (ix < 0 || ix >= bN[g.buff]) {
                 fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: block %d of group %d outside the buffer\n", k, i);
                 return nullptr;
             }
@@ -932,13 +939,16 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
     ensure_init();
     sdr_search_plan_t *p = new sdr_search_plan_t();
     p->flags = flags;
+    p->own_buf = !dev;
     PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
     for (int i = 0; i < n_buff; i++) {
-        uint8_t *d;
-        PSA_CHECK(hipMalloc(&d, buffs[i]->N));
-        PSA_CHECK(hipMemcpy(d, buffs[i]->data, buffs[i]->N, hipMemcpyHostToDevice));
+        uint8_t *d = const_cast<uint8_t *>(data[i]);
+        if (!dev) {
+            PSA_CHECK(hipMalloc(&d, bN[i]));
+            PSA_CHECK(hipMemcpy(d, data[i], bN[i], hipMemcpyHostToDevice));
+        }
         p->d_buf.push_back(d);
-        p->buffN.push_back(buffs[i]->N);
+        p->buffN.push_back(bN[i]);
     }
     // bytes per intermediate array (the inverse passes' Z is the largest);
     // PSA_CHUNK_MB overrides
@@ -1040,6 +1050,31 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
         s.cf_nat = nullptr;
     }
     return p;
+}
+
+extern "C" {
+
+sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buffs,
+    int n_sig, const sdr_search_sig_t *sigs, int flags)
+{
+    if (n_buff <= 0 || !buffs) {
+        fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new: invalid arguments\n");
+        return nullptr;
+    }
+    std::vector<const uint8_t *> data(n_buff);
+    std::vector<int> bN(n_buff);
+    for (int i = 0; i < n_buff; i++) {
+        if (!buffs[i]) return plan_create(0, nullptr, nullptr, false, n_sig, sigs, flags);
+        data[i] = buffs[i]->data;
+        bN[i] = buffs[i]->N;
+    }
+    return plan_create(n_buff, data.data(), bN.data(), false, n_sig, sigs, flags);
+}
+
+sdr_search_plan_t *sdr_search_plan_new_dev(int n_buff, const uint8_t *const *dbuf, const int *buffN,
+    int n_sig, const sdr_search_sig_t *sigs, int flags)
+{
+    return plan_create(n_buff, dbuf, buffN, true, n_sig, sigs, flags);
 }
 
 int sdr_search_plan_run(sdr_search_plan_t *p)
@@ -1186,7 +1221,9 @@ void sdr_search_plan_free(sdr_search_plan_t *p)
     if (!p) return;
     (void)hipStreamSynchronize(p->st);
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-    for (auto d : p->d_buf) (void)hipFree(d);
+    if (p->own_buf) {
+        for (auto d : p->d_buf) (void)hipFree(d);
+    }
     for (auto &s : p->sig) {
         (void)hipFree(s.cft); (void)hipFree(s.cfn); (void)hipFree(s.P); (void)hipFree(s.fds);
         (void)hipFree(s.mix_ix); (void)hipFree(s.mix_nseg); (void)hipFree(s.mix_p0);

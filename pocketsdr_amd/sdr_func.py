@@ -67,6 +67,8 @@ _decl("sdr_mix_carr", None, [C.POINTER(sdr_buff_t), C.c_int, C.c_int, C.c_double
                              C.c_double, c_void_p])
 _decl("sdr_search_plan_new", c_void_p, [C.c_int, C.POINTER(C.POINTER(sdr_buff_t)), C.c_int,
                                         C.POINTER(sdr_search_sig_t), C.c_int])
+_decl("sdr_search_plan_new_dev", c_void_p, [C.c_int, C.POINTER(c_void_p), c_int_p, C.c_int,
+                                             C.POINTER(sdr_search_sig_t), C.c_int])
 _decl("sdr_search_plan_run", C.c_int, [c_void_p])
 _decl("sdr_search_plan_result", C.c_int, [c_void_p, C.c_int, c_float_p, c_int_p, c_double_p])
 _decl("sdr_search_plan_power", C.c_int, [c_void_p, C.c_int, C.c_int, c_float_p])
@@ -237,14 +239,16 @@ FORM_FP32, FORM_PLAIN, FORM_FP16, FORM_BF16 = 0, 1, 2, 3
 class SearchPlan:
     """Batched HBM-resident acquisition over signal groups (sdr_search_plan_*).
 
-    buffs: list of uint8 cpx8 IF streams.  sigs: list of dicts with keys
-    buff, M, N, T, fs, fi, code_fft (n_prn, M) complex64, fds float32, n_blk,
-    ix0, blk_step.
+    buffs: list of uint8 cpx8 IF streams, or with dev_buffs a list of
+    (device pointer, length) pairs read in place (sdr_search_plan_new_dev).
+    sigs: list of dicts with keys buff, M, N, T, fs, fi, code_fft (n_prn, M)
+    complex64, fds float32, n_blk, ix0, blk_step.
     """
 
-    def __init__(self, buffs, sigs, flags=SEARCH_BATCHED):
-        self._bufs = [_buff(b) for b in buffs]
-        arr = (C.POINTER(sdr_buff_t) * len(buffs))(*[C.pointer(b) for b in self._bufs])
+    def __init__(self, buffs, sigs, flags=SEARCH_BATCHED, dev_buffs=False):
+        if not dev_buffs:
+            self._bufs = [_buff(b) for b in buffs]
+            arr = (C.POINTER(sdr_buff_t) * len(buffs))(*[C.pointer(b) for b in self._bufs])
         self._keep = []
         sa = (sdr_search_sig_t * len(sigs))()
         for i, s in enumerate(sigs):
@@ -255,7 +259,12 @@ class SearchPlan:
                                      cf.shape[0], _ptr(cf), len(fds), _fp(fds), s["n_blk"],
                                      s["ix0"], s["blk_step"])
         self.sigs = sigs
-        self.p = lib.sdr_search_plan_new(len(buffs), arr, len(sigs), sa, flags)
+        if dev_buffs:
+            dp_ = (c_void_p * len(buffs))(*[b[0] for b in buffs])
+            nb = (C.c_int * len(buffs))(*[b[1] for b in buffs])
+            self.p = lib.sdr_search_plan_new_dev(len(buffs), dp_, nb, len(sigs), sa, flags)
+        else:
+            self.p = lib.sdr_search_plan_new(len(buffs), arr, len(sigs), sa, flags)
         if not self.p:
             raise ValueError("sdr_search_plan_new failed")
 

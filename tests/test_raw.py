@@ -253,3 +253,55 @@ def test_arch_combine_bitexact(sdr, orc):
             ms = sdr.arch_combine_dev([din[j].data_ptr() for j in range(nrfch)], N, w,
                                       [dout[m].data_ptr() for m in range(narch)])
             assert ms > 0 and np.array_equal(dout.cpu().numpy(), want)
+
+
+# ---------------------------------------------------------------------------
+# front end feeding acquisition in device memory
+# ---------------------------------------------------------------------------
+def encode_raw8_ch0(buff):
+    """cpx8 samples with 2-bit components (+-1, +-3) -> RAW8 bytes whose low
+    nibble carries them in the front end's I/Q code (I: sign bit 1, magnitude
+    bit 0; Q the same with the sign inverted)."""
+    I, Q = dec(buff)
+    ii = np.select([I == 1, I == 3, I == -1, I == -3], [0, 1, 2, 3])
+    qq = np.select([Q == -1, Q == -3, Q == 1, Q == 3], [0, 1, 2, 3])
+    return (ii | (qq << 2)).astype(np.uint8)
+
+
+@pytest.mark.gpu
+def test_raw_to_search_on_device(sdr, orc):
+    """RAW8 bytes -> sdr_raw_unpack_dev -> sdr_search_plan_new_dev on the
+    device buffer: the same results as the plan on the host cpx8 buffer, also
+    after new samples are written into the device buffer between runs."""
+    import torch
+    from pocketsdr_amd import workloads
+    w = workloads.acq_l1ca(seed=21, n_prn=6, n_vis=3, t_int=3e-3, max_dop=2000.0,
+                           code_fft_fn=lambda c, T, fs, N: orc.gen_code_fft(c, T, 0.0, fs, N, N))
+    s = w["sig"]
+    buff = np.asarray(w["buff"], np.uint8)
+    n = (len(buff) + 15) // 16 * 16
+    luts = [orc.gen_lutc(FMT_RAW8, k, 2, 2) for k in range(2)]
+    rp = sdr.RawPlan(FMT_RAW8, luts, [0, 0])
+    draw = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    dif = torch.zeros((2, n), dtype=torch.uint8, device="cuda")
+    raw = encode_raw8_ch0(buff)
+    draw[:len(raw)] = torch.from_numpy(raw).cuda()
+    rp.unpack_dev(draw.data_ptr(), len(raw), 0, [dif[0].data_ptr(), dif[1].data_ptr()])
+    assert np.array_equal(dif[0, :len(buff)].cpu().numpy(), buff)
+    pd = sdr.SearchPlan([(dif[0].data_ptr(), len(buff))], [s], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH,
+                        dev_buffs=True)
+    for data in (buff, np.roll(buff, 777)):
+        raw = encode_raw8_ch0(data)
+        draw[:len(raw)] = torch.from_numpy(raw).cuda()
+        rp.unpack_dev(draw.data_ptr(), len(raw), 0, [dif[0].data_ptr(), dif[1].data_ptr()])
+        pd.run()
+        ph = sdr.SearchPlan([data], [s], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH)
+        ph.run()
+        for q in range(s["code_fft"].shape[0]):
+            assert np.array_equal(pd.power(0, q), ph.power(0, q))
+        c1, i1, d1 = pd.result(0)
+        c2, i2, d2 = ph.result(0)
+        assert np.array_equal(i1, i2) and np.array_equal(c1, c2) and np.array_equal(d1, d2)
+        ph.close()
+    pd.close()
+    rp.close()
