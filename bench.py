@@ -382,6 +382,54 @@ def bench_trk(args, sdr, workloads):
                 correlator_kernel_ms=round(ks[0], 4) if ks else None)
 
 
+L6_CLASSES = ("state", "fwd_col", "fwd_row", "inv_row", "inv_col", "csk")
+
+
+def bench_l6(args, sdr, orc, workloads):
+    """QZSS L6D FFT tracking with CSK: n_ch channels at 24 Msps, 4 ms epochs
+    (N = M = 96000), one second of IF (250 epochs) per run, IF and code
+    spectra resident in HBM.  Algorithmic bytes per (channel, epoch) item:
+    fwd column N IF bytes + 8M, fwd row 16M, inv row 16M (X, out), inv
+    column 16M, CSK ~2 x 511 x 16 B; the code spectra (SDR_N_CODES per
+    channel, 8M each) count once per run.  CPU: the oracle's L6 tracking loop
+    (PocketFFT, one thread) over the epochs of one channel."""
+    w = workloads.trk_l6(seed=args.seed + 2, fs=24e6, n_ch=args.l6_channels, n_epoch=args.l6_epochs)
+    plan = sdr.L6Plan(w["buff"], w["chans"], w["pos"], w["n_epoch"])
+    t = time_plan(plan, max(1, args.steps // 2), 1)
+    syms, _ = plan.result()
+    ok, seen = True, set()
+    for c, ch in enumerate(w["chans"]):
+        if ch["prn"] in w["truth"] and ch["prn"] not in seen:
+            seen.add(ch["prn"])
+            ok &= syms[c].tolist() == w["truth"][ch["prn"]]["syms"][:w["n_epoch"]].tolist()
+    plan.timing(True)
+    plan.run()
+    ks = {k: plan.kernel_ms(k) for k in L6_CLASSES}
+    plan.close()
+    n_ch, N, ne = len(w["chans"]), w["chans"][0]["N"], w["n_epoch"]
+    items = n_ch * ne
+    per_item = 57.0 * N + 2 * 511 * 16
+    alg = items * per_item + n_ch * 10 * 8.0 * N
+    gbs = alg / t / 1e9
+    res = dict(config=dict(n_ch=n_ch, fs=24e6, N=N, n_epoch=ne, n_pos=len(w["pos"]),
+                           signal="L6D (20460-entry TDM code, CSK 8 bit / 4 ms)"),
+               ms_per_run=round(t * 1e3, 3), epochs_ch_per_s=round(items / t, 1),
+               msps_ch=round(items * N / t / 1e6, 1), realtime_x=round(ne * 4e-3 / t, 1),
+               symbols_decoded=bool(ok and len(seen) > 0),
+               kernel_ms={k: round(v[0], 4) for k, v in ks.items() if v},
+               alg_bytes_per_item=int(per_item), alg_bytes_spectra=n_ch * 10 * 8 * N,
+               roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=round(gbs / HBM_PEAK_GBS, 4)))
+    if orc is not None:
+        ch = w["chans"][0]
+        n_e = ne
+        t0 = time.perf_counter()
+        orc.trk_l6_run(ch, w["len_code"], ch["code_fft"], w["buff"], n_e, w["pos"])
+        dt = time.perf_counter() - t0
+        res["cpu_epochs_ch_per_s"] = round(n_e / dt, 1)
+        res["cpu_sample"] = "%d epochs of one channel, oracle L6 loop, one thread (%.2f s)" % (n_e, dt)
+    return res
+
 ONCHIP_BYTES = 672 * 1024      # one CU: 160 KiB LDS + 512 KiB VGPR/AGPR
 
 
@@ -448,6 +496,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-trk", action="store_true")
     ap.add_argument("--skip-raw", action="store_true")
+    ap.add_argument("--skip-l6", action="store_true")
+    ap.add_argument("--l6-channels", type=int, default=16)
+    ap.add_argument("--l6-epochs", type=int, default=250)
     ap.add_argument("--cpu-prns", type=int, default=4)
     ap.add_argument("--workload", default="multi", choices=["multi", "l1ca", "l1cp"])
     ap.add_argument("--cpu-bins", type=int, default=0,
@@ -485,6 +536,8 @@ def main():
         r = bench_raw(args, sdr, orc)
         res["arch_combine"] = r.pop("arch_combine")
         res["raw_unpack"] = r
+    if not args.skip_l6:
+        res["l6_tracking"] = bench_l6(args, sdr, orc, workloads)
     line = dict(metric=METRIC, value=round(value, 1), unit="cells/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=round(t_step * 1e3, 4), higher_is_better=True,
                 scaling="weak", vs_baseline=None, dtype=FORM_DTYPE[args.form],

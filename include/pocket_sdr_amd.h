@@ -227,6 +227,53 @@ int sdr_trk_plan_kernel_ms(const sdr_trk_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);
 void sdr_trk_plan_free(sdr_trk_plan_t *plan);
 
+/* --- L6D / L6E FFT tracking and CSK ----------------------------------------
+ * n_ch channels of one signal (one N and one code length) on one IF stream,
+ * n_epoch consecutive epochs each.  Every epoch advances the channel
+ * open-loop as the tracking plan does (src/sdr_ch.c:694-703), then follows
+ * the L6D / L6E branch of track_sig (src/sdr_ch.c:705-720): i = floor(coff
+ * fs), bank j = (int)((coff fs - i) SDR_N_CODES), sdr_mix_carr from ring
+ * index (e N) mod buffN + i at phase phi + fc i / fs (the start taken mod
+ * buffN), sdr_corr_fft against code_fft + j N, and CSK (src/sdr_ch.c:
+ * 636-664): the symbol 255 - ix % 256 of the peak shift ix and the n_pos
+ * correlator taps interpolated at n + ix R + pos[k].
+ */
+typedef struct {
+    double fs, fi, fc_sig, T;  /* sampling rate, IF, signal carrier, period */
+    int N;                     /* samples per epoch = correlator FFT length */
+    int len_code;              /* code length (20460 for L6D / L6E, TDM) */
+    double fd0, drift;         /* Doppler (Hz), Doppler rate (Hz/s) */
+    double phi0, coff0;        /* initial carrier phase (cyc), code offset (s) */
+    const sdr_cpx_t *code_fft; /* SDR_N_CODES x N conjugated code spectra, the
+                                  layout of trk->code_fft (src/sdr_ch.c:256-262) */
+} sdr_l6_ch_t;
+
+#define SDR_L6_KEEP_CORR 1     /* keep every item's correlation for _corr */
+
+typedef struct sdr_l6_plan_tag sdr_l6_plan_t;
+
+sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch,
+    const sdr_l6_ch_t *chs, int n_pos, const double *pos, int n_epoch,
+    int flags);
+int sdr_l6_plan_run(sdr_l6_plan_t *plan);
+/* syms: n_ch x n_epoch CSK symbols; C: n_ch x n_epoch x n_pos taps (the
+ * trk->C of src/sdr_ch.c:660-662). */
+int sdr_l6_plan_result(const sdr_l6_plan_t *plan, uint8_t *syms,
+    sdr_cpx_t *C);
+/* N-sample correlation of one (channel, epoch), ch->corr of
+ * src/sdr_ch.c:715 (plans made with SDR_L6_KEEP_CORR). */
+int sdr_l6_plan_corr(const sdr_l6_plan_t *plan, int ch, int epoch,
+    sdr_cpx_t *corr);
+void sdr_l6_plan_timing(sdr_l6_plan_t *plan, int on);
+/* kernel classes: "state", "fwd_col", "fwd_row", "inv_row", "inv_col", "csk" */
+int sdr_l6_plan_kernel_ms(const sdr_l6_plan_t *plan, const char *name,
+    double *avg_ms, long long *count, double *elems);
+void sdr_l6_plan_free(sdr_l6_plan_t *plan);
+/* CSK of n_item correlations of N samples each (src/sdr_ch.c:636-664, a
+ * static function there): syms[n_item], C[n_item x n_pos]. */
+int sdr_csk(const sdr_cpx_t *corr, int n_item, int N, int len_code,
+    const double *pos, int n_pos, uint8_t *syms, sdr_cpx_t *C);
+
 /* --- front end: raw data to cpx8 IF samples ---------------------------------
  * The receiver's write_buff (src/sdr_rcv.c:834-914, a static function there)
  * on the GPU, for the formats SDR_FMT_INT8 ... SDR_FMT_CS16

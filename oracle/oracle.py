@@ -58,6 +58,9 @@ _d("orc_gen_code_fft_f64", None, [vp, C.c_int, C.c_double, C.c_double, C.c_doubl
                                   C.c_int, vp])
 _d("orc_gen_code_L1CA", C.c_int, [C.c_int, vp])
 _d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, C.c_int, vp, vp, dp])
+_d("orc_csk", C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, vp])
+_d("orc_trk_l6_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, vp, vp, C.c_int, C.c_int, dp, C.c_int,
+                            vp, vp, vp, dp])
 _d("orc_gen_lutc", None, [C.c_int, C.c_int, C.c_int, C.c_int, vp])
 _d("orc_gen_lutc_rtoc", None, [C.c_int, vp])
 _d("orc_gen_lut8", None, [C.c_int, C.c_double, C.c_double, vp])
@@ -216,6 +219,34 @@ def trk_run(ch, code_bank_arr, buff, n_epoch, pos):
                     pos.ctypes.data_as(dp), len(pos), _p(corr), _p(Cc), state.ctypes.data_as(dp))
     return corr, Cc, state
 
+
+def csk(corr, len_code, pos):
+    """L6 CSK (src/sdr_ch.c:627-664) of one correlation: (ix, sym, taps)."""
+    corr = np.ascontiguousarray(corr, np.complex64)
+    pos = np.ascontiguousarray(pos, np.float64)
+    taps = np.zeros(len(pos), np.complex64)
+    ix = lib.orc_csk(_p(corr), len(corr), len_code, pos.ctypes.data_as(dp), len(pos), _p(taps))
+    return ix, (255 - ix) & 0xFF, taps
+
+
+def trk_l6_run(ch, len_code, code_fft_banks, buff, n_epoch, pos, keep_corr=False):
+    """L6D / L6E FFT tracking of one channel (src/sdr_ch.c:694-720).
+    code_fft_banks: (10, N) complex64.  Returns syms, taps, corr (or None),
+    state (n_epoch x {phi, coff, i, j})."""
+    c = orc_trk_ch_t(ch["fs"], ch["fi"], ch["fc_sig"], ch["T"], ch["N"], ch["fd0"],
+                     ch["drift"], ch["phi0"], ch["coff0"])
+    cf = np.ascontiguousarray(code_fft_banks, np.complex64)
+    buff = np.ascontiguousarray(buff, np.uint8)
+    pos = np.ascontiguousarray(pos, np.float64)
+    N = ch["N"]
+    syms = np.zeros(n_epoch, np.uint8)
+    taps = np.zeros((n_epoch, len(pos)), np.complex64)
+    corr = np.zeros((n_epoch, N), np.complex64) if keep_corr else None
+    state = np.zeros((n_epoch, 4), np.float64)
+    lib.orc_trk_l6_run(C.byref(c), len_code, _p(cf), _p(buff), len(buff), n_epoch,
+                       pos.ctypes.data_as(dp), len(pos), _p(syms), _p(taps),
+                       _p(corr) if keep_corr else None, state.ctypes.data_as(dp))
+    return syms, taps, corr, state
 
 # front-end raw data to cpx8 (src/sdr_rcv.c:776-936)
 FMT_INT8, FMT_INT8X2, FMT_RAW8, FMT_RAW16, FMT_RAW16I, FMT_RAW32, FMT_CS8, FMT_CS16 = range(1, 9)

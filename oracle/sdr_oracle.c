@@ -561,6 +561,108 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
 
 
 /* ---------------------------------------------------------------------------
+ * L6D / L6E CSK decoding, src/sdr_ch.c:627-664 (interp_corr, CSK).
+ * R = N / (len_code / 2) samples per chip, n = (int)(280 R); the reference
+ * copies C[k] = corr[(k - n) mod N] for 0 <= k < 2n (lag 0 at index n).
+ * Indices outside [0, 2n), which the reference reads past its array when
+ * n + ix R + pos < 0 (small R, far negative taps), take the same circular
+ * extension here.  The reference build (gcc -O3 -mavx2 -mfma) contracts
+ * n + i R to fma(i, R, n), a0 C[i] + a1 C[i+1] to fma(a0, C[i], a1 C[i+1])
+ * in double, and sdr_cpx_abs (src/sdr_func.c:179-182) to
+ * sqrtf(fmaf(re, re, im * im)).  Returns the chip shift ix of the peak
+ * (-255..255; 0 when no power is > 0); C receives the n_pos taps.
+ */
+static int circ(long long k, int N)
+{
+    long long r = k % N;
+    return (int)(r < 0 ? r + N : r);
+}
+
+static void orc_interp_corr(const float *corr, int N, int n, double x, float *c)
+{
+    int i = (int)x;
+    double a1 = x - i, a0 = 1.0 - a1;
+    const float *c0 = corr + 2 * (size_t)circ((long long)i - n, N);
+    const float *c1 = corr + 2 * (size_t)circ((long long)i + 1 - n, N);
+    c[0] = (float)fma(a0, (double)c0[0], a1 * (double)c1[0]);
+    c[1] = (float)fma(a0, (double)c0[1], a1 * (double)c1[1]);
+}
+
+int orc_csk(const float *corr, int N, int len_code, const double *pos, int n_pos,
+            float *C)
+{
+    double R = (double)N / (len_code / 2);
+    int n = (int)(280 * R);
+    double P_max = 0.0;
+    int ix = 0;
+    for (int i = -255; i <= 255; i++) {
+        float c[2];
+        orc_interp_corr(corr, N, n, fma((double)i, R, (double)n), c);
+        double P = sqrtf(fmaf(c[0], c[0], c[1] * c[1]));
+        if (P <= P_max) continue;
+        P_max = P;
+        ix = i;
+    }
+    double base = fma((double)ix, R, (double)n);
+    for (int t = 0; t < n_pos; t++) {
+        orc_interp_corr(corr, N, n, base + pos[t], C + 2 * t);
+    }
+    return ix;
+}
+
+/* ---------------------------------------------------------------------------
+ * L6D / L6E FFT tracking epochs, src/sdr_ch.c:694-720: the open-loop advance
+ * of orc_trk_run, then i = floor(coff fs), bank j = (int)((coff fs - i) 10)
+ * (coff fs is one rounded product used twice, as the reference build
+ * computes it), sdr_mix_carr from ring index (e N) mod buffN + i at phase
+ * phi + fc i / fs, sdr_corr_fft with code_fft + j N, then CSK.  The ring
+ * start is taken mod buffN (the reference receiver passes at most buffN,
+ * where a start of buffN mixes the same samples as 0).  code_fft: 10 banks of
+ * N complex (src/sdr_ch.c:256-262).  Outputs per epoch: syms (uint8
+ * 255 - ix % 256), C[n_pos] taps, corr[N] when corr_out is not NULL,
+ * state[4] = {phi, coff, i, j}.
+ */
+void orc_trk_l6_run(const orc_trk_ch_t *ch, int len_code, const float *code_fft,
+                    const uint8_t *buff, int buffN, int n_epoch, const double *pos,
+                    int n_pos, uint8_t *syms, float *C, float *corr_out, double *state)
+{
+    double time = 0.0, phi = ch->phi0, coff = ch->coff0;
+    int N = ch->N;
+    int8_t *IQ = (int8_t *)malloc(2 * (size_t)N);
+    float *corr = (float *)malloc(sizeof(float) * 2 * (size_t)N);
+    for (int e = 0; e < n_epoch; e++) {
+        double t = (e + 1) * ch->T;
+        double tau = t - time;
+        double fd = ch->fd0 + ch->drift * t;
+        double fc = ch->fi + fd;
+        coff = fma(-(fd / ch->fc_sig), tau, coff);
+        phi = fma(fc, tau, phi);
+        phi -= floor(phi);
+        time = t;
+        if (coff >= ch->T) coff -= ch->T;
+        else if (coff < 0.0) coff += ch->T;
+        double cs = coff * ch->fs;
+        int i = (int)floor(cs);
+        int j = (int)((cs - i) * 10);
+        int ix = (int)(((long long)e * N) % buffN);
+        int start = circ((long long)ix + i, buffN);
+        orc_mix_carr(buff, buffN, start, N, ch->fs, fc, phi + fc * i / ch->fs, IQ);
+        orc_corr_fft(IQ, code_fft + 2 * (size_t)j * N, N, corr);
+        int x = orc_csk(corr, N, len_code, pos, n_pos, C + 2 * (size_t)e * n_pos);
+        syms[e] = (uint8_t)(255 - x % 256);
+        if (corr_out) memcpy(corr_out + 2 * (size_t)e * N, corr, sizeof(float) * 2 * (size_t)N);
+        if (state) {
+            state[4 * e] = phi;
+            state[4 * e + 1] = coff;
+            state[4 * e + 2] = i;
+            state[4 * e + 3] = j;
+        }
+    }
+    free(corr);
+    free(IQ);
+}
+
+/* ---------------------------------------------------------------------------
  * front-end raw data to cpx8, src/sdr_rcv.c:776-936 (write_buff and its LUT
  * generators); the LPF and array-combine steps after it are not included
  * ------------------------------------------------------------------------- */

@@ -101,6 +101,16 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
                  const double *pos, int n_pos, float *corr, float *C,
                  double *state);
 
+/* L6D / L6E CSK, src/sdr_ch.c:627-664: returns the chip shift ix of the
+ * peak, C receives the n_pos interpolated taps (cpx). */
+int orc_csk(const float *corr, int N, int len_code, const double *pos, int n_pos,
+            float *C);
+/* L6D / L6E FFT tracking, src/sdr_ch.c:694-720 (open loop as orc_trk_run).
+ * Per epoch: syms[e], C[n_pos] (cpx), corr[N] (cpx, if corr_out), state[4]. */
+void orc_trk_l6_run(const orc_trk_ch_t *ch, int len_code, const float *code_fft,
+                    const uint8_t *buff, int buffN, int n_epoch, const double *pos,
+                    int n_pos, uint8_t *syms, float *C, float *corr_out, double *state);
+
 /* Front-end raw data to cpx8 IF buffers, src/sdr_rcv.c:776-936.  fmt: the
  * SDR_FMT_* codes of src/pocket_sdr.h:85-92.  LUT generators: gen_LUTC
  * (256 cpx8), gen_LUTC_rtoc (4 x 256 cpx8, the fs/4 real-to-complex phases),

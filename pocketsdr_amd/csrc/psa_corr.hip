@@ -12,7 +12,7 @@
 #include <string.h>
 #include <algorithm>
 #include <string>
-#include "psa_internal.h"
+#include "psa_trk.h"
 
 namespace psa {
 
@@ -271,90 +271,25 @@ static void corr_std_impl(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N,
 // batched open-loop tracking
 // ===========================================================================
 
-// Per-channel parameters on the device.
-struct TrkChDev {
-    double fs, fi, fc_sig, T, fd0, drift, phi0, coff0;
-};
-
-// Epoch state of one (channel, epoch): mixing parameters and the code offset
-// in samples (cs = coff fs, the origin of the tap geometry).
-struct TrkEpoch {
-    int ix, nseg;
-    uint32_t p0, p1, s;
-    int pad;
-    double cs;
-};
-
-// Open-loop NCO / code advance, src/sdr_ch.c:694-703 with adj_coff
-// (674-689), in FP64, one 64-thread workgroup per channel.  Per chunk of 64
-// epochs the lanes compute the epoch-local terms (tau, fc = fi + fd and
-// -fd / fc_sig) in parallel, lane 0 runs the order-dependent recurrence
-// (phase wrap, code-offset wrap) over the chunk, and the lanes then derive
-// the mixing parameters.  The reference build fuses coff -= fd / fc * tau
-// and phi += fc * tau into fnmadd / fmadd; the same fused forms are written
-// here explicitly.
-#define TRK_SCHUNK 64
+// Open-loop state per (channel, epoch), one 64-thread workgroup per channel
+// (the recurrence of trk_open_loop), then the sdr_mix_carr parameters of
+// the epoch's N samples at ring index (e N) mod buffN and cs = coff fs.
 __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *__restrict__ chs, int n_epoch,
                                                               int N, int buffN, TrkEpoch *__restrict__ ep)
 {
 #pragma clang fp contract(off)
-    __shared__ double s_a[TRK_SCHUNK], s_tau[TRK_SCHUNK], s_fc[TRK_SCHUNK], s_coff[TRK_SCHUNK], s_phi[TRK_SCHUNK];
-    const int c = blockIdx.x, lane = threadIdx.x;
+    const int c = blockIdx.x;
     const TrkChDev ch = chs[c];
-    const double scale = 4294967296.0;
-    double phi = ch.phi0, coff = ch.coff0;    // lane 0's running state
-    for (int e0 = 0; e0 < n_epoch; e0 += TRK_SCHUNK) {
-        const int e = e0 + lane;
-        double fc = 0.0;
-        if (e < n_epoch) {
-            // t of this epoch and of the previous one ((double)e T, 0 at e = 0)
-            const double t = (double)(e + 1) * ch.T;
-            const double time = (double)e * ch.T;
-            const double fd = ch.fd0 + ch.drift * t;
-            fc = ch.fi + fd;
-            s_a[lane] = -(fd / ch.fc_sig);
-            s_tau[lane] = t - time;
-            s_fc[lane] = fc;
-        }
-        __syncthreads();
-        if (lane == 0) {
-            const int ne = min(TRK_SCHUNK, n_epoch - e0);
-            for (int i = 0; i < ne; i++) {
-                const double tau = s_tau[i];
-                coff = fma(s_a[i], tau, coff);
-                phi = fma(s_fc[i], tau, phi);
-                phi = phi - floor(phi);
-                if (coff >= ch.T) coff = coff - ch.T;
-                else if (coff < 0.0) coff = coff + ch.T;
-                s_coff[i] = coff;
-                s_phi[i] = phi;
-            }
-        }
-        __syncthreads();
-        if (e < n_epoch) {
-            const double ph = s_phi[lane];
-            // sdr_mix_carr phase setup (src/sdr_func.c:776-778, 824-836)
-            const int ix = (int)(((long long)e * N) % buffN);
-            const double step = fc / ch.fs;
-            TrkEpoch q;
-            q.ix = ix;
-            q.p0 = (uint32_t)((ph - floor(ph)) * scale);
-            q.s = (uint32_t)(int)(step * scale);
-            if (ix + N <= buffN) {
-                q.nseg = N;
-                q.p1 = 0;
-            } else {
-                const int n = buffN - ix;
-                const double phi2 = fma((double)n, step, ph);
-                q.nseg = n;
-                q.p1 = (uint32_t)((phi2 - floor(phi2)) * scale);
-            }
-            q.pad = 0;
-            q.cs = s_coff[lane] * ch.fs;
-            ep[(long long)c * n_epoch + e] = q;
-        }
-        __syncthreads();
-    }
+    trk_open_loop(ch, n_epoch, [&](int e, double ph, double coff, double fc) {
+#pragma clang fp contract(off)
+        const int ix = (int)(((long long)e * N) % buffN);
+        TrkEpoch q;
+        q.ix = ix;
+        trk_mix_params(buffN, ix, N, ch.fs, fc, ph, q.p0, q.p1, q.nseg, q.s);
+        q.pad = 0;
+        q.cs = coff * ch.fs;
+        ep[(long long)c * n_epoch + e] = q;
+    });
 }
 
 // Fused mix + correlate: one workgroup per (channel, epoch).  The epoch's N

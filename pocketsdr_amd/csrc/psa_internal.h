@@ -156,6 +156,7 @@ struct PassArgs {
     const int8_t *iq16;
     // LD_PROD: X (in, batch b -> b / prod_xdiv % prod_xmod) times Cf
     const float2 *cf;           // code spectrum, batch b -> (b / prod_cdiv)
+    const int *cf_index;        // FP32 passes: per-batch spectrum index (nullptr: b / prod_cdiv)
     int prod_xdiv, prod_xmod, prod_cdiv;
     long long cf_bstride;
     float prod_scale;
@@ -242,6 +243,49 @@ void fft_c2c(const FFTPlan *pl, const float2 *in, float2 *out, float2 *tmp,
 // bin k1 + M1*k2)
 void fft_to_transposed(const FFTPlan *pl, const float2 *in, float2 *out,
                        int nbatch, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// pass argument helpers for the correlator
+// ---------------------------------------------------------------------------
+inline void set_prod(const FFTPlan *pl, PassArgs &a, float scale)
+{
+    a.prod_scale = scale;
+    a.prod_V = pl->M >= 4 ? pl->M / 4 * 4 : 0;
+    a.prod_M1 = pl->M1;
+    a.prod_M2 = pl->M2;
+}
+
+inline void args_fwd_col(const FFTPlan *pl, PassArgs &a)
+{
+    col_pass_args(pl, 0, a);
+}
+
+inline void args_fwd_row(const FFTPlan *pl, PassArgs &a)
+{
+    row_pass_args(pl, 0, a);   // digit-transposed output
+}
+
+inline void args_inv_row(const FFTPlan *pl, PassArgs &a)
+{
+    row_pass_args(pl, 1, a);
+    if (pl->M2 > 1) {          // twiddle w_M^{-n_a k1} on store
+        a.thi = pl->thi_bwd;
+        a.tlo = pl->tlo_bwd;
+    }
+}
+
+inline void args_inv_col(const FFTPlan *pl, PassArgs &a)
+{
+    col_pass_args(pl, 1, a);   // natural-order output n_a + M2 n_b
+    a.thi = a.tlo = nullptr;
+}
+
+inline const FFTPlan *need_plan(int M, const char *who)
+{
+    const FFTPlan *pl = fft_plan(M);
+    if (!pl) PSA_FAIL("%s: FFT length %d has prime factors other than 2, 3, 5", who, M);
+    return pl;
+}
 
 // ---------------------------------------------------------------------------
 // misc kernels (psa_mix.hip)

@@ -1,0 +1,473 @@
+// psa_l6.hip - QZSS L6D / L6E FFT tracking and CSK decoding on gfx950
+// (src/sdr_ch.c:627-664, 694-720).
+//
+// Per (channel, epoch) item the reference mixes the epoch's N samples from
+// the code-offset sample i = floor(coff fs), FFT-correlates them against
+// the code spectrum bank j = (int)((coff fs - i) SDR_N_CODES) (src/sdr_ch.c:
+// 256-262: N-point circular spectra, the code shifted by -j / 10 sample) and
+// decodes the CSK symbol from the correlation: the peak of |corr| over code
+// shifts of -255 ... 255 chips, interpolated linearly between samples.
+//
+// The plan batches every item of every channel:
+//   l6_state_kernel : open-loop state per channel (FP64, trk_open_loop),
+//                     mixing parameters and spectrum index per item
+//   fwd column pass : IF bytes, carrier mix (LUT semantics), * CSCALE,
+//                     length-M1 FFTs, twiddle                     -> A
+//   fwd row pass    : length-M2 FFTs                              -> B
+//   inv row pass    : (B * Cf[index]) / N^2 (sdr_cpx_mul arithmetic),
+//                     length-M2 inverse FFTs, twiddle             -> A
+//   inv column pass : length-M1 inverse FFTs                      -> corr
+//   csk_kernel      : 511 interpolated powers, first maximum, symbol, taps
+// The FFT passes are those of the batched search (psa_fft.hip), with the
+// code spectrum chosen per item through PassArgs::cf_index.
+#include <math.h>
+#include <string.h>
+#include <string>
+#include <algorithm>
+#include "psa_trk.h"
+
+namespace psa {
+
+#define CSK_THREADS 64
+#define CSK_NP 511              // code shifts -255 ... 255 (src/sdr_ch.c:647)
+
+// Per-item state of the batch: the sdr_mix_carr parameters of the epoch's
+// samples and the index of its code spectrum (channel * SDR_N_CODES + j).
+struct L6Items {
+    int *ix, *nseg, *cf;
+    uint32_t *p0, *p1, *s;
+};
+
+// src/sdr_ch.c:705-712 after the open-loop advance: item c * n_epoch + e.
+// The ring start (e N) mod buffN + i is taken mod buffN (the receiver passes
+// at most buffN, where a start of buffN mixes the same samples as 0).
+__global__ void __launch_bounds__(TRK_SCHUNK) l6_state_kernel(const TrkChDev *__restrict__ chs, int n_epoch,
+                                                             int N, int buffN, L6Items it)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x;
+    const TrkChDev ch = chs[c];
+    trk_open_loop(ch, n_epoch, [&](int e, double ph, double coff, double fc) {
+#pragma clang fp contract(off)
+        const long long o = (long long)c * n_epoch + e;
+        const double cs = coff * ch.fs;           // one rounded product, used twice
+        const int i = (int)floor(cs);
+        int j = (int)((cs - i) * PSA_N_CODES);
+        j = min(max(j, 0), PSA_N_CODES - 1);      // (cs - i) lies in [0, 1)
+        long long st = ((long long)e * N) % buffN + i;
+        st %= buffN;
+        if (st < 0) st += buffN;
+        const double phi = ph + fc * i / ch.fs;
+        uint32_t p0, p1, s;
+        int nseg;
+        trk_mix_params(buffN, (int)st, N, ch.fs, fc, phi, p0, p1, nseg, s);
+        it.ix[o] = (int)st;
+        it.nseg[o] = nseg;
+        it.p0[o] = p0;
+        it.p1[o] = p1;
+        it.s[o] = s;
+        it.cf[o] = c * PSA_N_CODES + j;
+    });
+}
+
+// interp_corr (src/sdr_ch.c:627-633) on the correlation rotated so that lag
+// 0 sits at index n: C[k] = corr[(k - n) mod N], the reference's copy for
+// 0 <= k < 2n and its circular extension outside.  The reference build
+// contracts a0 C[i] + a1 C[i+1] to fma(a0, C[i], a1 C[i+1]) in double.
+__device__ __forceinline__ float2 csk_interp(const float2 *__restrict__ corr, int N, int n, double x)
+{
+#pragma clang fp contract(off)
+    const int i = (int)x;                        // truncation toward zero, as (int)
+    const double a1 = x - (double)i, a0 = 1.0 - a1;
+    int k0 = (int)(((long long)i - n) % N);
+    if (k0 < 0) k0 += N;
+    const int k1 = k0 + 1 == N ? 0 : k0 + 1;
+    const float2 c0 = corr[k0], c1 = corr[k1];
+    return make_float2((float)fma(a0, (double)c0.x, a1 * (double)c1.x),
+                       (float)fma(a0, (double)c0.y, a1 * (double)c1.y));
+}
+
+// CSK (src/sdr_ch.c:636-664), one wave per item.  Lane l evaluates shifts
+// -255 + l + 64 k, x = fma(i, R, n) as the reference build contracts
+// n + i R, and |c| = sqrtf(fmaf(re, re, im * im)) (sdr_cpx_abs,
+// src/sdr_func.c:179-182, contracted likewise), correctly rounded.  The
+// reference keeps the first strict maximum over values > 0 (ix = 0 when none
+// is), which the wave reduction reproduces: largest value, smallest shift
+// among equal ones.  A NaN power changes the meaning of the sequential test
+// (P <= P_max fails for it), so a wave that sees one replays the reference
+// loop over the 511 powers on lane 0.
+__global__ void __launch_bounds__(CSK_THREADS) csk_kernel(const float2 *__restrict__ corr_all, long long corr_stride,
+                                                         int N, int n, double R, const double *__restrict__ pos,
+                                                         int n_pos, uint8_t *__restrict__ syms,
+                                                         float2 *__restrict__ taps)
+{
+#pragma clang fp contract(off)
+    __shared__ float sP[CSK_NP + 1];
+    const long long item = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float2 *corr = corr_all + item * corr_stride;
+    const double dn = (double)n;
+    float bm = 0.0f;
+    int bi = 0x7FFFFFFF;
+    bool nan = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int i = -255 + lane + 64 * k;
+        if (i <= 255) {
+            const float2 c = csk_interp(corr, N, n, fma((double)i, R, dn));
+            const float P = __fsqrt_rn(__fmaf_rn(c.x, c.x, __fmul_rn(c.y, c.y)));
+            sP[i + 255] = P;
+            nan |= P != P;
+            if (P > bm) { bm = P; bi = i; }       // i rises with k: first wins
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(bm, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (om > bm || (om == bm && oi < bi)) { bm = om; bi = oi; }
+    }
+    int ix = bm > 0.0f ? bi : 0;
+    if (__any(nan)) {
+        __syncthreads();
+        if (lane == 0) {
+            double pm = 0.0;
+            int x = 0;
+            for (int i = -255; i <= 255; i++) {
+                const double P = (double)sP[i + 255];
+                if (P <= pm) continue;
+                pm = P;
+                x = i;
+            }
+            sP[CSK_NP] = __int_as_float(x);
+        }
+        __syncthreads();
+        ix = __float_as_int(sP[CSK_NP]);
+    }
+    if (lane == 0) syms[item] = (uint8_t)(255 - ix);
+    const double base = fma((double)ix, R, dn);
+    for (int t = lane; t < n_pos; t += CSK_THREADS) {
+        taps[item * n_pos + t] = csk_interp(corr, N, n, base + pos[t]);
+    }
+}
+
+// R = N / (len_code / 2) samples per chip, n = (int)(280 R) (src/sdr_ch.c:638-639)
+static void csk_geometry(int N, int len_code, double *R, int *n)
+{
+    *R = (double)N / (len_code / 2);
+    *n = (int)(280 * *R);
+}
+
+static void launch_csk(const float2 *corr, long long stride, int n_item, int N, int len_code,
+                       const double *pos, int n_pos, uint8_t *syms, float2 *taps, hipStream_t st)
+{
+    double R;
+    int n;
+    csk_geometry(N, len_code, &R, &n);
+    hipLaunchKernelGGL(csk_kernel, dim3((unsigned)n_item), dim3(CSK_THREADS), 0, st, corr, stride, N, n, R,
+                       pos, n_pos, syms, taps);
+    PSA_CHECK(hipGetLastError());
+}
+
+enum { L6_STATE = 0, L6_FWD_COL, L6_FWD_ROW, L6_INV_ROW, L6_INV_COL, L6_CSK, L6_NCLS };
+static const char *l6_names[L6_NCLS] = {"state", "fwd_col", "fwd_row", "inv_row", "inv_col", "csk"};
+
+}  // namespace psa
+
+using namespace psa;
+
+struct sdr_l6_plan_tag {
+    int n_ch = 0, n_pos = 0, n_epoch = 0, N = 0, buffN = 0, len_code = 0, flags = 0;
+    long long n_item = 0;
+    int chunk = 0;                  // items per batch of FFT passes
+    const FFTPlan *pl = nullptr;
+    uint8_t *d_buf = nullptr;
+    float2 *d_cf = nullptr;         // n_ch * SDR_N_CODES spectra, digit-transposed
+    TrkChDev *d_ch = nullptr;
+    double *d_pos = nullptr;
+    L6Items it = {};
+    float2 *A = nullptr, *B = nullptr, *corr = nullptr;   // corr: all items (KEEP_CORR)
+    uint8_t *d_sym = nullptr;
+    float2 *d_taps = nullptr;
+    hipStream_t st = nullptr;
+    bool timing = false;
+    struct Ev { int cls; hipEvent_t a, b; };
+    std::vector<Ev> evs;
+    size_t ev_used = 0;
+    double ms[L6_NCLS] = {}, elems[L6_NCLS] = {};
+    long long cnt[L6_NCLS] = {};
+};
+
+namespace psa {
+
+// event pair around one launch when timing is on
+struct L6Tick {
+    sdr_l6_plan_t *p;
+    size_t slot = 0;
+    L6Tick(sdr_l6_plan_t *p_, int cls) : p(p_)
+    {
+        if (!p->timing) return;
+        if (p->ev_used == p->evs.size()) {
+            sdr_l6_plan_t::Ev ev;
+            PSA_CHECK(hipEventCreate(&ev.a));
+            PSA_CHECK(hipEventCreate(&ev.b));
+            p->evs.push_back(ev);
+        }
+        slot = p->ev_used++;
+        p->evs[slot].cls = cls;
+        PSA_CHECK(hipEventRecord(p->evs[slot].a, p->st));
+    }
+    ~L6Tick()
+    {
+        if (p->timing) PSA_CHECK(hipEventRecord(p->evs[slot].b, p->st));
+    }
+};
+
+}  // namespace psa
+
+extern "C" {
+
+sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch_t *chs,
+    int n_pos, const double *pos, int n_epoch, int flags)
+{
+    if (!buff || !buff->data || buff->N <= 0 || n_ch <= 0 || n_epoch <= 0 || n_pos < 0 ||
+        n_pos > PSA_MAX_CORR || !chs || (n_pos > 0 && !pos)) {
+        fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: invalid arguments\n");
+        return nullptr;
+    }
+    const int N = chs[0].N, len_code = chs[0].len_code;
+    if (N <= 0 || N > buff->N || len_code < 2) {
+        fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: need 0 < N <= buffer length, len_code >= 2\n");
+        return nullptr;
+    }
+    for (int c = 0; c < n_ch; c++) {
+        if (chs[c].N != N || chs[c].len_code != len_code || !chs[c].code_fft) {
+            fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: channels need one N, one len_code and code spectra\n");
+            return nullptr;
+        }
+    }
+    const FFTPlan *pl = fft_plan(N);
+    if (!pl) {
+        fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: FFT length %d has prime factors other than 2, 3, 5\n", N);
+        return nullptr;
+    }
+    const long long n_item = (long long)n_ch * n_epoch;
+    const size_t item_bytes = sizeof(float2) * (size_t)N;
+    if ((flags & SDR_L6_KEEP_CORR) && (double)n_item * item_bytes > 64.0 * (1ull << 30)) {
+        fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: kept correlations exceed 64 GiB\n");
+        return nullptr;
+    }
+    ensure_init();
+    sdr_l6_plan_t *p = new sdr_l6_plan_t();
+    p->n_ch = n_ch; p->n_pos = n_pos; p->n_epoch = n_epoch; p->N = N; p->buffN = buff->N;
+    p->len_code = len_code; p->flags = flags; p->n_item = n_item; p->pl = pl;
+    // two work buffers of `chunk` transforms each, 2 GiB together at most
+    const long long cap = std::max<long long>(1, (2ll << 30) / (long long)(2 * item_bytes));
+    p->chunk = (int)std::min<long long>(n_item, std::min<long long>(cap, 1 << 20));
+    // PSA_L6_CHUNK caps the items per batch of passes (smaller batches)
+    if (const char *e = getenv("PSA_L6_CHUNK")) {
+        const long long c = atoll(e);
+        if (c > 0) p->chunk = (int)std::min<long long>(p->chunk, c);
+    }
+    PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+    PSA_CHECK(hipMalloc(&p->d_buf, buff->N));
+    PSA_CHECK(hipMemcpy(p->d_buf, buff->data, buff->N, hipMemcpyHostToDevice));
+    // code spectra: natural order from the caller, digit-transposed on the device
+    const size_t nspec = (size_t)n_ch * PSA_N_CODES;
+    PSA_CHECK(hipMalloc(&p->d_cf, nspec * item_bytes));
+    float2 *stage = nullptr;
+    PSA_CHECK(hipMalloc(&stage, nspec * item_bytes));
+    for (int c = 0; c < n_ch; c++) {
+        PSA_CHECK(hipMemcpy(stage + (size_t)c * PSA_N_CODES * N, chs[c].code_fft,
+                            PSA_N_CODES * item_bytes, hipMemcpyHostToDevice));
+    }
+    fft_to_transposed(pl, stage, p->d_cf, (int)nspec, p->st);
+    PSA_CHECK(hipStreamSynchronize(p->st));
+    PSA_CHECK(hipFree(stage));
+    std::vector<TrkChDev> hc(n_ch);
+    for (int c = 0; c < n_ch; c++) {
+        hc[c] = TrkChDev{chs[c].fs, chs[c].fi, chs[c].fc_sig, chs[c].T, chs[c].fd0,
+                         chs[c].drift, chs[c].phi0, chs[c].coff0};
+    }
+    PSA_CHECK(hipMalloc(&p->d_ch, sizeof(TrkChDev) * n_ch));
+    PSA_CHECK(hipMemcpy(p->d_ch, hc.data(), sizeof(TrkChDev) * n_ch, hipMemcpyHostToDevice));
+    PSA_CHECK(hipMalloc(&p->d_pos, sizeof(double) * std::max(n_pos, 1)));
+    if (n_pos > 0) PSA_CHECK(hipMemcpy(p->d_pos, pos, sizeof(double) * n_pos, hipMemcpyHostToDevice));
+    const size_t ni = (size_t)n_item;
+    PSA_CHECK(hipMalloc(&p->it.ix, sizeof(int) * ni));
+    PSA_CHECK(hipMalloc(&p->it.nseg, sizeof(int) * ni));
+    PSA_CHECK(hipMalloc(&p->it.cf, sizeof(int) * ni));
+    PSA_CHECK(hipMalloc(&p->it.p0, sizeof(uint32_t) * ni));
+    PSA_CHECK(hipMalloc(&p->it.p1, sizeof(uint32_t) * ni));
+    PSA_CHECK(hipMalloc(&p->it.s, sizeof(uint32_t) * ni));
+    PSA_CHECK(hipMalloc(&p->A, (size_t)p->chunk * item_bytes));
+    PSA_CHECK(hipMalloc(&p->B, (size_t)p->chunk * item_bytes));
+    if (flags & SDR_L6_KEEP_CORR) PSA_CHECK(hipMalloc(&p->corr, ni * item_bytes));
+    PSA_CHECK(hipMalloc(&p->d_sym, ni));
+    PSA_CHECK(hipMalloc(&p->d_taps, sizeof(float2) * ni * std::max(n_pos, 1)));
+    return p;
+}
+
+}  // extern "C"
+
+static void l6_run(sdr_l6_plan_t *p)
+{
+    hipStream_t st = p->st;
+    const FFTPlan *pl = p->pl;
+    const int M = pl->M;
+    {
+        L6Tick t(p, L6_STATE);
+        hipLaunchKernelGGL(l6_state_kernel, dim3(p->n_ch), dim3(TRK_SCHUNK), 0, st, p->d_ch, p->n_epoch, p->N,
+                           p->buffN, p->it);
+        PSA_CHECK(hipGetLastError());
+    }
+    for (long long i0 = 0; i0 < p->n_item; i0 += p->chunk) {
+        const int nb = (int)std::min<long long>(p->chunk, p->n_item - i0);
+        PassArgs a = {};
+        args_fwd_col(pl, a);
+        a.buf = p->d_buf; a.buffN = p->buffN; a.mix_compact = 0;
+        a.mix_ix = p->it.ix + i0; a.mix_nseg = p->it.nseg + i0;
+        a.mix_p0 = p->it.p0 + i0; a.mix_p1 = p->it.p1 + i0; a.mix_s = p->it.s + i0;
+        a.out = p->A; a.out_bstride = M;
+        {
+            L6Tick t(p, L6_FWD_COL);
+            launch_pass(LD_MIX, ST_CPX, a, nb, st);
+        }
+        PassArgs r = {};
+        args_fwd_row(pl, r);
+        r.in = p->A; r.out = p->B; r.in_bstride = r.out_bstride = M;
+        {
+            L6Tick t(p, L6_FWD_ROW);
+            launch_pass(LD_CPX, ST_CPX, r, nb, st);
+        }
+        PassArgs ir = {};
+        args_inv_row(pl, ir);
+        ir.in = p->B; ir.in_bstride = M;
+        ir.cf = p->d_cf; ir.cf_bstride = M; ir.cf_index = p->it.cf + i0;
+        ir.prod_xdiv = 1; ir.prod_xmod = nb; ir.prod_cdiv = 1;
+        set_prod(pl, ir, 1.0f / M / M);          // sdr_corr_fft: 1.0f / N / N
+        ir.out = p->A; ir.out_bstride = M;
+        {
+            L6Tick t(p, L6_INV_ROW);
+            launch_pass(LD_PROD, ST_CPX, ir, nb, st);
+        }
+        float2 *corr = p->corr ? p->corr + i0 * M : p->B;
+        PassArgs ic = {};
+        args_inv_col(pl, ic);
+        ic.in = p->A; ic.in_bstride = M; ic.out = corr; ic.out_bstride = M;
+        {
+            L6Tick t(p, L6_INV_COL);
+            launch_pass(LD_CPX, ST_CPX, ic, nb, st);
+        }
+        {
+            L6Tick t(p, L6_CSK);
+            launch_csk(corr, M, nb, p->N, p->len_code, p->d_pos, p->n_pos, p->d_sym + i0,
+                       p->d_taps + i0 * p->n_pos, st);
+        }
+    }
+}
+
+extern "C" {
+
+int sdr_l6_plan_run(sdr_l6_plan_t *p)
+{
+    if (!p) return 0;
+    p->ev_used = 0;
+    l6_run(p);
+    PSA_CHECK(hipStreamSynchronize(p->st));
+    if (p->timing) {
+        const double M = p->pl->M;
+        for (size_t i = 0; i < p->ev_used; i++) {
+            float ms = 0;
+            PSA_CHECK(hipEventElapsedTime(&ms, p->evs[i].a, p->evs[i].b));
+            p->ms[p->evs[i].cls] += ms;
+        }
+        for (int c = 0; c < L6_NCLS; c++) {
+            p->cnt[c]++;
+            p->elems[c] += c == L6_STATE ? (double)p->n_item : (double)p->n_item * M;
+        }
+    }
+    return 1;
+}
+
+int sdr_l6_plan_result(const sdr_l6_plan_t *p, uint8_t *syms, sdr_cpx_t *C)
+{
+    if (!p) return 0;
+    const size_t ni = (size_t)p->n_item;
+    if (syms) PSA_CHECK(hipMemcpy(syms, p->d_sym, ni, hipMemcpyDeviceToHost));
+    if (C && p->n_pos > 0) {
+        PSA_CHECK(hipMemcpy(C, p->d_taps, sizeof(float2) * ni * p->n_pos, hipMemcpyDeviceToHost));
+    }
+    return 1;
+}
+
+int sdr_l6_plan_corr(const sdr_l6_plan_t *p, int ch, int epoch, sdr_cpx_t *corr)
+{
+    if (!p || !p->corr || !corr || ch < 0 || ch >= p->n_ch || epoch < 0 || epoch >= p->n_epoch) return 0;
+    const size_t o = ((size_t)ch * p->n_epoch + epoch) * p->pl->M;
+    PSA_CHECK(hipMemcpy(corr, p->corr + o, sizeof(float2) * p->N, hipMemcpyDeviceToHost));
+    return 1;
+}
+
+void sdr_l6_plan_timing(sdr_l6_plan_t *p, int on)
+{
+    if (!p) return;
+    p->timing = on != 0;
+    for (int c = 0; c < L6_NCLS; c++) { p->ms[c] = 0; p->elems[c] = 0; p->cnt[c] = 0; }
+}
+
+int sdr_l6_plan_kernel_ms(const sdr_l6_plan_t *p, const char *name, double *avg_ms, long long *count,
+                          double *elems)
+{
+    if (!p || !name) return 0;
+    for (int c = 0; c < L6_NCLS; c++) {
+        if (strcmp(name, l6_names[c]) || !p->cnt[c]) continue;
+        if (avg_ms) *avg_ms = p->ms[c] / p->cnt[c];
+        if (count) *count = p->cnt[c];
+        if (elems) *elems = p->elems[c] / p->cnt[c];
+        return 1;
+    }
+    return 0;
+}
+
+void sdr_l6_plan_free(sdr_l6_plan_t *p)
+{
+    if (!p) return;
+    (void)hipStreamSynchronize(p->st);
+    (void)hipFree(p->d_buf); (void)hipFree(p->d_cf); (void)hipFree(p->d_ch); (void)hipFree(p->d_pos);
+    (void)hipFree(p->it.ix); (void)hipFree(p->it.nseg); (void)hipFree(p->it.cf);
+    (void)hipFree(p->it.p0); (void)hipFree(p->it.p1); (void)hipFree(p->it.s);
+    (void)hipFree(p->A); (void)hipFree(p->B); (void)hipFree(p->corr);
+    (void)hipFree(p->d_sym); (void)hipFree(p->d_taps);
+    for (auto &e : p->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    (void)hipStreamDestroy(p->st);
+    delete p;
+}
+
+int sdr_csk(const sdr_cpx_t *corr, int n_item, int N, int len_code, const double *pos, int n_pos,
+            uint8_t *syms, sdr_cpx_t *C)
+{
+    if (!corr || n_item <= 0 || N <= 0 || len_code < 2 || n_pos < 0 || (n_pos > 0 && (!pos || !C)) ||
+        !syms) {
+        fprintf(stderr, "pocketsdr_amd: sdr_csk: invalid arguments\n");
+        return 0;
+    }
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    const size_t cb = sizeof(float2) * (size_t)N * n_item;
+    const size_t tb = sizeof(float2) * (size_t)n_item * n_pos;
+    float2 *dc = (float2 *)tc.s[SC_A].get(cb);
+    uint8_t *dm = (uint8_t *)tc.s[SC_MISC].get(tb + sizeof(double) * (size_t)n_pos + (size_t)n_item + 64);
+    float2 *dt = (float2 *)dm;
+    double *dp = (double *)(dm + tb);
+    uint8_t *ds = dm + tb + sizeof(double) * (size_t)n_pos;
+    PSA_CHECK(hipMemcpyAsync(dc, corr, cb, hipMemcpyHostToDevice, st));
+    if (n_pos > 0) PSA_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * n_pos, hipMemcpyHostToDevice, st));
+    launch_csk(dc, N, n_item, N, len_code, dp, n_pos, ds, dt, st);
+    PSA_CHECK(hipMemcpyAsync(syms, ds, (size_t)n_item, hipMemcpyDeviceToHost, st));
+    if (n_pos > 0) PSA_CHECK(hipMemcpyAsync(C, dt, tb, hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    return 1;
+}
+
+}  // extern "C"

@@ -47,7 +47,7 @@ struct Loader {
         }
         if (LM == LD_PROD) {
             const long long xb = (b / a.prod_xdiv) % a.prod_xmod;
-            const long long cb = b / a.prod_cdiv;
+            const long long cb = a.cf_index ? a.cf_index[b] : b / a.prod_cdiv;
             xin = a.in + xb * a.in_bstride;
             cin = a.cf + cb * a.cf_bstride;
             prod_tail = a.prod_V < a.M;

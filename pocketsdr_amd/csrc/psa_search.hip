@@ -158,49 +158,6 @@ __global__ void __launch_bounds__(PEAK_THREADS) peak_final_kernel(const float *_
     }
 }
 
-// ---------------------------------------------------------------------------
-// pass argument helpers for the correlator
-// ---------------------------------------------------------------------------
-static void set_prod(const FFTPlan *pl, PassArgs &a, float scale)
-{
-    a.prod_scale = scale;
-    a.prod_V = pl->M >= 4 ? pl->M / 4 * 4 : 0;
-    a.prod_M1 = pl->M1;
-    a.prod_M2 = pl->M2;
-}
-
-static void args_fwd_col(const FFTPlan *pl, PassArgs &a)
-{
-    col_pass_args(pl, 0, a);
-}
-
-static void args_fwd_row(const FFTPlan *pl, PassArgs &a)
-{
-    row_pass_args(pl, 0, a);   // digit-transposed output
-}
-
-static void args_inv_row(const FFTPlan *pl, PassArgs &a)
-{
-    row_pass_args(pl, 1, a);
-    if (pl->M2 > 1) {          // twiddle w_M^{-n_a k1} on store
-        a.thi = pl->thi_bwd;
-        a.tlo = pl->tlo_bwd;
-    }
-}
-
-static void args_inv_col(const FFTPlan *pl, PassArgs &a)
-{
-    col_pass_args(pl, 1, a);   // natural-order output n_a + M2 n_b
-    a.thi = a.tlo = nullptr;
-}
-
-static const FFTPlan *need_plan(int M, const char *who)
-{
-    const FFTPlan *pl = fft_plan(M);
-    if (!pl) PSA_FAIL("%s: FFT length %d has prime factors other than 2, 3, 5", who, M);
-    return pl;
-}
-
 // Per-thread cache of device code spectra in digit-transposed order.
 struct SpecCache {
     const void *host = nullptr;

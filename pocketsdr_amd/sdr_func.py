@@ -39,6 +39,13 @@ class sdr_trk_ch_t(C.Structure):
                 ("code", c_void_p)]
 
 
+class sdr_l6_ch_t(C.Structure):
+    _fields_ = [("fs", C.c_double), ("fi", C.c_double), ("fc_sig", C.c_double),
+                ("T", C.c_double), ("N", C.c_int), ("len_code", C.c_int), ("fd0", C.c_double),
+                ("drift", C.c_double), ("phi0", C.c_double), ("coff0", C.c_double),
+                ("code_fft", c_void_p)]
+
+
 def _decl(name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -88,6 +95,17 @@ _decl("sdr_trk_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_trk_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                           C.POINTER(C.c_longlong), c_double_p])
 _decl("sdr_trk_plan_free", None, [c_void_p])
+_decl("sdr_l6_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sdr_l6_ch_t),
+                                    C.c_int, c_double_p, C.c_int, C.c_int])
+_decl("sdr_l6_plan_run", C.c_int, [c_void_p])
+_decl("sdr_l6_plan_result", C.c_int, [c_void_p, c_void_p, c_void_p])
+_decl("sdr_l6_plan_corr", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p])
+_decl("sdr_l6_plan_timing", None, [c_void_p, C.c_int])
+_decl("sdr_l6_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
+                                         C.POINTER(C.c_longlong), c_double_p])
+_decl("sdr_l6_plan_free", None, [c_void_p])
+_decl("sdr_csk", C.c_int, [c_void_p, C.c_int, C.c_int, C.c_int, c_double_p, C.c_int, c_void_p,
+                           c_void_p])
 _decl("sdr_raw_gen_lut", None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, c_void_p])
 _decl("sdr_raw_gen_lut8", None, [C.c_int, C.c_double, C.c_double, c_void_p])
 _decl("sdr_raw_plan_new", c_void_p, [C.c_int, c_int_p, C.POINTER(c_void_p)])
@@ -376,6 +394,80 @@ class TrkPlan:
     def __del__(self):
         self.close()
 
+
+# --- L6D / L6E FFT tracking and CSK (src/sdr_ch.c:627-664, 694-720) ----------
+SDR_L6_KEEP_CORR = 1
+
+
+def csk(corr, len_code, pos):
+    """CSK of a batch of correlations (n_item, N): (syms uint8, taps complex64)."""
+    corr = np.ascontiguousarray(np.atleast_2d(corr), np.complex64)
+    pos = np.ascontiguousarray(pos, np.float64)
+    n_item, N = corr.shape
+    syms = np.zeros(n_item, np.uint8)
+    taps = np.zeros((n_item, len(pos)), np.complex64)
+    if not lib.sdr_csk(_ptr(corr), n_item, N, len_code, _dp(pos), len(pos), _ptr(syms), _ptr(taps)):
+        raise ValueError("sdr_csk failed")
+    return syms, taps
+
+
+class L6Plan:
+    """Batched L6D / L6E FFT tracking with CSK decoding (sdr_l6_plan_*).
+    chans: dicts fs, fi, fc_sig, T, N, len_code, fd0, drift, phi0, coff0 and
+    code_fft, the (SDR_N_CODES, N) conjugated code spectra."""
+
+    def __init__(self, buff, chans, pos, n_epoch, keep_corr=False):
+        self._buf = _buff(buff)
+        self._keep = []
+        ca = (sdr_l6_ch_t * len(chans))()
+        for i, c in enumerate(chans):
+            cf = np.ascontiguousarray(c["code_fft"], dtype=np.complex64)
+            self._keep.append(cf)
+            ca[i] = sdr_l6_ch_t(c["fs"], c["fi"], c["fc_sig"], c["T"], c["N"], c["len_code"],
+                                c["fd0"], c["drift"], c["phi0"], c["coff0"], _ptr(cf))
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.n_ch, self.n_epoch, self.n_pos = len(chans), n_epoch, len(self.pos)
+        self.N = chans[0]["N"]
+        self.p = lib.sdr_l6_plan_new(C.byref(self._buf), len(chans), ca, self.n_pos, _dp(self.pos),
+                                     n_epoch, SDR_L6_KEEP_CORR if keep_corr else 0)
+        if not self.p:
+            raise ValueError("sdr_l6_plan_new failed")
+
+    def run(self):
+        if not lib.sdr_l6_plan_run(self.p):
+            raise RuntimeError("sdr_l6_plan_run failed")
+
+    def result(self):
+        syms = np.zeros((self.n_ch, self.n_epoch), np.uint8)
+        taps = np.zeros((self.n_ch, self.n_epoch, self.n_pos), np.complex64)
+        lib.sdr_l6_plan_result(self.p, _ptr(syms), _ptr(taps))
+        return syms, taps
+
+    def corr(self, ch, epoch):
+        out = np.zeros(self.N, np.complex64)
+        if not lib.sdr_l6_plan_corr(self.p, ch, epoch, _ptr(out)):
+            raise ValueError("sdr_l6_plan_corr failed (plan without keep_corr?)")
+        return out
+
+    def timing(self, on=True):
+        lib.sdr_l6_plan_timing(self.p, 1 if on else 0)
+
+    def kernel_ms(self, name):
+        ms = C.c_double(0)
+        cnt = C.c_longlong(0)
+        el = C.c_double(0)
+        if not lib.sdr_l6_plan_kernel_ms(self.p, name.encode(), C.byref(ms), C.byref(cnt),
+                                         C.byref(el)):
+            return None
+        return ms.value, cnt.value, el.value
+
+    def close(self):
+        if self.p:
+            lib.sdr_l6_plan_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.close()
 
 # --- front end: raw data to cpx8 (write_buff, src/sdr_rcv.c:776-936) ----------
 FMT_INT8, FMT_INT8X2, FMT_RAW8, FMT_RAW16, FMT_RAW16I, FMT_RAW32, FMT_CS8, FMT_CS16 = range(1, 9)

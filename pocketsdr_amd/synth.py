@@ -27,7 +27,8 @@ def pack_cpx8(I, Q):
 
 def synth_if(fs, n_samp, sats, seed, bits=2, fi=0.0, chunk=1 << 22):
     """sats: list of dicts with code (int8 chips), T (code period s), tau (s),
-    fd (Hz), drift (Hz/s, optional), cn0 (dB-Hz), fc (carrier Hz), phi (cyc).
+    fd (Hz), drift (Hz/s, optional), cn0 (dB-Hz), fc (carrier Hz), phi (cyc),
+    shift (optional: code delay in entries per code period, cycled).
     Returns the uint8 cpx8 buffer."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = np.empty(n_samp, np.uint8)
@@ -43,7 +44,13 @@ def synth_if(fs, n_samp, sats, seed, bits=2, fi=0.0, chunk=1 << 22):
             # code phase (chips): rate L/T scaled by (1 + fd / fc)
             ph_code = (t - sat["tau"]) * (L / sat["T"]) + \
                 (L / sat["T"]) * (sat["fd"] * t + 0.5 * drift * t * t) / sat["fc"]
-            chips = code[np.floor(ph_code).astype(np.int64) % L]
+            ic = np.floor(ph_code).astype(np.int64)
+            if "shift" in sat:
+                # code-shift keying: period k of the code is delayed by
+                # shift[k mod len] entries
+                sh = np.asarray(sat["shift"], np.int64)
+                ic = ic - sh[np.floor_divide(ic, L) % len(sh)]
+            chips = code[ic % L]
             ph = 2 * np.pi * ((fi + sat["fd"]) * t + 0.5 * drift * t * t + sat.get("phi", 0.0))
             A = np.sqrt(2.0 * 10 ** (sat["cn0"] / 10.0) / fs)
             x += A * chips * np.exp(1j * ph)

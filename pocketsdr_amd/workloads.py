@@ -7,6 +7,9 @@ acq_l1ca      GPS L1 C/A acquisition as pocket_acq runs it
 trk_l1ca      open-loop tracking correlators on one IF stream: per channel
               E/P/L at +/-0.5 chip, the N tap at -120 samples and 40 extra taps
               over +/-2 chips (src/sdr_ch.c:228-231, 804-814).
+trk_l6        QZSS L6D FFT tracking with CSK: 4 ms epochs, 20460-entry TDM
+              code, each code period of a visible satellite delayed by
+              255 - symbol chips (src/sdr_ch.c:636-664, 705-720).
 """
 import numpy as np
 
@@ -102,6 +105,55 @@ def trk_l1ca(seed=2, fs=24e6, n_ch=64, n_epoch=1000, n_vis=8, cn0=(38.0, 50.0),
                           phi0=phi0, coff0=coff0, code=codes.code_bank(cd[p], T, fs, N), prn=p))
     return dict(buff=buff, chans=chans, pos=trk_positions(fs, T, nposx=nposx), n_epoch=n_epoch,
                 truth=sats)
+
+
+L6_FREQ = 1278.75e6
+
+
+def trk_l6(seed=3, fs=24e6, n_ch=16, n_epoch=250, n_vis=4, cn0=(45.0, 50.0), nposx=8,
+           code_fft_fn=None, buff_epochs=None):
+    """Returns dict(buff, chans, pos, n_epoch, len_code, truth).  Channel c
+    tracks L6D PRN 193 + c % 10; the first n_vis PRNs are in the IF with
+    random CSK symbols (truth[prn]["syms"], one per 4 ms code period) and
+    their channels start on the true code phase and Doppler (code phase in
+    the middle half of the period, so the open-loop code offset never
+    wraps).  code_fft_fn(code, T, coff, fs, N) makes one N-point conjugated
+    code spectrum (default: codes.gen_code_fft)."""
+    rng = np.random.default_rng(seed)
+    T, len_code = 4e-3, 20460
+    N = int(fs * T)
+    fn = code_fft_fn or (lambda c, T_, coff, fs_, N_: codes.gen_code_fft(c, T_, coff, fs_, N_, 0))
+    n_buf = N * (buff_epochs or n_epoch + 1)
+    prns = [193 + c % 10 for c in range(n_ch)]
+    cd = {p: codes.gen_code_L6D(p) for p in sorted(set(prns))}
+    vis = sorted(set(prns))[:n_vis]
+    n_per = n_buf // N + 2
+    sats = []
+    for p in vis:
+        syms = rng.integers(0, 256, n_per)
+        sats.append(dict(prn=p, code=cd[p], T=T, fc=L6_FREQ, tau=float(rng.uniform(0.25, 0.75) * T),
+                         fd=float(rng.uniform(-4000, 4000)), drift=0.0,
+                         cn0=float(rng.uniform(*cn0)), phi=float(rng.uniform(0, 1)),
+                         syms=syms, shift=2 * (255 - syms)))
+    truth = {s_["prn"]: s_ for s_ in sats}
+    buff = synth.synth_if(fs, n_buf, sats, seed, bits=2)
+    banks = {p: np.stack([fn(cd[p], T, -k / fs / 10, fs, N) for k in range(10)]).astype(np.complex64)
+             for p in cd}
+    chans = []
+    seen = set()
+    for c, p in enumerate(prns):
+        if p in truth and p not in seen:
+            s_ = truth[p]
+            fd0, coff0, phi0 = s_["fd"], s_["tau"], 0.0
+        else:
+            fd0 = float(rng.uniform(-4000, 4000))
+            coff0 = float(rng.uniform(0, T))
+            phi0 = float(rng.uniform(0, 1))
+        seen.add(p)
+        chans.append(dict(fs=fs, fi=0.0, fc_sig=L6_FREQ, T=T, N=N, len_code=len_code, fd0=fd0,
+                          drift=0.0, phi0=phi0, coff0=coff0, code_fft=banks[p], prn=p))
+    return dict(buff=buff, chans=chans, pos=trk_positions(fs, T, len_code // 2, 0.25, nposx),
+                n_epoch=n_epoch, len_code=len_code, truth=truth)
 
 
 # ---------------------------------------------------------------------------
