@@ -382,17 +382,19 @@ def bench_trk(args, sdr, workloads):
                 correlator_kernel_ms=round(ks[0], 4) if ks else None)
 
 
-L6_CLASSES = ("state", "fwd_col", "fwd_row", "inv_row", "inv_col", "csk")
+L6_CLASSES = ("state", "fwd_col", "row", "fwd_row", "inv_row", "inv_col", "csk")
 
 
 def bench_l6(args, sdr, orc, workloads):
     """QZSS L6D FFT tracking with CSK: n_ch channels at 24 Msps, 4 ms epochs
     (N = M = 96000), one second of IF (250 epochs) per run, IF and code
     spectra resident in HBM.  Algorithmic bytes per (channel, epoch) item:
-    fwd column N IF bytes + 8M, fwd row 16M, inv row 16M (X, out), inv
-    column 16M, CSK ~2 x 511 x 16 B; the code spectra (SDR_N_CODES per
-    channel, 8M each) count once per run.  CPU: the oracle's L6 tracking loop
-    (PocketFFT, one thread) over the epochs of one channel."""
+    fwd column N IF bytes + 8M written, fused row pass 16M (read, write),
+    inv column 8M read + the stored lag window (16 win, |lag| < win =
+    255 R + max |pos| + 3, R = N / 10230), CSK 2 x 511 x 16 B; the code
+    spectra (SDR_N_CODES per channel, 8M each) count once per run.  CPU: the
+    oracle's L6 tracking loop (PocketFFT, one thread) over the epochs of one
+    channel."""
     w = workloads.trk_l6(seed=args.seed + 2, fs=24e6, n_ch=args.l6_channels, n_epoch=args.l6_epochs)
     plan = sdr.L6Plan(w["buff"], w["chans"], w["pos"], w["n_epoch"])
     t = time_plan(plan, max(1, args.steps // 2), 1)
@@ -408,7 +410,9 @@ def bench_l6(args, sdr, orc, workloads):
     plan.close()
     n_ch, N, ne = len(w["chans"]), w["chans"][0]["N"], w["n_epoch"]
     items = n_ch * ne
-    per_item = 57.0 * N + 2 * 511 * 16
+    win = int(np.floor(255 * N / (w["len_code"] // 2) + np.abs(w["pos"]).max())) + 3
+    row = 16.0 * N if ks.get("row") else 32.0 * N
+    per_item = N + 8.0 * N + row + 8.0 * N + 16.0 * win + 2 * 511 * 16
     alg = items * per_item + n_ch * 10 * 8.0 * N
     gbs = alg / t / 1e9
     res = dict(config=dict(n_ch=n_ch, fs=24e6, N=N, n_epoch=ne, n_pos=len(w["pos"]),

@@ -265,7 +265,8 @@ int sdr_l6_plan_result(const sdr_l6_plan_t *plan, uint8_t *syms,
 int sdr_l6_plan_corr(const sdr_l6_plan_t *plan, int ch, int epoch,
     sdr_cpx_t *corr);
 void sdr_l6_plan_timing(sdr_l6_plan_t *plan, int on);
-/* kernel classes: "state", "fwd_col", "fwd_row", "inv_row", "inv_col", "csk" */
+/* kernel classes: "state", "fwd_col", "row" (fused row pass), or "fwd_row" and
+ * "inv_row" (row lengths without a fused kernel), "inv_col", "csk" */
 int sdr_l6_plan_kernel_ms(const sdr_l6_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);
 void sdr_l6_plan_free(sdr_l6_plan_t *plan);

@@ -331,22 +331,17 @@ __device__ __forceinline__ void last_stage(const PassArgs &a, Storer<SM> &st, in
     }
 }
 
-// one item of the specialised pass: first stage from global, middle stages
-// in LDS, last stage to global
-template <int LM, int SM, int CL, int CG>
-__device__ __forceinline__ void pass_item(const PassArgs &a, int b, int g0, int gcount, float2 *buf,
-                                          const uint32_t *carr, float sgn, PeakPart *red, long long pslot)
+// first stage (Ns = 1, no twiddles): global -> registers -> LDS
+template <int LM, int CL, int CG>
+__device__ __forceinline__ void first_stage(const PassArgs &a, const Loader<LM> &ld, int g0, int gcount,
+                                            float2 *buf, float sgn)
 {
-    const float2 *tw = a.tw;
     constexpr RadPlan P = make_rad_plan(CL);
     constexpr int LP = CL + 1;
     constexpr int R0 = P.r[0];
     constexpr int Lr0 = CL / R0;
-    Loader<LM> ld;
-    ld.init(a, b, carr);
     const int igs = (int)a.in_gstride, ins = (int)a.in_nstride;
     const bool in_nfast = (ins == 1);
-    // ---- first stage (Ns = 1, no twiddles): global -> registers -> LDS ----
     {
         constexpr int NB = CG * Lr0;
         constexpr int PER = (NB + PASS_THREADS - 1) / PASS_THREADS;
@@ -374,6 +369,19 @@ __device__ __forceinline__ void pass_item(const PassArgs &a, int b, int g0, int 
         }
         __syncthreads();
     }
+}
+
+// one item of the specialised pass: first stage from global, middle stages
+// in LDS, last stage to global
+template <int LM, int SM, int CL, int CG>
+__device__ __forceinline__ void pass_item(const PassArgs &a, int b, int g0, int gcount, float2 *buf,
+                                          const uint32_t *carr, float sgn, PeakPart *red, long long pslot)
+{
+    const float2 *tw = a.tw;
+    constexpr int R0 = make_rad_plan(CL).r[0];
+    Loader<LM> ld;
+    ld.init(a, b, carr);
+    first_stage<LM, CL, CG>(a, ld, g0, gcount, buf, sgn);
     // ---- middle stages in place ----
     middle_stages<CL, CG, 1, R0>(buf, tw, sgn);
     // ---- last stage ----
@@ -405,6 +413,54 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
         pslot = row * ngroups + grp;
     }
     pass_item<LM, SM, CL, CG>(a, b, g0, gcount, buf, carr, sgn, red, pslot);
+}
+
+// Fused correlator row pass for one code spectrum per item (the FFT
+// tracking of L6D / L6E): per workgroup CG rows of the forward column
+// pass's output go through the forward length-M2 transforms, the product
+// with the item's code spectrum (sdr_cpx_mul arithmetic and scale, spectrum
+// cf_index[b], or b / prod_cdiv) and the inverse length-M2 transforms with
+// the inverse inter-pass twiddle on store, all in one LDS buffer: X and the
+// product never go through HBM.  The output is the inverse row pass's Z.
+template <int CL, int CG>
+__global__ void __launch_bounds__(PASS_THREADS) xcorr_row_kernel(PassArgs a, int ngroups)
+{
+    static_assert(make_rad_plan(CL).n >= 2, "specialised lengths need at least two radix stages");
+    constexpr RadPlan P = make_rad_plan(CL);
+    constexpr int LP = CL + 1;
+    constexpr int RL = P.r[P.n - 1];
+    __shared__ float2 buf[CG * LP];
+    const int b = (int)(blockIdx.x / (unsigned)ngroups);
+    const int grp = (int)blockIdx.x - b * ngroups;
+    const int g0 = grp * CG;
+    const int gcount = min(CG, a.nsub - g0);
+    // forward transforms, natural order in LDS
+    Loader<LD_CPX> ld;
+    ld.init(a, b, nullptr);
+    first_stage<LD_CPX, CL, CG>(a, ld, g0, gcount, buf, -1.0f);
+    middle_stages<CL, CG, 1, P.r[0]>(buf, a.tw, -1.0f);
+    stage_inplace<RL, CL, CG, CL / RL>(buf, a.tw, -1.0f);
+    // X * Cf at transposed index (g0 + g) M2 + k
+    {
+        const long long cb = a.cf_index ? a.cf_index[b] : b / a.prod_cdiv;
+        const float2 *__restrict__ cin = a.cf + cb * a.cf_bstride;
+        const bool tailm = a.prod_V < a.M;
+        for (int idx = threadIdx.x; idx < CG * CL; idx += PASS_THREADS) {
+            const int g = idx / CL, k = idx - g * CL;
+            if (g < gcount) {
+                const int e = (g0 + g) * CL + k;
+                bool tail = false;
+                if (tailm) tail = (e / a.prod_M2 + a.prod_M1 * (e % a.prod_M2)) >= a.prod_V;
+                buf[g * LP + k] = prod_value(buf[g * LP + k], cin[e], tail, a.prod_scale);
+            }
+        }
+        __syncthreads();
+    }
+    // inverse transforms: stages 0 .. n-2 in LDS, the last one to global
+    middle_stages<CL, CG, 0, 1>(buf, a.tw_inv, 1.0f);
+    Storer<ST_CPX> st;
+    st.init(a, b);
+    last_stage<ST_CPX, CL, CG>(a, st, g0, gcount, buf, 1.0f, a.tw_inv);
 }
 
 // ---------------------------------------------------------------------------
@@ -599,6 +655,31 @@ static void launch_modes(LoadMode lm, StoreMode sm, const PassArgs &a, const Rad
     PSA_LAUNCH(LD_PROD, ST_POW)
 #undef PSA_LAUNCH
     PSA_FAIL("unsupported FFT pass mode %d/%d", (int)lm, (int)sm);
+}
+
+template <int CL, int CG>
+static void launch_xrow_t(const PassArgs &a, int nbatch, hipStream_t st)
+{
+    const int ngroups = (a.nsub + CG - 1) / CG;
+    const long long nblk = (long long)ngroups * nbatch;
+    if (nblk <= 0) return;
+    if (nblk > 0x7FFFFFFFLL) PSA_FAIL("FFT launch too large (%lld workgroups)", nblk);
+    pass_grid = nblk * PASS_THREADS;
+    hipLaunchKernelGGL((xcorr_row_kernel<CL, CG>), dim3((unsigned)nblk), dim3(PASS_THREADS), 0, st, a, ngroups);
+    PSA_CHECK(hipGetLastError());
+}
+
+bool launch_xcorr_row(const PassArgs &a_in, int nbatch, hipStream_t st)
+{
+    PassArgs a = a_in;
+    if ((long long)a.M * 2 > 0x7FFFFFFFLL) PSA_FAIL("FFT length too large");
+    if (a.in_nstride != 1 || a.out_nstride != 1 || a.in_gstride != a.L || a.out_gstride != a.L) {
+        PSA_FAIL("fused correlator row pass: rows must be contiguous");
+    }
+#define PSA_X(L_, G_) if (a.L == L_) { a.G = G_; launch_xrow_t<L_, G_>(a, nbatch, st); return true; }
+    PSA_SPECS(PSA_X)
+#undef PSA_X
+    return false;
 }
 
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hipStream_t st)

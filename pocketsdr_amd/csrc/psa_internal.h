@@ -142,6 +142,9 @@ struct PassArgs {
     const float2 *in;
     float2 *out;
     long long in_bstride, out_bstride;
+    int out_win;                // ST_CPX: > 0 stores only the lags -out_win ... out_win - 1
+                                // of a natural-order output, at index lag + out_win
+    const float2 *tw_inv;       // fused correlator row pass: inverse sub-transform twiddles
     // LD_MIX: IF ring buffer and per-batch mixing parameters
     const uint32_t *carr;       // carrier table (set by launch_pass)
     const uint8_t *buf;
@@ -195,6 +198,9 @@ extern thread_local long long pass_grid;
 
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,
                  hipStream_t st);
+// fused forward row pass + code-spectrum product + inverse row pass
+// (specialised row lengths only; returns false for other lengths)
+bool launch_xcorr_row(const PassArgs &a, int nbatch, hipStream_t st);
 int pass_ngroups(const PassArgs &a);   // workgroups per batch item
 int pass_groups(int L, int nsub);
 bool spec_length(int L);          // L has a compile-time specialised kernel
@@ -278,6 +284,18 @@ inline void args_inv_col(const FFTPlan *pl, PassArgs &a)
 {
     col_pass_args(pl, 1, a);   // natural-order output n_a + M2 n_b
     a.thi = a.tlo = nullptr;
+}
+
+// the fused row pass: forward row geometry, inverse twiddles, the inverse
+// row pass's inter-pass twiddle on store
+inline void args_xcorr_row(const FFTPlan *pl, PassArgs &a)
+{
+    row_pass_args(pl, 0, a);
+    a.tw_inv = pl->s2.tw_bwd;
+    if (pl->M2 > 1) {
+        a.thi = pl->thi_bwd;
+        a.tlo = pl->tlo_bwd;
+    }
 }
 
 inline const FFTPlan *need_plan(int M, const char *who)

@@ -13,13 +13,16 @@
 //                     mixing parameters and spectrum index per item
 //   fwd column pass : IF bytes, carrier mix (LUT semantics), * CSCALE,
 //                     length-M1 FFTs, twiddle                     -> A
-//   fwd row pass    : length-M2 FFTs                              -> B
-//   inv row pass    : (B * Cf[index]) / N^2 (sdr_cpx_mul arithmetic),
-//                     length-M2 inverse FFTs, twiddle             -> A
-//   inv column pass : length-M1 inverse FFTs                      -> corr
+//   fused row pass  : length-M2 FFTs, * Cf[index] / N^2 (sdr_cpx_mul
+//                     arithmetic), length-M2 inverse FFTs, twiddle -> B
+//   inv column pass : length-M1 inverse FFTs; only the lags CSK reads
+//                     (|lag| < win) are stored, unless every
+//                     correlation is kept                         -> A
 //   csk_kernel      : 511 interpolated powers, first maximum, symbol, taps
-// The FFT passes are those of the batched search (psa_fft.hip), with the
-// code spectrum chosen per item through PassArgs::cf_index.
+// The FFT passes are those of the batched search (psa_fft.hip) with the
+// code spectrum chosen per item through PassArgs::cf_index; for row lengths
+// without a specialised kernel the row pass runs as the separate forward
+// and inverse passes.
 #include <math.h>
 #include <string.h>
 #include <string>
@@ -70,11 +73,22 @@ __global__ void __launch_bounds__(TRK_SCHUNK) l6_state_kernel(const TrkChDev *__
     });
 }
 
+// position of lag k (0 <= k < N, natural order) in a correlation stored
+// whole (win = 0) or as its lags -win ... win-1 at lag + win; the plan sizes
+// win to hold every lag CSK reads, the clamp only keeps a bad tap position
+// inside the buffer
+__device__ __forceinline__ int csk_slot(int k, int N, int win)
+{
+    if (win == 0) return k;
+    const int w = k < win ? k + win : k - (N - win);
+    return min(max(w, 0), 2 * win - 1);
+}
+
 // interp_corr (src/sdr_ch.c:627-633) on the correlation rotated so that lag
 // 0 sits at index n: C[k] = corr[(k - n) mod N], the reference's copy for
 // 0 <= k < 2n and its circular extension outside.  The reference build
 // contracts a0 C[i] + a1 C[i+1] to fma(a0, C[i], a1 C[i+1]) in double.
-__device__ __forceinline__ float2 csk_interp(const float2 *__restrict__ corr, int N, int n, double x)
+__device__ __forceinline__ float2 csk_interp(const float2 *__restrict__ corr, int N, int win, int n, double x)
 {
 #pragma clang fp contract(off)
     const int i = (int)x;                        // truncation toward zero, as (int)
@@ -82,7 +96,7 @@ __device__ __forceinline__ float2 csk_interp(const float2 *__restrict__ corr, in
     int k0 = (int)(((long long)i - n) % N);
     if (k0 < 0) k0 += N;
     const int k1 = k0 + 1 == N ? 0 : k0 + 1;
-    const float2 c0 = corr[k0], c1 = corr[k1];
+    const float2 c0 = corr[csk_slot(k0, N, win)], c1 = corr[csk_slot(k1, N, win)];
     return make_float2((float)fma(a0, (double)c0.x, a1 * (double)c1.x),
                        (float)fma(a0, (double)c0.y, a1 * (double)c1.y));
 }
@@ -97,7 +111,8 @@ __device__ __forceinline__ float2 csk_interp(const float2 *__restrict__ corr, in
 // (P <= P_max fails for it), so a wave that sees one replays the reference
 // loop over the 511 powers on lane 0.
 __global__ void __launch_bounds__(CSK_THREADS) csk_kernel(const float2 *__restrict__ corr_all, long long corr_stride,
-                                                         int N, int n, double R, const double *__restrict__ pos,
+                                                         int N, int win, int n, double R,
+                                                         const double *__restrict__ pos,
                                                          int n_pos, uint8_t *__restrict__ syms,
                                                          float2 *__restrict__ taps)
 {
@@ -114,7 +129,7 @@ __global__ void __launch_bounds__(CSK_THREADS) csk_kernel(const float2 *__restri
     for (int k = 0; k < 8; k++) {
         const int i = -255 + lane + 64 * k;
         if (i <= 255) {
-            const float2 c = csk_interp(corr, N, n, fma((double)i, R, dn));
+            const float2 c = csk_interp(corr, N, win, n, fma((double)i, R, dn));
             const float P = __fsqrt_rn(__fmaf_rn(c.x, c.x, __fmul_rn(c.y, c.y)));
             sP[i + 255] = P;
             nan |= P != P;
@@ -147,7 +162,7 @@ __global__ void __launch_bounds__(CSK_THREADS) csk_kernel(const float2 *__restri
     if (lane == 0) syms[item] = (uint8_t)(255 - ix);
     const double base = fma((double)ix, R, dn);
     for (int t = lane; t < n_pos; t += CSK_THREADS) {
-        taps[item * n_pos + t] = csk_interp(corr, N, n, base + pos[t]);
+        taps[item * n_pos + t] = csk_interp(corr, N, win, n, base + pos[t]);
     }
 }
 
@@ -158,19 +173,32 @@ static void csk_geometry(int N, int len_code, double *R, int *n)
     *n = (int)(280 * *R);
 }
 
-static void launch_csk(const float2 *corr, long long stride, int n_item, int N, int len_code,
+// half-width of the lag window that holds every lag CSK reads: |lag| <=
+// 255 R + max |pos| + 2 (0: the window would not be smaller than N)
+static int csk_window(int N, int len_code, const double *pos, int n_pos)
+{
+    double R;
+    int n;
+    csk_geometry(N, len_code, &R, &n);
+    double pm = 0.0;
+    for (int i = 0; i < n_pos; i++) pm = std::max(pm, fabs(pos[i]));
+    const double w = floor(255.0 * R + pm) + 3.0;
+    return 2.0 * w < (double)N ? (int)w : 0;
+}
+
+static void launch_csk(const float2 *corr, long long stride, int win, int n_item, int N, int len_code,
                        const double *pos, int n_pos, uint8_t *syms, float2 *taps, hipStream_t st)
 {
     double R;
     int n;
     csk_geometry(N, len_code, &R, &n);
-    hipLaunchKernelGGL(csk_kernel, dim3((unsigned)n_item), dim3(CSK_THREADS), 0, st, corr, stride, N, n, R,
-                       pos, n_pos, syms, taps);
+    hipLaunchKernelGGL(csk_kernel, dim3((unsigned)n_item), dim3(CSK_THREADS), 0, st, corr, stride, N, win, n,
+                       R, pos, n_pos, syms, taps);
     PSA_CHECK(hipGetLastError());
 }
 
-enum { L6_STATE = 0, L6_FWD_COL, L6_FWD_ROW, L6_INV_ROW, L6_INV_COL, L6_CSK, L6_NCLS };
-static const char *l6_names[L6_NCLS] = {"state", "fwd_col", "fwd_row", "inv_row", "inv_col", "csk"};
+enum { L6_STATE = 0, L6_FWD_COL, L6_ROW, L6_FWD_ROW, L6_INV_ROW, L6_INV_COL, L6_CSK, L6_NCLS };
+static const char *l6_names[L6_NCLS] = {"state", "fwd_col", "row", "fwd_row", "inv_row", "inv_col", "csk"};
 
 }  // namespace psa
 
@@ -178,6 +206,8 @@ using namespace psa;
 
 struct sdr_l6_plan_tag {
     int n_ch = 0, n_pos = 0, n_epoch = 0, N = 0, buffN = 0, len_code = 0, flags = 0;
+    int win = 0;                    // stored lag window of the correlations (0: whole)
+    bool fused = false;             // fused row pass (specialised row length)
     long long n_item = 0;
     int chunk = 0;                  // items per batch of FFT passes
     const FFTPlan *pl = nullptr;
@@ -303,6 +333,8 @@ sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch
     PSA_CHECK(hipMalloc(&p->A, (size_t)p->chunk * item_bytes));
     PSA_CHECK(hipMalloc(&p->B, (size_t)p->chunk * item_bytes));
     if (flags & SDR_L6_KEEP_CORR) PSA_CHECK(hipMalloc(&p->corr, ni * item_bytes));
+    else p->win = csk_window(N, len_code, pos, n_pos);
+    p->fused = spec_length(pl->M2) && !getenv("PSA_L6_UNFUSED");
     PSA_CHECK(hipMalloc(&p->d_sym, ni));
     PSA_CHECK(hipMalloc(&p->d_taps, sizeof(float2) * ni * std::max(n_pos, 1)));
     return p;
@@ -333,35 +365,52 @@ static void l6_run(sdr_l6_plan_t *p)
             L6Tick t(p, L6_FWD_COL);
             launch_pass(LD_MIX, ST_CPX, a, nb, st);
         }
-        PassArgs r = {};
-        args_fwd_row(pl, r);
-        r.in = p->A; r.out = p->B; r.in_bstride = r.out_bstride = M;
-        {
-            L6Tick t(p, L6_FWD_ROW);
-            launch_pass(LD_CPX, ST_CPX, r, nb, st);
+        float2 *Z, *spare;                       // inverse row output, free buffer
+        if (p->fused) {
+            PassArgs x = {};
+            args_xcorr_row(pl, x);
+            x.in = p->A; x.in_bstride = M; x.out = p->B; x.out_bstride = M;
+            x.cf = p->d_cf; x.cf_bstride = M; x.cf_index = p->it.cf + i0; x.prod_cdiv = 1;
+            set_prod(pl, x, 1.0f / M / M);        // sdr_corr_fft: 1.0f / N / N
+            L6Tick t(p, L6_ROW);
+            if (!launch_xcorr_row(x, nb, st)) PSA_FAIL("sdr_l6_plan_run: no fused row pass for M2 = %d", pl->M2);
+            Z = p->B;
+            spare = p->A;
+        } else {
+            PassArgs r = {};
+            args_fwd_row(pl, r);
+            r.in = p->A; r.out = p->B; r.in_bstride = r.out_bstride = M;
+            {
+                L6Tick t(p, L6_FWD_ROW);
+                launch_pass(LD_CPX, ST_CPX, r, nb, st);
+            }
+            PassArgs ir = {};
+            args_inv_row(pl, ir);
+            ir.in = p->B; ir.in_bstride = M;
+            ir.cf = p->d_cf; ir.cf_bstride = M; ir.cf_index = p->it.cf + i0;
+            ir.prod_xdiv = 1; ir.prod_xmod = nb; ir.prod_cdiv = 1;
+            set_prod(pl, ir, 1.0f / M / M);
+            ir.out = p->A; ir.out_bstride = M;
+            {
+                L6Tick t(p, L6_INV_ROW);
+                launch_pass(LD_PROD, ST_CPX, ir, nb, st);
+            }
+            Z = p->A;
+            spare = p->B;
         }
-        PassArgs ir = {};
-        args_inv_row(pl, ir);
-        ir.in = p->B; ir.in_bstride = M;
-        ir.cf = p->d_cf; ir.cf_bstride = M; ir.cf_index = p->it.cf + i0;
-        ir.prod_xdiv = 1; ir.prod_xmod = nb; ir.prod_cdiv = 1;
-        set_prod(pl, ir, 1.0f / M / M);          // sdr_corr_fft: 1.0f / N / N
-        ir.out = p->A; ir.out_bstride = M;
-        {
-            L6Tick t(p, L6_INV_ROW);
-            launch_pass(LD_PROD, ST_CPX, ir, nb, st);
-        }
-        float2 *corr = p->corr ? p->corr + i0 * M : p->B;
+        // correlations: all kept, or the lag window CSK reads
+        float2 *corr = p->corr ? p->corr + i0 * M : spare;
+        const long long cstride = p->win > 0 ? 2LL * p->win : M;
         PassArgs ic = {};
         args_inv_col(pl, ic);
-        ic.in = p->A; ic.in_bstride = M; ic.out = corr; ic.out_bstride = M;
+        ic.in = Z; ic.in_bstride = M; ic.out = corr; ic.out_bstride = cstride; ic.out_win = p->win;
         {
             L6Tick t(p, L6_INV_COL);
             launch_pass(LD_CPX, ST_CPX, ic, nb, st);
         }
         {
             L6Tick t(p, L6_CSK);
-            launch_csk(corr, M, nb, p->N, p->len_code, p->d_pos, p->n_pos, p->d_sym + i0,
+            launch_csk(corr, cstride, p->win, nb, p->N, p->len_code, p->d_pos, p->n_pos, p->d_sym + i0,
                        p->d_taps + i0 * p->n_pos, st);
         }
     }
@@ -382,7 +431,10 @@ int sdr_l6_plan_run(sdr_l6_plan_t *p)
             PSA_CHECK(hipEventElapsedTime(&ms, p->evs[i].a, p->evs[i].b));
             p->ms[p->evs[i].cls] += ms;
         }
+        bool seen[L6_NCLS] = {};
+        for (size_t i = 0; i < p->ev_used; i++) seen[p->evs[i].cls] = true;
         for (int c = 0; c < L6_NCLS; c++) {
+            if (!seen[c]) continue;
             p->cnt[c]++;
             p->elems[c] += c == L6_STATE ? (double)p->n_item : (double)p->n_item * M;
         }
@@ -463,7 +515,7 @@ int sdr_csk(const sdr_cpx_t *corr, int n_item, int N, int len_code, const double
     uint8_t *ds = dm + tb + sizeof(double) * (size_t)n_pos;
     PSA_CHECK(hipMemcpyAsync(dc, corr, cb, hipMemcpyHostToDevice, st));
     if (n_pos > 0) PSA_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * n_pos, hipMemcpyHostToDevice, st));
-    launch_csk(dc, N, n_item, N, len_code, dp, n_pos, ds, dt, st);
+    launch_csk(dc, N, 0, n_item, N, len_code, dp, n_pos, ds, dt, st);
     PSA_CHECK(hipMemcpyAsync(syms, ds, (size_t)n_item, hipMemcpyDeviceToHost, st));
     if (n_pos > 0) PSA_CHECK(hipMemcpyAsync(C, dt, tb, hipMemcpyDeviceToHost, st));
     PSA_CHECK(hipStreamSynchronize(st));

@@ -23,6 +23,24 @@ __device__ __forceinline__ float2 mul_i(float2 a, float sgn) { return make_float
 // ---------------------------------------------------------------------------
 // element loaders / storers of the pass kernels
 // ---------------------------------------------------------------------------
+// The product of sdr_cpx_mul (src/sdr_func.c:197-233) times its scale s:
+// the AVX2 lanes compute re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) +
+// fl(ai br) (src/sdr_func.c:203-213); the last M % 4 natural indices (tail)
+// take the scalar form, contracted by the reference build
+// (src/sdr_func.c:229-232).
+__device__ __forceinline__ float2 prod_value(float2 x, float2 c, bool tail, float s)
+{
+    float re, im;
+    if (!tail) {
+        re = x.x * c.x + x.y * -c.y;
+        im = x.x * c.y + x.y * c.x;
+    } else {
+        re = __builtin_fmaf(x.x, c.x, -(x.y * c.y));
+        im = __builtin_fmaf(x.x, c.y, x.y * c.x);
+    }
+    return make_float2(re * s, im * s);
+}
+
 template <int LM>
 struct Loader {
     const float2 *pin = nullptr;
@@ -87,22 +105,10 @@ struct Loader {
             const int2 m = mix_int(e);
             return make_float2((float)m.x * PSA_CSCALE, (float)m.y * PSA_CSCALE);
         }
-        // LD_PROD: sdr_cpx_mul AVX2 lanes (src/sdr_func.c:203-213):
-        // re = fl(ar br) + fl(ai (-bi)), im = fl(ar bi) + fl(ai br), * s; the
-        // last M % 4 natural indices take the scalar tail form
-        // (src/sdr_func.c:229-232, contracted by the reference build)
-        const float2 x = xin[e], c = cin[e];
+        // LD_PROD: X * Cf with the sdr_cpx_mul arithmetic
         bool tail = false;
         if (prod_tail) tail = (e / prod_M2 + prod_M1 * (e % prod_M2)) >= prod_V;
-        float re, im;
-        if (!tail) {
-            re = x.x * c.x + x.y * -c.y;
-            im = x.x * c.y + x.y * c.x;
-        } else {
-            re = __builtin_fmaf(x.x, c.x, -(x.y * c.y));
-            im = __builtin_fmaf(x.x, c.y, x.y * c.x);
-        }
-        return make_float2(re * prod_scale, im * prod_scale);
+        return prod_value(xin[e], cin[e], tail, prod_scale);
     }
 };
 
@@ -112,6 +118,7 @@ struct Storer {
     float *prow = nullptr;
     const float2 *thi = nullptr, *tlo = nullptr;
     int nlag = 0, ogs = 0, ons = 0;
+    int win = 0, wM = 0;            // ST_CPX lag window (0: every index)
     bool pinit = false, track = false;
     long long ibase = 0;            // row-major index of lag 0 of this row
     float bm = 0.0f;                // peak tracking of the values stored:
@@ -122,7 +129,11 @@ struct Storer {
     {
         thi = a.thi; tlo = a.tlo;
         ogs = (int)a.out_gstride; ons = (int)a.out_nstride;
-        if (SM == ST_CPX) pout = a.out + (long long)b * a.out_bstride;
+        if (SM == ST_CPX) {
+            pout = a.out + (long long)b * a.out_bstride;
+            win = a.out_win;
+            wM = a.M;
+        }
         if (SM == ST_POW) {
             const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
             prow = a.P + row * a.p_bstride;
@@ -143,7 +154,13 @@ struct Storer {
         }
         const int e = gg * ogs + k * ons;
         if (SM == ST_CPX) {
-            pout[e] = v;
+            if (win == 0) {
+                pout[e] = v;
+            } else {
+                // lags -win ... win-1 only, at e + win (mod M)
+                const int w = e < win ? e + win : e - (wM - win);
+                if (w >= 0 && w < 2 * win) pout[w] = v;
+            }
         } else if (e < nlag) {
             // reference build: P + fmaf(re, re, im * im) (src/sdr_func.c:678-680);
             // on the first block P is zero and 0 + x = x exactly

@@ -161,11 +161,13 @@ def l6_workload(orc, fs, n_ch, n_epoch, n_vis, buff_epochs=None, seed=3):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", ["6M", "12M-chunks", "wrap"])
+@pytest.mark.parametrize("variant", ["6M", "12M-chunks", "wrap", "unfused"])
 def test_l6_plan(sdr, orc, variant, monkeypatch):
     fs = 12e6 if variant == "12M-chunks" else 6e6
     if variant == "12M-chunks":
         monkeypatch.setenv("PSA_L6_CHUNK", "7")     # several batches and a remainder
+    if variant == "unfused":
+        monkeypatch.setenv("PSA_L6_UNFUSED", "1")   # separate forward and inverse row passes
     w = l6_workload(orc, fs, n_ch=4, n_epoch=9, n_vis=2,
                     buff_epochs=(5 if variant == "wrap" else None))
     buff = w["buff"]
@@ -192,6 +194,14 @@ def test_l6_plan(sdr, orc, variant, monkeypatch):
             else:
                 assert rel(taps[c, e], rt[e]) < 1e-4, (c, e)
     plan.close()
+    # without kept correlations only the lag window CSK reads is stored:
+    # the same symbols and taps, bit for bit
+    plan2 = sdr.L6Plan(buff, w["chans"], w["pos"], w["n_epoch"])
+    plan2.run()
+    s2, t2 = plan2.result()
+    assert np.array_equal(s2, syms)
+    assert np.array_equal(t2.view(np.uint32), taps.view(np.uint32))
+    plan2.close()
 
 
 @pytest.mark.gpu
