@@ -222,6 +222,10 @@ int sdr_trk_plan_run(sdr_trk_plan_t *plan);
  * (correlations before / after the code wrap, as sdr_corr_std's C). */
 int sdr_trk_plan_result(const sdr_trk_plan_t *plan, sdr_cpx_t *corr,
     sdr_cpx_t *C);
+/* P: n_ch x n_epoch full-period prompts, the trk->P history entries of
+ * src/sdr_ch.c:736-740 ((C[0] + previous epoch's C[1]) / N, zero before the
+ * first epoch). */
+int sdr_trk_plan_prompt(const sdr_trk_plan_t *plan, sdr_cpx_t *P);
 void sdr_trk_plan_timing(sdr_trk_plan_t *plan, int on);
 int sdr_trk_plan_kernel_ms(const sdr_trk_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);

@@ -58,6 +58,7 @@ _d("orc_gen_code_fft_f64", None, [vp, C.c_int, C.c_double, C.c_double, C.c_doubl
                                   C.c_int, vp])
 _d("orc_gen_code_L1CA", C.c_int, [C.c_int, vp])
 _d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, C.c_int, vp, vp, dp])
+_d("orc_trk_stitch", None, [vp, C.c_int, C.c_int, vp])
 _d("orc_csk", C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, vp])
 _d("orc_trk_l6_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, vp, vp, C.c_int, C.c_int, dp, C.c_int,
                             vp, vp, vp, dp])
@@ -218,6 +219,14 @@ def trk_run(ch, code_bank_arr, buff, n_epoch, pos):
     lib.orc_trk_run(C.byref(c), _p(code_bank_arr), _p(buff), len(buff), n_epoch,
                     pos.ctypes.data_as(dp), len(pos), _p(corr), _p(Cc), state.ctypes.data_as(dp))
     return corr, Cc, state
+
+
+def trk_stitch(Cc, N):
+    """Prompt history (src/sdr_ch.c:736-740) from trk_run's C (n_epoch, 2)."""
+    Cc = np.ascontiguousarray(Cc, np.complex64)
+    P = np.zeros(len(Cc), np.complex64)
+    lib.orc_trk_stitch(_p(Cc), len(Cc), N, _p(P))
+    return P
 
 
 def csk(corr, len_code, pos):

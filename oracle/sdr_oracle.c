@@ -561,6 +561,22 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
 
 
 /* ---------------------------------------------------------------------------
+ * prompt history, src/sdr_ch.c:736-740: P[e] = (C[e][0] + C[e-1][1]) / N in
+ * float, with trk->C1 = 0 before the first epoch (src/sdr_ch.c:368).  C holds
+ * n_epoch x 2 cpx (orc_trk_run's C), P n_epoch cpx.
+ */
+void orc_trk_stitch(const float *C, int n_epoch, int N, float *P)
+{
+    float prev[2] = {0.0f, 0.0f};
+    for (int e = 0; e < n_epoch; e++) {
+        for (int i = 0; i < 2; i++) {
+            P[2 * e + i] = (C[4 * e + i] + prev[i]) / N;
+            prev[i] = C[4 * e + 2 + i];
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
  * L6D / L6E CSK decoding, src/sdr_ch.c:627-664 (interp_corr, CSK).
  * R = N / (len_code / 2) samples per chip, n = (int)(280 R); the reference
  * copies C[k] = corr[(k - n) mod N] for 0 <= k < 2n (lag 0 at index n).

@@ -101,6 +101,10 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
                  const double *pos, int n_pos, float *corr, float *C,
                  double *state);
 
+/* Prompt history of the standard correlator branch, src/sdr_ch.c:736-740:
+ * P[e] = (C[e][0] + C[e-1][1]) / N, C[-1][1] = 0 (cpx arrays). */
+void orc_trk_stitch(const float *C, int n_epoch, int N, float *P);
+
 /* L6D / L6E CSK, src/sdr_ch.c:627-664: returns the chip shift ix of the
  * peak, C receives the n_pos interpolated taps (cpx). */
 int orc_csk(const float *corr, int N, int len_code, const double *pos, int n_pos,

@@ -504,6 +504,22 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     }
 }
 
+// Prompt history of track_sig (src/sdr_ch.c:736-740): the full-period prompt
+// of epoch e joins this epoch's correlation before the code wrap with the
+// previous epoch's after it, (C1[0] + trk->C1) / N in float, trk->C1 = 0
+// before the first epoch (src/sdr_ch.c:368).
+__global__ void trk_stitch_kernel(const float2 *__restrict__ C, long long n_item, int n_epoch, int N,
+                                  float2 *__restrict__ P)
+{
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_item) return;
+    const int e = (int)(o % n_epoch);
+    const float2 a = C[2 * o];
+    const float2 b = e > 0 ? C[2 * o - 1] : make_float2(0.0f, 0.0f);
+    const float fN = (float)N;
+    P[o] = make_float2(__fdiv_rn(__fadd_rn(a.x, b.x), fN), __fdiv_rn(__fadd_rn(a.y, b.y), fN));
+}
+
 }  // namespace psa
 
 using namespace psa;
@@ -518,7 +534,7 @@ struct sdr_trk_plan_tag {
     TrkChDev *d_ch = nullptr;
     double *d_pos = nullptr;
     TrkEpoch *d_ep = nullptr;
-    float2 *d_corr = nullptr, *d_C = nullptr;
+    float2 *d_corr = nullptr, *d_C = nullptr, *d_P = nullptr;
     hipStream_t st = nullptr;
     bool timing = false;
     hipEvent_t ev[4];
@@ -663,6 +679,7 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
     PSA_CHECK(hipMalloc(&p->d_ep, sizeof(TrkEpoch) * ne));
     PSA_CHECK(hipMalloc(&p->d_corr, sizeof(float2) * ne * n_pos));
     PSA_CHECK(hipMalloc(&p->d_C, sizeof(float2) * ne * 2));
+    PSA_CHECK(hipMalloc(&p->d_P, sizeof(float2) * ne));
     for (auto &e : p->ev) PSA_CHECK(hipEventCreate(&e));
     return p;
 }
@@ -706,6 +723,9 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
     const size_t shm = sizeof(uint32_t) * 2 * (size_t)(((p->N + 3) / 4 + 255) / 256 * 256);
     trk_launch(p, shm, (unsigned)ne);
     PSA_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(trk_stitch_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, p->d_C,
+                       (long long)ne, p->n_epoch, p->N, p->d_P);
+    PSA_CHECK(hipGetLastError());
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[2], st));
     PSA_CHECK(hipStreamSynchronize(st));
     if (p->timing) {
@@ -715,6 +735,14 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
         p->ms_state += a; p->n_state++;
         p->ms_corr += b; p->n_corr++;
     }
+    return 1;
+}
+
+int sdr_trk_plan_prompt(const sdr_trk_plan_t *p, sdr_cpx_t *P)
+{
+    if (!p || !P) return 0;
+    const size_t ne = (size_t)p->n_ch * p->n_epoch;
+    PSA_CHECK(hipMemcpy(P, p->d_P, sizeof(float2) * ne, hipMemcpyDeviceToHost));
     return 1;
 }
 
@@ -761,7 +789,7 @@ void sdr_trk_plan_free(sdr_trk_plan_t *p)
     (void)hipStreamSynchronize(p->st);
     (void)hipFree(p->d_buf); (void)hipFree(p->d_codes); (void)hipFree(p->d_ch);
     (void)hipFree(p->d_pos); (void)hipFree(p->d_ep);
-    (void)hipFree(p->d_corr); (void)hipFree(p->d_C);
+    (void)hipFree(p->d_corr); (void)hipFree(p->d_C); (void)hipFree(p->d_P);
     for (auto &e : p->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(p->st);
     delete p;

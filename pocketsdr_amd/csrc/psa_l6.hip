@@ -276,7 +276,14 @@ sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch
             return nullptr;
         }
     }
+    // PSA_L6_SPLIT = M1 picks another of the FFT's splits M = M1 x M2
     const FFTPlan *pl = fft_plan(N);
+    if (const char *e = getenv("PSA_L6_SPLIT")) {
+        int cand[8];
+        const int ns = fft_splits(N, cand, 8);
+        const int m1 = atoi(e);
+        if (std::find(cand, cand + ns, m1) != cand + ns) pl = fft_plan(N, m1);
+    }
     if (!pl) {
         fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: FFT length %d has prime factors other than 2, 3, 5\n", N);
         return nullptr;

@@ -91,6 +91,7 @@ _decl("sdr_trk_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(s
                                      C.c_int, c_double_p, C.c_int])
 _decl("sdr_trk_plan_run", C.c_int, [c_void_p])
 _decl("sdr_trk_plan_result", C.c_int, [c_void_p, c_void_p, c_void_p])
+_decl("sdr_trk_plan_prompt", C.c_int, [c_void_p, c_void_p])
 _decl("sdr_trk_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_trk_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                           C.POINTER(C.c_longlong), c_double_p])
@@ -373,6 +374,12 @@ class TrkPlan:
         Cc = np.zeros((self.n_ch, self.n_epoch, 2), np.complex64)
         lib.sdr_trk_plan_result(self.p, _ptr(corr), _ptr(Cc))
         return corr, Cc
+
+    def prompt(self):
+        """Full-period prompt per (channel, epoch), the trk->P history."""
+        P = np.zeros((self.n_ch, self.n_epoch), np.complex64)
+        lib.sdr_trk_plan_prompt(self.p, _ptr(P))
+        return P
 
     def timing(self, on=True):
         lib.sdr_trk_plan_timing(self.p, 1 if on else 0)

@@ -225,10 +225,13 @@ def test_trk_plan_bitexact(sdr, orc, variant):
     plan = sdr.TrkPlan(buff, w["chans"], w["pos"], w["n_epoch"])
     plan.run()
     corr, Cc = plan.result()
+    P = plan.prompt()
     for c, ch in enumerate(w["chans"]):
         rc, rC, _ = orc.trk_run(ch, ch["code"], buff, w["n_epoch"], w["pos"])
         assert np.array_equal(corr[c].view(np.uint32), rc.view(np.uint32)), c
         assert np.array_equal(Cc[c].view(np.uint32), rC.view(np.uint32)), c
+        rP = orc.trk_stitch(rC, ch["N"])
+        assert np.array_equal(P[c].view(np.uint32), rP.view(np.uint32)), c
     plan.close()
 
 

@@ -384,3 +384,43 @@ def test_trk_run_composition(orc):
         IQ = orc.mix_carr(buff, (e * N) % len(buff), N, fs, state[e, 2], state[e, 0])
         r, rC = orc.corr_std(IQ, bank, N, state[e, 1] * fs, pos)
         assert np.array_equal(r, corr[e]) and np.array_equal(rC, Cc[e])
+
+
+def test_trk_stitch_known_answers(orc):
+    """Prompt history (src/sdr_ch.c:736-740): this epoch's correlation before
+    the code wrap plus the previous epoch's after it, over N; zero before the
+    first epoch, so a -0 first half becomes +0 (-0 + 0 = +0)."""
+    Cc = np.array([[1 + 2j, 3 + 4j], [5 + 6j, 7 - 8j], [-0.5 + 1j, 0j]], np.complex64)
+    P = orc.trk_stitch(Cc, 4)
+    assert P.tolist() == [0.25 + 0.5j, 2 + 2.5j, 1.625 - 1.75j]
+    z = np.array([[complex(-0.0, -0.0), 0j]], np.complex64)
+    Pz = orc.trk_stitch(z, 8)
+    assert not np.signbit(Pz.real[0]) and not np.signbit(Pz.imag[0])
+
+
+def test_trk_stitch_full_period(orc):
+    """For a channel on the true code phase the stitched prompt is one whole
+    code period's correlation: it equals the sum over that period's samples
+    (previous epoch's tail, this epoch's head) of mixed sample x code, / N."""
+    from pocketsdr_amd import workloads
+    w = workloads.trk_l1ca(seed=4, fs=4e6, n_ch=2, n_epoch=6, n_vis=2, nposx=0)
+    ch = w["chans"][0]
+    N, fs = ch["N"], ch["fs"]
+    _, Cc, st = orc.trk_run(ch, ch["code"], w["buff"], w["n_epoch"], [0.0])
+    P = orc.trk_stitch(Cc, N)
+    bank = ch["code"].reshape(10, N, 2)[..., 0].astype(np.float64)
+    for e in range(1, w["n_epoch"]):
+        parts = []
+        for ee, half in ((e - 1, 1), (e, 0)):
+            IQ = orc.mix_carr(w["buff"], (ee * N) % len(w["buff"]), N, fs, st[ee, 2], st[ee, 0])
+            x = (IQ[:, 0] + 1j * IQ[:, 1]).astype(np.complex128) / 11.2
+            p = st[ee, 1] * fs
+            j = int(np.floor(p))
+            k = int((p - j) * 10)
+            code = bank[k]
+            if half == 1:
+                parts.append(np.dot(x[j:], code[:N - j]))
+            else:
+                parts.append(np.dot(x[:j], code[N - j:]))
+        want = sum(parts) / N
+        assert abs(P[e] - want) <= 1e-5 * abs(want) + 1e-7, e
