@@ -259,6 +259,12 @@ typedef struct sdr_l6_plan_tag sdr_l6_plan_t;
 sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch,
     const sdr_l6_ch_t *chs, int n_pos, const double *pos, int n_epoch,
     int flags);
+/* The same on an IF ring of buffN cpx8 bytes already in device memory (for
+ * example the front end's output), read in place at every run; the caller
+ * keeps it allocated until sdr_l6_plan_free. */
+sdr_l6_plan_t *sdr_l6_plan_new_dev(const uint8_t *dbuf, int buffN, int n_ch,
+    const sdr_l6_ch_t *chs, int n_pos, const double *pos, int n_epoch,
+    int flags);
 int sdr_l6_plan_run(sdr_l6_plan_t *plan);
 /* syms: n_ch x n_epoch CSK symbols; C: n_ch x n_epoch x n_pos taps (the
  * trk->C of src/sdr_ch.c:660-662). */

@@ -212,6 +212,7 @@ struct sdr_l6_plan_tag {
     int chunk = 0;                  // items per batch of FFT passes
     const FFTPlan *pl = nullptr;
     uint8_t *d_buf = nullptr;
+    bool own_buf = true;            // false: the IF ring is the caller's device memory
     float2 *d_cf = nullptr;         // n_ch * SDR_N_CODES spectra, digit-transposed
     TrkChDev *d_ch = nullptr;
     double *d_pos = nullptr;
@@ -255,18 +256,18 @@ struct L6Tick {
 
 }  // namespace psa
 
-extern "C" {
-
-sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch_t *chs,
-    int n_pos, const double *pos, int n_epoch, int flags)
+// data: the IF ring of buffN cpx8 bytes, host memory (copied to the
+// device) or, with dev, device memory read in place at every run
+static sdr_l6_plan_t *l6_create(const uint8_t *data, int buffN, bool dev, int n_ch, const sdr_l6_ch_t *chs,
+                                int n_pos, const double *pos, int n_epoch, int flags)
 {
-    if (!buff || !buff->data || buff->N <= 0 || n_ch <= 0 || n_epoch <= 0 || n_pos < 0 ||
-        n_pos > PSA_MAX_CORR || !chs || (n_pos > 0 && !pos)) {
+    if (!data || buffN <= 0 || n_ch <= 0 || n_epoch <= 0 || n_pos < 0 || n_pos > PSA_MAX_CORR || !chs ||
+        (n_pos > 0 && !pos)) {
         fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: invalid arguments\n");
         return nullptr;
     }
     const int N = chs[0].N, len_code = chs[0].len_code;
-    if (N <= 0 || N > buff->N || len_code < 2) {
+    if (N <= 0 || N > buffN || len_code < 2) {
         fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: need 0 < N <= buffer length, len_code >= 2\n");
         return nullptr;
     }
@@ -296,7 +297,7 @@ sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch
     }
     ensure_init();
     sdr_l6_plan_t *p = new sdr_l6_plan_t();
-    p->n_ch = n_ch; p->n_pos = n_pos; p->n_epoch = n_epoch; p->N = N; p->buffN = buff->N;
+    p->n_ch = n_ch; p->n_pos = n_pos; p->n_epoch = n_epoch; p->N = N; p->buffN = buffN;
     p->len_code = len_code; p->flags = flags; p->n_item = n_item; p->pl = pl;
     // two work buffers of `chunk` transforms each, 2 GiB together at most
     const long long cap = std::max<long long>(1, (2ll << 30) / (long long)(2 * item_bytes));
@@ -307,8 +308,13 @@ sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch
         if (c > 0) p->chunk = (int)std::min<long long>(p->chunk, c);
     }
     PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
-    PSA_CHECK(hipMalloc(&p->d_buf, buff->N));
-    PSA_CHECK(hipMemcpy(p->d_buf, buff->data, buff->N, hipMemcpyHostToDevice));
+    if (dev) {
+        p->d_buf = const_cast<uint8_t *>(data);
+        p->own_buf = false;
+    } else {
+        PSA_CHECK(hipMalloc(&p->d_buf, buffN));
+        PSA_CHECK(hipMemcpy(p->d_buf, data, buffN, hipMemcpyHostToDevice));
+    }
     // code spectra: natural order from the caller, digit-transposed on the device
     const size_t nspec = (size_t)n_ch * PSA_N_CODES;
     PSA_CHECK(hipMalloc(&p->d_cf, nspec * item_bytes));
@@ -345,6 +351,24 @@ sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch
     PSA_CHECK(hipMalloc(&p->d_sym, ni));
     PSA_CHECK(hipMalloc(&p->d_taps, sizeof(float2) * ni * std::max(n_pos, 1)));
     return p;
+}
+
+extern "C" {
+
+sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch_t *chs,
+    int n_pos, const double *pos, int n_epoch, int flags)
+{
+    if (!buff) {
+        fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new: invalid arguments\n");
+        return nullptr;
+    }
+    return l6_create(buff->data, buff->N, false, n_ch, chs, n_pos, pos, n_epoch, flags);
+}
+
+sdr_l6_plan_t *sdr_l6_plan_new_dev(const uint8_t *dbuf, int buffN, int n_ch, const sdr_l6_ch_t *chs,
+    int n_pos, const double *pos, int n_epoch, int flags)
+{
+    return l6_create(dbuf, buffN, true, n_ch, chs, n_pos, pos, n_epoch, flags);
 }
 
 }  // extern "C"
@@ -493,7 +517,8 @@ void sdr_l6_plan_free(sdr_l6_plan_t *p)
 {
     if (!p) return;
     (void)hipStreamSynchronize(p->st);
-    (void)hipFree(p->d_buf); (void)hipFree(p->d_cf); (void)hipFree(p->d_ch); (void)hipFree(p->d_pos);
+    if (p->own_buf) (void)hipFree(p->d_buf);
+    (void)hipFree(p->d_cf); (void)hipFree(p->d_ch); (void)hipFree(p->d_pos);
     (void)hipFree(p->it.ix); (void)hipFree(p->it.nseg); (void)hipFree(p->it.cf);
     (void)hipFree(p->it.p0); (void)hipFree(p->it.p1); (void)hipFree(p->it.s);
     (void)hipFree(p->A); (void)hipFree(p->B); (void)hipFree(p->corr);

@@ -98,6 +98,8 @@ _decl("sdr_trk_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
 _decl("sdr_trk_plan_free", None, [c_void_p])
 _decl("sdr_l6_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sdr_l6_ch_t),
                                     C.c_int, c_double_p, C.c_int, C.c_int])
+_decl("sdr_l6_plan_new_dev", c_void_p, [c_void_p, C.c_int, C.c_int, C.POINTER(sdr_l6_ch_t),
+                                        C.c_int, c_double_p, C.c_int, C.c_int])
 _decl("sdr_l6_plan_run", C.c_int, [c_void_p])
 _decl("sdr_l6_plan_result", C.c_int, [c_void_p, c_void_p, c_void_p])
 _decl("sdr_l6_plan_corr", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p])
@@ -421,10 +423,12 @@ def csk(corr, len_code, pos):
 class L6Plan:
     """Batched L6D / L6E FFT tracking with CSK decoding (sdr_l6_plan_*).
     chans: dicts fs, fi, fc_sig, T, N, len_code, fd0, drift, phi0, coff0 and
-    code_fft, the (SDR_N_CODES, N) conjugated code spectra."""
+    code_fft, the (SDR_N_CODES, N) conjugated code spectra.  buff: uint8 cpx8
+    IF ring, or with dev_buff a (device pointer, length) pair read in place
+    (sdr_l6_plan_new_dev)."""
 
-    def __init__(self, buff, chans, pos, n_epoch, keep_corr=False):
-        self._buf = _buff(buff)
+    def __init__(self, buff, chans, pos, n_epoch, keep_corr=False, dev_buff=False):
+        self._buf = None if dev_buff else _buff(buff)
         self._keep = []
         ca = (sdr_l6_ch_t * len(chans))()
         for i, c in enumerate(chans):
@@ -435,8 +439,13 @@ class L6Plan:
         self.pos = np.ascontiguousarray(pos, dtype=np.float64)
         self.n_ch, self.n_epoch, self.n_pos = len(chans), n_epoch, len(self.pos)
         self.N = chans[0]["N"]
-        self.p = lib.sdr_l6_plan_new(C.byref(self._buf), len(chans), ca, self.n_pos, _dp(self.pos),
-                                     n_epoch, SDR_L6_KEEP_CORR if keep_corr else 0)
+        flags = SDR_L6_KEEP_CORR if keep_corr else 0
+        if dev_buff:
+            self.p = lib.sdr_l6_plan_new_dev(buff[0], buff[1], len(chans), ca, self.n_pos,
+                                             _dp(self.pos), n_epoch, flags)
+        else:
+            self.p = lib.sdr_l6_plan_new(C.byref(self._buf), len(chans), ca, self.n_pos,
+                                         _dp(self.pos), n_epoch, flags)
         if not self.p:
             raise ValueError("sdr_l6_plan_new failed")
 

@@ -219,3 +219,25 @@ def test_l6_plan_decodes(sdr, orc):
     with pytest.raises(ValueError):
         plan.corr(0, 0)                              # made without keep_corr
     plan.close()
+
+
+@pytest.mark.gpu
+def test_l6_plan_device_ring(sdr, orc):
+    """sdr_l6_plan_new_dev on an IF ring in device memory: the same symbols
+    and taps as the plan on the host ring, also after new samples are
+    written into the device ring between runs."""
+    import torch
+    w = l6_workload(orc, 6e6, n_ch=3, n_epoch=6, n_vis=2, seed=12)
+    buff = np.asarray(w["buff"], np.uint8)
+    dring = torch.from_numpy(buff.copy()).cuda()
+    pd = sdr.L6Plan((dring.data_ptr(), len(buff)), w["chans"], w["pos"], w["n_epoch"], dev_buff=True)
+    for data in (buff, np.roll(buff, 4321)):
+        dring.copy_(torch.from_numpy(data))
+        pd.run()
+        ph = sdr.L6Plan(data, w["chans"], w["pos"], w["n_epoch"])
+        ph.run()
+        s1, t1 = pd.result()
+        s2, t2 = ph.result()
+        assert np.array_equal(s1, s2) and np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
+        ph.close()
+    pd.close()
