@@ -12,11 +12,13 @@ form (FP16 words, MFMA DFT-matrix stages, FP32 accumulation).
 Also on the same line: the FP32 batched path and the plain FP32 per-PRN GPU
 path (the reference dataflow: one search_code per PRN and block, unfused) on
 the same workload, the acquisition decisions of each form, the tracking
-correlators (64 channels at 24 Msps, 44 taps, Msps*ch), the roofline of the
-dominant kernel from HIP events, and the CPU baseline (the oracle
-restatement of sdr_search_code with the reference PocketFFT, one thread),
-whose sample also gives the peak-power error of the timed form against the
-FP64 oracle.
+correlators (64 channels at 24 Msps, 44 taps, Msps*ch, with their int8-dot
+roofline), the roofline of the dominant kernel from HIP events, a per-cell
+HBM model and a per-signal FLOP model of the whole step, the CPU baseline
+(the oracle restatement of sdr_search_code with the reference PocketFFT, one
+thread, and the same sample on a pool of host threads), whose sample also
+gives the peak-power error of the timed form against the FP64 oracle, the
+front end (raw unpack, array combine) and L6D FFT tracking with CSK.
 """
 import argparse
 import json
@@ -31,6 +33,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "acq search cells/s (batched+MFMA vs plain FP32, 1 GPU) + correlator Msps*ch"
 HBM_PEAK_GBS = 8000.0
+FP16_PEAK_TFLOPS = 2500.0       # dense FP16 / BF16 MFMA
+FP32_PEAK_TFLOPS = 157.3        # FP32 vector
+# int8 dot products: v_dot4 (4 MACs per lane) on 64 lanes per clock per CU,
+# 256 CUs at 2.4 GHz = 157 T MAC/s = 315 TOPS; dense int8 MFMA 5000 TOPS
+DOT4_PEAK_TOPS = 256 * 64 * 4 * 2 * 2.4e9 / 1e12
+INT8_MFMA_PEAK_TOPS = 5000.0
 
 
 def log(*a):
@@ -222,6 +230,21 @@ def bench_acq(args, sdr, orc):
                            launches_of=d["parts"], alg_bytes_per_launch=round(alg),
                            avg_launch_ms=round(avg_ms, 5))
     res["cell_model"] = cell_model(sigs, form, t_b)
+    # algorithmic FLOPs per cell-step F = 5 M log2 M (1 + 1 / n_prn) + 8 M (the
+    # forward FFT shared by a signal's PRNs, one inverse per PRN, product and
+    # power) over each signal's own kernel time, against the dense FP16 MFMA
+    # peak (the precision form's stages) or the FP32 vector peak
+    fl = {}
+    for i, sg in enumerate(sigs):
+        M, nq = sg["M"], sg["code_fft"].shape[0]
+        F = 5.0 * M * np.log2(M) * (1.0 + 1.0 / nq) + 8.0 * M
+        nstep = nq * len(sg["fds"]) * sg["n_blk"]
+        ms = sum(v["avg_ms"] * v["launches"] for k, v in ks.items() if k.endswith("@" + sg["name"])) / args.steps
+        tf = F * nstep / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        pk = FP32_PEAK_TFLOPS if res["form_per_signal"][sg["name"]] in ("fp32", "plain") else FP16_PEAK_TFLOPS
+        fl[sg["name"]] = dict(flop_per_cell_step=round(F), cell_steps=nstep, kernel_ms=round(ms, 4),
+                              tflops=round(tf, 2), peak_tflops=pk, frac=round(tf / pk, 5))
+    res["flop_model"] = fl
     res["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 5), launches=v["launches"],
                               gbs=round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1))
                       for k, v in ks.items()}
@@ -244,7 +267,8 @@ def bench_acq(args, sdr, orc):
         res["same_decisions_as_plain"] = same_decisions(sigs, r_main, rp)
         log("plain: %.3f ms/step (batched %s is %.2fx)" % (t_p * 1e3, form, t_p / t_b))
     if orc is not None and not args.skip_cpu:
-        res["cpu_baseline"], res["peak_power_err_vs_fp64"] = cpu_leg(args, orc, plan, buffs, sigs)
+        res["cpu_baseline"], res["cpu_baseline_threads"], res["peak_power_err_vs_fp64"] = \
+            cpu_leg(args, orc, plan, buffs, sigs)
     plan.close()
     cfg = dict(workload=wname, cells=cells,
                signals=[dict(name=s["name"], fs=s["fs"], T=s["T"], M=s["M"], n_prn=s["code_fft"].shape[0],
@@ -278,6 +302,35 @@ def cpu_leg(args, orc, plan, buffs, sigs):
                 sample="; ".join(parts) + " (%.1f s): oracle C restatement of sdr_search_code with "
                        "the reference PocketFFT (%s), one thread, no 1 ms sleeps"
                        % (t_cpu, "loaded" if orc.REF_LOADED else "absent"))
+    # the receiver model: one host thread per task, all cores busy - four
+    # times the PRNs of the sample above split into (signal, PRN, 8-bin)
+    # tasks over a thread pool (the oracle's C calls run outside the
+    # interpreter lock)
+    from concurrent.futures import ThreadPoolExecutor
+    nth = max(1, min(16, os.cpu_count() or 1))
+    tasks = []
+    for i, s in enumerate(sigs):
+        nq = min(4 * args.cpu_prns, s["code_fft"].shape[0])
+        nb = min(args.cpu_bins, len(s["fds"])) if args.cpu_bins else len(s["fds"])
+        tasks += [(i, q, b0, min(nb, b0 + 8)) for q in range(nq) for b0 in range(0, nb, 8)]
+
+    def task(tk):
+        i, q, b0, b1 = tk
+        s = sigs[i]
+        P = np.zeros((b1 - b0, s["M"]), np.float32)
+        for k in range(s["n_blk"]):
+            orc.search_code(s["code_fft"][q], s["T"], buffs[s["buff"]], s["ix0"] + k * s["blk_step"],
+                            s["fs"], s["fi"], s["fds"][b0:b1], P)
+        return b1 - b0
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(nth) as ex:
+        n_mt = sum(ex.map(task, tasks))
+    t_mt = time.perf_counter() - t0
+    log("cpu %d threads: %.1f cells/s over %.1f s" % (nth, n_mt / t_mt, t_mt))
+    base_mt = dict(value=round(n_mt / t_mt, 2), unit="cells/s", cores=nth, kind="port",
+                   sample="%s PRNs of every signal over all bins and blocks as (signal, PRN, 8-bin) "
+                          "tasks on %d host threads (%.1f s)" % (4 * args.cpu_prns, nth, t_mt))
     worst = 0.0
     for i, q, P in surf:
         s = sigs[i]
@@ -288,8 +341,8 @@ def cpu_leg(args, orc, plan, buffs, sigs):
                                 s["fs"], s["fi"], s["fds"][b:b + 1], ref)
         g = plan.power(i, q)
         worst = max(worst, abs(float(g[b, lag]) - ref[0, lag]) / ref[0, lag])
-    return base, dict(max_rel=float("%.3e" % worst), cells=len(surf),
-                      note="peak cell of each sampled PRN, timed form vs FP64 oracle")
+    return base, base_mt, dict(max_rel=float("%.3e" % worst), cells=len(surf),
+                               note="peak cell of each sampled PRN, timed form vs FP64 oracle")
 
 
 def traffic_of(inst, grids, profile="search_multi_traffic.md"):
@@ -377,9 +430,17 @@ def bench_trk(args, sdr, workloads):
     plan.close()
     n_ch, N = len(w["chans"]), w["chans"][0]["N"]
     msps_ch = n_ch * N * w["n_epoch"] / t / 1e6
-    return dict(correlator_msps_ch=round(msps_ch, 1), correlator_ms_per_step=round(t * 1e3, 3),
-                correlator_config=dict(n_ch=n_ch, fs=24e6, n_epoch=w["n_epoch"], n_pos=len(w["pos"])),
-                correlator_kernel_ms=round(ks[0], 4) if ks else None)
+    res = dict(correlator_msps_ch=round(msps_ch, 1), correlator_ms_per_step=round(t * 1e3, 3),
+               correlator_config=dict(n_ch=n_ch, fs=24e6, n_epoch=w["n_epoch"], n_pos=len(w["pos"])),
+               correlator_kernel_ms=round(ks[0], 4) if ks else None)
+    if ks:
+        # 2 n N int8 MACs per channel-epoch (I and Q), 2 ops each
+        tops = 2.0 * 2 * len(w["pos"]) * N * n_ch * w["n_epoch"] / (ks[0] * 1e-3) / 1e12
+        res["correlator_roofline"] = dict(bound="valu-int8-dot4", achieved=round(tops, 2),
+                                          peak=round(DOT4_PEAK_TOPS, 1), unit="TOPS",
+                                          frac=round(tops / DOT4_PEAK_TOPS, 4),
+                                          frac_of_int8_mfma=round(tops / INT8_MFMA_PEAK_TOPS, 5))
+    return res
 
 
 L6_CLASSES = ("state", "fwd_col", "row", "fwd_row", "inv_row", "inv_col", "csk")
