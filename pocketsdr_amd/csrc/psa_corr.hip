@@ -342,23 +342,46 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     __syncthreads();
 
     // ---- mix N samples into the I / Q planes (zero padded to Nq4 words) ----
+    // A word whose 4 samples lie in one ring segment takes its 4 IF bytes
+    // with two aligned dword loads and a byte align (the ring is padded by a
+    // dword) and steps the phase by s; other words go sample by sample.
     for (int w = threadIdx.x; w < Nq4; w += TRK_THREADS) {
+        const int i0 = 4 * w;
         uint32_t wi = 0, wq = 0;
+        const bool s1 = i0 + 4 <= N && i0 + 4 <= q.nseg, s2 = i0 + 4 <= N && i0 >= q.nseg;
+        if (s1 || s2) {
+            const int src = s1 ? q.ix + i0 : i0 - q.nseg;
+            uint32_t ph = s1 ? q.p0 + (uint32_t)i0 * q.s : q.p1 + (uint32_t)(i0 - q.nseg) * q.s;
+            const uint32_t *a = reinterpret_cast<const uint32_t *>(buf + (src & ~3));
+            const uint32_t d4 = __builtin_amdgcn_alignbyte(a[1], a[0], (uint32_t)(src & 3));
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            int i = w * 4 + b;
-            if (i < N) {
-                int src;
-                uint32_t ph;
-                if (i < q.nseg) { src = q.ix + i; ph = q.p0 + (uint32_t)i * q.s; }
-                else { src = i - q.nseg; ph = q.p1 + (uint32_t)(i - q.nseg) * q.s; }
-                uint32_t d = buf[src];
-                int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
-                int sQ = (int)((int8_t)(uint8_t)d) >> 4;
-                uint32_t c = carr[ph >> 24];
-                int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+            for (int b = 0; b < 4; b++) {
+                const uint32_t d = d4 >> (8 * b);
+                const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+                const int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+                const uint32_t c = carr[ph >> 24];
+                ph += q.s;
+                const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
                 wi |= (uint32_t)(uint8_t)(int8_t)(sI * cI - sQ * cQ) << (8 * b);
                 wq |= (uint32_t)(uint8_t)(int8_t)(sI * cQ + sQ * cI) << (8 * b);
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                int i = i0 + b;
+                if (i < N) {
+                    int src;
+                    uint32_t ph;
+                    if (i < q.nseg) { src = q.ix + i; ph = q.p0 + (uint32_t)i * q.s; }
+                    else { src = i - q.nseg; ph = q.p1 + (uint32_t)(i - q.nseg) * q.s; }
+                    uint32_t d = buf[src];
+                    int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+                    int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+                    uint32_t c = carr[ph >> 24];
+                    int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+                    wi |= (uint32_t)(uint8_t)(int8_t)(sI * cI - sQ * cQ) << (8 * b);
+                    wq |= (uint32_t)(uint8_t)(int8_t)(sI * cQ + sQ * cI) << (8 * b);
+                }
             }
         }
         planeI[w] = wi;
@@ -369,22 +392,31 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const uint32_t *cbase = codes + (long long)ch * code_ch_stride;   // [copy][bankI|bankQ][k][NP/4]
     const uint8_t *cb = (const uint8_t *)cbase;
     const int cstride = (int)code_copy_stride;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+    // the wave index and everything per tap are wave-uniform: held in scalar
+    // registers, so the half choice below is a scalar branch and the code
+    // loads take a scalar base
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
     const int QOFF = PSA_N_CODES * NP;       // byte offset of the Q banks
     // ---- taps: wave `wave` owns t = t0 + s nwave, s < TRK_TPW ----
     for (int t0 = wave; t0 < n_pos; t0 += nwave * TRK_TPW) {
-        int jt[TRK_TPW];
-        const uint32_t *cp[TRK_TPW];
-        int acc[TRK_TPW][4];
+        int jt[TRK_TPW], co[TRK_TPW];
+        // per tap and lane: the running sum of every word so far (tot), its
+        // value when the step holding sample j began (snap) and the half-1
+        // sum completed in that step (h1); half 2 is then tot - snap (exact
+        // int32 sums)
+        int tot[TRK_TPW][2], snap[TRK_TPW][2], h1[TRK_TPW][2];
 #pragma unroll
         for (int s = 0; s < TRK_TPW; s++) {
             const int t = t0 + s * nwave;
             const int2 g = jk[t < n_pos ? t : 0];
+            const int gx = __builtin_amdgcn_readfirstlane(g.x), gy = __builtin_amdgcn_readfirstlane(g.y);
             // code sample of data sample m: doubled bank byte N - j + m
-            const int c0 = N - g.x, cpy = c0 & 3;
-            jt[s] = t < n_pos ? g.x : N;
-            cp[s] = cbase + cpy * cstride + ((g.y * NP) >> 2) + (c0 - cpy) / 4;
-            acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0;
+            const int c0 = N - gx, cpy = c0 & 3;
+            jt[s] = t < n_pos ? gx : N;
+            co[s] = cpy * cstride + ((gy * NP) >> 2) + (c0 - cpy) / 4;
+#pragma unroll
+            for (int c = 0; c < 2; c++) tot[s][c] = snap[s][c] = h1[s][c] = 0;
         }
         // a step covers 256 words: lane l takes words 4 l .. 4 l + 3 of it with
         // one 16-byte load per tap (planes padded with zero words to Nq4)
@@ -395,8 +427,10 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
             uint4 cI[TRK_TPW], cQ[TRK_TPW];
 #pragma unroll
             for (int s = 0; s < TRK_TPW; s++) {
-                cI[s] = load_u4(cp[s] + w);
-                cQ[s] = SAMEQ ? cI[s] : load_u4(cp[s] + w + (QOFF >> 2));
+                // scalar base of the step, lane offset 16 l bytes
+                const uint32_t *cs = cbase + co[s] + wb;
+                cI[s] = load_u4(cs + 4 * lane);
+                cQ[s] = SAMEQ ? cI[s] : load_u4(cs + (QOFF >> 2) + 4 * lane);
             }
 #pragma unroll
             for (int s = 0; s < TRK_TPW; s++) {
@@ -404,40 +438,45 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
                 const uint32_t xi[4] = {xI.x, xI.y, xI.z, xI.w}, xq[4] = {xQ.x, xQ.y, xQ.z, xQ.w};
                 const uint32_t ci[4] = {cI[s].x, cI[s].y, cI[s].z, cI[s].w};
                 const uint32_t cq[4] = {cQ[s].x, cQ[s].y, cQ[s].z, cQ[s].w};
-                // this step covers samples 4 wb .. 4 wb + 1023 (wave-uniform test)
-                if (4 * wb + 1024 <= j) {
-                    dot4x4<EXACT>(xi, xq, ci, cq, acc[s][0], acc[s][1]);
-                } else if (4 * wb >= j) {
-                    dot4x4<EXACT>(xi, xq, ci, cq, acc[s][2], acc[s][3]);
+                // this step covers samples 4 wb .. 4 wb + 1023 (a scalar test)
+                if (4 * wb + 1024 <= j || 4 * wb > j) {
+                    dot4x4<EXACT>(xi, xq, ci, cq, tot[s][0], tot[s][1]);
                 } else {
-                    // words wholly before / after sample j; the straddling
-                    // word is added by the epilogue
+                    // the step holding sample j: words wholly before j end
+                    // half 1, words from j on start half 2, the word
+                    // straddling j is added by the epilogue
+                    snap[s][0] = tot[s][0];
+                    snap[s][1] = tot[s][1];
+                    int bI = 0, bQ = 0;
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         int dI = 0, dQ = 0;
                         dot4x4<EXACT, 1>(xi + v, xq + v, ci + v, cq + v, dI, dQ);
                         const int m = 4 * (w + v);
-                        if (m + 4 <= j) { acc[s][0] += dI; acc[s][1] += dQ; }
-                        else if (m >= j) { acc[s][2] += dI; acc[s][3] += dQ; }
+                        if (m + 4 <= j) { bI += dI; bQ += dQ; }
+                        else if (m >= j) { tot[s][0] += dI; tot[s][1] += dQ; }
                     }
+                    h1[s][0] = snap[s][0] + bI;
+                    h1[s][1] = snap[s][1] + bQ;
                 }
             }
         }
         // ---- per-tap float epilogue (the owning wave): the integer sums hold
         // the first 16 floor(len / 16) samples of each half, the rest are
         // added in float one by one (dot_IQ_code's AVX2 body and scalar
-        // tail, src/sdr_func.c:861-912).  Lanes 0-15 / 16-31 fetch the tail
-        // samples of half 1 / 2, lanes 32-35 the word straddling j, all at
-        // once; lane 0 then adds the tail in order.
+        // tail, src/sdr_func.c:861-912).  Per tap, lanes 0-15 / 16-31 fetch
+        // the tail samples of half 1 / 2 and lanes 32-35 the word straddling
+        // j (not in the running sums); each lane folds its integer correction
+        // (+ straddle, - tail) into its partial sums, all taps' partials are
+        // reduced together, and the ordered float tail is added to each.
         const uint8_t *pI = (const uint8_t *)planeI, *pQ = (const uint8_t *)planeQ;
+#pragma unroll
         for (int s = 0; s < TRK_TPW; s++) {
             const int t = t0 + s * nwave;
             if (t >= n_pos) break;
-            int S[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) S[c] = wave_sum(acc[s][c]);
-            const int2 g = jk[t];
-            const int j = g.x, k = g.y, len2 = N - j;
+            const int j = jt[s];
+            const int k = __builtin_amdgcn_readfirstlane(jk[t].y);
+            const int len2 = N - j;
             const int V1 = j >= 16 ? j / 16 * 16 : 0;
             const int V2 = len2 >= 16 ? len2 / 16 * 16 : 0;
             int m = -1, ci = 0, slot = 0;   // slot 0/1: tail of half 1/2; 2/3: straddle in half 1/2
@@ -459,13 +498,31 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
             }
             const int eI = EXACT ? sgn8(xI, vI) : xI * vI, eQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
             const int fI = xI * vI, fQ = xQ * vQ;
-            // integer sums: + straddle - tail, per half
-            const int sgn = slot >= 2 ? 1 : -1;
-            const bool h1 = slot == 0 || slot == 2;
-            const int aI1 = wave_sum(m >= 0 && h1 ? sgn * eI : 0), aQ1 = wave_sum(m >= 0 && h1 ? sgn * eQ : 0);
-            const int aI2 = wave_sum(m >= 0 && !h1 ? sgn * eI : 0), aQ2 = wave_sum(m >= 0 && !h1 ? sgn * eQ : 0);
-            float f1I = (float)(S[0] + aI1), f1Q = (float)(S[1] + aQ1);
-            float f2I = (float)(S[2] + aI2), f2Q = (float)(S[3] + aQ2);
+            const int sg = m < 0 ? 0 : (slot >= 2 ? 1 : -1);
+            const bool first = slot == 0 || slot == 2;
+            int p[4] = {h1[s][0] + (first ? sg * eI : 0), h1[s][1] + (first ? sg * eQ : 0),
+                        tot[s][0] - snap[s][0] + (first ? 0 : sg * eI),
+                        tot[s][1] - snap[s][1] + (first ? 0 : sg * eQ)};
+            // transposed butterfly over the 4 sums: the lanes with bit 5 (then
+            // 4) set keep the upper half of the values, then a plain sum over
+            // bits 3 .. 0; value c ends in lanes 16 c .. 16 c + 15
+            {
+                const bool up = (lane & 32) != 0;
+                const int k0 = up ? p[2] : p[0], k1 = up ? p[3] : p[1];
+                const int s0 = up ? p[0] : p[2], s1 = up ? p[1] : p[3];
+                p[0] = k0 + __shfl_xor(s0, 32, 64);
+                p[1] = k1 + __shfl_xor(s1, 32, 64);
+            }
+            {
+                const bool up = (lane & 16) != 0;
+                p[0] = (up ? p[1] : p[0]) + __shfl_xor(up ? p[0] : p[1], 16, 64);
+            }
+#pragma unroll
+            for (int o = 8; o >= 1; o >>= 1) p[0] += __shfl_xor(p[0], o, 64);
+            float f1I = (float)__builtin_amdgcn_readlane(p[0], 0);
+            float f1Q = (float)__builtin_amdgcn_readlane(p[0], 16);
+            float f2I = (float)__builtin_amdgcn_readlane(p[0], 32);
+            float f2Q = (float)__builtin_amdgcn_readlane(p[0], 48);
             for (int i = 0; i < j - V1; i++) {
                 f1I += (float)__builtin_amdgcn_readlane(fI, i);
                 f1Q += (float)__builtin_amdgcn_readlane(fQ, i);
@@ -640,7 +697,9 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
         if ((d & 0xF) == 0x8 || (d >> 4) == 0x8) p->exact = true;
     }
     PSA_CHECK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
-    PSA_CHECK(hipMalloc(&p->d_buf, buff->N));
+    // one dword of padding (zero) past the ring for the mixer's aligned loads
+    PSA_CHECK(hipMalloc(&p->d_buf, (size_t)buff->N + 8));
+    PSA_CHECK(hipMemset(p->d_buf + buff->N, 0, 8));
     PSA_CHECK(hipMemcpy(p->d_buf, buff->data, buff->N, hipMemcpyHostToDevice));
     // code layout per channel: [copy 0..3][I banks 0..9 | Q banks 0..9][NP bytes],
     // each bank doubled (byte i holds bank sample i mod N, i < 2N); copy c holds
