@@ -35,9 +35,10 @@ METRIC = "acq search cells/s (batched+MFMA vs plain FP32, 1 GPU) + correlator Ms
 HBM_PEAK_GBS = 8000.0
 FP16_PEAK_TFLOPS = 2500.0       # dense FP16 / BF16 MFMA
 FP32_PEAK_TFLOPS = 157.3        # FP32 vector
-# int8 dot products: v_dot4 (4 MACs per lane) on 64 lanes per clock per CU,
-# 256 CUs at 2.4 GHz = 157 T MAC/s = 315 TOPS; dense int8 MFMA 5000 TOPS
-DOT4_PEAK_TOPS = 256 * 64 * 4 * 2 * 2.4e9 / 1e12
+# int8 dot products: no published VALU figure; v_dot4_i32_i8 (4 MACs) issues
+# 22.4 T lane-ops/s = 179 TOPS on this GPU (make -C pocketsdr_amd probe,
+# 8 independent chains per lane); dense int8 MFMA 5000 TOPS (published)
+DOT4_PEAK_TOPS = 179.2
 INT8_MFMA_PEAK_TOPS = 5000.0
 
 
