@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 FP16_PEAK_TFLOPS = 2500.0       # dense FP16 / BF16 MFMA
 FP32_PEAK_TFLOPS = 157.3        # FP32 vector
 # int8 dot products: no published VALU figure; v_dot4_i32_i8 (4 MACs) issues
-# 22.4 T lane-ops/s = 179 TOPS on this GPU (make -C pocketsdr_amd probe,
+# 22.2 T lane-ops/s = 177.5 TOPS on this GPU (make -C pocketsdr_amd probe,
 # 8 independent chains per lane); dense int8 MFMA 5000 TOPS (published)
-DOT4_PEAK_TOPS = 179.2
+DOT4_PEAK_TOPS = 177.5
 INT8_MFMA_PEAK_TOPS = 5000.0
 
 
