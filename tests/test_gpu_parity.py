@@ -270,3 +270,27 @@ def test_search_plan_multi(sdr, orc, multi_acq, tune):
             q = s["prns"].index(sat["prn"])
             assert cn0[q] > 38.0, (s["name"], sat["prn"], cn0[q])
     plan.close()
+
+
+def test_trk_plan_full_size(sdr, orc):
+    """The tracking configuration of the bench at full size (64 channels x
+    1000 epochs at 24 Msps, 44 taps): three channels, the first, a middle and
+    the last, bit-exact against the oracle over all epochs, and the prompt
+    power of every visible satellite's channel well above the others'."""
+    from pocketsdr_amd import workloads
+    w = workloads.trk_l1ca(seed=2, n_epoch=1000)
+    plan = sdr.TrkPlan(w["buff"], w["chans"], w["pos"], w["n_epoch"])
+    plan.run()
+    corr, Cc = plan.result()
+    P = plan.prompt()
+    plan.close()
+    for c in (0, 37, len(w["chans"]) - 1):
+        ch = w["chans"][c]
+        rc, rC, _ = orc.trk_run(ch, ch["code"], w["buff"], w["n_epoch"], w["pos"])
+        assert np.array_equal(corr[c].view(np.uint32), rc.view(np.uint32)), c
+        assert np.array_equal(Cc[c].view(np.uint32), rC.view(np.uint32)), c
+    vis = {s["prn"] for s in w["truth"]}
+    pw = np.mean(np.abs(P[:, 1:]) ** 2, axis=1)
+    on = [pw[c] for c, ch in enumerate(w["chans"]) if ch["prn"] in vis]
+    off = [pw[c] for c, ch in enumerate(w["chans"]) if ch["prn"] not in vis]
+    assert min(on) > 3 * max(off)

@@ -241,3 +241,26 @@ def test_l6_plan_device_ring(sdr, orc):
         assert np.array_equal(s1, s2) and np.array_equal(t1.view(np.uint32), t2.view(np.uint32))
         ph.close()
     pd.close()
+
+
+@pytest.mark.gpu
+def test_l6_plan_full_size(sdr, orc):
+    """The L6 configuration of the bench at full size (16 channels x 250
+    epochs at 24 Msps, N = 96000): every planted symbol decoded, and one
+    visible channel's symbols and taps against the oracle."""
+    from pocketsdr_amd import workloads
+    w = workloads.trk_l6(seed=3, fs=24e6, n_ch=16, n_epoch=250)
+    plan = sdr.L6Plan(w["buff"], w["chans"], w["pos"], w["n_epoch"])
+    plan.run()
+    syms, taps = plan.result()
+    plan.close()
+    seen = set()
+    for c, ch in enumerate(w["chans"]):
+        if ch["prn"] in w["truth"] and ch["prn"] not in seen:
+            seen.add(ch["prn"])
+            assert syms[c].tolist() == w["truth"][ch["prn"]]["syms"][:w["n_epoch"]].tolist(), c
+    assert len(seen) == 4
+    ch = w["chans"][0]
+    rs, rt, _, _ = orc.trk_l6_run(ch, w["len_code"], ch["code_fft"], w["buff"], w["n_epoch"], w["pos"])
+    assert np.array_equal(syms[0], rs)
+    assert rel(taps[0], rt) < 1e-4
