@@ -421,8 +421,7 @@ def bench_raw(args, sdr, orc):
     return res
 
 
-def bench_trk(args, sdr, workloads):
-    w = workloads.trk_l1ca(seed=args.seed + 1, n_epoch=args.trk_epochs)
+def bench_trk(args, sdr, w):
     plan = sdr.TrkPlan(w["buff"], w["chans"], w["pos"], w["n_epoch"])
     t = time_plan(plan, max(1, args.steps // 2), 1)
     plan.timing(True)
@@ -499,6 +498,24 @@ def bench_l6(args, sdr, orc, workloads):
         res["cpu_epochs_ch_per_s"] = round(n_e / dt, 1)
         res["cpu_sample"] = "%d epochs of one channel, oracle L6 loop, one thread (%.2f s)" % (n_e, dt)
     return res
+
+
+def bench_trk_loop(args, sdr, w):
+    """Closed-loop tracking (FLL / PLL / DLL / C/N0 per epoch, one persistent
+    workgroup per channel) on the correlator workload: 64 channels x 1000
+    epochs at 24 Msps, 44 taps, every run from the acquisition states."""
+    loops = [dict(len_code=1023, costas=1, cn0=40.0) for _ in w["chans"]]
+    plan = sdr.TrkPlan(w["buff"], w["chans"], w["pos"], w["n_epoch"], loops=loops)
+    t = time_plan(plan, max(1, args.steps // 4), 1)
+    st = plan.state()
+    plan.close()
+    n_ch, N, ne = len(w["chans"]), w["chans"][0]["N"], w["n_epoch"]
+    vis = {s["prn"] for s in w["truth"]}
+    held = [bool(st[c, -1, 5] == 0) for c, ch in enumerate(w["chans"]) if ch["prn"] in vis]
+    return dict(config=dict(n_ch=n_ch, fs=24e6, n_epoch=ne, n_pos=len(w["pos"])),
+                ms_per_run=round(t * 1e3, 3), msps_ch=round(n_ch * N * ne / t / 1e6, 1),
+                realtime_x=round(ne * 1e-3 / t, 1), satellites_held=sum(held), satellites=len(held))
+
 
 ONCHIP_BYTES = 672 * 1024      # one CU: 160 KiB LDS + 512 KiB VGPR/AGPR
 
@@ -601,7 +618,9 @@ def main():
         from oracle import oracle as orc
     value, t_step, res, cfg = bench_acq(args, sdr, orc)
     if not args.skip_trk:
-        res.update(bench_trk(args, sdr, workloads))
+        w_trk = workloads.trk_l1ca(seed=args.seed + 1, n_epoch=args.trk_epochs)
+        res.update(bench_trk(args, sdr, w_trk))
+        res["closed_loop_tracking"] = bench_trk_loop(args, sdr, w_trk)
     if not args.skip_raw:
         r = bench_raw(args, sdr, orc)
         res["arch_combine"] = r.pop("arch_combine")

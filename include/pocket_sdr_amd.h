@@ -226,6 +226,25 @@ int sdr_trk_plan_result(const sdr_trk_plan_t *plan, sdr_cpx_t *corr,
  * src/sdr_ch.c:736-740 ((C[0] + previous epoch's C[1]) / N, zero before the
  * first epoch). */
 int sdr_trk_plan_prompt(const sdr_trk_plan_t *plan, sdr_cpx_t *P);
+/* Closed-loop tracking (src/sdr_ch.c:488-624, 694-763: FLL for the first
+ * second, then PLL, DLL, C/N0 and carrier-lock windows, the loss decision)
+ * for signals without secondary code, bump-jump or the L6 branch; each
+ * channel starts as start_track leaves it (src/sdr_ch.c:378-394) from
+ * chs[c].fd0, coff0, phi0 (drift is not used) and loops[c].  Taps 0-3 must
+ * be P, E, L and the noise tap.  Every run restarts from these states;
+ * atan and log10 are the GPU's (the results follow the reference to their
+ * last bits). */
+typedef struct {
+    int len_code;              /* code length (chips), the DLL's scale */
+    int costas;                /* 1: Costas discriminators (all but L6D / L6E) */
+    double cn0;                /* C/N0 at acquisition (dB-Hz) */
+} sdr_trk_loop_t;
+sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch,
+    const sdr_trk_ch_t *chs, const sdr_trk_loop_t *loops, int n_pos,
+    const double *pos, int n_epoch);
+/* closed-loop plans: n_ch x n_epoch x {fd, coff, phi, cn0, lock, idle}
+ * after each epoch (idle 1: lost at or before it, not processed) */
+int sdr_trk_plan_state(const sdr_trk_plan_t *plan, double *state);
 void sdr_trk_plan_timing(sdr_trk_plan_t *plan, int on);
 int sdr_trk_plan_kernel_ms(const sdr_trk_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);

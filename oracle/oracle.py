@@ -59,6 +59,8 @@ _d("orc_gen_code_fft_f64", None, [vp, C.c_int, C.c_double, C.c_double, C.c_doubl
 _d("orc_gen_code_L1CA", C.c_int, [C.c_int, vp])
 _d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, C.c_int, vp, vp, dp])
 _d("orc_trk_stitch", None, [vp, C.c_int, C.c_int, vp])
+_d("orc_trkl_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, C.c_int, C.c_double, vp, vp, C.c_int,
+                          C.c_int, dp, C.c_int, dp, vp, vp])
 _d("orc_csk", C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, vp])
 _d("orc_trk_l6_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, vp, vp, C.c_int, C.c_int, dp, C.c_int,
                             vp, vp, vp, dp])
@@ -219,6 +221,23 @@ def trk_run(ch, code_bank_arr, buff, n_epoch, pos):
     lib.orc_trk_run(C.byref(c), _p(code_bank_arr), _p(buff), len(buff), n_epoch,
                     pos.ctypes.data_as(dp), len(pos), _p(corr), _p(Cc), state.ctypes.data_as(dp))
     return corr, Cc, state
+
+
+def trkl_run(ch, len_code, costas, cn0, code_bank_arr, buff, n_epoch, pos):
+    """Closed-loop tracking of one channel (src/sdr_ch.c:488-624, 694-763):
+    state (n_epoch, 6) = fd, coff, phi, cn0, lock, idle; corr (n_epoch,
+    n_pos); P (n_epoch,)."""
+    c = orc_trk_ch_t(ch["fs"], ch["fi"], ch["fc_sig"], ch["T"], ch["N"], ch["fd0"],
+                     ch.get("drift", 0.0), ch["phi0"], ch["coff0"])
+    code_bank_arr = np.ascontiguousarray(code_bank_arr, np.int8)
+    buff = np.ascontiguousarray(buff, np.uint8)
+    pos = np.ascontiguousarray(pos, np.float64)
+    state = np.zeros((n_epoch, 6), np.float64)
+    corr = np.zeros((n_epoch, len(pos)), np.complex64)
+    P = np.zeros(n_epoch, np.complex64)
+    lib.orc_trkl_run(C.byref(c), len_code, costas, cn0, _p(code_bank_arr), _p(buff), len(buff), n_epoch,
+                     pos.ctypes.data_as(dp), len(pos), state.ctypes.data_as(dp), _p(corr), _p(P))
+    return state, corr, P
 
 
 def trk_stitch(Cc, N):

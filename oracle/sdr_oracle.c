@@ -561,6 +561,118 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
 
 
 /* ---------------------------------------------------------------------------
+ * closed-loop tracking, src/sdr_ch.c:488-624 (FLL, PLL, DLL, CN0, test_lost)
+ * and 694-763 (track_sig), for signals with no secondary code, bump-jump or
+ * L6 branch, from the state start_track leaves (src/sdr_ch.c:364-394), the
+ * loop constants at their defaults (src/sdr_ch.c:29-49).  Epoch e runs at
+ * time (e + 1) T on ring index (e N) mod buffN.  The reference build's
+ * contractions (gcc -O3 -mavx2 -mfma on replicas of these lines): FLL
+ * dot = fma(IP1, IP2, QP1 QP2), cross = fma(-QP1, IP2, IP1 QP2); PLL
+ * phas_acc = fma(W W W err, T, phas_acc), fd += fma(T, phas_acc,
+ * fma(2.4 W, err - err_prev, 1.1 W W err T)); DLL sums += fmaf(re, re, im im)
+ * (float, then double), code_int = fma(W W err, dt, code_int),
+ * coff = fma(-fma(1.414 W, err, code_int), dt, coff); C/N0 power sums as
+ * fmaf(re, re, +-im im), cn0 = fma(fma(log10(.), 10, -cn0), 0.5, cn0).
+ * Outputs per epoch: state[6] = {fd, coff, phi, cn0, lock, idle}, corr[n_pos]
+ * (cpx), P (cpx, the stitched prompt); epochs after a loss: idle = 1.
+ */
+#define TL_DPI (2.0 * PI_D)
+
+void orc_trkl_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0,
+                  const int8_t *code_bank, const uint8_t *buff, int buffN, int n_epoch,
+                  const double *pos, int n_pos, double *state, float *corr, float *P)
+{
+    const int N = ch->N;
+    double time = 0.0, fd = ch->fd0, coff = ch->coff0, phi = ch->phi0, cn0 = cn0_0, pli = 1.0;
+    double err_phas = 0.0, phas_acc = 0.0, code_int = 0.0;
+    double sumE = 0.0, sumL = 0.0, sumP = 0.0, sumN = 0.0, sumD = 0.0;
+    float C0[2] = {0.0f, 0.0f}, C1p[2] = {0.0f, 0.0f};
+    int lock = 0, lost_cnt = 0, pli_valid = 0, idle = 0;
+    int8_t *IQ = (int8_t *)malloc(2 * (size_t)N);
+    float *C = (float *)malloc(sizeof(float) * 2 * (size_t)(n_pos > 0 ? n_pos : 1));
+    float C1[4];
+    for (int e = 0; e < n_epoch; e++) {
+        double *st = state + 6 * (size_t)e;
+        float *pe = P + 2 * (size_t)e;
+        if (idle) {
+            st[0] = fd; st[1] = coff; st[2] = phi; st[3] = cn0; st[4] = lock; st[5] = 1;
+            pe[0] = pe[1] = 0.0f;
+            memset(corr + 2 * (size_t)e * n_pos, 0, sizeof(float) * 2 * n_pos);
+            continue;
+        }
+        double t = (e + 1) * ch->T;
+        double tau = t - time;
+        double fc = ch->fi + fd;
+        coff = fma(-(fd / ch->fc_sig), tau, coff);
+        phi = fma(fc, tau, phi);
+        phi -= floor(phi);
+        time = t;
+        if (coff >= ch->T) { coff -= ch->T; lock--; }
+        else if (coff < 0.0) { coff += ch->T; lock++; }
+        int ix = (int)(((long long)e * N) % buffN);
+        orc_mix_carr(buff, buffN, ix, N, ch->fs, fc, phi, IQ);
+        orc_corr_std(IQ, code_bank, N, coff * ch->fs, pos, n_pos, C, C1);
+        memcpy(corr + 2 * (size_t)e * n_pos, C, sizeof(float) * 2 * n_pos);
+        float Px = (C1[0] + C1p[0]) / N, Py = (C1[1] + C1p[1]) / N;
+        C1p[0] = C1[2]; C1p[1] = C1[3];
+        lock++;
+        if (lock * ch->T <= 1.0) {                           /* T_FPULLIN: FLL */
+            if (lock >= 2) {
+                double IP1 = C[0], QP1 = C[1], IP2 = C0[0], QP2 = C0[1];
+                double dot = fma(IP1, IP2, QP1 * QP2);
+                double cross = fma(-QP1, IP2, IP1 * QP2);
+                if (dot != 0.0) {
+                    double B = lock * ch->T < 0.5 ? 5.0 : 2.0;   /* B_FLL_W, B_FLL_N */
+                    double err = costas ? atan(cross / dot) : atan2(cross, dot);
+                    fd -= B / 0.25 * err / TL_DPI;
+                }
+            }
+            C0[0] = C[0]; C0[1] = C[1];
+        } else {                                              /* PLL */
+            double IP = C[0], QP = C[1];
+            if (IP != 0.0) {
+                double err = (costas ? atan(QP / IP) : atan2(QP, IP)) / TL_DPI;
+                double W = 5.0 / 0.7845;                      /* B_PLL */
+                phas_acc = fma(W * W * W * err, ch->T, phas_acc);
+                double u = 1.1 * W * W * err * ch->T;
+                fd += fma(ch->T, phas_acc, fma(2.4 * W, err - err_phas, u));
+                err_phas = err;
+            }
+        }
+        int ND = (int)(0.02 / ch->T);                         /* T_DLL */
+        if (ND < 1) ND = 1;
+        sumE += fmaf(C[2], C[2], C[3] * C[3]);
+        sumL += fmaf(C[4], C[4], C[5] * C[5]);
+        if (lock % ND == 0) {
+            double E = sqrt(sumE), L = sqrt(sumL);
+            if (E + L > 0.0) {
+                double err = (E - L) / (E + L) * 0.5 * ch->T / len_code;
+                double W = 0.25 / 0.53;                       /* B_DLL */
+                double dt = ch->T * ND;
+                code_int = fma(W * W * err, dt, code_int);
+                coff = fma(-fma(1.414 * W, err, code_int), dt, coff);
+            }
+            sumE = sumL = 0.0;
+        }
+        sumP += fmaf(Px, Px, Py * Py);
+        sumD += fmaf(Px, Px, -(Py * Py));
+        sumN += fmaf(C[6], C[6], C[7] * C[7]);
+        if (lock % (int)(0.5 / ch->T) == 0) {                 /* T_CN0 */
+            if (sumP > 0.0) { pli = sumD / sumP; pli_valid = 1; }
+            if (sumN > 0.0) cn0 = fma(fma(log10(sumP / sumN / ch->T), 10.0, -cn0), 0.5, cn0);
+            sumP = sumN = sumD = 0.0;
+            int bad = cn0 < 30.0 || (costas && pli_valid && pli < 0.25);
+            lost_cnt = bad ? lost_cnt + 1 : 0;
+            if (lost_cnt >= 4) { idle = 1; lock = 0; }
+        }
+        st[0] = fd; st[1] = coff; st[2] = phi; st[3] = cn0; st[4] = lock; st[5] = 0;
+        pe[0] = Px; pe[1] = Py;
+    }
+    free(C);
+    free(IQ);
+}
+
+/* ---------------------------------------------------------------------------
  * prompt history, src/sdr_ch.c:736-740: P[e] = (C[e][0] + C[e-1][1]) / N in
  * float, with trk->C1 = 0 before the first epoch (src/sdr_ch.c:368).  C holds
  * n_epoch x 2 cpx (orc_trk_run's C), P n_epoch cpx.

@@ -101,6 +101,13 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
                  const double *pos, int n_pos, float *corr, float *C,
                  double *state);
 
+/* Closed-loop tracking of one channel, src/sdr_ch.c:488-624, 694-763.
+ * Per epoch: state[6] = {fd, coff, phi, cn0, lock, idle}, corr[n_pos] (cpx),
+ * P (cpx). */
+void orc_trkl_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0,
+                  const int8_t *code_bank, const uint8_t *buff, int buffN, int n_epoch,
+                  const double *pos, int n_pos, double *state, float *corr, float *P);
+
 /* Prompt history of the standard correlator branch, src/sdr_ch.c:736-740:
  * P[e] = (C[e][0] + C[e-1][1]) / N, C[-1][1] = 0 (cpx arrays). */
 void orc_trk_stitch(const float *C, int n_epoch, int N, float *P);

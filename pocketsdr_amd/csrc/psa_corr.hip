@@ -308,28 +308,31 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
 #define TRK_THREADS 512
 #define TRK_TPW 6
 
+// LDS working set of one (channel, epoch) correlation
+struct TrkLds {
+    uint32_t *planeI, *planeQ;   // mixed samples, Nq4 words each
+    const uint32_t *carr;        // carrier table
+    float4 *c12s;                // per tap: half-1 / half-2 sums (I, Q)
+    int2 *jk;                    // per tap: code offset j, bank k
+    int *sgn;                    // the data-sign decision of the taps 0-2
+};
+
+// One epoch's sdr_mix_carr + sdr_corr_std by the whole workgroup: the
+// epoch's state q, the channel's code banks from cbase; corr_out receives the
+// n_pos taps and C_out the two halves of tap 0 (global or LDS).  Starts and
+// ends with the workgroup in step (barriers inside).
 template <bool EXACT, bool SAMEQ>
-__global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
-    const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
-    const TrkEpoch *__restrict__ ep, const double *__restrict__ pos,
-    const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride,
-    int NP,  // padded bank length in bytes (multiple of 4)
-    const uint32_t *__restrict__ carr_g,
-    float2 *__restrict__ corr_out, float2 *__restrict__ C_out)
+__device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *__restrict__ buf, int N,
+                                              int n_pos, const double *__restrict__ pos,
+                                              const uint32_t *__restrict__ cbase, long long code_copy_stride,
+                                              int NP, const TrkLds &L, float2 *corr_out, float2 *C_out)
 {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint32_t carr[256];
-    __shared__ int sums[PSA_MAX_CORR][4];
-    __shared__ float4 c12s[PSA_MAX_CORR];
-    __shared__ int2 jk[PSA_MAX_CORR];
     const int Nq = (N + 3) / 4;
     const int Nq4 = (Nq + 255) / 256 * 256;    // planes padded to whole steps
-    uint32_t *planeI = lds, *planeQ = lds + Nq4;
-    const long long o = blockIdx.x;          // channel * n_epoch + epoch
-    const int ch = (int)(o / n_epoch);
-    const TrkEpoch q = ep[o];
-    for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
-    for (int i = threadIdx.x; i < n_pos * 4; i += TRK_THREADS) (&sums[0][0])[i] = 0;
+    uint32_t *planeI = L.planeI, *planeQ = L.planeQ;
+    const uint32_t *carr = L.carr;
+    float4 *c12s = L.c12s;
+    int2 *jk = L.jk;
     // sdr_corr_std tap geometry (src/sdr_func.c:988-991)
     for (int i = threadIdx.x; i < n_pos; i += TRK_THREADS) {
         const double p = q.cs + pos[i];
@@ -389,8 +392,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     }
     __syncthreads();
 
-    const uint32_t *cbase = codes + (long long)ch * code_ch_stride;   // [copy][bankI|bankQ][k][NP/4]
-    const uint8_t *cb = (const uint8_t *)cbase;
+    const uint8_t *cb = (const uint8_t *)cbase;   // [copy][bankI|bankQ][k][NP]
     const int cstride = (int)code_copy_stride;
     // the wave index and everything per tap are wave-uniform: held in scalar
     // registers, so the half choice below is a scalar branch and the code
@@ -545,20 +547,246 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
             float4 c = c12s[i];
             dot = __fadd_rn(dot, __fmaf_rn(c.x, c.z, __fmul_rn(c.y, c.w)));
         }
-        sums[0][0] = dot < 0.0f ? -1 : 1;
+        *L.sgn = dot < 0.0f ? -1 : 1;
         if (n_pos > 0) {
-            C_out[2 * o] = make_float2(c12s[0].x, c12s[0].y);
-            C_out[2 * o + 1] = make_float2(c12s[0].z, c12s[0].w);
+            C_out[0] = make_float2(c12s[0].x, c12s[0].y);
+            C_out[1] = make_float2(c12s[0].z, c12s[0].w);
         }
     }
     __syncthreads();
-    const float sign = (float)sums[0][0];
+    const float sign = (float)*L.sgn;
     const float fN = (float)N;
     for (int t = threadIdx.x; t < n_pos; t += blockDim.x) {
         float4 c = c12s[t];
-        corr_out[o * n_pos + t] = make_float2(__fdiv_rn(__fadd_rn(c.x, __fmul_rn(sign, c.z)), fN),
-                                              __fdiv_rn(__fadd_rn(c.y, __fmul_rn(sign, c.w)), fN));
+        corr_out[t] = make_float2(__fdiv_rn(__fadd_rn(c.x, __fmul_rn(sign, c.z)), fN),
+                                  __fdiv_rn(__fadd_rn(c.y, __fmul_rn(sign, c.w)), fN));
     }
+    __syncthreads();
+}
+
+template <bool EXACT, bool SAMEQ>
+__global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
+    const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
+    const TrkEpoch *__restrict__ ep, const double *__restrict__ pos,
+    const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride,
+    int NP,  // padded bank length in bytes (multiple of 4)
+    const uint32_t *__restrict__ carr_g,
+    float2 *__restrict__ corr_out, float2 *__restrict__ C_out)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t carr[256];
+    __shared__ float4 c12s[PSA_MAX_CORR];
+    __shared__ int2 jk[PSA_MAX_CORR];
+    __shared__ int sgn;
+    const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
+    const long long o = blockIdx.x;          // channel * n_epoch + epoch
+    const int ch = (int)(o / n_epoch);
+    const TrkEpoch q = ep[o];
+    for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
+    const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn};
+    trk_correlate<EXACT, SAMEQ>(q, buf, N, n_pos, pos, codes + (long long)ch * code_ch_stride,
+                                code_copy_stride, NP, L, corr_out + o * n_pos, C_out + 2 * o);
+}
+
+// ===========================================================================
+// closed-loop tracking: the FLL / PLL / DLL, C/N0 and loss decision of
+// track_sig (src/sdr_ch.c:488-620, 694-763) for signals with no secondary
+// code, bump-jump or L6 branch.  One persistent workgroup per channel runs the
+// channel's epochs in order: lane 0 advances the NCO and code offset, the
+// workgroup mixes and correlates the epoch (trk_correlate), lane 0 stitches
+// the prompt and updates the loop filters.  The loop constants are the
+// reference's defaults (src/sdr_ch.c:29-49).  The reference build's fused
+// multiply-adds are written as explicit fma; atan and log10 are the GPU's, so
+// results follow the reference to the last bits of those functions.
+// ===========================================================================
+#define TL_T_DLL 0.02         // T_DLL
+#define TL_T_CN0 0.5          // T_CN0
+#define TL_T_FPULLIN 1.0      // T_FPULLIN
+#define TL_T_FPULLIN_W 0.5    // T_FPULLIN_W
+#define TL_B_DLL 0.25         // B_DLL
+#define TL_B_PLL 5.0          // B_PLL
+#define TL_B_FLL_W 5.0        // B_FLL_W
+#define TL_B_FLL_N 2.0        // B_FLL_N
+#define TL_THRES_CN0_U 30.0   // THRES_CN0_U
+#define TL_THRES_PLI 0.25     // THRES_PLI
+#define TL_LOST_TH 4          // LOST_TH
+#define TL_FILT_CN0 0.5       // FILT_CN0
+#define TL_DPI (2.0 * 3.1415926535897932)
+
+// one channel's tracking state (the sdr_ch_t / sdr_trk_t fields the loop uses)
+struct TrkLoopCh {
+    double fs, fi, fc_sig, T;
+    double time, fd, coff, phi, cn0, pli;
+    double err_phas, phas_acc, code_int, err_code;
+    double sumE, sumL, sumP, sumN, sumD;   // DLL E / L, C/N0 windows
+    float C0x, C0y, C1x, C1y;              // FLL previous prompt, stitch carry
+    int lock, lost_cnt, pli_valid, idle, len_code, costas, ep, pad;
+};
+
+// per (channel, epoch): state after the epoch (idle: the channel was lost at
+// or before it and the epoch was not processed)
+struct TrkLoopOut {
+    double fd, coff, phi, cn0;
+    int lock, idle;
+    float2 P;
+};
+
+// lane 0: open-loop part of track_sig (src/sdr_ch.c:694-703, adj_coff 674-689)
+__device__ __forceinline__ void tl_advance(TrkLoopCh &S, int N, int buffN, TrkEpoch &q)
+{
+#pragma clang fp contract(off)
+    const double t = (double)(S.ep + 1) * S.T;
+    const double tau = t - S.time;
+    const double fc = S.fi + S.fd;
+    S.coff = fma(-(S.fd / S.fc_sig), tau, S.coff);
+    S.phi = fma(fc, tau, S.phi);
+    S.phi = S.phi - floor(S.phi);
+    S.time = t;
+    if (S.coff >= S.T) { S.coff = S.coff - S.T; S.lock--; }
+    else if (S.coff < 0.0) { S.coff = S.coff + S.T; S.lock++; }
+    q.ix = (int)(((long long)S.ep * N) % buffN);
+    trk_mix_params(buffN, q.ix, N, S.fs, fc, S.phi, q.p0, q.p1, q.nseg, q.s);
+    q.pad = 0;
+    q.cs = S.coff * S.fs;
+}
+
+// lane 0: the loop after the correlation (src/sdr_ch.c:488-620, 736-763);
+// tap 0 = P, 1 = E, 2 = L, 3 = the noise tap; C = tap 0's two halves
+__device__ __forceinline__ void tl_update(TrkLoopCh &S, int N, const float2 *tap, const float2 *C,
+                                          TrkLoopOut &o)
+{
+#pragma clang fp contract(off)
+    // prompt history (src/sdr_ch.c:736-740)
+    const float fN = (float)N;
+    const float Px = __fdiv_rn(__fadd_rn(C[0].x, S.C1x), fN), Py = __fdiv_rn(__fadd_rn(C[0].y, S.C1y), fN);
+    S.C1x = C[1].x;
+    S.C1y = C[1].y;
+    S.lock++;
+    if (S.lock * S.T <= TL_T_FPULLIN) {
+        // FLL (src/sdr_ch.c:488-506)
+        if (S.lock >= 2) {
+            const double IP1 = tap[0].x, QP1 = tap[0].y, IP2 = S.C0x, QP2 = S.C0y;
+            const double dot = fma(IP1, IP2, QP1 * QP2);
+            const double cross = fma(-QP1, IP2, IP1 * QP2);
+            if (dot != 0.0) {
+                const double B = S.lock * S.T < TL_T_FPULLIN_W ? TL_B_FLL_W : TL_B_FLL_N;
+                const double err = S.costas ? atan(cross / dot) : atan2(cross, dot);
+                S.fd = S.fd - B / 0.25 * err / TL_DPI;
+            }
+        }
+        S.C0x = tap[0].x;
+        S.C0y = tap[0].y;
+    } else {
+        // PLL (src/sdr_ch.c:509-521)
+        const double IP = tap[0].x, QP = tap[0].y;
+        if (IP != 0.0) {
+            const double err = (S.costas ? atan(QP / IP) : atan2(QP, IP)) / TL_DPI;
+            const double W = TL_B_PLL / 0.7845;
+            S.phas_acc = fma(W * W * W * err, S.T, S.phas_acc);
+            const double u = 1.1 * W * W * err * S.T;
+            S.fd = S.fd + fma(S.T, S.phas_acc, fma(2.4 * W, err - S.err_phas, u));
+            S.err_phas = err;
+        }
+    }
+    // DLL (src/sdr_ch.c:524-549): the E and L power sums
+    const int ND = max(1, (int)(TL_T_DLL / S.T));
+    S.sumE = S.sumE + (double)__fmaf_rn(tap[1].x, tap[1].x, __fmul_rn(tap[1].y, tap[1].y));
+    S.sumL = S.sumL + (double)__fmaf_rn(tap[2].x, tap[2].x, __fmul_rn(tap[2].y, tap[2].y));
+    if (S.lock % ND == 0) {
+        const double E = sqrt(S.sumE), L = sqrt(S.sumL);
+        if (E + L > 0.0) {
+            const double err = (E - L) / (E + L) * 0.5 * S.T / S.len_code;
+            const double W = TL_B_DLL / 0.53;
+            const double dt = S.T * ND;
+            S.code_int = fma(W * W * err, dt, S.code_int);
+            S.coff = fma(-fma(1.414 * W, err, S.code_int), dt, S.coff);
+            S.err_code = err;
+        }
+        S.sumE = S.sumL = 0.0;
+    }
+    // C/N0 and carrier lock (src/sdr_ch.c:567-605)
+    S.sumP = S.sumP + (double)__fmaf_rn(Px, Px, __fmul_rn(Py, Py));
+    S.sumD = S.sumD + (double)__fmaf_rn(Px, Px, -__fmul_rn(Py, Py));
+    S.sumN = S.sumN + (double)__fmaf_rn(tap[3].x, tap[3].x, __fmul_rn(tap[3].y, tap[3].y));
+    if (S.lock % (int)(TL_T_CN0 / S.T) == 0) {
+        if (S.sumP > 0.0) {
+            S.pli = S.sumD / S.sumP;
+            S.pli_valid = 1;
+        }
+        if (S.sumN > 0.0) {
+            const double l = log10(S.sumP / S.sumN / S.T);
+            S.cn0 = fma(fma(l, 10.0, -S.cn0), TL_FILT_CN0, S.cn0);
+        }
+        S.sumP = S.sumN = S.sumD = 0.0;
+        // loss of lock (src/sdr_ch.c:608-624)
+        const bool bad = S.cn0 < TL_THRES_CN0_U || (S.costas && S.pli_valid && S.pli < TL_THRES_PLI);
+        S.lost_cnt = bad ? S.lost_cnt + 1 : 0;
+        if (S.lost_cnt >= TL_LOST_TH) {
+            S.idle = 1;
+            S.lock = 0;
+        }
+    }
+    S.ep++;
+    o.fd = S.fd; o.coff = S.coff; o.phi = S.phi; o.cn0 = S.cn0;
+    o.lock = S.lock; o.idle = 0;
+    o.P = make_float2(Px, Py);
+}
+
+template <bool EXACT, bool SAMEQ>
+__global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
+    const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
+    const double *__restrict__ pos, const uint32_t *__restrict__ codes, long long code_ch_stride,
+    long long code_copy_stride, int NP, const uint32_t *__restrict__ carr_g,
+    TrkLoopCh *__restrict__ chs, TrkLoopOut *__restrict__ out, float2 *__restrict__ corr_out,
+    float2 *__restrict__ C_out)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t carr[256];
+    __shared__ float4 c12s[PSA_MAX_CORR];
+    __shared__ int2 jk[PSA_MAX_CORR];
+    __shared__ float2 tap[PSA_MAX_CORR];
+    __shared__ float2 Ch[2];
+    __shared__ int sgn, stop;
+    __shared__ TrkEpoch qs;
+    __shared__ TrkLoopCh S;
+    const int c = blockIdx.x;
+    const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
+    for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
+    if (threadIdx.x == 0) {
+        S = chs[c];
+        stop = S.idle;
+    }
+    __syncthreads();
+    const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn};
+    const uint32_t *cbase = codes + (long long)c * code_ch_stride;
+    int e = 0;
+    for (; e < n_epoch; e++) {
+        if (stop) break;                       // written by lane 0 before a barrier
+        if (threadIdx.x == 0) tl_advance(S, N, buffN, qs);
+        __syncthreads();
+        const TrkEpoch q = qs;
+        trk_correlate<EXACT, SAMEQ>(q, buf, N, n_pos, pos, cbase, code_copy_stride, NP, L, tap, Ch);
+        const long long o = (long long)c * n_epoch + e;
+        for (int t = threadIdx.x; t < n_pos; t += TRK_THREADS) corr_out[o * n_pos + t] = tap[t];
+        if (threadIdx.x == 0) {
+            C_out[2 * o] = Ch[0];
+            C_out[2 * o + 1] = Ch[1];
+            TrkLoopOut r;
+            tl_update(S, N, tap, Ch, r);
+            out[o] = r;
+            stop = S.idle;
+        }
+        __syncthreads();
+    }
+    // epochs after a loss of lock: marked idle, state as at the loss
+    for (int e2 = e + threadIdx.x; e2 < n_epoch; e2 += TRK_THREADS) {
+        TrkLoopOut r;
+        r.fd = S.fd; r.coff = S.coff; r.phi = S.phi; r.cn0 = S.cn0;
+        r.lock = S.lock; r.idle = 1;
+        r.P = make_float2(0.0f, 0.0f);
+        out[(long long)c * n_epoch + e2] = r;
+    }
+    if (threadIdx.x == 0) chs[c] = S;
 }
 
 // Prompt history of track_sig (src/sdr_ch.c:736-740): the full-period prompt
@@ -592,6 +820,12 @@ struct sdr_trk_plan_tag {
     double *d_pos = nullptr;
     TrkEpoch *d_ep = nullptr;
     float2 *d_corr = nullptr, *d_C = nullptr, *d_P = nullptr;
+    // closed-loop mode: initial channel states, their working copies, and
+    // the per-epoch state records
+    bool loop = false;
+    std::vector<TrkLoopCh> lch0;
+    TrkLoopCh *d_lch = nullptr;
+    TrkLoopOut *d_lout = nullptr;
     hipStream_t st = nullptr;
     bool timing = false;
     hipEvent_t ev[4];
@@ -743,6 +977,36 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
     return p;
 }
 
+sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch, const sdr_trk_ch_t *chs,
+    const sdr_trk_loop_t *loops, int n_pos, const double *pos, int n_epoch)
+{
+    if (!loops || n_pos < 4) {
+        fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: need loop parameters and the P, E, L, N taps\n");
+        return nullptr;
+    }
+    for (int c = 0; c < n_ch; c++) {
+        if (loops[c].len_code <= 0 || !(chs[c].T > 0.0) || (int)(TL_T_CN0 / chs[c].T) <= 0) {
+            fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: need len_code > 0 and 0 < T <= 0.5 s\n");
+            return nullptr;
+        }
+    }
+    sdr_trk_plan_t *p = sdr_trk_plan_new(buff, n_ch, chs, n_pos, pos, n_epoch);
+    if (!p) return nullptr;
+    p->loop = true;
+    p->lch0.assign(n_ch, TrkLoopCh{});
+    for (int c = 0; c < n_ch; c++) {
+        // start_track (src/sdr_ch.c:378-394) and trk_init (:364-377)
+        TrkLoopCh &S = p->lch0[c];
+        S.fs = chs[c].fs; S.fi = chs[c].fi; S.fc_sig = chs[c].fc_sig; S.T = chs[c].T;
+        S.time = 0.0; S.fd = chs[c].fd0; S.coff = chs[c].coff0; S.phi = chs[c].phi0;
+        S.cn0 = loops[c].cn0; S.pli = 1.0;
+        S.len_code = loops[c].len_code; S.costas = loops[c].costas ? 1 : 0;
+    }
+    PSA_CHECK(hipMalloc(&p->d_lch, sizeof(TrkLoopCh) * n_ch));
+    PSA_CHECK(hipMalloc(&p->d_lout, sizeof(TrkLoopOut) * (size_t)n_ch * n_epoch));
+    return p;
+}
+
 }  // extern "C"
 
 template <bool EXACT, bool SAMEQ>
@@ -757,6 +1021,28 @@ static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)
                        p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
                        p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
                        dev_carrier(), p->d_corr, p->d_C);
+}
+
+template <bool EXACT, bool SAMEQ>
+static void trk_loop_launch_t(sdr_trk_plan_t *p, size_t shm)
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        PSA_CHECK(hipFuncSetAttribute((const void *)trk_loop_kernel<EXACT, SAMEQ>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    });
+    hipLaunchKernelGGL((trk_loop_kernel<EXACT, SAMEQ>), dim3(p->n_ch), dim3(TRK_THREADS), shm, p->st,
+                       p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_pos, p->d_codes,
+                       p->code_ch_stride, p->code_copy_stride, p->NP, dev_carrier(), p->d_lch, p->d_lout,
+                       p->d_corr, p->d_C);
+}
+
+static void trk_loop_launch(sdr_trk_plan_t *p, size_t shm)
+{
+    if (p->exact && p->sameq) trk_loop_launch_t<true, true>(p, shm);
+    else if (p->exact) trk_loop_launch_t<true, false>(p, shm);
+    else if (p->sameq) trk_loop_launch_t<false, true>(p, shm);
+    else trk_loop_launch_t<false, false>(p, shm);
 }
 
 static void trk_launch(sdr_trk_plan_t *p, size_t shm, unsigned ne)
@@ -774,6 +1060,23 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
     if (!p) return 0;
     hipStream_t st = p->st;
     const size_t ne = (size_t)p->n_ch * p->n_epoch;
+    if (p->loop) {
+        // every run starts from the channels' initial states
+        PSA_CHECK(hipMemcpyAsync(p->d_lch, p->lch0.data(), sizeof(TrkLoopCh) * p->n_ch, hipMemcpyHostToDevice,
+                                 st));
+        if (p->timing) PSA_CHECK(hipEventRecord(p->ev[0], st));
+        const size_t shm = sizeof(uint32_t) * 2 * (size_t)(((p->N + 3) / 4 + 255) / 256 * 256);
+        trk_loop_launch(p, shm);
+        PSA_CHECK(hipGetLastError());
+        if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
+        PSA_CHECK(hipStreamSynchronize(st));
+        if (p->timing) {
+            float a = 0;
+            PSA_CHECK(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
+            p->ms_corr += a; p->n_corr++;
+        }
+        return 1;
+    }
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[0], st));
     hipLaunchKernelGGL(trk_state_kernel, dim3(p->n_ch), dim3(TRK_SCHUNK), 0, st, p->d_ch, p->n_epoch, p->N,
                        p->buffN, p->d_ep);
@@ -801,7 +1104,27 @@ int sdr_trk_plan_prompt(const sdr_trk_plan_t *p, sdr_cpx_t *P)
 {
     if (!p || !P) return 0;
     const size_t ne = (size_t)p->n_ch * p->n_epoch;
+    if (p->loop) {
+        std::vector<TrkLoopOut> r(ne);
+        PSA_CHECK(hipMemcpy(r.data(), p->d_lout, sizeof(TrkLoopOut) * ne, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < ne; i++) { P[i][0] = r[i].P.x; P[i][1] = r[i].P.y; }
+        return 1;
+    }
     PSA_CHECK(hipMemcpy(P, p->d_P, sizeof(float2) * ne, hipMemcpyDeviceToHost));
+    return 1;
+}
+
+int sdr_trk_plan_state(const sdr_trk_plan_t *p, double *state)
+{
+    if (!p || !state || !p->loop) return 0;
+    const size_t ne = (size_t)p->n_ch * p->n_epoch;
+    std::vector<TrkLoopOut> r(ne);
+    PSA_CHECK(hipMemcpy(r.data(), p->d_lout, sizeof(TrkLoopOut) * ne, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ne; i++) {
+        double *q = state + 6 * i;
+        q[0] = r[i].fd; q[1] = r[i].coff; q[2] = r[i].phi; q[3] = r[i].cn0;
+        q[4] = r[i].lock; q[5] = r[i].idle;
+    }
     return 1;
 }
 
@@ -849,6 +1172,7 @@ void sdr_trk_plan_free(sdr_trk_plan_t *p)
     (void)hipFree(p->d_buf); (void)hipFree(p->d_codes); (void)hipFree(p->d_ch);
     (void)hipFree(p->d_pos); (void)hipFree(p->d_ep);
     (void)hipFree(p->d_corr); (void)hipFree(p->d_C); (void)hipFree(p->d_P);
+    (void)hipFree(p->d_lch); (void)hipFree(p->d_lout);
     for (auto &e : p->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(p->st);
     delete p;

@@ -39,6 +39,10 @@ class sdr_trk_ch_t(C.Structure):
                 ("code", c_void_p)]
 
 
+class sdr_trk_loop_t(C.Structure):
+    _fields_ = [("len_code", C.c_int), ("costas", C.c_int), ("cn0", C.c_double)]
+
+
 class sdr_l6_ch_t(C.Structure):
     _fields_ = [("fs", C.c_double), ("fi", C.c_double), ("fc_sig", C.c_double),
                 ("T", C.c_double), ("N", C.c_int), ("len_code", C.c_int), ("fd0", C.c_double),
@@ -92,6 +96,9 @@ _decl("sdr_trk_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(s
 _decl("sdr_trk_plan_run", C.c_int, [c_void_p])
 _decl("sdr_trk_plan_result", C.c_int, [c_void_p, c_void_p, c_void_p])
 _decl("sdr_trk_plan_prompt", C.c_int, [c_void_p, c_void_p])
+_decl("sdr_trk_plan_new_loop", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sdr_trk_ch_t),
+                                          C.POINTER(sdr_trk_loop_t), C.c_int, c_double_p, C.c_int])
+_decl("sdr_trk_plan_state", C.c_int, [c_void_p, c_void_p])
 _decl("sdr_trk_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_trk_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                           C.POINTER(C.c_longlong), c_double_p])
@@ -349,9 +356,11 @@ class SearchPlan:
 
 
 class TrkPlan:
-    """Batched open-loop tracking correlators (sdr_trk_plan_*)."""
+    """Batched tracking correlators (sdr_trk_plan_*): open loop, or with
+    loops (dicts len_code, costas, cn0 per channel) the closed-loop plan
+    (sdr_trk_plan_new_loop)."""
 
-    def __init__(self, buff, chans, pos, n_epoch):
+    def __init__(self, buff, chans, pos, n_epoch, loops=None):
         self._buf = _buff(buff)
         self._keep = []
         ca = (sdr_trk_ch_t * len(chans))()
@@ -362,8 +371,15 @@ class TrkPlan:
                                  c["drift"], c["phi0"], c["coff0"], _ptr(code))
         self.pos = np.ascontiguousarray(pos, dtype=np.float64)
         self.n_ch, self.n_epoch, self.n_pos = len(chans), n_epoch, len(self.pos)
-        self.p = lib.sdr_trk_plan_new(C.byref(self._buf), len(chans), ca, self.n_pos,
-                                      _dp(self.pos), n_epoch)
+        if loops is not None:
+            la = (sdr_trk_loop_t * len(chans))()
+            for i, lp in enumerate(loops):
+                la[i] = sdr_trk_loop_t(lp["len_code"], 1 if lp.get("costas", 1) else 0, lp["cn0"])
+            self.p = lib.sdr_trk_plan_new_loop(C.byref(self._buf), len(chans), ca, la, self.n_pos,
+                                               _dp(self.pos), n_epoch)
+        else:
+            self.p = lib.sdr_trk_plan_new(C.byref(self._buf), len(chans), ca, self.n_pos,
+                                          _dp(self.pos), n_epoch)
         if not self.p:
             raise ValueError("sdr_trk_plan_new failed")
 
@@ -382,6 +398,13 @@ class TrkPlan:
         P = np.zeros((self.n_ch, self.n_epoch), np.complex64)
         lib.sdr_trk_plan_prompt(self.p, _ptr(P))
         return P
+
+    def state(self):
+        """Closed-loop plans: (n_ch, n_epoch, 6) fd, coff, phi, cn0, lock, idle."""
+        st = np.zeros((self.n_ch, self.n_epoch, 6), np.float64)
+        if not lib.sdr_trk_plan_state(self.p, _ptr(st)):
+            raise ValueError("sdr_trk_plan_state: not a closed-loop plan")
+        return st
 
     def timing(self, on=True):
         lib.sdr_trk_plan_timing(self.p, 1 if on else 0)

@@ -1,0 +1,122 @@
+"""Closed-loop tracking (src/sdr_ch.c:488-624, 694-763): FLL, PLL, DLL, C/N0
+and the loss decision around the standard correlators.
+
+CPU tests pin the oracle's loop with what the physics fixes: channels
+started 40 Hz and 0.2 chip off a synthetic satellite pull in to its Doppler
+and code phase and report its C/N0; a channel with no signal is declared
+lost; before the first frequency update (epochs 0-1) the loop's correlations
+equal the open-loop tracking restatement's.  GPU tests compare the
+persistent-workgroup loop with the oracle: bit-identical until the first
+discriminator (atan, log10: the GPU's and the host's libm may differ in the
+last bit), then within stated tolerances, with the same lock and loss
+decisions.
+"""
+import numpy as np
+import pytest
+
+CHIP = 1.0 / 1.023e6
+
+
+def loop_case(seed=4, n_epoch=2500, n_ch=3, n_vis=2, cn0=45.0):
+    from pocketsdr_amd import workloads
+    w = workloads.trk_l1ca(seed=seed, fs=4e6, n_ch=n_ch, n_epoch=n_epoch, n_vis=n_vis, nposx=0,
+                           cn0=(cn0, cn0))
+    truth = {s["prn"]: s for s in w["truth"]}
+    chans = []
+    for ch in w["chans"]:
+        ch = dict(ch)
+        tr = truth.get(ch["prn"])
+        if tr is not None:                 # acquisition errors
+            ch["fd0"] = tr["fd"] + 40.0
+            ch["coff0"] = tr["tau"] + 0.2 * CHIP
+        ch["phi0"] = 0.0
+        chans.append(ch)
+    loops = [dict(len_code=1023, costas=1, cn0=40.0) for _ in chans]
+    return w, chans, loops, w["pos"][:4], truth
+
+
+def truth_track(tr, e, T=1e-3):
+    t = (e + 1) * T
+    fd = tr["fd"] + tr["drift"] * t
+    coff = tr["tau"] - (tr["fd"] * t + 0.5 * tr["drift"] * t * t) / tr["fc"]
+    return fd, coff
+
+
+def test_oracle_loop_pull_in_and_loss(orc):
+    w, chans, loops, pos, truth = loop_case()
+    for c, ch in enumerate(chans):
+        st, corr, P = orc.trkl_run(ch, 1023, 1, 40.0, ch["code"], w["buff"], w["n_epoch"], pos)
+        tr = truth.get(ch["prn"])
+        if tr is None:
+            assert st[-1, 5] == 1                      # lost
+            continue
+        assert not st[:, 5].any() and st[-1, 4] == w["n_epoch"]
+        for e in (999, 2499):                           # end of the FLL, end of the run
+            fd, coff = truth_track(tr, e)
+            assert abs(st[e, 0] - fd) < 2.0, (c, e, st[e, 0] - fd)
+        fd, coff = truth_track(tr, 2499)
+        assert abs(st[-1, 1] - coff) < 0.05 * CHIP, (c, (st[-1, 1] - coff) / CHIP)
+        assert abs(st[-1, 3] - 45.0) < 3.0, st[-1, 3]
+
+
+def test_oracle_loop_first_epochs_are_open_loop(orc):
+    """Epochs 0 and 1 precede any frequency update (the FLL needs two
+    prompts), so they equal the open-loop restatement at constant Doppler."""
+    w, chans, loops, pos, truth = loop_case(n_epoch=4)
+    for ch in chans:
+        st, corr, P = orc.trkl_run(ch, 1023, 1, 40.0, ch["code"], w["buff"], 4, pos)
+        ol = dict(ch, drift=0.0)
+        rc, rC, rs = orc.trk_run(ol, ch["code"], w["buff"], 4, pos)
+        assert np.array_equal(corr[:2].view(np.uint32), rc[:2].view(np.uint32))
+        rP = orc.trk_stitch(rC, ch["N"])
+        assert np.array_equal(P[:2].view(np.uint32), rP[:2].view(np.uint32))
+        assert np.array_equal(st[:2, 1], rs[:2, 1]) and np.array_equal(st[:2, 2], rs[:2, 0])
+
+
+@pytest.mark.gpu
+def test_trk_loop_vs_oracle(sdr, orc):
+    w, chans, loops, pos, truth = loop_case()
+    plan = sdr.TrkPlan(w["buff"], chans, pos, w["n_epoch"], loops=loops)
+    plan.run()
+    st = plan.state()
+    corr, _ = plan.result()
+    P = plan.prompt()
+    plan.close()
+    for c, ch in enumerate(chans):
+        rs, rc, rP = orc.trkl_run(ch, 1023, 1, 40.0, ch["code"], w["buff"], w["n_epoch"], pos)
+        # before the first discriminator: bit-identical
+        assert np.array_equal(corr[c, :2].view(np.uint32), rc[:2].view(np.uint32)), c
+        assert np.array_equal(st[c, :2, :3], rs[:2, :3]), c
+        # lock and loss decisions
+        assert np.array_equal(st[c, :, 5], rs[:, 5]), c
+        live = rs[:, 5] == 0
+        assert np.array_equal(st[c, live, 4], rs[live, 4]), c
+        # loop states: Doppler (Hz), code offset (s), C/N0 (dB-Hz); measured
+        # on MI355X: the satellites' channels bit-identical over all 2500
+        # epochs, the noise channel within 1.2e-13 Hz after a last-bit atan
+        # difference at epoch 402
+        assert np.max(np.abs(st[c, live, 0] - rs[live, 0])) < 1e-6, c
+        assert np.max(np.abs(st[c, live, 1] - rs[live, 1])) < 1e-6 * CHIP, c
+        assert np.max(np.abs(st[c, live, 3] - rs[live, 3])) < 1e-6, c
+        if ch["prn"] in truth:
+            a = np.abs(P[c, live]) ** 2
+            b = np.abs(rP[live]) ** 2
+            assert abs(a.mean() - b.mean()) < 1e-3 * b.mean(), c
+
+
+@pytest.mark.gpu
+def test_trk_loop_rerun_and_open_loop_unchanged(sdr, orc):
+    """A second run restarts from the initial states (same results); the
+    open-loop plan has no loop state."""
+    w, chans, loops, pos, truth = loop_case(n_epoch=600)
+    plan = sdr.TrkPlan(w["buff"], chans, pos, w["n_epoch"], loops=loops)
+    plan.run()
+    s1 = plan.state()
+    plan.run()
+    s2 = plan.state()
+    plan.close()
+    assert np.array_equal(s1, s2)
+    ol = sdr.TrkPlan(w["buff"], chans, pos, w["n_epoch"])
+    with pytest.raises(ValueError):
+        ol.state()
+    ol.close()
