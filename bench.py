@@ -309,11 +309,12 @@ def cpu_leg(args, orc, plan, buffs, sigs):
     # interpreter lock)
     from concurrent.futures import ThreadPoolExecutor
     nth = max(1, min(16, os.cpu_count() or 1))
-    tasks = []
+    tasks, mt_parts = [], []
     for i, s in enumerate(sigs):
         nq = min(4 * args.cpu_prns, s["code_fft"].shape[0])
         nb = min(args.cpu_bins, len(s["fds"])) if args.cpu_bins else len(s["fds"])
         tasks += [(i, q, b0, min(nb, b0 + 8)) for q in range(nq) for b0 in range(0, nb, 8)]
+        mt_parts.append("%s %d PRNs" % (s["name"], nq))
 
     def task(tk):
         i, q, b0, b1 = tk
@@ -330,8 +331,8 @@ def cpu_leg(args, orc, plan, buffs, sigs):
     t_mt = time.perf_counter() - t0
     log("cpu %d threads: %.1f cells/s over %.1f s" % (nth, n_mt / t_mt, t_mt))
     base_mt = dict(value=round(n_mt / t_mt, 2), unit="cells/s", cores=nth, kind="port",
-                   sample="%s PRNs of every signal over all bins and blocks as (signal, PRN, 8-bin) "
-                          "tasks on %d host threads (%.1f s)" % (4 * args.cpu_prns, nth, t_mt))
+                   sample="%s over all bins and blocks as (signal, PRN, 8-bin) tasks on %d host "
+                          "threads (%.1f s)" % (", ".join(mt_parts), nth, t_mt))
     worst = 0.0
     for i, q, P in surf:
         s = sigs[i]
@@ -586,7 +587,7 @@ def main():
     ap.add_argument("--skip-l6", action="store_true")
     ap.add_argument("--l6-channels", type=int, default=16)
     ap.add_argument("--l6-epochs", type=int, default=250)
-    ap.add_argument("--cpu-prns", type=int, default=4)
+    ap.add_argument("--cpu-prns", type=int, default=16)
     ap.add_argument("--workload", default="multi", choices=["multi", "l1ca", "l1cp"])
     ap.add_argument("--cpu-bins", type=int, default=0,
                     help="CPU sample: only the first this many Doppler bins (0: all)")
