@@ -661,15 +661,22 @@ int h_splits(int M, int *m1, int nmax)
     return n;
 }
 
-// A-operand table of the R-point DFT (BF16: hi part, then lo part)
+// A-operand table of the R-point DFT (BF16: hi part, then lo part), cached
+// per (R, dir, dt); amat_radix returns the R of a cached table (0: none)
+static std::mutex amat_mtx;
+static std::map<int, uint4 *> amat_cache;
+static int amat_radix(const uint4 *p)
+{
+    std::lock_guard<std::mutex> lk(amat_mtx);
+    for (const auto &kv : amat_cache) if (kv.second == p) return kv.first / 4;
+    return 0;
+}
 static const uint4 *make_amat(int R, int dir, int dt)
 {
-    static std::mutex mtx;
-    static std::map<int, uint4 *> cache;
-    std::lock_guard<std::mutex> lk(mtx);
+    std::lock_guard<std::mutex> lk(amat_mtx);
     const int key = (R * 2 + dir) * 2 + dt;
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
+    auto it = amat_cache.find(key);
+    if (it != amat_cache.end()) return it->second;
     const int KS = (2 * R + 15) / 16, RT = (2 * R + 31) / 32;
     const int n = 64 * KS * RT;         // uint4 per part
     uint4 *d;
@@ -682,7 +689,7 @@ static const uint4 *make_amat(int R, int dir, int dt)
     }
     PSA_CHECK(hipGetLastError());
     PSA_CHECK(hipDeviceSynchronize());
-    cache[key] = d;
+    amat_cache[key] = d;
     return d;
 }
 
@@ -757,10 +764,14 @@ const HPlan *hfft_plan(int M, int dt, int M1)
         p->tw2[dir] = make_twiddle(M2, sg, 1.0);
         p->tfull[dir] = make_digit_twiddle(M, M2, sg, dir == 0 ? p->osc_fc : p->osc_ir);
         if (dir == 1 && dt == HDT_F16) p->tfullh = pack_table_f16(p->tfull[dir], M);
-        const HSpec *cs = col_spec(M1), *rs = row_spec(M2);
-        for (int s = 0; s < 3; s++) {
-            p->am1[dir][s] = cs->r[s] > 1 ? make_amat(cs->r[s], dir, dt) : nullptr;
-            p->am2[dir][s] = rs->r[s] > 1 ? make_amat(rs->r[s], dir, dt) : nullptr;
+        // the two workgroup shapes of a length may split it into different
+        // radices: each shape has its own tables
+        for (int v = 0; v < 2; v++) {
+            const HSpec *cs = col_spec(M1, v), *rs = row_spec(M2, v);
+            for (int s = 0; s < 3; s++) {
+                if (cs) p->am1[v][dir][s] = cs->r[s] > 1 ? make_amat(cs->r[s], dir, dt) : nullptr;
+                if (rs) p->am2[v][dir][s] = rs->r[s] > 1 ? make_amat(rs->r[s], dir, dt) : nullptr;
+            }
         }
     }
     plans[key] = p;
@@ -770,6 +781,7 @@ const HPlan *hfft_plan(int M, int dt, int M1)
 void h_col_args(const HPlan *hp, int dir, PassArgs &a, int var)
 {
     const HSpec *cs = col_spec(hp->M1, var);
+    if (!cs) PSA_FAIL("MFMA pass: no column shape %d for length %d", var, hp->M1);
     a.L = hp->M1;
     a.nsub = hp->M2;
     a.G = cs->G;
@@ -783,12 +795,13 @@ void h_col_args(const HPlan *hp, int dir, PassArgs &a, int var)
     a.tfull = nullptr;
     a.osc = 1.0f;
     a.kappa = 1.0f;
-    for (int s = 0; s < 3; s++) a.amat[s] = hp->am1[dir][s];
+    for (int s = 0; s < 3; s++) a.amat[s] = hp->am1[var][dir][s];
 }
 
 void h_row_args(const HPlan *hp, int dir, PassArgs &a, int var)
 {
     const HSpec *rs = row_spec(hp->M2, var);
+    if (!rs) PSA_FAIL("MFMA pass: no row shape %d for length %d", var, hp->M2);
     a.L = hp->M2;
     a.nsub = hp->M1;
     a.G = rs->G;
@@ -802,7 +815,7 @@ void h_row_args(const HPlan *hp, int dir, PassArgs &a, int var)
     a.tfull = nullptr;
     a.osc = 1.0f;
     a.kappa = 1.0f;
-    for (int s = 0; s < 3; s++) a.amat[s] = hp->am2[dir][s];
+    for (int s = 0; s < 3; s++) a.amat[s] = hp->am2[var][dir][s];
 }
 
 template <int LM, int SM, int DT, int L, int R0, int R1, int R2, int G, bool NF, int NTH>
@@ -817,6 +830,9 @@ static void hlaunch_t(const PassArgs &a, int nbatch, hipStream_t st)
     if (nb <= 0) return;
     if (nb > 0x7FFFFFFFLL) PSA_FAIL("FFT launch too large (%lld workgroups)", nb);
     if (NF && (a.in_nstride != 1 || a.out_nstride != 1)) PSA_FAIL("MFMA row pass needs unit element strides");
+    if (amat_radix(a.amat[0]) != R0 || (R1 > 1 && amat_radix(a.amat[1]) != R1) ||
+        (R2 > 1 && amat_radix(a.amat[2]) != R2))
+        PSA_FAIL("MFMA pass: DFT tables do not match length %d (%d x %d x %d)", L, R0, R1, R2);
     pass_grid = nb * NTH;
     hipLaunchKernelGGL((mpass_kernel<LM, SM, DT, L, R0, R1, R2, G, NF, NTH>), dim3((unsigned)nb), dim3(NTH), 0,
                        st, a, ngroups);

@@ -221,7 +221,8 @@ struct HPlan {
     float2 *tw1[2] = {}, *tw2[2] = {};   // w_L^m of the two lengths, [dir]
     float2 *tfull[2] = {};               // inter-pass twiddle by output index, [dir]
     uint32_t *tfullh = nullptr;          // FP16 words of tfull[1] (FP16 plans)
-    const uint4 *am1[2][3] = {}, *am2[2][3] = {};   // [dir][stage]
+    // DFT tables of each workgroup shape's radices, [shape][dir][stage]
+    const uint4 *am1[2][2][3] = {}, *am2[2][2][3] = {};
     // power-of-two scales: forward column / row pass outputs, inverse row
     // pass output (folded into tfull where a pass has the twiddle)
     float osc_fc = 1, osc_fr = 1, osc_ir = 1;
