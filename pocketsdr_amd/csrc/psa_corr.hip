@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
         TrkEpoch q;
         q.ix = ix;
         trk_mix_params(buffN, ix, N, ch.fs, fc, ph, q.p0, q.p1, q.nseg, q.s);
-        q.pad = 0;
+        q.idle = 0;
         q.cs = coff * ch.fs;
         ep[(long long)c * n_epoch + e] = q;
     });
@@ -582,6 +582,11 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const long long o = blockIdx.x;          // channel * n_epoch + epoch
     const int ch = (int)(o / n_epoch);
     const TrkEpoch q = ep[o];
+    if (q.idle) {                            // uniform over the workgroup
+        for (int t = threadIdx.x; t < n_pos; t += TRK_THREADS) corr_out[o * n_pos + t] = make_float2(0.0f, 0.0f);
+        if (threadIdx.x < 2) C_out[2 * o + threadIdx.x] = make_float2(0.0f, 0.0f);
+        return;
+    }
     for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
     const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn};
     trk_correlate<EXACT, SAMEQ>(q, buf, N, n_pos, pos, codes + (long long)ch * code_ch_stride,
@@ -593,12 +598,17 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
 // track_sig (src/sdr_ch.c:488-620, 694-763) for signals with no secondary
 // code, bump-jump or L6 branch.  One persistent workgroup per channel runs the
 // channel's epochs in order: lane 0 advances the NCO and code offset, the
-// workgroup mixes and correlates the epoch (trk_correlate), lane 0 stitches
-// the prompt and updates the loop filters.  The loop constants are the
+// workgroup mixes and correlates the epoch (trk_correlate) on the taps the
+// loop reads (P, E, L and the noise tap, TL_NTAP), lane 0 stitches the prompt
+// and updates the loop filters.  Every epoch's mixing parameters go to ep
+// (idle after a loss); trk_corr_kernel then correlates all taps of all
+// epochs from them in one launch over the whole GPU.  The taps are computed
+// independently of each other, so the loop's four equal the full launch's.  The loop constants are the
 // reference's defaults (src/sdr_ch.c:29-49).  The reference build's fused
 // multiply-adds are written as explicit fma; atan and log10 are the GPU's, so
 // results follow the reference to the last bits of those functions.
 // ===========================================================================
+#define TL_NTAP 4             // taps the loop reads: P, E, L, noise
 #define TL_T_DLL 0.02         // T_DLL
 #define TL_T_CN0 0.5          // T_CN0
 #define TL_T_FPULLIN 1.0      // T_FPULLIN
@@ -646,7 +656,7 @@ __device__ __forceinline__ void tl_advance(TrkLoopCh &S, int N, int buffN, TrkEp
     else if (S.coff < 0.0) { S.coff = S.coff + S.T; S.lock++; }
     q.ix = (int)(((long long)S.ep * N) % buffN);
     trk_mix_params(buffN, q.ix, N, S.fs, fc, S.phi, q.p0, q.p1, q.nseg, q.s);
-    q.pad = 0;
+    q.idle = 0;
     q.cs = S.coff * S.fs;
 }
 
@@ -737,8 +747,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
     const double *__restrict__ pos, const uint32_t *__restrict__ codes, long long code_ch_stride,
     long long code_copy_stride, int NP, const uint32_t *__restrict__ carr_g,
-    TrkLoopCh *__restrict__ chs, TrkLoopOut *__restrict__ out, float2 *__restrict__ corr_out,
-    float2 *__restrict__ C_out)
+    TrkLoopCh *__restrict__ chs, TrkLoopOut *__restrict__ out, TrkEpoch *__restrict__ ep)
 {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t carr[256];
@@ -765,12 +774,10 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
         if (threadIdx.x == 0) tl_advance(S, N, buffN, qs);
         __syncthreads();
         const TrkEpoch q = qs;
-        trk_correlate<EXACT, SAMEQ>(q, buf, N, n_pos, pos, cbase, code_copy_stride, NP, L, tap, Ch);
+        trk_correlate<EXACT, SAMEQ>(q, buf, N, TL_NTAP, pos, cbase, code_copy_stride, NP, L, tap, Ch);
         const long long o = (long long)c * n_epoch + e;
-        for (int t = threadIdx.x; t < n_pos; t += TRK_THREADS) corr_out[o * n_pos + t] = tap[t];
         if (threadIdx.x == 0) {
-            C_out[2 * o] = Ch[0];
-            C_out[2 * o + 1] = Ch[1];
+            ep[o] = q;
             TrkLoopOut r;
             tl_update(S, N, tap, Ch, r);
             out[o] = r;
@@ -785,6 +792,9 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
         r.lock = S.lock; r.idle = 1;
         r.P = make_float2(0.0f, 0.0f);
         out[(long long)c * n_epoch + e2] = r;
+        TrkEpoch qi = {};
+        qi.idle = 1;
+        ep[(long long)c * n_epoch + e2] = qi;
     }
     if (threadIdx.x == 0) chs[c] = S;
 }
@@ -980,7 +990,7 @@ sdr_trk_plan_t *sdr_trk_plan_new(const sdr_buff_t *buff, int n_ch,
 sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch, const sdr_trk_ch_t *chs,
     const sdr_trk_loop_t *loops, int n_pos, const double *pos, int n_epoch)
 {
-    if (!loops || n_pos < 4) {
+    if (!loops || n_pos < TL_NTAP) {
         fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: need loop parameters and the P, E, L, N taps\n");
         return nullptr;
     }
@@ -1034,7 +1044,7 @@ static void trk_loop_launch_t(sdr_trk_plan_t *p, size_t shm)
     hipLaunchKernelGGL((trk_loop_kernel<EXACT, SAMEQ>), dim3(p->n_ch), dim3(TRK_THREADS), shm, p->st,
                        p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_pos, p->d_codes,
                        p->code_ch_stride, p->code_copy_stride, p->NP, dev_carrier(), p->d_lch, p->d_lout,
-                       p->d_corr, p->d_C);
+                       p->d_ep);
 }
 
 static void trk_loop_launch(sdr_trk_plan_t *p, size_t shm)
@@ -1067,6 +1077,8 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
         if (p->timing) PSA_CHECK(hipEventRecord(p->ev[0], st));
         const size_t shm = sizeof(uint32_t) * 2 * (size_t)(((p->N + 3) / 4 + 255) / 256 * 256);
         trk_loop_launch(p, shm);
+        PSA_CHECK(hipGetLastError());
+        trk_launch(p, shm, (unsigned)ne);          // all taps of every epoch
         PSA_CHECK(hipGetLastError());
         if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
         PSA_CHECK(hipStreamSynchronize(st));

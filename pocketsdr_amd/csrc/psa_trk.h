@@ -16,7 +16,7 @@ struct TrkChDev {
 struct TrkEpoch {
     int ix, nseg;
     uint32_t p0, p1, s;
-    int pad;
+    int idle;           // 1: no correlation (the channel has lost lock)
     double cs;
 };
 
