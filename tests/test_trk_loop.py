@@ -17,9 +17,9 @@ import pytest
 CHIP = 1.0 / 1.023e6
 
 
-def loop_case(seed=4, n_epoch=2500, n_ch=3, n_vis=2, cn0=45.0):
+def loop_case(seed=4, n_epoch=2500, n_ch=3, n_vis=2, cn0=45.0, nposx=0):
     from pocketsdr_amd import workloads
-    w = workloads.trk_l1ca(seed=seed, fs=4e6, n_ch=n_ch, n_epoch=n_epoch, n_vis=n_vis, nposx=0,
+    w = workloads.trk_l1ca(seed=seed, fs=4e6, n_ch=n_ch, n_epoch=n_epoch, n_vis=n_vis, nposx=nposx,
                            cn0=(cn0, cn0))
     truth = {s["prn"]: s for s in w["truth"]}
     chans = []
@@ -32,7 +32,7 @@ def loop_case(seed=4, n_epoch=2500, n_ch=3, n_vis=2, cn0=45.0):
         ch["phi0"] = 0.0
         chans.append(ch)
     loops = [dict(len_code=1023, costas=1, cn0=40.0) for _ in chans]
-    return w, chans, loops, w["pos"][:4], truth
+    return w, chans, loops, w["pos"], truth
 
 
 def truth_track(tr, e, T=1e-3):
@@ -75,7 +75,9 @@ def test_oracle_loop_first_epochs_are_open_loop(orc):
 
 @pytest.mark.gpu
 def test_trk_loop_vs_oracle(sdr, orc):
-    w, chans, loops, pos, truth = loop_case()
+    # the loop reads taps 0-3; the 8 plot taps come from the full-GPU pass
+    w, chans, loops, pos, truth = loop_case(nposx=8)
+    assert len(pos) == 12
     plan = sdr.TrkPlan(w["buff"], chans, pos, w["n_epoch"], loops=loops)
     plan.run()
     st = plan.state()
@@ -102,6 +104,11 @@ def test_trk_loop_vs_oracle(sdr, orc):
             a = np.abs(P[c, live]) ** 2
             b = np.abs(rP[live]) ** 2
             assert abs(a.mean() - b.mean()) < 1e-3 * b.mean(), c
+            # the satellites' loop states are bit-identical, so are all taps
+            # of every epoch (filled from the loop's recorded mixing states)
+            assert np.array_equal(corr[c].view(np.uint32), rc.view(np.uint32)), c
+        # epochs after a loss of lock correlate nothing
+        assert not np.any(corr[c, ~live]), c
 
 
 @pytest.mark.gpu
