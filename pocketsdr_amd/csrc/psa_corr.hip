@@ -315,6 +315,7 @@ struct TrkLds {
     float4 *c12s;                // per tap: half-1 / half-2 sums (I, Q)
     int2 *jk;                    // per tap: code offset j, bank k
     int *sgn;                    // the data-sign decision of the taps 0-2
+    int4 *psum;                  // per wave: word-range partial sums (split taps)
 };
 
 // One epoch's sdr_mix_carr + sdr_corr_std by the whole workgroup: the
@@ -401,7 +402,18 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     const int lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
     const int QOFF = PSA_N_CODES * NP;       // byte offset of the Q banks
     // ---- taps: wave `wave` owns t = t0 + s nwave, s < TRK_TPW ----
-    for (int t0 = wave; t0 < n_pos; t0 += nwave * TRK_TPW) {
+    // With at most half as many taps as waves (the closed loop's four), the
+    // words are split instead: wave w takes tap w % n_pos over word range
+    // part w / n_pos, the parts' exact integer sums meet in LDS and the
+    // wave of part 0 runs the epilogue, so the results equal the unsplit
+    // ones.
+    const int nsplit = n_pos > 0 && 2 * n_pos <= nwave ? nwave / n_pos : 1;
+    const int part = nsplit > 1 ? wave / n_pos : 0;
+    const int nstep = (Nq + 255) / 256, sps = (nstep + nsplit - 1) / nsplit;
+    const int wlo = min(part * sps, nstep) * 256, whi = min((part + 1) * sps * 256, Nq);
+    // (split: every wave runs one pass, waves past the last part with an
+    // empty word range, so the barrier below is reached by all)
+    for (int t0 = nsplit > 1 ? wave % n_pos : wave; t0 < n_pos; t0 += nwave * TRK_TPW) {
         int jt[TRK_TPW], co[TRK_TPW];
         // per tap and lane: the running sum of every word so far (tot), its
         // value when the step holding sample j began (snap) and the half-1
@@ -422,7 +434,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
         }
         // a step covers 256 words: lane l takes words 4 l .. 4 l + 3 of it with
         // one 16-byte load per tap (planes padded with zero words to Nq4)
-        for (int wb = 0; wb < Nq; wb += 256) {
+        for (int wb = wlo; wb < whi; wb += 256) {
             const int w = wb + 4 * lane;
             const uint4 xI = *reinterpret_cast<const uint4 *>(planeI + w);
             const uint4 xQ = *reinterpret_cast<const uint4 *>(planeQ + w);
@@ -462,6 +474,28 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
                     h1[s][1] = snap[s][1] + bQ;
                 }
             }
+        }
+        if (nsplit > 1) {
+            // a part wholly before the step holding j is all half 1
+#pragma unroll
+            for (int s = 0; s < TRK_TPW; s++) {
+                if (4 * whi <= jt[s]) {
+#pragma unroll
+                    for (int c = 0; c < 2; c++) h1[s][c] = snap[s][c] = tot[s][c];
+                }
+            }
+            if (part > 0) {
+                // this part's four sums, reduced over the lanes, to LDS
+                int p[4] = {h1[0][0], h1[0][1], tot[0][0] - snap[0][0], tot[0][1] - snap[0][1]};
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) p[c] += __shfl_xor(p[c], o, 64);
+                }
+                if (lane == 0) L.psum[wave] = make_int4(p[0], p[1], p[2], p[3]);
+            }
+            __syncthreads();
+            if (part > 0) continue;
         }
         // ---- per-tap float epilogue (the owning wave): the integer sums hold
         // the first 16 floor(len / 16) samples of each half, the rest are
@@ -521,10 +555,15 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             }
 #pragma unroll
             for (int o = 8; o >= 1; o >>= 1) p[0] += __shfl_xor(p[0], o, 64);
-            float f1I = (float)__builtin_amdgcn_readlane(p[0], 0);
-            float f1Q = (float)__builtin_amdgcn_readlane(p[0], 16);
-            float f2I = (float)__builtin_amdgcn_readlane(p[0], 32);
-            float f2Q = (float)__builtin_amdgcn_readlane(p[0], 48);
+            int4 ex = make_int4(0, 0, 0, 0);    // the other parts' sums (split)
+            for (int q = 1; q < nsplit; q++) {
+                const int4 v = L.psum[t + q * n_pos];
+                ex.x += v.x; ex.y += v.y; ex.z += v.z; ex.w += v.w;
+            }
+            float f1I = (float)(__builtin_amdgcn_readlane(p[0], 0) + ex.x);
+            float f1Q = (float)(__builtin_amdgcn_readlane(p[0], 16) + ex.y);
+            float f2I = (float)(__builtin_amdgcn_readlane(p[0], 32) + ex.z);
+            float f2Q = (float)(__builtin_amdgcn_readlane(p[0], 48) + ex.w);
             for (int i = 0; i < j - V1; i++) {
                 f1I += (float)__builtin_amdgcn_readlane(fI, i);
                 f1Q += (float)__builtin_amdgcn_readlane(fQ, i);
@@ -578,6 +617,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     __shared__ float4 c12s[PSA_MAX_CORR];
     __shared__ int2 jk[PSA_MAX_CORR];
     __shared__ int sgn;
+    __shared__ int4 psum[TRK_THREADS / 64];
     const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
     const long long o = blockIdx.x;          // channel * n_epoch + epoch
     const int ch = (int)(o / n_epoch);
@@ -588,7 +628,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
         return;
     }
     for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
-    const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn};
+    const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn, psum};
     trk_correlate<EXACT, SAMEQ>(q, buf, N, n_pos, pos, codes + (long long)ch * code_ch_stride,
                                 code_copy_stride, NP, L, corr_out + o * n_pos, C_out + 2 * o);
 }
@@ -756,6 +796,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
     __shared__ float2 tap[PSA_MAX_CORR];
     __shared__ float2 Ch[2];
     __shared__ int sgn, stop;
+    __shared__ int4 psum[TRK_THREADS / 64];
     __shared__ TrkEpoch qs;
     __shared__ TrkLoopCh S;
     const int c = blockIdx.x;
@@ -766,7 +807,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
         stop = S.idle;
     }
     __syncthreads();
-    const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn};
+    const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn, psum};
     const uint32_t *cbase = codes + (long long)c * code_ch_stride;
     int e = 0;
     for (; e < n_epoch; e++) {

@@ -272,6 +272,26 @@ def test_search_plan_multi(sdr, orc, multi_acq, tune):
     plan.close()
 
 
+@pytest.mark.parametrize("n_pos", [1, 3, 4])
+def test_trk_plan_few_taps(sdr, orc, n_pos):
+    """At most half as many taps as the workgroup's waves (the closed loop
+    correlates four): each tap's words are split over several waves and the
+    parts' sums combined; bit-exact as with many taps, on a ring whose length
+    is not a multiple of N (epochs split at the wrap)."""
+    from pocketsdr_amd import workloads
+    w = workloads.trk_l1ca(seed=5, fs=24e6, n_ch=4, n_epoch=10, n_vis=2, nposx=8, buff_epochs=7)
+    buff = np.concatenate([w["buff"], w["buff"][:777]])
+    pos = w["pos"][:n_pos]
+    plan = sdr.TrkPlan(buff, w["chans"], pos, w["n_epoch"])
+    plan.run()
+    corr, Cc = plan.result()
+    plan.close()
+    for c, ch in enumerate(w["chans"]):
+        rc, rC, _ = orc.trk_run(ch, ch["code"], buff, w["n_epoch"], pos)
+        assert np.array_equal(corr[c].view(np.uint32), rc.view(np.uint32)), c
+        assert np.array_equal(Cc[c].view(np.uint32), rC.view(np.uint32)), c
+
+
 def test_trk_plan_full_size(sdr, orc):
     """The tracking configuration of the bench at full size (64 channels x
     1000 epochs at 24 Msps, 44 taps): three channels, the first, a middle and
