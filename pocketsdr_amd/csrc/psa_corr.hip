@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
 
 // Fused mix + correlate: one workgroup per (channel, epoch).  The epoch's N
 // samples are carrier-mixed into LDS as separate int8 I and Q planes.  Each
-// wave then owns up to TRK_TPW taps and walks all data words (4 samples, one
+// wave then owns up to TPW taps and walks all data words (4 samples, one
 // per lane per step): per word one aligned code-word load per tap and v_dot4
 // products.  The bank of each tap is read circularly shifted by j from a
 // doubled copy (bank twice in a row, 4 byte-shifted copies so every word
@@ -322,7 +322,7 @@ struct TrkLds {
 // epoch's state q, the channel's code banks from cbase; corr_out receives the
 // n_pos taps and C_out the two halves of tap 0 (global or LDS).  Starts and
 // ends with the workgroup in step (barriers inside).
-template <bool EXACT, bool SAMEQ>
+template <bool EXACT, bool SAMEQ, int TPW = TRK_TPW, int NTH = TRK_THREADS>
 __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *__restrict__ buf, int N,
                                               int n_pos, const double *__restrict__ pos,
                                               const uint32_t *__restrict__ cbase, long long code_copy_stride,
@@ -335,7 +335,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     float4 *c12s = L.c12s;
     int2 *jk = L.jk;
     // sdr_corr_std tap geometry (src/sdr_func.c:988-991)
-    for (int i = threadIdx.x; i < n_pos; i += TRK_THREADS) {
+    for (int i = threadIdx.x; i < n_pos; i += NTH) {
         const double p = q.cs + pos[i];
         int j = (int)floor(p);
         const int k = (int)((p - j) * PSA_N_CODES);
@@ -349,7 +349,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     // A word whose 4 samples lie in one ring segment takes its 4 IF bytes
     // with two aligned dword loads and a byte align (the ring is padded by a
     // dword) and steps the phase by s; other words go sample by sample.
-    for (int w = threadIdx.x; w < Nq4; w += TRK_THREADS) {
+    for (int w = threadIdx.x; w < Nq4; w += NTH) {
         const int i0 = 4 * w;
         uint32_t wi = 0, wq = 0;
         const bool s1 = i0 + 4 <= N && i0 + 4 <= q.nseg, s2 = i0 + 4 <= N && i0 >= q.nseg;
@@ -401,7 +401,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
     const int QOFF = PSA_N_CODES * NP;       // byte offset of the Q banks
-    // ---- taps: wave `wave` owns t = t0 + s nwave, s < TRK_TPW ----
+    // ---- taps: wave `wave` owns t = t0 + s nwave, s < TPW ----
     // With at most half as many taps as waves (the closed loop's four), the
     // words are split instead: wave w takes tap w % n_pos over word range
     // part w / n_pos, the parts' exact integer sums meet in LDS and the
@@ -413,15 +413,15 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     const int wlo = min(part * sps, nstep) * 256, whi = min((part + 1) * sps * 256, Nq);
     // (split: every wave runs one pass, waves past the last part with an
     // empty word range, so the barrier below is reached by all)
-    for (int t0 = nsplit > 1 ? wave % n_pos : wave; t0 < n_pos; t0 += nwave * TRK_TPW) {
-        int jt[TRK_TPW], co[TRK_TPW];
+    for (int t0 = nsplit > 1 ? wave % n_pos : wave; t0 < n_pos; t0 += nwave * TPW) {
+        int jt[TPW], co[TPW];
         // per tap and lane: the running sum of every word so far (tot), its
         // value when the step holding sample j began (snap) and the half-1
         // sum completed in that step (h1); half 2 is then tot - snap (exact
         // int32 sums)
-        int tot[TRK_TPW][2], snap[TRK_TPW][2], h1[TRK_TPW][2];
+        int tot[TPW][2], snap[TPW][2], h1[TPW][2];
 #pragma unroll
-        for (int s = 0; s < TRK_TPW; s++) {
+        for (int s = 0; s < TPW; s++) {
             const int t = t0 + s * nwave;
             const int2 g = jk[t < n_pos ? t : 0];
             const int gx = __builtin_amdgcn_readfirstlane(g.x), gy = __builtin_amdgcn_readfirstlane(g.y);
@@ -438,16 +438,16 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             const int w = wb + 4 * lane;
             const uint4 xI = *reinterpret_cast<const uint4 *>(planeI + w);
             const uint4 xQ = *reinterpret_cast<const uint4 *>(planeQ + w);
-            uint4 cI[TRK_TPW], cQ[TRK_TPW];
+            uint4 cI[TPW], cQ[TPW];
 #pragma unroll
-            for (int s = 0; s < TRK_TPW; s++) {
+            for (int s = 0; s < TPW; s++) {
                 // scalar base of the step, lane offset 16 l bytes
                 const uint32_t *cs = cbase + co[s] + wb;
                 cI[s] = load_u4(cs + 4 * lane);
                 cQ[s] = SAMEQ ? cI[s] : load_u4(cs + (QOFF >> 2) + 4 * lane);
             }
 #pragma unroll
-            for (int s = 0; s < TRK_TPW; s++) {
+            for (int s = 0; s < TPW; s++) {
                 const int j = jt[s];
                 const uint32_t xi[4] = {xI.x, xI.y, xI.z, xI.w}, xq[4] = {xQ.x, xQ.y, xQ.z, xQ.w};
                 const uint32_t ci[4] = {cI[s].x, cI[s].y, cI[s].z, cI[s].w};
@@ -478,7 +478,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
         if (nsplit > 1) {
             // a part wholly before the step holding j is all half 1
 #pragma unroll
-            for (int s = 0; s < TRK_TPW; s++) {
+            for (int s = 0; s < TPW; s++) {
                 if (4 * whi <= jt[s]) {
 #pragma unroll
                     for (int c = 0; c < 2; c++) h1[s][c] = snap[s][c] = tot[s][c];
@@ -507,7 +507,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
         // reduced together, and the ordered float tail is added to each.
         const uint8_t *pI = (const uint8_t *)planeI, *pQ = (const uint8_t *)planeQ;
 #pragma unroll
-        for (int s = 0; s < TRK_TPW; s++) {
+        for (int s = 0; s < TPW; s++) {
             const int t = t0 + s * nwave;
             if (t >= n_pos) break;
             const int j = jt[s];
@@ -649,6 +649,10 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
 // results follow the reference to the last bits of those functions.
 // ===========================================================================
 #define TL_NTAP 4             // taps the loop reads: P, E, L, noise
+// loop workgroup size: 1024 threads where the correlator fits 128 VGPRs
+// (no -8 nibbles, real codes), else 512
+template <bool EXACT, bool SAMEQ>
+constexpr int trk_loop_threads() { return EXACT || !SAMEQ ? 512 : 1024; }
 #define TL_T_DLL 0.02         // T_DLL
 #define TL_T_CN0 0.5          // T_CN0
 #define TL_T_FPULLIN 1.0      // T_FPULLIN
@@ -782,8 +786,8 @@ __device__ __forceinline__ void tl_update(TrkLoopCh &S, int N, const float2 *tap
     o.P = make_float2(Px, Py);
 }
 
-template <bool EXACT, bool SAMEQ>
-__global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
+template <bool EXACT, bool SAMEQ, int NTH>
+__global__ void __launch_bounds__(NTH) trk_loop_kernel(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
     const double *__restrict__ pos, const uint32_t *__restrict__ codes, long long code_ch_stride,
     long long code_copy_stride, int NP, const uint32_t *__restrict__ carr_g,
@@ -796,12 +800,12 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
     __shared__ float2 tap[PSA_MAX_CORR];
     __shared__ float2 Ch[2];
     __shared__ int sgn, stop;
-    __shared__ int4 psum[TRK_THREADS / 64];
+    __shared__ int4 psum[NTH / 64];
     __shared__ TrkEpoch qs;
     __shared__ TrkLoopCh S;
     const int c = blockIdx.x;
     const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
-    for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
+    for (int i = threadIdx.x; i < 256; i += NTH) carr[i] = carr_g[i];
     if (threadIdx.x == 0) {
         S = chs[c];
         stop = S.idle;
@@ -815,7 +819,8 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
         if (threadIdx.x == 0) tl_advance(S, N, buffN, qs);
         __syncthreads();
         const TrkEpoch q = qs;
-        trk_correlate<EXACT, SAMEQ>(q, buf, N, TL_NTAP, pos, cbase, code_copy_stride, NP, L, tap, Ch);
+        trk_correlate<EXACT, SAMEQ, 1, NTH>(q, buf, N, TL_NTAP, pos, cbase, code_copy_stride, NP, L,
+                                                         tap, Ch);
         const long long o = (long long)c * n_epoch + e;
         if (threadIdx.x == 0) {
             ep[o] = q;
@@ -827,7 +832,7 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_loop_kernel(
         __syncthreads();
     }
     // epochs after a loss of lock: marked idle, state as at the loss
-    for (int e2 = e + threadIdx.x; e2 < n_epoch; e2 += TRK_THREADS) {
+    for (int e2 = e + threadIdx.x; e2 < n_epoch; e2 += NTH) {
         TrkLoopOut r;
         r.fd = S.fd; r.coff = S.coff; r.phi = S.phi; r.cn0 = S.cn0;
         r.lock = S.lock; r.idle = 1;
@@ -1078,11 +1083,12 @@ template <bool EXACT, bool SAMEQ>
 static void trk_loop_launch_t(sdr_trk_plan_t *p, size_t shm)
 {
     static std::once_flag once;
+    constexpr int NTH = trk_loop_threads<EXACT, SAMEQ>();
     std::call_once(once, [] {
-        PSA_CHECK(hipFuncSetAttribute((const void *)trk_loop_kernel<EXACT, SAMEQ>,
+        PSA_CHECK(hipFuncSetAttribute((const void *)trk_loop_kernel<EXACT, SAMEQ, NTH>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     });
-    hipLaunchKernelGGL((trk_loop_kernel<EXACT, SAMEQ>), dim3(p->n_ch), dim3(TRK_THREADS), shm, p->st,
+    hipLaunchKernelGGL((trk_loop_kernel<EXACT, SAMEQ, NTH>), dim3(p->n_ch), dim3(NTH), shm, p->st,
                        p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_pos, p->d_codes,
                        p->code_ch_stride, p->code_copy_stride, p->NP, dev_carrier(), p->d_lch, p->d_lout,
                        p->d_ep);

@@ -74,10 +74,15 @@ def test_oracle_loop_first_epochs_are_open_loop(orc):
 
 
 @pytest.mark.gpu
-def test_trk_loop_vs_oracle(sdr, orc):
-    # the loop reads taps 0-3; the 8 plot taps come from the full-GPU pass
+@pytest.mark.parametrize("variant", ["plain", "exact"])
+def test_trk_loop_vs_oracle(sdr, orc, variant):
+    # the loop reads taps 0-3; the 8 plot taps come from the full-GPU pass;
+    # "exact": -8 nibbles in the IF (the exact-sign correlator form)
     w, chans, loops, pos, truth = loop_case(nposx=8)
     assert len(pos) == 12
+    if variant == "exact":
+        w["buff"] = w["buff"].copy()
+        w["buff"][::97] = 0x88
     plan = sdr.TrkPlan(w["buff"], chans, pos, w["n_epoch"], loops=loops)
     plan.run()
     st = plan.state()
