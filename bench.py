@@ -278,17 +278,34 @@ def bench_acq(args, sdr, orc):
     return cells / t_b, t_b, res, cfg
 
 
+def cpu_sample(args, sigs, prn_mult=1):
+    """(PRNs, bins) per signal of the CPU sample: the first prn_mult x
+    cpu_prns PRNs over all bins (or cpu_bins), the bins cut in proportion
+    where the sample would exceed prn_mult x cpu_budget IF samples processed
+    (M x bins x blocks x PRNs; 1e9 is about 10 s of one core)."""
+    out = []
+    for s in sigs:
+        nq = min(prn_mult * args.cpu_prns, s["code_fft"].shape[0])
+        nb = min(args.cpu_bins, len(s["fds"])) if args.cpu_bins else len(s["fds"])
+        out.append([nq, nb])
+    work = sum(nq * nb * s["n_blk"] * s["M"] for (nq, nb), s in zip(out, sigs))
+    if work > prn_mult * args.cpu_budget:
+        f = prn_mult * args.cpu_budget / work
+        for o in out:
+            o[1] = max(1, int(o[1] * f))
+    return out
+
+
 def cpu_leg(args, orc, plan, buffs, sigs):
     """CPU baseline: the oracle's sdr_search_code (reference PocketFFT, one
     thread) timed on a bounded sample, the first cpu_prns PRNs of every signal
-    over all bins and blocks.  Untimed, on the same sample: the FP64 oracle's
-    power at each PRN's peak cell against the timed GPU form's surface."""
+    over all bins and blocks (fewer bins for the large workloads, cpu_sample).
+    Untimed, on the same sample: the FP64 oracle's power at each PRN's peak
+    cell against the timed GPU form's surface."""
     t0 = time.perf_counter()
     n_cells = 0
     parts, surf = [], []
-    for i, s in enumerate(sigs):
-        nq = min(args.cpu_prns, s["code_fft"].shape[0])
-        nb = min(args.cpu_bins, len(s["fds"])) if args.cpu_bins else len(s["fds"])
+    for i, (s, (nq, nb)) in enumerate(zip(sigs, cpu_sample(args, sigs))):
         for q in range(nq):
             P = np.zeros((nb, s["M"]), np.float32)
             for k in range(s["n_blk"]):
@@ -310,11 +327,9 @@ def cpu_leg(args, orc, plan, buffs, sigs):
     from concurrent.futures import ThreadPoolExecutor
     nth = max(1, min(16, os.cpu_count() or 1))
     tasks, mt_parts = [], []
-    for i, s in enumerate(sigs):
-        nq = min(4 * args.cpu_prns, s["code_fft"].shape[0])
-        nb = min(args.cpu_bins, len(s["fds"])) if args.cpu_bins else len(s["fds"])
+    for i, (s, (nq, nb)) in enumerate(zip(sigs, cpu_sample(args, sigs, 4))):
         tasks += [(i, q, b0, min(nb, b0 + 8)) for q in range(nq) for b0 in range(0, nb, 8)]
-        mt_parts.append("%s %d PRNs" % (s["name"], nq))
+        mt_parts.append("%s %d PRNs x %d bins" % (s["name"], nq, nb))
 
     def task(tk):
         i, q, b0, b1 = tk
@@ -331,7 +346,7 @@ def cpu_leg(args, orc, plan, buffs, sigs):
     t_mt = time.perf_counter() - t0
     log("cpu %d threads: %.1f cells/s over %.1f s" % (nth, n_mt / t_mt, t_mt))
     base_mt = dict(value=round(n_mt / t_mt, 2), unit="cells/s", cores=nth, kind="port",
-                   sample="%s over all bins and blocks as (signal, PRN, 8-bin) tasks on %d host "
+                   sample="%s over all blocks as (signal, PRN, 8-bin) tasks on %d host "
                           "threads (%.1f s)" % (", ".join(mt_parts), nth, t_mt))
     worst = 0.0
     for i, q, P in surf:
@@ -589,6 +604,8 @@ def main():
     ap.add_argument("--l6-epochs", type=int, default=250)
     ap.add_argument("--cpu-prns", type=int, default=16)
     ap.add_argument("--workload", default="multi", choices=["multi", "l1ca", "l1cp"])
+    ap.add_argument("--cpu-budget", type=float, default=1e9,
+                    help="CPU sample bound in IF samples processed (about 10 s of one core at 1e9)")
     ap.add_argument("--cpu-bins", type=int, default=0,
                     help="CPU sample: only the first this many Doppler bins (0: all)")
     ap.add_argument("--form", default="fp16", choices=["fp16", "bf16", "fp32"],

@@ -210,8 +210,11 @@ struct ATab {
 // pre() runs after the first tile's loads are issued and before anything
 // uses them or the stage's table (e.g. filling LDS tables and the barrier);
 // the stage then completes the table loads.
-template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, class IBase, class ILoad,
-          class ICook, class OBase, class OMul, class OPut, class Pre>
+// ALL: every tile's inputs of the wave are read before pre() and the first
+// output store (stages whose inputs and outputs share one LDS buffer, pre()
+// then being the barrier between them).
+template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, bool ALL = false,
+          class IBase, class ILoad, class ICook, class OBase, class OMul, class OPut, class Pre>
 __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &ibase, const ILoad &iload,
                                        const ICook &icook, const OBase &obase, const OMul &omul,
                                        const OPut &oput, const Pre &pre)
@@ -260,34 +263,8 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
     };
     // output t' = tc + 2 h is produced if t' < TH
     auto outok = [&](int tc) -> bool { return tc + 2 < TH || (tc < TH && h == 0); };
-    // (a wave without tiles fetches the last one, unused, so that no load
-    // sits in a branch)
-    int tile = wave, g = 0, j = 0;
-    bool cv = column(tile < NT ? tile : NT - 1, g, j);
-    uint4 B[KS];
-    Raw raw[KS][4];
-    const int ib0 = ibase(g, j);
-    if (PF) fetch(ib0, raw);
-    pre();
-    at.complete();
-    if (PF) cook(ib0, raw, B);
-    for (; tile < NT; tile += NW) {
-        const int tn = tile + NW;
-        int gn = 0, jn = 0, ibn = 0;
-        bool cvn = false;
-        if (PF) {
-            // next tile (a wave-uniform branch: the loads are not waited on
-            // inside it)
-            if (tn < NT) {
-                cvn = column(tn, gn, jn);
-                ibn = ibase(gn, jn);
-                fetch(ibn, raw);
-            }
-        } else {
-            const int ib = ibase(g, j);
-            fetch(ib, raw);
-            cook(ib, raw, B);
-        }
+    // one tile: multipliers, MFMAs, outputs
+    auto tile_body = [&](int g, int j, bool cv, const uint4 (&B)[KS]) {
         const int4 ob = obase(g, j);
         // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r + 4 m +
         // 2 h + w, real (imaginary) part
@@ -332,6 +309,62 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
                 }
             }
         }
+    };
+    // all of the wave's tiles read before the first is computed
+    constexpr int MT = (NT + NW - 1) / NW;
+    if constexpr (ALL) {
+        static_assert(MT <= 4, "all tiles in flight: at most 4 per wave");
+        Raw rw[MT][KS][4];
+        int ibs[MT], gs[MT], js[MT];
+        bool cvs[MT];
+#pragma unroll
+        for (int i = 0; i < MT; i++) {
+            const int ti = wave + i * NW;
+            cvs[i] = column(ti < NT ? ti : NT - 1, gs[i], js[i]);
+            ibs[i] = ibase(gs[i], js[i]);
+            fetch(ibs[i], rw[i]);
+        }
+        pre();
+        at.complete();
+#pragma unroll
+        for (int i = 0; i < MT; i++) {
+            if (wave + i * NW < NT) {
+                uint4 Bt[KS];
+                cook(ibs[i], rw[i], Bt);
+                tile_body(gs[i], js[i], cvs[i], Bt);
+            }
+        }
+        return;
+    }
+    // (a wave without tiles fetches the last one, unused, so that no load
+    // sits in a branch)
+    int tile = wave, g = 0, j = 0;
+    bool cv = column(tile < NT ? tile : NT - 1, g, j);
+    uint4 B[KS];
+    Raw raw[KS][4];
+    const int ib0 = ibase(g, j);
+    if (PF) fetch(ib0, raw);
+    pre();
+    at.complete();
+    if (PF) cook(ib0, raw, B);
+    for (; tile < NT; tile += NW) {
+        const int tn = tile + NW;
+        int gn = 0, jn = 0, ibn = 0;
+        bool cvn = false;
+        if (PF) {
+            // next tile (a wave-uniform branch: the loads are not waited on
+            // inside it)
+            if (tn < NT) {
+                cvn = column(tn, gn, jn);
+                ibn = ibase(gn, jn);
+                fetch(ibn, raw);
+            }
+        } else {
+            const int ib = ibase(g, j);
+            fetch(ib, raw);
+            cook(ib, raw, B);
+        }
+        tile_body(g, j, cv, B);
         if (PF) {
             if (tn < NT) cook(ibn, raw, B);
             g = gn; j = jn; cv = cvn;
@@ -402,7 +435,65 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     const int dummy = G * RP + (int)(threadIdx.x & 63);
 
     // ---- stage 0: global -> MFMA -> LDS ----
-    {
+    // output t of column j sits at q = j R0 + t (LDS j (R0 + PD) + t); the
+    // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages)
+    auto obase = [&](int g, int j) -> int4 { return make_int4(g * RP + j * (R0 + PD), 0, (j / R2) * R2, 1); };
+    auto omul = [&](const int4 &ob, int tc, int th, bool ok) -> TwT { return tws[(ok ? tc + th : 0) * ob.z]; };
+    auto oput = [&](const int4 &ob, int tc, int th, float2 v, TwT f, bool ok) {
+        uint32_t w = pack_c<DT>(v.x, v.y);
+        if constexpr (PKT) w = cmul_w<DT>(w, f);
+        buf0[ok ? ob.x + tc + th : dummy] = w;
+    };
+    // Code-product row passes with few tiles per wave stage their inputs:
+    // the workgroup loads its rows of X and Cf with 16-byte loads (instead of
+    // one 4-byte load per element and operand, in runs of L / R0 words),
+    // multiplies them (the same cmul_w as the per-element loader, so the
+    // results do not change) and stores the products unpadded in buf0;
+    // stage 0 then reads all its inputs from LDS before the barrier that
+    // precedes its first output store, so buf0 holds both.
+    constexpr int NT0 = (G * (L / R0) + 31) / 32, MT0 = (NT0 + NTH / 64 - 1) / (NTH / 64);
+    constexpr bool STG = LM == LD_PROD && NF && MT0 <= 4 && L % 4 == 0;
+    if constexpr (STG) {
+        const long long xo = (long long)((b / a.prod_xdiv) % a.prod_xmod) * a.in_bstride;
+        const long long co = (long long)(b / a.prod_cdiv) * a.cf_bstride;
+        const __amdgpu_buffer_rsrc_t xin = rsrc(a.hin + xo), cin = rsrc(a.hcf + co);
+        const int igs = (int)a.in_gstride;
+        constexpr int Q = G * L / 4;              // 16-byte words of the rows
+        constexpr int NQ = (Q + NTH - 1) / NTH;
+        uint4 xv[NQ], cv[NQ];
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int qq = min((int)threadIdx.x + u * NTH, Q - 1);
+            const int g = qq / (L / 4), n = (qq - g * (L / 4)) * 4;
+            const int e = (g0 + (g < gcount ? g : 0)) * igs + n;
+            xv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xin, e * 4, 0, 0));
+            cv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(cin, e * 4, 0, 0));
+        }
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int q0 = (int)threadIdx.x + u * NTH;
+            if (q0 < Q) {
+                const int g = q0 / (L / 4), n = (q0 - g * (L / 4)) * 4;
+                const uint32_t xw[4] = {xv[u].x, xv[u].y, xv[u].z, xv[u].w};
+                const uint32_t cw[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int nn = n + v;
+                    buf0[g * RP + nn] = cmul_w<DT>(xw[v], cw[v]);
+                }
+            }
+        }
+        fill();
+        const int tstride = L / R0;
+        // inputs unpadded: element n of row g at g RP + n
+        auto ibase = [&](int g, int j) -> int { return g * RP + j; };
+        auto iload = [&](int base, int tc, int th, bool ok) -> uint32_t {
+            return buf0[base + (ok ? tc + th : R0 - 1) * tstride];
+        };
+        auto icook = [](int, int, int, bool, uint32_t w) -> uint32_t { return w; };
+        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, true>(at0, ibase, iload, icook, obase, omul, oput,
+                                                                    barrier);
+    } else {
         // row passes (NF) read and write rows with unit element stride (a
         // compile-time constant: per-element offsets become immediates)
         const int igs = (int)a.in_gstride, ins = NF ? 1 : (int)a.in_nstride;
@@ -438,15 +529,6 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             } else {
                 return cmul_w<DT>(raw.x, raw.y);
             }
-        };
-        // output t of column j sits at q = j R0 + t (LDS j (R0 + PD) + t); the
-        // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages)
-        auto obase = [&](int g, int j) -> int4 { return make_int4(g * RP + j * (R0 + PD), 0, (j / R2) * R2, 1); };
-        auto omul = [&](const int4 &ob, int tc, int th, bool ok) -> TwT { return tws[(ok ? tc + th : 0) * ob.z]; };
-        auto oput = [&](const int4 &ob, int tc, int th, float2 v, TwT f, bool ok) {
-            uint32_t w = pack_c<DT>(v.x, v.y);
-            if constexpr (PKT) w = cmul_w<DT>(w, f);
-            buf0[ok ? ob.x + tc + th : dummy] = w;
         };
         mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true>(at0, ibase, iload, icook, obase, omul, oput, fill);
     }
