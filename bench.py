@@ -35,11 +35,7 @@ METRIC = "acq search cells/s (batched+MFMA vs plain FP32, 1 GPU) + correlator Ms
 HBM_PEAK_GBS = 8000.0
 FP16_PEAK_TFLOPS = 2500.0       # dense FP16 / BF16 MFMA
 FP32_PEAK_TFLOPS = 157.3        # FP32 vector
-# int8 dot products: no published VALU figure; v_dot4_i32_i8 (4 MACs) issues
-# 22.2 T lane-ops/s = 177.5 TOPS on this GPU (make -C pocketsdr_amd probe,
-# 8 independent chains per lane); dense int8 MFMA 5000 TOPS (published)
-DOT4_PEAK_TOPS = 177.5
-INT8_MFMA_PEAK_TOPS = 5000.0
+INT8_PEAK_TOPS = 5000.0         # dense int8 MFMA, the published int8 peak
 
 
 def log(*a):
@@ -181,7 +177,6 @@ def bench_acq(args, sdr, orc):
     buffs, sigs, truth, wname = build_workload(args.workload, args.seed)
     cells = sum(s["code_fft"].shape[0] * len(s["fds"]) for s in sigs)
     # counter summary of the same workload (profiles/search_<workload>_traffic.md)
-    traffic_profile = "search_%s_traffic.md" % args.workload
     res = {}
     form = args.form
     plan, t_b, r_main = run_form(sdr, buffs, sigs, form, args.graph, args.steps, args.warmup)
@@ -208,9 +203,31 @@ def bench_acq(args, sdr, orc):
                                                     grid=plan.kernel_grid("%s@%d" % (n, i)),
                                                     M=s["M"], cls=n, form=res["form_per_signal"][s["name"]],
                                                     tiles=tiles[s["name"]])
-    # dominant kernel: the instantiation (as rocprof names it) with the most
-    # device time; signal groups whose FFT lengths share a pass length share
-    # its launches, so time and algorithmic bytes are summed over them
+    # roofline of the step: the algorithmic HBM bytes of the whole step (the
+    # per-cell-step model, sum over every cell-step of every signal) over the
+    # step time against the HBM peak; per signal the same over the signal's
+    # own kernel time
+    model = cell_model(sigs, form, t_b)
+    cap = model["capacity"]
+    counted = step_traffic(args.workload, form)
+    res["roofline"] = dict(bound="hbm", achieved=cap["achieved_GBs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                           frac=cap["frac"], traffic=counted["bytes_per_step"] if counted else None,
+                           scope="whole step (all kernels of one search step)",
+                           alg_bytes_per_step=cap["bytes_per_step"],
+                           traffic_over_alg=round(counted["bytes_per_step"] / cap["bytes_per_step"], 3)
+                           if counted else None,
+                           traffic_source=counted["source"] if counted else None)
+    per_sig = {}
+    for i, sg in enumerate(sigs):
+        ms = sum(v["avg_ms"] * v["launches"] for k, v in ks.items() if k.endswith("@" + sg["name"])) / args.steps
+        b = cap["per_signal"][sg["name"]]
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        per_sig[sg["name"]] = dict(alg_bytes=round(b), kernel_ms=round(ms, 4), achieved_GBs=round(gbs, 1),
+                                   frac=round(gbs / HBM_PEAK_GBS, 4))
+    res["roofline_per_signal"] = per_sig
+    # the kernel instantiation (as rocprof names it) with the most device
+    # time, priced with its own operand bytes (X, code spectra, Z, P): a
+    # per-kernel figure, not the step's algorithmic model above
     inst_tot = {}
     for k, v in ks.items():
         inst = kernel_instance(v["cls"], v["M"], v["form"], v["tiles"])
@@ -219,18 +236,15 @@ def bench_acq(args, sdr, orc):
         t["launches"] += v["launches"]
         t["bytes"] += v["bytes"] * v["launches"]
         t["parts"].append(k)
-        g = v["grid"] or 0
-        t["grids"][g] = t["grids"].get(g, 0) + v["launches"]
     inst = max(inst_tot, key=lambda i: inst_tot[i]["ms"])
     d = inst_tot[inst]
     avg_ms = d["ms"] / d["launches"]
-    alg = d["bytes"] / d["launches"]
-    gbs = alg / (avg_ms * 1e-3) / 1e9
-    res["roofline"] = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic_of(inst, d["grids"], traffic_profile), kernel=inst,
-                           launches_of=d["parts"], alg_bytes_per_launch=round(alg),
-                           avg_launch_ms=round(avg_ms, 5))
-    res["cell_model"] = cell_model(sigs, form, t_b)
+    opb = d["bytes"] / d["launches"]
+    res["dominant_kernel"] = dict(kernel=inst, launches_of=d["parts"], avg_launch_ms=round(avg_ms, 5),
+                                  operand_bytes_per_launch=round(opb),
+                                  operand_GBs=round(opb / (avg_ms * 1e-3) / 1e9, 1),
+                                  share_of_step=round(d["ms"] / args.steps / (t_b * 1e3), 4))
+    res["cell_model"] = {k: {kk: vv for kk, vv in v.items() if kk != "per_signal"} for k, v in model.items()}
     # algorithmic FLOPs per cell-step F = 5 M log2 M (1 + 1 / n_prn) + 8 M (the
     # forward FFT shared by a signal's PRNs, one inverse per PRN, product and
     # power) over each signal's own kernel time, against the dense FP16 MFMA
@@ -368,29 +382,20 @@ def cpu_leg(args, orc, plan, buffs, sigs):
                                note="peak cell of each sampled PRN, timed form vs FP64 oracle")
 
 
-def traffic_of(inst, grids, profile="search_multi_traffic.md"):
-    """HBM bytes per launch of a pass instantiation from the committed
-    FETCH_SIZE / WRITE_SIZE summary of the same workload: the measured
-    traffic of each launch size (grid), averaged with the weights of the timed
-    launches, `grids` = {grid: launches}.  None if a size is missing."""
-    path = os.path.join(ROOT, "profiles", profile)
+def step_traffic(workload, form):
+    """Counted HBM bytes of one search step from the committed FETCH_SIZE /
+    WRITE_SIZE summary of the same workload and form
+    (profiles/search_<workload>_<form>_step_traffic.md, written from
+    `bench.py --profile-step` runs): the line `| step total | <bytes> |`.
+    None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "search_%s_%s_step_traffic.md" % (workload, form))
     if not os.path.exists(path):
         return None
-    per = {}
     for line in open(path):
         cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 6 and cells[1].strip("`") == inst:
-            try:
-                per[int(cells[2])] = (float(cells[4]) + float(cells[5])) * 1048576.0
-            except ValueError:
-                pass
-    tot = n = 0
-    for g, c in grids.items():
-        if int(round(g)) not in per:
-            return None
-        tot += per[int(round(g))] * c
-        n += c
-    return round(tot / n) if n else None
+        if len(cells) > 3 and cells[1] == "step total":
+            return dict(bytes_per_step=int(float(cells[2])), source="profiles/" + os.path.basename(path))
+    return None
 
 
 def bench_raw(args, sdr, orc):
@@ -443,7 +448,12 @@ def bench_raw(args, sdr, orc):
     return res
 
 
-def bench_trk(args, sdr, w):
+def bench_trk(args, sdr, orc, w):
+    """Tracking correlators (64 channels x 1000 epochs at 24 Msps, 44 taps):
+    Msps*ch, the int8 roofline of the correlator kernel against the published
+    dense int8 peak, and the CPU baseline: the oracle's open-loop tracking of
+    src/sdr_ch.c:694-740 (sdr_mix_carr + sdr_corr_std per epoch, AVX2-exact
+    integer dot products) on the first channels, one thread."""
     plan = sdr.TrkPlan(w["buff"], w["chans"], w["pos"], w["n_epoch"])
     t = time_plan(plan, max(1, args.steps // 2), 1)
     plan.timing(True)
@@ -458,11 +468,61 @@ def bench_trk(args, sdr, w):
     if ks:
         # 2 n N int8 MACs per channel-epoch (I and Q), 2 ops each
         tops = 2.0 * 2 * len(w["pos"]) * N * n_ch * w["n_epoch"] / (ks[0] * 1e-3) / 1e12
-        res["correlator_roofline"] = dict(bound="valu-int8-dot4", achieved=round(tops, 2),
-                                          peak=round(DOT4_PEAK_TOPS, 1), unit="TOPS",
-                                          frac=round(tops / DOT4_PEAK_TOPS, 4),
-                                          frac_of_int8_mfma=round(tops / INT8_MFMA_PEAK_TOPS, 5))
+        res["correlator_roofline"] = dict(bound="int8", achieved=round(tops, 2), peak=INT8_PEAK_TOPS,
+                                          unit="TOPS", frac=round(tops / INT8_PEAK_TOPS, 5),
+                                          note="v_dot4 int8 products on the vector ALUs; the peak is "
+                                               "the published dense int8 MFMA rate")
+    if orc is not None and not args.skip_cpu:
+        nc = max(1, min(args.cpu_trk_channels, n_ch))
+        t0 = time.perf_counter()
+        for ch in w["chans"][:nc]:
+            orc.trk_run(ch, ch["code"], w["buff"], w["n_epoch"], w["pos"])
+        dt = time.perf_counter() - t0
+        res["correlator_cpu_baseline"] = dict(
+            value=round(nc * N * w["n_epoch"] / dt / 1e6, 2), unit="Msps*ch", cores=1, kind="port",
+            sample="%d channels x %d epochs (%.1f s): oracle C restatement of track_sig's open loop "
+                   "(sdr_mix_carr + sdr_corr_std, %d taps), one thread" % (nc, w["n_epoch"], dt, len(w["pos"])))
     return res
+
+
+def bench_dropin(args, sdr, orc, w):
+    """Per-call time of the drop-in entry points a receiver links
+    (host buffers in and out, every call on the calling thread's stream):
+    sdr_mix_carr + sdr_corr_std at N = 24000 with 44 taps (one channel-epoch
+    of track_sig), sdr_search_code for L1 C/A at 12 Msps (21 bins, one
+    block), and the oracle's per-call time of the same calls (one thread)."""
+    ch = w["chans"][0]
+    N = ch["N"]
+    reps = max(20, args.steps * 4)
+    out = {}
+
+    def per_call(fn, n):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        return (time.perf_counter() - t0) / n * 1e6
+
+    iq = orc.mix_carr(w["buff"], 0, N, ch["fs"], ch["fi"] + ch["fd0"], ch["phi0"])
+    out["corr_std"] = dict(
+        N=N, taps=len(w["pos"]),
+        gpu_us=round(per_call(lambda: sdr.corr_std(iq, ch["code"], N, ch["coff0"], w["pos"]), reps), 1),
+        mix_carr_gpu_us=round(per_call(lambda: sdr.mix_carr(w["buff"], 0, N, ch["fs"], ch["fd0"], 0.3), reps), 1),
+        cpu_us=round(per_call(lambda: orc.corr_std(iq, ch["code"], N, ch["coff0"], w["pos"]), reps), 1),
+        mix_carr_cpu_us=round(per_call(lambda: orc.mix_carr(w["buff"], 0, N, ch["fs"], ch["fd0"], 0.3), reps), 1))
+    from pocketsdr_amd import workloads
+    a = workloads.acq_l1ca(seed=args.seed, n_prn=1, n_vis=1, t_int=1e-3)
+    sg = a["sig"]
+    cf = sg["code_fft"][0]
+    out["search_code"] = dict(
+        M=sg["M"], bins=len(sg["fds"]),
+        gpu_us=round(per_call(lambda: sdr.search_code(cf, sg["T"], a["buff"], 0, sg["fs"], sg["fi"],
+                                                       sg["fds"]), reps), 1),
+        cpu_us=round(per_call(lambda: orc.search_code(cf, sg["T"], a["buff"], 0, sg["fs"], sg["fi"],
+                                                       sg["fds"]), max(3, reps // 10)), 1))
+    out["note"] = ("host pointers in and out (PCIe copies in every GPU call); CPU: the oracle's C "
+                   "restatement, one thread")
+    return out
 
 
 L6_CLASSES = ("state", "fwd_col", "row", "fwd_row", "inv_row", "inv_col", "csk")
@@ -554,15 +614,17 @@ def cell_model(sigs, form, t_step):
     out = {}
     for name, rule in (("capacity", None), ("four_step", 2)):
         tot = 0.0
+        per = {}
         for s in sigs:
             M, N = s["M"], s["M"] // 2
             nprn, nbin, nblk = s["code_fft"].shape[0], len(s["fds"]), s["n_blk"]
             p = rule or (1 if M * bw <= ONCHIP_BYTES else 2)
             B = M / (nprn * nbin) + bw * M / (nbin * nblk) + 4.0 * N / nblk + (p - 1) * 4.0 * M * bw
-            tot += B * nprn * nbin * nblk
+            per[s["name"]] = B * nprn * nbin * nblk
+            tot += per[s["name"]]
         gbs = tot / t_step / 1e9
         out[name] = dict(bytes_per_step=round(tot), achieved_GBs=round(gbs, 1),
-                         frac=round(gbs / HBM_PEAK_GBS, 4))
+                         frac=round(gbs / HBM_PEAK_GBS, 4), per_signal=per)
     return out
 
 
@@ -594,6 +656,21 @@ def copy_probe(mb):
     print(json.dumps(dict(copy_probe_bytes=n, launches=3)), flush=True)
 
 
+def profile_step(args):
+    """Only the timed form's plan of the workload and `steps` search steps
+    (no graph, nothing else): the run the counter summaries of one step are
+    taken from, dividing the step kernels' counts by `steps`."""
+    import pocketsdr_amd  # noqa: F401
+    from pocketsdr_amd import sdr_func as sdr
+    sdr.func_init()
+    buffs, sigs, _, wname = build_workload(args.workload, args.seed)
+    plan = sdr.SearchPlan(buffs, sigs, FORM_FLAGS[args.form])
+    for _ in range(args.steps):
+        plan.run()
+    plan.close()
+    print(json.dumps(dict(profile_steps=args.steps, workload=wname, form=args.form)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -618,6 +695,10 @@ def main():
                     help="arithmetic form of the timed batched search")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--trk-epochs", type=int, default=1000)
+    ap.add_argument("--cpu-trk-channels", type=int, default=4,
+                    help="CPU baseline of the tracking correlators: this many channels over all epochs")
+    ap.add_argument("--profile-step", action="store_true",
+                    help="only build the timed plan and run --steps search steps (counter profiles)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a torch device copy of this many MiB (counter calibration)")
     ap.add_argument("--tiles", default="auto",
@@ -629,6 +710,9 @@ def main():
         os.environ["PSA_TILES"] = TILES.get(args.workload, "") if args.tiles == "auto" else tiles_arg(args.tiles)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.profile_step:
+        profile_step(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
@@ -643,7 +727,9 @@ def main():
     value, t_step, res, cfg = bench_acq(args, sdr, orc)
     if not args.skip_trk:
         w_trk = workloads.trk_l1ca(seed=args.seed + 1, n_epoch=args.trk_epochs)
-        res.update(bench_trk(args, sdr, w_trk))
+        res.update(bench_trk(args, sdr, orc, w_trk))
+        if orc is not None:
+            res["dropin_per_call"] = bench_dropin(args, sdr, orc, w_trk)
         res["closed_loop_tracking"] = bench_trk_loop(args, sdr, w_trk)
     if not args.skip_raw:
         r = bench_raw(args, sdr, orc)

@@ -157,7 +157,7 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
 /* Runs the whole search on the device (P surfaces are zeroed first), then
  * the peak search.  Blocks until done.  Returns 1 on success. */
 /* The same on IF buffers in device memory (buffN bytes each, e.g. written
- * by sdr_raw_unpack_dev / sdr_arch_combine_dev): read in place at every
+ * by sdr_raw_unpack_dev / sdr_arch_combine_raw_dev): read in place at every
  * run, so new samples written there between runs are searched by the next
  * run; the caller keeps them allocated while the plan lives. */
 sdr_search_plan_t *sdr_search_plan_new_dev(int n_buff, const uint8_t *const *dbuf,
@@ -338,15 +338,19 @@ int sdr_raw_unpack_dev(sdr_raw_plan_t *plan, const uint8_t *raw, long long N,
 double sdr_raw_plan_kernel_ms(const sdr_raw_plan_t *plan);
 void sdr_raw_plan_free(sdr_raw_plan_t *plan);
 
-/* Array channels: sdr_arch_combine (src/sdr_array.c:553-579) for narch beams
- * at once.  in[j]: cpx8 samples of RF channel j (nrfch 2-8); w: Q8 weights
+/* Array channels: the sample combine of sdr_arch_combine (src/sdr_array.c:
+ * 553-579) for narch beams at once, on raw sample arrays instead of the
+ * receiver structs.  Named apart from the reference's own sdr_arch_combine
+ * (src/pocket_sdr.h:522, another signature), so that both headers can be
+ * included in one translation unit and both libraries linked in one process.
+ * in[j]: cpx8 samples of RF channel j (nrfch 2-8); w: Q8 weights
  * [beam][channel][re, im] as sdr_arch_t.w (src/pocket_sdr.h:297) holds them;
  * out[m]: cpx8 samples of beam m (narch 1-8).  Host pointers (returns 1) or
  * device pointers, 16-byte aligned (returns the kernel time in ms, < 0 on
  * error). */
-int sdr_arch_combine(int nrfch, const uint8_t *const *in, int N, int narch,
+int sdr_arch_combine_raw(int nrfch, const uint8_t *const *in, int N, int narch,
     const int16_t *w, uint8_t *const *out);
-double sdr_arch_combine_dev(int nrfch, const uint8_t *const *in, long long N,
+double sdr_arch_combine_raw_dev(int nrfch, const uint8_t *const *in, long long N,
     int narch, const int16_t *w, uint8_t *const *out);
 
 /* library version string */

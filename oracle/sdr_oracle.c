@@ -147,7 +147,10 @@ static void dot_IQ_cpx_code(const int8_t *IQ, const int8_t *code, int N,
  * k = (int)((p - j) * 10), j wrapped once into [0, N);
  * corr1 = dot(IQ[0:j], bank_k[N-j:N]), corr2 = dot(IQ[j:N], bank_k[0:N-j]).
  * Polarity: the reference build contracts the EPL sum to
- * d += fmaf(c1.re, c2.re, c1.im * c2.im).
+ * d += fmaf(c1.re, c2.re, c1.im * c2.im).  The polarity sum always runs over
+ * taps 0-2 (src/sdr_func.c:994-997): with n < 3 the reference reads
+ * uninitialised corr1 / corr2 entries, so its result is undefined there; the
+ * restatement (and the GPU) take the missing taps as zero.
  */
 typedef void (*dot_fn)(const int8_t *, const int8_t *, int, float, float *);
 
@@ -155,7 +158,7 @@ static void corr_std_gen(dot_fn dot, const int8_t *IQ, const int8_t *code,
                          int N, double coff, const double *pos, int n,
                          float *corr, float *C)
 {
-    float corr1[ORC_MAX_CORR][2], corr2[ORC_MAX_CORR][2];
+    float corr1[ORC_MAX_CORR][2] = {{0.0f}}, corr2[ORC_MAX_CORR][2] = {{0.0f}};
     float dot_EPL = 0.0f, sign;
 
     for (int i = 0; i < n; i++) {

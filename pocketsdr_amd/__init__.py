@@ -20,7 +20,18 @@ def _load():
         raise ImportError(
             "libpocketsdr_amd.so is not built (run __graft_entry__.build() or "
             "make -C pocketsdr_amd)")
-    return ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    # The library's own symbols stay local: its drop-in names equal the
+    # reference libsdr.so's, which another component of the process may load.
+    h = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+    # Its HIP runtime, though, is made global, so that a framework loaded
+    # later with a bundled HIP runtime of its own (PyTorch's wheels carry one)
+    # binds to this one instead of starting a second runtime in the process,
+    # which then finds no device.
+    try:
+        ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL | os.RTLD_NOLOAD)
+    except OSError:
+        pass
+    return h
 
 
 lib = _load()

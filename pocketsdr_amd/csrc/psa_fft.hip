@@ -487,14 +487,20 @@ static void radix_seq(int L, SubPlan &sp)
     while (n % 3 == 0) { sp.rad[sp.nrad++] = 3; n /= 3; }
 }
 
-float2 *make_twiddle(int L, int sign, double scale)
+// scale * exp(sign 2 pi i m / L), m < L, rounded from double
+static void fill_twiddle(int L, int sign, double scale, float2 *h)
 {
     const double PI_D = 3.14159265358979323846;
-    std::vector<float2> h(L > 0 ? L : 1);
     for (int m = 0; m < L; m++) {
         double ang = sign * 2.0 * PI_D * (double)m / L;
         h[m] = make_float2((float)(scale * cos(ang)), (float)(scale * sin(ang)));
     }
+}
+
+float2 *make_twiddle(int L, int sign, double scale)
+{
+    std::vector<float2> h(L > 0 ? L : 1);
+    fill_twiddle(L, sign, scale, h.data());
     float2 *d;
     PSA_CHECK(hipMalloc(&d, sizeof(float2) * h.size()));
     PSA_CHECK(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
@@ -502,12 +508,12 @@ float2 *make_twiddle(int L, int sign, double scale)
 }
 
 // w_M^{a} for a = hi * 1024 + lo: T_hi[hi] = exp(s 2 pi i hi 1024 / M),
-// T_lo[lo] = exp(s 2 pi i lo / M), both rounded from double
-static void make_split_twiddle(int M, int sign, float2 **thi, float2 **tlo)
+// T_lo[lo] = exp(s 2 pi i lo / M), both rounded from double; hi has
+// (M + 1023) / 1024 entries, lo 1024
+static void fill_split_twiddle(int M, int sign, float2 *hi, float2 *lo)
 {
     const double PI_D = 3.14159265358979323846;
     int nhi = (M + 1023) / 1024;
-    std::vector<float2> hi(nhi), lo(1024);
     for (int i = 0; i < nhi; i++) {
         double ang = sign * 2.0 * PI_D * (double)((long long)i * 1024 % M) / M;
         hi[i] = make_float2((float)cos(ang), (float)sin(ang));
@@ -516,6 +522,13 @@ static void make_split_twiddle(int M, int sign, float2 **thi, float2 **tlo)
         double ang = sign * 2.0 * PI_D * (double)i / M;
         lo[i] = make_float2((float)cos(ang), (float)sin(ang));
     }
+}
+
+static void make_split_twiddle(int M, int sign, float2 **thi, float2 **tlo)
+{
+    int nhi = (M + 1023) / 1024;
+    std::vector<float2> hi(nhi), lo(1024);
+    fill_split_twiddle(M, sign, hi.data(), lo.data());
     PSA_CHECK(hipMalloc(thi, sizeof(float2) * nhi));
     PSA_CHECK(hipMalloc(tlo, sizeof(float2) * 1024));
     PSA_CHECK(hipMemcpy(*thi, hi.data(), sizeof(float2) * nhi, hipMemcpyHostToDevice));
@@ -571,6 +584,34 @@ const FFTPlan *fft_plan(int M, int M1)
     make_split_twiddle(M, +1, &p->thi_bwd, &p->tlo_bwd);
     g_plans[{M, M1}] = p;
     return p;
+}
+
+size_t fft_plan_tables(int M, FFTPlan *p, float2 *h, size_t cap, float2 *dev)
+{
+    if (!fft_supported(M)) return 0;
+    int M1, M2;
+    split_len(M, &M1, &M2);
+    if (M1 <= 0 || M1 > 4096 || M2 > 4096) return 0;
+    const int nhi = (M + 1023) / 1024;
+    const size_t n = 2 * (size_t)M1 + 2 * (size_t)M2 + 2 * ((size_t)nhi + 1024);
+    if (n > cap) return 0;
+    *p = FFTPlan();
+    p->M = M; p->M1 = M1; p->M2 = M2;
+    radix_seq(M1, p->s1);
+    radix_seq(M2, p->s2);
+    size_t o = 0;
+    auto take = [&](int len) { float2 *hp = h + o; float2 *dp = dev + o; o += len; return std::make_pair(hp, dp); };
+    auto t = take(M1); fill_twiddle(M1, -1, 1.0, t.first); p->s1.tw_fwd = t.second;
+    t = take(M1); fill_twiddle(M1, +1, 1.0, t.first); p->s1.tw_bwd = t.second;
+    t = take(M2); fill_twiddle(M2, -1, 1.0, t.first); p->s2.tw_fwd = t.second;
+    t = take(M2); fill_twiddle(M2, +1, 1.0, t.first); p->s2.tw_bwd = t.second;
+    auto hi = take(nhi), lo = take(1024);
+    fill_split_twiddle(M, -1, hi.first, lo.first);
+    p->thi_fwd = hi.second; p->tlo_fwd = lo.second;
+    hi = take(nhi); lo = take(1024);
+    fill_split_twiddle(M, +1, hi.first, lo.first);
+    p->thi_bwd = hi.second; p->tlo_bwd = lo.second;
+    return o;
 }
 
 // (L, G) pairs with compile-time kernels: the sub-lengths of the plans for
@@ -685,7 +726,7 @@ bool launch_xcorr_row(const PassArgs &a_in, int nbatch, hipStream_t st)
 void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hipStream_t st)
 {
     PassArgs a = a_in;
-    a.carr = dev_carrier();
+    if (!a.carr) a.carr = dev_carrier();
     RadixSeq rs;
     const RadPlan rp = make_rad_plan(a.L);
     rs.n = rp.n;

@@ -34,10 +34,12 @@ void sdr_cpx_free(sdr_cpx_t *cpx)
     free(cpx);
 }
 
-// src/sdr_func.c:182-185
+// src/sdr_func.c:182-185.  The reference build (-O3 -mavx2 -mfma, implicit
+// contraction) fuses SQR(re) + SQR(im) into fmaf(re, re, im * im); the fused
+// form is written out because this library compiles without contraction.
 float sdr_cpx_abs(sdr_cpx_t cpx)
 {
-    return sqrtf(cpx[0] * cpx[0] + cpx[1] * cpx[1]);
+    return sqrtf(fmaf(cpx[0], cpx[0], cpx[1] * cpx[1]));
 }
 
 // src/sdr_func.c:276-297 (sdr_malloc = calloc, src/sdr_cmn.c:39-48)

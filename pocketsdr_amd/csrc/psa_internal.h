@@ -47,6 +47,7 @@ namespace psa {
 // carrier table: round(11.2 cos/sin(-2 pi i / 256)) packed as
 // (uint8 cI) | (uint8 cQ) << 8, src/sdr_func.c:122-124
 void ensure_init();
+bool init_done();                  // ensure_init has completed
 const uint32_t *host_carrier();
 const uint32_t *dev_carrier();     // the same 256 words in device memory
 
@@ -91,6 +92,12 @@ struct FFTPlan {
     float2 *thi_bwd = nullptr, *tlo_bwd = nullptr;
 };
 bool fft_supported(int M);
+// Geometry and tables of the default split of M without allocating: the
+// tables (the values fft_plan(M) uploads) are written to host staging h
+// (capacity cap float2) and p's table pointers point at the same offsets of
+// the device array dev, to which the caller copies h.  Returns the float2
+// count, 0 if M is unsupported or cap is too small.
+size_t fft_plan_tables(int M, FFTPlan *p, float2 *h, size_t cap, float2 *dev);
 // device table of scale * exp(sign 2 pi i m / L), m < L, rounded from double
 float2 *make_twiddle(int L, int sign, double scale);
 // cached per (M, M1), thread-safe; M1 = 0 takes the default split (rows a
@@ -299,13 +306,6 @@ inline void args_xcorr_row(const FFTPlan *pl, PassArgs &a)
     }
 }
 
-inline const FFTPlan *need_plan(int M, const char *who)
-{
-    const FFTPlan *pl = fft_plan(M);
-    if (!pl) PSA_FAIL("%s: FFT length %d has prime factors other than 2, 3, 5", who, M);
-    return pl;
-}
-
 // ---------------------------------------------------------------------------
 // misc kernels (psa_mix.hip)
 // ---------------------------------------------------------------------------
@@ -316,7 +316,27 @@ void launch_mix(const uint8_t *buf, int buffN, int ix, int N, uint32_t p0,
                 int compact = 0);
 void mix_params(int buffN, int ix, int N, double fs, double fc, double phi,
                 uint32_t *p0, uint32_t *p1, int *nseg, uint32_t *s);
+// c[j] = a[j] * b * s for nbatch arrays a[j], c[j] of N (stride N), one b
 void launch_cpx_mul(const float2 *a, const float2 *b, int N, float s,
-                    float2 *c, hipStream_t st);
+                    float2 *c, hipStream_t st, int nbatch = 1);
+
+// ---------------------------------------------------------------------------
+// any-length DFT (psa_anylen.hip): Bluestein's chirp-z transform on a
+// 2-3-5-smooth FFT of length L >= 2 N - 1
+// ---------------------------------------------------------------------------
+struct AnyPlan {
+    int N = 0, L = 0;
+    const FFTPlan *pl = nullptr;    // length L
+    float2 *chirp[2] = {};          // [dir] exp(s i pi m^2 / N), m < N, s = -1 / +1
+    float2 *bf[2] = {};             // [dir] FFT of the conjugate chirp, / L
+};
+// cached per N, thread-safe; nullptr when no smooth L <= 4096^2 exists
+const AnyPlan *any_plan(int N);
+// nbatch contiguous length-N transforms (natural order); tmp holds
+// 2 nbatch L complex; out may equal in
+void fft_any(const AnyPlan *ap, const float2 *in, float2 *out, float2 *tmp, int nbatch, int dir,
+             hipStream_t st);
+// cpx16 (int8 pairs) -> float complex * CSCALE, n samples
+void launch_cpx16_to_float(const int8_t *iq, long long n, float2 *out, hipStream_t st);
 
 }  // namespace psa

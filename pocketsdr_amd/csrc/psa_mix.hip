@@ -95,7 +95,9 @@ __global__ void __launch_bounds__(256) cpx_mul_kernel(const float2 *__restrict__
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    float2 x = a[i], y = b[i];
+    // batch blockIdx.y: a and c advance by N, b is shared
+    const long long o = (long long)blockIdx.y * N;
+    float2 x = a[o + i], y = b[i];
     float re, im;
     if (i < V) {
         re = __fadd_rn(__fmul_rn(x.x, y.x), __fmul_rn(x.y, -y.y));
@@ -104,15 +106,16 @@ __global__ void __launch_bounds__(256) cpx_mul_kernel(const float2 *__restrict__
         re = __fmaf_rn(x.x, y.x, -__fmul_rn(x.y, y.y));
         im = __fmaf_rn(x.x, y.y, __fmul_rn(x.y, y.x));
     }
-    c[i] = make_float2(__fmul_rn(re, s), __fmul_rn(im, s));
+    c[o + i] = make_float2(__fmul_rn(re, s), __fmul_rn(im, s));
 }
 
 void launch_cpx_mul(const float2 *a, const float2 *b, int N, float s,
-                    float2 *c, hipStream_t st)
+                    float2 *c, hipStream_t st, int nbatch)
 {
-    if (N <= 0) return;
+    if (N <= 0 || nbatch <= 0) return;
+    if (nbatch > 65535) PSA_FAIL("cpx_mul: batch of %d exceeds the grid", nbatch);
     int V = N >= 4 ? N / 4 * 4 : 0;
-    hipLaunchKernelGGL(cpx_mul_kernel, dim3((N + 255) / 256), dim3(256), 0, st, a, b, N, V, s, c);
+    hipLaunchKernelGGL(cpx_mul_kernel, dim3((N + 255) / 256, nbatch), dim3(256), 0, st, a, b, N, V, s, c);
     PSA_CHECK(hipGetLastError());
 }
 

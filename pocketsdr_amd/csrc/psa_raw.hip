@@ -444,7 +444,7 @@ static float arch_launch(int nrfch, const uint8_t *const *in, long long N, int n
 static bool arch_args_ok(int nrfch, int narch, const void *in, const int16_t *w, const void *out)
 {
     if (nrfch < 2 || nrfch > 8 || narch < 1 || narch > 8 || !in || !w || !out) {
-        fprintf(stderr, "pocketsdr_amd: sdr_arch_combine: 2-8 RF channels and 1-8 beams\n");
+        fprintf(stderr, "pocketsdr_amd: sdr_arch_combine_raw: 2-8 RF channels and 1-8 beams\n");
         return false;
     }
     return true;
@@ -454,19 +454,19 @@ static bool arch_args_ok(int nrfch, int narch, const void *in, const int16_t *w,
 
 extern "C" {
 
-double sdr_arch_combine_dev(int nrfch, const uint8_t *const *in, long long N, int narch, const int16_t *w,
+double sdr_arch_combine_raw_dev(int nrfch, const uint8_t *const *in, long long N, int narch, const int16_t *w,
                             uint8_t *const *out)
 {
     if (!arch_args_ok(nrfch, narch, in, w, out) || N < 0) return -1.0;
     for (int j = 0; j < nrfch; j++) {
         if (!in[j] || ((uintptr_t)in[j] & 15)) {
-            fprintf(stderr, "pocketsdr_amd: sdr_arch_combine_dev: input %d must be 16-byte aligned\n", j);
+            fprintf(stderr, "pocketsdr_amd: sdr_arch_combine_raw_dev: input %d must be 16-byte aligned\n", j);
             return -1.0;
         }
     }
     for (int m = 0; m < narch; m++) {
         if (!out[m] || ((uintptr_t)out[m] & 15)) {
-            fprintf(stderr, "pocketsdr_amd: sdr_arch_combine_dev: output %d must be 16-byte aligned\n", m);
+            fprintf(stderr, "pocketsdr_amd: sdr_arch_combine_raw_dev: output %d must be 16-byte aligned\n", m);
             return -1.0;
         }
     }
@@ -475,7 +475,7 @@ double sdr_arch_combine_dev(int nrfch, const uint8_t *const *in, long long N, in
     return arch_launch(nrfch, in, N, narch, w, out, tctx().stream);
 }
 
-int sdr_arch_combine(int nrfch, const uint8_t *const *in, int N, int narch, const int16_t *w, uint8_t *const *out)
+int sdr_arch_combine_raw(int nrfch, const uint8_t *const *in, int N, int narch, const int16_t *w, uint8_t *const *out)
 {
     if (!arch_args_ok(nrfch, narch, in, w, out) || N < 0) return 0;
     if (N == 0) return 1;

@@ -1,5 +1,6 @@
 // psa_runtime.hip - device initialisation, carrier table, per-thread streams.
 #include <math.h>
+#include <atomic>
 #include "psa_internal.h"
 
 namespace psa {
@@ -7,6 +8,7 @@ namespace psa {
 static uint32_t h_carr[256];
 static uint32_t *d_carr = nullptr;
 static std::once_flag g_init_once;
+static std::atomic<bool> g_ready{false};
 
 // Carrier table of the mixing LUT, src/sdr_func.c:121-124:
 // carr = (int8)floor(cos/sin(-2 pi i / 256) / (1/11.2f) + 0.5).  The LUT of
@@ -32,7 +34,13 @@ void ensure_init()
         build_carrier();
         PSA_CHECK(hipMalloc(&d_carr, sizeof(h_carr)));
         PSA_CHECK(hipMemcpy(d_carr, h_carr, sizeof(h_carr), hipMemcpyHostToDevice));
+        g_ready.store(true);
     });
+}
+
+bool init_done()
+{
+    return g_ready.load();
 }
 
 const uint32_t *host_carrier()

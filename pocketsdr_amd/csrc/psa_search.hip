@@ -201,6 +201,93 @@ static const uint8_t *gather_ring(const sdr_buff_t *buff, int ix, int N, uint8_t
     return dst;
 }
 
+// The drop-ins' dataflow of src/sdr_func.c:1088-1102 for FFT lengths with a
+// prime factor above 5: IQ * CSCALE, any-length forward DFT, sdr_cpx_mul by
+// the code spectrum with 1 / N^2, any-length backward DFT.  Exits, as the
+// reference's allocation failures do, when no transform length exists.
+static const AnyPlan *need_any(int N, const char *who)
+{
+    const AnyPlan *ap = any_plan(N);
+    if (!ap) PSA_FAIL("%s: no FFT for length %d", who, N);
+    return ap;
+}
+
+static void corr_fft_any(const sdr_cpx16_t *IQ, const sdr_cpx_t *code_fft, int N, sdr_cpx_t *corr)
+{
+    const AnyPlan *ap = need_any(N, "sdr_corr_fft");
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    const size_t bytes = sizeof(float2) * (size_t)N;
+    int8_t *diq = (int8_t *)tc.s[SC_IQ].get((size_t)N * 2);
+    float2 *x = (float2 *)tc.s[SC_A].get(bytes);
+    float2 *cf = (float2 *)tc.s[SC_C].get(bytes);
+    float2 *w = (float2 *)tc.s[SC_B].get(2 * sizeof(float2) * (size_t)ap->L);
+    uint8_t *hp = (uint8_t *)tc.host_pinned(bytes);
+    memcpy(hp, IQ, (size_t)N * 2);
+    PSA_CHECK(hipMemcpyAsync(diq, hp, (size_t)N * 2, hipMemcpyHostToDevice, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(hp, code_fft, bytes);
+    PSA_CHECK(hipMemcpyAsync(cf, hp, bytes, hipMemcpyHostToDevice, st));
+    launch_cpx16_to_float(diq, N, x, st);
+    fft_any(ap, x, x, w, 1, 0, st);
+    launch_cpx_mul(x, cf, N, 1.0f / N / N, x, st);
+    fft_any(ap, x, x, w, 1, 1, st);
+    PSA_CHECK(hipMemcpyAsync(hp, x, bytes, hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(corr, hp, bytes);
+}
+
+// sdr_search_code (src/sdr_func.c:664-687) for lengths with a prime factor
+// above 5: per Doppler bin mix_carr at phase 0, the correlator above, P +=
+// |C|^2; the bins of a chunk are batched
+static void search_code_any(const sdr_cpx_t *code_fft, const sdr_buff_t *buff, int ix, int N, double fs,
+                            double fi, const float *fds, int len_fds, float *P)
+{
+    const AnyPlan *ap = need_any(N, "sdr_search_code");
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    // bins per pass: work buffers of at most about 1 GiB
+    const int chunk = (int)std::max<long long>(1, std::min<long long>(len_fds,
+                                                   (1LL << 30) / (16LL * ap->L + 10LL * N)));
+    const size_t bytes = sizeof(float2) * (size_t)N;
+    uint8_t *hp = (uint8_t *)tc.host_pinned(std::max(bytes, sizeof(float) * (size_t)N * chunk) + (size_t)N);
+    uint8_t *din = (uint8_t *)tc.s[SC_IN].get((size_t)N);
+    float2 *cf = (float2 *)tc.s[SC_C].get(bytes);
+    int8_t *diq = (int8_t *)tc.s[SC_IQ].get((size_t)N * 2 * chunk);
+    float2 *x = (float2 *)tc.s[SC_A].get(bytes * chunk);
+    float2 *w = (float2 *)tc.s[SC_B].get(2 * sizeof(float2) * (size_t)ap->L * chunk);
+    float *dP = (float *)tc.s[SC_P].get(sizeof(float) * (size_t)N * chunk);
+    gather_ring(buff, ix, N, hp);
+    PSA_CHECK(hipMemcpyAsync(din, hp, N, hipMemcpyHostToDevice, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(hp, code_fft, bytes);
+    PSA_CHECK(hipMemcpyAsync(cf, hp, bytes, hipMemcpyHostToDevice, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    for (int b0 = 0; b0 < len_fds; b0 += chunk) {
+        const int nb = std::min(chunk, len_fds - b0);
+        const size_t pb = sizeof(float) * (size_t)N * nb;
+        memcpy(hp, P + (size_t)b0 * N, pb);
+        PSA_CHECK(hipMemcpyAsync(dP, hp, pb, hipMemcpyHostToDevice, st));
+        for (int b = 0; b < nb; b++) {
+            uint32_t p0, p1, s;
+            int nseg;
+            mix_params(buff->N, ix, N, fs, fi + fds[b0 + b], 0.0, &p0, &p1, &nseg, &s);
+            launch_mix(din, N, 0, N, p0, p1, nseg, s, diq + (size_t)b * N * 2, st, 1);
+        }
+        launch_cpx16_to_float(diq, (long long)N * nb, x, st);
+        fft_any(ap, x, x, w, nb, 0, st);
+        launch_cpx_mul(x, cf, N, 1.0f / N / N, x, st, nb);
+        fft_any(ap, x, x, w, nb, 1, st);
+        const long long total = (long long)nb * N;
+        hipLaunchKernelGGL(power_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                           x, N, N, (long long)nb, dP, (long long)N);
+        PSA_CHECK(hipGetLastError());
+        PSA_CHECK(hipMemcpyAsync(hp, dP, pb, hipMemcpyDeviceToHost, st));
+        PSA_CHECK(hipStreamSynchronize(st));
+        memcpy(P + (size_t)b0 * N, hp, pb);
+    }
+}
+
 }  // namespace psa
 
 using namespace psa;
@@ -213,8 +300,10 @@ extern "C" {
 int sdr_cpx_fft(sdr_cpx_t *cpx1, int N, int dir, sdr_cpx_t *cpx2)
 {
     if (dir != SDR_FFT_FORWARD && dir != SDR_FFT_BACKWARD) return 0;
+    if (N <= 0) return 0;
     const FFTPlan *pl = fft_plan(N);
-    if (!pl) {
+    const AnyPlan *ap = pl ? nullptr : any_plan(N);
+    if (!pl && !ap) {
         fprintf(stderr, "pocketsdr_amd: sdr_cpx_fft: unsupported FFT length N=%d\n", N);
         return 0;
     }
@@ -222,12 +311,17 @@ int sdr_cpx_fft(sdr_cpx_t *cpx1, int N, int dir, sdr_cpx_t *cpx2)
     hipStream_t st = tc.stream;
     size_t bytes = sizeof(float2) * (size_t)N;
     float2 *a = (float2 *)tc.s[SC_A].get(bytes);
-    float2 *b = (float2 *)tc.s[SC_B].get(bytes);
     float2 *c = (float2 *)tc.s[SC_C].get(bytes);
     void *hp = tc.host_pinned(bytes);
     memcpy(hp, cpx1, bytes);
     PSA_CHECK(hipMemcpyAsync(a, hp, bytes, hipMemcpyHostToDevice, st));
-    fft_c2c(pl, a, c, b, 1, dir, st);
+    if (pl) {
+        float2 *b = (float2 *)tc.s[SC_B].get(bytes);
+        fft_c2c(pl, a, c, b, 1, dir, st);
+    } else {
+        float2 *b = (float2 *)tc.s[SC_B].get(2 * sizeof(float2) * (size_t)ap->L);
+        fft_any(ap, a, c, b, 1, dir, st);
+    }
     PSA_CHECK(hipMemcpyAsync(hp, c, bytes, hipMemcpyDeviceToHost, st));
     PSA_CHECK(hipStreamSynchronize(st));
     memcpy(cpx2, hp, bytes);
@@ -278,7 +372,11 @@ void sdr_mix_carr(const sdr_buff_t *buff, int ix, int N, double fs, double fc,
 void sdr_corr_fft(const sdr_cpx16_t *IQ, const sdr_cpx_t *code_fft, int N, sdr_cpx_t *corr)
 {
     if (N <= 0) return;
-    const FFTPlan *pl = need_plan(N, "sdr_corr_fft");
+    const FFTPlan *pl = fft_plan(N);
+    if (!pl) {
+        corr_fft_any(IQ, code_fft, N, corr);
+        return;
+    }
     ThreadCtx &tc = tctx();
     hipStream_t st = tc.stream;
     const float2 *cft = spec_cache().get(code_fft, pl, st);
@@ -337,7 +435,11 @@ void sdr_search_code(const sdr_cpx_t *code_fft, double T, const sdr_buff_t *buff
 {
     (void)T;
     if (N <= 0 || len_fds <= 0) return;
-    const FFTPlan *pl = need_plan(N, "sdr_search_code");
+    const FFTPlan *pl = fft_plan(N);
+    if (!pl) {
+        search_code_any(code_fft, buff, ix, N, fs, fi, fds, len_fds, P);
+        return;
+    }
     ThreadCtx &tc = tctx();
     hipStream_t st = tc.stream;
     const float2 *cft = spec_cache().get(code_fft, pl, st);
@@ -1182,7 +1284,8 @@ void sdr_search_plan_free(sdr_search_plan_t *p)
         for (auto d : p->d_buf) (void)hipFree(d);
     }
     for (auto &s : p->sig) {
-        (void)hipFree(s.cft); (void)hipFree(s.cfn); (void)hipFree(s.P); (void)hipFree(s.fds);
+        (void)hipFree(s.cft); (void)hipFree(s.cfn); (void)hipFree(s.cfh); (void)hipFree(s.cf_nat);
+        (void)hipFree(s.P); (void)hipFree(s.fds);
         (void)hipFree(s.mix_ix); (void)hipFree(s.mix_nseg); (void)hipFree(s.mix_p0);
         (void)hipFree(s.mix_p1); (void)hipFree(s.mix_s);
         (void)hipFree(s.cn0); (void)hipFree(s.ix); (void)hipFree(s.dop); (void)hipFree(s.part);

@@ -124,9 +124,9 @@ _decl("sdr_raw_unpack", C.c_int, [c_void_p, c_void_p, C.c_int, C.c_int64, C.POIN
 _decl("sdr_raw_unpack_dev", C.c_int, [c_void_p, c_void_p, C.c_longlong, C.c_int64, C.POINTER(c_void_p)])
 _decl("sdr_raw_plan_kernel_ms", C.c_double, [c_void_p])
 _decl("sdr_raw_plan_free", None, [c_void_p])
-_decl("sdr_arch_combine", C.c_int, [C.c_int, C.POINTER(c_void_p), C.c_int, C.c_int, c_void_p,
+_decl("sdr_arch_combine_raw", C.c_int, [C.c_int, C.POINTER(c_void_p), C.c_int, C.c_int, c_void_p,
                                      C.POINTER(c_void_p)])
-_decl("sdr_arch_combine_dev", C.c_double, [C.c_int, C.POINTER(c_void_p), C.c_longlong, C.c_int,
+_decl("sdr_arch_combine_raw_dev", C.c_double, [C.c_int, C.POINTER(c_void_p), C.c_longlong, C.c_int,
                                             c_void_p, C.POINTER(c_void_p)])
 _decl("sdr_amd_version", C.c_char_p, [])
 
@@ -579,8 +579,8 @@ def arch_combine(ins, w):
     out = np.zeros((narch, N), np.uint8)
     ip_ = (c_void_p * 8)(*[ins[j].ctypes.data for j in range(nrfch)])
     op = (c_void_p * 8)(*[out[m].ctypes.data for m in range(narch)])
-    if not lib.sdr_arch_combine(nrfch, ip_, N, narch, _ptr(w), op):
-        raise RuntimeError("sdr_arch_combine failed")
+    if not lib.sdr_arch_combine_raw(nrfch, ip_, N, narch, _ptr(w), op):
+        raise RuntimeError("sdr_arch_combine_raw failed")
     return out
 
 
@@ -589,8 +589,8 @@ def arch_combine_dev(in_ptrs, N, w, out_ptrs):
     w = np.ascontiguousarray(w, np.int16)
     ip_ = (c_void_p * 8)(*list(in_ptrs))
     op = (c_void_p * 8)(*list(out_ptrs))
-    ms = lib.sdr_arch_combine_dev(len(in_ptrs), ip_, N, len(out_ptrs), _ptr(w), op)
+    ms = lib.sdr_arch_combine_raw_dev(len(in_ptrs), ip_, N, len(out_ptrs), _ptr(w), op)
     if ms < 0:
-        raise RuntimeError("sdr_arch_combine_dev failed")
+        raise RuntimeError("sdr_arch_combine_raw_dev failed")
     return ms
 

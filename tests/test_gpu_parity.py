@@ -64,7 +64,9 @@ def test_cpx_fft(sdr, orc, N):
 
 def test_cpx_fft_bad_args(sdr):
     with pytest.raises(ValueError):
-        sdr.cpx_fft(np.ones(7 * 11, np.complex64), 0)   # prime factors outside {2,3,5}
+        sdr.cpx_fft(np.ones(0, np.complex64), 0)        # empty input
+    with pytest.raises(ValueError):
+        sdr.cpx_fft(np.ones(8, np.complex64), 2)        # bad direction
 
 
 # ---------------------------------------------------------------- correlators

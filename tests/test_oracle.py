@@ -141,7 +141,9 @@ def brute_corr_std(IQ, code, N, coff, pos, cpx=False):
         c1s.append(res[0])
         c2s.append(res[1])
     dot = np.float32(0)
-    for i in range(3):
+    # taps 0-2 decide the polarity; fewer taps contribute nothing (the
+    # reference reads uninitialised entries there, src/sdr_func.c:994-997)
+    for i in range(min(3, len(pos))):
         t = np.float32(np.float64(c1s[i][0]) * np.float64(c2s[i][0]) +
                        np.float64(np.float32(c1s[i][1] * c2s[i][1])))
         dot = np.float32(dot + t)
@@ -167,6 +169,25 @@ def test_corr_std_bruteforce(orc, N, cpx):
         ref, rC = brute_corr_std(IQ, code, N, coff, pos, cpx)
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
         assert np.array_equal(gC.view(np.uint32), rC.view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 2])
+def test_corr_std_fewer_than_three_taps(orc, n):
+    """With n < 3 taps the polarity sum takes the missing taps as zero."""
+    N = 1001
+    rng = np.random.default_rng(n)
+    IQ = rng.integers(-112, 113, (N, 2)).astype(np.int8)
+    code = rng.integers(-1, 2, (10 * N, 2)).astype(np.int8)
+    code[:, 1] = code[:, 0]
+    pos = np.array([0.0, -3.3])[:n]
+    flips = 0
+    for coff in rng.uniform(0, N, 40):
+        got, gC = orc.corr_std(IQ, code, N, coff, pos)
+        ref, rC = brute_corr_std(IQ, code, N, coff, pos)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+        assert np.array_equal(gC.view(np.uint32), rC.view(np.uint32))
+        flips += int(abs(got[0] - (gC[0] + gC[1]) / np.float32(N)) > 1e-9)
+    assert 0 < flips < 40        # both polarities occur
 
 
 def test_corr_std_self_correlation(orc):
