@@ -353,6 +353,21 @@ int sdr_arch_combine_raw(int nrfch, const uint8_t *const *in, int N, int narch,
 double sdr_arch_combine_raw_dev(int nrfch, const uint8_t *const *in, long long N,
     int narch, const int16_t *w, uint8_t *const *out);
 
+/* sdr_search_code without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as sdr_search_code (bit-identical P); the work
+ * buffers live in a fixed 64 MiB device array of the library (one per GPU,
+ * held until the process ends), so the FFT length N must leave room for at
+ * least one Doppler bin there (N up to about 1.8 million; bins are processed
+ * in chunks).  Requires sdr_func_init first.  Returns 1 on success, 0 (with a
+ * message on stderr, P untouched) for a length the array cannot hold, a
+ * length with a prime factor above 5, or invalid arguments.  Every call on a
+ * GPU shares its array: calls that overlap in time on one GPU race. */
+int sdr_search_code_noalloc(const sdr_cpx_t *code_fft, double T,
+    const sdr_buff_t *buff, int ix, int N, double fs, double fi,
+    const float *fds, int len_fds, float *P);
+
 /* library version string */
 const char *sdr_amd_version(void);
 

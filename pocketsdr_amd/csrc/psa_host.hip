@@ -15,6 +15,7 @@ void sdr_func_init(const char *file)
 {
     (void)file;
     ensure_init();
+    noalloc_setup();
 }
 
 // src/sdr_func.c:147-171 (fftwf_malloc: 32-byte aligned; exits on failure)

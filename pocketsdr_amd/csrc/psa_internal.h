@@ -48,6 +48,9 @@ namespace psa {
 // (uint8 cI) | (uint8 cQ) << 8, src/sdr_func.c:122-124
 void ensure_init();
 bool init_done();                  // ensure_init has completed
+// no-allocation entry point (psa_noalloc.hip): looks up the current
+// device's scratch array once per device
+void noalloc_setup();
 const uint32_t *host_carrier();
 const uint32_t *dev_carrier();     // the same 256 words in device memory
 

@@ -62,6 +62,8 @@ _decl("sdr_cpx_mul", None, [c_void_p, c_void_p, C.c_int, C.c_float, c_void_p])
 _decl("sdr_cpx_fft", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p])
 _decl("sdr_search_code", None, [c_void_p, C.c_double, C.POINTER(sdr_buff_t), C.c_int, C.c_int,
                                 C.c_double, C.c_double, c_float_p, C.c_int, c_float_p])
+_decl("sdr_search_code_noalloc", C.c_int, [c_void_p, C.c_double, C.POINTER(sdr_buff_t), C.c_int, C.c_int,
+                                        C.c_double, C.c_double, c_float_p, C.c_int, c_float_p])
 _decl("sdr_corr_max", C.c_float, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_double, c_int_p])
 _decl("sdr_fine_dop", C.c_double, [c_float_p, C.c_int, c_float_p, C.c_int, c_int_p])
 _decl("sdr_dop_bins", c_float_p, [C.c_double, C.c_float, C.c_float, c_int_p])
@@ -231,6 +233,20 @@ def search_code(code_fft, T, buff, ix, fs, fi, fds, P=None):
     b = _buff(buff)
     lib.sdr_search_code(_ptr(code_fft), T, C.byref(b), ix, N, fs, fi, _fp(fds), len(fds), _fp(P))
     return P
+
+
+def search_code_noalloc(code_fft, T, buff, ix, fs, fi, fds, P=None):
+    """sdr_search_code_noalloc: sdr_search_code on the library's fixed device
+    scratch (no device allocation).  Returns (ok, P); P is untouched when the
+    call is rejected (ok == 0)."""
+    code_fft = np.ascontiguousarray(code_fft, dtype=np.complex64)
+    fds = np.ascontiguousarray(fds, dtype=np.float32)
+    N = len(code_fft)
+    if P is None:
+        P = np.zeros((len(fds), N), dtype=np.float32)
+    b = _buff(buff)
+    ok = lib.sdr_search_code_noalloc(_ptr(code_fft), T, C.byref(b), ix, N, fs, fi, _fp(fds), len(fds), _fp(P))
+    return ok, P
 
 
 def corr_max(P, N, Nmax, M, T):
