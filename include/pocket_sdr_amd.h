@@ -59,9 +59,11 @@ float sdr_cpx_abs(sdr_cpx_t cpx);
 void sdr_cpx_mul(const sdr_cpx_t *a, const sdr_cpx_t *b, int N, float s,
     sdr_cpx_t *c);
 
-/* src/pocket_sdr.h:398, def. src/sdr_func.c:236-264.  Unnormalised FFT,
- * dir SDR_FFT_FORWARD / SDR_FFT_BACKWARD; returns 0 for a bad dir or a
- * length whose prime factors are not all in {2, 3, 5}. */
+/* src/pocket_sdr.h:398, def. src/sdr_func.c:236-264.  Unnormalised FFT of
+ * any length N >= 1, dir SDR_FFT_FORWARD / SDR_FFT_BACKWARD; returns 0 for a
+ * bad dir or N.  Lengths with a prime factor above 5 run Bluestein's chirp-z
+ * transform on a 2-3-5-smooth length (as do sdr_corr_fft and
+ * sdr_search_code). */
 int sdr_cpx_fft(sdr_cpx_t *cpx1, int N, int dir, sdr_cpx_t *cpx2);
 
 /* src/pocket_sdr.h:399-400, def. src/sdr_func.c:276-297 */
@@ -198,6 +200,8 @@ int sdr_search_plan_kernel_bytes(const sdr_search_plan_t *plan, const char *name
 int sdr_search_plan_kernel_grid(const sdr_search_plan_t *plan, const char *name,
     double *avg_threads);
 void sdr_search_plan_free(sdr_search_plan_t *plan);
+/* Device memory (bytes) held by all live search plans of the process. */
+long long sdr_search_plan_bytes(void);
 
 /* --- tracking correlators -------------------------------------------------
  * n_ch channels on one IF stream, n_epoch consecutive epochs of N samples.

@@ -231,9 +231,17 @@ static void fft_f64_impl(const double *in, int N, int sign, double *out)
     while (rem > 1) {
         int r = 2;
         while (rem % r) r++;
-        double a[2 * 64], b[2 * 64];
+        double a[2 * 64], b[2 * 64], wt[2 * 64];
         double *ar = r <= 64 ? a : (double *)malloc(sizeof(double) * 2 * r);
         double *br = r <= 64 ? b : (double *)malloc(sizeof(double) * 2 * r);
+        /* w_r^t, t < r: the r-point DFT's coefficients (index q m mod r
+         * taken in 64 bits: q m overflows an int for r > 46340) */
+        double *wr = r <= 64 ? wt : (double *)malloc(sizeof(double) * 2 * r);
+        for (int t = 0; t < r; t++) {
+            double ang = sign * 2.0 * PI_D * (double)t / r;
+            wr[2 * t] = cos(ang);
+            wr[2 * t + 1] = sin(ang);
+        }
         for (int j = 0; j < N / r; j++) {
             int k = j % Ns;
             for (int q = 0; q < r; q++) {
@@ -245,11 +253,13 @@ static void fft_f64_impl(const double *in, int N, int sign, double *out)
             }
             for (int m = 0; m < r; m++) {
                 double sr = 0.0, si = 0.0;
+                long long t = 0;            /* q m mod r, stepped by m */
                 for (int q = 0; q < r; q++) {
-                    double ang = sign * 2.0 * PI_D * (double)((q * m) % r) / r;
-                    double wr = cos(ang), wi = sin(ang);
-                    sr += ar[2 * q] * wr - ar[2 * q + 1] * wi;
-                    si += ar[2 * q] * wi + ar[2 * q + 1] * wr;
+                    double cr = wr[2 * t], ci = wr[2 * t + 1];
+                    sr += ar[2 * q] * cr - ar[2 * q + 1] * ci;
+                    si += ar[2 * q] * ci + ar[2 * q + 1] * cr;
+                    t += m;
+                    if (t >= r) t -= r;
                 }
                 br[2 * m] = sr; br[2 * m + 1] = si;
             }
@@ -259,7 +269,7 @@ static void fft_f64_impl(const double *in, int N, int sign, double *out)
                 y[2 * (o + q * Ns) + 1] = br[2 * q + 1];
             }
         }
-        if (r > 64) { free(ar); free(br); }
+        if (r > 64) { free(ar); free(br); free(wr); }
         double *t = x; x = y; y = t;
         Ns *= r;
         rem /= r;

@@ -18,6 +18,7 @@
 #include <string.h>
 #include <string>
 #include <algorithm>
+#include <unordered_map>
 #include "psa_internal.h"
 
 namespace psa {
@@ -538,6 +539,7 @@ struct SigDev {
     int nchunk_fused = 0;       // partials per PRN emitted by the last column pass
     bool fused_peak = false;    // the last column pass emits the peak partials
     bool xcd_order = true;      // XCD-aware workgroup order of the inverse row pass
+    int lane = 0;               // the lane (stream and work buffers) the group runs on
     int bin_chunk = 1, prn_chunk = 1;
     float tuned_ms = 0.0f;      // tuner: best first-block time (0: not tuned)
 };
@@ -548,16 +550,31 @@ static const char *k_names[K_NCLS] = {"fwd_col", "fwd_row", "inv_row", "inv_col"
 
 }  // namespace psa
 
+// Signal groups run on up to PSA_LANES lanes at once: a lane is a stream
+// with its own work buffers (sized for the largest of its groups); the groups
+// of one lane run in order.  The plan's stream forks to the lanes and joins
+// them through events, so a captured run is one graph with parallel branches.
+#define PSA_LANES 4
+
+struct SearchLane {
+    hipStream_t st = nullptr;
+    float2 *tmp = nullptr, *X = nullptr, *Z = nullptr;
+    int8_t *iq16 = nullptr;     // plain path mixed samples
+    size_t tmp_elems = 0, x_elems = 0, z_elems = 0;
+    hipEvent_t done = nullptr;  // end of the lane's groups in a run
+};
+
 struct sdr_search_plan_tag {
     int flags = 0;
     std::vector<uint8_t *> d_buf;
     bool own_buf = true;        // false: IF buffers in the caller's device memory
     std::vector<int> buffN;
     std::vector<psa::SigDev> sig;
-    float2 *tmp = nullptr, *X = nullptr, *Z = nullptr;
-    int8_t *iq16 = nullptr;     // plain path mixed samples
-    size_t tmp_elems = 0, x_elems = 0, z_elems = 0;
+    std::vector<SearchLane> lanes;
+    hipEvent_t fork = nullptr;  // start of a run on the plan's stream
+    std::vector<hipEvent_t> gdone;   // end of each group (timed runs chain them)
     hipStream_t st = nullptr;
+    hipStream_t cur_st = nullptr;    // stream of the group being enqueued
     hipGraphExec_t gexec = nullptr;
     bool timing = false;
     struct Ev { int cls, sig; hipEvent_t a, b; double elems, bytes, grid; };
@@ -570,6 +587,35 @@ struct sdr_search_plan_tag {
 };
 
 namespace psa {
+
+// Device memory of the search plans goes through pmalloc / pfree, which
+// account it, so that sdr_search_plan_bytes reports what live plans hold.
+static std::mutex g_pm_mtx;
+static std::unordered_map<void *, size_t> g_pm;
+static long long g_pm_live = 0;
+
+template <class T>
+static void pmalloc(T **p, size_t bytes)
+{
+    PSA_CHECK(hipMalloc((void **)p, bytes));
+    std::lock_guard<std::mutex> lk(g_pm_mtx);
+    g_pm[(void *)*p] = bytes;
+    g_pm_live += (long long)bytes;
+}
+
+static void pfree(void *p)
+{
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pm_mtx);
+        auto it = g_pm.find(p);
+        if (it != g_pm.end()) {
+            g_pm_live -= (long long)it->second;
+            g_pm.erase(it);
+        }
+    }
+    (void)hipFree(p);
+}
 
 // records a timed launch when timing is on; `bytes` is the launch's
 // algorithmic HBM traffic (compulsory reads + writes of its operands)
@@ -593,20 +639,21 @@ struct Tick {
         p->evs[slot].elems = elems;
         p->evs[slot].bytes = bytes;
         pass_grid = 0;
-        PSA_CHECK(hipEventRecord(p->evs[slot].a, p->st));
+        PSA_CHECK(hipEventRecord(p->evs[slot].a, p->cur_st));
     }
     ~Tick()
     {
         if (!p->timing) return;
         p->evs[slot].grid = (double)pass_grid;
-        PSA_CHECK(hipEventRecord(p->evs[slot].b, p->st));
+        PSA_CHECK(hipEventRecord(p->evs[slot].b, p->cur_st));
     }
 };
 
 // the batched, shared-forward-FFT schedule of one signal group
 static void run_batched(sdr_search_plan_t *p, SigDev &s)
 {
-    hipStream_t st = p->st;
+    SearchLane &ln = p->lanes[s.lane];
+    hipStream_t st = ln.st;
     const FFTPlan *pl = s.pl;
     const int M = pl->M, N = s.h.N, nbin = s.h.len_fds, nprn = s.h.n_prn;
     const uint8_t *buf = p->d_buf[s.h.buff];
@@ -621,14 +668,14 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
             a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
             a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
             a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
-            a.out = p->tmp; a.out_bstride = M;
+            a.out = ln.tmp; a.out_bstride = M;
             {
                 Tick t(p, K_FWD_COL, (double)nbc * M, (double)M + 8.0 * nbc * M);
                 launch_pass(LD_MIX, ST_CPX, a, nbc, st);
             }
             PassArgs r = {};
             args_fwd_row(pl, r);
-            r.in = p->tmp; r.out = p->X; r.in_bstride = r.out_bstride = M;
+            r.in = ln.tmp; r.out = ln.X; r.in_bstride = r.out_bstride = M;
             {
                 Tick t(p, K_FWD_ROW, (double)nbc * M, 16.0 * nbc * M);
                 launch_pass(LD_CPX, ST_CPX, r, nbc, st);
@@ -638,11 +685,11 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 const int nbatch = nqc * nbc;      // b = prn_local * nbc + bin_local
                 PassArgs ir = {};
                 args_inv_row(pl, ir);
-                ir.in = p->X; ir.in_bstride = M;
+                ir.in = ln.X; ir.in_bstride = M;
                 ir.cf = s.cft + (long long)q0 * M; ir.cf_bstride = M;
                 ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
                 set_prod(pl, ir, 1.0f / M / M);
-                ir.out = p->Z; ir.out_bstride = M;
+                ir.out = ln.Z; ir.out_bstride = M;
                 if (s.xcd_order && spec_length(pl->M2)) {
                     ir.xcd_nbin = nbc;
                     ir.xcd_nprn = nqc;
@@ -653,7 +700,7 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
                 }
                 PassArgs ic = {};
                 args_inv_col(pl, ic);
-                ic.in = p->Z; ic.in_bstride = M;
+                ic.in = ln.Z; ic.in_bstride = M;
                 ic.P = s.P; ic.p_bstride = N;
                 ic.p_div = nbc; ic.p_off1 = q0; ic.p_mul = nbin; ic.p_off2 = b0;
                 ic.nlag = N;
@@ -679,15 +726,16 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
 // reference's 1/11.2 sample scale and 1/M^2 product scale.
 static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
 {
-    hipStream_t st = p->st;
+    SearchLane &ln = p->lanes[s.lane];
+    hipStream_t st = ln.st;
     const HPlan *hp = s.hp;
     const int dt = hp->dt;
     const int M = hp->M, N = s.h.N, nbin = s.h.len_fds, nprn = s.h.n_prn;
     const uint8_t *buf = p->d_buf[s.h.buff];
     const int buffN = p->buffN[s.h.buff];
-    uint32_t *tmp = reinterpret_cast<uint32_t *>(p->tmp);
-    uint32_t *X = reinterpret_cast<uint32_t *>(p->X);
-    uint32_t *Z = reinterpret_cast<uint32_t *>(p->Z);
+    uint32_t *tmp = reinterpret_cast<uint32_t *>(ln.tmp);
+    uint32_t *X = reinterpret_cast<uint32_t *>(ln.X);
+    uint32_t *Z = reinterpret_cast<uint32_t *>(ln.Z);
     for (int k = 0; k < s.h.n_blk; k++) {
         for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
             const int nbc = std::min(s.bin_chunk, nbin - b0);
@@ -755,7 +803,8 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
 // each its own kernel, nothing shared across PRNs
 static void run_plain(sdr_search_plan_t *p, SigDev &s)
 {
-    hipStream_t st = p->st;
+    SearchLane &ln = p->lanes[s.lane];
+    hipStream_t st = ln.st;
     const FFTPlan *pl = s.pl;
     const int M = pl->M, N = s.h.N, nbin = s.h.len_fds, nprn = s.h.n_prn;
     const uint8_t *buf = p->d_buf[s.h.buff];
@@ -773,19 +822,19 @@ static void run_plain(sdr_search_plan_t *p, SigDev &s)
                     for (int b = 0; b < nbc; b++) {
                         size_t i = (size_t)k * nbin + b0 + b;
                         launch_mix(buf, buffN, s.hm.ix[i], M, s.hm.p0[i], s.hm.p1[i], s.hm.nseg[i],
-                                   s.hm.s[i], p->iq16 + (size_t)b * M * 2, st);
+                                   s.hm.s[i], ln.iq16 + (size_t)b * M * 2, st);
                     }
                 }
                 PassArgs a = {};
                 args_fwd_col(pl, a);
-                a.iq16 = p->iq16; a.in_bstride = M; a.out = p->tmp; a.out_bstride = M;
+                a.iq16 = ln.iq16; a.in_bstride = M; a.out = ln.tmp; a.out_bstride = M;
                 {
                     Tick t(p, K_FWD_COL, (double)nbc * M);
                     launch_pass(LD_CPX16, ST_CPX, a, nbc, st);
                 }
                 PassArgs r = {};
                 args_fwd_row(pl, r);
-                r.in = p->tmp; r.out = p->X; r.in_bstride = r.out_bstride = M;
+                r.in = ln.tmp; r.out = ln.X; r.in_bstride = r.out_bstride = M;
                 r.out_nstride = pl->M1; r.out_gstride = 1;    // natural order
                 {
                     Tick t(p, K_FWD_ROW, (double)nbc * M);
@@ -794,19 +843,19 @@ static void run_plain(sdr_search_plan_t *p, SigDev &s)
                 {
                     Tick t(p, K_MUL, (double)nbc * M);
                     for (int b = 0; b < nbc; b++) {
-                        launch_cpx_mul(p->X + (size_t)b * M, s.cfn + (size_t)q * M, M,
-                                       1.0f / M / M, p->Z + (size_t)b * M, st);
+                        launch_cpx_mul(ln.X + (size_t)b * M, s.cfn + (size_t)q * M, M,
+                                       1.0f / M / M, ln.Z + (size_t)b * M, st);
                     }
                 }
                 {
                     Tick t(p, K_INV_ROW, (double)nbc * M);
-                    fft_c2c(pl, p->Z, p->X, p->tmp, nbc, 1, st);
+                    fft_c2c(pl, ln.Z, ln.X, ln.tmp, nbc, 1, st);
                 }
                 {
                     Tick t(p, K_POW, (double)nbc * N);
                     long long total = (long long)nbc * N;
                     hipLaunchKernelGGL(power_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                                       p->X, M, N, (long long)nbc,
+                                       ln.X, M, N, (long long)nbc,
                                        s.P + ((long long)q * nbin + b0) * N, (long long)N);
                     PSA_CHECK(hipGetLastError());
                 }
@@ -834,9 +883,9 @@ static void h_setup(sdr_search_plan_t *p, SigDev &s, int M1, int vc, int vr)
     h_col_args(s.hp, 1, tmp, vc);
     const int nf = s.h.len_fds * pass_ngroups(tmp);
     if (nf != s.nchunk_fused || !s.part) {
-        if (s.part) PSA_CHECK(hipFree(s.part));
+        if (s.part) pfree(s.part);
         s.nchunk_fused = nf;
-        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused)));
+        pmalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * std::max(s.nchunk, s.nchunk_fused));
     }
 }
 
@@ -852,9 +901,9 @@ static void f_setup(sdr_search_plan_t *p, SigDev &s, int M1)
     col_pass_args(s.pl, 1, tmp);
     const int nf = s.h.len_fds * pass_ngroups(tmp);
     if (nf != s.nchunk_fused || !s.part) {
-        if (s.part) PSA_CHECK(hipFree(s.part));
+        if (s.part) pfree(s.part);
         s.nchunk_fused = nf;
-        PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)s.h.n_prn * std::max(s.nchunk, s.nchunk_fused)));
+        pmalloc(&s.part, sizeof(PeakPart) * (size_t)s.h.n_prn * std::max(s.nchunk, s.nchunk_fused));
     }
 }
 
@@ -916,10 +965,12 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
                 else f_setup(p, s, m1[i]);
                 float t = 1e30f;
                 for (int rep = 0; rep < 3; rep++) {
-                    PSA_CHECK(hipEventRecord(e0, p->st));
+                    hipStream_t lst = p->lanes[s.lane].st;
+                    p->cur_st = lst;
+                    PSA_CHECK(hipEventRecord(e0, lst));
                     if (h) run_batched_h(p, s);
                     else run_batched(p, s);
-                    PSA_CHECK(hipEventRecord(e1, p->st));
+                    PSA_CHECK(hipEventRecord(e1, lst));
                     PSA_CHECK(hipEventSynchronize(e1));
                     float ms = 0.0f;
                     PSA_CHECK(hipEventElapsedTime(&ms, e0, e1));
@@ -939,27 +990,45 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
     PSA_CHECK(hipEventDestroy(e1));
 }
 
+// One run: the plan's stream forks to the lanes, every group is enqueued on
+// its lane (its passes, then the peak search), and the lanes join back.
+// Timed runs chain the groups one after the other, so that the per-kernel
+// event times are those of kernels running alone.
 static void run_all(sdr_search_plan_t *p)
 {
+    PSA_CHECK(hipEventRecord(p->fork, p->st));
+    for (auto &ln : p->lanes) PSA_CHECK(hipStreamWaitEvent(ln.st, p->fork, 0));
     for (auto &s : p->sig) {
-        p->cur_sig = (int)(&s - p->sig.data());
+        const int gi = (int)(&s - p->sig.data());
+        p->cur_sig = gi;
+        hipStream_t st = p->lanes[s.lane].st;
+        p->cur_st = st;
+        if (p->timing && gi > 0) PSA_CHECK(hipStreamWaitEvent(st, p->gdone[gi - 1], 0));
         if (p->flags & SDR_SEARCH_PLAIN) run_plain(p, s);
         else if (s.hp) run_batched_h(p, s);
         else run_batched(p, s);
-        Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N,
-               4.0 * s.h.n_prn * s.h.len_fds * s.h.N);
-        const long long tot = (long long)s.h.len_fds * s.h.N;
-        const bool fused = s.fused_peak && !(p->flags & SDR_SEARCH_PLAIN);
-        if (!fused) {
-            hipLaunchKernelGGL(peak_part_kernel, dim3(s.h.n_prn * s.nchunk), dim3(PEAK_THREADS), 0, p->st,
-                               s.P, tot, s.nchunk, s.part);
+        {
+            Tick t(p, K_PEAK, (double)s.h.n_prn * s.h.len_fds * s.h.N,
+                   4.0 * s.h.n_prn * s.h.len_fds * s.h.N);
+            const long long tot = (long long)s.h.len_fds * s.h.N;
+            const bool fused = s.fused_peak && !(p->flags & SDR_SEARCH_PLAIN);
+            if (!fused) {
+                hipLaunchKernelGGL(peak_part_kernel, dim3(s.h.n_prn * s.nchunk), dim3(PEAK_THREADS), 0, st,
+                                   s.P, tot, s.nchunk, s.part);
+                PSA_CHECK(hipGetLastError());
+            }
+            const int nch = fused ? s.nchunk_fused : s.nchunk;
+            hipLaunchKernelGGL(peak_final_kernel, dim3(s.h.n_prn), dim3(PEAK_THREADS), 0, st, s.P, s.h.N,
+                               s.h.len_fds, s.h.T, nch, s.part, s.fds, s.cn0, s.ix, s.dop);
             PSA_CHECK(hipGetLastError());
         }
-        const int nch = fused ? s.nchunk_fused : s.nchunk;
-        hipLaunchKernelGGL(peak_final_kernel, dim3(s.h.n_prn), dim3(PEAK_THREADS), 0, p->st, s.P, s.h.N,
-                           s.h.len_fds, s.h.T, nch, s.part, s.fds, s.cn0, s.ix, s.dop);
-        PSA_CHECK(hipGetLastError());
+        PSA_CHECK(hipEventRecord(p->gdone[gi], st));
     }
+    for (auto &ln : p->lanes) {
+        PSA_CHECK(hipEventRecord(ln.done, ln.st));
+        PSA_CHECK(hipStreamWaitEvent(p->st, ln.done, 0));
+    }
+    p->cur_st = p->st;
 }
 
 }  // namespace psa
@@ -1003,7 +1072,7 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
     for (int i = 0; i < n_buff; i++) {
         uint8_t *d = const_cast<uint8_t *>(data[i]);
         if (!dev) {
-            PSA_CHECK(hipMalloc(&d, bN[i]));
+            pmalloc(&d, bN[i]);
             PSA_CHECK(hipMemcpy(d, data[i], bN[i], hipMemcpyHostToDevice));
         }
         p->d_buf.push_back(d);
@@ -1026,7 +1095,7 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         const size_t spec = sizeof(float2) * (size_t)M;
         // code spectra: natural (plain) and digit-transposed (batched)
         float2 *nat;
-        PSA_CHECK(hipMalloc(&nat, spec * nprn));
+        pmalloc(&nat, spec * nprn);
         PSA_CHECK(hipMemcpy(nat, sigs[i].code_fft, spec * nprn, hipMemcpyHostToDevice));
         // the precision form's last pass skips lags >= M / 2
         if (!(flags & SDR_SEARCH_PLAIN) && (flags & (SDR_SEARCH_FP16 | SDR_SEARCH_BF16)) && 2 * s.h.N <= M) {
@@ -1037,13 +1106,13 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         } else if (s.hp) {
             // converted for the chosen split below
             s.cf_nat = nat;
-            PSA_CHECK(hipMalloc(&s.cfh, sizeof(uint32_t) * (size_t)M * nprn));
+            pmalloc(&s.cfh, sizeof(uint32_t) * (size_t)M * nprn);
         } else {
             s.cf_nat = nat;
-            PSA_CHECK(hipMalloc(&s.cft, spec * nprn));
+            pmalloc(&s.cft, spec * nprn);
         }
-        PSA_CHECK(hipMalloc(&s.P, sizeof(float) * (size_t)nprn * nbin * s.h.N));
-        PSA_CHECK(hipMalloc(&s.fds, sizeof(float) * nbin));
+        pmalloc(&s.P, sizeof(float) * (size_t)nprn * nbin * s.h.N);
+        pmalloc(&s.fds, sizeof(float) * nbin);
         PSA_CHECK(hipMemcpy(s.fds, sigs[i].fds, sizeof(float) * nbin, hipMemcpyHostToDevice));
         // mixing parameters per (block, bin), phi = 0 (src/sdr_func.c:674)
         PlainMix &pm = s.hm;
@@ -1058,19 +1127,19 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
                            &pm.p0[j], &pm.p1[j], &pm.nseg[j], &pm.s[j]);
             }
         }
-        PSA_CHECK(hipMalloc(&s.mix_ix, sizeof(int) * nm));
-        PSA_CHECK(hipMalloc(&s.mix_nseg, sizeof(int) * nm));
-        PSA_CHECK(hipMalloc(&s.mix_p0, sizeof(uint32_t) * nm));
-        PSA_CHECK(hipMalloc(&s.mix_p1, sizeof(uint32_t) * nm));
-        PSA_CHECK(hipMalloc(&s.mix_s, sizeof(uint32_t) * nm));
+        pmalloc(&s.mix_ix, sizeof(int) * nm);
+        pmalloc(&s.mix_nseg, sizeof(int) * nm);
+        pmalloc(&s.mix_p0, sizeof(uint32_t) * nm);
+        pmalloc(&s.mix_p1, sizeof(uint32_t) * nm);
+        pmalloc(&s.mix_s, sizeof(uint32_t) * nm);
         PSA_CHECK(hipMemcpy(s.mix_ix, pm.ix.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
         PSA_CHECK(hipMemcpy(s.mix_nseg, pm.nseg.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
         PSA_CHECK(hipMemcpy(s.mix_p0, pm.p0.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
         PSA_CHECK(hipMemcpy(s.mix_p1, pm.p1.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
         PSA_CHECK(hipMemcpy(s.mix_s, pm.s.data(), sizeof(uint32_t) * nm, hipMemcpyHostToDevice));
-        PSA_CHECK(hipMalloc(&s.cn0, sizeof(float) * nprn));
-        PSA_CHECK(hipMalloc(&s.ix, sizeof(int) * 2 * nprn));
-        PSA_CHECK(hipMalloc(&s.dop, sizeof(double) * nprn));
+        pmalloc(&s.cn0, sizeof(float) * nprn);
+        pmalloc(&s.ix, sizeof(int) * 2 * nprn);
+        pmalloc(&s.dop, sizeof(double) * nprn);
         s.nchunk = (int)(((long long)nbin * s.h.N + PEAK_CHUNK - 1) / PEAK_CHUNK);
         // the last column pass emits one partial per workgroup when its
         // length has a specialised kernel
@@ -1078,7 +1147,7 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
                        !getenv("PSA_NO_FUSED_PEAK");
         s.xcd_order = xcd_order;
         if (flags & SDR_SEARCH_PLAIN) {
-            PSA_CHECK(hipMalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * s.nchunk));
+            pmalloc(&s.part, sizeof(PeakPart) * (size_t)nprn * s.nchunk);
         }
         // chunking of the intermediates (one word per complex in the
         // precision form)
@@ -1089,14 +1158,47 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         size_t pairs = std::max<size_t>(1, budget / zspec);
         s.prn_chunk = balance(nprn, (int)std::max<size_t>(1, std::min<size_t>(nprn, pairs / s.bin_chunk)));
         if (flags & SDR_SEARCH_PLAIN) s.prn_chunk = 1;
-        p->tmp_elems = std::max(p->tmp_elems, (size_t)s.bin_chunk * M);
-        p->x_elems = std::max(p->x_elems, (size_t)s.bin_chunk * M);
-        p->z_elems = std::max(p->z_elems, (size_t)s.bin_chunk * s.prn_chunk * M);
     }
-    PSA_CHECK(hipMalloc(&p->tmp, sizeof(float2) * p->tmp_elems));
-    PSA_CHECK(hipMalloc(&p->X, sizeof(float2) * p->x_elems));
-    PSA_CHECK(hipMalloc(&p->Z, sizeof(float2) * p->z_elems));
-    if (flags & SDR_SEARCH_PLAIN) PSA_CHECK(hipMalloc(&p->iq16, 2 * p->tmp_elems));
+    // lanes: the groups in decreasing order of work (FFT points over all
+    // cells and blocks) each go to the lane with the least work so far;
+    // PSA_LANES_N sets the lane count (1: every group in order on one stream)
+    int nl = std::min(PSA_LANES, n_sig);
+    if (const char *e = getenv("PSA_LANES_N")) nl = std::max(1, std::min(n_sig, atoi(e)));
+    p->lanes.resize(nl);
+    {
+        std::vector<int> order(n_sig);
+        std::vector<double> load(nl, 0.0);
+        for (int i = 0; i < n_sig; i++) order[i] = i;
+        auto work = [&](int i) {
+            const sdr_search_sig_t &g = p->sig[i].h;
+            return (double)g.M * g.n_prn * g.len_fds * g.n_blk;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+        for (int i : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            p->sig[i].lane = l;
+            load[l] += work(i);
+        }
+    }
+    for (auto &s : p->sig) {
+        SearchLane &ln = p->lanes[s.lane];
+        const size_t M = (size_t)s.h.M;
+        ln.tmp_elems = std::max(ln.tmp_elems, (size_t)s.bin_chunk * M);
+        ln.x_elems = std::max(ln.x_elems, (size_t)s.bin_chunk * M);
+        ln.z_elems = std::max(ln.z_elems, (size_t)s.bin_chunk * s.prn_chunk * M);
+    }
+    for (auto &ln : p->lanes) {
+        PSA_CHECK(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
+        PSA_CHECK(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
+        pmalloc(&ln.tmp, sizeof(float2) * ln.tmp_elems);
+        pmalloc(&ln.X, sizeof(float2) * ln.x_elems);
+        pmalloc(&ln.Z, sizeof(float2) * ln.z_elems);
+        if (flags & SDR_SEARCH_PLAIN) pmalloc(&ln.iq16, 2 * ln.tmp_elems);
+    }
+    PSA_CHECK(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+    p->gdone.resize(n_sig);
+    for (auto &e : p->gdone) PSA_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    p->cur_st = p->st;
     // batched forms: tile selection (SDR_SEARCH_TUNE), then the code spectra
     // in the chosen split's order
     for (auto &s : p->sig) {
@@ -1105,7 +1207,7 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         if ((flags & SDR_SEARCH_TUNE) || getenv("PSA_TILES")) tune_group(p, s);
         if (s.hp) h_setup(p, s, s.hp->M1, s.hvar_col, s.hvar_row);
         else f_setup(p, s, s.pl->M1);
-        PSA_CHECK(hipFree(s.cf_nat));
+        pfree(s.cf_nat);
         s.cf_nat = nullptr;
     }
     return p;
@@ -1191,6 +1293,12 @@ int sdr_search_plan_power(const sdr_search_plan_t *p, int sig, int prn_idx, floa
     size_t n = (size_t)s.h.len_fds * s.h.N;
     PSA_CHECK(hipMemcpy(P, s.P + (size_t)prn_idx * n, sizeof(float) * n, hipMemcpyDeviceToHost));
     return 1;
+}
+
+long long sdr_search_plan_bytes(void)
+{
+    std::lock_guard<std::mutex> lk(g_pm_mtx);
+    return g_pm_live;
 }
 
 int sdr_search_plan_form(const sdr_search_plan_t *p, int sig)
@@ -1281,16 +1389,23 @@ void sdr_search_plan_free(sdr_search_plan_t *p)
     (void)hipStreamSynchronize(p->st);
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->own_buf) {
-        for (auto d : p->d_buf) (void)hipFree(d);
+        for (auto d : p->d_buf) pfree(d);
     }
     for (auto &s : p->sig) {
-        (void)hipFree(s.cft); (void)hipFree(s.cfn); (void)hipFree(s.cfh); (void)hipFree(s.cf_nat);
-        (void)hipFree(s.P); (void)hipFree(s.fds);
-        (void)hipFree(s.mix_ix); (void)hipFree(s.mix_nseg); (void)hipFree(s.mix_p0);
-        (void)hipFree(s.mix_p1); (void)hipFree(s.mix_s);
-        (void)hipFree(s.cn0); (void)hipFree(s.ix); (void)hipFree(s.dop); (void)hipFree(s.part);
+        pfree(s.cft); pfree(s.cfn); pfree(s.cfh); pfree(s.cf_nat);
+        pfree(s.P); pfree(s.fds);
+        pfree(s.mix_ix); pfree(s.mix_nseg); pfree(s.mix_p0);
+        pfree(s.mix_p1); pfree(s.mix_s);
+        pfree(s.cn0); pfree(s.ix); pfree(s.dop); pfree(s.part);
     }
-    (void)hipFree(p->tmp); (void)hipFree(p->X); (void)hipFree(p->Z); (void)hipFree(p->iq16);
+    for (auto &ln : p->lanes) {
+        (void)hipStreamSynchronize(ln.st);
+        pfree(ln.tmp); pfree(ln.X); pfree(ln.Z); pfree(ln.iq16);
+        (void)hipEventDestroy(ln.done);
+        (void)hipStreamDestroy(ln.st);
+    }
+    for (auto &e : p->gdone) (void)hipEventDestroy(e);
+    if (p->fork) (void)hipEventDestroy(p->fork);
     for (auto &e : p->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     (void)hipStreamDestroy(p->st);
     delete p;

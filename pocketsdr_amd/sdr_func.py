@@ -93,6 +93,7 @@ _decl("sdr_search_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
 _decl("sdr_search_plan_kernel_bytes", C.c_int, [c_void_p, C.c_char_p, c_double_p])
 _decl("sdr_search_plan_kernel_grid", C.c_int, [c_void_p, C.c_char_p, c_double_p])
 _decl("sdr_search_plan_free", None, [c_void_p])
+_decl("sdr_search_plan_bytes", C.c_longlong, [])
 _decl("sdr_trk_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sdr_trk_ch_t),
                                      C.c_int, c_double_p, C.c_int])
 _decl("sdr_trk_plan_run", C.c_int, [c_void_p])
@@ -269,6 +270,11 @@ def dop_bins(T, dop, max_dop):
     out = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
     C.CDLL(None).free(C.cast(p, c_void_p))
     return out
+
+
+def search_plan_bytes():
+    """sdr_search_plan_bytes: device bytes held by the live search plans"""
+    return lib.sdr_search_plan_bytes()
 
 
 SEARCH_BATCHED = 0

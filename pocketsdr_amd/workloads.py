@@ -241,10 +241,35 @@ def acq_l1cp(seed=9, n_rf=8, fs=48e6, n_prn=32, n_vis=8, n_blk=20, max_dop=6000.
             c = int(np.argmin(np.abs(fds - s["fd"])))
             keep.update(range(max(0, c - bin_limit // 2), min(len(fds), c - bin_limit // 2 + bin_limit)))
         fds = fds[sorted(keep)]
-    buffs, sigs = [], []
+    sigs = []
+    rf_sats = []
     for r in range(n_rf):
-        rs = [dict(s, phi=float(rng.uniform(0, 1))) for s in sats]
-        buffs.append(synth.synth_if(fs, n_samp, rs, seed * 100 + r, bits=bits))
+        rf_sats.append([dict(s, phi=float(rng.uniform(0, 1))) for s in sats])
         sigs.append(dict(name="L1CP@rf%d" % r, buff=r, M=M, N=N, T=T, fs=fs, fi=0.0, code_fft=code_fft,
                          fds=fds, n_blk=n_blk, ix0=0, blk_step=N, prns=prns))
+    # the RF streams are independent: synthesised on a pool of host threads
+    # (numpy releases the interpreter lock in its array work)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, min(n_rf, 8))) as ex:
+        buffs = list(ex.map(lambda r: synth.synth_if(fs, n_samp, rf_sats[r], seed * 100 + r, bits=bits),
+                            range(n_rf)))
     return dict(buffs=buffs, sigs=sigs, truth={s["name"]: sats for s in sigs})
+
+
+# ---------------------------------------------------------------------------
+# real IF data
+# ---------------------------------------------------------------------------
+def read_if(path, IQ, n=None):
+    """int8 IF samples to the cpx8 buffer as sdr_read_data does
+    (src/sdr_func.c:451-510): I-sampling (IQ 1) takes SDR_CPX8(raw[i], 0),
+    IQ-sampling (IQ 2) SDR_CPX8(raw[2i], -raw[2i+1]) (Q polarity flipped);
+    n samples (None: all)."""
+    raw = np.fromfile(path, dtype=np.int8).astype(np.int64)
+    if IQ == 1:
+        I, Q = raw, np.zeros_like(raw)
+    else:
+        I, Q = raw[0::2], -raw[1::2]
+    if n is not None:
+        I, Q = I[:n], Q[:n]
+    return ((I & 0xF) | ((Q & 0xF) << 4)).astype(np.uint8)
+

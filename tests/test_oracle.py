@@ -445,3 +445,17 @@ def test_trk_stitch_full_period(orc):
                 parts.append(np.dot(x[:j], code[N - j:]))
         want = sum(parts) / N
         assert abs(P[e] - want) <= 1e-5 * abs(want) + 1e-7, e
+
+
+@pytest.mark.parametrize("N", [4093, 50021])
+def test_fft_f64_large_prime(orc, N):
+    """The FP64 transform of a prime length against numpy's FFT (a textbook
+    library routine): the generic radix's coefficient index q m mod r must
+    not overflow for r > 46340."""
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    for d in (0, 1):
+        ref = np.fft.fft(x) if d == 0 else np.fft.ifft(x) * N
+        got = orc.fft_f64(x, d)
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+
