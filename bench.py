@@ -398,6 +398,35 @@ def step_traffic(workload, form):
     return None
 
 
+def bench_l1cp_full(args, sdr):
+    """The weak-signal wide search at its full size on this GPU: GPS L1C-P on
+    all 8 RF channels of the array (32 PRNs x 241 bins x 20 blocks of 10 ms at
+    48 Msps each, M = 960000; 61,696 cells, 118.5 GB of power surfaces), FP16
+    precision form, seeded synthetic 3-bit IF.  Roofline by the per-cell-step
+    model (p = 2: M b_w = 3.84 MB does not fit one CU)."""
+    from pocketsdr_amd import workloads
+    w = workloads.acq_l1cp(seed=9)
+    sigs = w["sigs"]
+    cells = sum(s["code_fft"].shape[0] * len(s["fds"]) for s in sigs)
+    tiles = os.environ.get("PSA_TILES", "")
+    os.environ["PSA_TILES"] = (tiles + "," if tiles else "") + TILES["l1cp"]
+    plan = sdr.SearchPlan(w["buffs"], sigs, FORM_FLAGS["fp16"])
+    os.environ["PSA_TILES"] = tiles
+    steps = max(1, args.l1cp_steps)
+    t = time_plan(plan, steps, 1)
+    res = [plan.result(i) for i in range(len(sigs))]
+    plan.close()
+    model = cell_model(sigs, "fp16", t)["capacity"]
+    located = [len(set(decisions([s], [r], {s["name"]: w["truth"][s["name"]]})[s["name"]]["located"]))
+               for s, r in zip(sigs, res)]
+    log("l1cp full size: %.1f ms/step (%d cells)" % (t * 1e3, cells))
+    return dict(workload="acq_weak_L1CP_48Msps_10ms_x20_241bins_32prn_8rf", cells=cells, steps=steps, warmup=1,
+                ms_per_step=round(t * 1e3, 2), cells_per_s=round(cells / t, 1), form="fp16",
+                roofline=dict(bound="hbm", achieved=model["achieved_GBs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=model["frac"], alg_bytes_per_step=model["bytes_per_step"]),
+                located_per_rf=located, planted=len(w["truth"][sigs[0]["name"]]))
+
+
 def bench_raw(args, sdr, orc):
     """Front end (write_buff): one second of the 8-channel L1 array's raw
     stream (Pocket SDR FE 8CH, RAW32, 2-bit I/Q, 48 Msps) unpacked to the 8
@@ -683,6 +712,9 @@ def main():
     ap.add_argument("--skip-trk", action="store_true")
     ap.add_argument("--skip-raw", action="store_true")
     ap.add_argument("--skip-l6", action="store_true")
+    ap.add_argument("--skip-l1cp", action="store_true",
+                    help="skip the full-size 8-channel L1C-P search leg")
+    ap.add_argument("--l1cp-steps", type=int, default=2)
     ap.add_argument("--l6-channels", type=int, default=16)
     ap.add_argument("--l6-epochs", type=int, default=250)
     ap.add_argument("--cpu-prns", type=int, default=16)
@@ -731,6 +763,8 @@ def main():
         if orc is not None:
             res["dropin_per_call"] = bench_dropin(args, sdr, orc, w_trk)
         res["closed_loop_tracking"] = bench_trk_loop(args, sdr, w_trk)
+    if not args.skip_l1cp:
+        res["l1cp_full"] = bench_l1cp_full(args, sdr)
     if not args.skip_raw:
         r = bench_raw(args, sdr, orc)
         res["arch_combine"] = r.pop("arch_combine")

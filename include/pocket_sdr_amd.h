@@ -164,6 +164,26 @@ sdr_search_plan_t *sdr_search_plan_new(int n_buff, const sdr_buff_t *const *buff
  * run; the caller keeps them allocated while the plan lives. */
 sdr_search_plan_t *sdr_search_plan_new_dev(int n_buff, const uint8_t *const *dbuf,
     const int *buffN, int n_sig, const sdr_search_sig_t *sigs, int flags);
+/* The same on packed raw rings in device memory as the front end receives
+ * them (no cpx8 staging pass): the mixing loaders read each sample's raw
+ * byte for the stream's RF channel and convert it with the channel's table
+ * as write_buff does (src/sdr_rcv.c:840-914), so the power surfaces equal
+ * those of a plan on the unpacked cpx8 stream.  Batched forms only (not
+ * SDR_SEARCH_PLAIN); N x bytes per sample < 2^31.  Returns NULL on invalid
+ * input. */
+typedef struct {
+    const uint8_t *raw;        /* device memory: N samples of the format */
+    int N;                     /* samples in the ring */
+    int fmt;                   /* SDR_FMT_RAW8, _RAW16, _RAW16I or _RAW32
+                                  (src/pocket_sdr.h:87-90) */
+    int rfch;                  /* RF channel of the format */
+    int rtoc;                  /* 1: fs/4 real-to-complex tables (not RAW16I) */
+    const sdr_cpx8_t *lut;     /* host: the channel's table as sdr_raw_gen_lut
+                                  makes it (256 entries, 1024 with rtoc) */
+    int ph0;                   /* fs/4 phase of ring sample 0, (ix N) & 3 */
+} sdr_raw_stream_t;
+sdr_search_plan_t *sdr_search_plan_new_raw(int n_stream, const sdr_raw_stream_t *streams,
+    int n_sig, const sdr_search_sig_t *sigs, int flags);
 int sdr_search_plan_run(sdr_search_plan_t *plan);
 /* Per-PRN results of signal group `sig` as sdr_corr_max / sdr_fine_dop
  * produce them: cn0[n_prn], ix[2 n_prn] (bin, lag), dop[n_prn]. */

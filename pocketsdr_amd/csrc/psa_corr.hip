@@ -181,14 +181,27 @@ static void tap_geometry(int N, double coff, const double *pos, int n, int2 *jk)
     }
 }
 
-// Per-thread cache of device code banks: a bank is re-uploaded only when its
-// pointer, size or contents differ from the cached shadow copy.
+// Per-thread cache of device code banks (the per-tap path): keyed on the
+// bank's address, size and a 64-entry fingerprint, as the reference's banks
+// are built once per channel and never modified (src/sdr_ch.c:266-272).
+static uint64_t bank_fp(const uint8_t *p, size_t bytes)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (int i = 0; i < 64; i++) {
+        const size_t o = bytes * (size_t)i / 64;
+        h = (h ^ p[o]) * 1099511628211ull;
+        h = (h ^ p[o + 1 < bytes ? o + 1 : o]) * 1099511628211ull;
+    }
+    return h;
+}
+
 struct BankCache {
     struct Ent {
         const void *host = nullptr;
         size_t bytes = 0;
-        std::vector<uint8_t> shadow;
+        uint64_t fp = 0;
         void *dev = nullptr;
+        size_t cap = 0;
         unsigned long long used = 0;
     } e[4];
     unsigned long long tick = 0;
@@ -199,22 +212,23 @@ struct BankCache {
     const void *get(const void *host, size_t bytes, hipStream_t st)
     {
         tick++;
+        const uint64_t fp = bank_fp((const uint8_t *)host, bytes);
         for (auto &x : e) {
-            if (x.host == host && x.bytes == bytes && x.dev &&
-                memcmp(x.shadow.data(), host, bytes) == 0) {
+            if (x.host == host && x.bytes == bytes && x.dev && x.fp == fp) {
                 x.used = tick;
                 return x.dev;
             }
         }
         Ent *v = &e[0];
         for (auto &x : e) if (x.used < v->used) v = &x;
-        if (v->bytes < bytes || !v->dev) {
+        if (v->cap < bytes || !v->dev) {
             if (v->dev) PSA_CHECK(hipFree(v->dev));
             PSA_CHECK(hipMalloc(&v->dev, bytes));
+            v->cap = bytes;
         }
         v->host = host;
         v->bytes = bytes;
-        v->shadow.assign((const uint8_t *)host, (const uint8_t *)host + bytes);
+        v->fp = fp;
         v->used = tick;
         PSA_CHECK(hipMemcpyAsync(v->dev, host, bytes, hipMemcpyHostToDevice, st));
         return v->dev;
@@ -322,7 +336,10 @@ struct TrkLds {
 // epoch's state q, the channel's code banks from cbase; corr_out receives the
 // n_pos taps and C_out the two halves of tap 0 (global or LDS).  Starts and
 // ends with the workgroup in step (barriers inside).
-template <bool EXACT, bool SAMEQ, int TPW = TRK_TPW, int NTH = TRK_THREADS>
+// FROM16: buf holds N already-mixed cpx16 samples (int8 I, Q pairs, 8-byte
+// aligned) instead of the IF ring; they are split into the planes as they are
+// (sdr_corr_std's input).
+template <bool EXACT, bool SAMEQ, int TPW = TRK_TPW, int NTH = TRK_THREADS, bool FROM16 = false>
 __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *__restrict__ buf, int N,
                                               int n_pos, const double *__restrict__ pos,
                                               const uint32_t *__restrict__ cbase, long long code_copy_stride,
@@ -349,6 +366,32 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     // A word whose 4 samples lie in one ring segment takes its 4 IF bytes
     // with two aligned dword loads and a byte align (the ring is padded by a
     // dword) and steps the phase by s; other words go sample by sample.
+    if constexpr (FROM16) {
+        const uint2 *__restrict__ iq = reinterpret_cast<const uint2 *>(buf);
+        for (int w = threadIdx.x; w < Nq4; w += NTH) {
+            const int i0 = 4 * w;
+            uint32_t wi = 0, wq = 0;
+            if (i0 + 4 <= N) {
+                // samples i0 .. i0 + 3 as (I, Q) byte pairs in one 8-byte
+                // load: I in bytes 0, 2 of each dword, Q in bytes 1, 3
+                const uint2 v = iq[w];
+                wi = (v.x & 0xFFu) | ((v.x >> 8) & 0xFF00u) | ((v.y & 0xFFu) << 16) | ((v.y << 8) & 0xFF000000u);
+                wq = ((v.x >> 8) & 0xFFu) | ((v.x >> 16) & 0xFF00u) | ((v.y << 8) & 0xFF0000u) | (v.y & 0xFF000000u);
+            } else {
+                const uint8_t *b8 = buf;
+                for (int b = 0; b < 4; b++) {
+                    const int i = i0 + b;
+                    if (i < N) {
+                        wi |= (uint32_t)b8[2 * i] << (8 * b);
+                        wq |= (uint32_t)b8[2 * i + 1] << (8 * b);
+                    }
+                }
+            }
+            planeI[w] = wi;
+            planeQ[w] = wq;
+        }
+        __syncthreads();
+    } else {
     for (int w = threadIdx.x; w < Nq4; w += NTH) {
         const int i0 = 4 * w;
         uint32_t wi = 0, wq = 0;
@@ -392,6 +435,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
         planeQ[w] = wq;
     }
     __syncthreads();
+    }
 
     const uint8_t *cb = (const uint8_t *)cbase;   // [copy][bankI|bankQ][k][NP]
     const int cstride = (int)code_copy_stride;
@@ -889,12 +933,175 @@ struct sdr_trk_plan_tag {
     long long n_state = 0, n_corr = 0;
 };
 
+// ---------------------------------------------------------------------------
+// sdr_corr_std on the tracking correlator's device code: one workgroup splits
+// the caller's mixed samples into int8 I / Q planes in LDS and correlates all
+// taps with v_dot4 words from the bank in the correlator's layout (each of the
+// SDR_N_CODES banks doubled and in four byte-shifted copies).  The device
+// banks are cached per calling thread, keyed on the bank's address, size and
+// a 64-entry fingerprint: the reference's banks are built once per channel
+// and never modified (src/sdr_ch.c:266-272), so no call compares the whole
+// bank.
+// ---------------------------------------------------------------------------
+namespace psa {
+
+// doubled, byte-shifted code layout of one bank set: [copy 0..3][I banks
+// 0..9 | Q banks 0..9][NP bytes], byte i of a row = bank sample (i + copy)
+// mod N for i < 2 N - copy, zero after
+static void build_trk_bank(const sdr_cpx16_t *code, int N, int NP, uint8_t *dst)
+{
+    const size_t bank_bytes = (size_t)2 * PSA_N_CODES * NP;
+    for (int cp = 0; cp < 4; cp++) {
+        uint8_t *d = dst + (size_t)cp * bank_bytes;
+        for (int comp = 0; comp < 2; comp++) {
+            for (int k = 0; k < PSA_N_CODES; k++) {
+                uint8_t *row = d + ((size_t)comp * PSA_N_CODES + k) * NP;
+                const sdr_cpx16_t *src = code + (size_t)k * N;
+                for (int i = 0; i < 2 * N - cp; i++) {
+                    const sdr_cpx16_t &v = src[(i + cp) % N];
+                    row[i] = (uint8_t)(comp == 0 ? v.I : v.Q);
+                }
+            }
+        }
+    }
+}
+
+template <bool EXACT, bool SAMEQ>
+__global__ void __launch_bounds__(TRK_THREADS) corr_dropin_kernel(
+    const uint8_t *__restrict__ iq, int N, double coff, int n_pos, const double *__restrict__ pos,
+    const uint32_t *__restrict__ codes, long long code_copy_stride, int NP, float2 *__restrict__ corr_out,
+    float2 *__restrict__ C_out)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ float4 c12s[PSA_MAX_CORR];
+    __shared__ int2 jk[PSA_MAX_CORR];
+    __shared__ int sgn;
+    __shared__ int4 psum[TRK_THREADS / 64];
+    const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
+    TrkEpoch q = {};
+    q.cs = coff;
+    const TrkLds L{lds, lds + Nq4, nullptr, c12s, jk, &sgn, psum};
+    trk_correlate<EXACT, SAMEQ, TRK_TPW, TRK_THREADS, true>(q, iq, N, n_pos, pos, codes, code_copy_stride, NP, L,
+                                                            corr_out, C_out);
+}
+
+struct DropinBank {
+    const void *host = nullptr;
+    size_t bytes = 0;
+    uint64_t fp = 0;
+    uint8_t *dev = nullptr;
+    int N = 0, NP = 0;
+    bool sameq = false, ok = false;     // ok: values in {-1, 0, 1}
+    unsigned long long used = 0;
+};
+
+struct DropinBankCache {
+    DropinBank e[4];
+    unsigned long long tick = 0;
+    ~DropinBankCache()
+    {
+        for (auto &x : e) if (x.dev) (void)hipFree(x.dev);
+    }
+    const DropinBank &get(const sdr_cpx16_t *code, int N)
+    {
+        tick++;
+        const size_t bytes = sizeof(sdr_cpx16_t) * (size_t)N * PSA_N_CODES;
+        const uint64_t fp = bank_fp((const uint8_t *)code, bytes);
+        for (auto &x : e) {
+            if (x.host == code && x.bytes == bytes && x.fp == fp && x.N == N) {
+                x.used = tick;
+                return x;
+            }
+        }
+        DropinBank *v = &e[0];
+        for (auto &x : e) if (x.used < v->used) v = &x;
+        v->host = code; v->bytes = bytes; v->fp = fp; v->N = N; v->used = tick;
+        v->ok = true;
+        v->sameq = true;
+        for (size_t i = 0; i < (size_t)N * PSA_N_CODES; i++) {
+            const int a = code[i].I, b = code[i].Q;
+            if (a < -1 || a > 1 || b < -1 || b > 1) v->ok = false;
+            if (a != b) v->sameq = false;
+        }
+        v->NP = (2 * N + 1024 + 16 + 3) / 4 * 4;
+        const size_t lay = (size_t)4 * 2 * PSA_N_CODES * v->NP;
+        if (v->dev) { PSA_CHECK(hipFree(v->dev)); v->dev = nullptr; }
+        if (v->ok) {
+            std::vector<uint8_t> h(lay, 0);
+            build_trk_bank(code, N, v->NP, h.data());
+            PSA_CHECK(hipMalloc(&v->dev, lay));
+            PSA_CHECK(hipMemcpy(v->dev, h.data(), lay, hipMemcpyHostToDevice));
+        }
+        return *v;
+    }
+};
+
+template <bool EXACT, bool SAMEQ>
+static void dropin_launch_t(const uint8_t *iq, int N, double coff, int n, const double *pos, const DropinBank &bk,
+                            float2 *corr, float2 *C, hipStream_t st)
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        PSA_CHECK(hipFuncSetAttribute((const void *)corr_dropin_kernel<EXACT, SAMEQ>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    });
+    const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
+    const long long copy_stride = (long long)(2 * PSA_N_CODES) * bk.NP / 4;
+    hipLaunchKernelGGL((corr_dropin_kernel<EXACT, SAMEQ>), dim3(1), dim3(TRK_THREADS), sizeof(uint32_t) * 2 * Nq4,
+                       st, iq, N, coff, n, pos, (const uint32_t *)bk.dev, copy_stride, bk.NP, corr, C);
+    PSA_CHECK(hipGetLastError());
+}
+
+// sdr_corr_std through the tracking correlator; false when the bank holds
+// values outside {-1, 0, 1} or N exceeds one workgroup's LDS (the caller
+// then takes the per-tap kernels)
+static bool corr_std_fast(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N, double coff, const double *pos,
+                          int n, sdr_cpx_t *corr, sdr_cpx_t *C)
+{
+    if (n > PSA_MAX_CORR) PSA_FAIL("corr_std: n=%d exceeds SDR_MAX_CORR", n);
+    const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
+    if ((size_t)Nq4 * 8 + 4096 > 150 * 1024) return false;
+    thread_local DropinBankCache cache;
+    const DropinBank &bk = cache.get(code, N);
+    if (!bk.ok) return false;
+    std::vector<int2> jk(n);
+    tap_geometry(N, coff, pos, n, jk.data());      // fails on taps outside one period
+    ThreadCtx &tc = tctx();
+    hipStream_t st = tc.stream;
+    const size_t iq_bytes = sizeof(sdr_cpx16_t) * (size_t)N;
+    const size_t po = (iq_bytes + 255) / 256 * 256;
+    uint8_t *din = (uint8_t *)tc.s[SC_IQ].get(po + sizeof(double) * n);
+    uint8_t *hpin = (uint8_t *)tc.host_pinned(std::max(po + sizeof(double) * n, sizeof(float2) * (n + 2)));
+    memcpy(hpin, IQ, iq_bytes);
+    memcpy(hpin + po, pos, sizeof(double) * n);
+    // _mm256_sign_epi8 semantics only matter for samples of -128
+    const bool exact = memchr(hpin, 0x80, iq_bytes) != nullptr;
+    PSA_CHECK(hipMemcpyAsync(din, hpin, po + sizeof(double) * n, hipMemcpyHostToDevice, st));
+    float2 *dout = (float2 *)tc.s[SC_MISC].get(sizeof(float2) * (n + 2));
+    const double *dpos = (const double *)(din + po);
+    if (exact) {
+        if (bk.sameq) dropin_launch_t<true, true>(din, N, coff, n, dpos, bk, dout, dout + n, st);
+        else dropin_launch_t<true, false>(din, N, coff, n, dpos, bk, dout, dout + n, st);
+    } else {
+        if (bk.sameq) dropin_launch_t<false, true>(din, N, coff, n, dpos, bk, dout, dout + n, st);
+        else dropin_launch_t<false, false>(din, N, coff, n, dpos, bk, dout, dout + n, st);
+    }
+    PSA_CHECK(hipMemcpyAsync(hpin, dout, sizeof(float2) * (n + 2), hipMemcpyDeviceToHost, st));
+    PSA_CHECK(hipStreamSynchronize(st));
+    memcpy(corr, hpin, sizeof(float2) * n);
+    memcpy(C, hpin + sizeof(float2) * n, sizeof(float2) * 2);
+    return true;
+}
+
+}  // namespace psa
+
 extern "C" {
 
 void sdr_corr_std(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code, int N,
     double coff, const double *pos, int n, sdr_cpx_t *corr, sdr_cpx_t *C)
 {
-    corr_std_impl<false>(IQ, code, N, coff, pos, n, corr, C);
+    if (n <= 0 || N <= 0) return;
+    if (!corr_std_fast(IQ, code, N, coff, pos, n, corr, C)) corr_std_impl<false>(IQ, code, N, coff, pos, n, corr, C);
 }
 
 void sdr_corr_std_cpx_code(const sdr_cpx16_t *IQ, const sdr_cpx16_t *code,

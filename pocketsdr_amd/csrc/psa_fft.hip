@@ -181,7 +181,7 @@ template <int LM, int SM>
 __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_rt(PassArgs a, RadixSeq rs, int ngroups)
 {
     extern __shared__ float2 smem[];
-    __shared__ uint32_t carr[256];
+    __shared__ uint32_t carr[512];      // carrier table, then a raw ring's table
     const int L = a.L, G = a.G;
     float2 *buf0 = smem, *buf1 = smem + G * L;
     const int b = (int)(blockIdx.x / (unsigned)ngroups);
@@ -190,6 +190,10 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_rt(PassArgs a, Radix
     const float sgn = a.dir == 0 ? -1.0f : 1.0f;
     if (LM == LD_MIX) {
         for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
+        if (a.raw_bps) {
+            for (int i = threadIdx.x; i < 256; i += PASS_THREADS)
+                carr[256 + i] = reinterpret_cast<const uint32_t *>(a.raw_lut)[i];
+        }
         __syncthreads();
     }
     Loader<LM> ld;
@@ -396,7 +400,7 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
 {
     static_assert(make_rad_plan(CL).n >= 2, "specialised lengths need at least two radix stages");
     __shared__ float2 buf[CG * (CL + 1)];
-    __shared__ uint32_t carr[256];
+    __shared__ uint32_t carr[512];      // carrier table, then a raw ring's table
     __shared__ PeakPart red[PASS_THREADS / 64];
     int b, grp;
     if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp)) return;
@@ -405,6 +409,10 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
     const float sgn = a.dir == 0 ? -1.0f : 1.0f;
     if (LM == LD_MIX) {
         for (int i = threadIdx.x; i < 256; i += PASS_THREADS) carr[i] = a.carr[i];
+        if (a.raw_bps) {
+            for (int i = threadIdx.x; i < 256; i += PASS_THREADS)
+                carr[256 + i] = reinterpret_cast<const uint32_t *>(a.raw_lut)[i];
+        }
         __syncthreads();
     }
     long long pslot = 0;

@@ -403,7 +403,8 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     // copies, unused)
     constexpr int NTW = (L + NTH - 1) / NTH;
     __shared__ TwT tws[NTW * NTH];
-    __shared__ uint32_t carr[LM == LD_MIX ? NTH : 1];
+    // carrier table, then a raw ring's table (256 words each)
+    __shared__ uint32_t carr[LM == LD_MIX ? 512 : 1];
     __shared__ PeakPart red[NTH / 64];
     int b, grp;
     if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp)) return;
@@ -416,15 +417,29 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     float2 twv[NTW];
 #pragma unroll
     for (int q = 0; q < NTW; q++) twv[q] = a.tw[min((int)threadIdx.x + q * NTH, L - 1)];
-    uint32_t cav = 0;
-    if constexpr (LM == LD_MIX) cav = a.carr[threadIdx.x & 255];
+    constexpr int NCW = LM == LD_MIX ? (512 + NTH - 1) / NTH : 1;
+    uint32_t cav[NCW] = {};
+    if constexpr (LM == LD_MIX) {
+#pragma unroll
+        for (int q = 0; q < NCW; q++) {
+            const int i = (int)threadIdx.x + q * NTH;
+            if (i < 256) cav[q] = a.carr[i];
+            else if (i < 512 && a.raw_bps) cav[q] = reinterpret_cast<const uint32_t *>(a.raw_lut)[i - 256];
+        }
+    }
     auto fill = [&]() {
 #pragma unroll
         for (int q = 0; q < NTW; q++) {
             if constexpr (PKT) tws[threadIdx.x + q * NTH] = pack_c<DT>(twv[q].x, twv[q].y);
             else tws[threadIdx.x + q * NTH] = twv[q];
         }
-        if constexpr (LM == LD_MIX) carr[threadIdx.x] = cav;
+        if constexpr (LM == LD_MIX) {
+#pragma unroll
+            for (int q = 0; q < NCW; q++) {
+                const int i = (int)threadIdx.x + q * NTH;
+                if (i < 512) carr[i] = cav[q];
+            }
+        }
         __syncthreads();
     };
     auto barrier = []() { __syncthreads(); };

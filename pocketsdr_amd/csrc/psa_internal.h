@@ -165,6 +165,13 @@ struct PassArgs {
     const uint32_t *mix_s;      // [nbatch] phase step
     const int *mix_nseg;        // [nbatch] samples before the wrap split
     int mix_compact;            // 1: buf holds the block contiguously (no ring)
+    // LD_MIX on a packed raw ring (raw_bps > 0) instead of cpx8: sample i's
+    // byte for the channel is buf[i raw_bps + raw_off], its cpx8 value
+    // raw_lut[(raw_rtoc ? ((raw_ph0 + i) & 3) << 8 : 0) + byte]
+    // (write_buff, src/sdr_rcv.c:840-914); the 1024-byte table is staged in
+    // LDS after the carrier table
+    int raw_bps, raw_off, raw_rtoc, raw_ph0;
+    const uint8_t *raw_lut;
     // LD_CPX16: cpx16 input
     const int8_t *iq16;
     // LD_PROD: X (in, batch b -> b / prod_xdiv % prod_xmod) times Cf

@@ -49,6 +49,8 @@ struct Loader {
     const uint8_t *buf = nullptr;
     const uint32_t *carr = nullptr;     // LDS copy of the carrier table
     int ix = 0, nseg = 0, compact = 0;
+    int rbps = 0, roff = 0, rrtoc = 0, rph0 = 0;     // packed raw ring (rbps > 0)
+    const uint8_t *rlut = nullptr;                  // its table (LDS, after the carrier table)
     uint32_t p0 = 0, p1 = 0, s = 0;
     bool prod_tail = false;
     int prod_V = 0, prod_M1 = 1, prod_M2 = 1;
@@ -62,6 +64,8 @@ struct Loader {
             carr = carr_lds; buf = a.buf; compact = a.mix_compact;
             ix = a.mix_ix[b]; nseg = a.mix_nseg[b];
             p0 = a.mix_p0[b]; p1 = a.mix_p1[b]; s = a.mix_s[b];
+            rbps = a.raw_bps; roff = a.raw_off; rrtoc = a.raw_rtoc; rph0 = a.raw_ph0;
+            rlut = reinterpret_cast<const uint8_t *>(carr_lds + 256);
         }
         if (LM == LD_PROD) {
             const long long xb = (b / a.prod_xdiv) % a.prod_xmod;
@@ -78,13 +82,20 @@ struct Loader {
     // (src/sdr_func.c:113-134, 786-806): mix_byte fetches the IF byte,
     // mix_prod forms the product (split so that callers can keep loads in
     // flight)
+    __device__ __forceinline__ int mix_index(int e) const
+    {
+        return e < nseg ? ix + e : (compact ? e : e - nseg);
+    }
+    // the IF byte (raw rings: the byte holding the channel's bits)
     __device__ __forceinline__ uint32_t mix_byte(int e) const
     {
-        return buf[e < nseg ? ix + e : (compact ? e : e - nseg)];
+        const int i = mix_index(e);
+        return rbps ? buf[i * rbps + roff] : buf[i];
     }
     __device__ __forceinline__ int2 mix_int(int e) const { return mix_prod(e, mix_byte(e)); }
     __device__ __forceinline__ int2 mix_prod(int e, uint32_t d) const
     {
+        if (rbps) d = rlut[(rrtoc ? ((rph0 + mix_index(e)) & 3) << 8 : 0) + d];
         const uint32_t ph = e < nseg ? p0 + (uint32_t)e * s : p1 + (uint32_t)(e - nseg) * s;
         const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
         const int sQ = (int)((int8_t)(uint8_t)d) >> 4;

@@ -159,24 +159,37 @@ __global__ void __launch_bounds__(PEAK_THREADS) peak_final_kernel(const float *_
     }
 }
 
-// Per-thread cache of device code spectra in digit-transposed order.
+// Per-thread cache of device code spectra in digit-transposed order, keyed
+// on the spectrum's address, length and a 64-entry fingerprint: the
+// reference's code spectra are made once per channel and never modified
+// (src/sdr_ch.c:197-201), so no call compares the whole spectrum.
 struct SpecCache {
     const void *host = nullptr;
     int M = 0;
-    std::vector<uint8_t> shadow;
+    uint64_t fp = 0;
     float2 *dev = nullptr;
     ~SpecCache() { if (dev) (void)hipFree(dev); }
+    static uint64_t fingerprint(const sdr_cpx_t *c, int M)
+    {
+        uint64_t h = 1469598103934665603ull;
+        for (int i = 0; i < 64; i++) {
+            uint32_t w[2];
+            memcpy(w, c[(size_t)M * i / 64], sizeof(w));
+            h = (h ^ w[0]) * 1099511628211ull;
+            h = (h ^ w[1]) * 1099511628211ull;
+        }
+        return h;
+    }
     const float2 *get(const sdr_cpx_t *code_fft, const FFTPlan *pl, hipStream_t st)
     {
         size_t bytes = sizeof(float2) * (size_t)pl->M;
-        if (host == code_fft && M == pl->M && dev && memcmp(shadow.data(), code_fft, bytes) == 0) {
-            return dev;
-        }
+        const uint64_t f = fingerprint(code_fft, pl->M);
+        if (host == code_fft && M == pl->M && dev && fp == f) return dev;
         if (dev && M != pl->M) { PSA_CHECK(hipFree(dev)); dev = nullptr; }
         if (!dev) PSA_CHECK(hipMalloc(&dev, bytes * 2));
-        shadow.assign((const uint8_t *)code_fft, (const uint8_t *)code_fft + bytes);
         host = code_fft;
         M = pl->M;
+        fp = f;
         PSA_CHECK(hipMemcpyAsync(dev + pl->M, code_fft, bytes, hipMemcpyHostToDevice, st));
         fft_to_transposed(pl, dev + pl->M, dev, 1, st);
         return dev;
@@ -571,6 +584,10 @@ struct sdr_search_plan_tag {
     std::vector<int> buffN;
     std::vector<psa::SigDev> sig;
     std::vector<SearchLane> lanes;
+    // per IF stream: a packed raw ring (bps > 0, read through the channel's
+    // table by the mixing loaders) or cpx8 samples (bps 0)
+    struct RawDesc { int bps = 0, off = 0, rtoc = 0, ph0 = 0; uint8_t *dlut = nullptr; };
+    std::vector<RawDesc> raw;
     hipEvent_t fork = nullptr;  // start of a run on the plan's stream
     std::vector<hipEvent_t> gdone;   // end of each group (timed runs chain them)
     hipStream_t st = nullptr;
@@ -649,6 +666,14 @@ struct Tick {
     }
 };
 
+// the raw-ring parameters of IF stream i for the mixing loaders
+static void set_raw(const sdr_search_plan_t *p, int i, PassArgs &a)
+{
+    if (p->raw.empty() || !p->raw[i].bps) return;
+    const sdr_search_plan_t::RawDesc &r = p->raw[i];
+    a.raw_bps = r.bps; a.raw_off = r.off; a.raw_rtoc = r.rtoc; a.raw_ph0 = r.ph0; a.raw_lut = r.dlut;
+}
+
 // the batched, shared-forward-FFT schedule of one signal group
 static void run_batched(sdr_search_plan_t *p, SigDev &s)
 {
@@ -666,6 +691,7 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
             PassArgs a = {};
             args_fwd_col(pl, a);
             a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
+            set_raw(p, s.h.buff, a);
             a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
             a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
             a.out = ln.tmp; a.out_bstride = M;
@@ -744,6 +770,7 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
             h_col_args(hp, 0, a, s.hvar_col);
             a.tfull = hp->tfull[0];
             a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
+            set_raw(p, s.h.buff, a);
             a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
             a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
             a.hout = tmp; a.out_bstride = M;
@@ -1183,9 +1210,11 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
     for (auto &s : p->sig) {
         SearchLane &ln = p->lanes[s.lane];
         const size_t M = (size_t)s.h.M;
-        ln.tmp_elems = std::max(ln.tmp_elems, (size_t)s.bin_chunk * M);
-        ln.x_elems = std::max(ln.x_elems, (size_t)s.bin_chunk * M);
-        ln.z_elems = std::max(ln.z_elems, (size_t)s.bin_chunk * s.prn_chunk * M);
+        // (float2 elements; a word of the precision form is half of one)
+        const size_t wd = s.hp ? 2 : 1;
+        ln.tmp_elems = std::max(ln.tmp_elems, ((size_t)s.bin_chunk * M + wd - 1) / wd);
+        ln.x_elems = std::max(ln.x_elems, ((size_t)s.bin_chunk * M + wd - 1) / wd);
+        ln.z_elems = std::max(ln.z_elems, ((size_t)s.bin_chunk * s.prn_chunk * M + wd - 1) / wd);
     }
     for (auto &ln : p->lanes) {
         PSA_CHECK(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
@@ -1236,6 +1265,56 @@ sdr_search_plan_t *sdr_search_plan_new_dev(int n_buff, const uint8_t *const *dbu
     int n_sig, const sdr_search_sig_t *sigs, int flags)
 {
     return plan_create(n_buff, dbuf, buffN, true, n_sig, sigs, flags);
+}
+
+sdr_search_plan_t *sdr_search_plan_new_raw(int n_stream, const sdr_raw_stream_t *streams, int n_sig,
+    const sdr_search_sig_t *sigs, int flags)
+{
+    if (n_stream <= 0 || !streams) {
+        fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new_raw: invalid arguments\n");
+        return nullptr;
+    }
+    if (flags & SDR_SEARCH_PLAIN) {
+        fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new_raw: raw rings need a batched form\n");
+        return nullptr;
+    }
+    std::vector<const uint8_t *> data(n_stream);
+    std::vector<int> bN(n_stream);
+    std::vector<sdr_search_plan_t::RawDesc> rd(n_stream);
+    for (int i = 0; i < n_stream; i++) {
+        const sdr_raw_stream_t &r = streams[i];
+        // bytes per sample and the byte holding channel rfch (write_buff,
+        // src/sdr_rcv.c:860-905): RAW8 2 ch / 1 B, RAW16 4 ch / 2 B (2 per
+        // byte), RAW16I 8 ch / 2 B (4 per byte), RAW32 8 ch / 4 B (2 per byte)
+        int bps = 0, nch = 0, per = 1;
+        switch (r.fmt) {
+        case 3: bps = 1; nch = 2; per = 2; break;
+        case 4: bps = 2; nch = 4; per = 2; break;
+        case 5: bps = 2; nch = 8; per = 4; break;
+        case 6: bps = 4; nch = 8; per = 2; break;
+        }
+        if (!bps || r.rfch < 0 || r.rfch >= nch || !r.raw || !r.lut || r.N <= 0 ||
+            (long long)r.N * bps > 0x7FFFFFFFLL || (r.rtoc && r.fmt == 5)) {
+            fprintf(stderr, "pocketsdr_amd: sdr_search_plan_new_raw: invalid raw stream %d\n", i);
+            return nullptr;
+        }
+        data[i] = r.raw;
+        bN[i] = r.N;
+        rd[i].bps = bps;
+        rd[i].off = r.rfch / per;
+        rd[i].rtoc = r.rtoc ? 1 : 0;
+        rd[i].ph0 = r.ph0 & 3;
+    }
+    sdr_search_plan_t *p = plan_create(n_stream, data.data(), bN.data(), true, n_sig, sigs, flags);
+    if (!p) return nullptr;
+    for (int i = 0; i < n_stream; i++) {
+        uint8_t lut[1024] = {0};
+        memcpy(lut, streams[i].lut, streams[i].rtoc ? 1024 : 256);
+        pmalloc(&rd[i].dlut, sizeof(lut));
+        PSA_CHECK(hipMemcpy(rd[i].dlut, lut, sizeof(lut), hipMemcpyHostToDevice));
+    }
+    p->raw = rd;
+    return p;
 }
 
 int sdr_search_plan_run(sdr_search_plan_t *p)
@@ -1405,6 +1484,7 @@ void sdr_search_plan_free(sdr_search_plan_t *p)
         (void)hipStreamDestroy(ln.st);
     }
     for (auto &e : p->gdone) (void)hipEventDestroy(e);
+    for (auto &r : p->raw) pfree(r.dlut);
     if (p->fork) (void)hipEventDestroy(p->fork);
     for (auto &e : p->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     (void)hipStreamDestroy(p->st);

@@ -32,6 +32,11 @@ class sdr_search_sig_t(C.Structure):
                 ("n_blk", C.c_int), ("ix0", C.c_int), ("blk_step", C.c_int)]
 
 
+class sdr_raw_stream_t(C.Structure):
+    _fields_ = [("raw", c_void_p), ("N", C.c_int), ("fmt", C.c_int), ("rfch", C.c_int),
+                ("rtoc", C.c_int), ("lut", c_void_p), ("ph0", C.c_int)]
+
+
 class sdr_trk_ch_t(C.Structure):
     _fields_ = [("fs", C.c_double), ("fi", C.c_double), ("fc_sig", C.c_double),
                 ("T", C.c_double), ("N", C.c_int), ("fd0", C.c_double),
@@ -81,6 +86,8 @@ _decl("sdr_mix_carr", None, [C.POINTER(sdr_buff_t), C.c_int, C.c_int, C.c_double
 _decl("sdr_search_plan_new", c_void_p, [C.c_int, C.POINTER(C.POINTER(sdr_buff_t)), C.c_int,
                                         C.POINTER(sdr_search_sig_t), C.c_int])
 _decl("sdr_search_plan_new_dev", c_void_p, [C.c_int, C.POINTER(c_void_p), c_int_p, C.c_int,
+                                             C.POINTER(sdr_search_sig_t), C.c_int])
+_decl("sdr_search_plan_new_raw", c_void_p, [C.c_int, C.POINTER(sdr_raw_stream_t), C.c_int,
                                              C.POINTER(sdr_search_sig_t), C.c_int])
 _decl("sdr_search_plan_run", C.c_int, [c_void_p])
 _decl("sdr_search_plan_result", C.c_int, [c_void_p, C.c_int, c_float_p, c_int_p, c_double_p])
@@ -295,7 +302,13 @@ class SearchPlan:
     complex64, fds float32, n_blk, ix0, blk_step.
     """
 
-    def __init__(self, buffs, sigs, flags=SEARCH_BATCHED, dev_buffs=False):
+    def __init__(self, buffs, sigs, flags=SEARCH_BATCHED, dev_buffs=False, raw_streams=None):
+        """raw_streams: instead of buffs, packed raw rings in device memory:
+        dicts raw (device pointer), N, fmt, rfch, rtoc, lut (uint8 table),
+        ph0 (sdr_search_plan_new_raw)."""
+        if raw_streams is not None:
+            buffs = []
+            dev_buffs = True
         if not dev_buffs:
             self._bufs = [_buff(b) for b in buffs]
             arr = (C.POINTER(sdr_buff_t) * len(buffs))(*[C.pointer(b) for b in self._bufs])
@@ -309,7 +322,15 @@ class SearchPlan:
                                      cf.shape[0], _ptr(cf), len(fds), _fp(fds), s["n_blk"],
                                      s["ix0"], s["blk_step"])
         self.sigs = sigs
-        if dev_buffs:
+        if raw_streams is not None:
+            luts = [np.ascontiguousarray(r["lut"], np.uint8) for r in raw_streams]
+            self._keep += luts
+            ra = (sdr_raw_stream_t * len(raw_streams))()
+            for i, r in enumerate(raw_streams):
+                ra[i] = sdr_raw_stream_t(r["raw"], r["N"], r["fmt"], r["rfch"], int(r.get("rtoc", 0)),
+                                         _ptr(luts[i]), int(r.get("ph0", 0)))
+            self.p = lib.sdr_search_plan_new_raw(len(raw_streams), ra, len(sigs), sa, flags)
+        elif dev_buffs:
             dp_ = (c_void_p * len(buffs))(*[b[0] for b in buffs])
             nb = (C.c_int * len(buffs))(*[b[1] for b in buffs])
             self.p = lib.sdr_search_plan_new_dev(len(buffs), dp_, nb, len(sigs), sa, flags)

@@ -94,6 +94,27 @@ def test_corr_std_bitexact(sdr, orc, N, cpx):
         assert np.array_equal(gC.view(np.uint32), rC.view(np.uint32))
 
 
+def test_corr_std_dropin_banks(sdr, orc):
+    """The drop-in on the tracking correlator: real-code banks with I != Q,
+    five banks in turn (more than the per-thread device cache holds, so some
+    are rebuilt), 107 taps (three passes of the workgroup's tap slots), the
+    -128 corner in some calls only."""
+    N = 24000
+    rng = np.random.default_rng(77)
+    banks = [rng.integers(-1, 2, (10 * N, 2)).astype(np.int8) for _ in range(5)]
+    pos = np.concatenate([[0.0, -11.7, 11.7, -120.0], rng.uniform(-400.0, 400.0, 103)])
+    for it in range(12):
+        bank = banks[it % 5]
+        IQ = rng.integers(-112, 113, (N, 2)).astype(np.int8)
+        if it % 3 == 0:
+            IQ[rng.integers(0, N, 5)] = -128
+        coff = float(rng.uniform(0, N))
+        got, gC = sdr.corr_std(IQ, bank, N, coff, pos)
+        ref, rC = orc.corr_std(IQ, bank, N, coff, pos)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), it
+        assert np.array_equal(gC.view(np.uint32), rC.view(np.uint32)), it
+
+
 def test_corr_std_cpx(sdr, orc):
     N = 4000
     rng = np.random.default_rng(11)

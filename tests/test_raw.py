@@ -305,3 +305,63 @@ def test_raw_to_search_on_device(sdr, orc):
         ph.close()
     pd.close()
     rp.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt,chans", [(FMT_RAW32, [(1, 0), (4, 1)]), (FMT_RAW16, [(2, 0), (3, 1)]),
+                                       (FMT_RAW8, [(0, 1)]), (FMT_RAW16I, [(5, 0)])])
+def test_search_plan_on_raw_ring(sdr, orc, fmt, chans):
+    """Search plans on packed raw rings (sdr_search_plan_new_raw): the mixing
+    loaders read each raw byte and convert it with the channel's table, so
+    the FP32 and FP16 power surfaces and results are bit-identical to plans on
+    the cpx8 rings sdr_raw_unpack_dev makes of the same bytes (channels with
+    and without the fs/4 phase tables, ring phase 1, blocks across the wrap),
+    and the FP32 surface is within 1e-5 of the oracle's write_buff followed
+    by its sdr_search_code."""
+    import torch
+    from pocketsdr_amd import workloads
+    fs, T = 12e6, 1e-3
+    N = int(fs * T)
+    nring = 3 * N + 1232                     # a multiple of 16 samples
+    rng = np.random.default_rng(fmt)
+    bps = {FMT_RAW8: 1, FMT_RAW16: 2, FMT_RAW16I: 2, FMT_RAW32: 4}[fmt]
+    raw = rng.integers(0, 256, nring * bps).astype(np.uint8)
+    ph0 = 1
+    nch = {FMT_RAW8: 2, FMT_RAW16: 4, FMT_RAW16I: 8, FMT_RAW32: 8}[fmt]
+    rtoc = [0] * nch
+    for k, r in chans:
+        rtoc[k] = r
+    luts = [orc.gen_lutc_rtoc(k) if rtoc[k] else orc.gen_lutc(fmt, k, 2, 2) for k in range(nch)]
+    draw = torch.from_numpy(raw).cuda()
+    # the cpx8 rings of the same bytes with ring sample 0 at phase 1: three
+    # samples written as block 3 of length 3 (phases 1, 2, 3), the rest as
+    # block 0 (phases 0, 1, ...)
+    ring = np.concatenate([orc.write_buff(fmt, raw[:3 * bps], 3, 3, rtoc, luts),
+                           orc.write_buff(fmt, raw[3 * bps:], nring - 3, 0, rtoc, luts)], axis=1)
+    dif = torch.from_numpy(np.ascontiguousarray(ring)).cuda()
+    cds = [workloads.l1ca_code(p) for p in (3, 7, 11)]
+    cf = np.stack([orc.gen_code_fft(c, T, 0.0, fs, N, N) for c in cds]).astype(np.complex64)
+    sigs = [dict(buff=i, M=2 * N, N=N, T=T, fs=fs, fi=0.0, code_fft=cf, fds=workloads.dop_bins(T, 0.0, 1500.0),
+                 n_blk=3, ix0=N // 2 + 17, blk_step=N + 301, prns=[3, 7, 11]) for i in range(len(chans))]
+    streams = [dict(raw=draw.data_ptr(), N=nring, fmt=fmt, rfch=k, rtoc=r, lut=luts[k], ph0=ph0) for k, r in chans]
+    bufs = [(dif[k].data_ptr(), nring) for k, _ in chans]
+    for flag in (sdr.SEARCH_BATCHED, sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH):
+        pr = sdr.SearchPlan(None, sigs, flag, raw_streams=streams)
+        pc = sdr.SearchPlan(bufs, sigs, flag, dev_buffs=True)
+        pr.run()
+        pc.run()
+        for i in range(len(chans)):
+            for q in range(3):
+                assert np.array_equal(pr.power(i, q).view(np.uint32), pc.power(i, q).view(np.uint32)), (flag, i, q)
+            for a, b in zip(pr.result(i), pc.result(i)):
+                assert np.array_equal(a, b)
+            if flag == sdr.SEARCH_BATCHED:
+                s = sigs[i]
+                k = chans[i][0]
+                R = np.zeros((len(s["fds"]), s["M"]), np.float32)
+                for blk in range(s["n_blk"]):
+                    orc.search_code(cf[0], T, ring[k], s["ix0"] + blk * s["blk_step"], fs, 0.0, s["fds"], R)
+                P = pr.power(i, 0)
+                assert np.linalg.norm(P - R[:, :N]) / np.linalg.norm(R[:, :N]) < 1e-5
+        pr.close()
+        pc.close()
