@@ -71,12 +71,18 @@ HROW = {(160, 0): (16, 10, 1, 32, 256), (160, 1): (16, 10, 1, 32, 512),
         (960, 0): (32, 30, 1, 8, 256), (960, 1): (32, 30, 1, 8, 512)}
 
 
+# fused cell inverse kernels of the FP16 form by split (M1, M2)
+CELL = {(150, 160): "cell_kernel<160, 16, 10, 150, 10, 15, 32, 32, 512>"}
+
+
 def kernel_instance(cls, M, form="fp32", tiles=None):
     """Name of the pass kernel a (class, FFT length, form) launches, as
     rocprof lists it.  FP32: the four-step split M = M1 * M2 (rows M2 a
     multiple of 16 when possible, then the most balanced), column passes of
     length M1, row passes of length M2 (the plan's tiles when given).
     FP16 / BF16: the plan's tiles (M1, M2, column shape, row shape)."""
+    if cls == "cell" and tiles and tuple(tiles[:2]) in CELL:
+        return CELL[tuple(tiles[:2])]
     if cls not in MODES:
         return cls
     if form in ("fp16", "bf16") and tiles:
@@ -195,7 +201,7 @@ def bench_acq(args, sdr, orc):
     for _ in range(args.steps):
         plan.run()
     ks = {}
-    for n in ["fwd_col", "fwd_row", "inv_row", "inv_col", "peak", "fill"]:
+    for n in ["fwd_col", "fwd_row", "inv_row", "inv_col", "cell", "peak", "fill"]:
         for i, s in enumerate(sigs):
             r = plan.kernel_ms("%s@%d" % (n, i))
             if r:

@@ -690,6 +690,203 @@ __global__ void h_spec_kernel(const float2 *__restrict__ in, uint32_t *__restric
 }
 
 // ---------------------------------------------------------------------------
+// Fused inverse of search cells (FP16).  One workgroup per cell (PRN, Doppler
+// bin) runs, for every block k, the code-product row pass (M1 rows of length
+// M2 = LR = RR0 RR1), the column pass (M2 columns of length M1 = LC = RC0 RC1)
+// and the power accumulation, with the intermediate Z (M words) and the
+// cell's power surface P (N <= M / 2 floats) resident in LDS: Z never goes
+// to HBM, X and the code spectrum are read once per block (from L2: the
+// workgroups of one XCD hold one bin at a time), and P is written once, on
+// the last block, with the cell's peak partial.  Every stage is the mstage of
+// the two-pass form with the same inputs, DFT tables, twiddles and roundings,
+// so P is bit-identical to mpass_kernel's.
+//
+// LDS (32-bit words): Z[r][c] at r ZP + c + c / RR0 (a pad word after every
+// RR0 words of a row, as the row pass's first stage stores them; ZP odd), one
+// dummy word per lane for masked stores, P[lag], the row and column stage
+// twiddles (packed FP16), the peak reduction records.  The in-place stages
+// (all but the last column stage) read every input of their tiles before a
+// barrier and then write (mstage ALL).
+template <int LR, int RR0, int RR1, int LC, int RC0, int RC1, int GR, int GC, int NTH>
+struct CellGeom {
+    static constexpr int M1 = LC, M2 = LR, M = LC * LR;
+    static constexpr int ZP = (M2 + M2 / RR0) | 1;
+    static constexpr int ZW = M1 * ZP;
+    static constexpr int PO = ZW + 64;                    // P
+    // words up to the peak records (8-byte aligned) for P of N floats
+    static constexpr int tail(int N) { return (PO + N + LR + LC + 1) & ~1; }
+    // the four DFT tables (16-byte aligned) after the peak records
+    static constexpr int ATW[4] = {64 * ((2 * RR0 + 31) / 32) * ((2 * RR0 + 15) / 16),
+                                   64 * ((2 * RR1 + 31) / 32) * ((2 * RR1 + 15) / 16),
+                                   64 * ((2 * RC0 + 31) / 32) * ((2 * RC0 + 15) / 16),
+                                   64 * ((2 * RC1 + 31) / 32) * ((2 * RC1 + 15) / 16)};
+    static constexpr int ato(int N)
+    {
+        return (tail(N) + (int)(sizeof(PeakPart) * (NTH / 64) / 4) + 3) & ~3;
+    }
+    static constexpr size_t bytes(int N) { return (size_t)ato(N) * 4 + 16 * (size_t)(ATW[0] + ATW[1] + ATW[2] + ATW[3]); }
+};
+
+template <int LR, int RR0, int RR1, int LC, int RC0, int RC1, int GR, int GC, int NTH>
+__global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r0, const uint4 *am_r1,
+                                                   const uint4 *am_c0, const uint4 *am_c1, const float2 *tw_r,
+                                                   const float2 *tw_c, const uint32_t *tfullh)
+{
+    constexpr int DT = HDT_F16;
+    using Geo = CellGeom<LR, RR0, RR1, LC, RC0, RC1, GR, GC, NTH>;
+    constexpr int M1 = Geo::M1, M2 = Geo::M2, M = Geo::M, ZP = Geo::ZP, ZW = Geo::ZW;
+    constexpr int THL = (RC1 + 1) / 2;        // last stage outputs kept (lags < M / 2)
+    static_assert(M2 % GC == 0, "whole column groups");
+    extern __shared__ uint32_t lds[];
+    uint32_t *Z = lds;
+    uint32_t *twr = lds + Geo::PO + a.N, *twc = twr + LR;
+    PeakPart *red = reinterpret_cast<PeakPart *>(lds + Geo::tail(a.N));
+    const int dummy = ZW + (int)(threadIdx.x & 63);
+    const int N = a.N;
+    // cell of this workgroup: round r = w / 8 over the 8 XCDs (w % 8); an
+    // XCD runs the PRNs of one bin back to back (bins x, x + 8, ...)
+    const int w = (int)blockIdx.x, r = w >> 3;
+    const int prn = r % a.nprn, bin = (r / a.nprn) * 8 + (w & 7);
+    if (bin >= a.nbin) return;
+
+    // the stages' DFT tables in LDS (each stage loads its A operands from
+    // there, keeping registers for the tiles)
+    uint4 *atl = reinterpret_cast<uint4 *>(lds + Geo::ato(N));
+    uint4 *at_r0 = atl, *at_r1 = at_r0 + Geo::ATW[0], *at_c0 = at_r1 + Geo::ATW[1], *at_c1 = at_c0 + Geo::ATW[2];
+    for (int i = (int)threadIdx.x; i < Geo::ATW[0]; i += NTH) at_r0[i] = am_r0[i];
+    for (int i = (int)threadIdx.x; i < Geo::ATW[1]; i += NTH) at_r1[i] = am_r1[i];
+    for (int i = (int)threadIdx.x; i < Geo::ATW[2]; i += NTH) at_c0[i] = am_c0[i];
+    for (int i = (int)threadIdx.x; i < Geo::ATW[3]; i += NTH) at_c1[i] = am_c1[i];
+    for (int i = (int)threadIdx.x; i < LR; i += NTH) twr[i] = pack_c<DT>(tw_r[i].x, tw_r[i].y);
+    for (int i = (int)threadIdx.x; i < LC; i += NTH) twc[i] = pack_c<DT>(tw_c[i].x, tw_c[i].y);
+
+    const __amdgpu_buffer_rsrc_t cin = rsrc(a.C + (long long)prn * M);
+    const __amdgpu_buffer_rsrc_t tfr = rsrc(tfullh);
+    const __amdgpu_buffer_rsrc_t prr = rsrc(a.P + ((long long)prn * a.nbin + bin) * N);
+    auto barrier = []() { __syncthreads(); };
+    auto none = []() {};
+    auto ident = [](int, int, int, bool, uint32_t v) -> uint32_t { return v; };
+    Storer<ST_POW> sp;
+    sp.nlag = N;
+    sp.ibase = (long long)bin * N;
+
+    for (int k = 0; k < a.nblk; k++) {
+        const __amdgpu_buffer_rsrc_t xin = rsrc(a.X + (long long)k * a.xblk + (long long)bin * M);
+        // ---- row pass: groups of GR rows ----
+        for (int g0 = 0; g0 < M1; g0 += GR) {
+            const int gcount = min(GR, M1 - g0);
+            // the products X Cf of the group's rows, unpadded in their Z rows
+            // (16-byte loads; rows past the end read row g0 and are not stored)
+            constexpr int Q = GR * LR / 4, NQ = (Q + NTH - 1) / NTH;
+            uint4 xv[NQ], cv[NQ];
+#pragma unroll
+            for (int u = 0; u < NQ; u++) {
+                const int qq = min((int)threadIdx.x + u * NTH, Q - 1);
+                const int g = qq / (LR / 4), n = (qq - g * (LR / 4)) * 4;
+                const int e = (g0 + (g < gcount ? g : 0)) * M2 + n;
+                xv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xin, e * 4, 0, 0));
+                cv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(cin, e * 4, 0, 0));
+            }
+#pragma unroll
+            for (int u = 0; u < NQ; u++) {
+                const int q0 = (int)threadIdx.x + u * NTH;
+                const int g = q0 / (LR / 4), n = (q0 - g * (LR / 4)) * 4;
+                if (q0 < Q && g < gcount) {
+                    const uint32_t xw[4] = {xv[u].x, xv[u].y, xv[u].z, xv[u].w};
+                    const uint32_t cw[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
+#pragma unroll
+                    for (int v = 0; v < 4; v++) Z[(g0 + g) * ZP + n + v] = cmul_w<DT>(xw[v], cw[v]);
+                }
+            }
+            __syncthreads();
+            // stage 0 (radix RR0): input t of column j at n = j + t LR / RR0,
+            // output t at q = j RR0 + t (padded: j (RR0 + 1) + t), next
+            // twiddle w_LR^{t j}
+            auto rb0 = [&](int g, int j) -> int { return (g0 + (g < gcount ? g : 0)) * ZP + j; };
+            auto rl0 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+                return Z[base + (ok ? tc + th : RR0 - 1) * (LR / RR0)];
+            };
+            auto ro0 = [&](int g, int j) -> int4 { return make_int4((g0 + g) * ZP + j * (RR0 + 1), 0, j, g < gcount); };
+            auto rm0 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t { return twr[(ok ? tc + th : 0) * ob.z]; };
+            auto rp0 = [&](const int4 &ob, int tc, int th, float2 v, uint32_t f, bool ok) {
+                Z[ok ? ob.x + tc + th : dummy] = cmul_w<DT>(pack_c<DT>(v.x, v.y), f);
+            };
+            ATab<DT, RR0, RR0> ar0;
+            ar0.load(at_r0);
+            mstage<DT, NTH, LR, RR0, GR, true, RR0, 3, true, true>(ar0, rb0, rl0, ident, ro0, rm0, rp0, barrier);
+            __syncthreads();
+            // stage 1 (radix RR1): input t of column j (< RR0) at element
+            // j + t RR0 (padded j + t (RR0 + 1)); output t is element
+            // k = j + t RR0 of the row, times the inter-pass twiddle
+            // tfullh[row M2 + k], in place
+            auto rb1 = [&](int g, int j) -> int { return (g0 + (g < gcount ? g : 0)) * ZP + j; };
+            auto rl1 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+                return Z[base + (ok ? tc + th : RR1 - 1) * (RR0 + 1)];
+            };
+            auto ro1 = [&](int g, int j) -> int4 {
+                return make_int4((g0 + g) * ZP + j, (g0 + (g < gcount ? g : 0)) * M2 + j, 0, g < gcount);
+            };
+            auto rm1 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t {
+                return bload32_at(tfr, ob.y + th * RR0, tc * RR0, ok);
+            };
+            auto rp1 = [&](const int4 &ob, int tc, int th, float2 v, uint32_t f, bool ok) {
+                Z[ok ? ob.x + (tc + th) * (RR0 + 1) : dummy] = cmul_w<DT>(pack_c<DT>(v.x, v.y), f);
+            };
+            ATab<DT, RR1, RR1> ar1;
+            ar1.load(at_r1);
+            mstage<DT, NTH, LR, RR1, GR, true, RR1, 3, true, true>(ar1, rb1, rl1, ident, ro1, rm1, rp1, barrier);
+        }
+        __syncthreads();
+        // ---- column pass: groups of GC columns ----
+        sp.pinit = k == 0;
+        sp.track = k == a.nblk - 1;
+        const bool last = k == a.nblk - 1;
+        for (int c0 = 0; c0 < M2; c0 += GC) {
+            // Z column c (sub-transform g = c - c0), element (row) n
+            auto zc = [&](int g) -> int { return c0 + g + (c0 + g) / RR0; };
+            // stage 0 (radix RC0): input t of column j at row j + t LC / RC0,
+            // output t at row j RC0 + t, next twiddle w_LC^{t j}
+            auto cb0 = [&](int g, int j) -> int { return j * ZP + zc(g); };
+            auto cl0 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+                return Z[base + (ok ? tc + th : RC0 - 1) * (LC / RC0) * ZP];
+            };
+            auto co0 = [&](int g, int j) -> int4 { return make_int4(j * RC0 * ZP + zc(g), 0, j, 1); };
+            auto cm0 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t { return twc[(ok ? tc + th : 0) * ob.z]; };
+            auto cp0 = [&](const int4 &ob, int tc, int th, float2 v, uint32_t f, bool ok) {
+                Z[ok ? ob.x + (tc + th) * ZP : dummy] = cmul_w<DT>(pack_c<DT>(v.x, v.y), f);
+            };
+            ATab<DT, RC0, RC0> ac0;
+            ac0.load(at_c0);
+            mstage<DT, NTH, LC, RC0, GC, false, RC0, 3, true, true>(ac0, cb0, cl0, ident, co0, cm0, cp0, barrier);
+            __syncthreads();
+            // stage 1 (radix RC1, last): input t of column j (< RC0) at row
+            // j + t RC0; output t' < THL is element k = j + t' RC0 of the
+            // column: lag c + M2 k, P += kappa^2 |v|^2 for lags < N (in LDS;
+            // on the last block to global P, with the peak tracking)
+            auto cb1 = [&](int g, int j) -> int { return j * ZP + zc(g); };
+            auto cl1 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+                return Z[base + (ok ? tc + th : RC1 - 1) * RC0 * ZP];
+            };
+            auto co1 = [&](int g, int j) -> int4 { return make_int4(c0 + g + M2 * j, 0, 0, 1); };
+            auto cm1 = [](const int4 &, int, int, bool) -> int { return 0; };
+            auto cp1 = [&](const int4 &ob, int tc, int th, float2 v, int, bool ok) {
+                const int e = ob.x + (tc + th) * RC0 * M2;
+                const float pold = __uint_as_float(lds[Geo::PO + (e < N ? e : 0)]);
+                sp.store_pow_sel(e, v, a.kap2, pold, ok, [&](bool ok2, int e2, float nv) {
+                    if (last) bstore32(prr, e2, __float_as_uint(nv), ok2);
+                    else lds[ok2 ? Geo::PO + e2 : dummy] = __float_as_uint(nv);
+                });
+            };
+            ATab<DT, RC1, THL> ac1;
+            ac1.load(at_c1);
+            mstage<DT, NTH, LC, RC1, GC, false, THL, 0, true, true>(ac1, cb1, cl1, ident, co1, cm1, cp1, none);
+        }
+        __syncthreads();
+    }
+    sp.flush_peak(a.part, (long long)prn * a.nbin + bin, red);
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 // (L, R0, R1, R2, G, threads, variant): column-pass lengths (G columns =
@@ -977,6 +1174,67 @@ void h_code_spec(const HPlan *hp, const float2 *nat, uint32_t *out, int n, float
     hipLaunchKernelGGL(h_spec_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, nat, out, hp->M,
                        hp->M1, hp->M2, total, scale, hp->dt);
     PSA_CHECK(hipGetLastError());
+}
+
+}  // namespace psa
+
+namespace psa {
+
+// cell kernels: (M1 = LC, RC0, RC1, M2 = LR, RR0, RR1, GR, GC, threads) with
+// the radices of the two-pass shapes of those lengths (variant 0), so that
+// both forms use the same DFT tables
+#define PSA_CELL(X) X(150, 10, 15, 160, 16, 10, 50, 80, 1024)
+
+int cell_split(int M)
+{
+#define PSA_CS(LC_, C0_, C1_, LR_, R0_, R1_, GR_, GC_, T_) if (M == LC_ * LR_) return LC_;
+    PSA_CELL(PSA_CS)
+#undef PSA_CS
+    return 0;
+}
+
+// prepare: only raise the kernel's dynamic LDS limit (once; called at plan
+// set-up, outside any stream capture)
+template <int LR, int RR0, int RR1, int LC, int RC0, int RC1, int GR, int GC, int NTH>
+static void cell_launch_t(const HPlan *hp, const CellArgs &a, hipStream_t st, bool prepare)
+{
+    using Geo = CellGeom<LR, RR0, RR1, LC, RC0, RC1, GR, GC, NTH>;
+    auto kern = cell_kernel<LR, RR0, RR1, LC, RC0, RC1, GR, GC, NTH>;
+    static std::once_flag once;
+    std::call_once(once, [&]() {
+        PSA_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)Geo::bytes(Geo::M / 2)));
+    });
+    if (prepare) return;
+    const size_t lds = Geo::bytes(a.N);
+    if (2 * a.N > hp->M || lds > 160 * 1024 || Geo::bytes(Geo::M / 2) > 160 * 1024)
+        PSA_FAIL("cell kernel: N = %d does not fit (M = %d, %zu bytes of LDS)", a.N, hp->M, lds);
+    const HSpec *cs = col_spec(LC, 0), *rs = row_spec(LR, 0);
+    if (!cs || !rs || cs->r[0] != RC0 || cs->r[1] != RC1 || rs->r[0] != RR0 || rs->r[1] != RR1)
+        PSA_FAIL("cell kernel: radices differ from the pass shapes of %d x %d", LC, LR);
+    if (amat_radix(hp->am2[0][1][0]) != RR0 || amat_radix(hp->am2[0][1][1]) != RR1 ||
+        amat_radix(hp->am1[0][1][0]) != RC0 || amat_radix(hp->am1[0][1][1]) != RC1)
+        PSA_FAIL("cell kernel: DFT tables do not match %d x %d", LC, LR);
+    const long long nb = (long long)(a.nbin + 7) / 8 * 8 * a.nprn;
+    if (nb <= 0) return;
+    if (nb > 0x7FFFFFFFLL) PSA_FAIL("cell kernel launch too large (%lld workgroups)", nb);
+    pass_grid = nb * NTH;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(NTH), lds, st, a, hp->am2[0][1][0], hp->am2[0][1][1],
+                       hp->am1[0][1][0], hp->am1[0][1][1], hp->tw2[1], hp->tw1[1], hp->tfullh);
+    PSA_CHECK(hipGetLastError());
+}
+
+void launch_cells(const HPlan *hp, const CellArgs &a, hipStream_t st, bool prepare)
+{
+    if (hp->dt != HDT_F16 || !hp->tfullh) PSA_FAIL("cell kernel: FP16 plans only");
+#define PSA_CL(LC_, C0_, C1_, LR_, R0_, R1_, GR_, GC_, T_)                                      \
+    if (hp->M1 == LC_ && hp->M2 == LR_) {                                                    \
+        cell_launch_t<LR_, R0_, R1_, LC_, C0_, C1_, GR_, GC_, T_>(hp, a, st, prepare);      \
+        return;                                                                              \
+    }
+    PSA_CELL(PSA_CL)
+#undef PSA_CL
+    PSA_FAIL("no cell kernel for %d x %d", hp->M1, hp->M2);
 }
 
 }  // namespace psa

@@ -258,6 +258,26 @@ void launch_hpass(int dt, LoadMode lm, StoreMode sm, const PassArgs &a, int nbat
 void h_code_spec(const HPlan *hp, const float2 *nat, uint32_t *out, int n, float scale,
                  hipStream_t st);
 
+// Fused inverse of whole search cells (FP16 form): one workgroup per (PRN,
+// Doppler bin) runs the code-product row pass, the column pass and the power
+// accumulation of every block with the intermediate and the cell's power
+// surface in LDS, then writes P and the cell's peak partial once.
+struct CellArgs {
+    const uint32_t *X;      // forward spectra [n_blk][n_bin][M] words (digit-transposed)
+    long long xblk;         // words per block of X (n_bin M)
+    const uint32_t *C;      // code spectra [n_prn][M] words
+    float *P;               // [n_prn][n_bin][N]
+    PeakPart *part;         // [n_prn][n_bin]: one partial per cell
+    int nprn, nbin, nblk, N;
+    float kap2;             // power scale kappa^2
+};
+// the column length M1 of the split of M that has a cell kernel (0: none)
+int cell_split(int M);
+// launches the cell kernel of hp's split (hp: FP16, split cell_split(M));
+// N <= M / 2.  prepare: only set the kernel up (plan creation, before any
+// capture), no launch
+void launch_cells(const HPlan *hp, const CellArgs &a, hipStream_t st, bool prepare = false);
+
 // Full forward / backward transforms of `nbatch` contiguous length-M arrays
 // (natural order in and out), used by sdr_cpx_fft and the code spectra.
 void fft_c2c(const FFTPlan *pl, const float2 *in, float2 *out, float2 *tmp,

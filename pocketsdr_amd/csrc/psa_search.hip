@@ -552,14 +552,16 @@ struct SigDev {
     int nchunk_fused = 0;       // partials per PRN emitted by the last column pass
     bool fused_peak = false;    // the last column pass emits the peak partials
     bool xcd_order = true;      // XCD-aware workgroup order of the inverse row pass
+    bool cell = false;          // FP16 inverse as fused cell kernels (launch_cells)
     int lane = 0;               // the lane (stream and work buffers) the group runs on
     int bin_chunk = 1, prn_chunk = 1;
     float tuned_ms = 0.0f;      // tuner: best first-block time (0: not tuned)
 };
 
-enum { K_FWD_COL = 0, K_FWD_ROW, K_INV_ROW, K_INV_COL, K_MIX, K_CVT, K_MUL, K_POW, K_PEAK, K_FILL, K_NCLS };
+enum { K_FWD_COL = 0, K_FWD_ROW, K_INV_ROW, K_INV_COL, K_MIX, K_CVT, K_MUL, K_POW, K_PEAK, K_FILL, K_CELL,
+       K_NCLS };
 static const char *k_names[K_NCLS] = {"fwd_col", "fwd_row", "inv_row", "inv_col", "mix",
-                                      "cvt", "mul", "pow", "peak", "fill"};
+                                      "cvt", "mul", "pow", "peak", "fill", "cell"};
 
 }  // namespace psa
 
@@ -762,6 +764,44 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
     uint32_t *tmp = reinterpret_cast<uint32_t *>(ln.tmp);
     uint32_t *X = reinterpret_cast<uint32_t *>(ln.X);
     uint32_t *Z = reinterpret_cast<uint32_t *>(ln.Z);
+    if (s.cell) {
+        // the forward passes of every (block, bin) in one launch each, then
+        // one cell kernel over all (PRN, bin) cells and blocks
+        const int nall = s.h.n_blk * nbin;
+        PassArgs a = {};
+        h_col_args(hp, 0, a, s.hvar_col);
+        a.tfull = hp->tfull[0];
+        a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
+        set_raw(p, s.h.buff, a);
+        a.mix_ix = s.mix_ix; a.mix_nseg = s.mix_nseg;
+        a.mix_p0 = s.mix_p0; a.mix_p1 = s.mix_p1; a.mix_s = s.mix_s;
+        a.hout = tmp; a.out_bstride = M;
+        {
+            Tick t(p, K_FWD_COL, (double)nall * M, (double)s.h.n_blk * M + 4.0 * nall * M);
+            launch_hpass(dt, LD_MIX, ST_CPX, a, nall, st);
+        }
+        PassArgs r = {};
+        h_row_args(hp, 0, r, s.hvar_row);
+        r.osc = hp->osc_fr;
+        r.hin = tmp; r.hout = X; r.in_bstride = r.out_bstride = M;
+        {
+            Tick t(p, K_FWD_ROW, (double)nall * M, 8.0 * nall * M);
+            launch_hpass(dt, LD_CPX, ST_CPX, r, nall, st);
+        }
+        CellArgs c = {};
+        c.X = X; c.xblk = (long long)nbin * M;
+        c.C = s.cfh;
+        c.P = s.P; c.part = s.part;
+        c.nprn = nprn; c.nbin = nbin; c.nblk = s.h.n_blk; c.N = N;
+        c.kap2 = s.kappa * s.kappa;
+        {
+            // X of every block and bin, the code spectra, P once
+            Tick t(p, K_CELL, (double)nall * nprn * M,
+                   4.0 * M * ((double)nall + nprn) + 4.0 * (double)nprn * nbin * N);
+            launch_cells(hp, c, st);
+        }
+        return;
+    }
     for (int k = 0; k < s.h.n_blk; k++) {
         for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
             const int nbc = std::min(s.bin_chunk, nbin - b0);
@@ -908,7 +948,10 @@ static void h_setup(sdr_search_plan_t *p, SigDev &s, int M1, int vc, int vr)
     s.kappa = (float)((double)PSA_CSCALE / ((double)M * M * s.hp->osc_fc * s.hp->osc_fr * sc * s.hp->osc_ir));
     PassArgs tmp = {};
     h_col_args(s.hp, 1, tmp, vc);
-    const int nf = s.h.len_fds * pass_ngroups(tmp);
+    // one peak partial per workgroup of the last column pass (cell kernels:
+    // one per cell)
+    const int nf = s.cell ? s.h.len_fds : s.h.len_fds * pass_ngroups(tmp);
+    if (s.cell) launch_cells(s.hp, CellArgs{}, p->st, true);
     if (nf != s.nchunk_fused || !s.part) {
         if (s.part) pfree(s.part);
         s.nchunk_fused = nf;
@@ -978,8 +1021,9 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
     PSA_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
     int bm = m1[0], bc = 0, br = 0;
+    // cell groups have the one split of their cell kernel
     const bool fixed = fixed_tiles(s, &bm, &bc, &br);
-    if (!fixed && !(p->flags & SDR_SEARCH_TUNE)) {
+    if (s.cell || (!fixed && !(p->flags & SDR_SEARCH_TUNE))) {
         PSA_CHECK(hipEventDestroy(e0));
         PSA_CHECK(hipEventDestroy(e1));
         return;
@@ -1127,6 +1171,12 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         // the precision form's last pass skips lags >= M / 2
         if (!(flags & SDR_SEARCH_PLAIN) && (flags & (SDR_SEARCH_FP16 | SDR_SEARCH_BF16)) && 2 * s.h.N <= M) {
             s.hp = hfft_plan(M, (flags & SDR_SEARCH_BF16) ? HDT_BF16 : HDT_F16);
+            // FP16 lengths with a cell kernel take its split; PSA_NO_CELL
+            // keeps the two-pass inverse
+            if (s.hp && s.hp->dt == HDT_F16 && cell_split(M) && !getenv("PSA_NO_CELL")) {
+                s.cell = true;
+                s.hp = hfft_plan(M, HDT_F16, cell_split(M));
+            }
         }
         if (flags & SDR_SEARCH_PLAIN) {
             s.cfn = nat;
@@ -1212,6 +1262,13 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         const size_t M = (size_t)s.h.M;
         // (float2 elements; a word of the precision form is half of one)
         const size_t wd = s.hp ? 2 : 1;
+        if (s.cell) {
+            // forward spectra of every block and bin; no Z
+            const size_t nall = (size_t)s.h.n_blk * s.h.len_fds;
+            ln.tmp_elems = std::max(ln.tmp_elems, (nall * M + wd - 1) / wd);
+            ln.x_elems = std::max(ln.x_elems, (nall * M + wd - 1) / wd);
+            continue;
+        }
         ln.tmp_elems = std::max(ln.tmp_elems, ((size_t)s.bin_chunk * M + wd - 1) / wd);
         ln.x_elems = std::max(ln.x_elems, ((size_t)s.bin_chunk * M + wd - 1) / wd);
         ln.z_elems = std::max(ln.z_elems, ((size_t)s.bin_chunk * s.prn_chunk * M + wd - 1) / wd);
@@ -1221,7 +1278,7 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         PSA_CHECK(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
         pmalloc(&ln.tmp, sizeof(float2) * ln.tmp_elems);
         pmalloc(&ln.X, sizeof(float2) * ln.x_elems);
-        pmalloc(&ln.Z, sizeof(float2) * ln.z_elems);
+        if (ln.z_elems) pmalloc(&ln.Z, sizeof(float2) * ln.z_elems);
         if (flags & SDR_SEARCH_PLAIN) pmalloc(&ln.iq16, 2 * ln.tmp_elems);
     }
     PSA_CHECK(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
