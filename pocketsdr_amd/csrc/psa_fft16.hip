@@ -214,7 +214,7 @@ struct ATab {
 // output store (stages whose inputs and outputs share one LDS buffer, pre()
 // then being the barrier between them).
 template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, bool ALL = false,
-          class IBase, class ILoad, class ICook, class OBase, class OMul, class OPut, class Pre>
+          bool MASKT = true, bool PFM = false, class IBase, class ILoad, class ICook, class OBase, class OMul, class OPut, class Pre>
 __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &ibase, const ILoad &iload,
                                        const ICook &icook, const OBase &obase, const OMul &omul,
                                        const OPut &oput, const Pre &pre)
@@ -263,13 +263,14 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
     };
     // output t' = tc + 2 h is produced if t' < TH
     auto outok = [&](int tc) -> bool { return tc + 2 < TH || (tc < TH && h == 0); };
-    // one tile: multipliers, MFMAs, outputs
-    auto tile_body = [&](int g, int j, bool cv, const uint4 (&B)[KS]) {
-        const int4 ob = obase(g, j);
-        // accumulator 4 m + 2 w (+1) of row tile r: output t' = 16 r + 4 m +
-        // 2 h + w, real (imaginary) part
-        using MulT = decltype(omul(ob, 0, 0, true));
-        MulT mul[RT][4][2];
+    // the multipliers of a tile's outputs (accumulator 4 m + 2 w (+1) of row
+    // tile r is output t' = 16 r + 4 m + 2 h + w, real (imaginary) part);
+    // PFM: fetched one tile ahead (ALL: with the inputs), so that global
+    // multipliers (twiddle words, previous P values) are in flight across
+    // the tile before - at the cost of registers
+    using MulT = decltype(omul(int4{}, 0, 0, true));
+    struct Muls { MulT v[RT][4][2]; };
+    auto get_muls = [&](const int4 &ob, Muls &mu) {
 #pragma unroll
         for (int r = 0; r < RT; r++) {
 #pragma unroll
@@ -277,10 +278,13 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
 #pragma unroll
                 for (int w = 0; w < 2; w++) {
                     const int tc = 16 * r + 4 * m + w;
-                    if (MULK && tc < TH) mul[r][m][w] = omul(ob, tc, 2 * h, outok(tc));
+                    if (MULK && tc < TH) mu.v[r][m][w] = omul(ob, tc, 2 * h, outok(tc));
                 }
             }
         }
+    };
+    // one tile: MFMAs, outputs
+    auto tile_body = [&](const int4 &ob, bool cv, const uint4 (&B)[KS], const Muls &mu) {
         fx16 acc[RT];
 #pragma unroll
         for (int r = 0; r < RT; r++) {
@@ -292,7 +296,11 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
                 for (int k = 0; k < KS; k++) acc[r] = mfma<DT>(A[x][r][k], B[k], acc[r]);
             }
         }
-        const bool ok = cv && ob.w;
+        // MASKT: ok marks lanes past the last column or row; without it only
+        // outputs t' >= TH are masked (the caller's invalid lanes then store
+        // duplicates of valid ones, or into rows nothing reads), and the
+        // masks of all but the outputs next to TH fold away
+        const bool ok = MASKT ? (cv && ob.w) : true;
 #pragma unroll
         for (int r = 0; r < RT; r++) {
 #pragma unroll
@@ -302,7 +310,7 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
                     const int tc = 16 * r + 4 * m + w;
                     if (tc >= TH) continue;      // past TH in both halves
                     float2 v = make_float2(acc[r][4 * m + 2 * w], acc[r][4 * m + 2 * w + 1]);
-                    const MulT f = MULK ? mul[r][m][w] : MulT{};
+                    const MulT f = MULK ? mu.v[r][m][w] : MulT{};
                     if constexpr (MULK == 2) v = cmul(v, f);
                     else if constexpr (MULK == 1) v = make_float2(v.x * f.x, v.y * f.x);
                     oput(ob, tc, 2 * h, v, f, ok && outok(tc));
@@ -315,14 +323,19 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
     if constexpr (ALL) {
         static_assert(MT <= 4, "all tiles in flight: at most 4 per wave");
         Raw rw[MT][KS][4];
-        int ibs[MT], gs[MT], js[MT];
+        Muls mus[MT];
+        int ibs[MT];
+        int4 obs[MT];
         bool cvs[MT];
 #pragma unroll
         for (int i = 0; i < MT; i++) {
             const int ti = wave + i * NW;
-            cvs[i] = column(ti < NT ? ti : NT - 1, gs[i], js[i]);
-            ibs[i] = ibase(gs[i], js[i]);
+            int g, j;
+            cvs[i] = column(ti < NT ? ti : NT - 1, g, j);
+            ibs[i] = ibase(g, j);
+            obs[i] = obase(g, j);
             fetch(ibs[i], rw[i]);
+            if (PFM) get_muls(obs[i], mus[i]);
         }
         pre();
         at.complete();
@@ -331,7 +344,8 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
             if (wave + i * NW < NT) {
                 uint4 Bt[KS];
                 cook(ibs[i], rw[i], Bt);
-                tile_body(gs[i], js[i], cvs[i], Bt);
+                if (!PFM) get_muls(obs[i], mus[i]);
+                tile_body(obs[i], cvs[i], Bt, mus[i]);
             }
         }
         return;
@@ -342,34 +356,47 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
     bool cv = column(tile < NT ? tile : NT - 1, g, j);
     uint4 B[KS];
     Raw raw[KS][4];
+    Muls mu;
+    int4 ob = obase(g, j);
     const int ib0 = ibase(g, j);
     if (PF) fetch(ib0, raw);
     pre();
+    if (PFM) get_muls(ob, mu);
     at.complete();
     if (PF) cook(ib0, raw, B);
     for (; tile < NT; tile += NW) {
         const int tn = tile + NW;
         int gn = 0, jn = 0, ibn = 0;
         bool cvn = false;
-        if (PF) {
-            // next tile (a wave-uniform branch: the loads are not waited on
-            // inside it)
-            if (tn < NT) {
-                cvn = column(tn, gn, jn);
+        int4 obn = ob;
+        Muls mun;
+        // next tile (a wave-uniform branch: its loads are not waited on
+        // inside it)
+        if (tn < NT && (PF || PFM)) {
+            cvn = column(tn, gn, jn);
+            obn = obase(gn, jn);
+            if (PFM) get_muls(obn, mun);
+            if (PF) {
                 ibn = ibase(gn, jn);
                 fetch(ibn, raw);
             }
-        } else {
+        }
+        if (!PF) {
             const int ib = ibase(g, j);
             fetch(ib, raw);
             cook(ib, raw, B);
         }
-        tile_body(g, j, cv, B);
-        if (PF) {
-            if (tn < NT) cook(ibn, raw, B);
-            g = gn; j = jn; cv = cvn;
-        } else if (tn < NT) {
-            cv = column(tn, g, j);
+        if (!PFM) get_muls(ob, mu);
+        tile_body(ob, cv, B, mu);
+        if (tn < NT) {
+            if (PF) cook(ibn, raw, B);
+            if (PF || PFM) {
+                g = gn; j = jn; cv = cvn; ob = obn;
+                if (PFM) mu = mun;
+            } else {
+                cv = column(tn, g, j);
+                ob = obase(g, j);
+            }
         }
     }
 }
@@ -414,9 +441,15 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     // inputs (in mstage); the LDS fill then waits only for the first two
     ATab<DT, R0, R0> at0;
     at0.load(a.amat[0]);
+    // stage twiddles: two-stage lengths as [t][j] = w_L^{t j} (t < R0,
+    // j < L / R0), so that the lanes of a row pass, which walk j, read
+    // consecutive words at immediate offsets; three-stage lengths as w_L^m
     float2 twv[NTW];
 #pragma unroll
-    for (int q = 0; q < NTW; q++) twv[q] = a.tw[min((int)threadIdx.x + q * NTH, L - 1)];
+    for (int q = 0; q < NTW; q++) {
+        const int i = min((int)threadIdx.x + q * NTH, L - 1);
+        twv[q] = a.tw[S3 ? i : (i / (L / R0)) * (i % (L / R0))];
+    }
     constexpr int NCW = LM == LD_MIX ? (512 + NTH - 1) / NTH : 1;
     uint32_t cav[NCW] = {};
     if constexpr (LM == LD_MIX) {
@@ -451,9 +484,13 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
 
     // ---- stage 0: global -> MFMA -> LDS ----
     // output t of column j sits at q = j R0 + t (LDS j (R0 + PD) + t); the
-    // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages)
+    // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages,
+    // tws[t L / R0 + j])
     auto obase = [&](int g, int j) -> int4 { return make_int4(g * RP + j * (R0 + PD), 0, (j / R2) * R2, 1); };
-    auto omul = [&](const int4 &ob, int tc, int th, bool ok) -> TwT { return tws[(ok ? tc + th : 0) * ob.z]; };
+    auto omul = [&](const int4 &ob, int tc, int th, bool ok) -> TwT {
+        if constexpr (S3) return tws[(ok ? tc + th : 0) * ob.z];
+        else return tws[(ok ? tc + th : 0) * (L / R0) + ob.z];
+    };
     auto oput = [&](const int4 &ob, int tc, int th, float2 v, TwT f, bool ok) {
         uint32_t w = pack_c<DT>(v.x, v.y);
         if constexpr (PKT) w = cmul_w<DT>(w, f);
@@ -506,8 +543,8 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             return buf0[base + (ok ? tc + th : R0 - 1) * tstride];
         };
         auto icook = [](int, int, int, bool, uint32_t w) -> uint32_t { return w; };
-        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, true>(at0, ibase, iload, icook, obase, omul, oput,
-                                                                    barrier);
+        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, true, false>(at0, ibase, iload, icook, obase, omul,
+                                                                           oput, barrier);
     } else {
         // row passes (NF) read and write rows with unit element stride (a
         // compile-time constant: per-element offsets become immediates)
@@ -545,7 +582,8 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
                 return cmul_w<DT>(raw.x, raw.y);
             }
         };
-        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true>(at0, ibase, iload, icook, obase, omul, oput, fill);
+        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, false, false>(at0, ibase, iload, icook, obase, omul,
+                                                                            oput, fill);
     }
 
     // ---- last stage: LDS -> MFMA -> global ----
@@ -603,7 +641,8 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     if constexpr (!S3) {
         ATab<DT, R1, THL> at1;
         at1.load(a.amat[1]);
-        mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false>(at1, lds_base, iat0, ident, gbase, gmul, gput, barrier);
+        mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false, false, true, SM == ST_POW>(at1, lds_base, iat0, ident,
+                                                                                  gbase, gmul, gput, barrier);
     } else {
         // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
         // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
@@ -621,15 +660,16 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         };
         ATab<DT, R1, R1> at1;
         at1.load(a.amat[1]);
-        mstage<DT, NTH, L, R1, G, NF, R1, PKT ? 3 : 2, false>(at1, lds_base, iat0, ident, obase1, omul1, oput1,
-                                                              barrier);
+        mstage<DT, NTH, L, R1, G, NF, R1, PKT ? 3 : 2, false, false, false>(at1, lds_base, iat0, ident, obase1,
+                                                                             omul1, oput1, barrier);
         // stage 2 input t of column j: n = j + t R0 R1
         auto iat1 = [&](int base, int tc, int th, bool ok) -> uint32_t {
             return buf1[base + (ok ? tc + th : R2 - 1) * (R0 * R1 + PD * R1)];
         };
         ATab<DT, R2, THL> at2;
         at2.load(a.amat[2]);
-        mstage<DT, NTH, L, R2, G, NF, THL, MULKL, false>(at2, lds_base, iat1, ident, gbase, gmul, gput, barrier);
+        mstage<DT, NTH, L, R2, G, NF, THL, MULKL, false, false, true, SM == ST_POW>(at2, lds_base, iat1, ident,
+                                                                                  gbase, gmul, gput, barrier);
     }
     if constexpr (SM == ST_POW) {
         if (a.p_part) {
@@ -757,8 +797,15 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
     for (int i = (int)threadIdx.x; i < Geo::ATW[1]; i += NTH) at_r1[i] = am_r1[i];
     for (int i = (int)threadIdx.x; i < Geo::ATW[2]; i += NTH) at_c0[i] = am_c0[i];
     for (int i = (int)threadIdx.x; i < Geo::ATW[3]; i += NTH) at_c1[i] = am_c1[i];
-    for (int i = (int)threadIdx.x; i < LR; i += NTH) twr[i] = pack_c<DT>(tw_r[i].x, tw_r[i].y);
-    for (int i = (int)threadIdx.x; i < LC; i += NTH) twc[i] = pack_c<DT>(tw_c[i].x, tw_c[i].y);
+    // stage-0 twiddles as [t][j] = w_L^{t j} (t < R0, j < L / R0)
+    for (int i = (int)threadIdx.x; i < LR; i += NTH) {
+        const float2 v = tw_r[(i / (LR / RR0)) * (i % (LR / RR0))];
+        twr[i] = pack_c<DT>(v.x, v.y);
+    }
+    for (int i = (int)threadIdx.x; i < LC; i += NTH) {
+        const float2 v = tw_c[(i / (LC / RC0)) * (i % (LC / RC0))];
+        twc[i] = pack_c<DT>(v.x, v.y);
+    }
 
     const __amdgpu_buffer_rsrc_t cin = rsrc(a.C + (long long)prn * M);
     const __amdgpu_buffer_rsrc_t tfr = rsrc(tfullh);
@@ -801,19 +848,25 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
             __syncthreads();
             // stage 0 (radix RR0): input t of column j at n = j + t LR / RR0,
             // output t at q = j RR0 + t (padded: j (RR0 + 1) + t), next
-            // twiddle w_LR^{t j}
+            // twiddle w_LR^{t j} = twr[t LR / RR0 + j]
             auto rb0 = [&](int g, int j) -> int { return (g0 + (g < gcount ? g : 0)) * ZP + j; };
             auto rl0 = [&](int base, int tc, int th, bool ok) -> uint32_t {
                 return Z[base + (ok ? tc + th : RR0 - 1) * (LR / RR0)];
             };
-            auto ro0 = [&](int g, int j) -> int4 { return make_int4((g0 + g) * ZP + j * (RR0 + 1), 0, j, g < gcount); };
-            auto rm0 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t { return twr[(ok ? tc + th : 0) * ob.z]; };
+            // (rows past the end: the clamped row's outputs again, to its place)
+            auto ro0 = [&](int g, int j) -> int4 {
+                return make_int4((g0 + (g < gcount ? g : 0)) * ZP + j * (RR0 + 1), 0, j, 1);
+            };
+            auto rm0 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t {
+                return twr[(ok ? tc + th : 0) * (LR / RR0) + ob.z];
+            };
             auto rp0 = [&](const int4 &ob, int tc, int th, float2 v, uint32_t f, bool ok) {
                 Z[ok ? ob.x + tc + th : dummy] = cmul_w<DT>(pack_c<DT>(v.x, v.y), f);
             };
             ATab<DT, RR0, RR0> ar0;
             ar0.load(at_r0);
-            mstage<DT, NTH, LR, RR0, GR, true, RR0, 3, true, true>(ar0, rb0, rl0, ident, ro0, rm0, rp0, barrier);
+            mstage<DT, NTH, LR, RR0, GR, true, RR0, 3, true, true, false>(ar0, rb0, rl0, ident, ro0, rm0, rp0,
+                                                                          barrier);
             __syncthreads();
             // stage 1 (radix RR1): input t of column j (< RR0) at element
             // j + t RR0 (padded j + t (RR0 + 1)); output t is element
@@ -824,7 +877,8 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
                 return Z[base + (ok ? tc + th : RR1 - 1) * (RR0 + 1)];
             };
             auto ro1 = [&](int g, int j) -> int4 {
-                return make_int4((g0 + g) * ZP + j, (g0 + (g < gcount ? g : 0)) * M2 + j, 0, g < gcount);
+                const int gg = g0 + (g < gcount ? g : 0);
+                return make_int4(gg * ZP + j, gg * M2 + j, 0, 1);
             };
             auto rm1 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t {
                 return bload32_at(tfr, ob.y + th * RR0, tc * RR0, ok);
@@ -834,7 +888,8 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
             };
             ATab<DT, RR1, RR1> ar1;
             ar1.load(at_r1);
-            mstage<DT, NTH, LR, RR1, GR, true, RR1, 3, true, true>(ar1, rb1, rl1, ident, ro1, rm1, rp1, barrier);
+            mstage<DT, NTH, LR, RR1, GR, true, RR1, 3, true, true, false, true>(ar1, rb1, rl1, ident, ro1, rm1,
+                                                                                rp1, barrier);
         }
         __syncthreads();
         // ---- column pass: groups of GC columns ----
@@ -851,13 +906,16 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
                 return Z[base + (ok ? tc + th : RC0 - 1) * (LC / RC0) * ZP];
             };
             auto co0 = [&](int g, int j) -> int4 { return make_int4(j * RC0 * ZP + zc(g), 0, j, 1); };
-            auto cm0 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t { return twc[(ok ? tc + th : 0) * ob.z]; };
+            auto cm0 = [&](const int4 &ob, int tc, int th, bool ok) -> uint32_t {
+                return twc[(ok ? tc + th : 0) * (LC / RC0) + ob.z];
+            };
             auto cp0 = [&](const int4 &ob, int tc, int th, float2 v, uint32_t f, bool ok) {
                 Z[ok ? ob.x + (tc + th) * ZP : dummy] = cmul_w<DT>(pack_c<DT>(v.x, v.y), f);
             };
             ATab<DT, RC0, RC0> ac0;
             ac0.load(at_c0);
-            mstage<DT, NTH, LC, RC0, GC, false, RC0, 3, true, true>(ac0, cb0, cl0, ident, co0, cm0, cp0, barrier);
+            mstage<DT, NTH, LC, RC0, GC, false, RC0, 3, true, true, false>(ac0, cb0, cl0, ident, co0, cm0, cp0,
+                                                                           barrier);
             __syncthreads();
             // stage 1 (radix RC1, last): input t of column j (< RC0) at row
             // j + t RC0; output t' < THL is element k = j + t' RC0 of the
