@@ -212,7 +212,8 @@ struct ATab {
 // the stage then completes the table loads.
 // ALL: every tile's inputs of the wave are read before pre() and the first
 // output store (stages whose inputs and outputs share one LDS buffer, pre()
-// then being the barrier between them).
+// then being the barrier between them; first stages with few tiles, whose
+// global loads are then all in flight together).
 template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, bool ALL = false,
           bool MASKT = true, bool PFM = false, class IBase, class ILoad, class ICook, class OBase, class OMul, class OPut, class Pre>
 __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &ibase, const ILoad &iload,
@@ -321,7 +322,7 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
     // all of the wave's tiles read before the first is computed
     constexpr int MT = (NT + NW - 1) / NW;
     if constexpr (ALL) {
-        static_assert(MT <= 4, "all tiles in flight: at most 4 per wave");
+        static_assert(MT <= 6, "all tiles in flight: at most 6 per wave");
         Raw rw[MT][KS][4];
         Muls mus[MT];
         int ibs[MT];
@@ -582,8 +583,13 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
                 return cmul_w<DT>(raw.x, raw.y);
             }
         };
-        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, false, false>(at0, ibase, iload, icook, obase, omul,
-                                                                            oput, fill);
+        // complex-word inputs with few tiles per wave: every tile's loads
+        // are issued at once (before the LDS tables are filled), not one
+        // tile ahead (the mixing and product loaders hold more registers
+        // per input)
+        constexpr bool ALL0 = LM == LD_CPX && MT0 <= 6;
+        mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, ALL0, false>(at0, ibase, iload, icook, obase, omul,
+                                                                           oput, fill);
     }
 
     // ---- last stage: LDS -> MFMA -> global ----
