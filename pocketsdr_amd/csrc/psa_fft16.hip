@@ -756,7 +756,10 @@ __global__ void h_spec_kernel(const float2 *__restrict__ in, uint32_t *__restric
 template <int LR, int RR0, int RR1, int LC, int RC0, int RC1, int GR, int GC, int NTH>
 struct CellGeom {
     static constexpr int M1 = LC, M2 = LR, M = LC * LR;
-    static constexpr int ZP = (M2 + M2 / RR0) | 1;
+    // row pitch: a multiple of 4 words (16-byte staging stores), and 16
+    // banks apart from row to row, so that the last row stage's RR0
+    // (= 16) lanes per row read disjoint banks
+    static constexpr int ZP = (M2 + M2 / RR0 + 15) / 32 * 32 + 16;
     static constexpr int ZW = M1 * ZP;
     static constexpr int PO = ZW + 64;                    // P
     // words up to the peak records (8-byte aligned) for P of N floats
@@ -823,23 +826,32 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
     sp.nlag = N;
     sp.ibase = (long long)bin * N;
 
+    // the X and Cf words of a row group (16-byte loads; rows past the end
+    // read row g0), fetched one group ahead: the next group's loads are
+    // issued once the current group's last stage has issued its own (so
+    // that waiting for those does not wait for these)
+    constexpr int Q = GR * LR / 4, NQ = (Q + NTH - 1) / NTH;
+    uint4 xv[NQ], cv[NQ];
+    auto fetch_group = [&](int kk, int gg0) {
+        const __amdgpu_buffer_rsrc_t xin = rsrc(a.X + (long long)kk * a.xblk + (long long)bin * M);
+        const int gc = min(GR, M1 - gg0);
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int qq = min((int)threadIdx.x + u * NTH, Q - 1);
+            const int g = qq / (LR / 4), n = (qq - g * (LR / 4)) * 4;
+            const int e = (gg0 + (g < gc ? g : 0)) * M2 + n;
+            xv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xin, e * 4, 0, 0));
+            cv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(cin, e * 4, 0, 0));
+        }
+    };
+    fetch_group(0, 0);
+
     for (int k = 0; k < a.nblk; k++) {
-        const __amdgpu_buffer_rsrc_t xin = rsrc(a.X + (long long)k * a.xblk + (long long)bin * M);
         // ---- row pass: groups of GR rows ----
         for (int g0 = 0; g0 < M1; g0 += GR) {
             const int gcount = min(GR, M1 - g0);
             // the products X Cf of the group's rows, unpadded in their Z rows
-            // (16-byte loads; rows past the end read row g0 and are not stored)
-            constexpr int Q = GR * LR / 4, NQ = (Q + NTH - 1) / NTH;
-            uint4 xv[NQ], cv[NQ];
-#pragma unroll
-            for (int u = 0; u < NQ; u++) {
-                const int qq = min((int)threadIdx.x + u * NTH, Q - 1);
-                const int g = qq / (LR / 4), n = (qq - g * (LR / 4)) * 4;
-                const int e = (g0 + (g < gcount ? g : 0)) * M2 + n;
-                xv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xin, e * 4, 0, 0));
-                cv[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(cin, e * 4, 0, 0));
-            }
+            // (rows past the end are not stored)
 #pragma unroll
             for (int u = 0; u < NQ; u++) {
                 const int q0 = (int)threadIdx.x + u * NTH;
@@ -847,8 +859,12 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
                 if (q0 < Q && g < gcount) {
                     const uint32_t xw[4] = {xv[u].x, xv[u].y, xv[u].z, xv[u].w};
                     const uint32_t cw[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
-#pragma unroll
-                    for (int v = 0; v < 4; v++) Z[(g0 + g) * ZP + n + v] = cmul_w<DT>(xw[v], cw[v]);
+                    uint4 pw;
+                    pw.x = cmul_w<DT>(xw[0], cw[0]);
+                    pw.y = cmul_w<DT>(xw[1], cw[1]);
+                    pw.z = cmul_w<DT>(xw[2], cw[2]);
+                    pw.w = cmul_w<DT>(xw[3], cw[3]);
+                    *reinterpret_cast<uint4 *>(Z + (g0 + g) * ZP + n) = pw;
                 }
             }
             __syncthreads();
@@ -894,8 +910,15 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
             };
             ATab<DT, RR1, RR1> ar1;
             ar1.load(at_r1);
+            // after this stage's loads: the next group's (or the next
+            // block's first group's) X and Cf words, then the barrier
+            auto pre1 = [&]() {
+                if (g0 + GR < M1) fetch_group(k, g0 + GR);
+                else if (k + 1 < a.nblk) fetch_group(k + 1, 0);
+                __syncthreads();
+            };
             mstage<DT, NTH, LR, RR1, GR, true, RR1, 3, true, true, false, true>(ar1, rb1, rl1, ident, ro1, rm1,
-                                                                                rp1, barrier);
+                                                                                rp1, pre1);
         }
         __syncthreads();
         // ---- column pass: groups of GC columns ----
@@ -933,17 +956,29 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
             };
             auto co1 = [&](int g, int j) -> int4 { return make_int4(c0 + g + M2 * j, 0, 0, 1); };
             auto cm1 = [](const int4 &, int, int, bool) -> int { return 0; };
-            auto cp1 = [&](const int4 &ob, int tc, int th, float2 v, int, bool ok) {
-                const int e = ob.x + (tc + th) * RC0 * M2;
-                const float pold = __uint_as_float(lds[Geo::PO + (e < N ? e : 0)]);
-                sp.store_pow_sel(e, v, a.kap2, pold, ok, [&](bool ok2, int e2, float nv) {
-                    if (last) bstore32(prr, e2, __float_as_uint(nv), ok2);
-                    else lds[ok2 ? Geo::PO + e2 : dummy] = __float_as_uint(nv);
-                });
-            };
             ATab<DT, RC1, THL> ac1;
             ac1.load(at_c1);
-            mstage<DT, NTH, LC, RC1, GC, false, THL, 0, true, true>(ac1, cb1, cl1, ident, co1, cm1, cp1, none);
+            if (last) {
+                // P to global memory, with the peak tracking
+                auto cp1 = [&](const int4 &ob, int tc, int th, float2 v, int, bool ok) {
+                    const int e = ob.x + (tc + th) * RC0 * M2;
+                    const float pold = __uint_as_float(lds[Geo::PO + (e < N ? e : 0)]);
+                    sp.store_pow_sel(e, v, a.kap2, pold, ok, [&](bool ok2, int e2, float nv) {
+                        bstore32(prr, e2, __float_as_uint(nv), ok2);
+                    });
+                };
+                mstage<DT, NTH, LC, RC1, GC, false, THL, 0, true, true>(ac1, cb1, cl1, ident, co1, cm1, cp1, none);
+            } else {
+                // P in LDS (the same arithmetic as store_pow_sel)
+                auto cp1 = [&](const int4 &ob, int tc, int th, float2 v, int, bool ok) {
+                    const int e = ob.x + (tc + th) * RC0 * M2;
+                    const bool in = ok && e < N;
+                    const float pw = a.kap2 * __builtin_fmaf(v.x, v.x, v.y * v.y);
+                    const float pold = __uint_as_float(lds[Geo::PO + (in ? e : 0)]);
+                    lds[in ? Geo::PO + e : dummy] = __float_as_uint(k == 0 ? pw : pold + pw);
+                };
+                mstage<DT, NTH, LC, RC1, GC, false, THL, 0, true, true>(ac1, cb1, cl1, ident, co1, cm1, cp1, none);
+            }
         }
         __syncthreads();
     }
