@@ -61,6 +61,20 @@ _d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, 
 _d("orc_trk_stitch", None, [vp, C.c_int, C.c_int, vp])
 _d("orc_trkl_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, C.c_int, C.c_double, vp, vp, C.c_int,
                           C.c_int, dp, C.c_int, dp, vp, vp])
+class orc_tl_t(C.Structure):
+    _fields_ = [("T", C.c_double), ("len_code", C.c_int), ("costas", C.c_int),
+                ("fd", C.c_double), ("coff", C.c_double), ("cn0", C.c_double), ("pli", C.c_double),
+                ("err_phas", C.c_double), ("phas_acc", C.c_double), ("code_int", C.c_double),
+                ("sumE", C.c_double), ("sumL", C.c_double), ("sumP", C.c_double), ("sumN", C.c_double),
+                ("sumD", C.c_double), ("C0", C.c_float * 2),
+                ("lock", C.c_int), ("lost_cnt", C.c_int), ("pli_valid", C.c_int), ("idle", C.c_int)]
+
+
+_d("orc_tl_fll", None, [C.POINTER(orc_tl_t), fp])
+_d("orc_tl_pll", None, [C.POINTER(orc_tl_t), fp])
+_d("orc_tl_dll", None, [C.POINTER(orc_tl_t), fp])
+_d("orc_tl_cn0", C.c_int, [C.POINTER(orc_tl_t), fp, C.c_float, C.c_float])
+_d("orc_tl_update", None, [C.POINTER(orc_tl_t), fp, C.c_float, C.c_float])
 _d("orc_csk", C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, vp])
 _d("orc_trk_l6_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, vp, vp, C.c_int, C.c_int, dp, C.c_int,
                             vp, vp, vp, dp])
@@ -327,3 +341,34 @@ def arch_combine(ins, w):
     lib.orc_arch_combine(nrfch, ip_, N, _p(w), _p(out))
     return out
 
+
+
+def tl_state(T=1e-3, len_code=1023, costas=0, **kw):
+    """A closed-loop state (orc_tl_t) with the given fields."""
+    st = orc_tl_t()
+    st.T, st.len_code, st.costas, st.pli = T, len_code, costas, 1.0
+    for k, v in kw.items():
+        if k == "C0":
+            st.C0[0], st.C0[1] = v
+        else:
+            setattr(st, k, v)
+    return st
+
+
+
+def tl_step(step, st, corr, Px=0.0, Py=0.0):
+    """One loop step of the oracle on state st: step "fll", "pll", "dll",
+    "cn0" or "update"; corr = the epoch's correlators as (re, im) pairs
+    P, E, L, N (8 floats, missing ones zero).  Returns orc_tl_cn0's window
+    flag for "cn0", else None."""
+    c = np.zeros(8, np.float32)
+    c[:len(corr)] = corr
+    ref = C.byref(st)
+    cp = c.ctypes.data_as(fp)
+    if step == "cn0":
+        return lib.orc_tl_cn0(ref, cp, Px, Py)
+    if step == "update":
+        lib.orc_tl_update(ref, cp, Px, Py)
+    else:
+        getattr(lib, "orc_tl_" + step)(ref, cp)
+    return None

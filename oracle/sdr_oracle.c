@@ -591,94 +591,132 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
  */
 #define TL_DPI (2.0 * PI_D)
 
+/* FLL (src/sdr_ch.c:488-506), the pull-in branch of track_sig: from the
+ * second locked epoch, the phase change of the prompt since the previous
+ * epoch, atan2 (atan with a Costas discriminator) of cross and dot, steers
+ * fd with the wide (5 Hz, lock T < 0.5 s) or narrow (2 Hz) band; C is the
+ * epoch's correlators, the prompt first.  Called after lock is counted. */
+void orc_tl_fll(orc_tl_t *s, const float *C)
+{
+    if (s->lock >= 2) {
+        double IP1 = C[0], QP1 = C[1], IP2 = s->C0[0], QP2 = s->C0[1];
+        double dot = fma(IP1, IP2, QP1 * QP2);
+        double cross = fma(-QP1, IP2, IP1 * QP2);
+        if (dot != 0.0) {
+            double B = s->lock * s->T < 0.5 ? 5.0 : 2.0;   /* B_FLL_W, B_FLL_N */
+            double err = s->costas ? atan(cross / dot) : atan2(cross, dot);
+            s->fd -= B / 0.25 * err / TL_DPI;
+        }
+    }
+    s->C0[0] = C[0]; s->C0[1] = C[1];
+}
+
+/* PLL (src/sdr_ch.c:508-521): third order, W = B_PLL / 0.7845 */
+void orc_tl_pll(orc_tl_t *s, const float *C)
+{
+    double IP = C[0], QP = C[1];
+    if (IP != 0.0) {
+        double err = (s->costas ? atan(QP / IP) : atan2(QP, IP)) / TL_DPI;
+        double W = 5.0 / 0.7845;                      /* B_PLL */
+        s->phas_acc = fma(W * W * W * err, s->T, s->phas_acc);
+        double u = 1.1 * W * W * err * s->T;
+        s->fd += fma(s->T, s->phas_acc, fma(2.4 * W, err - s->err_phas, u));
+        s->err_phas = err;
+    }
+}
+
+/* DLL (src/sdr_ch.c:523-550): early (C[2..3]) and late (C[4..5]) powers
+ * summed over T_DLL, then the second-order filter, W = B_DLL / 0.53 */
+void orc_tl_dll(orc_tl_t *s, const float *C)
+{
+    int ND = (int)(0.02 / s->T);                          /* T_DLL */
+    if (ND < 1) ND = 1;
+    s->sumE += fmaf(C[2], C[2], C[3] * C[3]);
+    s->sumL += fmaf(C[4], C[4], C[5] * C[5]);
+    if (s->lock % ND == 0) {
+        double E = sqrt(s->sumE), L = sqrt(s->sumL);
+        if (E + L > 0.0) {
+            double err = (E - L) / (E + L) * 0.5 * s->T / s->len_code;
+            double W = 0.25 / 0.53;                       /* B_DLL */
+            double dt = s->T * ND;
+            s->code_int = fma(W * W * err, dt, s->code_int);
+            s->coff = fma(-fma(1.414 * W, err, s->code_int), dt, s->coff);
+        }
+        s->sumE = s->sumL = 0.0;
+    }
+}
+
+/* C/N0, carrier lock and loss (src/sdr_ch.c:574-624): prompt power P (the
+ * stitched prompt), IP^2 - QP^2 and the noise tap's power (C[6..7]) summed
+ * over T_CN0; at each window PLI = sumD / sumP, the C/N0 estimate filtered
+ * by FILT_CN0 = 0.5, and four bad windows in a row (C/N0 < 30 dB-Hz, or a
+ * Costas loop's PLI < 0.25) declare the signal lost.  Returns 1 at a window. */
+int orc_tl_cn0(orc_tl_t *s, const float *C, float Px, float Py)
+{
+    s->sumP += fmaf(Px, Px, Py * Py);
+    s->sumD += fmaf(Px, Px, -(Py * Py));
+    s->sumN += fmaf(C[6], C[6], C[7] * C[7]);
+    if (s->lock % (int)(0.5 / s->T) != 0) return 0;       /* T_CN0 */
+    if (s->sumP > 0.0) { s->pli = s->sumD / s->sumP; s->pli_valid = 1; }
+    if (s->sumN > 0.0) s->cn0 = fma(fma(log10(s->sumP / s->sumN / s->T), 10.0, -s->cn0), 0.5, s->cn0);
+    s->sumP = s->sumN = s->sumD = 0.0;
+    int bad = s->cn0 < 30.0 || (s->costas && s->pli_valid && s->pli < 0.25);
+    s->lost_cnt = bad ? s->lost_cnt + 1 : 0;
+    if (s->lost_cnt >= 4) { s->idle = 1; s->lock = 0; }
+    return 1;
+}
+
+/* one epoch's loop update after the correlators: lock counted, FLL during
+ * the pull-in (T_FPULLIN = 1 s) else PLL, DLL, C/N0 */
+void orc_tl_update(orc_tl_t *s, const float *C, float Px, float Py)
+{
+    s->lock++;
+    if (s->lock * s->T <= 1.0) orc_tl_fll(s, C);           /* T_FPULLIN */
+    else orc_tl_pll(s, C);
+    orc_tl_dll(s, C);
+    orc_tl_cn0(s, C, Px, Py);
+}
+
 void orc_trkl_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0,
                   const int8_t *code_bank, const uint8_t *buff, int buffN, int n_epoch,
                   const double *pos, int n_pos, double *state, float *corr, float *P)
 {
     const int N = ch->N;
-    double time = 0.0, fd = ch->fd0, coff = ch->coff0, phi = ch->phi0, cn0 = cn0_0, pli = 1.0;
-    double err_phas = 0.0, phas_acc = 0.0, code_int = 0.0;
-    double sumE = 0.0, sumL = 0.0, sumP = 0.0, sumN = 0.0, sumD = 0.0;
-    float C0[2] = {0.0f, 0.0f}, C1p[2] = {0.0f, 0.0f};
-    int lock = 0, lost_cnt = 0, pli_valid = 0, idle = 0;
+    orc_tl_t s;
+    memset(&s, 0, sizeof(s));
+    s.T = ch->T; s.len_code = len_code; s.costas = costas;
+    s.fd = ch->fd0; s.coff = ch->coff0; s.cn0 = cn0_0; s.pli = 1.0;
+    double time = 0.0, phi = ch->phi0;
+    float C1p[2] = {0.0f, 0.0f};
     int8_t *IQ = (int8_t *)malloc(2 * (size_t)N);
     float *C = (float *)malloc(sizeof(float) * 2 * (size_t)(n_pos > 0 ? n_pos : 1));
     float C1[4];
     for (int e = 0; e < n_epoch; e++) {
         double *st = state + 6 * (size_t)e;
         float *pe = P + 2 * (size_t)e;
-        if (idle) {
-            st[0] = fd; st[1] = coff; st[2] = phi; st[3] = cn0; st[4] = lock; st[5] = 1;
+        if (s.idle) {
+            st[0] = s.fd; st[1] = s.coff; st[2] = phi; st[3] = s.cn0; st[4] = s.lock; st[5] = 1;
             pe[0] = pe[1] = 0.0f;
             memset(corr + 2 * (size_t)e * n_pos, 0, sizeof(float) * 2 * n_pos);
             continue;
         }
         double t = (e + 1) * ch->T;
         double tau = t - time;
-        double fc = ch->fi + fd;
-        coff = fma(-(fd / ch->fc_sig), tau, coff);
+        double fc = ch->fi + s.fd;
+        s.coff = fma(-(s.fd / ch->fc_sig), tau, s.coff);
         phi = fma(fc, tau, phi);
         phi -= floor(phi);
         time = t;
-        if (coff >= ch->T) { coff -= ch->T; lock--; }
-        else if (coff < 0.0) { coff += ch->T; lock++; }
+        if (s.coff >= ch->T) { s.coff -= ch->T; s.lock--; }
+        else if (s.coff < 0.0) { s.coff += ch->T; s.lock++; }
         int ix = (int)(((long long)e * N) % buffN);
         orc_mix_carr(buff, buffN, ix, N, ch->fs, fc, phi, IQ);
-        orc_corr_std(IQ, code_bank, N, coff * ch->fs, pos, n_pos, C, C1);
+        orc_corr_std(IQ, code_bank, N, s.coff * ch->fs, pos, n_pos, C, C1);
         memcpy(corr + 2 * (size_t)e * n_pos, C, sizeof(float) * 2 * n_pos);
         float Px = (C1[0] + C1p[0]) / N, Py = (C1[1] + C1p[1]) / N;
         C1p[0] = C1[2]; C1p[1] = C1[3];
-        lock++;
-        if (lock * ch->T <= 1.0) {                           /* T_FPULLIN: FLL */
-            if (lock >= 2) {
-                double IP1 = C[0], QP1 = C[1], IP2 = C0[0], QP2 = C0[1];
-                double dot = fma(IP1, IP2, QP1 * QP2);
-                double cross = fma(-QP1, IP2, IP1 * QP2);
-                if (dot != 0.0) {
-                    double B = lock * ch->T < 0.5 ? 5.0 : 2.0;   /* B_FLL_W, B_FLL_N */
-                    double err = costas ? atan(cross / dot) : atan2(cross, dot);
-                    fd -= B / 0.25 * err / TL_DPI;
-                }
-            }
-            C0[0] = C[0]; C0[1] = C[1];
-        } else {                                              /* PLL */
-            double IP = C[0], QP = C[1];
-            if (IP != 0.0) {
-                double err = (costas ? atan(QP / IP) : atan2(QP, IP)) / TL_DPI;
-                double W = 5.0 / 0.7845;                      /* B_PLL */
-                phas_acc = fma(W * W * W * err, ch->T, phas_acc);
-                double u = 1.1 * W * W * err * ch->T;
-                fd += fma(ch->T, phas_acc, fma(2.4 * W, err - err_phas, u));
-                err_phas = err;
-            }
-        }
-        int ND = (int)(0.02 / ch->T);                         /* T_DLL */
-        if (ND < 1) ND = 1;
-        sumE += fmaf(C[2], C[2], C[3] * C[3]);
-        sumL += fmaf(C[4], C[4], C[5] * C[5]);
-        if (lock % ND == 0) {
-            double E = sqrt(sumE), L = sqrt(sumL);
-            if (E + L > 0.0) {
-                double err = (E - L) / (E + L) * 0.5 * ch->T / len_code;
-                double W = 0.25 / 0.53;                       /* B_DLL */
-                double dt = ch->T * ND;
-                code_int = fma(W * W * err, dt, code_int);
-                coff = fma(-fma(1.414 * W, err, code_int), dt, coff);
-            }
-            sumE = sumL = 0.0;
-        }
-        sumP += fmaf(Px, Px, Py * Py);
-        sumD += fmaf(Px, Px, -(Py * Py));
-        sumN += fmaf(C[6], C[6], C[7] * C[7]);
-        if (lock % (int)(0.5 / ch->T) == 0) {                 /* T_CN0 */
-            if (sumP > 0.0) { pli = sumD / sumP; pli_valid = 1; }
-            if (sumN > 0.0) cn0 = fma(fma(log10(sumP / sumN / ch->T), 10.0, -cn0), 0.5, cn0);
-            sumP = sumN = sumD = 0.0;
-            int bad = cn0 < 30.0 || (costas && pli_valid && pli < 0.25);
-            lost_cnt = bad ? lost_cnt + 1 : 0;
-            if (lost_cnt >= 4) { idle = 1; lock = 0; }
-        }
-        st[0] = fd; st[1] = coff; st[2] = phi; st[3] = cn0; st[4] = lock; st[5] = 0;
+        orc_tl_update(&s, C, Px, Py);
+        st[0] = s.fd; st[1] = s.coff; st[2] = phi; st[3] = s.cn0; st[4] = s.lock; st[5] = 0;
         pe[0] = Px; pe[1] = Py;
     }
     free(C);

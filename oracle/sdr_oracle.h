@@ -104,6 +104,21 @@ void orc_trk_run(const orc_trk_ch_t *ch, const int8_t *code_bank,
 /* Closed-loop tracking of one channel, src/sdr_ch.c:488-624, 694-763.
  * Per epoch: state[6] = {fd, coff, phi, cn0, lock, idle}, corr[n_pos] (cpx),
  * P (cpx). */
+/* closed-loop state and the single-epoch updates of orc_trkl_run */
+typedef struct {
+    double T;
+    int len_code, costas;
+    double fd, coff, cn0, pli;
+    double err_phas, phas_acc, code_int;
+    double sumE, sumL, sumP, sumN, sumD;
+    float C0[2];
+    int lock, lost_cnt, pli_valid, idle;
+} orc_tl_t;
+void orc_tl_fll(orc_tl_t *s, const float *C);
+void orc_tl_pll(orc_tl_t *s, const float *C);
+void orc_tl_dll(orc_tl_t *s, const float *C);
+int orc_tl_cn0(orc_tl_t *s, const float *C, float Px, float Py);
+void orc_tl_update(orc_tl_t *s, const float *C, float Px, float Py);
 void orc_trkl_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0,
                   const int8_t *code_bank, const uint8_t *buff, int buffN, int n_epoch,
                   const double *pos, int n_pos, double *state, float *corr, float *P);
