@@ -746,6 +746,14 @@ static void run_batched(sdr_search_plan_t *p, SigDev &s)
     }
 }
 
+// precision-form groups whose forward passes run for all blocks and bins at
+// once (spectra of every block held): cell groups, and groups whose bins are
+// not chunked
+static bool fwd_all_blocks(const SigDev &s)
+{
+    return s.hp && (s.cell || s.bin_chunk == s.h.len_fds);
+}
+
 // The precision form of run_batched: FP16 / BF16 words between the passes,
 // MFMA DFT-matrix stages.  Scales (powers of two): forward column pass output
 // osc_fc (folded into its twiddle table), forward row pass osc_fr, code
@@ -764,9 +772,10 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
     uint32_t *tmp = reinterpret_cast<uint32_t *>(ln.tmp);
     uint32_t *X = reinterpret_cast<uint32_t *>(ln.X);
     uint32_t *Z = reinterpret_cast<uint32_t *>(ln.Z);
-    if (s.cell) {
-        // the forward passes of every (block, bin) in one launch each, then
-        // one cell kernel over all (PRN, bin) cells and blocks
+    // unchunked bins (and cell groups): the forward passes of every (block,
+    // bin) in one launch each, before the inverse passes
+    const bool fwd_all = fwd_all_blocks(s);
+    if (fwd_all) {
         const int nall = s.h.n_blk * nbin;
         PassArgs a = {};
         h_col_args(hp, 0, a, s.hvar_col);
@@ -788,6 +797,10 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
             Tick t(p, K_FWD_ROW, (double)nall * M, 8.0 * nall * M);
             launch_hpass(dt, LD_CPX, ST_CPX, r, nall, st);
         }
+    }
+    if (s.cell) {
+        // one cell kernel over all (PRN, bin) cells and blocks
+        const int nall = s.h.n_blk * nbin;
         CellArgs c = {};
         c.X = X; c.xblk = (long long)nbin * M;
         c.C = s.cfh;
@@ -806,26 +819,30 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
         for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
             const int nbc = std::min(s.bin_chunk, nbin - b0);
             const long long mo = (long long)k * nbin + b0;
-            PassArgs a = {};
-            h_col_args(hp, 0, a, s.hvar_col);
-            a.tfull = hp->tfull[0];
-            a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
-            set_raw(p, s.h.buff, a);
-            a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
-            a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
-            a.hout = tmp; a.out_bstride = M;
-            {
-                Tick t(p, K_FWD_COL, (double)nbc * M, (double)M + 4.0 * nbc * M);
-                launch_hpass(dt, LD_MIX, ST_CPX, a, nbc, st);
+            if (!fwd_all) {
+                PassArgs a = {};
+                h_col_args(hp, 0, a, s.hvar_col);
+                a.tfull = hp->tfull[0];
+                a.buf = buf; a.buffN = buffN; a.mix_compact = 0;
+                set_raw(p, s.h.buff, a);
+                a.mix_ix = s.mix_ix + mo; a.mix_nseg = s.mix_nseg + mo;
+                a.mix_p0 = s.mix_p0 + mo; a.mix_p1 = s.mix_p1 + mo; a.mix_s = s.mix_s + mo;
+                a.hout = tmp; a.out_bstride = M;
+                {
+                    Tick t(p, K_FWD_COL, (double)nbc * M, (double)M + 4.0 * nbc * M);
+                    launch_hpass(dt, LD_MIX, ST_CPX, a, nbc, st);
+                }
+                PassArgs r = {};
+                h_row_args(hp, 0, r, s.hvar_row);
+                r.osc = hp->osc_fr;
+                r.hin = tmp; r.hout = X; r.in_bstride = r.out_bstride = M;
+                {
+                    Tick t(p, K_FWD_ROW, (double)nbc * M, 8.0 * nbc * M);
+                    launch_hpass(dt, LD_CPX, ST_CPX, r, nbc, st);
+                }
             }
-            PassArgs r = {};
-            h_row_args(hp, 0, r, s.hvar_row);
-            r.osc = hp->osc_fr;
-            r.hin = tmp; r.hout = X; r.in_bstride = r.out_bstride = M;
-            {
-                Tick t(p, K_FWD_ROW, (double)nbc * M, 8.0 * nbc * M);
-                launch_hpass(dt, LD_CPX, ST_CPX, r, nbc, st);
-            }
+            // this block's spectra (X of every block when computed at once)
+            const uint32_t *Xk = fwd_all ? X + mo * M : X;
             for (int q0 = 0; q0 < nprn; q0 += s.prn_chunk) {
                 const int nqc = std::min(s.prn_chunk, nprn - q0);
                 const int nbatch = nqc * nbc;      // b = prn_local * nbc + bin_local
@@ -833,7 +850,7 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
                 h_row_args(hp, 1, ir, s.hvar_row);
                 ir.tfull = hp->tfull[1];
                 ir.tfullh = hp->tfullh;
-                ir.hin = X; ir.in_bstride = M;
+                ir.hin = Xk; ir.in_bstride = M;
                 ir.hcf = s.cfh + (long long)q0 * M; ir.cf_bstride = M;
                 ir.prod_xdiv = 1; ir.prod_xmod = nbc; ir.prod_cdiv = nbc;
                 ir.hout = Z; ir.out_bstride = M;
@@ -1262,15 +1279,11 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         const size_t M = (size_t)s.h.M;
         // (float2 elements; a word of the precision form is half of one)
         const size_t wd = s.hp ? 2 : 1;
-        if (s.cell) {
-            // forward spectra of every block and bin; no Z
-            const size_t nall = (size_t)s.h.n_blk * s.h.len_fds;
-            ln.tmp_elems = std::max(ln.tmp_elems, (nall * M + wd - 1) / wd);
-            ln.x_elems = std::max(ln.x_elems, (nall * M + wd - 1) / wd);
-            continue;
-        }
-        ln.tmp_elems = std::max(ln.tmp_elems, ((size_t)s.bin_chunk * M + wd - 1) / wd);
-        ln.x_elems = std::max(ln.x_elems, ((size_t)s.bin_chunk * M + wd - 1) / wd);
+        // forward spectra of every block and bin (cell groups need no Z)
+        const size_t nfwd = fwd_all_blocks(s) ? (size_t)s.h.n_blk * s.h.len_fds : (size_t)s.bin_chunk;
+        ln.tmp_elems = std::max(ln.tmp_elems, (nfwd * M + wd - 1) / wd);
+        ln.x_elems = std::max(ln.x_elems, (nfwd * M + wd - 1) / wd);
+        if (s.cell) continue;
         ln.z_elems = std::max(ln.z_elems, ((size_t)s.bin_chunk * s.prn_chunk * M + wd - 1) / wd);
     }
     for (auto &ln : p->lanes) {
