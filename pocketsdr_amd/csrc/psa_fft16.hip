@@ -406,7 +406,13 @@ template <int LM, int SM, int DT, int L, int R0, int R1, int R2, int G, bool NF,
 __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
 {
     using Geo = MGeom<L, R0>;
-    constexpr int RP = Geo::RP, PD = Geo::PAD;
+    // code-product row passes with few tiles per wave stage their inputs in
+    // LDS (STG, below); their rows are 16-byte aligned for 16-byte staging
+    // stores and 16 banks apart
+    constexpr int NT0 = (G * (L / R0) + 31) / 32, MT0 = (NT0 + NTH / 64 - 1) / (NTH / 64);
+    constexpr bool STG = LM == LD_PROD && NF && MT0 <= 4 && L % 4 == 0;
+    constexpr int RP = STG ? (L + (Geo::PAD ? L / R0 : 0) + 15) / 32 * 32 + 16 : Geo::RP;
+    constexpr int PD = Geo::PAD;
     constexpr bool S3 = R2 > 1;
     constexpr int RL = S3 ? R2 : R1;          // last radix
     constexpr int NSL = L / RL;               // its NS (= columns per sub-transform)
@@ -425,7 +431,7 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     constexpr bool TWP = PKT && TW && LM == LD_PROD;
     using TwT = std::conditional_t<PKT, uint32_t, float2>;
     // one dummy word per lane after the rows takes the masked LDS stores
-    __shared__ uint32_t buf0[G * RP + 64];
+    __shared__ __align__(16) uint32_t buf0[G * RP + 64];
     __shared__ uint32_t buf1[S3 ? G * RP + 64 : 1];
     // the LDS tables are filled without branches (entries past the end are
     // copies, unused)
@@ -504,8 +510,6 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     // results do not change) and stores the products unpadded in buf0;
     // stage 0 then reads all its inputs from LDS before the barrier that
     // precedes its first output store, so buf0 holds both.
-    constexpr int NT0 = (G * (L / R0) + 31) / 32, MT0 = (NT0 + NTH / 64 - 1) / (NTH / 64);
-    constexpr bool STG = LM == LD_PROD && NF && MT0 <= 4 && L % 4 == 0;
     if constexpr (STG) {
         const long long xo = (long long)((b / a.prod_xdiv) % a.prod_xmod) * a.in_bstride;
         const long long co = (long long)(b / a.prod_cdiv) * a.cf_bstride;
@@ -527,13 +531,12 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             const int q0 = (int)threadIdx.x + u * NTH;
             if (q0 < Q) {
                 const int g = q0 / (L / 4), n = (q0 - g * (L / 4)) * 4;
-                const uint32_t xw[4] = {xv[u].x, xv[u].y, xv[u].z, xv[u].w};
-                const uint32_t cw[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int nn = n + v;
-                    buf0[g * RP + nn] = cmul_w<DT>(xw[v], cw[v]);
-                }
+                uint4 pw;
+                pw.x = cmul_w<DT>(xv[u].x, cv[u].x);
+                pw.y = cmul_w<DT>(xv[u].y, cv[u].y);
+                pw.z = cmul_w<DT>(xv[u].z, cv[u].z);
+                pw.w = cmul_w<DT>(xv[u].w, cv[u].w);
+                *reinterpret_cast<uint4 *>(&buf0[g * RP + n]) = pw;
             }
         }
         fill();

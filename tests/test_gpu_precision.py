@@ -64,7 +64,10 @@ def test_precision_form(sdr, orc, multi_weak, form, monkeypatch):
         M1, M2, vc, vr = ph.tiles(i)
         assert M1 * M2 == s["M"] and vc in (0, 1) and vr in (0, 1)
         if form.endswith("fixed"):
-            assert (M1, vc, vr) == FIXED[s["M"]], (s["name"], ph.tiles(i))
+            # (M = 24000 runs the fused cell kernel in FP16: one split, no
+            # workgroup shapes to choose)
+            want_t = (150, 0, 0) if s["M"] == 24000 else FIXED[s["M"]]
+            assert (M1, vc, vr) == want_t, (s["name"], ph.tiles(i))
         c32, ix32, d32 = p32.result(i)
         ch, ixh, dh = ph.result(i)
         det32 = c32 >= CN0_THRES
