@@ -72,7 +72,7 @@ HROW = {(160, 0): (16, 10, 1, 32, 256), (160, 1): (16, 10, 1, 32, 512),
 
 
 # fused cell inverse kernels of the FP16 form by split (M1, M2)
-CELL = {(150, 160): "cell_kernel<160, 16, 10, 150, 10, 15, 32, 32, 512>"}
+CELL = {(150, 160): "cell_kernel<160, 16, 10, 150, 10, 15, 50, 80, 1024>"}
 
 
 def kernel_instance(cls, M, form="fp32", tiles=None):

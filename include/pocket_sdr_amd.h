@@ -250,18 +250,29 @@ int sdr_trk_plan_result(const sdr_trk_plan_t *plan, sdr_cpx_t *corr,
  * src/sdr_ch.c:736-740 ((C[0] + previous epoch's C[1]) / N, zero before the
  * first epoch). */
 int sdr_trk_plan_prompt(const sdr_trk_plan_t *plan, sdr_cpx_t *P);
-/* Closed-loop tracking (src/sdr_ch.c:488-624, 694-763: FLL for the first
- * second, then PLL, DLL, C/N0 and carrier-lock windows, the loss decision)
- * for signals without secondary code, bump-jump or the L6 branch; each
- * channel starts as start_track leaves it (src/sdr_ch.c:378-394) from
- * chs[c].fd0, coff0, phi0 (drift is not used) and loops[c].  Taps 0-3 must
- * be P, E, L and the noise tap.  Every run restarts from these states;
- * atan and log10 are the GPU's (the results follow the reference to their
- * last bits). */
+/* Closed-loop tracking (src/sdr_ch.c:453-624, 674-763: FLL for the first
+ * second, then PLL, DLL, C/N0 and carrier-lock windows, the loss decision,
+ * the secondary-code synchronisation and wipe-off from 1.5 s, bump-jump for
+ * BOC signals; not the L6 branch); each channel starts as start_track
+ * leaves it (src/sdr_ch.c:378-394) from chs[c].fd0, coff0, phi0 (drift is
+ * not used) and loops[c].  Taps 0-3 must be P, E, L and the noise tap, and
+ * with bump-jump taps 4 and 5 VE and VL (trk->pos, src/sdr_ch.c:228-243).
+ * The prompt history (SDR_N_HIST entries, shifted with the code-offset
+ * wraps as adj_coff does) lives on the GPU.  With a secondary code the
+ * taps of sdr_trk_plan_result and the prompts of sdr_trk_plan_prompt are
+ * the wiped ones trk->C and trk->P hold.  Every run restarts from these
+ * states; atan and log10 are the GPU's (the results follow the reference to
+ * their last bits). */
 typedef struct {
     int len_code;              /* code length (chips), the DLL's scale */
     int costas;                /* 1: Costas discriminators (all but L6D / L6E) */
     double cn0;                /* C/N0 at acquisition (dB-Hz) */
+    int len_sec_code;          /* secondary code length (0 or 1: none; at most
+                                  SDR_N_HIST) */
+    const int8_t *sec_code;    /* ch->sec_code (+-1), host memory */
+    double bump_step;          /* bump_jump's code step (s): T / len_code, / 3
+                                  for E5ABQ, / 2 for G1OCP / G2OCP
+                                  (src/sdr_ch.c:552-562); 0: no bump-jump */
 } sdr_trk_loop_t;
 sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch,
     const sdr_trk_ch_t *chs, const sdr_trk_loop_t *loops, int n_pos,

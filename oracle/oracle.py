@@ -60,21 +60,26 @@ _d("orc_gen_code_L1CA", C.c_int, [C.c_int, vp])
 _d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, C.c_int, vp, vp, dp])
 _d("orc_trk_stitch", None, [vp, C.c_int, C.c_int, vp])
 _d("orc_trkl_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, C.c_int, C.c_double, vp, vp, C.c_int,
-                          C.c_int, dp, C.c_int, dp, vp, vp])
+                          C.c_int, dp, C.c_int, dp, vp, vp, C.c_int, vp, C.c_double])
 class orc_tl_t(C.Structure):
     _fields_ = [("T", C.c_double), ("len_code", C.c_int), ("costas", C.c_int),
                 ("fd", C.c_double), ("coff", C.c_double), ("cn0", C.c_double), ("pli", C.c_double),
                 ("err_phas", C.c_double), ("phas_acc", C.c_double), ("code_int", C.c_double),
                 ("sumE", C.c_double), ("sumL", C.c_double), ("sumP", C.c_double), ("sumN", C.c_double),
                 ("sumD", C.c_double), ("C0", C.c_float * 2),
-                ("lock", C.c_int), ("lost_cnt", C.c_int), ("pli_valid", C.c_int), ("idle", C.c_int)]
+                ("lock", C.c_int), ("lost_cnt", C.c_int), ("pli_valid", C.c_int), ("idle", C.c_int),
+                ("len_sec", C.c_int), ("sec_code", vp), ("sec_sync", C.c_int), ("sec_pol", C.c_int),
+                ("bump_step", C.c_double), ("sumVE", C.c_double), ("sumVL", C.c_double),
+                ("hist", fp)]
 
 
 _d("orc_tl_fll", None, [C.POINTER(orc_tl_t), fp])
 _d("orc_tl_pll", None, [C.POINTER(orc_tl_t), fp])
 _d("orc_tl_dll", None, [C.POINTER(orc_tl_t), fp])
 _d("orc_tl_cn0", C.c_int, [C.POINTER(orc_tl_t), fp, C.c_float, C.c_float])
-_d("orc_tl_update", None, [C.POINTER(orc_tl_t), fp, C.c_float, C.c_float])
+_d("orc_tl_update", C.c_int, [C.POINTER(orc_tl_t), fp, C.c_float, C.c_float])
+_d("orc_tl_push", None, [C.POINTER(orc_tl_t), C.c_float, C.c_float])
+_d("orc_tl_sec", C.c_int, [C.POINTER(orc_tl_t), fp, C.c_int])
 _d("orc_csk", C.c_int, [vp, C.c_int, C.c_int, dp, C.c_int, vp])
 _d("orc_trk_l6_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, vp, vp, C.c_int, C.c_int, dp, C.c_int,
                             vp, vp, vp, dp])
@@ -237,10 +242,13 @@ def trk_run(ch, code_bank_arr, buff, n_epoch, pos):
     return corr, Cc, state
 
 
-def trkl_run(ch, len_code, costas, cn0, code_bank_arr, buff, n_epoch, pos):
+def trkl_run(ch, len_code, costas, cn0, code_bank_arr, buff, n_epoch, pos, sec_code=None, bump_step=0.0):
     """Closed-loop tracking of one channel (src/sdr_ch.c:488-624, 694-763):
     state (n_epoch, 6) = fd, coff, phi, cn0, lock, idle; corr (n_epoch,
-    n_pos); P (n_epoch,)."""
+    n_pos); P (n_epoch,).  sec_code: the secondary code (int8 +-1) to
+    synchronise and wipe off; bump_step > 0: taps 4 and 5 are VE and VL and
+    bump-jump moves the code by that step (s)."""
+    sec = None if sec_code is None else np.ascontiguousarray(sec_code, np.int8)
     c = orc_trk_ch_t(ch["fs"], ch["fi"], ch["fc_sig"], ch["T"], ch["N"], ch["fd0"],
                      ch.get("drift", 0.0), ch["phi0"], ch["coff0"])
     code_bank_arr = np.ascontiguousarray(code_bank_arr, np.int8)
@@ -250,7 +258,8 @@ def trkl_run(ch, len_code, costas, cn0, code_bank_arr, buff, n_epoch, pos):
     corr = np.zeros((n_epoch, len(pos)), np.complex64)
     P = np.zeros(n_epoch, np.complex64)
     lib.orc_trkl_run(C.byref(c), len_code, costas, cn0, _p(code_bank_arr), _p(buff), len(buff), n_epoch,
-                     pos.ctypes.data_as(dp), len(pos), state.ctypes.data_as(dp), _p(corr), _p(P))
+                     pos.ctypes.data_as(dp), len(pos), state.ctypes.data_as(dp), _p(corr), _p(P),
+                     0 if sec is None else len(sec), None if sec is None else _p(sec), float(bump_step))
     return state, corr, P
 
 
@@ -359,16 +368,43 @@ def tl_state(T=1e-3, len_code=1023, costas=0, **kw):
 def tl_step(step, st, corr, Px=0.0, Py=0.0):
     """One loop step of the oracle on state st: step "fll", "pll", "dll",
     "cn0" or "update"; corr = the epoch's correlators as (re, im) pairs
-    P, E, L, N (8 floats, missing ones zero).  Returns orc_tl_cn0's window
-    flag for "cn0", else None."""
-    c = np.zeros(8, np.float32)
+    P, E, L, N, VE, VL (12 floats, missing ones zero).  Returns orc_tl_cn0's
+    window flag for "cn0", orc_tl_update's secondary-code multiplier for
+    "update", else None."""
+    c = np.zeros(12, np.float32)
     c[:len(corr)] = corr
     ref = C.byref(st)
     cp = c.ctypes.data_as(fp)
     if step == "cn0":
         return lib.orc_tl_cn0(ref, cp, Px, Py)
     if step == "update":
-        lib.orc_tl_update(ref, cp, Px, Py)
-    else:
-        getattr(lib, "orc_tl_" + step)(ref, cp)
+        return lib.orc_tl_update(ref, cp, Px, Py)
+    getattr(lib, "orc_tl_" + step)(ref, cp)
     return None
+
+
+def tl_hist(st):
+    """Attach a zeroed prompt history (SDR_N_HIST entries) to state st;
+    returns the (n, 2) float32 array behind it."""
+    h = np.zeros((5000, 2), np.float32)
+    st._hist = h
+    st.hist = h.ctypes.data_as(fp)
+    return h
+
+
+def tl_sec_code(st, sec_code):
+    """Attach a secondary code (int8 +-1) to state st."""
+    code = np.ascontiguousarray(sec_code, np.int8)
+    st._sec = code
+    st.len_sec = len(code)
+    st.sec_code = code.ctypes.data_as(vp)
+
+
+def tl_sec(st, sec_code, taps):
+    """orc_tl_sec on state st (its history attached) with the secondary code
+    and the epoch's taps ((re, im) pairs, wiped in place); returns the
+    multiplier and the wiped taps."""
+    tl_sec_code(st, sec_code)
+    c = np.ascontiguousarray(taps, np.float32).copy()
+    m = lib.orc_tl_sec(C.byref(st), c.ctypes.data_as(fp), len(c) // 2)
+    return m, c

@@ -656,9 +656,21 @@ int orc_tl_cn0(orc_tl_t *s, const float *C, float Px, float Py)
     s->sumP += fmaf(Px, Px, Py * Py);
     s->sumD += fmaf(Px, Px, -(Py * Py));
     s->sumN += fmaf(C[6], C[6], C[7] * C[7]);
+    if (s->bump_step > 0.0) {                             /* npos >= 6: VE, VL */
+        s->sumVE += fmaf(C[8], C[8], C[9] * C[9]);
+        s->sumVL += fmaf(C[10], C[10], C[11] * C[11]);
+    }
     if (s->lock % (int)(0.5 / s->T) != 0) return 0;       /* T_CN0 */
     if (s->sumP > 0.0) { s->pli = s->sumD / s->sumP; s->pli_valid = 1; }
     if (s->sumN > 0.0) s->cn0 = fma(fma(log10(s->sumP / s->sumN / s->T), 10.0, -s->cn0), 0.5, s->cn0);
+    if (s->bump_step > 0.0) {
+        /* bump-jump (src/sdr_ch.c:552-572): a very-late (very-early) power
+         * above BUMP_K = 1.3 times the prompt's, itself above 1.3 times the
+         * other side's, moves the code one step later (earlier) */
+        if (s->sumVL > 1.3 * s->sumP && s->sumP > 1.3 * s->sumVE) s->coff += s->bump_step;
+        else if (s->sumVE > 1.3 * s->sumP && s->sumP > 1.3 * s->sumVL) s->coff -= s->bump_step;
+        s->sumVE = s->sumVL = 0.0;
+    }
     s->sumP = s->sumN = s->sumD = 0.0;
     int bad = s->cn0 < 30.0 || (s->costas && s->pli_valid && s->pli < 0.25);
     s->lost_cnt = bad ? s->lost_cnt + 1 : 0;
@@ -666,26 +678,94 @@ int orc_tl_cn0(orc_tl_t *s, const float *C, float Px, float Py)
     return 1;
 }
 
-/* one epoch's loop update after the correlators: lock counted, FLL during
- * the pull-in (T_FPULLIN = 1 s) else PLL, DLL, C/N0 */
-void orc_tl_update(orc_tl_t *s, const float *C, float Px, float Py)
+/* the prompt history's sdr_add_buff (src/sdr_ch.c:740): drop the oldest,
+ * append (Px, Py) */
+void orc_tl_push(orc_tl_t *s, float Px, float Py)
 {
+    memmove(s->hist, s->hist + 2, sizeof(float) * 2 * (ORC_N_HIST - 1));
+    s->hist[2 * (ORC_N_HIST - 1)] = Px;
+    s->hist[2 * (ORC_N_HIST - 1) + 1] = Py;
+}
+
+/* sync_sec_code (src/sdr_ch.c:453-486) over the last N = len_sec prompts:
+ * unsynchronised, the in-phase prompts correlated with the secondary code,
+ * P = sum x_i c_i / N and R = sum |x_i| / N in float, declare sync at this
+ * lock count with the sign of P when |P| >= THRES_SEC_RATIO (0.8) R and
+ * R >= THRES_SYNC (0.02); synchronised, every N epochs after the sync the
+ * plain mean of the last N prompts below THRES_LOST (0.002) drops it.
+ * While synchronised the epoch's taps (C, n_tap (re, im) pairs) and the
+ * newest prompt are multiplied by the code chip of this epoch times the
+ * polarity.  Returns that multiplier (1 when not synchronised).  A lock
+ * count that fell more than N below the sync point (the reference's index
+ * then goes negative) wraps into the code instead. */
+int orc_tl_sec(orc_tl_t *s, float *C, int n_tap)
+{
+    const int N = s->len_sec;
+    const float *P = s->hist + 2 * (ORC_N_HIST - N);
+    if (s->sec_sync == 0) {
+        float Pc = 0.0f, R = 0.0f;
+        for (int i = 0; i < N; i++) {
+            const int j = i % s->len_sec;
+            Pc += P[2 * i] * (float)s->sec_code[j] / (float)N;
+            R += fabsf(P[2 * i]) / (float)N;
+        }
+        if ((double)fabsf(Pc) >= 0.8 * (double)R && (double)R >= 0.02) {
+            s->sec_sync = s->lock;
+            s->sec_pol = Pc > 0.0f ? 1 : -1;
+        }
+    } else if ((s->lock - s->sec_sync) % N == 0) {
+        float Pm = 0.0f;
+        for (int i = 0; i < N; i++) Pm += P[2 * i] / (float)N;
+        if ((double)fabsf(Pm) < 0.002) s->sec_sync = s->sec_pol = 0;
+    }
+    if (s->sec_sync <= 0) return 1;
+    int idx = (s->lock - s->sec_sync - 1 + N) % N % s->len_sec;
+    if (idx < 0) idx += s->len_sec;
+    const int m = s->sec_code[idx] * s->sec_pol;
+    for (int i = 0; i < 2 * n_tap; i++) C[i] *= (float)m;
+    s->hist[2 * (ORC_N_HIST - 1)] *= (float)m;
+    s->hist[2 * (ORC_N_HIST - 1) + 1] *= (float)m;
+    return m;
+}
+
+/* one epoch's loop update after the correlators (src/sdr_ch.c:736-763): the
+ * stitched prompt (Px, Py) appended to the history (when kept), lock
+ * counted, the secondary code synchronised and wiped off (from T_NPULLIN =
+ * 1.5 s), FLL during the pull-in (T_FPULLIN = 1 s) else PLL, DLL, C/N0 with
+ * bump-jump and the loss decision.  C: the loop's taps P, E, L, N (and VE,
+ * VL with bump-jump) as (re, im) pairs, wiped in place.  Returns the
+ * secondary-code multiplier applied to this epoch (1: none). */
+int orc_tl_update(orc_tl_t *s, float *C, float Px, float Py)
+{
+    int m = 1;
+    if (s->hist) orc_tl_push(s, Px, Py);
     s->lock++;
+    if (s->len_sec >= 2 && s->hist && s->lock * s->T >= 1.5) {   /* T_NPULLIN */
+        m = orc_tl_sec(s, C, s->bump_step > 0.0 ? 6 : 4);
+        Px = s->hist[2 * (ORC_N_HIST - 1)];
+        Py = s->hist[2 * (ORC_N_HIST - 1) + 1];
+    }
     if (s->lock * s->T <= 1.0) orc_tl_fll(s, C);           /* T_FPULLIN */
     else orc_tl_pll(s, C);
     orc_tl_dll(s, C);
     orc_tl_cn0(s, C, Px, Py);
+    return m;
 }
 
 void orc_trkl_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0,
                   const int8_t *code_bank, const uint8_t *buff, int buffN, int n_epoch,
-                  const double *pos, int n_pos, double *state, float *corr, float *P)
+                  const double *pos, int n_pos, double *state, float *corr, float *P,
+                  int len_sec, const int8_t *sec_code, double bump_step)
 {
     const int N = ch->N;
     orc_tl_t s;
     memset(&s, 0, sizeof(s));
     s.T = ch->T; s.len_code = len_code; s.costas = costas;
     s.fd = ch->fd0; s.coff = ch->coff0; s.cn0 = cn0_0; s.pli = 1.0;
+    s.len_sec = len_sec; s.sec_code = sec_code; s.bump_step = bump_step;
+    s.hist = (float *)calloc(2 * ORC_N_HIST, sizeof(float));   /* trk->P = 0, src/sdr_ch.c:369 */
+    const int nlt = bump_step > 0.0 ? 6 : 4;                   /* the loop's taps */
+    float Cl[12];
     double time = 0.0, phi = ch->phi0;
     float C1p[2] = {0.0f, 0.0f};
     int8_t *IQ = (int8_t *)malloc(2 * (size_t)N);
@@ -707,18 +787,27 @@ void orc_trkl_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0
         phi = fma(fc, tau, phi);
         phi -= floor(phi);
         time = t;
-        if (s.coff >= ch->T) { s.coff -= ch->T; s.lock--; }
-        else if (s.coff < 0.0) { s.coff += ch->T; s.lock++; }
+        /* adj_coff (src/sdr_ch.c:674-689), with its shift of the history */
+        if (s.coff >= ch->T) {
+            s.coff -= ch->T; s.lock--;
+            memmove(s.hist + 2, s.hist, sizeof(float) * 2 * (ORC_N_HIST - 1));
+        } else if (s.coff < 0.0) {
+            s.coff += ch->T; s.lock++;
+            memmove(s.hist, s.hist + 2, sizeof(float) * 2 * (ORC_N_HIST - 1));
+        }
         int ix = (int)(((long long)e * N) % buffN);
         orc_mix_carr(buff, buffN, ix, N, ch->fs, fc, phi, IQ);
         orc_corr_std(IQ, code_bank, N, s.coff * ch->fs, pos, n_pos, C, C1);
-        memcpy(corr + 2 * (size_t)e * n_pos, C, sizeof(float) * 2 * n_pos);
         float Px = (C1[0] + C1p[0]) / N, Py = (C1[1] + C1p[1]) / N;
         C1p[0] = C1[2]; C1p[1] = C1[3];
-        orc_tl_update(&s, C, Px, Py);
+        memcpy(Cl, C, sizeof(float) * 2 * nlt);
+        const float m = (float)orc_tl_update(&s, Cl, Px, Py);
+        /* the taps as trk->C holds them: the secondary code wiped off */
+        for (int i = 0; i < 2 * n_pos; i++) corr[2 * (size_t)e * n_pos + i] = C[i] * m;
         st[0] = s.fd; st[1] = s.coff; st[2] = phi; st[3] = s.cn0; st[4] = s.lock; st[5] = 0;
-        pe[0] = Px; pe[1] = Py;
+        pe[0] = s.hist[2 * (ORC_N_HIST - 1)]; pe[1] = s.hist[2 * (ORC_N_HIST - 1) + 1];
     }
+    free(s.hist);
     free(C);
     free(IQ);
 }

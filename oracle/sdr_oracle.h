@@ -113,15 +113,26 @@ typedef struct {
     double sumE, sumL, sumP, sumN, sumD;
     float C0[2];
     int lock, lost_cnt, pli_valid, idle;
+    /* secondary code (sync_sec_code), bump-jump and the prompt history */
+    int len_sec;                /* secondary code length (< 2: none) */
+    const int8_t *sec_code;
+    int sec_sync, sec_pol;
+    double bump_step;           /* bump-jump code step (s); 0: no VE / VL taps */
+    double sumVE, sumVL;
+    float *hist;                /* ORC_N_HIST (re, im) prompts, newest last, or NULL */
 } orc_tl_t;
+#define ORC_N_HIST 5000         /* SDR_N_HIST, src/pocket_sdr.h:59 */
 void orc_tl_fll(orc_tl_t *s, const float *C);
 void orc_tl_pll(orc_tl_t *s, const float *C);
 void orc_tl_dll(orc_tl_t *s, const float *C);
 int orc_tl_cn0(orc_tl_t *s, const float *C, float Px, float Py);
-void orc_tl_update(orc_tl_t *s, const float *C, float Px, float Py);
+void orc_tl_push(orc_tl_t *s, float Px, float Py);
+int orc_tl_sec(orc_tl_t *s, float *C, int n_tap);
+int orc_tl_update(orc_tl_t *s, float *C, float Px, float Py);
 void orc_trkl_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0,
                   const int8_t *code_bank, const uint8_t *buff, int buffN, int n_epoch,
-                  const double *pos, int n_pos, double *state, float *corr, float *P);
+                  const double *pos, int n_pos, double *state, float *corr, float *P,
+                  int len_sec, const int8_t *sec_code, double bump_step);
 
 /* Prompt history of the standard correlator branch, src/sdr_ch.c:736-740:
  * P[e] = (C[e][0] + C[e-1][1]) / N, C[-1][1] = 0 (cpx arrays). */

@@ -45,6 +45,22 @@ def gen_code_L1CA(prn):
     return (_G1 * _G2[idx]).astype(np.int8)
 
 
+def _nh(bits):
+    return np.array([1 if b == "0" else -1 for b in bits], np.int8)
+
+
+def sec_code_NH10():
+    """10-bit Neuman-Hoffman code 0000110101 (IS-GPS-705 L5-I), bit 0 -> +1,
+    1 -> -1 (src/sdr_code.c:693-695)."""
+    return _nh("0000110101")
+
+
+def sec_code_NH20():
+    """20-bit Neuman-Hoffman code 00000100110101001110 (IS-GPS-705 L5-Q), bit
+    0 -> +1, 1 -> -1 (src/sdr_code.c:697-699, 1114-1118)."""
+    return _nh("00000100110101001110")
+
+
 def res_index(len_code, T, coff, fs, N):
     """chip index of each sample: (int)((coff fs + i) dx) % len, dx = len / T / fs"""
     dx = len_code / T / fs

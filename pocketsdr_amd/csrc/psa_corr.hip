@@ -301,6 +301,7 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
         q.ix = ix;
         trk_mix_params(buffN, ix, N, ch.fs, fc, ph, q.p0, q.p1, q.nseg, q.s);
         q.idle = 0;
+        q.neg = 0;
         q.cs = coff * ch.fs;
         ep[(long long)c * n_epoch + e] = q;
     });
@@ -675,6 +676,13 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
     const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn, psum};
     trk_correlate<EXACT, SAMEQ>(q, buf, N, n_pos, pos, codes + (long long)ch * code_ch_stride,
                                 code_copy_stride, NP, L, corr_out + o * n_pos, C_out + 2 * o);
+    if (q.neg) {                             // a closed loop wiped its secondary code: trk->C
+        __syncthreads();
+        for (int t = threadIdx.x; t < n_pos; t += TRK_THREADS) {
+            const float2 v = corr_out[o * n_pos + t];
+            corr_out[o * n_pos + t] = make_float2(-v.x, -v.y);
+        }
+    }
 }
 
 // ===========================================================================
@@ -711,15 +719,42 @@ constexpr int trk_loop_threads() { return EXACT || !SAMEQ ? 512 : 1024; }
 #define TL_FILT_CN0 0.5       // FILT_CN0
 #define TL_DPI (2.0 * 3.1415926535897932)
 
+#define TL_N_HIST 5000          // SDR_N_HIST (src/pocket_sdr.h:59)
+#define TL_T_NPULLIN 1.5        // T_NPULLIN
+#define TL_THRES_SYNC 0.02      // THRES_SYNC
+#define TL_THRES_LOST 0.002     // THRES_LOST
+#define TL_THRES_SEC_RATIO 0.8  // THRES_SEC_RATIO
+#define TL_BUMP_K 1.3           // BUMP_K
+
 // one channel's tracking state (the sdr_ch_t / sdr_trk_t fields the loop uses)
 struct TrkLoopCh {
     double fs, fi, fc_sig, T;
     double time, fd, coff, phi, cn0, pli;
     double err_phas, phas_acc, code_int, err_code;
     double sumE, sumL, sumP, sumN, sumD;   // DLL E / L, C/N0 windows
+    double bump_step, sumVE, sumVL;        // bump-jump step (0: none) and VE / VL sums
     float C0x, C0y, C1x, C1y;              // FLL previous prompt, stitch carry
-    int lock, lost_cnt, pli_valid, idle, len_code, costas, ep, pad;
+    int lock, lost_cnt, pli_valid, idle, len_code, costas, ep;
+    int len_sec, sec_off, sec_sync, sec_pol;   // secondary code (length < 2: none)
+    int hb;                                // prompt history: logical P[0] at hist[hb]
 };
+
+// The prompt history trk->P (SDR_N_HIST entries, newest last) as a ring in
+// LDS: logical entry i at hist[(hb + i) mod TL_N_HIST].  sdr_add_buff drops
+// the oldest and appends; adj_coff's memmoves shift the whole history by one
+// entry either way (src/sdr_ch.c:674-689), each an index step plus the one
+// entry the memmove leaves in place.
+__device__ __forceinline__ float2 &tl_hist(float2 *hist, const TrkLoopCh &S, int i)
+{
+    int k = S.hb + i;
+    if (k >= TL_N_HIST) k -= TL_N_HIST;
+    return hist[k];
+}
+__device__ __forceinline__ void tl_hist_push(float2 *hist, TrkLoopCh &S, float2 v)
+{
+    S.hb = S.hb + 1 == TL_N_HIST ? 0 : S.hb + 1;
+    tl_hist(hist, S, TL_N_HIST - 1) = v;
+}
 
 // per (channel, epoch): state after the epoch (idle: the channel was lost at
 // or before it and the epoch was not processed)
@@ -730,7 +765,7 @@ struct TrkLoopOut {
 };
 
 // lane 0: open-loop part of track_sig (src/sdr_ch.c:694-703, adj_coff 674-689)
-__device__ __forceinline__ void tl_advance(TrkLoopCh &S, int N, int buffN, TrkEpoch &q)
+__device__ __forceinline__ void tl_advance(TrkLoopCh &S, int N, int buffN, TrkEpoch &q, float2 *hist)
 {
 #pragma clang fp contract(off)
     const double t = (double)(S.ep + 1) * S.T;
@@ -740,26 +775,92 @@ __device__ __forceinline__ void tl_advance(TrkLoopCh &S, int N, int buffN, TrkEp
     S.phi = fma(fc, tau, S.phi);
     S.phi = S.phi - floor(S.phi);
     S.time = t;
-    if (S.coff >= S.T) { S.coff = S.coff - S.T; S.lock--; }
-    else if (S.coff < 0.0) { S.coff = S.coff + S.T; S.lock++; }
+    if (S.coff >= S.T) {
+        S.coff = S.coff - S.T; S.lock--;
+        if (S.len_sec >= 2) {
+            // memmove(P + 1, P): every entry one later, P[0] kept
+            const float2 p0 = tl_hist(hist, S, 0);
+            S.hb = S.hb == 0 ? TL_N_HIST - 1 : S.hb - 1;
+            tl_hist(hist, S, 0) = p0;
+        }
+    } else if (S.coff < 0.0) {
+        S.coff = S.coff + S.T; S.lock++;
+        if (S.len_sec >= 2) {
+            // memmove(P, P + 1): every entry one earlier, P[N_HIST - 1] kept
+            const float2 pl = tl_hist(hist, S, TL_N_HIST - 1);
+            S.hb = S.hb + 1 == TL_N_HIST ? 0 : S.hb + 1;
+            tl_hist(hist, S, TL_N_HIST - 1) = pl;
+        }
+    }
     q.ix = (int)(((long long)S.ep * N) % buffN);
     trk_mix_params(buffN, q.ix, N, S.fs, fc, S.phi, q.p0, q.p1, q.nseg, q.s);
     q.idle = 0;
+    q.neg = 0;
     q.cs = S.coff * S.fs;
 }
 
-// lane 0: the loop after the correlation (src/sdr_ch.c:488-620, 736-763);
-// tap 0 = P, 1 = E, 2 = L, 3 = the noise tap; C = tap 0's two halves
-__device__ __forceinline__ void tl_update(TrkLoopCh &S, int N, const float2 *tap, const float2 *C,
-                                          TrkLoopOut &o)
+// lane 0: sync_sec_code (src/sdr_ch.c:453-486) over the last N = len_sec
+// prompts: unsynchronised, P = sum x_i c_i / N and R = sum |x_i| / N in
+// float (x the in-phase prompts, c the code) declare sync at this lock count
+// with the sign of P when |P| >= 0.8 R and R >= 0.02; synchronised, every N
+// epochs the plain mean below 0.002 drops it.  Returns the multiplier of
+// this epoch's taps and newest prompt (1 when not synchronised).  A lock
+// count more than N below the sync point (the reference's index then goes
+// negative) wraps into the code.
+__device__ __forceinline__ int tl_sec(TrkLoopCh &S, float2 *hist, const int8_t *__restrict__ sec)
+{
+#pragma clang fp contract(off)
+    const int N = S.len_sec;
+    const float fN = (float)N;
+    if (S.sec_sync == 0) {
+        float P = 0.0f, R = 0.0f;
+        for (int i = 0; i < N; i++) {
+            const float x = tl_hist(hist, S, TL_N_HIST - N + i).x;
+            P = __fadd_rn(P, __fdiv_rn(__fmul_rn(x, (float)sec[i % N]), fN));
+            R = __fadd_rn(R, __fdiv_rn(fabsf(x), fN));
+        }
+        if ((double)fabsf(P) >= TL_THRES_SEC_RATIO * (double)R && (double)R >= TL_THRES_SYNC) {
+            S.sec_sync = S.lock;
+            S.sec_pol = P > 0.0f ? 1 : -1;
+        }
+    } else if ((S.lock - S.sec_sync) % N == 0) {
+        float P = 0.0f;
+        for (int i = 0; i < N; i++) P = __fadd_rn(P, __fdiv_rn(tl_hist(hist, S, TL_N_HIST - N + i).x, fN));
+        if ((double)fabsf(P) < TL_THRES_LOST) S.sec_sync = S.sec_pol = 0;
+    }
+    if (S.sec_sync <= 0) return 1;
+    int idx = (S.lock - S.sec_sync - 1 + N) % N % N;
+    if (idx < 0) idx += N;
+    return sec[idx] * S.sec_pol;
+}
+
+// lane 0: the loop after the correlation (src/sdr_ch.c:453-620, 736-763);
+// tap 0 = P, 1 = E, 2 = L, 3 = the noise tap (4, 5 = VE, VL with bump-jump),
+// wiped in place; C = tap 0's two halves; q.neg records a negated epoch
+__device__ __forceinline__ void tl_update(TrkLoopCh &S, int N, float2 *tap, const float2 *C, TrkLoopOut &o,
+                                          float2 *hist, const int8_t *__restrict__ sec, TrkEpoch &q)
 {
 #pragma clang fp contract(off)
     // prompt history (src/sdr_ch.c:736-740)
     const float fN = (float)N;
-    const float Px = __fdiv_rn(__fadd_rn(C[0].x, S.C1x), fN), Py = __fdiv_rn(__fadd_rn(C[0].y, S.C1y), fN);
+    float Px = __fdiv_rn(__fadd_rn(C[0].x, S.C1x), fN), Py = __fdiv_rn(__fadd_rn(C[0].y, S.C1y), fN);
     S.C1x = C[1].x;
     S.C1y = C[1].y;
+    if (S.len_sec >= 2) tl_hist_push(hist, S, make_float2(Px, Py));
     S.lock++;
+    // secondary code (src/sdr_ch.c:743-746)
+    if (S.len_sec >= 2 && S.lock * S.T >= TL_T_NPULLIN) {
+        const int m = tl_sec(S, hist, sec);
+        if (m < 0) {
+            const int nt = S.bump_step > 0.0 ? 6 : TL_NTAP;
+            for (int i = 0; i < nt; i++) tap[i] = make_float2(-tap[i].x, -tap[i].y);
+            float2 &pn = tl_hist(hist, S, TL_N_HIST - 1);
+            pn = make_float2(-pn.x, -pn.y);
+            q.neg = 1;
+        }
+        Px = tl_hist(hist, S, TL_N_HIST - 1).x;
+        Py = tl_hist(hist, S, TL_N_HIST - 1).y;
+    }
     if (S.lock * S.T <= TL_T_FPULLIN) {
         // FLL (src/sdr_ch.c:488-506)
         if (S.lock >= 2) {
@@ -806,6 +907,10 @@ __device__ __forceinline__ void tl_update(TrkLoopCh &S, int N, const float2 *tap
     S.sumP = S.sumP + (double)__fmaf_rn(Px, Px, __fmul_rn(Py, Py));
     S.sumD = S.sumD + (double)__fmaf_rn(Px, Px, -__fmul_rn(Py, Py));
     S.sumN = S.sumN + (double)__fmaf_rn(tap[3].x, tap[3].x, __fmul_rn(tap[3].y, tap[3].y));
+    if (S.bump_step > 0.0) {                   // npos >= 6: VE, VL (src/sdr_ch.c:586-589)
+        S.sumVE = S.sumVE + (double)__fmaf_rn(tap[4].x, tap[4].x, __fmul_rn(tap[4].y, tap[4].y));
+        S.sumVL = S.sumVL + (double)__fmaf_rn(tap[5].x, tap[5].x, __fmul_rn(tap[5].y, tap[5].y));
+    }
     if (S.lock % (int)(TL_T_CN0 / S.T) == 0) {
         if (S.sumP > 0.0) {
             S.pli = S.sumD / S.sumP;
@@ -814,6 +919,12 @@ __device__ __forceinline__ void tl_update(TrkLoopCh &S, int N, const float2 *tap
         if (S.sumN > 0.0) {
             const double l = log10(S.sumP / S.sumN / S.T);
             S.cn0 = fma(fma(l, 10.0, -S.cn0), TL_FILT_CN0, S.cn0);
+        }
+        if (S.bump_step > 0.0) {
+            // bump_jump (src/sdr_ch.c:552-572)
+            if (S.sumVL > TL_BUMP_K * S.sumP && S.sumP > TL_BUMP_K * S.sumVE) S.coff = S.coff + S.bump_step;
+            else if (S.sumVE > TL_BUMP_K * S.sumP && S.sumP > TL_BUMP_K * S.sumVL) S.coff = S.coff - S.bump_step;
+            S.sumVE = S.sumVL = 0.0;
         }
         S.sumP = S.sumN = S.sumD = 0.0;
         // loss of lock (src/sdr_ch.c:608-624)
@@ -835,7 +946,8 @@ __global__ void __launch_bounds__(NTH) trk_loop_kernel(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
     const double *__restrict__ pos, const uint32_t *__restrict__ codes, long long code_ch_stride,
     long long code_copy_stride, int NP, const uint32_t *__restrict__ carr_g,
-    TrkLoopCh *__restrict__ chs, TrkLoopOut *__restrict__ out, TrkEpoch *__restrict__ ep)
+    TrkLoopCh *__restrict__ chs, TrkLoopOut *__restrict__ out, TrkEpoch *__restrict__ ep,
+    const int8_t *__restrict__ sec_all)
 {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t carr[256];
@@ -857,19 +969,29 @@ __global__ void __launch_bounds__(NTH) trk_loop_kernel(
     __syncthreads();
     const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn, psum};
     const uint32_t *cbase = codes + (long long)c * code_ch_stride;
+    // the prompt history after the IF planes (channels with a secondary
+    // code; trk->P starts at zero, src/sdr_ch.c:369)
+    float2 *hist = reinterpret_cast<float2 *>(lds + 2 * Nq4);
+    const bool use_hist = S.len_sec >= 2;
+    if (use_hist) {
+        for (int i = threadIdx.x; i < TL_N_HIST; i += NTH) hist[i] = make_float2(0.0f, 0.0f);
+    }
+    const int8_t *sec = sec_all + (use_hist ? S.sec_off : 0);
+    const int nlt = S.bump_step > 0.0 ? 6 : TL_NTAP;       // the loop's taps
+    __syncthreads();
     int e = 0;
     for (; e < n_epoch; e++) {
         if (stop) break;                       // written by lane 0 before a barrier
-        if (threadIdx.x == 0) tl_advance(S, N, buffN, qs);
+        if (threadIdx.x == 0) tl_advance(S, N, buffN, qs, hist);
         __syncthreads();
         const TrkEpoch q = qs;
-        trk_correlate<EXACT, SAMEQ, 1, NTH>(q, buf, N, TL_NTAP, pos, cbase, code_copy_stride, NP, L,
-                                                         tap, Ch);
+        trk_correlate<EXACT, SAMEQ, 1, NTH>(q, buf, N, nlt, pos, cbase, code_copy_stride, NP, L, tap, Ch);
         const long long o = (long long)c * n_epoch + e;
         if (threadIdx.x == 0) {
-            ep[o] = q;
+            TrkEpoch qe = q;
             TrkLoopOut r;
-            tl_update(S, N, tap, Ch, r);
+            tl_update(S, N, tap, Ch, r, hist, sec, qe);
+            ep[o] = qe;
             out[o] = r;
             stop = S.idle;
         }
@@ -926,6 +1048,7 @@ struct sdr_trk_plan_tag {
     std::vector<TrkLoopCh> lch0;
     TrkLoopCh *d_lch = nullptr;
     TrkLoopOut *d_lout = nullptr;
+    int8_t *d_sec = nullptr;                   // secondary codes of the loop channels
     hipStream_t st = nullptr;
     bool timing = false;
     hipEvent_t ev[4];
@@ -1247,14 +1370,32 @@ sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch, const sd
         fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: need loop parameters and the P, E, L, N taps\n");
         return nullptr;
     }
+    size_t n_sec = 0;
     for (int c = 0; c < n_ch; c++) {
         if (loops[c].len_code <= 0 || !(chs[c].T > 0.0) || (int)(TL_T_CN0 / chs[c].T) <= 0) {
             fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: need len_code > 0 and 0 < T <= 0.5 s\n");
             return nullptr;
         }
+        if (loops[c].len_sec_code >= 2 && (!loops[c].sec_code || loops[c].len_sec_code > TL_N_HIST)) {
+            fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: a secondary code needs its chips "
+                            "(at most %d)\n", TL_N_HIST);
+            return nullptr;
+        }
+        if (loops[c].bump_step > 0.0 && n_pos < 6) {
+            fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: bump-jump needs the VE and VL taps (4, 5)\n");
+            return nullptr;
+        }
+        if (loops[c].len_sec_code >= 2) n_sec += (size_t)loops[c].len_sec_code;
     }
     sdr_trk_plan_t *p = sdr_trk_plan_new(buff, n_ch, chs, n_pos, pos, n_epoch);
     if (!p) return nullptr;
+    if (n_sec && (size_t)(((p->N + 3) / 4 + 255) / 256 * 256) * 8 + 4096 + sizeof(float2) * TL_N_HIST >
+                     150 * 1024) {
+        fprintf(stderr, "pocketsdr_amd: sdr_trk_plan_new_loop: N=%d too large for a secondary-code loop "
+                        "(the IF planes and the prompt history share one workgroup's LDS)\n", p->N);
+        sdr_trk_plan_free(p);
+        return nullptr;
+    }
     p->loop = true;
     p->lch0.assign(n_ch, TrkLoopCh{});
     for (int c = 0; c < n_ch; c++) {
@@ -1264,6 +1405,21 @@ sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch, const sd
         S.time = 0.0; S.fd = chs[c].fd0; S.coff = chs[c].coff0; S.phi = chs[c].phi0;
         S.cn0 = loops[c].cn0; S.pli = 1.0;
         S.len_code = loops[c].len_code; S.costas = loops[c].costas ? 1 : 0;
+        S.bump_step = loops[c].bump_step > 0.0 ? loops[c].bump_step : 0.0;
+    }
+    if (n_sec) {
+        // the channels' secondary codes, one after the other
+        std::vector<int8_t> h(n_sec);
+        size_t off = 0;
+        for (int c = 0; c < n_ch; c++) {
+            if (loops[c].len_sec_code < 2) continue;
+            p->lch0[c].len_sec = loops[c].len_sec_code;
+            p->lch0[c].sec_off = (int)off;
+            memcpy(h.data() + off, loops[c].sec_code, (size_t)loops[c].len_sec_code);
+            off += (size_t)loops[c].len_sec_code;
+        }
+        PSA_CHECK(hipMalloc(&p->d_sec, n_sec));
+        PSA_CHECK(hipMemcpy(p->d_sec, h.data(), n_sec, hipMemcpyHostToDevice));
     }
     PSA_CHECK(hipMalloc(&p->d_lch, sizeof(TrkLoopCh) * n_ch));
     PSA_CHECK(hipMalloc(&p->d_lout, sizeof(TrkLoopOut) * (size_t)n_ch * n_epoch));
@@ -1298,7 +1454,7 @@ static void trk_loop_launch_t(sdr_trk_plan_t *p, size_t shm)
     hipLaunchKernelGGL((trk_loop_kernel<EXACT, SAMEQ, NTH>), dim3(p->n_ch), dim3(NTH), shm, p->st,
                        p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_pos, p->d_codes,
                        p->code_ch_stride, p->code_copy_stride, p->NP, dev_carrier(), p->d_lch, p->d_lout,
-                       p->d_ep);
+                       p->d_ep, p->d_sec);
 }
 
 static void trk_loop_launch(sdr_trk_plan_t *p, size_t shm)
@@ -1330,7 +1486,8 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
                                  st));
         if (p->timing) PSA_CHECK(hipEventRecord(p->ev[0], st));
         const size_t shm = sizeof(uint32_t) * 2 * (size_t)(((p->N + 3) / 4 + 255) / 256 * 256);
-        trk_loop_launch(p, shm);
+        // (the loop kernel keeps the prompt histories after the planes)
+        trk_loop_launch(p, shm + (p->d_sec ? sizeof(float2) * TL_N_HIST : 0));
         PSA_CHECK(hipGetLastError());
         trk_launch(p, shm, (unsigned)ne);          // all taps of every epoch
         PSA_CHECK(hipGetLastError());
@@ -1438,7 +1595,7 @@ void sdr_trk_plan_free(sdr_trk_plan_t *p)
     (void)hipFree(p->d_buf); (void)hipFree(p->d_codes); (void)hipFree(p->d_ch);
     (void)hipFree(p->d_pos); (void)hipFree(p->d_ep);
     (void)hipFree(p->d_corr); (void)hipFree(p->d_C); (void)hipFree(p->d_P);
-    (void)hipFree(p->d_lch); (void)hipFree(p->d_lout);
+    (void)hipFree(p->d_lch); (void)hipFree(p->d_lout); (void)hipFree(p->d_sec);
     for (auto &e : p->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(p->st);
     delete p;

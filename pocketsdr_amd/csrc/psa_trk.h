@@ -17,6 +17,7 @@ struct TrkEpoch {
     int ix, nseg;
     uint32_t p0, p1, s;
     int idle;           // 1: no correlation (the channel has lost lock)
+    int neg;            // 1: the taps are negated (a wiped secondary-code chip)
     double cs;
 };
 

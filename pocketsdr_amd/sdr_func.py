@@ -45,7 +45,8 @@ class sdr_trk_ch_t(C.Structure):
 
 
 class sdr_trk_loop_t(C.Structure):
-    _fields_ = [("len_code", C.c_int), ("costas", C.c_int), ("cn0", C.c_double)]
+    _fields_ = [("len_code", C.c_int), ("costas", C.c_int), ("cn0", C.c_double),
+                ("len_sec_code", C.c_int), ("sec_code", c_void_p), ("bump_step", C.c_double)]
 
 
 class sdr_l6_ch_t(C.Structure):
@@ -417,7 +418,13 @@ class TrkPlan:
         if loops is not None:
             la = (sdr_trk_loop_t * len(chans))()
             for i, lp in enumerate(loops):
-                la[i] = sdr_trk_loop_t(lp["len_code"], 1 if lp.get("costas", 1) else 0, lp["cn0"])
+                sec = lp.get("sec_code")
+                if sec is not None:
+                    sec = np.ascontiguousarray(sec, np.int8)
+                    self._keep.append(sec)
+                la[i] = sdr_trk_loop_t(lp["len_code"], 1 if lp.get("costas", 1) else 0, lp["cn0"],
+                                       0 if sec is None else len(sec), None if sec is None else _ptr(sec),
+                                       float(lp.get("bump_step", 0.0)))
             self.p = lib.sdr_trk_plan_new_loop(C.byref(self._buf), len(chans), ca, la, self.n_pos,
                                                _dp(self.pos), n_epoch)
         else:

@@ -28,7 +28,8 @@ def pack_cpx8(I, Q):
 def synth_if(fs, n_samp, sats, seed, bits=2, fi=0.0, chunk=1 << 22):
     """sats: list of dicts with code (int8 chips), T (code period s), tau (s),
     fd (Hz), drift (Hz/s, optional), cn0 (dB-Hz), fc (carrier Hz), phi (cyc),
-    shift (optional: code delay in entries per code period, cycled).
+    shift (optional: code delay in entries per code period, cycled), sec
+    (optional: secondary code, one chip per code period, cycled).
     Returns the uint8 cpx8 buffer."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = np.empty(n_samp, np.uint8)
@@ -51,6 +52,9 @@ def synth_if(fs, n_samp, sats, seed, bits=2, fi=0.0, chunk=1 << 22):
                 sh = np.asarray(sat["shift"], np.int64)
                 ic = ic - sh[np.floor_divide(ic, L) % len(sh)]
             chips = code[ic % L]
+            if "sec" in sat:
+                sec = np.asarray(sat["sec"], np.float64)
+                chips = chips * sec[np.floor_divide(ic, L) % len(sec)]
             ph = 2 * np.pi * ((fi + sat["fd"]) * t + 0.5 * drift * t * t + sat.get("phi", 0.0))
             A = np.sqrt(2.0 * 10 ** (sat["cn0"] / 10.0) / fs)
             x += A * chips * np.exp(1j * ph)

@@ -10,6 +10,10 @@ trk_l1ca      open-loop tracking correlators on one IF stream: per channel
 trk_l6        QZSS L6D FFT tracking with CSK: 4 ms epochs, 20460-entry TDM
               code, each code period of a visible satellite delayed by
               255 - symbol chips (src/sdr_ch.c:636-664, 705-720).
+trk_loop_sig  closed-loop channels of GPS L1 C/A, GPS L5-Q (NH20 secondary
+              code) or Galileo E1-B (BOC(1,1), the VE / VL taps of
+              bump-jump) started off the satellites' Doppler and code phase
+              (src/sdr_ch.c:221-244, 453-486, 552-572).
 """
 import numpy as np
 
@@ -273,3 +277,61 @@ def read_if(path, IQ, n=None):
         I, Q = I[:n], Q[:n]
     return ((I & 0xF) | ((Q & 0xF) << 4)).astype(np.uint8)
 
+
+
+L5_FREQ = 1176.45e6
+
+# signal: code generator, code period (s), chips per period, carrier (Hz),
+# PRNs, secondary code, BOC (VE / VL taps and bump-jump)
+_LOOP_SIGS = {
+    "L1CA": (l1ca_code, 1e-3, 1023, L1_FREQ, range(1, 33), None, False),
+    "L5Q": (codes.gen_code_L5Q, 1e-3, 10230, L5_FREQ, range(1, 33), codes.sec_code_NH20, False),
+    "E1B": (codes.gen_code_E1B, 4e-3, 4092, L1_FREQ, range(1, 37), None, True),
+}
+
+
+def trk_loop_sig(sig="L5Q", seed=6, fs=12e6, n_ch=3, n_epoch=2200, n_vis=2, cn0=(45.0, 45.0),
+                 fd_err=20.0, coff_err=0.1, sp_corr=0.25, nposx=0, bump=None, pos_n=-120.0):
+    """Closed-loop case: dict(buff, chans, loops, pos, n_epoch, truth, sig).
+    The IF holds the first n_vis channels' satellites (with the signal's
+    secondary code, one chip per code period); their channels start fd_err
+    Hz and coff_err chips off.  Taps (src/sdr_ch.c:227-244): P, E and L at
+    -/+ sp_corr / 2 chip, the noise tap at pos_n samples, for a BOC signal
+    (bump: default on) VE and VL at -/+ half a chip, then nposx extra taps
+    over +/-2 chips.  The loops carry the code length (entries), Costas, a
+    starting C/N0 of 40 dB-Hz, the secondary code and the bump-jump step of
+    one code entry (src/sdr_ch.c:551-561)."""
+    gen, T, chips, fc, prn_range, sec_fn, boc = _LOOP_SIGS[sig]
+    bump = boc if bump is None else bump
+    rng = np.random.default_rng(seed)
+    N = int(round(fs * T))
+    prns = list(prn_range)[:n_ch]
+    cd = {p: np.asarray(gen(p), np.int8) for p in prns}
+    len_code = len(cd[prns[0]])
+    sec = None if sec_fn is None else sec_fn()
+    sats = synth.random_sats(rng, prns[:n_vis], cd, T, fc, cn0, (-4000.0, 4000.0))
+    for s_ in sats:
+        if sec is not None:
+            s_["sec"] = sec
+    truth = {s_["prn"]: s_ for s_ in sats}
+    buff = synth.synth_if(fs, N * n_epoch, sats, seed, bits=2)
+    chip = T / chips
+    chans, loops = [], []
+    for p in prns:
+        tr = truth.get(p)
+        if tr is not None:
+            fd0, coff0 = tr["fd"] + fd_err, tr["tau"] + coff_err * chip
+        else:
+            fd0, coff0 = float(rng.uniform(-4000, 4000)), float(rng.uniform(0, T))
+        chans.append(dict(fs=fs, fi=0.0, fc_sig=fc, T=T, N=N, fd0=fd0, drift=0.0, phi0=0.0,
+                          coff0=coff0, code=codes.code_bank(cd[p], T, fs, N), prn=p))
+        loops.append(dict(len_code=len_code, costas=1, cn0=40.0, sec_code=sec,
+                          bump_step=T / len_code if bump else 0.0))
+    sc = chip * fs                                      # samples per chip
+    pos = [0.0, -0.5 * sp_corr * sc, 0.5 * sp_corr * sc, pos_n]
+    if bump:
+        pos += [-0.5 * sc, 0.5 * sc]
+    for i in range(nposx):
+        pos.append(float(i - (nposx - 1) // 2) / nposx * 4.0 * chip * fs)
+    return dict(buff=buff, chans=chans, loops=loops, pos=np.array(pos, np.float64), n_epoch=n_epoch,
+                truth=truth, sig=sig)

@@ -2,7 +2,10 @@
 (src/sdr_ch.c:488-624: FLL, PLL, DLL, C/N0 with the loss decision), so that
 a wrong gain, band, filter constant, window or sign in the restatement fails
 here and not only the pull-in physics of tests/test_trk_loop.py.  The GPU
-loop is compared with this restatement epoch by epoch.
+loop is compared with this restatement epoch by epoch.  The secondary-code
+sync (sync_sec_code) and bump-jump have their own cases at the end: planted
+prompt histories of the NH20 code at known amplitude and phase, and VE / VL
+powers on either side of BUMP_K.
 
 Each case plants correlator values whose discriminator output is known in
 closed form (a prompt rotated by a known angle, early and late amplitudes in
@@ -201,3 +204,148 @@ def test_update_switches_fll_to_pll(orc):
     e = phase(c) / TWO_PI
     assert st.lock == 1001
     assert st.fd == pytest.approx(fd0 + 2.4 * W * e + 1.1 * W * W * e * T + W ** 3 * e * T * T, abs=1e-9)
+
+
+# --- secondary code (src/sdr_ch.c:453-486, 743-746) -----------------------
+# THRES_SEC_RATIO = 0.8, THRES_SYNC = 0.02, THRES_LOST = 0.002, T_NPULLIN =
+# 1.5 s; the sync compares the last len_sec in-phase prompts with the code
+# from its first chip, so it fires on the epoch that ends a code period.
+
+def nh20():
+    from pocketsdr_amd import codes
+    return codes.sec_code_NH20().astype(np.int64)
+
+
+def shift_in(h, v):
+    """the history's sdr_add_buff: drop the oldest, append v = (re, im)"""
+    h[:-1] = h[1:]
+    h[-1] = v
+
+
+@pytest.mark.parametrize("amp", [0.05, -0.05])
+def test_sec_sync_and_wipe(orc, amp):
+    # prompts A c_0 .. A c_19: sync at this lock count with the sign of A;
+    # the epoch's multiplier c_19 * pol, and every later epoch's the next chip
+    code = nh20()
+    st = orc.tl_state(T=T, lock=1600)
+    h = orc.tl_hist(st)
+    h[-20:, 0] = amp * code
+    h[-20:, 1] = 0.01
+    taps = [amp * code[-1], 0.01, 0.3, -0.2, 0.2, 0.1, 0.1, 0.0]
+    m, c = orc.tl_sec(st, code, taps)
+    pol = 1 if amp > 0 else -1
+    assert st.sec_sync == 1600 and st.sec_pol == pol and m == code[-1] * pol
+    assert np.array_equal(c, np.float32(taps) * m)
+    assert h[-1, 0] == np.float32(abs(amp)) and h[-1, 1] == np.float32(0.01 * m)
+    for k in range(45):                      # two periods and more
+        shift_in(h, (amp * code[k % 20], 0.0))
+        st.lock += 1
+        m, c = orc.tl_sec(st, code, [1.0, 2.0])
+        assert m == code[k % 20] * pol, k
+        assert h[-1, 0] == np.float32(abs(amp)), k         # wiped: the constant sign
+        assert list(c) == [m, 2.0 * m]
+
+
+def test_sec_sync_thresholds(orc):
+    code = nh20()
+
+    def trial(x):
+        st = orc.tl_state(T=T, lock=2000)
+        h = orc.tl_hist(st)
+        h[-20:, 0] = x
+        m, _ = orc.tl_sec(st, code, [1.0, 0.0])
+        return st.sec_sync, m
+
+    A = 0.05
+    flip1 = A * code.astype(np.float64)
+    flip1[3] *= -1                            # |P| = 0.9 R
+    assert trial(flip1)[0] == 2000
+    flip3 = flip1.copy()
+    flip3[[7, 11]] *= -1                      # |P| = 0.7 R < 0.8 R
+    assert trial(flip3) == (0, 1)
+    assert trial(0.021 * code)[0] == 2000     # R = 0.021 >= THRES_SYNC
+    assert trial(0.019 * code) == (0, 1)      # R = 0.019 < THRES_SYNC
+    # the code at any other phase: NH20's sidelobes stay below 0.8
+    for k in range(1, 20):
+        assert trial(A * np.roll(code, k)) == (0, 1), k
+    # reversed in time: no sync either (the chips are read oldest first)
+    assert trial(A * code[::-1])[0] == 0
+
+
+def test_sec_loss_check_every_period(orc):
+    code = nh20()
+
+    def trial(lock, x):
+        st = orc.tl_state(T=T, lock=lock, sec_sync=1000, sec_pol=1)
+        h = orc.tl_hist(st)
+        h[-20:, 0] = x
+        m, _ = orc.tl_sec(st, code, [1.0, 0.0])
+        return st.sec_sync, st.sec_pol, m
+
+    alt = 0.05 * np.tile([1.0, -1.0], 10)            # mean 0
+    assert trial(1020, alt) == (0, 0, 1)             # a period boundary: lost
+    assert trial(1021, alt) == (1000, 1, code[0])    # between boundaries: kept
+    bias = alt + 0.0025                              # mean 0.0025 >= THRES_LOST
+    assert trial(1040, bias) == (1000, 1, code[19])
+
+
+def test_sec_index_below_sync(orc):
+    # adj_coff can take the lock count below the sync point: the chip index
+    # (lock - sync - 1 + N) % N; more than N below, the reference's index
+    # would go negative, and the oracle wraps it into the code
+    code = nh20()
+    for lock, idx in ((95, 14), (81, 0), (70, 9)):
+        st = orc.tl_state(T=T, lock=lock, sec_sync=100, sec_pol=-1)
+        orc.tl_hist(st)
+        m, _ = orc.tl_sec(st, code, [1.0, 0.0])
+        assert m == -code[idx], lock
+
+
+@pytest.mark.parametrize("lock0,synced", [(1498, False), (1499, True)])
+def test_update_wipes_from_npullin(orc, lock0, synced):
+    # prompts -0.05 c_i with the newest pushed by this update: from lock
+    # 1500 (T_NPULLIN) the sync fires with polarity -1; the code's last chip
+    # is +1, so the multiplier is -1 and a plain PLL (atan2) sees the tap
+    # planted at pi + 0.1 as 0.1; one epoch earlier it sees pi + 0.1
+    code = nh20()
+    assert code[-1] == 1
+    st = orc.tl_state(T=T, lock=lock0, fd=10.0)
+    h = orc.tl_hist(st)
+    orc.tl_sec_code(st, code)
+    h[-19:, 0] = -0.05 * code[:19]
+    tap = [-math.cos(0.1), -math.sin(0.1)]
+    tap = [float(np.float32(v)) for v in tap]
+    m = orc.tl_step("update", st, tap + [0.0] * 6, -0.05, 0.0)
+    W = 5.0 / 0.7845
+    if synced:
+        assert m == -1 and st.sec_sync == 1500 and st.sec_pol == -1
+        assert h[-1, 0] == np.float32(0.05)
+        e = math.atan2(-tap[1], -tap[0]) / TWO_PI
+    else:
+        assert m == 1 and st.sec_sync == 0 and h[-1, 0] == np.float32(-0.05)
+        e = math.atan2(tap[1], tap[0]) / TWO_PI
+    assert st.fd == pytest.approx(10.0 + 2.4 * W * e + 1.1 * W * W * e * T + W ** 3 * e * T * T, abs=1e-9)
+
+
+# --- bump-jump (src/sdr_ch.c:551-572, 586-601): BUMP_K = 1.3 ----------------
+
+@pytest.mark.parametrize("ve,vl,move", [(1.2, 0.5, -1), (0.5, 1.2, 1), (1.1, 0.5, 0), (1.2, 0.9, 0),
+                                        (0.9, 1.2, 0)])
+def test_bump_jump_window(orc, ve, vl, move):
+    step = T / 8184
+    st = orc.tl_state(T=T, lock=499, coff=2e-4, bump_step=step, cn0=40.0)
+    corr = [0.0] * 6 + [0.1, 0.0, ve, 0.0, vl, 0.0]
+    orc.tl_step("cn0", st, corr, 1.0, 0.0)          # not a window end: sums only
+    assert st.coff == 2e-4
+    assert st.sumVE == pytest.approx(ve * ve, rel=1e-6) and st.sumVL == pytest.approx(vl * vl, rel=1e-6)
+    st.sumVE = st.sumVL = st.sumP = st.sumN = st.sumD = 0.0
+    st.lock = 500
+    assert orc.tl_step("cn0", st, corr, 1.0, 0.0) == 1
+    assert st.coff == 2e-4 + move * step
+    assert st.sumVE == 0.0 and st.sumVL == 0.0
+
+
+def test_no_bump_without_step(orc):
+    st = orc.tl_state(T=T, lock=500, coff=2e-4, cn0=40.0)
+    orc.tl_step("cn0", st, [0.0] * 6 + [0.1, 0.0, 1.5, 0.0, 0.1, 0.0], 1.0, 0.0)
+    assert st.coff == 2e-4 and st.sumVE == 0.0
