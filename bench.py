@@ -634,6 +634,26 @@ def bench_trk_loop(args, sdr, w):
                 realtime_x=round(ne * 1e-3 / t, 1), satellites_held=sum(held), satellites=len(held))
 
 
+def bench_trk_loop_sec(args, sdr, workloads):
+    """Closed loop with the secondary-code sync and wipe-off: GPS L5-Q (NH20)
+    on 32 channels x 2000 epochs at 12 Msps, 4 satellites in the IF started
+    20 Hz and 0.1 chip off, 12 taps; the sync starts at 1.5 s."""
+    w = workloads.trk_loop_sig("L5Q", seed=args.seed + 4, fs=12e6, n_ch=32, n_vis=4, n_epoch=2000,
+                               nposx=8)
+    plan = sdr.TrkPlan(w["buff"], w["chans"], w["pos"], w["n_epoch"], loops=w["loops"])
+    t = time_plan(plan, max(1, args.steps // 4), 1)
+    st = plan.state()
+    P = plan.prompt()
+    plan.close()
+    n_ch, N, ne = len(w["chans"]), w["chans"][0]["N"], w["n_epoch"]
+    vis = [c for c, ch in enumerate(w["chans"]) if ch["prn"] in w["truth"]]
+    synced = [bool((P[c, 1520:].real > 0).all() and st[c, -1, 5] == 0) for c in vis]
+    return dict(config=dict(signal="L5Q", n_ch=n_ch, fs=12e6, n_epoch=ne, n_pos=len(w["pos"]),
+                            sec_code="NH20"),
+                ms_per_run=round(t * 1e3, 3), msps_ch=round(n_ch * N * ne / t / 1e6, 1),
+                realtime_x=round(ne * 1e-3 / t, 1), satellites_synced=sum(synced), satellites=len(vis))
+
+
 ONCHIP_BYTES = 672 * 1024      # one CU: 160 KiB LDS + 512 KiB VGPR/AGPR
 
 
@@ -769,6 +789,7 @@ def main():
         if orc is not None:
             res["dropin_per_call"] = bench_dropin(args, sdr, orc, w_trk)
         res["closed_loop_tracking"] = bench_trk_loop(args, sdr, w_trk)
+        res["closed_loop_secondary_code"] = bench_trk_loop_sec(args, sdr, workloads)
     if not args.skip_l1cp:
         res["l1cp_full"] = bench_l1cp_full(args, sdr)
     if not args.skip_raw:
