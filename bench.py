@@ -606,6 +606,27 @@ def bench_l6(args, sdr, orc, workloads):
                alg_bytes_per_item=int(per_item), alg_bytes_spectra=n_ch * 10 * 8 * N,
                roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                              frac=round(gbs / HBM_PEAK_GBS, 4)))
+    # the same channels in the closed loop (sdr_l6_plan_new_loop): epochs in
+    # order, the channels of one epoch in one batch of passes
+    loops = [dict(len_code=w["len_code"], costas=0, cn0=40.0) for _ in w["chans"]]
+    lp = sdr.L6Plan(w["buff"], w["chans"], w["pos"], w["n_epoch"], loops=loops)
+    tl = time_plan(lp, max(1, args.steps // 4), 1)
+    syms, _ = lp.result()
+    st = lp.state()
+    lp.timing(True)
+    lp.run()
+    lks = {k: lp.kernel_ms(k) for k in L6_CLASSES + ("loop",)}
+    lp.close()
+    ok, seen = True, set()
+    for c, ch in enumerate(w["chans"]):
+        if ch["prn"] in w["truth"] and ch["prn"] not in seen:
+            seen.add(ch["prn"])
+            ok &= syms[c].tolist() == w["truth"][ch["prn"]]["syms"][:w["n_epoch"]].tolist()
+            ok &= not st[c, :, 5].any()
+    res["closed_loop"] = dict(ms_per_run=round(tl * 1e3, 3), epochs_ch_per_s=round(items / tl, 1),
+                              realtime_x=round(ne * 4e-3 / tl, 1), symbols_decoded=bool(ok and len(seen) > 0),
+                              launches_per_epoch=5,
+                              kernel_ms={k: round(v[0], 4) for k, v in lks.items() if v})
     if orc is not None:
         ch = w["chans"][0]
         n_e = ne

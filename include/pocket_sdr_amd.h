@@ -319,6 +319,21 @@ sdr_l6_plan_t *sdr_l6_plan_new(const sdr_buff_t *buff, int n_ch,
 sdr_l6_plan_t *sdr_l6_plan_new_dev(const uint8_t *dbuf, int buffN, int n_ch,
     const sdr_l6_ch_t *chs, int n_pos, const double *pos, int n_epoch,
     int flags);
+/* Closed-loop L6D / L6E tracking: the L6 branch of track_sig
+ * (src/sdr_ch.c:705-720) inside the loop of :741-763 (FLL / PLL / DLL, C/N0,
+ * loss), the CSK's P tap as the prompt.  Per channel the loop parameters
+ * of sdr_trk_plan_new_loop: {ch->len_code, ch->costas (0 for L6D / L6E,
+ * src/sdr_ch.c:326), cn0, no secondary code, no bump-jump}; pos starts
+ * with P, E, L, N (src/sdr_ch.c:228-231).  The epochs run in order, all
+ * channels of one epoch in one batch of FFT passes.  sdr_l6_plan_result
+ * and sdr_l6_plan_state then give every epoch's symbol, taps and loop
+ * state (zeros and idle after a loss of lock). */
+sdr_l6_plan_t *sdr_l6_plan_new_loop(const sdr_buff_t *buff, int n_ch,
+    const sdr_l6_ch_t *chs, const sdr_trk_loop_t *loops, int n_pos,
+    const double *pos, int n_epoch, int flags);
+/* state: n_ch x n_epoch x {fd, coff, phi, cn0, lock, idle} after each epoch
+ * (closed-loop plans) */
+int sdr_l6_plan_state(const sdr_l6_plan_t *plan, double *state);
 int sdr_l6_plan_run(sdr_l6_plan_t *plan);
 /* syms: n_ch x n_epoch CSK symbols; C: n_ch x n_epoch x n_pos taps (the
  * trk->C of src/sdr_ch.c:660-662). */
@@ -330,7 +345,8 @@ int sdr_l6_plan_corr(const sdr_l6_plan_t *plan, int ch, int epoch,
     sdr_cpx_t *corr);
 void sdr_l6_plan_timing(sdr_l6_plan_t *plan, int on);
 /* kernel classes: "state", "fwd_col", "row" (fused row pass), or "fwd_row" and
- * "inv_row" (row lengths without a fused kernel), "inv_col", "csk" */
+ * "inv_row" (row lengths without a fused kernel), "inv_col", "csk", "loop"
+ * (closed-loop plans) */
 int sdr_l6_plan_kernel_ms(const sdr_l6_plan_t *plan, const char *name,
     double *avg_ms, long long *count, double *elems);
 void sdr_l6_plan_free(sdr_l6_plan_t *plan);

@@ -59,6 +59,8 @@ _d("orc_gen_code_fft_f64", None, [vp, C.c_int, C.c_double, C.c_double, C.c_doubl
 _d("orc_gen_code_L1CA", C.c_int, [C.c_int, vp])
 _d("orc_trk_run", None, [C.POINTER(orc_trk_ch_t), vp, vp, C.c_int, C.c_int, dp, C.c_int, vp, vp, dp])
 _d("orc_trk_stitch", None, [vp, C.c_int, C.c_int, vp])
+_d("orc_l6l_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, C.c_int, C.c_double, vp, vp, C.c_int, C.c_int,
+                         dp, C.c_int, dp, vp, vp])
 _d("orc_trkl_run", None, [C.POINTER(orc_trk_ch_t), C.c_int, C.c_int, C.c_double, vp, vp, C.c_int,
                           C.c_int, dp, C.c_int, dp, vp, vp, C.c_int, vp, C.c_double])
 class orc_tl_t(C.Structure):
@@ -298,6 +300,23 @@ def trk_l6_run(ch, len_code, code_fft_banks, buff, n_epoch, pos, keep_corr=False
                        pos.ctypes.data_as(dp), len(pos), _p(syms), _p(taps),
                        _p(corr) if keep_corr else None, state.ctypes.data_as(dp))
     return syms, taps, corr, state
+
+def l6l_run(ch, len_code, cn0, code_fft_banks, buff, n_epoch, pos, costas=0):
+    """L6D / L6E closed-loop tracking of one channel (src/sdr_ch.c:694-720,
+    741-763): state (n_epoch, 6) = fd, coff, phi, cn0, lock, idle; syms
+    (n_epoch,); taps (n_epoch, n_pos)."""
+    c = orc_trk_ch_t(ch["fs"], ch["fi"], ch["fc_sig"], ch["T"], ch["N"], ch["fd0"],
+                     ch.get("drift", 0.0), ch["phi0"], ch["coff0"])
+    cf = np.ascontiguousarray(code_fft_banks, np.complex64)
+    buff = np.ascontiguousarray(buff, np.uint8)
+    pos = np.ascontiguousarray(pos, np.float64)
+    state = np.zeros((n_epoch, 6), np.float64)
+    syms = np.zeros(n_epoch, np.uint8)
+    taps = np.zeros((n_epoch, len(pos)), np.complex64)
+    lib.orc_l6l_run(C.byref(c), len_code, costas, cn0, _p(cf), _p(buff), len(buff), n_epoch,
+                    pos.ctypes.data_as(dp), len(pos), state.ctypes.data_as(dp), _p(syms), _p(taps))
+    return state, syms, taps
+
 
 # front-end raw data to cpx8 (src/sdr_rcv.c:776-936)
 FMT_INT8, FMT_INT8X2, FMT_RAW8, FMT_RAW16, FMT_RAW16I, FMT_RAW32, FMT_CS8, FMT_CS16 = range(1, 9)

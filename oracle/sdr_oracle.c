@@ -930,6 +930,65 @@ void orc_trk_l6_run(const orc_trk_ch_t *ch, int len_code, const float *code_fft,
     free(IQ);
 }
 
+/* L6D / L6E closed-loop tracking, src/sdr_ch.c:694-720 with the loop of
+ * :741-763: per epoch the NCO and code offset advance by the loop's
+ * Doppler (adj_coff moves the lock count), the epoch is mixed from the
+ * code-offset sample, FFT-correlated and CSK-decoded as in orc_trk_l6_run,
+ * the prompt history takes the CSK's P tap (C[0]), and the loop update
+ * (FLL / PLL / DLL / C/N0 / loss, orc_tl_update) reads the CSK's taps.
+ * L6D / L6E have no secondary code and are not BOC; the reference runs
+ * them with a plain PLL (ch->costas = 0, src/sdr_ch.c:326).  Per epoch:
+ * state[6] = {fd, coff, phi, cn0, lock, idle}, syms[e], C[n_pos] (cpx);
+ * idle epochs (after a loss) hold zeros. */
+void orc_l6l_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0, const float *code_fft,
+                 const uint8_t *buff, int buffN, int n_epoch, const double *pos, int n_pos,
+                 double *state, uint8_t *syms, float *C)
+{
+    const int N = ch->N;
+    orc_tl_t s;
+    memset(&s, 0, sizeof(s));
+    s.T = ch->T; s.len_code = len_code; s.costas = costas;
+    s.fd = ch->fd0; s.coff = ch->coff0; s.cn0 = cn0_0; s.pli = 1.0;
+    double time = 0.0, phi = ch->phi0;
+    int8_t *IQ = (int8_t *)malloc(2 * (size_t)N);
+    float *corr = (float *)malloc(sizeof(float) * 2 * (size_t)N);
+    float Cl[12];
+    for (int e = 0; e < n_epoch; e++) {
+        double *st = state + 6 * (size_t)e;
+        float *Ce = C + 2 * (size_t)e * n_pos;
+        if (s.idle) {
+            st[0] = s.fd; st[1] = s.coff; st[2] = phi; st[3] = s.cn0; st[4] = s.lock; st[5] = 1;
+            syms[e] = 0;
+            memset(Ce, 0, sizeof(float) * 2 * n_pos);
+            continue;
+        }
+        double t = (e + 1) * ch->T;
+        double tau = t - time;
+        double fc = ch->fi + s.fd;
+        s.coff = fma(-(s.fd / ch->fc_sig), tau, s.coff);
+        phi = fma(fc, tau, phi);
+        phi -= floor(phi);
+        time = t;
+        if (s.coff >= ch->T) { s.coff -= ch->T; s.lock--; }          /* adj_coff */
+        else if (s.coff < 0.0) { s.coff += ch->T; s.lock++; }
+        double cs = s.coff * ch->fs;
+        int i = (int)floor(cs);
+        int j = (int)((cs - i) * 10);
+        int ix = (int)(((long long)e * N) % buffN);
+        int start = circ((long long)ix + i, buffN);
+        orc_mix_carr(buff, buffN, start, N, ch->fs, fc, phi + fc * i / ch->fs, IQ);
+        orc_corr_fft(IQ, code_fft + 2 * (size_t)j * N, N, corr);
+        int x = orc_csk(corr, N, len_code, pos, n_pos, Ce);
+        syms[e] = (uint8_t)(255 - x % 256);
+        memset(Cl, 0, sizeof(Cl));
+        memcpy(Cl, Ce, sizeof(float) * 2 * (n_pos < 6 ? n_pos : 6));
+        orc_tl_update(&s, Cl, Ce[0], Ce[1]);
+        st[0] = s.fd; st[1] = s.coff; st[2] = phi; st[3] = s.cn0; st[4] = s.lock; st[5] = 0;
+    }
+    free(corr);
+    free(IQ);
+}
+
 /* ---------------------------------------------------------------------------
  * front-end raw data to cpx8, src/sdr_rcv.c:776-936 (write_buff and its LUT
  * generators); the LPF and array-combine steps after it are not included

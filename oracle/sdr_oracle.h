@@ -148,6 +148,13 @@ void orc_trk_l6_run(const orc_trk_ch_t *ch, int len_code, const float *code_fft,
                     const uint8_t *buff, int buffN, int n_epoch, const double *pos,
                     int n_pos, uint8_t *syms, float *C, float *corr_out, double *state);
 
+/* L6D / L6E closed-loop tracking (src/sdr_ch.c:694-720, 741-763): the FFT
+ * correlator and CSK inside the FLL / PLL / DLL / C/N0 loop.  Per epoch:
+ * state[6] = {fd, coff, phi, cn0, lock, idle}, syms[e], C[n_pos] (cpx). */
+void orc_l6l_run(const orc_trk_ch_t *ch, int len_code, int costas, double cn0_0, const float *code_fft,
+                 const uint8_t *buff, int buffN, int n_epoch, const double *pos, int n_pos,
+                 double *state, uint8_t *syms, float *C);
+
 /* Front-end raw data to cpx8 IF buffers, src/sdr_rcv.c:776-936.  fmt: the
  * SDR_FMT_* codes of src/pocket_sdr.h:85-92.  LUT generators: gen_LUTC
  * (256 cpx8), gen_LUTC_rtoc (4 x 256 cpx8, the fs/4 real-to-complex phases),

@@ -23,11 +23,19 @@
 // code spectrum chosen per item through PassArgs::cf_index; for row lengths
 // without a specialised kernel the row pass runs as the separate forward
 // and inverse passes.
+//
+// Closed loop (sdr_l6_plan_new_loop): the epochs run in order, each a batch
+// of the channels' items (item e n_ch + c): l6_loop_kernel advances every
+// channel's NCO and code offset by its loop's Doppler (psa_loop.h) and sets
+// the item's mixing parameters and spectrum index, the FFT passes and
+// csk_kernel run on the batch, and the next l6_loop_kernel launch feeds the
+// CSK's taps to the loop update (FLL / PLL / DLL / C/N0 / loss, the P tap as
+// the prompt, src/sdr_ch.c:717-763) before advancing the next epoch.
 #include <math.h>
 #include <string.h>
 #include <string>
 #include <algorithm>
-#include "psa_trk.h"
+#include "psa_loop.h"
 
 namespace psa {
 
@@ -71,6 +79,67 @@ __global__ void __launch_bounds__(TRK_SCHUNK) l6_state_kernel(const TrkChDev *__
         it.s[o] = s;
         it.cf[o] = c * PSA_N_CODES + j;
     });
+}
+
+// Closed loop, one lane per channel: the loop update of epoch e (e >= 0)
+// from the CSK's symbol and taps of the batch (item c of it), results to
+// the channel-major outputs, then the advance to epoch e + 1 and its mixing
+// parameters (src/sdr_ch.c:694-712).  A channel lost at or before epoch e
+// records idle epochs (zero symbol and taps) and keeps its last mixing
+// parameters for the batch's passes, whose outputs it ignores.
+__global__ void l6_loop_kernel(TrkLoopCh *__restrict__ chs, int n_ch, int e, int n_epoch, int N, int buffN,
+                               L6Items it, const uint8_t *__restrict__ sym_e, const float2 *__restrict__ taps_e,
+                               int n_pos, uint8_t *__restrict__ sym_out, float2 *__restrict__ taps_out,
+                               TrkLoopOut *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_ch) return;
+    TrkLoopCh S = chs[c];
+    if (e >= 0) {
+        const long long oc = (long long)c * n_epoch + e;
+        if (S.idle) {
+            TrkLoopOut r;
+            r.fd = S.fd; r.coff = S.coff; r.phi = S.phi; r.cn0 = S.cn0;
+            r.lock = S.lock; r.idle = 1; r.P = make_float2(0.0f, 0.0f);
+            out[oc] = r;
+            sym_out[oc] = 0;
+            for (int k = 0; k < n_pos; k++) taps_out[oc * n_pos + k] = make_float2(0.0f, 0.0f);
+        } else {
+            const float2 *tp = taps_e + (long long)c * n_pos;
+            float2 tap[6];
+            for (int k = 0; k < 6; k++) tap[k] = k < n_pos ? tp[k] : make_float2(0.0f, 0.0f);
+            TrkEpoch q = {};
+            TrkLoopOut r;
+            tl_update_p(S, tap, tap[0].x, tap[0].y, r, nullptr, nullptr, q);
+            out[oc] = r;
+            sym_out[oc] = sym_e[c];
+            for (int k = 0; k < n_pos; k++) taps_out[oc * n_pos + k] = tp[k];
+        }
+    }
+    if (e + 1 < n_epoch) {
+        if (!S.idle) tl_nco(S, nullptr);
+        const double fc = S.fi + S.fd;
+        const double cs = S.coff * S.fs;
+        const int i = (int)floor(cs);
+        int j = (int)((cs - i) * PSA_N_CODES);
+        j = min(max(j, 0), PSA_N_CODES - 1);
+        long long st = ((long long)(e + 1) * N) % buffN + i;
+        st %= buffN;
+        if (st < 0) st += buffN;
+        const double phi = S.phi + fc * i / S.fs;
+        uint32_t p0, p1, sv;
+        int nseg;
+        trk_mix_params(buffN, (int)st, N, S.fs, fc, phi, p0, p1, nseg, sv);
+        const long long o = (long long)(e + 1) * n_ch + c;
+        it.ix[o] = (int)st;
+        it.nseg[o] = nseg;
+        it.p0[o] = p0;
+        it.p1[o] = p1;
+        it.s[o] = sv;
+        it.cf[o] = c * PSA_N_CODES + j;
+    }
+    chs[c] = S;
 }
 
 // position of lag k (0 <= k < N, natural order) in a correlation stored
@@ -197,8 +266,9 @@ static void launch_csk(const float2 *corr, long long stride, int win, int n_item
     PSA_CHECK(hipGetLastError());
 }
 
-enum { L6_STATE = 0, L6_FWD_COL, L6_ROW, L6_FWD_ROW, L6_INV_ROW, L6_INV_COL, L6_CSK, L6_NCLS };
-static const char *l6_names[L6_NCLS] = {"state", "fwd_col", "row", "fwd_row", "inv_row", "inv_col", "csk"};
+enum { L6_STATE = 0, L6_FWD_COL, L6_ROW, L6_FWD_ROW, L6_INV_ROW, L6_INV_COL, L6_CSK, L6_LOOP, L6_NCLS };
+static const char *l6_names[L6_NCLS] = {"state", "fwd_col", "row", "fwd_row", "inv_row", "inv_col", "csk",
+                                        "loop"};
 
 }  // namespace psa
 
@@ -220,6 +290,14 @@ struct sdr_l6_plan_tag {
     float2 *A = nullptr, *B = nullptr, *corr = nullptr;   // corr: all items (KEEP_CORR)
     uint8_t *d_sym = nullptr;
     float2 *d_taps = nullptr;
+    // closed loop: initial channel states, their working copies, per-epoch
+    // results, and the CSK outputs of one epoch's batch
+    bool loop = false;
+    std::vector<TrkLoopCh> lch0;
+    TrkLoopCh *d_lch = nullptr;
+    TrkLoopOut *d_lout = nullptr;
+    uint8_t *d_sym_e = nullptr;
+    float2 *d_taps_e = nullptr;
     hipStream_t st = nullptr;
     bool timing = false;
     struct Ev { int cls; hipEvent_t a, b; };
@@ -371,21 +449,75 @@ sdr_l6_plan_t *sdr_l6_plan_new_dev(const uint8_t *dbuf, int buffN, int n_ch, con
     return l6_create(dbuf, buffN, true, n_ch, chs, n_pos, pos, n_epoch, flags);
 }
 
+sdr_l6_plan_t *sdr_l6_plan_new_loop(const sdr_buff_t *buff, int n_ch, const sdr_l6_ch_t *chs,
+    const sdr_trk_loop_t *loops, int n_pos, const double *pos, int n_epoch, int flags)
+{
+    if (!buff || !loops || !chs || n_ch <= 0 || n_pos < TL_NTAP) {
+        fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new_loop: need loop parameters and the P, E, L, N taps\n");
+        return nullptr;
+    }
+    for (int c = 0; c < n_ch; c++) {
+        if (loops[c].len_code <= 0 || !(chs[c].T > 0.0) || (int)(TL_T_CN0 / chs[c].T) <= 0) {
+            fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new_loop: need len_code > 0 and 0 < T <= 0.5 s\n");
+            return nullptr;
+        }
+        if (loops[c].len_sec_code >= 2 || (loops[c].bump_step > 0.0 && n_pos < 6)) {
+            fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new_loop: L6 channels take no secondary code; "
+                            "bump-jump needs the VE and VL taps (4, 5)\n");
+            return nullptr;
+        }
+    }
+    sdr_l6_plan_t *p = l6_create(buff->data, buff->N, false, n_ch, chs, n_pos, pos, n_epoch, flags);
+    if (!p) return nullptr;
+    if (p->chunk < n_ch) {
+        fprintf(stderr, "pocketsdr_amd: sdr_l6_plan_new_loop: %d channels exceed one batch of %d\n", n_ch,
+                p->chunk);
+        sdr_l6_plan_free(p);
+        return nullptr;
+    }
+    p->loop = true;
+    p->lch0.assign(n_ch, TrkLoopCh{});
+    for (int c = 0; c < n_ch; c++) {
+        // start_track (src/sdr_ch.c:378-394) and trk_init (:364-377)
+        TrkLoopCh &S = p->lch0[c];
+        S.fs = chs[c].fs; S.fi = chs[c].fi; S.fc_sig = chs[c].fc_sig; S.T = chs[c].T;
+        S.time = 0.0; S.fd = chs[c].fd0; S.coff = chs[c].coff0; S.phi = chs[c].phi0;
+        S.cn0 = loops[c].cn0; S.pli = 1.0;
+        S.len_code = loops[c].len_code; S.costas = loops[c].costas ? 1 : 0;
+        S.bump_step = loops[c].bump_step > 0.0 ? loops[c].bump_step : 0.0;
+    }
+    const size_t ni = (size_t)p->n_item;
+    PSA_CHECK(hipMalloc(&p->d_lch, sizeof(TrkLoopCh) * n_ch));
+    PSA_CHECK(hipMalloc(&p->d_lout, sizeof(TrkLoopOut) * ni));
+    PSA_CHECK(hipMalloc(&p->d_sym_e, (size_t)n_ch));
+    PSA_CHECK(hipMalloc(&p->d_taps_e, sizeof(float2) * (size_t)n_ch * n_pos));
+    return p;
+}
+
+int sdr_l6_plan_state(const sdr_l6_plan_t *p, double *state)
+{
+    if (!p || !state || !p->loop) return 0;
+    const size_t ne = (size_t)p->n_item;
+    std::vector<TrkLoopOut> r(ne);
+    PSA_CHECK(hipMemcpy(r.data(), p->d_lout, sizeof(TrkLoopOut) * ne, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ne; i++) {
+        double *q = state + 6 * i;
+        q[0] = r[i].fd; q[1] = r[i].coff; q[2] = r[i].phi; q[3] = r[i].cn0;
+        q[4] = r[i].lock; q[5] = r[i].idle;
+    }
+    return 1;
+}
+
 }  // extern "C"
 
-static void l6_run(sdr_l6_plan_t *p)
+// the FFT passes and CSK of items i0 ... i0 + nb - 1; symbols and taps to
+// sym / taps (item i0 first)
+static void l6_batch(sdr_l6_plan_t *p, long long i0, int nb, uint8_t *sym, float2 *taps)
 {
     hipStream_t st = p->st;
     const FFTPlan *pl = p->pl;
     const int M = pl->M;
     {
-        L6Tick t(p, L6_STATE);
-        hipLaunchKernelGGL(l6_state_kernel, dim3(p->n_ch), dim3(TRK_SCHUNK), 0, st, p->d_ch, p->n_epoch, p->N,
-                           p->buffN, p->it);
-        PSA_CHECK(hipGetLastError());
-    }
-    for (long long i0 = 0; i0 < p->n_item; i0 += p->chunk) {
-        const int nb = (int)std::min<long long>(p->chunk, p->n_item - i0);
         PassArgs a = {};
         args_fwd_col(pl, a);
         a.buf = p->d_buf; a.buffN = p->buffN; a.mix_compact = 0;
@@ -441,9 +573,43 @@ static void l6_run(sdr_l6_plan_t *p)
         }
         {
             L6Tick t(p, L6_CSK);
-            launch_csk(corr, cstride, p->win, nb, p->N, p->len_code, p->d_pos, p->n_pos, p->d_sym + i0,
-                       p->d_taps + i0 * p->n_pos, st);
+            launch_csk(corr, cstride, p->win, nb, p->N, p->len_code, p->d_pos, p->n_pos, sym, taps, st);
         }
+    }
+}
+
+static void l6_loop_launch(sdr_l6_plan_t *p, int e)
+{
+    L6Tick t(p, L6_LOOP);
+    hipLaunchKernelGGL(l6_loop_kernel, dim3((p->n_ch + 63) / 64), dim3(64), 0, p->st, p->d_lch, p->n_ch, e,
+                       p->n_epoch, p->N, p->buffN, p->it, p->d_sym_e, p->d_taps_e, p->n_pos, p->d_sym,
+                       p->d_taps, p->d_lout);
+    PSA_CHECK(hipGetLastError());
+}
+
+static void l6_run(sdr_l6_plan_t *p)
+{
+    hipStream_t st = p->st;
+    if (p->loop) {
+        // every run starts from the channels' initial states
+        PSA_CHECK(hipMemcpyAsync(p->d_lch, p->lch0.data(), sizeof(TrkLoopCh) * p->n_ch, hipMemcpyHostToDevice,
+                                 st));
+        l6_loop_launch(p, -1);
+        for (int e = 0; e < p->n_epoch; e++) {
+            l6_batch(p, (long long)e * p->n_ch, p->n_ch, p->d_sym_e, p->d_taps_e);
+            l6_loop_launch(p, e);
+        }
+        return;
+    }
+    {
+        L6Tick t(p, L6_STATE);
+        hipLaunchKernelGGL(l6_state_kernel, dim3(p->n_ch), dim3(TRK_SCHUNK), 0, st, p->d_ch, p->n_epoch, p->N,
+                           p->buffN, p->it);
+        PSA_CHECK(hipGetLastError());
+    }
+    for (long long i0 = 0; i0 < p->n_item; i0 += p->chunk) {
+        const int nb = (int)std::min<long long>(p->chunk, p->n_item - i0);
+        l6_batch(p, i0, nb, p->d_sym + i0, p->d_taps + i0 * p->n_pos);
     }
 }
 
@@ -467,7 +633,7 @@ int sdr_l6_plan_run(sdr_l6_plan_t *p)
         for (int c = 0; c < L6_NCLS; c++) {
             if (!seen[c]) continue;
             p->cnt[c]++;
-            p->elems[c] += c == L6_STATE ? (double)p->n_item : (double)p->n_item * M;
+            p->elems[c] += c == L6_STATE || c == L6_LOOP ? (double)p->n_item : (double)p->n_item * M;
         }
     }
     return 1;
@@ -487,7 +653,8 @@ int sdr_l6_plan_result(const sdr_l6_plan_t *p, uint8_t *syms, sdr_cpx_t *C)
 int sdr_l6_plan_corr(const sdr_l6_plan_t *p, int ch, int epoch, sdr_cpx_t *corr)
 {
     if (!p || !p->corr || !corr || ch < 0 || ch >= p->n_ch || epoch < 0 || epoch >= p->n_epoch) return 0;
-    const size_t o = ((size_t)ch * p->n_epoch + epoch) * p->pl->M;
+    const size_t item = p->loop ? (size_t)epoch * p->n_ch + ch : (size_t)ch * p->n_epoch + epoch;
+    const size_t o = item * p->pl->M;
     PSA_CHECK(hipMemcpy(corr, p->corr + o, sizeof(float2) * p->N, hipMemcpyDeviceToHost));
     return 1;
 }
@@ -523,6 +690,7 @@ void sdr_l6_plan_free(sdr_l6_plan_t *p)
     (void)hipFree(p->it.p0); (void)hipFree(p->it.p1); (void)hipFree(p->it.s);
     (void)hipFree(p->A); (void)hipFree(p->B); (void)hipFree(p->corr);
     (void)hipFree(p->d_sym); (void)hipFree(p->d_taps);
+    (void)hipFree(p->d_lch); (void)hipFree(p->d_lout); (void)hipFree(p->d_sym_e); (void)hipFree(p->d_taps_e);
     for (auto &e : p->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     (void)hipStreamDestroy(p->st);
     delete p;

@@ -118,6 +118,9 @@ _decl("sdr_l6_plan_new", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sd
                                     C.c_int, c_double_p, C.c_int, C.c_int])
 _decl("sdr_l6_plan_new_dev", c_void_p, [c_void_p, C.c_int, C.c_int, C.POINTER(sdr_l6_ch_t),
                                         C.c_int, c_double_p, C.c_int, C.c_int])
+_decl("sdr_l6_plan_new_loop", c_void_p, [C.POINTER(sdr_buff_t), C.c_int, C.POINTER(sdr_l6_ch_t),
+                                         C.POINTER(sdr_trk_loop_t), C.c_int, c_double_p, C.c_int, C.c_int])
+_decl("sdr_l6_plan_state", C.c_int, [c_void_p, c_void_p])
 _decl("sdr_l6_plan_run", C.c_int, [c_void_p])
 _decl("sdr_l6_plan_result", C.c_int, [c_void_p, c_void_p, c_void_p])
 _decl("sdr_l6_plan_corr", C.c_int, [c_void_p, C.c_int, C.c_int, c_void_p])
@@ -500,7 +503,9 @@ class L6Plan:
     IF ring, or with dev_buff a (device pointer, length) pair read in place
     (sdr_l6_plan_new_dev)."""
 
-    def __init__(self, buff, chans, pos, n_epoch, keep_corr=False, dev_buff=False):
+    def __init__(self, buff, chans, pos, n_epoch, keep_corr=False, dev_buff=False, loops=None):
+        """loops: per channel dicts len_code, costas, cn0 (bump_step) for the
+        closed-loop plan (sdr_l6_plan_new_loop)."""
         self._buf = None if dev_buff else _buff(buff)
         self._keep = []
         ca = (sdr_l6_ch_t * len(chans))()
@@ -513,7 +518,22 @@ class L6Plan:
         self.n_ch, self.n_epoch, self.n_pos = len(chans), n_epoch, len(self.pos)
         self.N = chans[0]["N"]
         flags = SDR_L6_KEEP_CORR if keep_corr else 0
-        if dev_buff:
+        self.loop = loops is not None
+        if loops is not None:
+            if dev_buff:
+                raise ValueError("closed-loop L6 plans take a host IF ring")
+            la = (sdr_trk_loop_t * len(chans))()
+            for i, lp in enumerate(loops):
+                sec = lp.get("sec_code")
+                if sec is not None:
+                    sec = np.ascontiguousarray(sec, np.int8)
+                    self._keep.append(sec)
+                la[i] = sdr_trk_loop_t(lp["len_code"], 1 if lp.get("costas", 0) else 0, lp["cn0"],
+                                       0 if sec is None else len(sec), None if sec is None else _ptr(sec),
+                                       float(lp.get("bump_step", 0.0)))
+            self.p = lib.sdr_l6_plan_new_loop(C.byref(self._buf), len(chans), ca, la, self.n_pos,
+                                              _dp(self.pos), n_epoch, flags)
+        elif dev_buff:
             self.p = lib.sdr_l6_plan_new_dev(buff[0], buff[1], len(chans), ca, self.n_pos,
                                              _dp(self.pos), n_epoch, flags)
         else:
@@ -531,6 +551,13 @@ class L6Plan:
         taps = np.zeros((self.n_ch, self.n_epoch, self.n_pos), np.complex64)
         lib.sdr_l6_plan_result(self.p, _ptr(syms), _ptr(taps))
         return syms, taps
+
+    def state(self):
+        """Closed-loop plans: (n_ch, n_epoch, 6) fd, coff, phi, cn0, lock, idle."""
+        st = np.zeros((self.n_ch, self.n_epoch, 6), np.float64)
+        if not lib.sdr_l6_plan_state(self.p, _ptr(st)):
+            raise ValueError("sdr_l6_plan_state: not a closed-loop plan")
+        return st
 
     def corr(self, ch, epoch):
         out = np.zeros(self.N, np.complex64)

@@ -264,3 +264,104 @@ def test_l6_plan_full_size(sdr, orc):
     rs, rt, _, _ = orc.trk_l6_run(ch, w["len_code"], ch["code_fft"], w["buff"], w["n_epoch"], w["pos"])
     assert np.array_equal(syms[0], rs)
     assert rel(taps[0], rt) < 1e-4
+
+
+# ------------------------------------------------------------ closed loop
+CHIP_L6 = 4e-3 / 10230
+
+
+def l6_loop_case(orc, fs=6e6, n_ch=3, n_epoch=500, n_vis=2, seed=3, fd_err=15.0, coff_err=0.1):
+    """Channels of the visible satellites started fd_err Hz and coff_err
+    chips off; loops as start_track sets them for L6D (plain PLL)."""
+    w = l6_workload(orc, fs, n_ch, n_epoch, n_vis, seed=seed)
+    seen = set()
+    for ch in w["chans"]:
+        tr = w["truth"].get(ch["prn"])
+        if tr is not None and ch["prn"] not in seen:
+            seen.add(ch["prn"])
+            ch["fd0"] = tr["fd"] + fd_err
+            ch["coff0"] = tr["tau"] + coff_err * CHIP_L6
+    w["loops"] = [dict(len_code=LEN_L6, costas=0, cn0=40.0) for _ in w["chans"]]
+    return w
+
+
+def test_oracle_l6_loop_pull_in(orc):
+    """The L6 branch inside the loop (src/sdr_ch.c:705-720, 741-763): the
+    satellites' channels pull in from 15 Hz and 0.1 chip off and decode
+    every planted symbol; before the first frequency update (epochs 0-1)
+    the loop equals the open-loop restatement."""
+    w = l6_loop_case(orc)
+    for c, ch in enumerate(w["chans"][:2]):
+        tr = w["truth"][ch["prn"]]
+        st, syms, taps = orc.l6l_run(ch, LEN_L6, 40.0, ch["code_fft"], w["buff"], w["n_epoch"], w["pos"])
+        assert not st[:, 5].any() and st[-1, 4] == w["n_epoch"]
+        assert syms.tolist() == tr["syms"][:w["n_epoch"]].tolist()
+        assert abs(st[-1, 0] - tr["fd"]) < 1.0 and abs(st[249, 0] - tr["fd"]) < 1.0
+        coff = tr["tau"] - tr["fd"] * w["n_epoch"] * 4e-3 / ch["fc_sig"]
+        assert abs(st[-1, 1] - coff) < 0.1 * CHIP_L6
+        s0, t0, _, st0 = orc.trk_l6_run(dict(ch, drift=0.0), LEN_L6, ch["code_fft"], w["buff"], 2, w["pos"])
+        assert np.array_equal(syms[:2], s0) and np.array_equal(taps[:2].view(np.uint32), t0.view(np.uint32))
+        assert np.array_equal(st[:2, 2], st0[:, 0]) and np.array_equal(st[:2, 1], st0[:, 1])
+
+
+def test_l6_loop_rejects_bad_loop_parameters():
+    from pocketsdr_amd import sdr_func
+    from oracle import oracle as orc
+    w = l6_loop_case(orc, n_ch=1, n_epoch=2, n_vis=1)
+    bad = [(dict(len_code=LEN_L6, cn0=40.0, sec_code=np.ones(20, np.int8)), w["pos"]),
+           (dict(len_code=LEN_L6, cn0=40.0, bump_step=1e-7), w["pos"][:4]),
+           (dict(len_code=LEN_L6, cn0=40.0), w["pos"][:3])]
+    for lp, pos in bad:
+        with pytest.raises((ValueError, TypeError)):
+            sdr_func.L6Plan(w["buff"], w["chans"], pos, 2, loops=[lp])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fs,n_ch,n_epoch", [(6e6, 3, 500), (24e6, 10, 300)])
+def test_l6_loop_vs_oracle(sdr, orc, fs, n_ch, n_epoch):
+    w = l6_loop_case(orc, fs=fs, n_ch=n_ch, n_epoch=n_epoch, n_vis=2 if n_ch == 3 else 4)
+    plan = sdr.L6Plan(w["buff"], w["chans"], w["pos"], w["n_epoch"], loops=w["loops"])
+    plan.run()
+    st = plan.state()
+    syms, taps = plan.result()
+    plan.run()                                        # a rerun starts from the initial states
+    assert np.array_equal(plan.state(), st)
+    plan.close()
+    seen = set()
+    for c, ch in enumerate(w["chans"]):
+        rs, rsym, rtaps = orc.l6l_run(ch, LEN_L6, 40.0, ch["code_fft"], w["buff"], w["n_epoch"], w["pos"])
+        assert not np.any(syms[c, st[c, :, 5] == 1]) and not np.any(taps[c, st[c, :, 5] == 1]), c
+        if ch["prn"] not in w["truth"] or ch["prn"] in seen:
+            # a channel on noise: its CSK peak is the largest of 511 noise
+            # powers, which the last-bit differences of the correlations
+            # move, so only the epochs before the first frequency update
+            # (the FLL's at epoch 1) agree
+            assert np.array_equal(syms[c, :2], rsym[:2]), c
+            assert np.array_equal(st[c, :1, :3], rs[:1, :3]), c
+            continue
+        seen.add(ch["prn"])
+        # a satellite: the same lock counts, no loss, every planted symbol
+        assert not rs[:, 5].any() and not st[c, :, 5].any(), c
+        assert np.array_equal(st[c, :, 4], rs[:, 4]), c
+        assert np.array_equal(syms[c], rsym), c
+        assert syms[c].tolist() == w["truth"][ch["prn"]]["syms"][:w["n_epoch"]].tolist(), c
+        # the correlations differ from the oracle's PocketFFT in the last
+        # bits (FP32 FFTs of other orders), so the loops drift apart by
+        # rounding: Doppler (Hz), code offset (chip), C/N0 (dB-Hz)
+        assert np.max(np.abs(st[c, :, 0] - rs[:, 0])) < 1e-2, c
+        assert np.max(np.abs(st[c, :, 1] - rs[:, 1])) < 1e-4 * CHIP_L6, c
+        assert np.max(np.abs(st[c, :, 3] - rs[:, 3])) < 1e-2, c
+        # the taps follow: the loops' carrier phases differ by rounding, which
+        # rotates the taps (measured 5.8e-4 of the largest tap at 6 Msps)
+        d = np.abs(taps[c] - rtaps)
+        assert np.max(d) <= 2e-3 * np.max(np.abs(rtaps)), c
+        dp = np.abs(np.abs(taps[c]) - np.abs(rtaps))
+        assert np.max(dp) <= 1e-3 * np.max(np.abs(rtaps)), c
+    assert len(seen) == (2 if n_ch == 3 else 4)
+    # epochs 0-1 precede any loop update: the loop plan's symbols and taps
+    # equal the open-loop plan's (the same GPU passes) bit for bit
+    ol = sdr.L6Plan(w["buff"], w["chans"], w["pos"], 2)
+    ol.run()
+    s0, t0 = ol.result()
+    ol.close()
+    assert np.array_equal(syms[:, :2], s0) and np.array_equal(taps[:, :2].view(np.uint32), t0.view(np.uint32))
