@@ -107,9 +107,14 @@ __device__ __forceinline__ void bstore32(__amdgpu_buffer_rsrc_t r, int i, uint32
 {
     __builtin_amdgcn_raw_buffer_store_b32(v, r, boff(i, 4, ok), 0, 0);
 }
+// (the masked base passes through an empty asm statement: otherwise the
+// compiler folds + c into both arms of the select, one select and one add
+// per store instead of a shared base and immediate offsets)
 __device__ __forceinline__ void bstore32_at(__amdgpu_buffer_rsrc_t r, int base, int c, uint32_t v, bool ok)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)((uint32_t)boff(base, 4, ok) + (uint32_t)c * 4u), 0, 0);
+    int vb = boff(base, 4, ok);
+    asm("" : "+v"(vb));
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)((uint32_t)vb + (uint32_t)c * 4u), 0, 0);
 }
 
 template <int DT>
