@@ -446,7 +446,9 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     __shared__ uint32_t carr[LM == LD_MIX ? 512 : 1];
     __shared__ PeakPart red[NTH / 64];
     int b, grp;
-    if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp)) return;
+    // column passes of FP16 words: G words per row segment
+    constexpr int KX = (!NF && G < 32) ? 32 / G : 1;
+    if (!block_item<KX>(a, LM == LD_PROD, ngroups, &b, &grp)) return;
     const int g0 = grp * G;
     const int gcount = min(G, a.nsub - g0);
     // issue order: stage-0 table, the LDS tables' words, the first tile's
@@ -1009,14 +1011,14 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
     X(200, 10, 20, 1, 32, 512, 0) X(200, 10, 20, 1, 32, 256, 1) \
     X(300, 15, 20, 1, 32, 512, 0) X(300, 15, 20, 1, 32, 256, 1) \
     X(400, 16, 25, 1, 32, 512, 0) X(400, 16, 25, 1, 32, 256, 1) \
-    X(960, 32, 30, 1, 16, 512, 0) X(960, 32, 30, 1, 16, 256, 1) \
-    X(1000, 10, 10, 10, 8, 512, 0) X(1000, 10, 10, 10, 16, 512, 1)
+    X(960, 32, 30, 1, 32, 512, 0) X(960, 32, 30, 1, 16, 256, 1) \
+    X(1000, 10, 10, 10, 8, 512, 0) X(1000, 25, 40, 1, 16, 512, 1)
 #define PSA_HROW(X) \
     X(160, 16, 10, 1, 32, 256, 0) X(160, 16, 10, 1, 32, 512, 1) \
     X(240, 16, 15, 1, 32, 256, 0) X(240, 16, 15, 1, 32, 512, 1) \
     X(320, 16, 20, 1, 16, 256, 0) X(320, 16, 20, 1, 32, 512, 1) \
     X(480, 16, 30, 1, 16, 512, 0) X(480, 16, 30, 1, 16, 256, 1) \
-    X(1000, 10, 10, 10, 4, 256, 0) X(1000, 10, 10, 10, 8, 512, 1) \
+    X(1000, 10, 10, 10, 4, 256, 0) X(1000, 25, 40, 1, 8, 512, 1) \
     X(960, 32, 30, 1, 8, 256, 0) X(960, 32, 30, 1, 8, 512, 1)
 
 struct HSpec { int L, r[3], G, nth, var; };

@@ -260,8 +260,25 @@ struct Storer {
 // the 8 XCDs, so tile T = grp * nbin + bin keeps XCD T % 8 and its PRNs run
 // back to back there, re-reading the shared X rows from that XCD's L2.  The
 // grid is padded to whole rounds of 8; returns false for the padding.
+// KX > 1 (column passes whose G columns span less than a 128-byte line):
+// the KX workgroups of adjacent column groups, which read and write the two
+// (four) parts of the same lines, go to one XCD in consecutive rounds, so
+// that each line is fetched from HBM into one L2 only (workgroup h of a
+// whole block of 8 KX runs logical item (s / KX) 8 KX + (h % 8) KX + s % KX,
+// s = h / 8; the last partial block keeps the identity order).
+template <int KX = 1>
 __device__ __forceinline__ bool block_item(const PassArgs &a, bool prod, int ngroups, int *b, int *grp)
 {
+    if (KX > 1 && !prod) {
+        int h = (int)blockIdx.x;
+        if (h < (int)(gridDim.x / (8u * KX)) * 8 * KX) {
+            const int s = h >> 3;
+            h = (s / KX) * (8 * KX) + (h & 7) * KX + s % KX;
+        }
+        *b = h / ngroups;
+        *grp = h - *b * ngroups;
+        return true;
+    }
     if (prod && a.xcd_nbin > 0) {
         const int w = (int)blockIdx.x;
         const int r = w >> 3;

@@ -53,8 +53,20 @@ def test_l1cp_fp32_vs_oracle(sdr, orc, l1cp):
     plan.close()
 
 
-def test_l1cp_fp16_vs_fp64(sdr, orc, l1cp):
+# FP16 tilings of M = 960000 (M1 / column shape / row shape): the default
+# (columns of 960 in 32-column workgroups, rows of 1000 as 10 x 10 x 10), the
+# 16-column workgroups paired on one XCD, rows of 1000 as 25 x 40, and
+# columns of 1000 as 25 x 40 with rows of 960
+TILINGS = [None, "960/1/1", "960/0/1", "1000/1/0"]
+
+
+@pytest.mark.parametrize("tiles", TILINGS)
+def test_l1cp_fp16_vs_fp64(sdr, orc, l1cp, tiles, monkeypatch):
     w = l1cp
+    if tiles:
+        monkeypatch.setenv("PSA_TILES", "h960000=" + tiles)
+    else:
+        monkeypatch.delenv("PSA_TILES", raising=False)
     p32 = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_BATCHED)
     ph = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH)
     p32.run()
@@ -62,6 +74,9 @@ def test_l1cp_fp16_vs_fp64(sdr, orc, l1cp):
     worst = 0.0
     for i, s in enumerate(w["sigs"]):
         assert ph.form(i) == sdr.FORM_FP16
+        if tiles:
+            M1, vc, vr = (int(x) for x in tiles.split("/"))
+            assert ph.tiles(i) == (M1, s["M"] // M1, vc, vr)
         _, ix32, _ = p32.result(i)
         _, ixh, _ = ph.result(i)
         for q in range(s["code_fft"].shape[0]):
@@ -78,6 +93,6 @@ def test_l1cp_fp16_vs_fp64(sdr, orc, l1cp):
         for sat in w["truth"][s["name"]]:
             q = s["prns"].index(sat["prn"])
             assert ixh[q].tolist() == ix32[q].tolist(), (s["name"], sat["prn"])
-    print("L1CP fp16 worst peak-power error vs FP64: %.2e" % worst)
+    print("L1CP fp16 (%s) worst peak-power error vs FP64: %.2e" % (tiles, worst))
     p32.close()
     ph.close()
