@@ -135,7 +135,7 @@ FORM_FLAGS = {"fp32": 16, "fp16": 4 | 16, "bf16": 8 | 16}    # batched forms, tu
 TILES = {"multi": "h24000=150/0/0,h96000=200/1/1,h48000=300/0/0,h192000=400/0/1,"
                   "f24000=150,f96000=200,f48000=200,f192000=200",
          "l1ca": "h24000=150/1/0,f24000=150",
-         "l1cp": "h960000=1000/1/0"}
+         "l1cp": "h960000=1000/0/0"}
 FORM_DTYPE = {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}
 
 

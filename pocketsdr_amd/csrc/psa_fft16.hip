@@ -999,9 +999,11 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
 // host side
 // ---------------------------------------------------------------------------
 // (L, R0, R1, R2, G, threads, variant): column-pass lengths (G columns =
-// 128 bytes of words per row segment; 16 for 960 to fit two workgroups per
-// CU) and row-pass lengths (G rows, 4-8 K points per workgroup, G L / R a
-// multiple of 32 where possible).  Each length has two workgroup shapes; the
+// 128 bytes of words per row segment; the 16-column shapes of 960 and 1000
+// fit two workgroups per CU, their workgroups paired on one XCD) and
+// row-pass lengths (G rows, 4-8 K points per workgroup, G L / R a multiple
+// of 32 where possible).  1000 runs as 25 x 40 (two stages, 20 or 40 outputs
+// per lane and tile) or as 10 x 10 x 10.  Each length has two workgroup shapes; the
 // plan's tuner (SDR_SEARCH_TUNE) times both on the real shapes, variant 0 is
 // the default.  Splits: 24000 = 150 x 160, 48000 = 150 x 320 or 200 x 240,
 // 96000 = 300 x 320 or 200 x 480, 192000 = 400 x 480, 960000 = 960 x 1000
@@ -1012,14 +1014,14 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
     X(300, 15, 20, 1, 32, 512, 0) X(300, 15, 20, 1, 32, 256, 1) \
     X(400, 16, 25, 1, 32, 512, 0) X(400, 16, 25, 1, 32, 256, 1) \
     X(960, 32, 30, 1, 32, 512, 0) X(960, 32, 30, 1, 16, 256, 1) \
-    X(1000, 10, 10, 10, 8, 512, 0) X(1000, 25, 40, 1, 16, 512, 1)
+    X(1000, 25, 40, 1, 32, 512, 0) X(1000, 25, 40, 1, 16, 512, 1)
 #define PSA_HROW(X) \
     X(160, 16, 10, 1, 32, 256, 0) X(160, 16, 10, 1, 32, 512, 1) \
     X(240, 16, 15, 1, 32, 256, 0) X(240, 16, 15, 1, 32, 512, 1) \
     X(320, 16, 20, 1, 16, 256, 0) X(320, 16, 20, 1, 32, 512, 1) \
     X(480, 16, 30, 1, 16, 512, 0) X(480, 16, 30, 1, 16, 256, 1) \
     X(1000, 10, 10, 10, 4, 256, 0) X(1000, 25, 40, 1, 8, 512, 1) \
-    X(960, 32, 30, 1, 8, 256, 0) X(960, 32, 30, 1, 8, 512, 1)
+    X(960, 32, 30, 1, 8, 256, 0) X(960, 32, 30, 1, 16, 512, 1)
 
 struct HSpec { int L, r[3], G, nth, var; };
 static const HSpec h_col_specs[] = {

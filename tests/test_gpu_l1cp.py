@@ -56,8 +56,9 @@ def test_l1cp_fp32_vs_oracle(sdr, orc, l1cp):
 # FP16 tilings of M = 960000 (M1 / column shape / row shape): the default
 # (columns of 960 in 32-column workgroups, rows of 1000 as 10 x 10 x 10), the
 # 16-column workgroups paired on one XCD, rows of 1000 as 25 x 40, and
-# columns of 1000 as 25 x 40 with rows of 960
-TILINGS = [None, "960/1/1", "960/0/1", "1000/1/0"]
+# columns of 1000 as 25 x 40 (16- and 32-column workgroups) with rows of 960
+# (8- and 16-row workgroups)
+TILINGS = [None, "960/1/1", "960/0/1", "1000/1/0", "1000/0/1", "1000/0/0"]
 
 
 @pytest.mark.parametrize("tiles", TILINGS)
