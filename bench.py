@@ -55,22 +55,6 @@ SPEC_G = {150: 16, 160: 16, 200: 16, 240: 16, 300: 16, 320: 12, 400: 16, 480: 8,
 MODES = {"fwd_col": (2, 0), "fwd_row": (0, 0), "inv_row": (3, 0), "inv_col": (0, 1)}
 
 
-# precision-form passes: (R0, R1, R2, G, threads) per (length, workgroup-shape
-# variant) of the column and row passes
-HCOL = {(150, 0): (10, 15, 1, 32, 256), (150, 1): (10, 15, 1, 32, 512),
-        (200, 0): (10, 20, 1, 32, 512), (200, 1): (10, 20, 1, 32, 256),
-        (300, 0): (15, 20, 1, 32, 512), (300, 1): (15, 20, 1, 32, 256),
-        (400, 0): (16, 25, 1, 32, 512), (400, 1): (16, 25, 1, 32, 256),
-        (960, 0): (32, 30, 1, 16, 512), (960, 1): (32, 30, 1, 16, 256),
-        (1000, 0): (10, 10, 10, 8, 512), (1000, 1): (10, 10, 10, 16, 512)}
-HROW = {(160, 0): (16, 10, 1, 32, 256), (160, 1): (16, 10, 1, 32, 512),
-        (240, 0): (16, 15, 1, 32, 256), (240, 1): (16, 15, 1, 32, 512),
-        (320, 0): (16, 20, 1, 16, 256), (320, 1): (16, 20, 1, 32, 512),
-        (480, 0): (16, 30, 1, 16, 512), (480, 1): (16, 30, 1, 16, 256),
-        (1000, 0): (10, 10, 10, 4, 256), (1000, 1): (10, 10, 10, 8, 512),
-        (960, 0): (32, 30, 1, 8, 256), (960, 1): (32, 30, 1, 8, 512)}
-
-
 # fused cell inverse kernels of the FP16 form by split (M1, M2)
 CELL = {(150, 160): "cell_kernel<160, 16, 10, 150, 10, 15, 50, 80, 1024>"}
 
@@ -89,7 +73,8 @@ def kernel_instance(cls, M, form="fp32", tiles=None):
         M1, M2, vc, vr = tiles
         col = cls in ("fwd_col", "inv_col")
         L, v = (M1, vc) if col else (M2, vr)
-        r0, r1, r2, G, nth = HCOL[(L, v)] if col else HROW[(L, v)]
+        from pocketsdr_amd import sdr_func
+        r0, r1, r2, G, nth = sdr_func.pass_shape(col, L, v)
         lm, sm = MODES[cls]
         return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s, %d>" % (
             lm, sm, 0 if form == "fp16" else 1, L, r0, r1, r2, G, "false" if col else "true", nth)

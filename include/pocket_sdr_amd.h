@@ -199,9 +199,14 @@ int sdr_search_plan_power(const sdr_search_plan_t *plan, int sig, int prn_idx,
 #define SDR_FORM_BF16   3      /* batched, BF16 words, MFMA stages */
 int sdr_search_plan_form(const sdr_search_plan_t *plan, int sig);
 /* Tiles of group `sig`: tiles[0..3] = column length M1, row length M2 of the
- * FFT split, column and row workgroup-shape variant (0 or 1; precision form
+ * FFT split, column and row workgroup-shape variant (0 to 3; precision form
  * only).  Returns 0 on a bad argument. */
 int sdr_search_plan_tiles(const sdr_search_plan_t *plan, int sig, int *tiles);
+/* Workgroup shape `var` of the precision form's column (col = 1) or row
+ * (col = 0) passes of length L: shape[0..4] = the radices R0, R1, R2 (1 when
+ * unused), G sub-transforms and threads per workgroup.  Returns 0 when the
+ * length has no such shape. */
+int sdr_search_pass_shape(int col, int L, int var, int *shape);
 /* Enables per-kernel HIP-event timing of later runs (0 disables). */
 void sdr_search_plan_timing(sdr_search_plan_t *plan, int on);
 /* Average device time (ms) and launch count of kernel class `name`

@@ -1011,15 +1011,22 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
 #define PSA_HCOL(X) \
     X(150, 10, 15, 1, 32, 256, 0) X(150, 10, 15, 1, 32, 512, 1) \
     X(200, 10, 20, 1, 32, 512, 0) X(200, 10, 20, 1, 32, 256, 1) \
+    X(200, 8, 25, 1, 32, 256, 2) X(200, 5, 40, 1, 32, 256, 3) \
     X(300, 15, 20, 1, 32, 512, 0) X(300, 15, 20, 1, 32, 256, 1) \
+    X(300, 10, 30, 1, 32, 512, 2) X(300, 12, 25, 1, 32, 512, 3) \
     X(400, 16, 25, 1, 32, 512, 0) X(400, 16, 25, 1, 32, 256, 1) \
+    X(400, 10, 40, 1, 32, 512, 2) X(400, 20, 20, 1, 32, 512, 3) \
     X(960, 32, 30, 1, 32, 512, 0) X(960, 32, 30, 1, 16, 256, 1) \
     X(1000, 25, 40, 1, 32, 512, 0) X(1000, 25, 40, 1, 16, 512, 1)
 #define PSA_HROW(X) \
     X(160, 16, 10, 1, 32, 256, 0) X(160, 16, 10, 1, 32, 512, 1) \
+    X(160, 10, 16, 1, 32, 256, 2) X(160, 20, 8, 1, 32, 256, 3) \
     X(240, 16, 15, 1, 32, 256, 0) X(240, 16, 15, 1, 32, 512, 1) \
+    X(240, 8, 30, 1, 32, 256, 2) X(240, 15, 16, 1, 32, 256, 3) \
     X(320, 16, 20, 1, 16, 256, 0) X(320, 16, 20, 1, 32, 512, 1) \
+    X(320, 32, 10, 1, 16, 256, 2) X(320, 10, 32, 1, 16, 256, 3) \
     X(480, 16, 30, 1, 16, 512, 0) X(480, 16, 30, 1, 16, 256, 1) \
+    X(480, 32, 15, 1, 16, 256, 2) X(480, 12, 40, 1, 16, 256, 3) \
     X(1000, 10, 10, 10, 4, 256, 0) X(1000, 25, 40, 1, 8, 512, 1) \
     X(960, 32, 30, 1, 8, 256, 0) X(960, 32, 30, 1, 16, 512, 1)
 
@@ -1045,6 +1052,21 @@ static const HSpec *col_spec(int L, int var = 0)
 static const HSpec *row_spec(int L, int var = 0)
 {
     return find_spec(h_row_specs, (int)(sizeof(h_row_specs) / sizeof(HSpec)), L, var);
+}
+
+extern "C" int sdr_search_pass_shape(int col, int L, int var, int *shape)
+{
+    const psa::HSpec *h = col ? psa::col_spec(L, var) : psa::row_spec(L, var);
+    if (!h || !shape) return 0;
+    for (int i = 0; i < 3; i++) shape[i] = h->r[i];
+    shape[3] = h->G;
+    shape[4] = h->nth;
+    return 1;
+}
+
+bool h_shape(int M, int M1, int vc, int vr)
+{
+    return M1 > 0 && M % M1 == 0 && col_spec(M1, vc) && row_spec(M / M1, vr);
 }
 
 int h_splits(int M, int *m1, int nmax)
@@ -1171,7 +1193,7 @@ const HPlan *hfft_plan(int M, int dt, int M1)
         if (dir == 1 && dt == HDT_F16) p->tfullh = pack_table_f16(p->tfull[dir], M);
         // the two workgroup shapes of a length may split it into different
         // radices: each shape has its own tables
-        for (int v = 0; v < 2; v++) {
+        for (int v = 0; v < PSA_HVARS; v++) {
             const HSpec *cs = col_spec(M1, v), *rs = row_spec(M2, v);
             for (int s = 0; s < 3; s++) {
                 if (cs) p->am1[v][dir][s] = cs->r[s] > 1 ? make_amat(cs->r[s], dir, dt) : nullptr;

@@ -231,6 +231,8 @@ void row_pass_args(const FFTPlan *pl, int dir, PassArgs &a);
 // precision form (psa_fft16.hip): FP16 / BF16 words, MFMA DFT-matrix stages
 // ---------------------------------------------------------------------------
 enum { HDT_F16 = 0, HDT_BF16 = 1 };
+// workgroup shapes (radix split, workgroup size) per pass length
+#define PSA_HVARS 4
 
 struct HPlan {
     int M = 0, M1 = 0, M2 = 0;  // column length M1, row length M2
@@ -239,7 +241,7 @@ struct HPlan {
     float2 *tfull[2] = {};               // inter-pass twiddle by output index, [dir]
     uint32_t *tfullh = nullptr;          // FP16 words of tfull[1] (FP16 plans)
     // DFT tables of each workgroup shape's radices, [shape][dir][stage]
-    const uint4 *am1[2][2][3] = {}, *am2[2][2][3] = {};
+    const uint4 *am1[PSA_HVARS][2][3] = {}, *am2[PSA_HVARS][2][3] = {};
     // power-of-two scales: forward column / row pass outputs, inverse row
     // pass output (folded into tfull where a pass has the twiddle)
     float osc_fc = 1, osc_fr = 1, osc_ir = 1;
@@ -249,7 +251,10 @@ struct HPlan {
 const HPlan *hfft_plan(int M, int dt, int M1 = 0);
 // column lengths M1 of the MFMA splits of M, default first; returns the count
 int h_splits(int M, int *m1, int nmax);
-// pass geometry; var selects one of the two workgroup shapes of the length
+// whether M1 (column length) and M / M1 (row length) have workgroup shapes
+// vc and vr
+bool h_shape(int M, int M1, int vc, int vr);
+// pass geometry; var selects one of the workgroup shapes of the length
 void h_col_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);
 void h_row_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);
 void launch_hpass(int dt, LoadMode lm, StoreMode sm, const PassArgs &a, int nbatch,

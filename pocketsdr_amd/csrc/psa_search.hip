@@ -1010,8 +1010,8 @@ static bool fixed_tiles(const SigDev &s, int *m1, int *vc, int *vr)
         bool form_ok = true;
         if (*q == 'h' || *q == 'f') form_ok = (*q++ == 'h') == (s.hp != nullptr);
         const int n = sscanf(q, "%d=%d/%d/%d", &M, &a, &c, &r);
-        if (form_ok && n >= 2 && M == s.h.M && std::find(cand, cand + ns, a) != cand + ns && c >= 0 && c < 2 && r >= 0 &&
-            r < 2) {
+        if (form_ok && n >= 2 && M == s.h.M && std::find(cand, cand + ns, a) != cand + ns &&
+            (!s.hp || h_shape(M, a, c, r))) {
             *m1 = a; *vc = s.hp ? c : 0; *vr = s.hp ? r : 0;
             return true;
         }
@@ -1031,7 +1031,7 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
     int m1[8];
     const bool h = s.hp != nullptr;
     const int ns = h ? h_splits(s.h.M, m1, 8) : fft_splits(s.h.M, m1, 8);
-    const int nv = h ? 2 : 1;
+    const int nv = h ? PSA_HVARS : 1;
     const int nblk = s.h.n_blk;
     hipEvent_t e0, e1;
     PSA_CHECK(hipEventCreate(&e0));
@@ -1049,6 +1049,7 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
     for (int i = 0; i < (fixed ? 0 : ns); i++) {
         for (int vc = 0; vc < nv; vc++) {
             for (int vr = 0; vr < nv; vr++) {
+                if (h && !h_shape(s.h.M, m1[i], vc, vr)) continue;
                 if (h) h_setup(p, s, m1[i], vc, vr);
                 else f_setup(p, s, m1[i]);
                 float t = 1e30f;

@@ -95,6 +95,7 @@ _decl("sdr_search_plan_result", C.c_int, [c_void_p, C.c_int, c_float_p, c_int_p,
 _decl("sdr_search_plan_power", C.c_int, [c_void_p, C.c_int, C.c_int, c_float_p])
 _decl("sdr_search_plan_form", C.c_int, [c_void_p, C.c_int])
 _decl("sdr_search_plan_tiles", C.c_int, [c_void_p, C.c_int, c_int_p])
+_decl("sdr_search_pass_shape", C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p])
 _decl("sdr_search_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_search_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                              C.POINTER(C.c_longlong), c_double_p])
@@ -281,6 +282,15 @@ def dop_bins(T, dop, max_dop):
     out = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
     C.CDLL(None).free(C.cast(p, c_void_p))
     return out
+
+
+def pass_shape(col, L, var):
+    """(R0, R1, R2, G, threads) of workgroup shape `var` of the precision
+    form's column (col True) or row passes of length L, else None."""
+    t = (C.c_int * 5)()
+    if not lib.sdr_search_pass_shape(1 if col else 0, L, var, C.cast(t, c_int_p)):
+        return None
+    return tuple(t)
 
 
 def search_plan_bytes():

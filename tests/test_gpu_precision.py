@@ -62,7 +62,7 @@ def test_precision_form(sdr, orc, multi_weak, form, monkeypatch):
     for i, s in enumerate(w["sigs"]):
         assert ph.form(i) == want, s["name"]
         M1, M2, vc, vr = ph.tiles(i)
-        assert M1 * M2 == s["M"] and vc in (0, 1) and vr in (0, 1)
+        assert M1 * M2 == s["M"] and 0 <= vc < 4 and 0 <= vr < 4
         if form.endswith("fixed"):
             # (M = 24000 runs the fused cell kernel in FP16: one split, no
             # workgroup shapes to choose)
@@ -94,6 +94,51 @@ def test_precision_form(sdr, orc, multi_weak, form, monkeypatch):
     print("%s worst peak-power error vs FP64: %.2e" % (form, worst))
     p32.close()
     ph.close()
+
+
+# column lengths of the MFMA splits (the row length is M / M1)
+SPLITS = {96000: (200, 300, 400), 48000: (150, 200, 300), 192000: (200, 400)}
+
+
+@pytest.mark.parametrize("name", ["E1B", "L5Q", "L6D"])
+def test_every_shape(sdr, orc, multi_weak, name, monkeypatch):
+    """Every workgroup shape (radix split, workgroup size) of every split of
+    the group's FFT length, fixed through PSA_TILES: power surface within
+    2e-2 of FP32, peak power of each PRN within 1e-2 of FP64, the FP32 peak
+    cell of each detection."""
+    w = multi_weak
+    i = [k for k, s in enumerate(w["sigs"]) if s["name"] == name][0]
+    s = w["sigs"][i]
+    M = s["M"]
+    monkeypatch.delenv("PSA_TILES", raising=False)
+    p32 = sdr.SearchPlan(w["buffs"], [s], sdr.SEARCH_BATCHED)
+    p32.run()
+    c32, ix32, _ = p32.result(0)
+    P32 = [p32.power(0, q) for q in range(len(s["prns"]))]
+    ref = [peak_f64(orc, w, s, q, ix32[q][0])[ix32[q][1]] for q in range(len(s["prns"]))]
+    p32.close()
+    done = []
+    for M1 in SPLITS[M]:
+        for vc in range(4):
+            for vr in range(4):
+                monkeypatch.setenv("PSA_TILES", "h%d=%d/%d/%d" % (M, M1, vc, vr))
+                ph = sdr.SearchPlan(w["buffs"], [s], sdr.SEARCH_FP16)
+                if ph.tiles(0) != (M1, M // M1, vc, vr):     # no such shape
+                    ph.close()
+                    continue
+                ph.run()
+                ch, ixh, _ = ph.result(0)
+                for q in range(len(s["prns"])):
+                    Ph = ph.power(0, q)
+                    assert rel(Ph, P32[q]) < 2e-2, (M1, vc, vr, q, rel(Ph, P32[q]))
+                    b, lag = ix32[q]
+                    assert abs(float(Ph[b, lag]) - ref[q]) / ref[q] < 1e-2, (M1, vc, vr, q)
+                    if c32[q] >= CN0_THRES:
+                        assert ixh[q].tolist() == ix32[q].tolist(), (M1, vc, vr, q)
+                ph.close()
+                done.append((M1, vc, vr))
+    assert len(done) >= 2 * len(SPLITS[M]), done
+    print("%s: %d shapes" % (name, len(done)))
 
 
 def test_precision_plain_flag_ignored(sdr, multi_weak):
