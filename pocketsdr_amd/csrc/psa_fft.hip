@@ -403,7 +403,11 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
     __shared__ uint32_t carr[512];      // carrier table, then a raw ring's table
     __shared__ PeakPart red[PASS_THREADS / 64];
     int b, grp;
-    if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp)) return;
+    // column passes: CG float2 (inverse) or CG IF bytes (mixing) per row
+    // segment, kx workgroups to one 128-byte line
+    const int segb = LM == LD_MIX ? CG : 8 * CG;
+    const int kx = a.in_nstride != 1 && segb < 128 ? 128 / segb : 1;
+    if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp, kx)) return;
     const int g0 = grp * CG;
     const int gcount = min(CG, a.nsub - g0);
     const float sgn = a.dir == 0 ? -1.0f : 1.0f;

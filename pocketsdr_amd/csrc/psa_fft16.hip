@@ -446,9 +446,11 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     __shared__ uint32_t carr[LM == LD_MIX ? 512 : 1];
     __shared__ PeakPart red[NTH / 64];
     int b, grp;
-    // column passes of FP16 words: G words per row segment
-    constexpr int KX = (!NF && G < 32) ? 32 / G : 1;
-    if (!block_item<KX>(a, LM == LD_PROD, ngroups, &b, &grp)) return;
+    // column passes: G words (inverse) or G IF bytes (mixing) per row
+    // segment, KX workgroups to one 128-byte line
+    constexpr int SEGB = LM == LD_MIX ? G : 4 * G;
+    constexpr int KX = (!NF && SEGB < 128) ? 128 / SEGB : 1;
+    if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp, KX)) return;
     const int g0 = grp * G;
     const int gcount = min(G, a.nsub - g0);
     // issue order: stage-0 table, the LDS tables' words, the first tile's

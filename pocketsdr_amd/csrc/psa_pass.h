@@ -261,19 +261,18 @@ struct Storer {
 // back to back there, re-reading the shared X rows from that XCD's L2.  The
 // grid is padded to whole rounds of 8; returns false for the padding.
 // KX > 1 (column passes whose G columns span less than a 128-byte line):
-// the KX workgroups of adjacent column groups, which read and write the two
-// (four) parts of the same lines, go to one XCD in consecutive rounds, so
+// the KX workgroups of adjacent column groups, which read and write the KX
+// parts of the same lines, go to one XCD in consecutive rounds, so
 // that each line is fetched from HBM into one L2 only (workgroup h of a
 // whole block of 8 KX runs logical item (s / KX) 8 KX + (h % 8) KX + s % KX,
 // s = h / 8; the last partial block keeps the identity order).
-template <int KX = 1>
-__device__ __forceinline__ bool block_item(const PassArgs &a, bool prod, int ngroups, int *b, int *grp)
+__device__ __forceinline__ bool block_item(const PassArgs &a, bool prod, int ngroups, int *b, int *grp, int kx = 1)
 {
-    if (KX > 1 && !prod) {
+    if (kx > 1 && !prod) {
         int h = (int)blockIdx.x;
-        if (h < (int)(gridDim.x / (8u * KX)) * 8 * KX) {
+        if (h < (int)(gridDim.x / (8u * kx)) * 8 * kx) {
             const int s = h >> 3;
-            h = (s / KX) * (8 * KX) + (h & 7) * KX + s % KX;
+            h = (s / kx) * (8 * kx) + (h & 7) * kx + s % kx;
         }
         *b = h / ngroups;
         *grp = h - *b * ngroups;
