@@ -219,8 +219,12 @@ struct ATab {
 // output store (stages whose inputs and outputs share one LDS buffer, pre()
 // then being the barrier between them; first stages with few tiles, whose
 // global loads are then all in flight together).
+// UNR: the tile loop fully unrolled (stages reading LDS), ob.y of each tile
+// set to its iteration index, so that the functors can keep per-tile values
+// in registers (the paired-block power pass).
 template <int DT, int NTH, int L, int R, int G, bool NF, int TH, int MULK, bool PF, bool ALL = false,
-          bool MASKT = true, bool PFM = false, class IBase, class ILoad, class ICook, class OBase, class OMul, class OPut, class Pre>
+          bool MASKT = true, bool PFM = false, bool UNR = false, class IBase, class ILoad, class ICook, class OBase,
+          class OMul, class OPut, class Pre>
 __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &ibase, const ILoad &iload,
                                        const ICook &icook, const OBase &obase, const OMul &omul,
                                        const OPut &oput, const Pre &pre)
@@ -352,6 +356,47 @@ __device__ __forceinline__ void mstage(const ATab<DT, R, TH> &at, const IBase &i
                 cook(ibs[i], rw[i], Bt);
                 if (!PFM) get_muls(obs[i], mus[i]);
                 tile_body(obs[i], cvs[i], Bt, mus[i]);
+            }
+        }
+        return;
+    }
+    if constexpr (UNR) {
+        static_assert(!PF, "unrolled tile loop: stages reading LDS");
+        Muls mus[2];
+        {
+            int g = 0, j = 0;
+            column(wave < NT ? wave : NT - 1, g, j);
+            int4 ob = obase(g, j);
+            ob.y = 0;
+            pre();
+            if (PFM) get_muls(ob, mus[0]);
+        }
+        at.complete();
+#pragma unroll
+        for (int it = 0; it < MT; it++) {
+            const int tl = wave + it * NW;
+            if (tl < NT) {
+                int g = 0, j = 0;
+                const bool cv = column(tl, g, j);
+                int4 ob = obase(g, j);
+                ob.y = it;
+                if (PFM && it + 1 < MT) {
+                    // the next tile's multipliers (a wave-uniform branch)
+                    if (tl + NW < NT) {
+                        int gn = 0, jn = 0;
+                        column(tl + NW, gn, jn);
+                        int4 obn = obase(gn, jn);
+                        obn.y = it + 1;
+                        get_muls(obn, mus[(it + 1) & 1]);
+                    }
+                }
+                const int ib = ibase(g, j);
+                Raw raw[KS][4];
+                uint4 B[KS];
+                fetch(ib, raw);
+                cook(ib, raw, B);
+                if (!PFM) get_muls(ob, mus[it & 1]);
+                tile_body(ob, cv, B, mus[it & 1]);
             }
         }
         return;
@@ -498,6 +543,124 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     auto lds_base = [&](int g, int j) -> int { return g * RP + j + (PD ? j / R0 : 0); };
     const int dummy = G * RP + (int)(threadIdx.x & 63);
 
+    // ---- last stage (LDS -> MFMA -> global): storers and multipliers ----
+    Storer<ST_POW> sp;
+    if constexpr (SM == ST_POW) sp.init(a, b);
+    const __amdgpu_buffer_rsrc_t hout = rsrc(a.hout + (SM == ST_CPX ? (long long)b * a.out_bstride : 0LL));
+    const __amdgpu_buffer_rsrc_t tfr = rsrc(TWP ? (const void *)a.tfullh : (const void *)a.tfull);
+    const float osc = a.osc, kap2 = a.kappa * a.kappa;
+    const int ogs = (int)a.out_gstride, ons = NF ? 1 : (int)a.out_nstride;
+    const int estep = NSL * ons;
+    // output t of column j (< NSL) is element k = j + t NSL of its
+    // sub-transform, at e = gg ogs + k ons.  ST_CPX multiplies by the
+    // inter-pass twiddle (its table is laid out like the output) or by osc;
+    // ST_POW scales the power by kappa^2 (the multiplier is 1), and the
+    // previous P value rides along in .y (fetched with the tile's other
+    // loads; 0 on the first block)
+    const bool pacc = SM == ST_POW && !a.p_init;
+    const __amdgpu_buffer_rsrc_t prr = rsrc(sp.prow);
+    const int nlag = a.nlag;
+    auto gbase = [&](int g, int j) -> int4 {
+        const bool ok = g < gcount;
+        return make_int4((g0 + (ok ? g : 0)) * ogs + j * ons, 0, 0, ok);
+    };
+    auto gmul = [&](const int4 &ob, int tc, int th, bool ok) {
+        if constexpr (SM == ST_POW) {
+            const int e = ob.x + (tc + th) * estep;
+            return make_float2(1.0f, __uint_as_float(bload32(prr, e, pacc && ok && e < nlag)));
+        } else if constexpr (TWP) {
+            return bload32_at(tfr, ob.x + th * estep, tc * estep, ok);
+        } else if constexpr (TW) {
+            return bload_f2_at(tfr, ob.x + th * estep, tc * estep, ok);
+        } else {
+            return make_float2(osc, 0.0f);
+        }
+    };
+    using GMulT = decltype(gmul(int4{}, 0, 0, true));
+    auto gput = [&](const int4 &ob, int tc, int th, float2 v, GMulT f, bool ok) {
+        if constexpr (SM == ST_POW) {
+            sp.store_pow_sel(ob.x + (tc + th) * estep, v, kap2, f.y, ok, [&](bool ok2, int e2, float nv) {
+                bstore32(prr, e2, __float_as_uint(nv), ok2);
+            });
+        } else {
+            uint32_t w = pack_c<DT>(v.x, v.y);
+            if constexpr (TWP) w = cmul_w<DT>(w, f);
+            bstore32_at(hout, ob.x + th * estep, tc * estep, w, ok);
+        }
+    };
+    constexpr int MULKL = TWP ? 3 : TW ? 2 : 1;
+    // stage 1 input t of column j: n = j + t R0 R2
+    // (unused inputs read t = R - 1 of the stage: written words)
+    auto iat0 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+        return buf0[base + (ok ? tc + th : R1 - 1) * (R0 * R2 + PD * R2)];
+    };
+    auto ident = [](int, int, int, bool, uint32_t w) -> uint32_t { return w; };
+    // paired blocks (PB): a power pass whose workgroup fills the CU's LDS
+    // (one workgroup per CU: the registers of the kept powers cost no
+    // occupancy) may take two blocks' inputs (hin, hin2) and write P once
+    constexpr int RLT = S3 ? R2 : R1;
+    constexpr int NTL = (G * (L / RLT) + 31) / 32, MTL = (NTL + NTH / 64 - 1) / (NTH / 64);
+    constexpr int RTL = (2 * THL + 31) / 32;
+    constexpr bool PB = DT == HDT_F16 && SM == ST_POW && !S3 && G * RP * 4 > 80 * 1024;
+    float keep[PB ? MTL : 1][PB ? RTL * 8 : 1];
+    // output tc = 16 r + 4 m + w of a tile: accumulator slot 8 r + 2 m + w
+    auto kidx = [](int tc) { return (tc / 16) * 8 + ((tc % 16) / 4) * 2 + (tc % 4); };
+    (void)kidx;
+    // MODE 0: one block; 1, 2: the first and second block of a pair
+    auto run_last = [&](auto mode_c) {
+        constexpr int MODE = decltype(mode_c)::value;
+        if constexpr (!S3) {
+            ATab<DT, R1, THL> at1;
+            at1.load(a.amat[1]);
+            if constexpr (MODE == 0) {
+                mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false, false, true, SM == ST_POW>(at1, lds_base, iat0, ident,
+                                                                                          gbase, gmul, gput, barrier);
+            } else if constexpr (MODE == 1) {
+                // first block of a pair: its accumulated power (with the previous
+                // P value) stays in registers, by tile and output
+                auto gput1 = [&](const int4 &ob, int tc, int th, float2 v, float2 f, bool) {
+                    const float pw = kap2 * __builtin_fmaf(v.x, v.x, v.y * v.y);
+                    keep[ob.y][kidx(tc)] = sp.pinit ? pw : f.y + pw;
+                };
+                mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false, false, true, true, true>(at1, lds_base, iat0, ident,
+                                                                                        gbase, gmul, gput1, barrier);
+            } else {
+                // second block: the kept value is the previous P value
+                auto gmul2 = [&](const int4 &ob, int tc, int, bool) { return make_float2(1.0f, keep[ob.y][kidx(tc)]); };
+                sp.pinit = false;
+                mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false, false, true, false, true>(at1, lds_base, iat0, ident,
+                                                                                         gbase, gmul2, gput, barrier);
+            }
+        } else {
+            // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
+            // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
+            auto obase1 = [&](int g, int j) -> int4 {
+                const int jh = j / R0, jl = j - jh * R0;
+                return make_int4(g * RP + jh * R0 * R1 + jl + (PD ? jh * R1 : 0), jh * jl, jh * R0, 1);
+            };
+            auto omul1 = [&](const int4 &ob, int tc, int th, bool ok) -> TwT {
+                return tws[ob.y + (ok ? tc + th : 0) * ob.z];
+            };
+            auto oput1 = [&](const int4 &ob, int tc, int th, float2 v, TwT f, bool ok) {
+                uint32_t w = pack_c<DT>(v.x, v.y);
+                if constexpr (PKT) w = cmul_w<DT>(w, f);
+                buf1[ok ? ob.x + (tc + th) * (R0 + PD) : dummy] = w;
+            };
+            ATab<DT, R1, R1> at1;
+            at1.load(a.amat[1]);
+            mstage<DT, NTH, L, R1, G, NF, R1, PKT ? 3 : 2, false, false, false>(at1, lds_base, iat0, ident, obase1,
+                                                                                 omul1, oput1, barrier);
+            // stage 2 input t of column j: n = j + t R0 R1
+            auto iat1 = [&](int base, int tc, int th, bool ok) -> uint32_t {
+                return buf1[base + (ok ? tc + th : R2 - 1) * (R0 * R1 + PD * R1)];
+            };
+            ATab<DT, R2, THL> at2;
+            at2.load(a.amat[2]);
+            mstage<DT, NTH, L, R2, G, NF, THL, MULKL, false, false, true, SM == ST_POW>(at2, lds_base, iat1, ident,
+                                                                                      gbase, gmul, gput, barrier);
+        }
+    };
+
     // ---- stage 0: global -> MFMA -> LDS ----
     // output t of column j sits at q = j R0 + t (LDS j (R0 + PD) + t); the
     // next stage's twiddle is w_L^{t (j / R2) R2} (w_L^{j t}: two stages,
@@ -519,6 +682,7 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
     // results do not change) and stores the products unpadded in buf0;
     // stage 0 then reads all its inputs from LDS before the barrier that
     // precedes its first output store, so buf0 holds both.
+    bool paired = false;
     if constexpr (STG) {
         const long long xo = (long long)((b / a.prod_xdiv) % a.prod_xmod) * a.in_bstride;
         const long long co = (long long)(b / a.prod_cdiv) * a.cf_bstride;
@@ -571,7 +735,8 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
             xo = (long long)((b / a.prod_xdiv) % a.prod_xmod) * a.in_bstride;
             co = (long long)(b / a.prod_cdiv) * a.cf_bstride;
         }
-        const __amdgpu_buffer_rsrc_t xin = rsrc(a.hin + xo), cin = rsrc(a.hcf + co);
+        __amdgpu_buffer_rsrc_t xin = rsrc(a.hin + xo);
+        const __amdgpu_buffer_rsrc_t cin = rsrc(a.hcf + co);
         // rows past the end of the batch item (last group) read row 0
         auto ibase = [&](int g, int j) -> int { return (g0 + (g < gcount ? g : 0)) * igs + j * ins; };
         // unused inputs (ok = false): the IF byte of t = R0 - 1, or a masked
@@ -602,93 +767,22 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         constexpr bool ALL0 = LM == LD_CPX && MT0 <= 6;
         mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, ALL0, false>(at0, ibase, iload, icook, obase, omul,
                                                                            oput, fill);
+        if constexpr (PB) {
+            if (a.hin2) {                   // uniform over the grid
+                run_last(std::integral_constant<int, 1>{});
+                // the second block's item: its stage 0 stores to buf0 after
+                // the barrier that follows every wave's reads of the first
+                xin = rsrc(a.hin2 + xo);
+                // (loads one tile ahead: the kept powers hold registers)
+                mstage<DT, NTH, L, R0, G, NF, R0, PKT ? 3 : 2, true, false, false>(at0, ibase, iload, icook, obase,
+                                                                                    omul, oput, barrier);
+                run_last(std::integral_constant<int, 2>{});
+                paired = true;
+            }
+        }
     }
 
-    // ---- last stage: LDS -> MFMA -> global ----
-    Storer<ST_POW> sp;
-    if constexpr (SM == ST_POW) sp.init(a, b);
-    const __amdgpu_buffer_rsrc_t hout = rsrc(a.hout + (SM == ST_CPX ? (long long)b * a.out_bstride : 0LL));
-    const __amdgpu_buffer_rsrc_t tfr = rsrc(TWP ? (const void *)a.tfullh : (const void *)a.tfull);
-    const float osc = a.osc, kap2 = a.kappa * a.kappa;
-    const int ogs = (int)a.out_gstride, ons = NF ? 1 : (int)a.out_nstride;
-    const int estep = NSL * ons;
-    // output t of column j (< NSL) is element k = j + t NSL of its
-    // sub-transform, at e = gg ogs + k ons.  ST_CPX multiplies by the
-    // inter-pass twiddle (its table is laid out like the output) or by osc;
-    // ST_POW scales the power by kappa^2 (the multiplier is 1), and the
-    // previous P value rides along in .y (fetched with the tile's other
-    // loads; 0 on the first block)
-    const bool pacc = SM == ST_POW && !a.p_init;
-    const __amdgpu_buffer_rsrc_t prr = rsrc(sp.prow);
-    const int nlag = a.nlag;
-    auto gbase = [&](int g, int j) -> int4 {
-        const bool ok = g < gcount;
-        return make_int4((g0 + (ok ? g : 0)) * ogs + j * ons, 0, 0, ok);
-    };
-    auto gmul = [&](const int4 &ob, int tc, int th, bool ok) {
-        if constexpr (SM == ST_POW) {
-            const int e = ob.x + (tc + th) * estep;
-            return make_float2(1.0f, __uint_as_float(bload32(prr, e, pacc && ok && e < nlag)));
-        } else if constexpr (TWP) {
-            return bload32_at(tfr, ob.x + th * estep, tc * estep, ok);
-        } else if constexpr (TW) {
-            return bload_f2_at(tfr, ob.x + th * estep, tc * estep, ok);
-        } else {
-            return make_float2(osc, 0.0f);
-        }
-    };
-    using GMulT = decltype(gmul(int4{}, 0, 0, true));
-    auto gput = [&](const int4 &ob, int tc, int th, float2 v, GMulT f, bool ok) {
-        if constexpr (SM == ST_POW) {
-            sp.store_pow_sel(ob.x + (tc + th) * estep, v, kap2, f.y, ok, [&](bool ok2, int e2, float nv) {
-                bstore32(prr, e2, __float_as_uint(nv), ok2);
-            });
-        } else {
-            uint32_t w = pack_c<DT>(v.x, v.y);
-            if constexpr (TWP) w = cmul_w<DT>(w, f);
-            bstore32_at(hout, ob.x + th * estep, tc * estep, w, ok);
-        }
-    };
-    constexpr int MULKL = TWP ? 3 : TW ? 2 : 1;
-    // stage 1 input t of column j: n = j + t R0 R2
-    // (unused inputs read t = R - 1 of the stage: written words)
-    auto iat0 = [&](int base, int tc, int th, bool ok) -> uint32_t {
-        return buf0[base + (ok ? tc + th : R1 - 1) * (R0 * R2 + PD * R2)];
-    };
-    auto ident = [](int, int, int, bool, uint32_t w) -> uint32_t { return w; };
-    if constexpr (!S3) {
-        ATab<DT, R1, THL> at1;
-        at1.load(a.amat[1]);
-        mstage<DT, NTH, L, R1, G, NF, THL, MULKL, false, false, true, SM == ST_POW>(at1, lds_base, iat0, ident,
-                                                                                  gbase, gmul, gput, barrier);
-    } else {
-        // middle stage (NS = R0): output t of column j at q = (j / R0) R0 R1
-        // + j % R0 + t R0, next twiddle w_L^{(j / R0)(j % R0 + t R0)}
-        auto obase1 = [&](int g, int j) -> int4 {
-            const int jh = j / R0, jl = j - jh * R0;
-            return make_int4(g * RP + jh * R0 * R1 + jl + (PD ? jh * R1 : 0), jh * jl, jh * R0, 1);
-        };
-        auto omul1 = [&](const int4 &ob, int tc, int th, bool ok) -> TwT {
-            return tws[ob.y + (ok ? tc + th : 0) * ob.z];
-        };
-        auto oput1 = [&](const int4 &ob, int tc, int th, float2 v, TwT f, bool ok) {
-            uint32_t w = pack_c<DT>(v.x, v.y);
-            if constexpr (PKT) w = cmul_w<DT>(w, f);
-            buf1[ok ? ob.x + (tc + th) * (R0 + PD) : dummy] = w;
-        };
-        ATab<DT, R1, R1> at1;
-        at1.load(a.amat[1]);
-        mstage<DT, NTH, L, R1, G, NF, R1, PKT ? 3 : 2, false, false, false>(at1, lds_base, iat0, ident, obase1,
-                                                                             omul1, oput1, barrier);
-        // stage 2 input t of column j: n = j + t R0 R1
-        auto iat1 = [&](int base, int tc, int th, bool ok) -> uint32_t {
-            return buf1[base + (ok ? tc + th : R2 - 1) * (R0 * R1 + PD * R1)];
-        };
-        ATab<DT, R2, THL> at2;
-        at2.load(a.amat[2]);
-        mstage<DT, NTH, L, R2, G, NF, THL, MULKL, false, false, true, SM == ST_POW>(at2, lds_base, iat1, ident,
-                                                                                  gbase, gmul, gput, barrier);
-    }
+    if (!paired) run_last(std::integral_constant<int, 0>{});
     if constexpr (SM == ST_POW) {
         if (a.p_part) {
             const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
@@ -1066,6 +1160,17 @@ extern "C" int sdr_search_pass_shape(int col, int L, int var, int *shape)
     return 1;
 }
 
+bool h_col_pairs(int dt, int L, int var)
+{
+    // the kernel's rule (mpass_kernel PB): FP16, two stages, G rows of pitch
+    // RP words filling more than 80 KiB of LDS
+    const HSpec *h = col_spec(L, var);
+    if (dt != HDT_F16 || !h || h->r[2] > 1) return false;
+    const int pad = h->r[0] % 2 == 0 ? 1 : 0;
+    const int rp = (L + (pad ? L / h->r[0] : 0)) | 1;
+    return (long long)h->G * rp * 4 > 80 * 1024;
+}
+
 bool h_shape(int M, int M1, int vc, int vr)
 {
     return M1 > 0 && M % M1 == 0 && col_spec(M1, vc) && row_spec(M / M1, vr);
@@ -1296,6 +1401,8 @@ void launch_hpass(int dt, LoadMode lm, StoreMode sm, const PassArgs &a_in, int n
     // the forward row pass does not
     if (sm == ST_CPX && (lm != LD_CPX) != (a_in.tfull != nullptr)) PSA_FAIL("MFMA pass: twiddle table mismatch");
     if (dt == HDT_F16 && lm == LD_PROD && !a_in.tfullh) PSA_FAIL("MFMA pass: FP16 twiddle table missing");
+    if (a_in.hin2 && (sm != ST_POW || !h_col_pairs(dt, a_in.L, a_in.hvar)))
+        PSA_FAIL("MFMA pass: length %d shape %d takes no paired blocks", a_in.L, a_in.hvar);
     PassArgs a = a_in;
     a.carr = dev_carrier();
     if (dt == HDT_F16) hlaunch_dt<HDT_F16>(lm, sm, a, nbatch, st);

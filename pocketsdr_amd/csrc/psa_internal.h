@@ -199,6 +199,7 @@ struct PassArgs {
     // precision form (psa_fft16.hip): a complex value is one 32-bit word of
     // two FP16 or BF16 halves, real part in the low half
     const uint32_t *hin;        // LD_CPX / LD_PROD (X) input words
+    const uint32_t *hin2;       // paired power passes: the next block's input words
     uint32_t *hout;             // ST_CPX output words
     const uint32_t *hcf;        // LD_PROD code-spectrum words
     const uint4 *amat[3];       // MFMA A operands of the stage DFT matrices
@@ -254,6 +255,9 @@ int h_splits(int M, int *m1, int nmax);
 // whether M1 (column length) and M / M1 (row length) have workgroup shapes
 // vc and vr
 bool h_shape(int M, int M1, int vc, int vr);
+// whether the power (inverse column) pass of length L, shape var, takes two
+// blocks' inputs in one launch (PassArgs::hin2)
+bool h_col_pairs(int dt, int L, int var);
 // pass geometry; var selects one of the workgroup shapes of the length
 void h_col_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);
 void h_row_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);

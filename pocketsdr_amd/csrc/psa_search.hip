@@ -552,6 +552,7 @@ struct SigDev {
     int nchunk_fused = 0;       // partials per PRN emitted by the last column pass
     bool fused_peak = false;    // the last column pass emits the peak partials
     bool xcd_order = true;      // XCD-aware workgroup order of the inverse row pass
+    bool pair_blocks = false;   // the power pass of its column shape takes block pairs
     bool cell = false;          // FP16 inverse as fused cell kernels (launch_cells)
     int lane = 0;               // the lane (stream and work buffers) the group runs on
     int bin_chunk = 1, prn_chunk = 1;
@@ -815,7 +816,19 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
         }
         return;
     }
-    for (int k = 0; k < s.h.n_blk; k++) {
+    // blocks in pairs (one power pass per pair, P read and written once)
+    // when the spectra of every block are resident and the power pass takes
+    // two inputs; the pair's intermediates then live in tmp, free after the
+    // forward passes
+    const size_t zc = (size_t)s.bin_chunk * s.prn_chunk * M;    // words of one Z chunk
+    const bool pairs = fwd_all && s.pair_blocks && s.h.n_blk >= 2 && 2 * zc <= 2 * ln.tmp_elems;
+    uint32_t *Z2 = nullptr;
+    if (pairs) {
+        Z = tmp;
+        Z2 = tmp + zc;
+    }
+    for (int k = 0; k < s.h.n_blk; k += pairs ? 2 : 1) {
+        const bool two = pairs && k + 1 < s.h.n_blk;
         for (int b0 = 0; b0 < nbin; b0 += s.bin_chunk) {
             const int nbc = std::min(s.bin_chunk, nbin - b0);
             const long long mo = (long long)k * nbin + b0;
@@ -862,19 +875,28 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
                     Tick t(p, K_INV_ROW, (double)nbatch * M, 4.0 * M * (nbc + nqc + (double)nbatch));
                     launch_hpass(dt, LD_PROD, ST_CPX, ir, nbatch, st);
                 }
+                if (two) {
+                    // the next block's product row pass into Z2
+                    ir.hin = Xk + (long long)nbin * M;
+                    ir.hout = Z2;
+                    Tick t(p, K_INV_ROW, (double)nbatch * M, 4.0 * M * (nbc + nqc + (double)nbatch));
+                    launch_hpass(dt, LD_PROD, ST_CPX, ir, nbatch, st);
+                }
                 PassArgs ic = {};
                 h_col_args(hp, 1, ic, s.hvar_col);
                 ic.kappa = s.kappa;
                 ic.hin = Z; ic.in_bstride = M;
+                ic.hin2 = two ? Z2 : nullptr;
                 ic.P = s.P; ic.p_bstride = N;
                 ic.p_div = nbc; ic.p_off1 = q0; ic.p_mul = nbin; ic.p_off2 = b0;
                 ic.nlag = N;
                 ic.p_nbin = nbin;
                 ic.p_init = k == 0;
-                if (k == s.h.n_blk - 1 && s.fused_peak) ic.p_part = s.part;
+                const int klast = two ? k + 1 : k;
+                if (klast == s.h.n_blk - 1 && s.fused_peak) ic.p_part = s.part;
                 {
-                    Tick t(p, K_INV_COL, (double)nbatch * M,
-                           (double)nbatch * (4.0 * M + (k == 0 ? 4.0 : 8.0) * N));
+                    Tick t(p, K_INV_COL, (double)nbatch * M * (two ? 2 : 1),
+                           (double)nbatch * ((two ? 8.0 : 4.0) * M + (k == 0 ? 4.0 : 8.0) * N));
                     launch_hpass(dt, LD_CPX, ST_POW, ic, nbatch, st);
                 }
             }
@@ -958,6 +980,7 @@ static void h_setup(sdr_search_plan_t *p, SigDev &s, int M1, int vc, int vr)
     s.hp = hfft_plan(M, s.hp->dt, M1);
     s.hvar_col = vc;
     s.hvar_row = vr;
+    s.pair_blocks = h_col_pairs(s.hp->dt, M1, vc) && !getenv("PSA_NO_PAIR");
     const int e = (int)ceil(log2((double)s.h.N));
     const double sc = ldexp(1.0, -e);
     h_code_spec(s.hp, s.cf_nat, s.cfh, nprn, (float)sc, p->st);
@@ -1074,6 +1097,7 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
     else s.pl = fft_plan(s.h.M, bm);
     s.hvar_col = bc;
     s.hvar_row = br;
+    s.pair_blocks = h && h_col_pairs(s.hp->dt, s.hp->M1, bc) && !getenv("PSA_NO_PAIR");
     s.tuned_ms = fixed ? 0.0f : best;
     PSA_CHECK(hipEventDestroy(e0));
     PSA_CHECK(hipEventDestroy(e1));

@@ -97,3 +97,37 @@ def test_l1cp_fp16_vs_fp64(sdr, orc, l1cp, tiles, monkeypatch):
     print("L1CP fp16 (%s) worst peak-power error vs FP64: %.2e" % (tiles, worst))
     p32.close()
     ph.close()
+
+
+@pytest.fixture(scope="module")
+def l1cp3():
+    from oracle import oracle as orc
+    from pocketsdr_amd import workloads
+    return workloads.acq_l1cp(seed=4, n_rf=1, n_blk=3, prn_limit=2, n_vis=1, bin_limit=2, cn0=(40.0, 45.0),
+                              code_fft_fn=lambda c, T, fs, N: orc.gen_code_fft(c, T, 0.0, fs, N, N))
+
+
+@pytest.mark.parametrize("tiles", ["960/0/0", "1000/0/0"])
+def test_l1cp_paired_blocks_bitexact(sdr, l1cp3, tiles, monkeypatch):
+    """The power pass of the 32-column shapes takes two blocks per launch
+    (the first block's powers kept in registers, P read and written once per
+    pair; three blocks: one pair and one single): P, peaks, C/N0 and Doppler
+    bit-identical to one block per launch (PSA_NO_PAIR)."""
+    w = l1cp3
+    monkeypatch.setenv("PSA_TILES", "h960000=" + tiles)
+    monkeypatch.delenv("PSA_NO_PAIR", raising=False)
+    pp = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH)
+    monkeypatch.setenv("PSA_NO_PAIR", "1")
+    p1 = sdr.SearchPlan(w["buffs"], w["sigs"], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH)
+    for rep in range(2):
+        pp.run()
+        p1.run()
+        for i, s in enumerate(w["sigs"]):
+            assert pp.tiles(i)[0] == int(tiles.split("/")[0])
+            for q in range(s["code_fft"].shape[0]):
+                assert np.array_equal(pp.power(i, q), p1.power(i, q)), (i, q)
+            ra, rb = pp.result(i), p1.result(i)
+            for x, y in zip(ra, rb):
+                assert np.array_equal(x, y)
+    pp.close()
+    p1.close()
