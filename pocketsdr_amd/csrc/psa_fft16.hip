@@ -1124,7 +1124,8 @@ __global__ void __launch_bounds__(NTH) cell_kernel(CellArgs a, const uint4 *am_r
     X(480, 16, 30, 1, 16, 512, 0) X(480, 16, 30, 1, 16, 256, 1) \
     X(480, 32, 15, 1, 16, 256, 2) X(480, 12, 40, 1, 16, 256, 3) \
     X(1000, 10, 10, 10, 4, 256, 0) X(1000, 25, 40, 1, 8, 512, 1) \
-    X(960, 32, 30, 1, 8, 256, 0) X(960, 32, 30, 1, 16, 512, 1)
+    X(960, 32, 30, 1, 8, 256, 0) X(960, 32, 30, 1, 16, 512, 1) \
+    X(960, 30, 32, 1, 8, 256, 2) X(960, 32, 30, 1, 4, 256, 3)
 
 struct HSpec { int L, r[3], G, nth, var; };
 static const HSpec h_col_specs[] = {

@@ -58,7 +58,7 @@ def test_l1cp_fp32_vs_oracle(sdr, orc, l1cp):
 # 16-column workgroups paired on one XCD, rows of 1000 as 25 x 40, and
 # columns of 1000 as 25 x 40 (16- and 32-column workgroups) with rows of 960
 # (8- and 16-row workgroups)
-TILINGS = [None, "960/1/1", "960/0/1", "1000/1/0", "1000/0/1", "1000/0/0"]
+TILINGS = [None, "960/1/1", "960/0/1", "1000/1/0", "1000/0/1", "1000/0/0", "1000/0/2", "1000/0/3"]
 
 
 @pytest.mark.parametrize("tiles", TILINGS)
