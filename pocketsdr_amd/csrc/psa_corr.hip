@@ -393,15 +393,25 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
         }
         __syncthreads();
     } else {
-    for (int w = threadIdx.x; w < Nq4; w += NTH) {
+    // phase of sample i in the first segment's terms, p0 + i s, carried
+    // from word to word; the second segment's p1 + (i - nseg) s differs from
+    // it by a constant (uint32 arithmetic, mod 2^32)
+    const uint32_t dseg = q.p1 - q.p0 - (uint32_t)q.nseg * q.s;
+    uint32_t phw = q.p0 + (uint32_t)(4 * (int)threadIdx.x) * q.s;
+    const uint32_t phstep = (uint32_t)(4 * NTH) * q.s;
+    const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(buf), (short)0,
+                                                                         0x7FFFFFFF, 0x00020000);
+    for (int w = threadIdx.x; w < Nq4; w += NTH, phw += phstep) {
         const int i0 = 4 * w;
         uint32_t wi = 0, wq = 0;
         const bool s1 = i0 + 4 <= N && i0 + 4 <= q.nseg, s2 = i0 + 4 <= N && i0 >= q.nseg;
         if (s1 || s2) {
             const int src = s1 ? q.ix + i0 : i0 - q.nseg;
-            uint32_t ph = s1 ? q.p0 + (uint32_t)i0 * q.s : q.p1 + (uint32_t)(i0 - q.nseg) * q.s;
-            const uint32_t *a = reinterpret_cast<const uint32_t *>(buf + (src & ~3));
-            const uint32_t d4 = __builtin_amdgcn_alignbyte(a[1], a[0], (uint32_t)(src & 3));
+            uint32_t ph = s1 ? phw : phw + dseg;
+            const int ab = src & ~3;
+            const uint32_t a0 = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 0, 0);
+            const uint32_t a1 = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 4, 0);
+            const uint32_t d4 = __builtin_amdgcn_alignbyte(a1, a0, (uint32_t)(src & 3));
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const uint32_t d = d4 >> (8 * b);
@@ -410,8 +420,10 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
                 const uint32_t c = carr[ph >> 24];
                 ph += q.s;
                 const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
-                wi |= (uint32_t)(uint8_t)(int8_t)(sI * cI - sQ * cQ) << (8 * b);
-                wq |= (uint32_t)(uint8_t)(int8_t)(sI * cQ + sQ * cI) << (8 * b);
+                // 24-bit products (full-rate multiplies); the low byte is
+                // the int8 wrap of the exact value
+                wi |= ((uint32_t)(__mul24(sI, cI) - __mul24(sQ, cQ)) & 0xFFu) << (8 * b);
+                wq |= ((uint32_t)(__mul24(sI, cQ) + __mul24(sQ, cI)) & 0xFFu) << (8 * b);
             }
         } else {
 #pragma unroll
@@ -461,9 +473,9 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     for (int t0 = nsplit > 1 ? wave % n_pos : wave; t0 < n_pos; t0 += nwave * TPW) {
         int jt[TPW], co[TPW];
         // per tap and lane: the running sum of every word so far (tot), its
-        // value when the step holding sample j began (snap) and the half-1
-        // sum completed in that step (h1); half 2 is then tot - snap (exact
-        // int32 sums)
+        // value through the word straddling sample j (snap) and through the
+        // last word wholly before j (h1, the half-1 sum); half 2 is then
+        // tot - snap (exact int32 sums)
         int tot[TPW][2], snap[TPW][2], h1[TPW][2];
 #pragma unroll
         for (int s = 0; s < TPW; s++) {
@@ -478,7 +490,10 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             for (int c = 0; c < 2; c++) tot[s][c] = snap[s][c] = h1[s][c] = 0;
         }
         // a step covers 256 words: lane l takes words 4 l .. 4 l + 3 of it with
-        // one 16-byte load per tap (planes padded with zero words to Nq4)
+        // one 16-byte load per tap (planes padded with zero words to Nq4):
+        // buffer loads at the tap's scalar byte offset plus 16 l
+        const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint32_t *>(cbase), (short)0, 0x7FFFFFFF, 0x00020000);
         for (int wb = wlo; wb < whi; wb += 256) {
             const int w = wb + 4 * lane;
             const uint4 xI = *reinterpret_cast<const uint4 *>(planeI + w);
@@ -486,10 +501,11 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             uint4 cI[TPW], cQ[TPW];
 #pragma unroll
             for (int s = 0; s < TPW; s++) {
-                // scalar base of the step, lane offset 16 l bytes
-                const uint32_t *cs = cbase + co[s] + wb;
-                cI[s] = load_u4(cs + 4 * lane);
-                cQ[s] = SAMEQ ? cI[s] : load_u4(cs + (QOFF >> 2) + 4 * lane);
+                const int so = 4 * (co[s] + wb);
+                cI[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(crs, 16 * lane, so, 0));
+                cQ[s] = SAMEQ ? cI[s]
+                              : __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(crs, 16 * lane,
+                                                                                                 so + QOFF, 0));
             }
 #pragma unroll
             for (int s = 0; s < TPW; s++) {
@@ -498,26 +514,27 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
                 const uint32_t ci[4] = {cI[s].x, cI[s].y, cI[s].z, cI[s].w};
                 const uint32_t cq[4] = {cQ[s].x, cQ[s].y, cQ[s].z, cQ[s].w};
                 // this step covers samples 4 wb .. 4 wb + 1023 (a scalar test)
-                if (4 * wb + 1024 <= j || 4 * wb > j) {
-                    dot4x4<EXACT>(xi, xq, ci, cq, tot[s][0], tot[s][1]);
-                } else {
-                    // the step holding sample j: words wholly before j end
-                    // half 1, words from j on start half 2, the word
-                    // straddling j is added by the epilogue
-                    snap[s][0] = tot[s][0];
-                    snap[s][1] = tot[s][1];
-                    int bI = 0, bQ = 0;
+                if (!(4 * wb + 1024 <= j || 4 * wb > j)) {
+                    // the step holding sample j: half 1 ends with the words
+                    // wholly before j (h1), half 2 is the final sum minus
+                    // everything up to and including the word straddling j
+                    // (snap), which the epilogue adds on its own
+                    int bI = 0, bQ = 0, sI = 0, sQ = 0;
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         int dI = 0, dQ = 0;
                         dot4x4<EXACT, 1>(xi + v, xq + v, ci + v, cq + v, dI, dQ);
                         const int m = 4 * (w + v);
                         if (m + 4 <= j) { bI += dI; bQ += dQ; }
-                        else if (m >= j) { tot[s][0] += dI; tot[s][1] += dQ; }
+                        else if (m < j) { sI += dI; sQ += dQ; }
                     }
-                    h1[s][0] = snap[s][0] + bI;
-                    h1[s][1] = snap[s][1] + bQ;
+                    h1[s][0] = tot[s][0] + bI;
+                    h1[s][1] = tot[s][1] + bQ;
+                    snap[s][0] = h1[s][0] + sI;
+                    snap[s][1] = h1[s][1] + sQ;
                 }
+                // every step adds all its words to the running sum
+                dot4x4<EXACT>(xi, xq, ci, cq, tot[s][0], tot[s][1]);
             }
         }
         if (nsplit > 1) {
@@ -579,11 +596,19 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             }
             const int eI = EXACT ? sgn8(xI, vI) : xI * vI, eQ = EXACT ? sgn8(xQ, vQ) : xQ * vQ;
             const int fI = xI * vI, fQ = xQ * vQ;
-            const int sg = m < 0 ? 0 : (slot >= 2 ? 1 : -1);
+            // tail samples leave the running sums (-e) and are added in float
+            // one by one below; with N < 120000 every partial sum is an
+            // integer below 2^24 (vector part |x sign(c)| <= 128 per sample,
+            // at most 15 tail products |x c| <= 128^2), so those float adds
+            // are exact and the tail joins the integer sum instead (+f - e),
+            // converted once: the same float
+            const bool itail = N < 120000;
+            const int tI = itail ? fI - eI : -eI, tQ = itail ? fQ - eQ : -eQ;
+            const int aI = m < 0 ? 0 : (slot >= 2 ? eI : tI), aQ = m < 0 ? 0 : (slot >= 2 ? eQ : tQ);
             const bool first = slot == 0 || slot == 2;
-            int p[4] = {h1[s][0] + (first ? sg * eI : 0), h1[s][1] + (first ? sg * eQ : 0),
-                        tot[s][0] - snap[s][0] + (first ? 0 : sg * eI),
-                        tot[s][1] - snap[s][1] + (first ? 0 : sg * eQ)};
+            int p[4] = {h1[s][0] + (first ? aI : 0), h1[s][1] + (first ? aQ : 0),
+                        tot[s][0] - snap[s][0] + (first ? 0 : aI),
+                        tot[s][1] - snap[s][1] + (first ? 0 : aQ)};
             // transposed butterfly over the 4 sums: the lanes with bit 5 (then
             // 4) set keep the upper half of the values, then a plain sum over
             // bits 3 .. 0; value c ends in lanes 16 c .. 16 c + 15
@@ -609,11 +634,11 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             float f1Q = (float)(__builtin_amdgcn_readlane(p[0], 16) + ex.y);
             float f2I = (float)(__builtin_amdgcn_readlane(p[0], 32) + ex.z);
             float f2Q = (float)(__builtin_amdgcn_readlane(p[0], 48) + ex.w);
-            for (int i = 0; i < j - V1; i++) {
+            for (int i = 0; !itail && i < j - V1; i++) {
                 f1I += (float)__builtin_amdgcn_readlane(fI, i);
                 f1Q += (float)__builtin_amdgcn_readlane(fQ, i);
             }
-            for (int i = 0; i < len2 - V2; i++) {
+            for (int i = 0; !itail && i < len2 - V2; i++) {
                 f2I += (float)__builtin_amdgcn_readlane(fI, 16 + i);
                 f2Q += (float)__builtin_amdgcn_readlane(fQ, 16 + i);
             }

@@ -94,6 +94,27 @@ def test_corr_std_bitexact(sdr, orc, N, cpx):
         assert np.array_equal(gC.view(np.uint32), rC.view(np.uint32))
 
 
+@pytest.mark.parametrize("N", [119999, 200000])
+def test_corr_std_large_sums(sdr, orc, N):
+    """IQ matched to the bank (+-112 against +-1 chips), so the prompt's sums
+    reach 112 N: beyond 2^24 at N = 200000, where the float adds of the
+    scalar tail round and must follow the reference's order; N = 119999 is
+    the longest block whose tail joins the integer sums."""
+    rng = np.random.default_rng(N)
+    code = (2 * rng.integers(0, 2, (10 * N, 1)) - 1).astype(np.int8).repeat(2, axis=1)
+    IQ = (112 * code[:N]).astype(np.int8)
+    IQ[rng.integers(0, N, 50)] = rng.integers(-112, 113, (50, 2))
+    pos = np.array([0.0, -11.7, 11.7, 0.5, -3.0, 17.0, 1.25])
+    for coff in (0.0, 0.31, 13.0):
+        got, gC = sdr.corr_std(IQ, code, N, coff, pos)
+        ref, rC = orc.corr_std(IQ, code, N, coff, pos)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (N, coff)
+        assert np.array_equal(gC.view(np.uint32), rC.view(np.uint32))
+        if coff == 0.0:
+            # the prompt's integer sum (corr N 11.2): past 2^24 at N = 200000
+            assert (abs(float(ref[0].real)) * N * 11.2 > 2 ** 24) == (N > 150000)
+
+
 def test_corr_std_dropin_banks(sdr, orc):
     """The drop-in on the tracking correlator: real-code banks with I != Q,
     five banks in turn (more than the per-thread device cache holds, so some
