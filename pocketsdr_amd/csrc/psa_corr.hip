@@ -674,7 +674,7 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
 }
 
 template <bool EXACT, bool SAMEQ>
-__global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
+__device__ __forceinline__ void trk_corr_body(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,
     const TrkEpoch *__restrict__ ep, const double *__restrict__ pos,
     const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride,
@@ -708,6 +708,28 @@ __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(
             corr_out[o * n_pos + t] = make_float2(-v.x, -v.y);
         }
     }
+}
+
+#define PSA_TRK_ARGS                                                                                   \
+    const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,                         \
+        const TrkEpoch *__restrict__ ep, const double *__restrict__ pos,                               \
+        const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride, int NP, \
+        const uint32_t *__restrict__ carr_g, float2 *__restrict__ corr_out, float2 *__restrict__ C_out
+#define PSA_TRK_PASS \
+    buf, buffN, N, n_epoch, n_pos, ep, pos, codes, code_ch_stride, code_copy_stride, NP, carr_g, corr_out, C_out
+
+template <bool EXACT, bool SAMEQ>
+__global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(PSA_TRK_ARGS)
+{
+    trk_corr_body<EXACT, SAMEQ>(PSA_TRK_PASS);
+}
+
+// real codes without -8 nibbles (the common form): compiled for six waves per
+// SIMD (80 registers), so that three workgroups share a CU
+__global__ void __launch_bounds__(TRK_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
+trk_corr_kernel_real(PSA_TRK_ARGS)
+{
+    trk_corr_body<false, true>(PSA_TRK_PASS);
 }
 
 // ===========================================================================
@@ -1218,15 +1240,24 @@ sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch, const sd
 template <bool EXACT, bool SAMEQ>
 static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)
 {
+    const void *kern = (EXACT || !SAMEQ) ? (const void *)trk_corr_kernel<EXACT, SAMEQ>
+                                         : (const void *)trk_corr_kernel_real;
     static std::once_flag once;
-    std::call_once(once, [] {
-        PSA_CHECK(hipFuncSetAttribute((const void *)trk_corr_kernel<EXACT, SAMEQ>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    std::call_once(once, [kern] {
+        PSA_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     });
-    hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ>), dim3(ne), dim3(TRK_THREADS), shm, p->st,
-                       p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
-                       p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
-                       dev_carrier(), p->d_corr, p->d_C);
+    if (EXACT || !SAMEQ) {
+        hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ>), dim3(ne), dim3(TRK_THREADS), shm, p->st,
+                           p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
+                           p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
+                           dev_carrier(), p->d_corr, p->d_C);
+    } else {
+        hipLaunchKernelGGL(trk_corr_kernel_real, dim3(ne), dim3(TRK_THREADS), shm, p->st,
+                           p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
+                           p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
+                           dev_carrier(), p->d_corr, p->d_C);
+    }
+    PSA_CHECK(hipGetLastError());
 }
 
 template <bool EXACT, bool SAMEQ>
