@@ -139,6 +139,18 @@ struct MGeom {
     static constexpr int RP = (L + (PAD ? L / R0 : 0)) | 1;
 };
 
+// Power-pass shapes that take paired blocks: their LDS limits the CU to one
+// 512-thread workgroup (960 and 1000 with 32 columns: up to 256 registers)
+// or to two (400 as 16 x 25 with 32 columns: up to 128 registers), and the
+// kept powers (THL outputs of the last radix per lane and tile) fit those
+// registers (checked with the compiler's counts: 202, 241 and 108)
+constexpr bool pb_shape(int L, int R0, int R1, int R2, int G, int NTH)
+{
+    return R2 == 1 && G == 32 && NTH == 512 &&
+           ((L == 960 && R0 == 32 && R1 == 30) || (L == 1000 && R0 == 25 && R1 == 40) ||
+            (L == 400 && R0 == 16 && R1 == 25));
+}
+
 // One Stockham radix-R stage (NS = product of the earlier radices) over the
 // G sub-transforms of the workgroup.  Column (g, j), j < L / R, reads inputs
 // t < R at j + t L / R; its output t' sits at the Stockham position
@@ -595,13 +607,14 @@ __global__ void __launch_bounds__(NTH) mpass_kernel(PassArgs a, int ngroups)
         return buf0[base + (ok ? tc + th : R1 - 1) * (R0 * R2 + PD * R2)];
     };
     auto ident = [](int, int, int, bool, uint32_t w) -> uint32_t { return w; };
-    // paired blocks (PB): a power pass whose workgroup fills the CU's LDS
-    // (one workgroup per CU: the registers of the kept powers cost no
-    // occupancy) may take two blocks' inputs (hin, hin2) and write P once
+    // paired blocks (PB): a power pass whose LDS already limits the CU to
+    // the workgroups its registers allow with the kept powers added (the
+    // shapes of pb_shape), may take two blocks' inputs (hin, hin2) and
+    // write P once
     constexpr int RLT = S3 ? R2 : R1;
     constexpr int NTL = (G * (L / RLT) + 31) / 32, MTL = (NTL + NTH / 64 - 1) / (NTH / 64);
     constexpr int RTL = (2 * THL + 31) / 32;
-    constexpr bool PB = DT == HDT_F16 && SM == ST_POW && !S3 && G * RP * 4 > 80 * 1024;
+    constexpr bool PB = DT == HDT_F16 && SM == ST_POW && pb_shape(L, R0, R1, R2, G, NTH);
     float keep[PB ? MTL : 1][PB ? RTL * 8 : 1];
     // output tc = 16 r + 4 m + w of a tile: accumulator slot 8 r + 2 m + w
     auto kidx = [](int tc) { return (tc / 16) * 8 + ((tc % 16) / 4) * 2 + (tc % 4); };
@@ -1163,13 +1176,16 @@ extern "C" int sdr_search_pass_shape(int col, int L, int var, int *shape)
 
 bool h_col_pairs(int dt, int L, int var)
 {
-    // the kernel's rule (mpass_kernel PB): FP16, two stages, G rows of pitch
-    // RP words filling more than 80 KiB of LDS
+    // the kernel's rule (mpass_kernel PB)
     const HSpec *h = col_spec(L, var);
-    if (dt != HDT_F16 || !h || h->r[2] > 1) return false;
-    const int pad = h->r[0] % 2 == 0 ? 1 : 0;
-    const int rp = (L + (pad ? L / h->r[0] : 0)) | 1;
-    return (long long)h->G * rp * 4 > 80 * 1024;
+    return dt == HDT_F16 && h && pb_shape(L, h->r[0], h->r[1], h->r[2], h->G, h->nth);
+}
+
+bool h_any_pairs(int dt, int M)
+{
+    for (const HSpec &cs : h_col_specs)
+        if (M % cs.L == 0 && row_spec(M / cs.L, 0) && h_col_pairs(dt, cs.L, cs.var)) return true;
+    return false;
 }
 
 bool h_shape(int M, int M1, int vc, int vr)

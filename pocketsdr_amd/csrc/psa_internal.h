@@ -258,6 +258,8 @@ bool h_shape(int M, int M1, int vc, int vr);
 // whether the power (inverse column) pass of length L, shape var, takes two
 // blocks' inputs in one launch (PassArgs::hin2)
 bool h_col_pairs(int dt, int L, int var);
+// whether some column shape of some split of M takes paired blocks
+bool h_any_pairs(int dt, int M);
 // pass geometry; var selects one of the workgroup shapes of the length
 void h_col_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);
 void h_row_args(const HPlan *hp, int dir, PassArgs &a, int var = 0);

@@ -821,11 +821,12 @@ static void run_batched_h(sdr_search_plan_t *p, SigDev &s)
     // two inputs; the pair's intermediates then live in tmp, free after the
     // forward passes
     const size_t zc = (size_t)s.bin_chunk * s.prn_chunk * M;    // words of one Z chunk
-    const bool pairs = fwd_all && s.pair_blocks && s.h.n_blk >= 2 && 2 * zc <= 2 * ln.tmp_elems;
+    const bool in_tmp = 2 * zc <= 2 * ln.tmp_elems, in_z = 2 * zc <= 2 * ln.z_elems;
+    const bool pairs = fwd_all && s.pair_blocks && s.h.n_blk >= 2 && (in_tmp || in_z);
     uint32_t *Z2 = nullptr;
     if (pairs) {
-        Z = tmp;
-        Z2 = tmp + zc;
+        if (in_tmp) Z = tmp;
+        Z2 = Z + zc;
     }
     for (int k = 0; k < s.h.n_blk; k += pairs ? 2 : 1) {
         const bool two = pairs && k + 1 < s.h.n_blk;
@@ -1309,7 +1310,12 @@ static sdr_search_plan_t *plan_create(int n_buff, const uint8_t *const *data, co
         ln.tmp_elems = std::max(ln.tmp_elems, (nfwd * M + wd - 1) / wd);
         ln.x_elems = std::max(ln.x_elems, (nfwd * M + wd - 1) / wd);
         if (s.cell) continue;
-        ln.z_elems = std::max(ln.z_elems, ((size_t)s.bin_chunk * s.prn_chunk * M + wd - 1) / wd);
+        // room for two blocks' intermediates where the power pass may pair
+        // blocks and the forward scratch (tmp) cannot hold them
+        const size_t zc = (size_t)s.bin_chunk * s.prn_chunk * M;
+        const bool may_pair = s.hp && s.h.n_blk >= 2 && fwd_all_blocks(s) && h_any_pairs(s.hp->dt, s.h.M);
+        const size_t zn = may_pair && 2 * zc > nfwd * M ? 2 * zc : zc;
+        ln.z_elems = std::max(ln.z_elems, (zn + wd - 1) / wd);
     }
     for (auto &ln : p->lanes) {
         PSA_CHECK(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));

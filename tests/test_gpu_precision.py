@@ -141,6 +141,30 @@ def test_every_shape(sdr, orc, multi_weak, name, monkeypatch):
     print("%s: %d shapes" % (name, len(done)))
 
 
+def test_paired_blocks_bitexact(sdr, multi_weak, monkeypatch):
+    """L6D's power pass of 400 (16 x 25, 32 columns) takes both blocks in
+    one launch: P, peaks, C/N0 and Doppler bit-identical to one block per
+    launch (PSA_NO_PAIR)."""
+    w = multi_weak
+    s = [x for x in w["sigs"] if x["name"] == "L6D"][0]
+    assert s["n_blk"] == 2
+    monkeypatch.setenv("PSA_TILES", "h192000=400/0/1")
+    monkeypatch.delenv("PSA_NO_PAIR", raising=False)
+    pp = sdr.SearchPlan(w["buffs"], [s], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH)
+    monkeypatch.setenv("PSA_NO_PAIR", "1")
+    p1 = sdr.SearchPlan(w["buffs"], [s], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH)
+    for rep in range(2):
+        pp.run()
+        p1.run()
+        assert pp.tiles(0) == (400, 480, 0, 1)
+        for q in range(len(s["prns"])):
+            assert np.array_equal(pp.power(0, q), p1.power(0, q)), q
+        for x, y in zip(pp.result(0), p1.result(0)):
+            assert np.array_equal(x, y)
+    pp.close()
+    p1.close()
+
+
 def test_precision_plain_flag_ignored(sdr, multi_weak):
     """PLAIN keeps the FP32 reference dataflow whatever the precision flags."""
     w = multi_weak
