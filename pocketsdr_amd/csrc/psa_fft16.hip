@@ -139,16 +139,18 @@ struct MGeom {
     static constexpr int RP = (L + (PAD ? L / R0 : 0)) | 1;
 };
 
-// Power-pass shapes that take paired blocks: their LDS limits the CU to one
-// 512-thread workgroup (960 and 1000 with 32 columns: up to 256 registers)
-// or to two (400 as 16 x 25 with 32 columns: up to 128 registers), and the
-// kept powers (THL outputs of the last radix per lane and tile) fit those
-// registers (checked with the compiler's counts: 202, 241 and 108)
+// Power-pass shapes that take paired blocks (measured faster): their LDS
+// limits the CU to one 512-thread workgroup (960 and 1000 with 32 columns:
+// up to 256 registers) or to two (400 as 16 x 25 with 32 columns: up to 128
+// registers), and the kept powers (THL outputs of the last radix per lane
+// and tile) fit those registers (the compiler's counts: 202, 241 and 108);
+// 300 as 15 x 20 (three workgroups per CU by LDS) loses one of them to the
+// kept powers and still moves enough fewer bytes to gain 11%
 constexpr bool pb_shape(int L, int R0, int R1, int R2, int G, int NTH)
 {
     return R2 == 1 && G == 32 && NTH == 512 &&
            ((L == 960 && R0 == 32 && R1 == 30) || (L == 1000 && R0 == 25 && R1 == 40) ||
-            (L == 400 && R0 == 16 && R1 == 25));
+            (L == 400 && R0 == 16 && R1 == 25) || (L == 300 && R0 == 15 && R1 == 20));
 }
 
 // One Stockham radix-R stage (NS = product of the earlier radices) over the

@@ -141,14 +141,17 @@ def test_every_shape(sdr, orc, multi_weak, name, monkeypatch):
     print("%s: %d shapes" % (name, len(done)))
 
 
-def test_paired_blocks_bitexact(sdr, multi_weak, monkeypatch):
-    """L6D's power pass of 400 (16 x 25, 32 columns) takes both blocks in
-    one launch: P, peaks, C/N0 and Doppler bit-identical to one block per
+@pytest.mark.parametrize("name,M,tiles", [("L6D", 192000, (400, 0, 1)), ("E1B", 96000, (400, 0, 1)),
+                                          ("L5Q", 48000, (300, 0, 0))])
+def test_paired_blocks_bitexact(sdr, multi_weak, monkeypatch, name, M, tiles):
+    """The power passes of 400 (16 x 25) and 300 (15 x 20) with 32 columns
+    take two blocks per launch (L6D and E1-B: both blocks, L5-Q: five
+    pairs): P, peaks, C/N0 and Doppler bit-identical to one block per
     launch (PSA_NO_PAIR)."""
     w = multi_weak
-    s = [x for x in w["sigs"] if x["name"] == "L6D"][0]
-    assert s["n_blk"] == 2
-    monkeypatch.setenv("PSA_TILES", "h192000=400/0/1")
+    s = [x for x in w["sigs"] if x["name"] == name][0]
+    assert s["n_blk"] >= 2
+    monkeypatch.setenv("PSA_TILES", "h%d=%d/%d/%d" % ((M,) + tiles))
     monkeypatch.delenv("PSA_NO_PAIR", raising=False)
     pp = sdr.SearchPlan(w["buffs"], [s], sdr.SEARCH_FP16 | sdr.SEARCH_GRAPH)
     monkeypatch.setenv("PSA_NO_PAIR", "1")
@@ -156,7 +159,7 @@ def test_paired_blocks_bitexact(sdr, multi_weak, monkeypatch):
     for rep in range(2):
         pp.run()
         p1.run()
-        assert pp.tiles(0) == (400, 480, 0, 1)
+        assert pp.tiles(0) == (tiles[0], M // tiles[0], tiles[1], tiles[2])
         for q in range(len(s["prns"])):
             assert np.array_equal(pp.power(0, q), p1.power(0, q)), q
         for x, y in zip(pp.result(0), p1.result(0)):
