@@ -404,9 +404,9 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
     __shared__ PeakPart red[PASS_THREADS / 64];
     int b, grp;
     // column passes: CG float2 (inverse) or CG IF bytes (mixing) per row
-    // segment, kx workgroups to one 128-byte line
+    // segment, kx workgroups to one 128-byte line (xcd_pair = 0: identity)
     const int segb = LM == LD_MIX ? CG : 8 * CG;
-    const int kx = a.in_nstride != 1 && segb < 128 ? 128 / segb : 1;
+    const int kx = a.xcd_pair && a.in_nstride != 1 && segb < 128 ? 128 / segb : 1;
     if (!block_item(a, LM == LD_PROD, ngroups, &b, &grp, kx)) return;
     const int g0 = grp * CG;
     const int gcount = min(CG, a.nsub - g0);
@@ -739,6 +739,12 @@ void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hi
 {
     PassArgs a = a_in;
     if (!a.carr) a.carr = dev_carrier();
+    // XCD pairing of narrow column groups (PSA_XCD_PAIR=0 turns it off)
+    static const int pair = [] {
+        const char *e = getenv("PSA_XCD_PAIR");
+        return e ? atoi(e) : 1;
+    }();
+    a.xcd_pair = pair;
     RadixSeq rs;
     const RadPlan rp = make_rad_plan(a.L);
     rs.n = rp.n;

@@ -196,6 +196,7 @@ struct PassArgs {
     // b = prn * xcd_nbin + bin, prn < xcd_nprn; the PRNs of one (row group,
     // bin) tile run back to back on one XCD
     int xcd_nbin, xcd_nprn;
+    int xcd_pair;               // FP32 column passes: XCD pairing of narrow column groups
     // precision form (psa_fft16.hip): a complex value is one 32-bit word of
     // two FP16 or BF16 halves, real part in the low half
     const uint32_t *hin;        // LD_CPX / LD_PROD (X) input words
