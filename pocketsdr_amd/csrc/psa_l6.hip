@@ -299,6 +299,7 @@ struct sdr_l6_plan_tag {
     uint8_t *d_sym_e = nullptr;
     float2 *d_taps_e = nullptr;
     hipStream_t st = nullptr;
+    hipGraphExec_t gexec = nullptr;     // the closed loop's launches, captured once
     bool timing = false;
     struct Ev { int cls; hipEvent_t a, b; };
     std::vector<Ev> evs;
@@ -587,6 +588,17 @@ static void l6_loop_launch(sdr_l6_plan_t *p, int e)
     PSA_CHECK(hipGetLastError());
 }
 
+// the closed loop's launches: the initial loop update, then per epoch the
+// FFT passes and CSK of the epoch's batch and the loop update
+static void l6_loop_body(sdr_l6_plan_t *p)
+{
+    l6_loop_launch(p, -1);
+    for (int e = 0; e < p->n_epoch; e++) {
+        l6_batch(p, (long long)e * p->n_ch, p->n_ch, p->d_sym_e, p->d_taps_e);
+        l6_loop_launch(p, e);
+    }
+}
+
 static void l6_run(sdr_l6_plan_t *p)
 {
     hipStream_t st = p->st;
@@ -594,11 +606,24 @@ static void l6_run(sdr_l6_plan_t *p)
         // every run starts from the channels' initial states
         PSA_CHECK(hipMemcpyAsync(p->d_lch, p->lch0.data(), sizeof(TrkLoopCh) * p->n_ch, hipMemcpyHostToDevice,
                                  st));
-        l6_loop_launch(p, -1);
-        for (int e = 0; e < p->n_epoch; e++) {
-            l6_batch(p, (long long)e * p->n_ch, p->n_ch, p->d_sym_e, p->d_taps_e);
-            l6_loop_launch(p, e);
+        // the 5 n_epoch + 1 launches (every argument fixed at plan creation,
+        // the loop state in device memory) replayed as one hipGraph, so that
+        // no host enqueue sits between the small per-epoch kernels; timed
+        // runs launch them one by one (PSA_L6_NO_GRAPH=1 too)
+        static const bool no_graph = getenv("PSA_L6_NO_GRAPH") != nullptr;
+        if (p->timing || no_graph) {
+            l6_loop_body(p);
+            return;
         }
+        if (!p->gexec) {
+            hipGraph_t g;
+            PSA_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            l6_loop_body(p);
+            PSA_CHECK(hipStreamEndCapture(st, &g));
+            PSA_CHECK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+            PSA_CHECK(hipGraphDestroy(g));
+        }
+        PSA_CHECK(hipGraphLaunch(p->gexec, st));
         return;
     }
     {
@@ -692,6 +717,7 @@ void sdr_l6_plan_free(sdr_l6_plan_t *p)
     (void)hipFree(p->d_sym); (void)hipFree(p->d_taps);
     (void)hipFree(p->d_lch); (void)hipFree(p->d_lout); (void)hipFree(p->d_sym_e); (void)hipFree(p->d_taps_e);
     for (auto &e : p->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     (void)hipStreamDestroy(p->st);
     delete p;
 }

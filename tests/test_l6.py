@@ -318,14 +318,21 @@ def test_l6_loop_rejects_bad_loop_parameters():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("fs,n_ch,n_epoch", [(6e6, 3, 500), (24e6, 10, 300)])
-def test_l6_loop_vs_oracle(sdr, orc, fs, n_ch, n_epoch):
+def test_l6_loop_vs_oracle(sdr, orc, fs, n_ch, n_epoch, monkeypatch):
     w = l6_loop_case(orc, fs=fs, n_ch=n_ch, n_epoch=n_epoch, n_vis=2 if n_ch == 3 else 4)
     plan = sdr.L6Plan(w["buff"], w["chans"], w["pos"], w["n_epoch"], loops=w["loops"])
     plan.run()
     st = plan.state()
     syms, taps = plan.result()
-    plan.run()                                        # a rerun starts from the initial states
-    assert np.array_equal(plan.state(), st)
+    # reruns (replays of the captured loop) start from the initial states:
+    # the same states, symbols and taps every time, and the same as a timed
+    # run, which launches the kernels one by one
+    for rep in range(3):
+        plan.timing(rep == 1)
+        plan.run()
+        assert np.array_equal(plan.state(), st), rep
+        s2, t2 = plan.result()
+        assert np.array_equal(s2, syms) and np.array_equal(t2.view(np.uint32), taps.view(np.uint32)), rep
     plan.close()
     seen = set()
     for c, ch in enumerate(w["chans"]):
