@@ -117,7 +117,7 @@ FORM_FLAGS = {"fp32": 16, "fp16": 4 | 16, "bf16": 8 | 16}    # batched forms, tu
 # and workgroup shape timed over 3 x 5 steps), fixed so that repeated runs
 # and their counter profiles launch the same kernels; groups without an
 # entry are timed by the plan tuner (SDR_SEARCH_TUNE) at plan creation
-TILES = {"multi": "h24000=150/0/0,h96000=400/0/1,h48000=300/0/0,h192000=400/0/1,"
+TILES = {"multi": "h24000=150/0/0,h96000=400/0/1,h48000=300/0/0,h192000=400/0/2,"
                   "f24000=150,f96000=200,f48000=200,f192000=200",
          "l1ca": "h24000=150/1/0,f24000=150",
          "l1cp": "h960000=1000/0/0"}

@@ -1047,7 +1047,7 @@ static bool fixed_tiles(const SigDev &s, int *m1, int *vc, int *vr)
 }
 
 // SDR_SEARCH_TUNE: time every candidate tiling of the group's passes on its
-// own shapes - the first block of the search, best of two runs after a
+// own shapes - the first two blocks of the search, best of two runs after a
 // warm-up - and keep the fastest.  Precision form: (split, column shape, row
 // shape); FP32: the split.
 static void tune_group(sdr_search_plan_t *p, SigDev &s)
@@ -1069,7 +1069,9 @@ static void tune_group(sdr_search_plan_t *p, SigDev &s)
         PSA_CHECK(hipEventDestroy(e1));
         return;
     }
-    s.h.n_blk = 1;
+    // the first two blocks (one pair for the shapes whose power pass pairs
+    // blocks)
+    s.h.n_blk = std::min(nblk, 2);
     for (int i = 0; i < (fixed ? 0 : ns); i++) {
         for (int vc = 0; vc < nv; vc++) {
             for (int vr = 0; vr < nv; vr++) {
