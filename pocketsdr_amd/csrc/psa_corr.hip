@@ -324,9 +324,22 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
 #define TRK_TPW 6
 
 // LDS working set of one (channel, epoch) correlation
+// A carrier word (int8 cI in byte 0, cQ in byte 1) as one v_dot4 operand,
+// the int8 lanes (cI, cQ, kI, kQ) with kI = 8 cI + 7 cQ and kQ = 8 (cI + cQ)
+// modulo 256.  A sample with uI = sI + 8, uQ = sQ + 8 and vQ = 7 - sQ (all
+// 0 .. 15) gives sI cI - sQ cQ as the product with (uI, vQ, -1, 0) and
+// sI cQ + sQ cI as the product with (uQ, uI, 0, -1), modulo 256 (only the
+// low byte, the int8 wrap, is kept).
+__device__ __forceinline__ uint32_t carr_dot_entry(uint32_t c)
+{
+    const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+    return (uint32_t)(uint8_t)cI | ((uint32_t)(uint8_t)cQ << 8) | ((uint32_t)(uint8_t)(8 * cI + 7 * cQ) << 16) |
+           ((uint32_t)(uint8_t)(8 * (cI + cQ)) << 24);
+}
+
 struct TrkLds {
     uint32_t *planeI, *planeQ;   // mixed samples, Nq4 words each
-    const uint32_t *carr;        // carrier table
+    const uint32_t *carr;        // carrier table as v_dot4 operands
     float4 *c12s;                // per tap: half-1 / half-2 sums (I, Q)
     int2 *jk;                    // per tap: code offset j, bank k
     int *sgn;                    // the data-sign decision of the taps 0-2
@@ -412,19 +425,28 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             const uint32_t a0 = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 0, 0);
             const uint32_t a1 = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 4, 0);
             const uint32_t d4 = __builtin_amdgcn_alignbyte(a1, a0, (uint32_t)(src & 3));
+            // per byte: uI = sI + 8, uQ = sQ + 8 and vQ = 7 - sQ (0 .. 15)
+            const uint32_t uI = (d4 & 0x0F0F0F0Fu) ^ 0x08080808u;
+            const uint32_t uQ = ((d4 >> 4) & 0x0F0F0F0Fu) ^ 0x08080808u;
+            const uint32_t vQ = 0x0F0F0F0Fu - uQ;
+            int vi[4], vq[4];
 #pragma unroll
             for (int b = 0; b < 4; b++) {
-                const uint32_t d = d4 >> (8 * b);
-                const int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
-                const int sQ = (int)((int8_t)(uint8_t)d) >> 4;
-                const uint32_t c = carr[ph >> 24];
+                // (uI_b, vQ_b, -1, 0) and (uQ_b, uI_b, 0, -1)
+                const int x1 = (int)__builtin_amdgcn_perm(vQ, uI, 0x0C0D0400u | (0x0101u * b));
+                const int x2 = (int)__builtin_amdgcn_perm(uQ, uI, 0x0D0C0004u | (0x0101u * b));
+                const int c = (int)carr[ph >> 24];
                 ph += q.s;
-                const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
-                // 24-bit products (full-rate multiplies); the low byte is
-                // the int8 wrap of the exact value
-                wi |= ((uint32_t)(__mul24(sI, cI) - __mul24(sQ, cQ)) & 0xFFu) << (8 * b);
-                wq |= ((uint32_t)(__mul24(sI, cQ) + __mul24(sQ, cI)) & 0xFFu) << (8 * b);
+                vi[b] = __builtin_amdgcn_sdot4(x1, c, 0, false);
+                vq[b] = __builtin_amdgcn_sdot4(x2, c, 0, false);
             }
+            // byte 0 of each product into byte b of the word
+            wi = __builtin_amdgcn_perm(__builtin_amdgcn_perm((uint32_t)vi[3], (uint32_t)vi[2], 0x0C0C0400u),
+                                       __builtin_amdgcn_perm((uint32_t)vi[1], (uint32_t)vi[0], 0x0C0C0400u),
+                                       0x05040100u);
+            wq = __builtin_amdgcn_perm(__builtin_amdgcn_perm((uint32_t)vq[3], (uint32_t)vq[2], 0x0C0C0400u),
+                                       __builtin_amdgcn_perm((uint32_t)vq[1], (uint32_t)vq[0], 0x0C0C0400u),
+                                       0x05040100u);
         } else {
 #pragma unroll
             for (int b = 0; b < 4; b++) {
@@ -437,8 +459,8 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
                     uint32_t d = buf[src];
                     int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
                     int sQ = (int)((int8_t)(uint8_t)d) >> 4;
-                    uint32_t c = carr[ph >> 24];
-                    int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+                    const uint32_t c = carr[ph >> 24];
+                    const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
                     wi |= (uint32_t)(uint8_t)(int8_t)(sI * cI - sQ * cQ) << (8 * b);
                     wq |= (uint32_t)(uint8_t)(int8_t)(sI * cQ + sQ * cI) << (8 * b);
                 }
@@ -697,7 +719,7 @@ __device__ __forceinline__ void trk_corr_body(
         if (threadIdx.x < 2) C_out[2 * o + threadIdx.x] = make_float2(0.0f, 0.0f);
         return;
     }
-    for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_g[i];
+    for (int i = threadIdx.x; i < 256; i += TRK_THREADS) carr[i] = carr_dot_entry(carr_g[i]);
     const TrkLds L{lds, lds + Nq4, carr, c12s, jk, &sgn, psum};
     trk_correlate<EXACT, SAMEQ>(q, buf, N, n_pos, pos, codes + (long long)ch * code_ch_stride,
                                 code_copy_stride, NP, L, corr_out + o * n_pos, C_out + 2 * o);
@@ -770,7 +792,7 @@ __global__ void __launch_bounds__(NTH) trk_loop_kernel(
     __shared__ TrkLoopCh S;
     const int c = blockIdx.x;
     const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
-    for (int i = threadIdx.x; i < 256; i += NTH) carr[i] = carr_g[i];
+    for (int i = threadIdx.x; i < 256; i += NTH) carr[i] = carr_dot_entry(carr_g[i]);
     if (threadIdx.x == 0) {
         S = chs[c];
         stop = S.idle;
