@@ -51,17 +51,47 @@ __device__ __forceinline__ uint4 load_u4(const uint32_t *p)
     return v;
 }
 
-__device__ __forceinline__ int wave_sum(int v)
+// v + the value of lane (lane ^ 1), (lane ^ 2), 7 - lane or 15 - lane within
+// a row of 16 (DPP quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror); the whole wave must be active
+template <int CTRL>
+__device__ __forceinline__ int dpp_add(int v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    return v + __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+
+// Sums of four int32 values over the 64 lanes of a (fully active) wave:
+// a0 .. a3's totals end in every lane of rows 0 .. 3 (lanes 16 c .. 16 c + 15).
+// v_permlane32_swap trades the upper half of its first operand for the lower
+// half of its second, so the two outputs' sum holds a0 over both halves in
+// lanes 0-31 and a2 in lanes 32-63; v_permlane16_swap does the same with
+// rows 1, 3 of the first and rows 0, 2 of the second; four DPP adds then sum
+// each row.  Integer sums: any order gives the same result.
+__device__ __forceinline__ int wave_sum4_rows(int a0, int a1, int a2, int a3)
+{
+    const auto s02 = __builtin_amdgcn_permlane32_swap((unsigned)a0, (unsigned)a2, false, false);
+    const auto s13 = __builtin_amdgcn_permlane32_swap((unsigned)a1, (unsigned)a3, false, false);
+    const int k0 = (int)(s02[0] + s02[1]), k1 = (int)(s13[0] + s13[1]);
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)k0, (unsigned)k1, false, false);
+    int v = (int)(r[0] + r[1]);
+    v = dpp_add<0xB1>(v);
+    v = dpp_add<0x4E>(v);
+    v = dpp_add<0x141>(v);
+    return dpp_add<0x140>(v);
+}
+
+// the four totals of wave_sum4_rows as one int4 (uniform)
+__device__ __forceinline__ int4 wave_sum4(int a0, int a1, int a2, int a3)
+{
+    const int v = wave_sum4_rows(a0, a1, a2, a3);
+    return make_int4(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16),
+                     __builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48));
 }
 
 // block-wide sum of 4 ints (256 threads); result valid in all threads
 __device__ __forceinline__ int4 block_sum4(int4 v, int4 *red)
 {
-    v.x = wave_sum(v.x); v.y = wave_sum(v.y); v.z = wave_sum(v.z); v.w = wave_sum(v.w);
+    v = wave_sum4(v.x, v.y, v.z, v.w);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     __syncthreads();
     if (l == 0) red[w] = v;
@@ -414,60 +444,79 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
     const uint32_t phstep = (uint32_t)(4 * NTH) * q.s;
     const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(buf), (short)0,
                                                                          0x7FFFFFFF, 0x00020000);
-    for (int w = threadIdx.x; w < Nq4; w += NTH, phw += phstep) {
-        const int i0 = 4 * w;
-        uint32_t wi = 0, wq = 0;
-        const bool s1 = i0 + 4 <= N && i0 + 4 <= q.nseg, s2 = i0 + 4 <= N && i0 >= q.nseg;
-        if (s1 || s2) {
-            const int src = s1 ? q.ix + i0 : i0 - q.nseg;
-            uint32_t ph = s1 ? phw : phw + dseg;
-            const int ab = src & ~3;
-            const uint32_t a0 = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 0, 0);
-            const uint32_t a1 = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 4, 0);
-            const uint32_t d4 = __builtin_amdgcn_alignbyte(a1, a0, (uint32_t)(src & 3));
-            // per byte: uI = sI + 8, uQ = sQ + 8 and vQ = 7 - sQ (0 .. 15)
-            const uint32_t uI = (d4 & 0x0F0F0F0Fu) ^ 0x08080808u;
-            const uint32_t uQ = ((d4 >> 4) & 0x0F0F0F0Fu) ^ 0x08080808u;
-            const uint32_t vQ = 0x0F0F0F0Fu - uQ;
-            int vi[4], vq[4];
+    // MU words per thread at a time: their IF loads are all in flight before
+    // the first is mixed (the closed loop's single workgroup per CU would
+    // otherwise wait out one load latency per word).  Whether a word lies
+    // below Nq4 is the same for a whole wave (Nq4 and NTH are multiples of
+    // 64), so those tests are scalar branches.
+    constexpr int MU = 6;
+    for (int w0 = threadIdx.x; w0 < Nq4; w0 += MU * NTH) {
+        uint32_t A0[MU], A1[MU];
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                // (uI_b, vQ_b, -1, 0) and (uQ_b, uI_b, 0, -1)
-                const int x1 = (int)__builtin_amdgcn_perm(vQ, uI, 0x0C0D0400u | (0x0101u * b));
-                const int x2 = (int)__builtin_amdgcn_perm(uQ, uI, 0x0D0C0004u | (0x0101u * b));
-                const int c = (int)carr[ph >> 24];
-                ph += q.s;
-                vi[b] = __builtin_amdgcn_sdot4(x1, c, 0, false);
-                vq[b] = __builtin_amdgcn_sdot4(x2, c, 0, false);
-            }
-            // byte 0 of each product into byte b of the word
-            wi = __builtin_amdgcn_perm(__builtin_amdgcn_perm((uint32_t)vi[3], (uint32_t)vi[2], 0x0C0C0400u),
-                                       __builtin_amdgcn_perm((uint32_t)vi[1], (uint32_t)vi[0], 0x0C0C0400u),
-                                       0x05040100u);
-            wq = __builtin_amdgcn_perm(__builtin_amdgcn_perm((uint32_t)vq[3], (uint32_t)vq[2], 0x0C0C0400u),
-                                       __builtin_amdgcn_perm((uint32_t)vq[1], (uint32_t)vq[0], 0x0C0C0400u),
-                                       0x05040100u);
-        } else {
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                int i = i0 + b;
-                if (i < N) {
-                    int src;
-                    uint32_t ph;
-                    if (i < q.nseg) { src = q.ix + i; ph = q.p0 + (uint32_t)i * q.s; }
-                    else { src = i - q.nseg; ph = q.p1 + (uint32_t)(i - q.nseg) * q.s; }
-                    uint32_t d = buf[src];
-                    int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
-                    int sQ = (int)((int8_t)(uint8_t)d) >> 4;
-                    const uint32_t c = carr[ph >> 24];
-                    const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
-                    wi |= (uint32_t)(uint8_t)(int8_t)(sI * cI - sQ * cQ) << (8 * b);
-                    wq |= (uint32_t)(uint8_t)(int8_t)(sI * cQ + sQ * cI) << (8 * b);
-                }
+        for (int u = 0; u < MU; u++) {
+            const int w = w0 + u * NTH, i0 = 4 * w;
+            if (w < Nq4) {
+                const bool s1 = i0 + 4 <= N && i0 + 4 <= q.nseg, s2 = i0 + 4 <= N && i0 >= q.nseg;
+                // words that go sample by sample load the ring's first dword pair
+                const int ab = (s1 ? q.ix + i0 : s2 ? i0 - q.nseg : 0) & ~3;
+                A0[u] = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 0, 0);
+                A1[u] = __builtin_amdgcn_raw_buffer_load_b32(brs, ab, 4, 0);
             }
         }
-        planeI[w] = wi;
-        planeQ[w] = wq;
+#pragma unroll
+        for (int u = 0; u < MU; u++, phw += phstep) {
+            const int w = w0 + u * NTH, i0 = 4 * w;
+            if (w >= Nq4) break;
+            uint32_t wi = 0, wq = 0;
+            const bool s1 = i0 + 4 <= N && i0 + 4 <= q.nseg, s2 = i0 + 4 <= N && i0 >= q.nseg;
+            if (s1 || s2) {
+                const int src = s1 ? q.ix + i0 : i0 - q.nseg;
+                uint32_t ph = s1 ? phw : phw + dseg;
+                const uint32_t d4 = __builtin_amdgcn_alignbyte(A1[u], A0[u], (uint32_t)(src & 3));
+                // per byte: uI = sI + 8, uQ = sQ + 8 and vQ = 7 - sQ (0 .. 15)
+                const uint32_t uI = (d4 & 0x0F0F0F0Fu) ^ 0x08080808u;
+                const uint32_t uQ = ((d4 >> 4) & 0x0F0F0F0Fu) ^ 0x08080808u;
+                const uint32_t vQ = 0x0F0F0F0Fu - uQ;
+                int vi[4], vq[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    // (uI_b, vQ_b, -1, 0) and (uQ_b, uI_b, 0, -1)
+                    const int x1 = (int)__builtin_amdgcn_perm(vQ, uI, 0x0C0D0400u | (0x0101u * b));
+                    const int x2 = (int)__builtin_amdgcn_perm(uQ, uI, 0x0D0C0004u | (0x0101u * b));
+                    const int c = (int)carr[ph >> 24];
+                    ph += q.s;
+                    vi[b] = __builtin_amdgcn_sdot4(x1, c, 0, false);
+                    vq[b] = __builtin_amdgcn_sdot4(x2, c, 0, false);
+                }
+                // byte 0 of each product into byte b of the word
+                wi = __builtin_amdgcn_perm(__builtin_amdgcn_perm((uint32_t)vi[3], (uint32_t)vi[2], 0x0C0C0400u),
+                                           __builtin_amdgcn_perm((uint32_t)vi[1], (uint32_t)vi[0], 0x0C0C0400u),
+                                           0x05040100u);
+                wq = __builtin_amdgcn_perm(__builtin_amdgcn_perm((uint32_t)vq[3], (uint32_t)vq[2], 0x0C0C0400u),
+                                           __builtin_amdgcn_perm((uint32_t)vq[1], (uint32_t)vq[0], 0x0C0C0400u),
+                                           0x05040100u);
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    int i = i0 + b;
+                    if (i < N) {
+                        int src;
+                        uint32_t ph;
+                        if (i < q.nseg) { src = q.ix + i; ph = q.p0 + (uint32_t)i * q.s; }
+                        else { src = i - q.nseg; ph = q.p1 + (uint32_t)(i - q.nseg) * q.s; }
+                        uint32_t d = buf[src];
+                        int sI = (int)((int8_t)(uint8_t)(d << 4)) >> 4;
+                        int sQ = (int)((int8_t)(uint8_t)d) >> 4;
+                        const uint32_t c = carr[ph >> 24];
+                        const int cI = (int8_t)(c & 0xFF), cQ = (int8_t)(c >> 8);
+                        wi |= (uint32_t)(uint8_t)(int8_t)(sI * cI - sQ * cQ) << (8 * b);
+                        wq |= (uint32_t)(uint8_t)(int8_t)(sI * cQ + sQ * cI) << (8 * b);
+                    }
+                }
+            }
+            planeI[w] = wi;
+            planeQ[w] = wq;
+        }
     }
     __syncthreads();
     }
@@ -516,19 +565,12 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
         // buffer loads at the tap's scalar byte offset plus 16 l
         const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<uint32_t *>(cbase), (short)0, 0x7FFFFFFF, 0x00020000);
-        for (int wb = wlo; wb < whi; wb += 256) {
+        // one step: this lane's 4 words of the planes against each tap's
+        // code words cI / cQ
+        auto step = [&](const int wb, const uint4 *cI, const uint4 *cQ) {
             const int w = wb + 4 * lane;
             const uint4 xI = *reinterpret_cast<const uint4 *>(planeI + w);
             const uint4 xQ = *reinterpret_cast<const uint4 *>(planeQ + w);
-            uint4 cI[TPW], cQ[TPW];
-#pragma unroll
-            for (int s = 0; s < TPW; s++) {
-                const int so = 4 * (co[s] + wb);
-                cI[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(crs, 16 * lane, so, 0));
-                cQ[s] = SAMEQ ? cI[s]
-                              : __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(crs, 16 * lane,
-                                                                                                 so + QOFF, 0));
-            }
 #pragma unroll
             for (int s = 0; s < TPW; s++) {
                 const int j = jt[s];
@@ -558,6 +600,38 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
                 // every step adds all its words to the running sum
                 dot4x4<EXACT>(xi, xq, ci, cq, tot[s][0], tot[s][1]);
             }
+        };
+        auto load_codes = [&](const int wb, uint4 *cI, uint4 *cQ) {
+#pragma unroll
+            for (int s = 0; s < TPW; s++) {
+                const int so = 4 * (co[s] + wb);
+                cI[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(crs, 16 * lane, so, 0));
+                cQ[s] = SAMEQ ? cI[s]
+                              : __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(crs, 16 * lane,
+                                                                                                 so + QOFF, 0));
+            }
+        };
+        if constexpr (TPW == 1) {
+            // one tap per wave (the closed loop): the code words of KB steps
+            // are in flight together, one load latency per KB steps
+            constexpr int KB = 4;
+            for (int wb0 = wlo; wb0 < whi; wb0 += KB * 256) {
+                uint4 cI[KB], cQ[KB];
+#pragma unroll
+                for (int k = 0; k < KB; k++) {
+                    if (wb0 + k * 256 < whi) load_codes(wb0 + k * 256, &cI[k], &cQ[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < KB; k++) {
+                    if (wb0 + k * 256 < whi) step(wb0 + k * 256, &cI[k], &cQ[k]);
+                }
+            }
+        } else {
+            for (int wb = wlo; wb < whi; wb += 256) {
+                uint4 cI[TPW], cQ[TPW];
+                load_codes(wb, cI, cQ);
+                step(wb, cI, cQ);
+            }
         }
         if (nsplit > 1) {
             // a part wholly before the step holding j is all half 1
@@ -570,13 +644,8 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             }
             if (part > 0) {
                 // this part's four sums, reduced over the lanes, to LDS
-                int p[4] = {h1[0][0], h1[0][1], tot[0][0] - snap[0][0], tot[0][1] - snap[0][1]};
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-#pragma unroll
-                    for (int o = 32; o >= 1; o >>= 1) p[c] += __shfl_xor(p[c], o, 64);
-                }
-                if (lane == 0) L.psum[wave] = make_int4(p[0], p[1], p[2], p[3]);
+                const int4 p = wave_sum4(h1[0][0], h1[0][1], tot[0][0] - snap[0][0], tot[0][1] - snap[0][1]);
+                if (lane == 0) L.psum[wave] = p;
             }
             __syncthreads();
             if (part > 0) continue;
@@ -631,31 +700,16 @@ __device__ __forceinline__ void trk_correlate(const TrkEpoch &q, const uint8_t *
             int p[4] = {h1[s][0] + (first ? aI : 0), h1[s][1] + (first ? aQ : 0),
                         tot[s][0] - snap[s][0] + (first ? 0 : aI),
                         tot[s][1] - snap[s][1] + (first ? 0 : aQ)};
-            // transposed butterfly over the 4 sums: the lanes with bit 5 (then
-            // 4) set keep the upper half of the values, then a plain sum over
-            // bits 3 .. 0; value c ends in lanes 16 c .. 16 c + 15
-            {
-                const bool up = (lane & 32) != 0;
-                const int k0 = up ? p[2] : p[0], k1 = up ? p[3] : p[1];
-                const int s0 = up ? p[0] : p[2], s1 = up ? p[1] : p[3];
-                p[0] = k0 + __shfl_xor(s0, 32, 64);
-                p[1] = k1 + __shfl_xor(s1, 32, 64);
-            }
-            {
-                const bool up = (lane & 16) != 0;
-                p[0] = (up ? p[1] : p[0]) + __shfl_xor(up ? p[0] : p[1], 16, 64);
-            }
-#pragma unroll
-            for (int o = 8; o >= 1; o >>= 1) p[0] += __shfl_xor(p[0], o, 64);
+            const int4 r = wave_sum4(p[0], p[1], p[2], p[3]);
             int4 ex = make_int4(0, 0, 0, 0);    // the other parts' sums (split)
             for (int q = 1; q < nsplit; q++) {
                 const int4 v = L.psum[t + q * n_pos];
                 ex.x += v.x; ex.y += v.y; ex.z += v.z; ex.w += v.w;
             }
-            float f1I = (float)(__builtin_amdgcn_readlane(p[0], 0) + ex.x);
-            float f1Q = (float)(__builtin_amdgcn_readlane(p[0], 16) + ex.y);
-            float f2I = (float)(__builtin_amdgcn_readlane(p[0], 32) + ex.z);
-            float f2Q = (float)(__builtin_amdgcn_readlane(p[0], 48) + ex.w);
+            float f1I = (float)(r.x + ex.x);
+            float f1Q = (float)(r.y + ex.y);
+            float f2I = (float)(r.z + ex.z);
+            float f2Q = (float)(r.w + ex.w);
             for (int i = 0; !itail && i < j - V1; i++) {
                 f1I += (float)__builtin_amdgcn_readlane(fI, i);
                 f1Q += (float)__builtin_amdgcn_readlane(fQ, i);
