@@ -21,6 +21,52 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 __device__ __forceinline__ float2 mul_i(float2 a, float sgn) { return make_float2(-sgn * a.y, sgn * a.x); }
 
 // ---------------------------------------------------------------------------
+// wave reduction of a tracked peak (value bm at index bi, -1 = none) and a
+// sum bs: the larger value wins, the smaller index on ties (a commutative
+// rule, so a lane and its partner reach the same record).  The whole wave
+// must be active.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void peak_take(float &bm, long long &bi, float om, long long oi)
+{
+    if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
+}
+
+// partner within a row of 16 by DPP control CTRL (quad_perm [1,0,3,2] and
+// [2,3,0,1], row_half_mirror, row_mirror): after the four steps every lane
+// of a row holds the row's record
+template <int CTRL>
+__device__ __forceinline__ void peak_dpp_step(float &bm, long long &bi, double &bs)
+{
+    auto mv = [](unsigned v) { return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true); };
+    const unsigned long long ui = (unsigned long long)bi, us = (unsigned long long)__double_as_longlong(bs);
+    const float om = __uint_as_float(mv(__float_as_uint(bm)));
+    const long long oi = (long long)(((unsigned long long)mv((unsigned)(ui >> 32)) << 32) | mv((unsigned)ui));
+    bs += __longlong_as_double((long long)(((unsigned long long)mv((unsigned)(us >> 32)) << 32) | mv((unsigned)us)));
+    peak_take(bm, bi, om, oi);
+}
+
+// rows 0, 1 with rows 2, 3 (W32 = false: row 0 with 1 and 2 with 3; true:
+// lanes 0-31 with 32-63) by v_permlane16_swap / v_permlane32_swap of a value
+// with itself: the two outputs hold the lower and the upper group's value in
+// every lane of both groups
+template <bool W32>
+__device__ __forceinline__ void peak_swap_step(float &bm, long long &bi, double &bs)
+{
+    auto sw = [](unsigned v) {
+        if constexpr (W32) return __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        else return __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    };
+    const unsigned long long ui = (unsigned long long)bi, us = (unsigned long long)__double_as_longlong(bs);
+    const auto m = sw(__float_as_uint(bm)), ih = sw((unsigned)(ui >> 32)), il = sw((unsigned)ui);
+    const auto sh = sw((unsigned)(us >> 32)), sl = sw((unsigned)us);
+    bm = __uint_as_float(m[0]);
+    bi = (long long)(((unsigned long long)ih[0] << 32) | il[0]);
+    bs = __longlong_as_double((long long)(((unsigned long long)sh[0] << 32) | sl[0])) +
+         __longlong_as_double((long long)(((unsigned long long)sh[1] << 32) | sl[1]));
+    peak_take(bm, bi, __uint_as_float(m[1]), (long long)(((unsigned long long)ih[1] << 32) | il[1]));
+}
+
+// ---------------------------------------------------------------------------
 // element loaders / storers of the pass kernels
 // ---------------------------------------------------------------------------
 // The product of sdr_cpx_mul (src/sdr_func.c:197-233) times its scale s:
@@ -231,13 +277,13 @@ struct Storer {
         bs += (double)fsum;
         fsum = 0.0f;
         long long bi = be >= 0 ? ibase + be : -1;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const float om = __shfl_xor(bm, o, 64);
-            const long long oi = __shfl_xor(bi, o, 64);
-            bs += __shfl_xor(bs, o, 64);
-            if (oi >= 0 && (bi < 0 || om > bm || (om == bm && oi < bi))) { bm = om; bi = oi; }
-        }
+        // within rows of 16 lanes: partners by DPP; then across rows
+        peak_dpp_step<0xB1>(bm, bi, bs);
+        peak_dpp_step<0x4E>(bm, bi, bs);
+        peak_dpp_step<0x141>(bm, bi, bs);
+        peak_dpp_step<0x140>(bm, bi, bs);
+        peak_swap_step<false>(bm, bi, bs);
+        peak_swap_step<true>(bm, bi, bs);
         const int w = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) { red[w].m = bm; red[w].i = bi; red[w].s = bs; }
         __syncthreads();
