@@ -377,11 +377,13 @@ __device__ __forceinline__ void first_stage(const PassArgs &a, const Loader<LM> 
 
 // one item of the specialised pass: first stage from global, middle stages
 // in LDS, last stage to global
+// (tw: the length-CL twiddle table, staged in LDS by the caller before the
+// first stage's closing barrier)
 template <int LM, int SM, int CL, int CG>
 __device__ __forceinline__ void pass_item(const PassArgs &a, int b, int g0, int gcount, float2 *buf,
-                                          const uint32_t *carr, float sgn, PeakPart *red, long long pslot)
+                                          const uint32_t *carr, float sgn, PeakPart *red, long long pslot,
+                                          const float2 *tw)
 {
-    const float2 *tw = a.tw;
     constexpr int R0 = make_rad_plan(CL).r[0];
     Loader<LM> ld;
     ld.init(a, b, carr);
@@ -400,6 +402,7 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
 {
     static_assert(make_rad_plan(CL).n >= 2, "specialised lengths need at least two radix stages");
     __shared__ float2 buf[CG * (CL + 1)];
+    __shared__ float2 twl[CL];          // the stages' twiddles W_CL^i
     __shared__ uint32_t carr[512];      // carrier table, then a raw ring's table
     __shared__ PeakPart red[PASS_THREADS / 64];
     int b, grp;
@@ -424,7 +427,8 @@ __global__ void __launch_bounds__(PASS_THREADS) pass_kernel_ct(PassArgs a, int n
         const long long row = (long long)(b / a.p_div + a.p_off1) * a.p_mul + a.p_off2 + b % a.p_div;
         pslot = row * ngroups + grp;
     }
-    pass_item<LM, SM, CL, CG>(a, b, g0, gcount, buf, carr, sgn, red, pslot);
+    for (int i = threadIdx.x; i < CL; i += PASS_THREADS) twl[i] = a.tw[i];
+    pass_item<LM, SM, CL, CG>(a, b, g0, gcount, buf, carr, sgn, red, pslot, twl);
 }
 
 // Fused correlator row pass for one code spectrum per item (the FFT
@@ -442,16 +446,22 @@ __global__ void __launch_bounds__(PASS_THREADS) xcorr_row_kernel(PassArgs a, int
     constexpr int LP = CL + 1;
     constexpr int RL = P.r[P.n - 1];
     __shared__ float2 buf[CG * LP];
+    __shared__ float2 twl[CL];          // forward, then inverse twiddles W_CL^(-/+i)
     const int b = (int)(blockIdx.x / (unsigned)ngroups);
     const int grp = (int)blockIdx.x - b * ngroups;
     const int g0 = grp * CG;
     const int gcount = min(CG, a.nsub - g0);
+    for (int i = threadIdx.x; i < CL; i += PASS_THREADS) twl[i] = a.tw[i];
     // forward transforms, natural order in LDS
     Loader<LD_CPX> ld;
     ld.init(a, b, nullptr);
     first_stage<LD_CPX, CL, CG>(a, ld, g0, gcount, buf, -1.0f);
-    middle_stages<CL, CG, 1, P.r[0]>(buf, a.tw, -1.0f);
-    stage_inplace<RL, CL, CG, CL / RL>(buf, a.tw, -1.0f);
+    middle_stages<CL, CG, 1, P.r[0]>(buf, twl, -1.0f);
+    stage_inplace<RL, CL, CG, CL / RL>(buf, twl, -1.0f);
+    // (the last forward stage ends with a barrier: the inverse table
+    // replaces the forward one during the product, whose closing barrier
+    // precedes the inverse stages)
+    for (int i = threadIdx.x; i < CL; i += PASS_THREADS) twl[i] = a.tw_inv[i];
     // X * Cf at transposed index (g0 + g) M2 + k
     {
         const long long cb = a.cf_index ? a.cf_index[b] : b / a.prod_cdiv;
@@ -469,10 +479,10 @@ __global__ void __launch_bounds__(PASS_THREADS) xcorr_row_kernel(PassArgs a, int
         __syncthreads();
     }
     // inverse transforms: stages 0 .. n-2 in LDS, the last one to global
-    middle_stages<CL, CG, 0, 1>(buf, a.tw_inv, 1.0f);
+    middle_stages<CL, CG, 0, 1>(buf, twl, 1.0f);
     Storer<ST_CPX> st;
     st.init(a, b);
-    last_stage<ST_CPX, CL, CG>(a, st, g0, gcount, buf, 1.0f, a.tw_inv);
+    last_stage<ST_CPX, CL, CG>(a, st, g0, gcount, buf, 1.0f, twl);
 }
 
 // ---------------------------------------------------------------------------
