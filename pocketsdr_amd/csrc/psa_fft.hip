@@ -643,7 +643,7 @@ size_t fft_plan_tables(int M, FFTPlan *p, float2 *h, size_t cap, float2 *dev)
 // about 4096 complex per workgroup.
 #define PSA_SPECS(X) \
     X(150, 16) X(160, 16) X(200, 16) X(240, 16) X(300, 16) X(320, 12) \
-    X(400, 16) X(480, 8) X(1000, 8) X(960, 4)
+    X(400, 16) X(480, 4) X(1000, 8) X(960, 2)
 
 bool spec_length(int L);
 
