@@ -739,7 +739,19 @@ bool launch_xcorr_row(const PassArgs &a_in, int nbatch, hipStream_t st)
     if (a.in_nstride != 1 || a.out_nstride != 1 || a.in_gstride != a.L || a.out_gstride != a.L) {
         PSA_FAIL("fused correlator row pass: rows must be contiguous");
     }
-#define PSA_X(L_, G_) if (a.L == L_) { a.G = G_; launch_xrow_t<L_, G_>(a, nbatch, st); return true; }
+    // small batches (the closed loop's one epoch of a few channels): half
+    // the rows per workgroup, so that the launch still spreads over the CUs
+#define PSA_X(L_, G_)                                                                          \
+    if (a.L == L_) {                                                                           \
+        if (G_ >= 2 && (long long)((a.nsub + G_ - 1) / G_) * nbatch < 2048) {                  \
+            a.G = (G_ + 1) / 2;                                                                \
+            launch_xrow_t<L_, (G_ + 1) / 2>(a, nbatch, st);                                    \
+        } else {                                                                               \
+            a.G = G_;                                                                          \
+            launch_xrow_t<L_, G_>(a, nbatch, st);                                              \
+        }                                                                                      \
+        return true;                                                                           \
+    }
     PSA_SPECS(PSA_X)
 #undef PSA_X
     return false;
