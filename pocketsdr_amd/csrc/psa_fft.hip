@@ -772,7 +772,20 @@ void launch_pass(LoadMode lm, StoreMode sm, const PassArgs &a_in, int nbatch, hi
     rs.n = rp.n;
     for (int i = 0; i < 16; i++) rs.r[i] = rp.r[i];
     if ((long long)a.M * 2 > 0x7FFFFFFFLL) PSA_FAIL("FFT length too large");
-#define PSA_L(L_, G_) if (a.L == L_ && a.G == G_) { launch_modes<L_, G_>(lm, sm, a, rs, nbatch, st); return; }
+    // small batches (the closed loop's one epoch of a few channels): half
+    // the sub-transforms per workgroup, so that the launch still spreads
+    // over the CUs; not for power passes, whose peak partials are laid out
+    // by the plan's group count
+#define PSA_L(L_, G_)                                                                          \
+    if (a.L == L_ && a.G == G_) {                                                              \
+        if (G_ >= 2 && sm != ST_POW && (long long)((a.nsub + G_ - 1) / G_) * nbatch < 2048) {  \
+            a.G = (G_ + 1) / 2;                                                                \
+            launch_modes<L_, (G_ + 1) / 2>(lm, sm, a, rs, nbatch, st);                         \
+        } else {                                                                               \
+            launch_modes<L_, G_>(lm, sm, a, rs, nbatch, st);                                   \
+        }                                                                                      \
+        return;                                                                                \
+    }
     PSA_SPECS(PSA_L)
 #undef PSA_L
     launch_modes<0, 0>(lm, sm, a, rs, nbatch, st);
