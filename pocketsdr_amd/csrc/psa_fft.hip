@@ -642,7 +642,7 @@ size_t fft_plan_tables(int M, FFTPlan *p, float2 *h, size_t cap, float2 *dev)
 // (one 128-byte segment per row; 8 for 1000 to fit LDS), row lengths keep
 // about 4096 complex per workgroup.
 #define PSA_SPECS(X) \
-    X(150, 16) X(160, 16) X(200, 8) X(240, 16) X(300, 16) X(320, 12) \
+    X(150, 8) X(160, 8) X(200, 8) X(240, 8) X(300, 16) X(320, 12) \
     X(400, 16) X(480, 4) X(1000, 8) X(960, 2)
 
 bool spec_length(int L);
