@@ -357,11 +357,22 @@ template <int NRF>
 __global__ void __launch_bounds__(256) arch_combine_kernel(ArchArgs a)
 {
     const int m = blockIdx.y;
-    int wr[NRF], wi[NRF];
+    // A sample's nibbles enter as u = nibble ^ 8 = value + 8 (0 .. 15), two
+    // int16 lanes (uI, uQ) per channel: v_dot2 with (wr, -wi) gives
+    // uI wr - uQ wi and with (wi, wr) uI wi + uQ wr; the offsets' share,
+    // 8 sum (wr - wi) and 8 sum (wi + wr), is the accumulators' start value
+    // (exact integers throughout).
+    typedef short v2s __attribute__((ext_vector_type(2)));
+    v2s w1[NRF], w2[NRF];
+    int kI = 0, kQ = 0;
 #pragma unroll
     for (int j = 0; j < NRF; j++) {
-        wr[j] = __builtin_amdgcn_readfirstlane((int)a.w[m][j][0]);
-        wi[j] = __builtin_amdgcn_readfirstlane((int)a.w[m][j][1]);
+        const int wr = __builtin_amdgcn_readfirstlane((int)a.w[m][j][0]);
+        const int wi = __builtin_amdgcn_readfirstlane((int)a.w[m][j][1]);
+        w1[j] = v2s{(short)wr, (short)-wi};
+        w2[j] = v2s{(short)wi, (short)wr};
+        kI -= 8 * (wr - wi);
+        kQ -= 8 * (wi + wr);
     }
     uint8_t *out = a.out[m];
     const long long stride = (long long)gridDim.x * 256 * 16;
@@ -378,17 +389,29 @@ __global__ void __launch_bounds__(256) arch_combine_kernel(ArchArgs a)
                 for (int q = 0; q < nq; q++) x[j][q >> 2] |= (uint32_t)a.in[j][s0 + q] << (8 * (q & 3));
             }
         }
+        // per byte: the I nibble ^ 8 (lo) and the Q nibble ^ 8 (hi)
+        uint32_t lo[NRF][4], hi[NRF][4];
+#pragma unroll
+        for (int j = 0; j < NRF; j++) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                lo[j][k] = (x[j][k] & 0x0F0F0F0Fu) ^ 0x08080808u;
+                hi[j][k] = ((x[j][k] >> 4) & 0x0F0F0F0Fu) ^ 0x08080808u;
+            }
+        }
         uint32_t o[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            int32_t I = 0, Q = 0;
+            int32_t I = kI, Q = kQ;
+            const int b = q & 3;
 #pragma unroll
             for (int j = 0; j < NRF; j++) {
-                // sign-extended nibbles of the cpx8 byte (bit-field extracts)
-                const int bi = __builtin_amdgcn_sbfe((int)x[j][q >> 2], 8 * (q & 3), 4);
-                const int bq = __builtin_amdgcn_sbfe((int)x[j][q >> 2], 8 * (q & 3) + 4, 4);
-                I += bi * wr[j] - bq * wi[j];
-                Q += bi * wi[j] + bq * wr[j];
+                // int16 lanes (uI, uQ) of sample q: byte b of lo, byte b of hi
+                const uint32_t u = __builtin_amdgcn_perm(hi[j][q >> 2], lo[j][q >> 2],
+                                                         0x0C040C00u + (uint32_t)b + ((uint32_t)b << 16));
+                const v2s uv = __builtin_bit_cast(v2s, u);
+                I = __builtin_amdgcn_sdot2(uv, w1[j], I, false);
+                Q = __builtin_amdgcn_sdot2(uv, w2[j], Q, false);
             }
             // round half away from zero, divide by the Q8 scale (C division
             // truncates), clip to the cpx8 range
