@@ -756,7 +756,7 @@ __device__ __forceinline__ void trk_corr_body(
     const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride,
     int NP,  // padded bank length in bytes (multiple of 4)
     const uint32_t *__restrict__ carr_g,
-    float2 *__restrict__ corr_out, float2 *__restrict__ C_out)
+    float2 *__restrict__ corr_out, float2 *__restrict__ C_out, int e0, int nec)
 {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t carr[256];
@@ -765,7 +765,8 @@ __device__ __forceinline__ void trk_corr_body(
     __shared__ int sgn;
     __shared__ int4 psum[TRK_THREADS / 64];
     const int Nq4 = ((N + 3) / 4 + 255) / 256 * 256;
-    const long long o = blockIdx.x;          // channel * n_epoch + epoch
+    // the grid covers epochs e0 .. e0 + nec - 1 of every channel
+    const long long o = (long long)(blockIdx.x / (unsigned)nec) * n_epoch + e0 + blockIdx.x % (unsigned)nec;
     const int ch = (int)(o / n_epoch);
     const TrkEpoch q = ep[o];
     if (q.idle) {                            // uniform over the workgroup
@@ -790,9 +791,11 @@ __device__ __forceinline__ void trk_corr_body(
     const uint8_t *__restrict__ buf, int buffN, int N, int n_epoch, int n_pos,                         \
         const TrkEpoch *__restrict__ ep, const double *__restrict__ pos,                               \
         const uint32_t *__restrict__ codes, long long code_ch_stride, long long code_copy_stride, int NP, \
-        const uint32_t *__restrict__ carr_g, float2 *__restrict__ corr_out, float2 *__restrict__ C_out
+        const uint32_t *__restrict__ carr_g, float2 *__restrict__ corr_out, float2 *__restrict__ C_out, int e0, \
+        int nec
 #define PSA_TRK_PASS \
-    buf, buffN, N, n_epoch, n_pos, ep, pos, codes, code_ch_stride, code_copy_stride, NP, carr_g, corr_out, C_out
+    buf, buffN, N, n_epoch, n_pos, ep, pos, codes, code_ch_stride, code_copy_stride, NP, carr_g, corr_out, C_out, \
+        e0, nec
 
 template <bool EXACT, bool SAMEQ>
 __global__ void __launch_bounds__(TRK_THREADS) trk_corr_kernel(PSA_TRK_ARGS)
@@ -832,7 +835,7 @@ __global__ void __launch_bounds__(NTH) trk_loop_kernel(
     const double *__restrict__ pos, const uint32_t *__restrict__ codes, long long code_ch_stride,
     long long code_copy_stride, int NP, const uint32_t *__restrict__ carr_g,
     TrkLoopCh *__restrict__ chs, TrkLoopOut *__restrict__ out, TrkEpoch *__restrict__ ep,
-    const int8_t *__restrict__ sec_all)
+    const int8_t *__restrict__ sec_all, int e0, int e1, float2 *__restrict__ ghist)
 {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t carr[256];
@@ -858,14 +861,17 @@ __global__ void __launch_bounds__(NTH) trk_loop_kernel(
     // code; trk->P starts at zero, src/sdr_ch.c:369)
     float2 *hist = reinterpret_cast<float2 *>(lds + 2 * Nq4);
     const bool use_hist = S.len_sec >= 2;
+    // epochs e0 .. e1 - 1 of a run: the prompt history continues from the
+    // previous launch's (kept in ghist)
     if (use_hist) {
-        for (int i = threadIdx.x; i < TL_N_HIST; i += NTH) hist[i] = make_float2(0.0f, 0.0f);
+        for (int i = threadIdx.x; i < TL_N_HIST; i += NTH)
+            hist[i] = e0 == 0 ? make_float2(0.0f, 0.0f) : ghist[(long long)c * TL_N_HIST + i];
     }
     const int8_t *sec = sec_all + (use_hist ? S.sec_off : 0);
     const int nlt = S.bump_step > 0.0 ? 6 : TL_NTAP;       // the loop's taps
     __syncthreads();
-    int e = 0;
-    for (; e < n_epoch; e++) {
+    int e = e0;
+    for (; e < e1; e++) {
         if (stop) break;                       // written by lane 0 before a barrier
         if (threadIdx.x == 0) tl_advance(S, N, buffN, qs, hist);
         __syncthreads();
@@ -883,7 +889,7 @@ __global__ void __launch_bounds__(NTH) trk_loop_kernel(
         __syncthreads();
     }
     // epochs after a loss of lock: marked idle, state as at the loss
-    for (int e2 = e + threadIdx.x; e2 < n_epoch; e2 += NTH) {
+    for (int e2 = e + threadIdx.x; e2 < e1; e2 += NTH) {
         TrkLoopOut r;
         r.fd = S.fd; r.coff = S.coff; r.phi = S.phi; r.cn0 = S.cn0;
         r.lock = S.lock; r.idle = 1;
@@ -892,6 +898,9 @@ __global__ void __launch_bounds__(NTH) trk_loop_kernel(
         TrkEpoch qi = {};
         qi.idle = 1;
         ep[(long long)c * n_epoch + e2] = qi;
+    }
+    if (use_hist && e1 < n_epoch) {
+        for (int i = threadIdx.x; i < TL_N_HIST; i += NTH) ghist[(long long)c * TL_N_HIST + i] = hist[i];
     }
     if (threadIdx.x == 0) chs[c] = S;
 }
@@ -934,7 +943,12 @@ struct sdr_trk_plan_tag {
     TrkLoopCh *d_lch = nullptr;
     TrkLoopOut *d_lout = nullptr;
     int8_t *d_sec = nullptr;                   // secondary codes of the loop channels
+    float2 *d_hist = nullptr;                  // their prompt histories between launches
     hipStream_t st = nullptr;
+    // closed loop: the all-tap pass of each chunk of epochs runs on st2
+    // while the loop continues on st
+    hipStream_t st2 = nullptr;
+    std::vector<hipEvent_t> evc;
     bool timing = false;
     hipEvent_t ev[4];
     double ms_state = 0, ms_corr = 0;
@@ -1308,14 +1322,24 @@ sdr_trk_plan_t *sdr_trk_plan_new_loop(const sdr_buff_t *buff, int n_ch, const sd
     }
     PSA_CHECK(hipMalloc(&p->d_lch, sizeof(TrkLoopCh) * n_ch));
     PSA_CHECK(hipMalloc(&p->d_lout, sizeof(TrkLoopOut) * (size_t)n_ch * n_epoch));
+    // up to 8 chunks of at least 32 epochs (the all-tap pass of the last one
+    // is what the run waits for after the loop)
+    const int nchunk = std::max(1, std::min(8, n_epoch / 32));
+    if (nchunk > 1) {
+        if (n_sec) PSA_CHECK(hipMalloc(&p->d_hist, sizeof(float2) * TL_N_HIST * (size_t)n_ch));
+    }
+    PSA_CHECK(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
+    p->evc.resize(nchunk + 1);
+    for (auto &e : p->evc) PSA_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return p;
 }
 
 }  // extern "C"
 
 template <bool EXACT, bool SAMEQ>
-static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)
+static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, hipStream_t st, int e0, int nec)
 {
+    const unsigned ne = (unsigned)((size_t)p->n_ch * nec);
     const void *kern = (EXACT || !SAMEQ) ? (const void *)trk_corr_kernel<EXACT, SAMEQ>
                                          : (const void *)trk_corr_kernel_real;
     static std::once_flag once;
@@ -1323,21 +1347,21 @@ static void trk_launch_t(sdr_trk_plan_t *p, size_t shm, unsigned ne)
         PSA_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     });
     if (EXACT || !SAMEQ) {
-        hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ>), dim3(ne), dim3(TRK_THREADS), shm, p->st,
+        hipLaunchKernelGGL((trk_corr_kernel<EXACT, SAMEQ>), dim3(ne), dim3(TRK_THREADS), shm, st,
                            p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
                            p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
-                           dev_carrier(), p->d_corr, p->d_C);
+                           dev_carrier(), p->d_corr, p->d_C, e0, nec);
     } else {
-        hipLaunchKernelGGL(trk_corr_kernel_real, dim3(ne), dim3(TRK_THREADS), shm, p->st,
+        hipLaunchKernelGGL(trk_corr_kernel_real, dim3(ne), dim3(TRK_THREADS), shm, st,
                            p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_ep, p->d_pos,
                            p->d_codes, p->code_ch_stride, p->code_copy_stride, p->NP,
-                           dev_carrier(), p->d_corr, p->d_C);
+                           dev_carrier(), p->d_corr, p->d_C, e0, nec);
     }
     PSA_CHECK(hipGetLastError());
 }
 
 template <bool EXACT, bool SAMEQ>
-static void trk_loop_launch_t(sdr_trk_plan_t *p, size_t shm)
+static void trk_loop_launch_t(sdr_trk_plan_t *p, size_t shm, int e0, int e1)
 {
     static std::once_flag once;
     constexpr int NTH = trk_loop_threads<EXACT, SAMEQ>();
@@ -1348,23 +1372,24 @@ static void trk_loop_launch_t(sdr_trk_plan_t *p, size_t shm)
     hipLaunchKernelGGL((trk_loop_kernel<EXACT, SAMEQ, NTH>), dim3(p->n_ch), dim3(NTH), shm, p->st,
                        p->d_buf, p->buffN, p->N, p->n_epoch, p->n_pos, p->d_pos, p->d_codes,
                        p->code_ch_stride, p->code_copy_stride, p->NP, dev_carrier(), p->d_lch, p->d_lout,
-                       p->d_ep, p->d_sec);
+                       p->d_ep, p->d_sec, e0, e1, p->d_hist);
 }
 
-static void trk_loop_launch(sdr_trk_plan_t *p, size_t shm)
+static void trk_loop_launch(sdr_trk_plan_t *p, size_t shm, int e0, int e1)
 {
-    if (p->exact && p->sameq) trk_loop_launch_t<true, true>(p, shm);
-    else if (p->exact) trk_loop_launch_t<true, false>(p, shm);
-    else if (p->sameq) trk_loop_launch_t<false, true>(p, shm);
-    else trk_loop_launch_t<false, false>(p, shm);
+    if (p->exact && p->sameq) trk_loop_launch_t<true, true>(p, shm, e0, e1);
+    else if (p->exact) trk_loop_launch_t<true, false>(p, shm, e0, e1);
+    else if (p->sameq) trk_loop_launch_t<false, true>(p, shm, e0, e1);
+    else trk_loop_launch_t<false, false>(p, shm, e0, e1);
 }
 
-static void trk_launch(sdr_trk_plan_t *p, size_t shm, unsigned ne)
+// the all-tap pass over epochs e0 .. e0 + nec - 1 of every channel, on st
+static void trk_launch(sdr_trk_plan_t *p, size_t shm, hipStream_t st, int e0, int nec)
 {
-    if (p->exact && p->sameq) trk_launch_t<true, true>(p, shm, ne);
-    else if (p->exact) trk_launch_t<true, false>(p, shm, ne);
-    else if (p->sameq) trk_launch_t<false, true>(p, shm, ne);
-    else trk_launch_t<false, false>(p, shm, ne);
+    if (p->exact && p->sameq) trk_launch_t<true, true>(p, shm, st, e0, nec);
+    else if (p->exact) trk_launch_t<true, false>(p, shm, st, e0, nec);
+    else if (p->sameq) trk_launch_t<false, true>(p, shm, st, e0, nec);
+    else trk_launch_t<false, false>(p, shm, st, e0, nec);
 }
 
 extern "C" {
@@ -1380,11 +1405,24 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
                                  st));
         if (p->timing) PSA_CHECK(hipEventRecord(p->ev[0], st));
         const size_t shm = sizeof(uint32_t) * 2 * (size_t)(((p->N + 3) / 4 + 255) / 256 * 256);
-        // (the loop kernel keeps the prompt histories after the planes)
-        trk_loop_launch(p, shm + (p->d_sec ? sizeof(float2) * TL_N_HIST : 0));
-        PSA_CHECK(hipGetLastError());
-        trk_launch(p, shm, (unsigned)ne);          // all taps of every epoch
-        PSA_CHECK(hipGetLastError());
+        // The loop runs in chunks of epochs on st (each launch continues from
+        // the channel states and prompt histories the previous one left);
+        // the all-tap pass of a chunk (one workgroup per channel and epoch)
+        // runs on st2 once its chunk's epoch records exist, on the CUs the
+        // loop's few workgroups leave free, while the loop goes on.
+        const int nchunk = (int)p->evc.size() - 1;   // (the last event joins st2 to st)
+        const int csz = (p->n_epoch + nchunk - 1) / nchunk;
+        for (int k = 0, e0 = 0; e0 < p->n_epoch; k++, e0 += csz) {
+            const int e1 = std::min(p->n_epoch, e0 + csz);
+            // (the loop kernel keeps the prompt histories after the planes)
+            trk_loop_launch(p, shm + (p->d_sec ? sizeof(float2) * TL_N_HIST : 0), e0, e1);
+            PSA_CHECK(hipGetLastError());
+            PSA_CHECK(hipEventRecord(p->evc[k], st));
+            PSA_CHECK(hipStreamWaitEvent(p->st2, p->evc[k], 0));
+            trk_launch(p, shm, p->st2, e0, e1 - e0);   // all taps of the chunk's epochs
+        }
+        PSA_CHECK(hipEventRecord(p->evc[nchunk], p->st2));
+        PSA_CHECK(hipStreamWaitEvent(st, p->evc[nchunk], 0));
         if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
         PSA_CHECK(hipStreamSynchronize(st));
         if (p->timing) {
@@ -1400,7 +1438,7 @@ int sdr_trk_plan_run(sdr_trk_plan_t *p)
     PSA_CHECK(hipGetLastError());
     if (p->timing) PSA_CHECK(hipEventRecord(p->ev[1], st));
     const size_t shm = sizeof(uint32_t) * 2 * (size_t)(((p->N + 3) / 4 + 255) / 256 * 256);
-    trk_launch(p, shm, (unsigned)ne);
+    trk_launch(p, shm, st, 0, p->n_epoch);
     PSA_CHECK(hipGetLastError());
     hipLaunchKernelGGL(trk_stitch_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, p->d_C,
                        (long long)ne, p->n_epoch, p->N, p->d_P);
@@ -1489,8 +1527,10 @@ void sdr_trk_plan_free(sdr_trk_plan_t *p)
     (void)hipFree(p->d_buf); (void)hipFree(p->d_codes); (void)hipFree(p->d_ch);
     (void)hipFree(p->d_pos); (void)hipFree(p->d_ep);
     (void)hipFree(p->d_corr); (void)hipFree(p->d_C); (void)hipFree(p->d_P);
-    (void)hipFree(p->d_lch); (void)hipFree(p->d_lout); (void)hipFree(p->d_sec);
+    (void)hipFree(p->d_lch); (void)hipFree(p->d_lout); (void)hipFree(p->d_sec); (void)hipFree(p->d_hist);
     for (auto &e : p->ev) (void)hipEventDestroy(e);
+    for (auto &e : p->evc) (void)hipEventDestroy(e);
+    if (p->st2) (void)hipStreamDestroy(p->st2);
     (void)hipStreamDestroy(p->st);
     delete p;
 }
