@@ -51,7 +51,6 @@ def time_plan(plan, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-SPEC_G = {150: 16, 160: 16, 200: 16, 240: 16, 300: 16, 320: 12, 400: 16, 480: 8, 1000: 8, 960: 4}
 MODES = {"fwd_col": (2, 0), "fwd_row": (0, 0), "inv_row": (3, 0), "inv_col": (0, 1)}
 
 
@@ -69,11 +68,11 @@ def kernel_instance(cls, M, form="fp32", tiles=None):
         return CELL[tuple(tiles[:2])]
     if cls not in MODES:
         return cls
+    from pocketsdr_amd import sdr_func
     if form in ("fp16", "bf16") and tiles:
         M1, M2, vc, vr = tiles
         col = cls in ("fwd_col", "inv_col")
         L, v = (M1, vc) if col else (M2, vr)
-        from pocketsdr_amd import sdr_func
         r0, r1, r2, G, nth = sdr_func.pass_shape(col, L, v)
         lm, sm = MODES[cls]
         return "mpass_kernel<%d, %d, %d, %d, %d, %d, %d, %d, %s, %d>" % (
@@ -88,8 +87,9 @@ def kernel_instance(cls, M, form="fp32", tiles=None):
         M2 = M // M1
     L = M1 if cls in ("fwd_col", "inv_col") else M2
     lm, sm = MODES[cls]
-    if L in SPEC_G:
-        return "pass_kernel_ct<%d, %d, %d, %d>" % (lm, sm, L, SPEC_G[L])
+    g = sdr_func.fft_pass_groups(L)
+    if g:
+        return "pass_kernel_ct<%d, %d, %d, %d>" % (lm, sm, L, g)
     return "pass_kernel_rt<%d, %d>" % (lm, sm)
 
 

@@ -207,6 +207,10 @@ int sdr_search_plan_tiles(const sdr_search_plan_t *plan, int sig, int *tiles);
  * unused), G sub-transforms and threads per workgroup.  Returns 0 when the
  * length has no such shape. */
 int sdr_search_pass_shape(int col, int L, int var, int *shape);
+/* Sub-transforms per workgroup of the FP32 passes' compiled kernel of length
+ * L (0 when L runs on the runtime-length kernel); launches of fewer than
+ * 2048 workgroups other than power passes take half as many. */
+int sdr_fft_pass_groups(int L);
 /* Enables per-kernel HIP-event timing of later runs (0 disables). */
 void sdr_search_plan_timing(sdr_search_plan_t *plan, int on);
 /* Average device time (ms) and launch count of kernel class `name`

@@ -796,6 +796,11 @@ bool spec_length(int L)
     return spec_groups(L) > 0;
 }
 
+extern "C" int sdr_fft_pass_groups(int L)
+{
+    return spec_groups(L);
+}
+
 int fft_splits(int M, int *m1, int nmax)
 {
     int d1, d2, n = 0;

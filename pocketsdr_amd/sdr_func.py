@@ -96,6 +96,7 @@ _decl("sdr_search_plan_power", C.c_int, [c_void_p, C.c_int, C.c_int, c_float_p])
 _decl("sdr_search_plan_form", C.c_int, [c_void_p, C.c_int])
 _decl("sdr_search_plan_tiles", C.c_int, [c_void_p, C.c_int, c_int_p])
 _decl("sdr_search_pass_shape", C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p])
+_decl("sdr_fft_pass_groups", C.c_int, [C.c_int])
 _decl("sdr_search_plan_timing", None, [c_void_p, C.c_int])
 _decl("sdr_search_plan_kernel_ms", C.c_int, [c_void_p, C.c_char_p, c_double_p,
                                              C.POINTER(C.c_longlong), c_double_p])
@@ -291,6 +292,12 @@ def pass_shape(col, L, var):
     if not lib.sdr_search_pass_shape(1 if col else 0, L, var, C.cast(t, c_int_p)):
         return None
     return tuple(t)
+
+
+def fft_pass_groups(L):
+    """Sub-transforms per workgroup of the FP32 passes' compiled kernel of
+    length L (0: the runtime-length kernel)."""
+    return lib.sdr_fft_pass_groups(L)
 
 
 def search_plan_bytes():
