@@ -739,11 +739,16 @@ bool launch_xcorr_row(const PassArgs &a_in, int nbatch, hipStream_t st)
     if (a.in_nstride != 1 || a.out_nstride != 1 || a.in_gstride != a.L || a.out_gstride != a.L) {
         PSA_FAIL("fused correlator row pass: rows must be contiguous");
     }
-    // small batches (the closed loop's one epoch of a few channels): half
-    // the rows per workgroup, so that the launch still spreads over the CUs
+    // small batches (the closed loop's one epoch of a few channels): a
+    // half or a quarter of the rows per workgroup, so that the launch still
+    // spreads over the CUs
 #define PSA_X(L_, G_)                                                                          \
     if (a.L == L_) {                                                                           \
-        if (G_ >= 2 && (long long)((a.nsub + G_ - 1) / G_) * nbatch < 2048) {                  \
+        const long long nb = (long long)((a.nsub + G_ - 1) / G_) * nbatch;                     \
+        if (G_ >= 4 && nb < 1024) {                                                            \
+            a.G = G_ >= 4 ? G_ / 4 : 1;                                                        \
+            launch_xrow_t<L_, (G_ >= 4 ? G_ / 4 : 1)>(a, nbatch, st);                          \
+        } else if (G_ >= 2 && nb < 2048) {                                                     \
             a.G = (G_ + 1) / 2;                                                                \
             launch_xrow_t<L_, (G_ + 1) / 2>(a, nbatch, st);                                    \
         } else {                                                                               \
