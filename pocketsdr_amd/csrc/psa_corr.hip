@@ -353,7 +353,6 @@ __global__ void __launch_bounds__(TRK_SCHUNK) trk_state_kernel(const TrkChDev *_
 #define TRK_THREADS 512
 #define TRK_TPW 6
 
-// LDS working set of one (channel, epoch) correlation
 // A carrier word (int8 cI in byte 0, cQ in byte 1) as one v_dot4 operand,
 // the int8 lanes (cI, cQ, kI, kQ) with kI = 8 cI + 7 cQ and kQ = 8 (cI + cQ)
 // modulo 256.  A sample with uI = sI + 8, uQ = sQ + 8 and vQ = 7 - sQ (all
@@ -367,6 +366,7 @@ __device__ __forceinline__ uint32_t carr_dot_entry(uint32_t c)
            ((uint32_t)(uint8_t)(8 * (cI + cQ)) << 24);
 }
 
+// LDS working set of one (channel, epoch) correlation
 struct TrkLds {
     uint32_t *planeI, *planeQ;   // mixed samples, Nq4 words each
     const uint32_t *carr;        // carrier table as v_dot4 operands
